@@ -1,0 +1,1083 @@
+/*
+ * psb_oracle.c -- CPU restatement (FP64) of PSI-BOIL's pressure-Poisson path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library, never shipped
+ * behind it.  Compile with -O2 -ffp-contract=off (no fused multiply-add) so
+ * that every per-cell expression rounds like the reference built with g++.
+ *
+ * How each function is pinned (tests/test_oracle*.py):
+ *   ora_spmv, ora_resid          bit-exact against oracle/_ref (the
+ *                                reference's own Scalar operators) on the
+ *                                golden fixtures; dense-matrix brute force;
+ *                                A*1 = 0 on pure-Neumann rows; symmetry.
+ *   ora_dot, ora_dot_voldiv_avg  closed form (math.fsum) on random vectors.
+ *   ora_cg, ora_bicgs            residual histories and iteration counts of
+ *                                the reference's CG::solve / BiCGS::solve
+ *                                (golden fixtures); dense pseudo-inverse
+ *                                solution on tiny grids.
+ *   ora_jacobi                   reference Jacobi::solve history (fixtures);
+ *                                closed-form iteration on a diagonal matrix.
+ *   ora_levels                   reference level sizes (fixtures) and the
+ *                                factor() known answers.
+ *   ora_coarsen_flag/_system,
+ *   ora_restriction,
+ *   ora_interpolation            reference level matrices / restricted RHS
+ *                                (fixtures) and dense P^T A P, P^T(f-A phi)
+ *                                with explicit P on 8^3 -> 4^3.
+ *   ora_ac_vcycle                reference AC::vcycle residual history
+ *                                (fixtures).
+ *   ora_discretize, ora_rhs      reference Pressure::discretize / update_rhs
+ *                                on the same grid metrics and density
+ *                                (fixtures); closed form on uniform grids.
+ *   ora_ac_coarse_dense,
+ *   ora_ac_precond, ora_acpcg,
+ *   ora_pcg_rel                  not in the reference: dense pseudo-inverse
+ *                                solve on tiny grids, symmetry of M^-1
+ *                                (u.M^-1 v = v.M^-1 u), A-orthogonality.
+ */
+#include "psb_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define IDX(i, j, k, ny, nz) (((size_t)(i) * (size_t)(ny) + (size_t)(j)) * (size_t)(nz) + (size_t)(k))
+
+static const double PICO = 1.0e-12; /* Src/Global/global_constants.cpp:17 */
+static const double ATTO = 1.0e-18; /* Src/Global/global_constants.cpp:19 */
+static const double UNREAL = 1.0e+24; /* boil::unreal = yotta */
+
+/* ------------------------------------------------------------------------ */
+/* generator                                                                 */
+/* ------------------------------------------------------------------------ */
+uint64_t ora_splitmix64(uint64_t *state) {
+  uint64_t z = (*state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+double ora_u01(uint64_t *state) {
+  return (double)(ora_splitmix64(state) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+void ora_gen_spheres(uint64_t seed, int nsph, double rmin, double rmax,
+                     double Lx, double Ly, double Lz, double *out) {
+  uint64_t st = seed;
+  for (int s = 0; s < nsph; s++) {
+    double r = rmin + (rmax - rmin) * ora_u01(&st);
+    double cx = r + (Lx - 2.0 * r) * ora_u01(&st);
+    double cy = r + (Ly - 2.0 * r) * ora_u01(&st);
+    double cz = r + (Lz - 2.0 * r) * ora_u01(&st);
+    out[4 * s + 0] = cx;
+    out[4 * s + 1] = cy;
+    out[4 * s + 2] = cz;
+    out[4 * s + 3] = r;
+  }
+}
+
+/* Src/Grid/grid1d_correct_boundaries.cpp: BndGrid::wall() keeps the ghost
+ * nodes on the wall, so the ghost cell centre is the wall node itself. */
+void ora_metrics(int n, const double *xn, double *xc, double *dxc, double *dxw,
+                 double *dxe) {
+  for (int i = 0; i < n; i++) {
+    xc[i] = 0.5 * (xn[i + 1] + xn[i]);
+    dxc[i] = xn[i + 1] - xn[i];
+  }
+  for (int i = 0; i < n; i++) {
+    double xm = (i == 0) ? 0.5 * (xn[0] + xn[0]) : xc[i - 1];
+    double xp = (i == n - 1) ? 0.5 * (xn[n] + xn[n]) : xc[i + 1];
+    dxw[i] = xc[i] - xm;
+    dxe[i] = xp - xc[i];
+  }
+}
+
+void ora_density(int nx, int ny, int nz, const double *xc, const double *yc,
+                 const double *zc, int nsph, const double *sph, double rho_in,
+                 double rho_out, double *rho) {
+  const int gy = ny + 2, gz = nz + 2;
+  for (int i = -1; i <= nx; i++)
+    for (int j = -1; j <= ny; j++)
+      for (int k = -1; k <= nz; k++) {
+        int ii = i < 0 ? 0 : (i >= nx ? nx - 1 : i);
+        int jj = j < 0 ? 0 : (j >= ny ? ny - 1 : j);
+        int kk = k < 0 ? 0 : (k >= nz ? nz - 1 : k);
+        int inside = 0;
+        for (int s = 0; s < nsph && !inside; s++) {
+          double dx = xc[ii] - sph[4 * s + 0];
+          double dy = yc[jj] - sph[4 * s + 1];
+          double dz = zc[kk] - sph[4 * s + 2];
+          if (sqrt(dx * dx + dy * dy + dz * dz) < sph[4 * s + 3]) inside = 1;
+        }
+        rho[((size_t)(i + 1) * gy + (j + 1)) * gz + (k + 1)] =
+            inside ? rho_in : rho_out;
+      }
+}
+
+void ora_discretize(int nx, int ny, int nz, const double *dxc, const double *dxw,
+                    const double *dxe, const double *dyc, const double *dys,
+                    const double *dyn, const double *dzc, const double *dzb,
+                    const double *dzt, const double *rho, const uint8_t *off,
+                    ora_mat *A, double *ci, double *dV) {
+  const int gy = ny + 2, gz = nz + 2;
+#define RHO(i, j, k) rho[((size_t)((i) + 1) * gy + ((j) + 1)) * gz + ((k) + 1)]
+  const size_t N = (size_t)nx * ny * nz;
+  double *csum = off ? (double *)malloc(N * sizeof(double)) : NULL;
+  A->nx = nx; A->ny = ny; A->nz = nz;
+  /* neighbouring coefficients, pressure_discretize.cpp:24-57 */
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        size_t P = IDX(i, j, k, ny, nz);
+        double rp = RHO(i, j, k);
+        double a_x = dyc[j] * dzc[k];
+        double a_y = dxc[i] * dzc[k];
+        double a_z = dxc[i] * dyc[j];
+        A->w[P] = a_x / dxw[i] / (0.5 * (rp + RHO(i - 1, j, k)));
+        A->e[P] = a_x / dxe[i] / (0.5 * (RHO(i + 1, j, k) + rp));
+        A->s[P] = a_y / dys[j] / (0.5 * (rp + RHO(i, j - 1, k)));
+        A->n[P] = a_y / dyn[j] / (0.5 * (RHO(i, j + 1, k) + rp));
+        A->b[P] = a_z / dzb[k] / (0.5 * (rp + RHO(i, j, k - 1)));
+        A->t[P] = a_z / dzt[k] / (0.5 * (RHO(i, j, k + 1) + rp));
+        /* central coefficient, :65-67 */
+        A->c[P] = A->w[P] + A->e[P] + A->s[P] + A->n[P] + A->b[P] + A->t[P];
+        if (csum) csum[P] = A->c[P];
+      }
+  if (off) {
+    /* voxel body: zero the couplings of off cells on both sides (:93-106) */
+    for (int i = 0; i < nx; i++)
+      for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz; k++) {
+          size_t P = IDX(i, j, k, ny, nz);
+          if (!off[P]) continue;
+          A->w[P] = A->e[P] = A->s[P] = A->n[P] = A->b[P] = A->t[P] = 0.0;
+          if (i > 0) A->e[IDX(i - 1, j, k, ny, nz)] = 0.0;
+          if (i < nx - 1) A->w[IDX(i + 1, j, k, ny, nz)] = 0.0;
+          if (j > 0) A->n[IDX(i, j - 1, k, ny, nz)] = 0.0;
+          if (j < ny - 1) A->s[IDX(i, j + 1, k, ny, nz)] = 0.0;
+          if (k > 0) A->t[IDX(i, j, k - 1, ny, nz)] = 0.0;
+          if (k < nz - 1) A->b[IDX(i, j, k + 1, ny, nz)] = 0.0;
+        }
+    /* central coefficient of cells that are not off (:112-117) */
+    for (size_t P = 0; P < N; P++)
+      A->c[P] = off[P] ? csum[P]
+                       : A->w[P] + A->e[P] + A->s[P] + A->n[P] + A->b[P] + A->t[P];
+  }
+  /* Neumann walls in the order imin, imax, jmin, jmax, kmin, kmax
+   * (centered_system_bnd.cpp:27-61) */
+  for (int j = 0; j < ny; j++)
+    for (int k = 0; k < nz; k++) {
+      size_t P = IDX(0, j, k, ny, nz);
+      A->c[P] -= A->w[P]; A->w[P] = 0.0;
+    }
+  for (int j = 0; j < ny; j++)
+    for (int k = 0; k < nz; k++) {
+      size_t P = IDX(nx - 1, j, k, ny, nz);
+      A->c[P] -= A->e[P]; A->e[P] = 0.0;
+    }
+  for (int i = 0; i < nx; i++)
+    for (int k = 0; k < nz; k++) {
+      size_t P = IDX(i, 0, k, ny, nz);
+      A->c[P] -= A->s[P]; A->s[P] = 0.0;
+    }
+  for (int i = 0; i < nx; i++)
+    for (int k = 0; k < nz; k++) {
+      size_t P = IDX(i, ny - 1, k, ny, nz);
+      A->c[P] -= A->n[P]; A->n[P] = 0.0;
+    }
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      size_t P = IDX(i, j, 0, ny, nz);
+      A->c[P] -= A->b[P]; A->b[P] = 0.0;
+    }
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      size_t P = IDX(i, j, nz - 1, ny, nz);
+      A->c[P] -= A->t[P]; A->t[P] = 0.0;
+    }
+  /* ci = 1/c before the off-cell scaling (:127), then c *= pico (:132-148) */
+  for (size_t P = 0; P < N; P++) {
+    if (ci) ci[P] = 1.0 / A->c[P];
+    if (off && off[P]) A->c[P] *= PICO;
+  }
+  if (dV)
+    for (int i = 0; i < nx; i++)
+      for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz; k++)
+          dV[IDX(i, j, k, ny, nz)] = dxc[i] * dyc[j] * dzc[k];
+  free(csum);
+#undef RHO
+}
+
+void ora_face_velocity(uint64_t seed, int nx, int ny, int nz, double *u,
+                       double *v, double *w) {
+  uint64_t st = seed;
+  for (int i = 0; i <= nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        double val = 2.0 * ora_u01(&st) - 1.0;
+        u[((size_t)i * ny + j) * nz + k] = (i == 0 || i == nx) ? 0.0 : val;
+      }
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j <= ny; j++)
+      for (int k = 0; k < nz; k++) {
+        double val = 2.0 * ora_u01(&st) - 1.0;
+        v[((size_t)i * (ny + 1) + j) * nz + k] = (j == 0 || j == ny) ? 0.0 : val;
+      }
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k <= nz; k++) {
+        double val = 2.0 * ora_u01(&st) - 1.0;
+        w[((size_t)i * ny + j) * (nz + 1) + k] = (k == 0 || k == nz) ? 0.0 : val;
+      }
+}
+
+/* pressure_update_rhs.cpp:27-52 with dti = 1, off cells zero (:56-57) */
+void ora_rhs(int nx, int ny, int nz, const double *dxc, const double *dyc,
+             const double *dzc, const double *u, const double *v, const double *w,
+             const uint8_t *off, int zero_mean, double *b) {
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        size_t P = IDX(i, j, k, ny, nz);
+        double a_x = dyc[j] * dzc[k] * 1.0;
+        double a_y = dxc[i] * dzc[k] * 1.0;
+        double a_z = dxc[i] * dyc[j] * 1.0;
+        double f = 0.0;
+        f += a_x * u[((size_t)i * ny + j) * nz + k];
+        f -= a_x * u[((size_t)(i + 1) * ny + j) * nz + k];
+        f += a_y * v[((size_t)i * (ny + 1) + j) * nz + k];
+        f -= a_y * v[((size_t)i * (ny + 1) + j + 1) * nz + k];
+        f += a_z * w[((size_t)i * ny + j) * (nz + 1) + k];
+        f -= a_z * w[((size_t)i * ny + j) * (nz + 1) + k + 1];
+        if (off && off[P]) f = 0.0;
+        b[P] = f;
+      }
+  if (zero_mean) {
+    const size_t N = (size_t)nx * ny * nz;
+    double sum = 0.0;
+    long cnt = 0;
+    for (size_t P = 0; P < N; P++)
+      if (!(off && off[P])) { sum += b[P]; cnt++; }
+    if (cnt > 0) {
+      double mean = sum / (double)cnt;
+      for (size_t P = 0; P < N; P++)
+        if (!(off && off[P])) b[P] -= mean;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* vector kernels                                                            */
+/* ------------------------------------------------------------------------ */
+static inline double nbx(const double *x, int i, int j, int k, int nx, int ny,
+                         int nz) {
+  if (i < 0 || j < 0 || k < 0 || i >= nx || j >= ny || k >= nz) return 0.0;
+  return x[IDX(i, j, k, ny, nz)];
+}
+
+/* Src/Field/Scalar/scalar_operators.cpp:49-72 (four passes per cell, same
+ * grouping of the products within each pass) */
+void ora_spmv(const ora_mat *A, const double *x, double *y) {
+  const int nx = A->nx, ny = A->ny, nz = A->nz;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        size_t P = IDX(i, j, k, ny, nz);
+        double v = A->c[P] * x[P];
+        v -= (A->w[P] * nbx(x, i - 1, j, k, nx, ny, nz) +
+              A->e[P] * nbx(x, i + 1, j, k, nx, ny, nz));
+        v -= (A->s[P] * nbx(x, i, j - 1, k, nx, ny, nz) +
+              A->n[P] * nbx(x, i, j + 1, k, nx, ny, nz));
+        v -= (A->b[P] * nbx(x, i, j, k - 1, nx, ny, nz) +
+              A->t[P] * nbx(x, i, j, k + 1, nx, ny, nz));
+        y[P] = v;
+      }
+}
+
+/* Src/Field/Scalar/scalar_operators.cpp:22-46 */
+void ora_resid(const ora_mat *A, const double *b, const double *x, double *r) {
+  const int nx = A->nx, ny = A->ny, nz = A->nz;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        size_t P = IDX(i, j, k, ny, nz);
+        double v = b[P] - A->c[P] * x[P];
+        v += A->w[P] * nbx(x, i - 1, j, k, nx, ny, nz) +
+             A->e[P] * nbx(x, i + 1, j, k, nx, ny, nz);
+        v += A->s[P] * nbx(x, i, j - 1, k, nx, ny, nz) +
+             A->n[P] * nbx(x, i, j + 1, k, nx, ny, nz);
+        v += A->b[P] * nbx(x, i, j, k - 1, nx, ny, nz) +
+             A->t[P] * nbx(x, i, j, k + 1, nx, ny, nz);
+        r[P] = v;
+      }
+}
+
+/* Src/Field/Scalar/scalar.h:296-298 */
+double ora_dot(size_t n, const double *x, const double *y) {
+  double d = 0.0;
+  for (size_t P = 0; P < n; P++) d += x[P] * y[P];
+  return d;
+}
+
+/* Src/Field/Scalar/scalar.h:317-326 */
+double ora_dot_voldiv_avg(size_t n, const double *x, const double *y,
+                          const double *dV) {
+  double d = 0.0;
+  for (size_t P = 0; P < n; P++) d += x[P] * y[P] / dV[P] / dV[P];
+  return d / (double)n / (double)n;
+}
+
+static size_t mat_n(const ora_mat *A) { return (size_t)A->nx * A->ny * A->nz; }
+
+static double *dalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
+
+/* staleness window shared by CG and BiCGS (cg.cpp:36-41,138-160) */
+static int stale_check(double *win, int cnt, double res) {
+  for (int q = 0; q < cnt; q++)
+    if (res < win[q]) {
+      memmove(win, win + 1, (size_t)(cnt - 1) * sizeof(double));
+      win[cnt - 1] = res;
+      return 0;
+    }
+  return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CG, Src/Solver/Linear/Krylov/cg.cpp:10-182 with Diagonal preconditioning  */
+/* (preconditioner_form.cpp:4-13, preconditioner_solve.cpp:4-11)             */
+/* ------------------------------------------------------------------------ */
+int ora_cg(const ora_mat *A, double *x, const double *b, const double *dV,
+           int mini, int maxit, double res_rat, double res_tol, double scale,
+           int stalecount, double *hist, ora_result *out) {
+  const size_t N = mat_n(A);
+  double *p = dalloc(N), *q = dalloc(N), *r = dalloc(N), *z = dalloc(N),
+         *Mc = dalloc(N);
+  double *win = stalecount > 0 ? dalloc((size_t)stalecount) : NULL;
+  double alfa, beta, rho = 0.0, rho_old = 1.0, pq;
+  for (size_t P = 0; P < N; P++) Mc[P] = 1.0 / A->c[P];
+  for (int q2 = 0; q2 < stalecount; q2++) win[q2] = UNREAL;
+
+  ora_resid(A, b, x, r);
+  double res = sqrt(ora_dot_voldiv_avg(N, r, r, dV)) / scale;
+  double res0 = res;
+  if (stalecount > 0) win[stalecount - 1] = res;
+  if (hist) hist[0] = res;
+  ora_result R = {0, 0, 0, res0, res};
+  if (res < res_tol) {
+    R.converged = 1; R.reason = 7; R.iters = 0;
+    goto done;
+  }
+  int i;
+  for (i = 0; i < maxit; i++) {
+    for (size_t P = 0; P < N; P++) z[P] = r[P] * Mc[P];
+    rho = ora_dot(N, r, z);
+    if (i == 0) {
+      memcpy(p, z, N * sizeof(double));
+    } else {
+      beta = rho / rho_old;
+      for (size_t P = 0; P < N; P++) p[P] = z[P] + beta * p[P];
+    }
+    ora_spmv(A, p, q);
+    pq = ora_dot(N, p, q);
+    alfa = rho / pq;
+    for (size_t P = 0; P < N; P++) x[P] += alfa * p[P];
+    for (size_t P = 0; P < N; P++) r[P] -= alfa * q[P];
+    res = sqrt(ora_dot_voldiv_avg(N, r, r, dV)) / scale;
+    if (hist) hist[i + 1] = res;
+    if (res < res_tol && i >= mini - 1) { R.converged = 1; R.reason = 1; break; }
+    if (res < res0 * res_rat && i >= mini - 1) { R.converged = 1; R.reason = 2; break; }
+    if (stalecount > 0 && stale_check(win, stalecount, res)) { R.reason = 3; break; }
+    rho_old = rho;
+  }
+  R.iters = (i < maxit) ? i + 1 : maxit + 1; /* printed "i+1" (cg.cpp:178) */
+  R.res = res;
+done:
+  if (out) *out = R;
+  free(p); free(q); free(r); free(z); free(Mc); free(win);
+  return R.converged;
+}
+
+/* ------------------------------------------------------------------------ */
+/* BiCGS, Src/Solver/Linear/Krylov/bicgs.cpp:9-229                           */
+/* ------------------------------------------------------------------------ */
+int ora_bicgs(const ora_mat *A, double *x, const double *b, const double *dV,
+              int mini, int maxit, double res_rat, double res_tol, double scale,
+              int stalecount, double *hist, ora_result *out) {
+  const size_t N = mat_n(A);
+  double *p = dalloc(N), *r = dalloc(N), *s = dalloc(N), *t = dalloc(N),
+         *v = dalloc(N), *p_ = dalloc(N), *r_ = dalloc(N), *s_ = dalloc(N),
+         *Mc = dalloc(N);
+  double *win = stalecount > 0 ? dalloc((size_t)stalecount) : NULL;
+  double alfa = 0.0, beta, omega = 1.0, rho, rho_old = 1.0, r_v, ss, ts, tt;
+  for (size_t P = 0; P < N; P++) Mc[P] = 1.0 / A->c[P];
+  for (int q2 = 0; q2 < stalecount; q2++) win[q2] = UNREAL;
+
+  ora_resid(A, b, x, r);
+  double res = sqrt(ora_dot_voldiv_avg(N, r, r, dV)) / scale;
+  double res0 = res;
+  if (stalecount > 0) win[stalecount - 1] = res;
+  if (hist) hist[0] = res;
+  ora_result R = {0, 0, 0, res0, res};
+  if (res < res_tol) {
+    R.converged = 1; R.reason = 7; R.iters = 0;
+    goto done;
+  }
+  memcpy(r_, r, N * sizeof(double));
+  int i;
+  for (i = 0; i < maxit; i++) {
+    rho = ora_dot(N, r_, r);
+    if (sqrt(fabs(rho)) < res_tol) { R.reason = 4; break; }
+    if (i == 0) {
+      memcpy(p, r, N * sizeof(double));
+    } else {
+      beta = (rho / rho_old) * (alfa / omega);
+      for (size_t P = 0; P < N; P++) p[P] = r[P] + beta * p[P];
+      double bo = beta * omega;
+      for (size_t P = 0; P < N; P++) p[P] -= bo * v[P];
+    }
+    for (size_t P = 0; P < N; P++) p_[P] = p[P] * Mc[P];
+    ora_spmv(A, p_, v);
+    r_v = ora_dot(N, r_, v);
+    alfa = rho / r_v;
+    for (size_t P = 0; P < N; P++) s[P] = r[P] - alfa * v[P];
+    ss = ora_dot(N, s, s);
+    if (sqrt(ss) < res_tol) {
+      for (size_t P = 0; P < N; P++) x[P] += alfa * p_[P];
+      R.reason = 5;
+      break;
+    }
+    for (size_t P = 0; P < N; P++) s_[P] = s[P] * Mc[P];
+    ora_spmv(A, s_, t);
+    ts = ora_dot(N, t, s);
+    tt = ora_dot(N, t, t);
+    omega = ts / tt;
+    for (size_t P = 0; P < N; P++) x[P] += alfa * p_[P];
+    for (size_t P = 0; P < N; P++) x[P] += omega * s_[P];
+    for (size_t P = 0; P < N; P++) r[P] = s[P] - omega * t[P];
+    res = sqrt(ora_dot_voldiv_avg(N, r, r, dV)) / scale;
+    if (hist) hist[i + 1] = res;
+    if (res < res_tol && i >= mini - 1) { R.converged = 1; R.reason = 1; break; }
+    if (res < res0 * res_rat && i >= mini - 1) { R.converged = 1; R.reason = 2; break; }
+    if (omega >= -res_tol && omega <= res_tol) { R.reason = 6; break; }
+    if (stalecount > 0 && stale_check(win, stalecount, res)) { R.reason = 3; break; }
+    rho_old = rho;
+  }
+  R.iters = (i < maxit) ? i + 1 : maxit + 1;
+  R.res = res;
+done:
+  if (out) *out = R;
+  free(p); free(r); free(s); free(t); free(v); free(p_); free(r_); free(s_);
+  free(Mc); free(win);
+  return R.converged;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Jacobi, Src/Solver/Linear/Iterative/jacobi.cpp:10-75 (x = ci (r + c x))   */
+/* ------------------------------------------------------------------------ */
+int ora_jacobi(const ora_mat *A, double *x, const double *b, int mini, int maxit,
+               double res_rat, double res_tol, ora_result *out) {
+  const size_t N = mat_n(A);
+  double *r = dalloc(N);
+  ora_resid(A, b, x, r);
+  double res = ora_dot(N, r, r);
+  double res0 = res;
+  ora_result R = {0, 0, 0, sqrt(res0), sqrt(res)};
+  if (sqrt(res) < res_tol) {
+    R.converged = 1; R.reason = 7;
+    goto done;
+  }
+  int i;
+  for (i = 0; i < maxit; i++) {
+    for (size_t P = 0; P < N; P++) {
+      double ci = 1.0 / A->c[P];
+      x[P] = ci * (r[P] + A->c[P] * x[P]);
+    }
+    ora_resid(A, b, x, r);
+    res = ora_dot(N, r, r);
+    if (sqrt(res) < res_tol && i >= mini - 1) { R.converged = 1; R.reason = 1; break; }
+    if (sqrt(res) < sqrt(res0) * res_rat && i >= mini - 1) { R.converged = 1; R.reason = 2; break; }
+  }
+  R.iters = (i < maxit) ? i + 1 : maxit + 1;
+  R.res = sqrt(res);
+done:
+  if (out) *out = R;
+  free(r);
+  return R.converged;
+}
+
+/* ------------------------------------------------------------------------ */
+/* AC level construction                                                     */
+/* ------------------------------------------------------------------------ */
+/* Src/Global/global_factor.cpp:2-19: the first factor found is n/m for the
+ * largest proper divisor m, i.e. the smallest prime factor; a prime n gives n */
+int ora_smallest_factor(int n) {
+  for (int m = n - 1; m >= 2; m--)
+    if (n % m == 0) return n / m;
+  return n;
+}
+
+int ora_levels(int rule, int nx, int ny, int nz, int maxlev, int *lx, int *ly,
+               int *lz) {
+  int L = 0;
+  lx[0] = nx; ly[0] = ny; lz[0] = nz;
+  L = 1;
+  while (L < maxlev) {
+    int n[3] = {lx[L - 1], ly[L - 1], lz[L - 1]};
+    int m[3];
+    int any = 0;
+    for (int d = 0; d < 3; d++) {
+      m[d] = n[d];
+      if (rule == 0) {
+        /* Src/Domain/domain_coarsen.cpp:33-71 */
+        int f = ora_smallest_factor(n[d]);
+        if (n[d] % f == 0 && n[d] > 4 && n[d] != f) { m[d] = n[d] / f; any = 1; }
+      } else {
+        if (n[d] % 2 == 0 && n[d] / 2 >= 8) { m[d] = n[d] / 2; any = 1; }
+      }
+    }
+    if (!any) break;
+    lx[L] = m[0]; ly[L] = m[1]; lz[L] = m[2];
+    L++;
+  }
+  return L;
+}
+
+static void mat_alloc(ora_mat *A, int nx, int ny, int nz) {
+  size_t N = (size_t)nx * ny * nz;
+  A->nx = nx; A->ny = ny; A->nz = nz;
+  A->c = dalloc(N); A->w = dalloc(N); A->e = dalloc(N); A->s = dalloc(N);
+  A->n = dalloc(N); A->b = dalloc(N); A->t = dalloc(N);
+}
+
+static void mat_free(ora_mat *A) {
+  free(A->c); free(A->w); free(A->e); free(A->s); free(A->n); free(A->b);
+  free(A->t);
+  memset(A, 0, sizeof(*A));
+}
+
+int ora_ac_build(ora_ac *ac, int rule, const ora_mat *A0, const double *dV0,
+                 const uint8_t *off0, int maxlev) {
+  memset(ac, 0, sizeof(*ac));
+  if (maxlev > 24) maxlev = 24;
+  ac->nlev = ora_levels(rule, A0->nx, A0->ny, A0->nz, maxlev, ac->nx, ac->ny, ac->nz);
+  for (int l = 0; l < ac->nlev; l++) {
+    size_t N = (size_t)ac->nx[l] * ac->ny[l] * ac->nz[l];
+    mat_alloc(&ac->A[l], ac->nx[l], ac->ny[l], ac->nz[l]);
+    ac->aflag[l] = (int *)calloc(N, sizeof(int));
+    ac->dV[l] = dalloc(N);
+    ac->phi[l] = dalloc(N);
+    ac->f[l] = dalloc(N);
+    ac->res[l] = dalloc(N);
+  }
+  size_t N0 = (size_t)A0->nx * A0->ny * A0->nz;
+  memcpy(ac->A[0].c, A0->c, N0 * 8); memcpy(ac->A[0].w, A0->w, N0 * 8);
+  memcpy(ac->A[0].e, A0->e, N0 * 8); memcpy(ac->A[0].s, A0->s, N0 * 8);
+  memcpy(ac->A[0].n, A0->n, N0 * 8); memcpy(ac->A[0].b, A0->b, N0 * 8);
+  memcpy(ac->A[0].t, A0->t, N0 * 8);
+  memcpy(ac->dV[0], dV0, N0 * 8);
+  for (size_t P = 0; P < N0; P++) ac->aflag[0][P] = off0 ? !off0[P] : 1;
+  for (int l = 1; l < ac->nlev; l++) {
+    ora_coarsen_flag(ac, l);
+    ora_coarsen_system(ac, l);
+    /* coarse cell volume: sum over the block (the reference's coarse grid
+     * spans the same nodes, so this equals its dV up to rounding) */
+    int CI = ac->nx[l - 1] / ac->nx[l], CJ = ac->ny[l - 1] / ac->ny[l],
+        CK = ac->nz[l - 1] / ac->nz[l];
+    for (int I = 0; I < ac->nx[l]; I++)
+      for (int J = 0; J < ac->ny[l]; J++)
+        for (int K = 0; K < ac->nz[l]; K++) {
+          double s = 0.0;
+          for (int a = 0; a < CI; a++)
+            for (int bb = 0; bb < CJ; bb++)
+              for (int cc = 0; cc < CK; cc++)
+                s += ac->dV[l - 1][IDX(I * CI + a, J * CJ + bb, K * CK + cc,
+                                        ac->ny[l - 1], ac->nz[l - 1])];
+          ac->dV[l][IDX(I, J, K, ac->ny[l], ac->nz[l])] = s;
+        }
+  }
+  return ac->nlev;
+}
+
+void ora_ac_free(ora_ac *ac) {
+  for (int l = 0; l < ac->nlev; l++) {
+    mat_free(&ac->A[l]);
+    free(ac->aflag[l]); free(ac->dV[l]); free(ac->phi[l]); free(ac->f[l]);
+    free(ac->res[l]);
+  }
+  free(ac->G);
+  memset(ac, 0, sizeof(*ac));
+}
+
+/* Src/Solver/Additive/additive_coarsen_flag.cpp:4-41: OR over the block */
+void ora_coarsen_flag(const ora_ac *ac, int l) {
+  const int nX = ac->nx[l], nY = ac->ny[l], nZ = ac->nz[l];
+  const int nx = ac->nx[l - 1], ny = ac->ny[l - 1], nz = ac->nz[l - 1];
+  const int CI = nx / nX, CJ = ny / nY, CK = nz / nZ;
+  for (int I = 0; I < nX; I++)
+    for (int J = 0; J < nY; J++)
+      for (int K = 0; K < nZ; K++) {
+        int fl = 0;
+        for (int a = 0; a < CI; a++)
+          for (int bb = 0; bb < CJ; bb++)
+            for (int cc = 0; cc < CK; cc++)
+              fl |= ac->aflag[l - 1][IDX(I * CI + a, J * CJ + bb, K * CK + cc, ny, nz)];
+        ac->aflag[l][IDX(I, J, K, nY, nZ)] = fl;
+      }
+}
+
+/* Src/Solver/Additive/additive_coarsen_system.cpp:4-140.  Within a block the
+ * fine cells are visited as the reference does: c summed over (i,j,k), then
+ * the internal e/w couplings per (j,k) line, then n/s per (i,k), then t/b
+ * per (i,j); the external couplings are the sums over the block's faces. */
+void ora_coarsen_system(const ora_ac *ac, int l) {
+  const int nX = ac->nx[l], nY = ac->ny[l], nZ = ac->nz[l];
+  const int ny = ac->ny[l - 1], nz = ac->nz[l - 1];
+  const int CI = ac->nx[l - 1] / nX, CJ = ny / nY, CK = nz / nZ;
+  const ora_mat *h = &ac->A[l - 1];
+  const ora_mat *H = &ac->A[l];
+#define F(i, j, k) IDX(I * CI + (i), J * CJ + (j), K * CK + (k), ny, nz)
+  for (int I = 0; I < nX; I++)
+    for (int J = 0; J < nY; J++)
+      for (int K = 0; K < nZ; K++) {
+        size_t Q = IDX(I, J, K, nY, nZ);
+        double c = 0.0;
+        for (int i = 0; i < CI; i++)
+          for (int j = 0; j < CJ; j++)
+            for (int k = 0; k < CK; k++) c += h->c[F(i, j, k)];
+        for (int j = 0; j < CJ; j++)
+          for (int k = 0; k < CK; k++) {
+            for (int i = 0; i < CI - 1; i++) c -= h->e[F(i, j, k)];
+            for (int i = 1; i < CI; i++) c -= h->w[F(i, j, k)];
+          }
+        double e = 0.0, w = 0.0;
+        for (int j = 0; j < CJ; j++)
+          for (int k = 0; k < CK; k++) {
+            e += h->e[F(CI - 1, j, k)];
+            w += h->w[F(0, j, k)];
+          }
+        for (int i = 0; i < CI; i++)
+          for (int k = 0; k < CK; k++) {
+            for (int j = 0; j < CJ - 1; j++) c -= h->n[F(i, j, k)];
+            for (int j = 1; j < CJ; j++) c -= h->s[F(i, j, k)];
+          }
+        double n = 0.0, s = 0.0;
+        for (int i = 0; i < CI; i++)
+          for (int k = 0; k < CK; k++) {
+            n += h->n[F(i, CJ - 1, k)];
+            s += h->s[F(i, 0, k)];
+          }
+        for (int i = 0; i < CI; i++)
+          for (int j = 0; j < CJ; j++) {
+            for (int k = 0; k < CK - 1; k++) c -= h->t[F(i, j, k)];
+            for (int k = 1; k < CK; k++) c -= h->b[F(i, j, k)];
+          }
+        double t = 0.0, b = 0.0;
+        for (int i = 0; i < CI; i++)
+          for (int j = 0; j < CJ; j++) {
+            t += h->t[F(i, j, CK - 1)];
+            b += h->b[F(i, j, 0)];
+          }
+        H->c[Q] = c; H->e[Q] = e; H->w[Q] = w; H->n[Q] = n; H->s[Q] = s;
+        H->t[Q] = t; H->b[Q] = b;
+      }
+#undef F
+}
+
+/* Src/Solver/Additive/additive_restriction.cpp:12-177: H.f = sum over the
+ * block of (f - A phi)_h, written as sum f + external couplings * phi outside
+ * - D phi with D = c - internal couplings; inactive cells 0; mean over active
+ * cells removed. */
+void ora_restriction(const ora_ac *ac, int l) {
+  const int nx = ac->nx[l], ny = ac->ny[l], nz = ac->nz[l];
+  const int nX = ac->nx[l + 1], nY = ac->ny[l + 1], nZ = ac->nz[l + 1];
+  const int CI = nx / nX, CJ = ny / nY, CK = nz / nZ;
+  const ora_mat *h = &ac->A[l];
+  const double *phi = ac->phi[l], *f = ac->f[l];
+  double *fH = ac->f[l + 1];
+  double *D = dalloc((size_t)CI * CJ * CK);
+#define DD(i, j, k) D[((size_t)(i) * CJ + (j)) * CK + (k)]
+  for (int I = 0; I < nX; I++)
+    for (int J = 0; J < nY; J++)
+      for (int K = 0; K < nZ; K++) {
+        const int i0 = I * CI, j0 = J * CJ, k0 = K * CK;
+#define F(i, j, k) IDX(i0 + (i), j0 + (j), k0 + (k), ny, nz)
+#define PH(i, j, k) nbx(phi, i0 + (i), j0 + (j), k0 + (k), nx, ny, nz)
+        double s = 0.0;
+        for (int i = 0; i < CI; i++)
+          for (int j = 0; j < CJ; j++)
+            for (int k = 0; k < CK; k++) {
+              DD(i, j, k) = h->c[F(i, j, k)];
+              s += f[F(i, j, k)];
+            }
+        for (int j = 0; j < CJ; j++)
+          for (int k = 0; k < CK; k++) {
+            s += h->e[F(CI - 1, j, k)] * PH(CI, j, k);
+            s += h->w[F(0, j, k)] * PH(-1, j, k);
+            for (int i = 0; i < CI - 1; i++) DD(i, j, k) -= h->e[F(i, j, k)];
+            for (int i = CI - 1; i > 0; i--) DD(i, j, k) -= h->w[F(i, j, k)];
+          }
+        for (int i = 0; i < CI; i++)
+          for (int k = 0; k < CK; k++) {
+            s += h->n[F(i, CJ - 1, k)] * PH(i, CJ, k);
+            s += h->s[F(i, 0, k)] * PH(i, -1, k);
+            for (int j = 0; j < CJ - 1; j++) DD(i, j, k) -= h->n[F(i, j, k)];
+            for (int j = CJ - 1; j > 0; j--) DD(i, j, k) -= h->s[F(i, j, k)];
+          }
+        for (int i = 0; i < CI; i++)
+          for (int j = 0; j < CJ; j++) {
+            s += h->t[F(i, j, CK - 1)] * PH(i, j, CK);
+            s += h->b[F(i, j, 0)] * PH(i, j, -1);
+            for (int k = 0; k < CK - 1; k++) DD(i, j, k) -= h->t[F(i, j, k)];
+            for (int k = CK - 1; k > 0; k--) DD(i, j, k) -= h->b[F(i, j, k)];
+          }
+        for (int i = 0; i < CI; i++)
+          for (int j = 0; j < CJ; j++)
+            for (int k = 0; k < CK; k++) s -= DD(i, j, k) * phi[F(i, j, k)];
+        fH[IDX(I, J, K, nY, nZ)] = s;
+#undef F
+#undef PH
+      }
+#undef DD
+  free(D);
+  const size_t NH = (size_t)nX * nY * nZ;
+  const int *fl = ac->aflag[l + 1];
+  for (size_t Q = 0; Q < NH; Q++)
+    if (!fl[Q]) fH[Q] = 0.0;
+  double sum = 0.0;
+  long ncell = 0;
+  for (size_t Q = 0; Q < NH; Q++)
+    if (fl[Q]) { sum += fH[Q]; ncell++; }
+  sum /= (double)ncell;
+  for (size_t Q = 0; Q < NH; Q++)
+    if (fl[Q]) fH[Q] -= sum;
+}
+
+/* Src/Solver/Additive/additive_interpolation.cpp:12-64: h.phi += H.phi */
+void ora_interpolation(const ora_ac *ac, int l) {
+  const int nX = ac->nx[l], nY = ac->ny[l], nZ = ac->nz[l];
+  const int ny = ac->ny[l - 1], nz = ac->nz[l - 1];
+  const int CI = ac->nx[l - 1] / nX, CJ = ny / nY, CK = nz / nZ;
+  for (int I = 0; I < nX; I++)
+    for (int J = 0; J < nY; J++)
+      for (int K = 0; K < nZ; K++) {
+        double v = ac->phi[l][IDX(I, J, K, nY, nZ)];
+        for (int i = 0; i < CI; i++)
+          for (int j = 0; j < CJ; j++)
+            for (int k = 0; k < CK; k++)
+              ac->phi[l - 1][IDX(I * CI + i, J * CJ + j, K * CK + k, ny, nz)] += v;
+      }
+}
+
+/* AC::residual (additive_residual.cpp:19-29): sqrt(dot_voldiv_avg) * dt */
+static double ac_residual(ora_ac *ac, double dt) {
+  const ora_mat *A = &ac->A[0];
+  size_t N = mat_n(A);
+  ora_resid(A, ac->f[0], ac->phi[0], ac->res[0]);
+  return sqrt(ora_dot_voldiv_avg(N, ac->res[0], ac->res[0], ac->dV[0])) * dt;
+}
+
+static void ac_cg(ora_ac *ac, int l, int mi, double rr, double rt, double scale) {
+  ora_cg(&ac->A[l], ac->phi[l], ac->f[l], ac->dV[l], 1, mi, rr, rt, scale, -1,
+         NULL, NULL);
+}
+
+/* additive_recursive.cpp:4-34 with AC's defaults resrat = restol = atto
+ * (additive.cpp:16-17) and the coarsest call MaxIter 100, ResRat -1. */
+static void ac_v_cycle(ora_ac *ac, int l, const int mv[3], double scale) {
+  if (l != ac->nlev - 1) {
+    ac_cg(ac, l, mv[0], ATTO, ATTO, scale);
+    ora_restriction(ac, l);
+    ac_v_cycle(ac, l + 1, mv, scale);
+    ac_cg(ac, l, mv[2], ATTO, ATTO, scale);
+    if (l > 0) ora_interpolation(ac, l);
+  } else {
+    ac_cg(ac, l, 100, -1.0, ATTO, scale);
+    ora_interpolation(ac, l);
+  }
+}
+
+/* AC::cycle(Cycle::none(), Cycle::V(), ...) : additive_cycle.cpp:4-99 with
+ * init_cycles / converged from additive_cycle_functions.cpp:12-87 (stop_if_div
+ * and priority_min_cyc true, use_linf false: additive.cpp:11-19). */
+int ora_ac_vcycle(ora_ac *ac, double *phi0, const double *f0, double toler,
+                  double factor, int max_cyc, int min_cyc, const int mv[3],
+                  double dt, double *hist, int *ncyc) {
+  const size_t N0 = mat_n(&ac->A[0]);
+  const double dti = 1.0 / dt;
+  double *fold = dalloc(N0);
+  int ret = 0;
+  memcpy(ac->phi[0], phi0, N0 * 8);
+  memcpy(ac->f[0], f0, N0 * 8);
+  double res_0 = ac_residual(ac, dt);
+  if (hist) hist[0] = res_0;
+  memcpy(fold, ac->phi[0], N0 * 8);
+  if ((res_0 < toler && min_cyc == 0) || res_0 == 0.0) {
+    if (ncyc) *ncyc = 0;
+    ret = 1;
+    goto done;
+  }
+  for (int l = 1; l < ac->nlev; l++) {
+    size_t N = (size_t)ac->nx[l] * ac->ny[l] * ac->nz[l];
+    memset(ac->phi[l], 0, N * 8);
+    memset(ac->f[l], 0, N * 8);
+  }
+  double res_m1 = ac_residual(ac, dt);
+  for (int c = 1; c <= max_cyc; c++) {
+    ac_v_cycle(ac, 0, mv, dti * 1.0);
+    double res1 = ac_residual(ac, dt);
+    if (hist) hist[c] = res1;
+    if (c >= min_cyc && (res1 / res_0 < factor || res1 < toler)) {
+      if (ncyc) *ncyc = c;
+      ret = 1;
+      goto done;
+    }
+    if (!(c < min_cyc) && res1 >= res_m1) {
+      memcpy(ac->phi[0], fold, N0 * 8);
+      if (ncyc) *ncyc = c;
+      ret = 0;
+      goto done;
+    }
+    res_m1 = res1;
+    memcpy(fold, ac->phi[0], N0 * 8);
+    if (c == max_cyc && ncyc) *ncyc = max_cyc;
+  }
+  ret = 0;
+done:
+  memcpy(phi0, ac->phi[0], N0 * 8);
+  free(fold);
+  return ret;
+}
+
+/* ------------------------------------------------------------------------ */
+/* benchmark composition: V-cycle preconditioned CG (not in the reference)   */
+/* ------------------------------------------------------------------------ */
+int ora_ac_coarse_dense(ora_ac *ac) {
+  const int l = ac->nlev - 1;
+  const ora_mat *A = &ac->A[l];
+  const int nx = A->nx, ny = A->ny, nz = A->nz;
+  const int m = nx * ny * nz;
+  double *M = dalloc((size_t)m * m);
+  double *B = dalloc((size_t)m * m);
+  int *pin = (int *)calloc((size_t)m, sizeof(int));
+  const int *fl = ac->aflag[l];
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        int P = (int)IDX(i, j, k, ny, nz);
+        if (!fl[P]) continue;
+        M[(size_t)P * m + P] = A->c[P];
+        if (i > 0) M[(size_t)P * m + IDX(i - 1, j, k, ny, nz)] = -A->w[P];
+        if (i < nx - 1) M[(size_t)P * m + IDX(i + 1, j, k, ny, nz)] = -A->e[P];
+        if (j > 0) M[(size_t)P * m + IDX(i, j - 1, k, ny, nz)] = -A->s[P];
+        if (j < ny - 1) M[(size_t)P * m + IDX(i, j + 1, k, ny, nz)] = -A->n[P];
+        if (k > 0) M[(size_t)P * m + IDX(i, j, k - 1, ny, nz)] = -A->b[P];
+        if (k < nz - 1) M[(size_t)P * m + IDX(i, j, k + 1, ny, nz)] = -A->t[P];
+      }
+  /* Cholesky (lower, in place) with pinning of vanishing pivots: a pivot
+   * below 1e-10 of its original diagonal is the null direction of a
+   * connected Neumann block; that unknown is fixed to zero. */
+  for (int P = 0; P < m; P++)
+    if (!fl[P]) pin[P] = 1;
+  for (int j = 0; j < m; j++) {
+    if (pin[j]) continue;
+    double d = M[(size_t)j * m + j];
+    double orig = d;
+    for (int k = 0; k < j; k++)
+      if (!pin[k]) d -= M[(size_t)j * m + k] * M[(size_t)j * m + k];
+    (void)orig;
+    if (d <= 1e-10 * fabs(A->c[j])) { pin[j] = 1; continue; }
+    double Ljj = sqrt(d);
+    M[(size_t)j * m + j] = Ljj;
+    for (int i = j + 1; i < m; i++) {
+      if (pin[i]) continue;
+      double s = M[(size_t)i * m + j];
+      for (int k = 0; k < j; k++)
+        if (!pin[k]) s -= M[(size_t)i * m + k] * M[(size_t)j * m + k];
+      M[(size_t)i * m + j] = s / Ljj;
+    }
+  }
+  /* B = (L L^T)^-1 on unpinned rows/cols, zero elsewhere */
+  double *y = dalloc((size_t)m);
+  for (int col = 0; col < m; col++) {
+    if (pin[col]) continue;
+    for (int i = 0; i < m; i++) y[i] = (i == col) ? 1.0 : 0.0;
+    for (int i = 0; i < m; i++) {
+      if (pin[i]) { y[i] = 0.0; continue; }
+      double s = y[i];
+      for (int k = 0; k < i; k++)
+        if (!pin[k]) s -= M[(size_t)i * m + k] * y[k];
+      y[i] = s / M[(size_t)i * m + i];
+    }
+    for (int i = m - 1; i >= 0; i--) {
+      if (pin[i]) { y[i] = 0.0; continue; }
+      double s = y[i];
+      for (int k = i + 1; k < m; k++)
+        if (!pin[k]) s -= M[(size_t)k * m + i] * y[k];
+      y[i] = s / M[(size_t)i * m + i];
+    }
+    for (int i = 0; i < m; i++) B[(size_t)i * m + col] = y[i];
+  }
+  /* G = Pi B Pi over active cells */
+  int na = 0;
+  for (int P = 0; P < m; P++) na += fl[P] ? 1 : 0;
+  double *rm = dalloc((size_t)m), *cm = dalloc((size_t)m);
+  double tm = 0.0;
+  for (int i = 0; i < m; i++) {
+    if (!fl[i]) continue;
+    for (int j = 0; j < m; j++) {
+      if (!fl[j]) continue;
+      rm[i] += B[(size_t)i * m + j];
+      cm[j] += B[(size_t)i * m + j];
+      tm += B[(size_t)i * m + j];
+    }
+  }
+  for (int i = 0; i < m; i++) { rm[i] /= na; cm[i] /= na; }
+  tm /= (double)na * (double)na;
+  free(ac->G);
+  ac->G = dalloc((size_t)m * m);
+  for (int i = 0; i < m; i++)
+    for (int j = 0; j < m; j++)
+      ac->G[(size_t)i * m + j] =
+          (fl[i] && fl[j]) ? B[(size_t)i * m + j] - rm[i] - cm[j] + tm : 0.0;
+  free(rm); free(cm); free(y); free(M); free(B); free(pin);
+  return m;
+}
+
+static void jacobi_sweep(const ora_mat *A, const double *r, double *e, double *t,
+                         double omega) {
+  size_t N = mat_n(A);
+  ora_resid(A, r, e, t);
+  for (size_t P = 0; P < N; P++) e[P] += omega * t[P] / A->c[P];
+}
+
+void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
+                    int nu2, double omega, double cfac) {
+  const ora_mat *A = &ac->A[l];
+  const size_t N = mat_n(A);
+  if (l == ac->nlev - 1) {
+    const double *G = ac->G;
+    for (size_t i = 0; i < N; i++) {
+      double s = 0.0;
+      for (size_t j = 0; j < N; j++) s += G[i * N + j] * r[j];
+      z[i] = s;
+    }
+    return;
+  }
+  double *t = ac->res[l];
+  /* first sweep from zero: e = omega r / c */
+  for (size_t P = 0; P < N; P++) z[P] = nu1 > 0 ? omega * r[P] / A->c[P] : 0.0;
+  for (int s = 1; s < nu1; s++) jacobi_sweep(A, r, z, t, omega);
+  /* residual and restriction (block sum, inactive 0, mean removed) */
+  ora_resid(A, r, z, t);
+  const int nX = ac->nx[l + 1], nY = ac->ny[l + 1], nZ = ac->nz[l + 1];
+  const int ny = A->ny, nz = A->nz;
+  const int CI = A->nx / nX, CJ = ny / nY, CK = nz / nZ;
+  double *rH = ac->f[l + 1], *eH = ac->phi[l + 1];
+  const size_t NH = (size_t)nX * nY * nZ;
+  const int *fl = ac->aflag[l + 1];
+  for (int I = 0; I < nX; I++)
+    for (int J = 0; J < nY; J++)
+      for (int K = 0; K < nZ; K++) {
+        double s = 0.0;
+        for (int i = 0; i < CI; i++)
+          for (int j = 0; j < CJ; j++)
+            for (int k = 0; k < CK; k++)
+              s += t[IDX(I * CI + i, J * CJ + j, K * CK + k, ny, nz)];
+        rH[IDX(I, J, K, nY, nZ)] = s;
+      }
+  double sum = 0.0;
+  long cnt = 0;
+  for (size_t Q = 0; Q < NH; Q++) {
+    if (!fl[Q]) { rH[Q] = 0.0; continue; }
+    sum += rH[Q];
+    cnt++;
+  }
+  if (cnt) {
+    sum /= (double)cnt;
+    for (size_t Q = 0; Q < NH; Q++)
+      if (fl[Q]) rH[Q] -= sum;
+  }
+  ora_ac_precond(ac, l + 1, rH, eH, nu1, nu2, omega, cfac);
+  for (int I = 0; I < nX; I++)
+    for (int J = 0; J < nY; J++)
+      for (int K = 0; K < nZ; K++) {
+        double v = cfac * eH[IDX(I, J, K, nY, nZ)];
+        for (int i = 0; i < CI; i++)
+          for (int j = 0; j < CJ; j++)
+            for (int k = 0; k < CK; k++)
+              z[IDX(I * CI + i, J * CJ + j, K * CK + k, ny, nz)] += v;
+      }
+  for (int s = 0; s < nu2; s++) jacobi_sweep(A, r, z, t, omega);
+}
+
+static double nrm2(size_t n, const double *x) { return sqrt(ora_dot(n, x, x)); }
+
+int ora_acpcg(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
+              int nu1, int nu2, double omega, double cfac, double *hist,
+              int *iters) {
+  const ora_mat *A = &ac->A[0];
+  const size_t N = mat_n(A);
+  if (!ac->G) ora_ac_coarse_dense(ac);
+  double *r = dalloc(N), *z = dalloc(N), *p = dalloc(N), *q = dalloc(N);
+  ora_resid(A, b, x, r);
+  double r0 = nrm2(N, r);
+  int it = 0, conv = 0;
+  if (hist) hist[0] = 1.0;
+  if (r0 == 0.0) { conv = 1; goto done; }
+  ora_ac_precond(ac, 0, r, z, nu1, nu2, omega, cfac);
+  memcpy(p, z, N * 8);
+  double rho = ora_dot(N, r, z);
+  for (it = 1; it <= maxit; it++) {
+    ora_spmv(A, p, q);
+    double alpha = rho / ora_dot(N, p, q);
+    for (size_t P = 0; P < N; P++) x[P] += alpha * p[P];
+    for (size_t P = 0; P < N; P++) r[P] -= alpha * q[P];
+    double rel = nrm2(N, r) / r0;
+    if (hist) hist[it] = rel;
+    if (rel < rtol) { conv = 1; break; }
+    ora_ac_precond(ac, 0, r, z, nu1, nu2, omega, cfac);
+    double rho_new = ora_dot(N, r, z);
+    double beta = rho_new / rho;
+    rho = rho_new;
+    for (size_t P = 0; P < N; P++) p[P] = z[P] + beta * p[P];
+  }
+  if (it > maxit) it = maxit;
+done:
+  if (iters) *iters = it;
+  free(r); free(z); free(p); free(q);
+  return conv;
+}
+
+int ora_pcg_rel(const ora_mat *A, double *x, const double *b, double rtol,
+                int maxit, double *hist, int *iters) {
+  const size_t N = mat_n(A);
+  double *r = dalloc(N), *z = dalloc(N), *p = dalloc(N), *q = dalloc(N);
+  ora_resid(A, b, x, r);
+  double r0 = nrm2(N, r);
+  int it = 0, conv = 0;
+  if (hist) hist[0] = 1.0;
+  if (r0 == 0.0) { conv = 1; goto done; }
+  for (size_t P = 0; P < N; P++) z[P] = r[P] / A->c[P];
+  memcpy(p, z, N * 8);
+  double rho = ora_dot(N, r, z);
+  for (it = 1; it <= maxit; it++) {
+    ora_spmv(A, p, q);
+    double alpha = rho / ora_dot(N, p, q);
+    for (size_t P = 0; P < N; P++) x[P] += alpha * p[P];
+    for (size_t P = 0; P < N; P++) r[P] -= alpha * q[P];
+    double rel = nrm2(N, r) / r0;
+    if (hist) hist[it] = rel;
+    if (rel < rtol) { conv = 1; break; }
+    for (size_t P = 0; P < N; P++) z[P] = r[P] / A->c[P];
+    double rho_new = ora_dot(N, r, z);
+    double beta = rho_new / rho;
+    rho = rho_new;
+    for (size_t P = 0; P < N; P++) p[P] = z[P] + beta * p[P];
+  }
+  if (it > maxit) it = maxit;
+done:
+  if (iters) *iters = it;
+  free(r); free(z); free(p); free(q);
+  return conv;
+}

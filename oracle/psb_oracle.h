@@ -1,0 +1,168 @@
+/*
+ * psb_oracle.h -- CPU restatement (FP64, plain C) of PSI-BOIL's pressure-Poisson
+ * solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by the
+ * tests, by the smoke check and by the CPU-baseline leg of the benchmark, and
+ * never by the product library.  Nothing in psi-boil_amd/ links or calls it.
+ *
+ * Layout of every field: interior cells only, index (i*ny + j)*nz + k, k
+ * fastest -- the reference's [i][j][k] order without its BW=3 ghost padding
+ * (Src/Global/global_malloc.cpp:35-56, Src/Field/Scalar/scalar.cpp:77-93).
+ * Neighbours outside the interior are read as 0.0; the reference reads ghost
+ * values there, but every coefficient on a Neumann wall is 0, so the product
+ * is the same (Src/Equation/Centered/centered_system_bnd.cpp:19-61).
+ *
+ * Each function cites the reference file:line it follows.  Which checks pin
+ * which function is listed at the top of psb_oracle.c.
+ */
+#ifndef PSB_ORACLE_H
+#define PSB_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 7-diagonal matrix: c, w, e, s, n, b, t (Src/Matrix/matrix.h:23-54).
+ * Off-diagonals positive; (Ax)_P = c_P x_P - sum a_nb x_nb. */
+typedef struct {
+  int nx, ny, nz;
+  double *c, *w, *e, *s, *n, *b, *t;
+} ora_mat;
+
+typedef struct {
+  int iters;       /* reference's printed count: i+1 (cg.cpp:178)           */
+  int converged;   /* return value of solve()                               */
+  int reason;      /* 0 none/maxiter, 1 res_tol, 2 res_rat, 3 stale,
+                      4 breakdown rho, 5 small s, 6 small omega, 7 res0<tol  */
+  double res0, res;
+} ora_result;
+
+/* ---- splitmix64 and the synthetic case generator ----------------------- */
+uint64_t ora_splitmix64(uint64_t *state);
+double   ora_u01(uint64_t *state);
+
+/* Spheres: centres uniform in [r, L-r] per axis, radius uniform in [rmin,rmax].
+ * out: 4*nsph doubles (cx, cy, cz, r).  Draw order: r, cx, cy, cz.          */
+void ora_gen_spheres(uint64_t seed, int nsph, double rmin, double rmax,
+                     double Lx, double Ly, double Lz, double *out);
+
+/* Cell-centre metrics from interior nodes xn[0..n] with the reference's
+ * default BndGrid::wall() ghost rule (Src/Grid/grid1d_correct_boundaries.cpp).
+ * xc[n], dxc[n], dxw[n] (=dxn(i)), dxe[n] (=dxn(i+1)).                      */
+void ora_metrics(int n, const double *xn, double *xc, double *dxc,
+                 double *dxw, double *dxe);
+
+/* Cell density from spheres: inside <=> |x_c - c| < r (any sphere).  rho is
+ * (nx+2)(ny+2)(nz+2) with a one-cell ghost layer holding the adjacent
+ * interior value (Neumann colour function).                                  */
+void ora_density(int nx, int ny, int nz, const double *xc, const double *yc,
+                 const double *zc, int nsph, const double *sph,
+                 double rho_in, double rho_out, double *rho);
+
+/* Pressure::discretize with all six walls Neumann and an optional voxel
+ * "off" mask (1 = cell inside a solid).  Src/Equation/Centered/Pressure/
+ * pressure_discretize.cpp:11-151, Src/Matter/property_value.cpp:18-32,
+ * Src/Equation/Centered/centered_system_bnd.cpp:19-61.  ci may be NULL.      */
+void ora_discretize(int nx, int ny, int nz,
+                    const double *dxc, const double *dxw, const double *dxe,
+                    const double *dyc, const double *dys, const double *dyn,
+                    const double *dzc, const double *dzb, const double *dzt,
+                    const double *rho, const uint8_t *off,
+                    ora_mat *A, double *ci, double *dV);
+
+/* Pressure::update_rhs with dti = 1: b = (u_w-u_e)dSx + (v_s-v_n)dSy +
+ * (w_b-w_t)dSz (pressure_update_rhs.cpp:27-52), b = 0 on off cells, then the
+ * zero-mean shift over active cells that the benchmark cases add.
+ * u: (nx+1)*ny*nz, v: nx*(ny+1)*nz, w: nx*ny*(nz+1), face index along its
+ * own axis outermost-first like the cells.                                  */
+void ora_rhs(int nx, int ny, int nz, const double *dxc, const double *dyc,
+             const double *dzc, const double *u, const double *v,
+             const double *w, const uint8_t *off, int zero_mean, double *b);
+
+/* Face velocities uniform(-1,1), wall-normal boundary faces 0.
+ * Draw order: all u faces, then v, then w, each in storage order.           */
+void ora_face_velocity(uint64_t seed, int nx, int ny, int nz,
+                       double *u, double *v, double *w);
+
+/* ---- vector kernels ----------------------------------------------------- */
+void   ora_spmv(const ora_mat *A, const double *x, double *y);
+void   ora_resid(const ora_mat *A, const double *b, const double *x, double *r);
+double ora_dot(size_t n, const double *x, const double *y);
+double ora_dot_voldiv_avg(size_t n, const double *x, const double *y,
+                          const double *dV);
+
+/* ---- Krylov / iterative solvers (reference semantics) ------------------- */
+int ora_cg(const ora_mat *A, double *x, const double *b, const double *dV,
+           int mini, int maxit, double res_rat, double res_tol, double scale,
+           int stalecount, double *hist, ora_result *out);
+int ora_bicgs(const ora_mat *A, double *x, const double *b, const double *dV,
+              int mini, int maxit, double res_rat, double res_tol, double scale,
+              int stalecount, double *hist, ora_result *out);
+int ora_jacobi(const ora_mat *A, double *x, const double *b, int mini,
+               int maxit, double res_rat, double res_tol, ora_result *out);
+
+/* ---- additive-correction multigrid -------------------------------------- */
+int  ora_smallest_factor(int n);                 /* Src/Global/global_factor.cpp */
+/* Level sizes by Src/Domain/domain_coarsen.cpp:6-112 (rule 0) or by the
+ * benchmark rule "halve a direction while it stays >= 8" (rule 1).
+ * Returns the number of levels; nl{x,y,z}[0] = n{x,y,z}.                    */
+int  ora_levels(int rule, int nx, int ny, int nz, int maxlev,
+                int *lx, int *ly, int *lz);
+
+typedef struct {
+  int nlev;
+  int nx[24], ny[24], nz[24];
+  ora_mat A[24];
+  int *aflag[24];
+  double *dV[24];
+  double *phi[24], *f[24], *res[24];
+  /* dense coarsest-level operator for the preconditioned form (NULL until
+   * ora_ac_coarse_dense is called) */
+  double *G;
+} ora_ac;
+
+/* Allocate levels, copy the fine matrix, coarsen aflag and the Galerkin
+ * system for every level (additive.cpp:8-48, additive_coarsen_flag.cpp,
+ * additive_coarsen_system.cpp).  aflag0 may be NULL (all active).           */
+int  ora_ac_build(ora_ac *ac, int rule, const ora_mat *A0, const double *dV0,
+                  const uint8_t *off0, int maxlev);
+void ora_ac_free(ora_ac *ac);
+void ora_coarsen_flag(const ora_ac *ac, int l);       /* level l-1 -> l       */
+void ora_coarsen_system(const ora_ac *ac, int l);     /* level l-1 -> l       */
+void ora_restriction(const ora_ac *ac, int l);        /* level l   -> l+1     */
+void ora_interpolation(const ora_ac *ac, int l);      /* level l   -> l-1     */
+
+/* AC::cycle(none, V, toler, factor, mv, ms) with CG-di smoothers on every
+ * level (additive_cycle.cpp, additive_cycle_functions.cpp,
+ * additive_recursive.cpp:4-34).  phi0/f0 are level 0's phi and fnew.
+ * hist: residual after each cycle (max_cyc+1 entries, [0] = initial).       */
+int  ora_ac_vcycle(ora_ac *ac, double *phi0, const double *f0, double toler,
+                   double factor, int max_cyc, int min_cyc, const int mv[3],
+                   double dt, double *hist, int *ncyc);
+
+/* ---- the benchmark's composition (not in the reference) ----------------- */
+/* Dense operator of the coarsest level: G = Pi * (A_red)^-1 * Pi over active
+ * cells, Pi the mean projector, A_red the matrix with its last pivot pinned. */
+int  ora_ac_coarse_dense(ora_ac *ac);
+/* One V-cycle as a preconditioner z = M^-1 r: nu1 damped-Jacobi sweeps from
+ * zero, residual restriction with mean removal, recursion, piecewise-constant
+ * prolongation scaled by cfac, nu2 sweeps; coarsest level z = G r.          */
+void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
+                    int nu2, double omega, double cfac);
+/* PCG with the V-cycle preconditioner; stops on ||r||2/||r0||2 < rtol.
+ * hist[i] = relative residual after iteration i (hist[0] = 1).             */
+int  ora_acpcg(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
+               int nu1, int nu2, double omega, double cfac, double *hist,
+               int *iters);
+/* Diagonal PCG with the same relative-2-norm stopping rule. */
+int  ora_pcg_rel(const ora_mat *A, double *x, const double *b, double rtol,
+                 int maxit, double *hist, int *iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,435 @@
+// ref_driver.cpp -- drives the reference's own classes (Niceno/PSI-BOIL,
+// compiled from its sources by oracle/ref.mk) on the benchmark's synthetic
+// cases and writes what they compute, so the C oracle and the HIP library can
+// be checked against the reference itself.
+//
+// TEST INFRASTRUCTURE ONLY.  Output files (raw little-endian float64 / int32)
+// go to the directory named on the command line; tests/golden/make_golden.py
+// turns small ones into committed fixtures.
+//
+//   ref_driver case  OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel [cx cy cz]
+//   ref_driver bench-cg nx nsph rmin rmax rho_in rho_out seed_sph seed_vel maxit
+//   ref_driver bench-ac nx nsph rmin rmax rho_in rho_out seed_sph seed_vel cycles
+//
+// Private and protected members of the reference's classes (AC::L,
+// Centered::A, ...) are reached by widening access before its headers are
+// read; the layout of the classes does not change.
+#include <algorithm>
+#include <array>
+#include <cassert>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <list>
+#include <map>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#define private public
+#define protected public
+#include "Domain/domain.h"
+#include "Equation/Centered/Pressure/pressure.h"
+#include "Field/Scalar/scalar.h"
+#include "Field/Vector/vector.h"
+#include "Grid/grid1d.h"
+#include "Matter/matter.h"
+#include "SimulationTime/simulation_time.h"
+#include "Solver/Additive/additive.h"
+#include "Solver/Linear/Iterative/iterative.h"
+#include "Solver/Linear/Krylov/krylov.h"
+#undef private
+#undef protected
+
+extern "C" {
+#include "psb_oracle.h"
+}
+
+namespace {
+
+std::string outdir;
+
+void dump(const std::string &name, const std::vector<double> &v) {
+  std::ofstream f(outdir + "/" + name + ".f64", std::ios::binary);
+  f.write(reinterpret_cast<const char *>(v.data()), v.size() * sizeof(double));
+}
+
+void dumpi(const std::string &name, const std::vector<int> &v) {
+  std::ofstream f(outdir + "/" + name + ".i32", std::ios::binary);
+  f.write(reinterpret_cast<const char *>(v.data()), v.size() * sizeof(int));
+}
+
+// interior of a Scalar in [i][j][k] order
+std::vector<double> interior(const Scalar &s) {
+  std::vector<double> v;
+  v.reserve((size_t)(s.ei() - s.si() + 1) * (s.ej() - s.sj() + 1) * (s.ek() - s.sk() + 1));
+  for (int i = s.si(); i <= s.ei(); i++)
+    for (int j = s.sj(); j <= s.ej(); j++)
+      for (int k = s.sk(); k <= s.ek(); k++) v.push_back(s[i][j][k]);
+  return v;
+}
+
+std::vector<int> interior_i(const ScalarInt &s) {
+  std::vector<int> v;
+  for (int i = s.si(); i <= s.ei(); i++)
+    for (int j = s.sj(); j <= s.ej(); j++)
+      for (int k = s.sk(); k <= s.ek(); k++) v.push_back(s[i][j][k]);
+  return v;
+}
+
+void set_interior(Scalar &s, const std::vector<double> &v) {
+  size_t q = 0;
+  for (int i = s.si(); i <= s.ei(); i++)
+    for (int j = s.sj(); j <= s.ej(); j++)
+      for (int k = s.sk(); k <= s.ek(); k++) s[i][j][k] = v[q++];
+}
+
+void neumann(Scalar &s) {
+  s.bc().add(BndCnd(Dir::imin(), BndType::neumann()));
+  s.bc().add(BndCnd(Dir::imax(), BndType::neumann()));
+  s.bc().add(BndCnd(Dir::jmin(), BndType::neumann()));
+  s.bc().add(BndCnd(Dir::jmax(), BndType::neumann()));
+  s.bc().add(BndCnd(Dir::kmin(), BndType::neumann()));
+  s.bc().add(BndCnd(Dir::kmax(), BndType::neumann()));
+}
+
+std::vector<double> seeded(uint64_t seed, size_t n) {
+  std::vector<double> v(n);
+  uint64_t st = seed;
+  for (auto &x : v) x = 2.0 * ora_u01(&st) - 1.0;
+  return v;
+}
+
+// Pressure whose right-hand side is the reference's update_rhs() followed by
+// the zero-mean shift the benchmark cases add.
+class CasePressure : public Pressure {
+ public:
+  CasePressure(const Scalar &p, const Scalar &f, const Vector &u, Times &t,
+               Linear *sm, Matter *m)
+      : Pressure(p, f, u, t, sm, m) {}
+  real update_rhs() {
+    real r = Pressure::update_rhs();
+    double sum = 0.0;
+    long cnt = 0;
+    for_ijk(i, j, k) { sum += fnew[i][j][k]; cnt++; }
+    double mean = sum / (double)cnt;
+    for_ijk(i, j, k) fnew[i][j][k] -= mean;
+    return r;
+  }
+};
+
+struct Case {
+  int nx, ny, nz, nsph;
+  double rmin, rmax, rho_in, rho_out;
+  uint64_t seed_sph, seed_vel;
+  bool centred = false;
+  double cx = 0, cy = 0, cz = 0;
+};
+
+struct World {
+  Grid1D *gx, *gy, *gz;
+  Domain *d;
+  Times *time;
+  Krylov *solver;
+  Vector *uvw;
+  Scalar *p, *f, *c;
+  Matter *in, *out, *mixed;
+  CasePressure *pr;
+  std::vector<double> sph;
+};
+
+World build_world(const Case &cs, Krylov *solver_override = nullptr) {
+  World w;
+  const int nmax = std::max(cs.nx, std::max(cs.ny, cs.nz));
+  const double h = 1.0 / nmax;
+  w.gx = new Grid1D(Range<real>(0.0, cs.nx * h), Range<real>(h, h), cs.nx, Periodic::no());
+  w.gy = new Grid1D(Range<real>(0.0, cs.ny * h), Range<real>(h, h), cs.ny, Periodic::no());
+  w.gz = new Grid1D(Range<real>(0.0, cs.nz * h), Range<real>(h, h), cs.nz, Periodic::no());
+  w.d = new Domain(*w.gx, *w.gy, *w.gz);
+  w.time = new Times(1, 1.0);
+  w.solver = solver_override ? solver_override : new CG(*w.d, Prec::di());
+  w.uvw = new Vector(*w.d);
+  w.p = new Scalar(*w.d);
+  w.f = new Scalar(*w.d);
+  w.c = new Scalar(*w.d);
+  for_m(m) {
+    w.uvw->bc(m).add(BndCnd(Dir::imin(), BndType::wall()));
+    w.uvw->bc(m).add(BndCnd(Dir::imax(), BndType::wall()));
+    w.uvw->bc(m).add(BndCnd(Dir::jmin(), BndType::wall()));
+    w.uvw->bc(m).add(BndCnd(Dir::jmax(), BndType::wall()));
+    w.uvw->bc(m).add(BndCnd(Dir::kmin(), BndType::wall()));
+    w.uvw->bc(m).add(BndCnd(Dir::kmax(), BndType::wall()));
+  }
+  Scalar *sc[2] = {w.p, w.c};
+  for (Scalar *s : sc) neumann(*s);
+  // colour function: 1 inside any sphere (cell-centre test), 0 outside
+  if (cs.centred) {
+    w.sph = {cs.cx, cs.cy, cs.cz, cs.rmin};
+  } else {
+    w.sph.assign(4 * std::max(cs.nsph, 1), 0.0);
+    ora_gen_spheres(cs.seed_sph, cs.nsph, cs.rmin, cs.rmax, cs.nx * h, cs.ny * h,
+                    cs.nz * h, w.sph.data());
+    w.sph.resize(4 * cs.nsph);
+  }
+  Scalar &c = *w.c;
+  for_vijk(c, i, j, k) {
+    bool inside = false;
+    for (size_t s = 0; s + 3 < w.sph.size() && !inside; s += 4) {
+      double dx = c.xc(i) - w.sph[s], dy = c.yc(j) - w.sph[s + 1],
+             dz = c.zc(k) - w.sph[s + 2];
+      if (std::sqrt(dx * dx + dy * dy + dz * dz) < w.sph[s + 3]) inside = true;
+    }
+    c[i][j][k] = inside ? 1.0 : 0.0;
+  }
+  c.bnd_update();
+  c.exchange();
+  w.in = new Matter(*w.d);
+  w.out = new Matter(*w.d);
+  w.in->rho(cs.rho_in);
+  w.in->mu(1.0e-3);
+  w.out->rho(cs.rho_out);
+  w.out->mu(1.0e-5);
+  w.mixed = new Matter(*w.in, *w.out, w.c);
+  // face velocities from the shared generator, wall-normal faces 0
+  std::vector<double> u((size_t)(cs.nx + 1) * cs.ny * cs.nz),
+      v((size_t)cs.nx * (cs.ny + 1) * cs.nz), ww((size_t)cs.nx * cs.ny * (cs.nz + 1));
+  ora_face_velocity(cs.seed_vel, cs.nx, cs.ny, cs.nz, u.data(), v.data(), ww.data());
+  Vector &U = *w.uvw;
+  const int si = w.p->si(), sj = w.p->sj(), sk = w.p->sk();
+  for (int i = 0; i <= cs.nx; i++)
+    for (int j = 0; j < cs.ny; j++)
+      for (int k = 0; k < cs.nz; k++)
+        U[Comp::u()][si + i][sj + j][sk + k] = u[((size_t)i * cs.ny + j) * cs.nz + k];
+  for (int i = 0; i < cs.nx; i++)
+    for (int j = 0; j <= cs.ny; j++)
+      for (int k = 0; k < cs.nz; k++)
+        U[Comp::v()][si + i][sj + j][sk + k] = v[((size_t)i * (cs.ny + 1) + j) * cs.nz + k];
+  for (int i = 0; i < cs.nx; i++)
+    for (int j = 0; j < cs.ny; j++)
+      for (int k = 0; k <= cs.nz; k++)
+        U[Comp::w()][si + i][sj + j][sk + k] = ww[((size_t)i * cs.ny + j) * (cs.nz + 1) + k];
+  w.pr = new CasePressure(*w.p, *w.f, *w.uvw, *w.time, w.solver, w.mixed);
+  w.pr->update_rhs();
+  return w;
+}
+
+void dump_case(World &w, const Case &cs) {
+  CasePressure &pr = *w.pr;
+  const Domain &d = *w.d;
+  const Scalar &p = pr.phi;
+  std::vector<double> xn, yn, zn, m;
+  for (int i = p.si(); i <= p.ei() + 1; i++) xn.push_back(d.xn(i));
+  for (int j = p.sj(); j <= p.ej() + 1; j++) yn.push_back(d.yn(j));
+  for (int k = p.sk(); k <= p.ek() + 1; k++) zn.push_back(d.zn(k));
+  dump("xn", xn); dump("yn", yn); dump("zn", zn);
+  // metrics as Pressure::discretize reads them
+  std::vector<double> dxc, dxw, dxe, dyc, dys, dyn, dzc, dzb, dzt;
+  for (int i = p.si(); i <= p.ei(); i++) { dxc.push_back(d.dxc(i)); dxw.push_back(d.dxw(i)); dxe.push_back(d.dxe(i)); }
+  for (int j = p.sj(); j <= p.ej(); j++) { dyc.push_back(d.dyc(j)); dys.push_back(d.dys(j)); dyn.push_back(d.dyn(j)); }
+  for (int k = p.sk(); k <= p.ek(); k++) { dzc.push_back(d.dzc(k)); dzb.push_back(d.dzb(k)); dzt.push_back(d.dzt(k)); }
+  dump("dxc", dxc); dump("dxw", dxw); dump("dxe", dxe);
+  dump("dyc", dyc); dump("dys", dys); dump("dyn", dyn);
+  dump("dzc", dzc); dump("dzb", dzb); dump("dzt", dzt);
+  dump("spheres", w.sph);
+  // cell density with a one-cell ghost layer, as Property reads it
+  std::vector<double> rho;
+  for (int i = p.si() - 1; i <= p.ei() + 1; i++)
+    for (int j = p.sj() - 1; j <= p.ej() + 1; j++)
+      for (int k = p.sk() - 1; k <= p.ek() + 1; k++) rho.push_back(w.mixed->rho(i, j, k));
+  dump("rho", rho);
+  Matrix &A = pr.A;
+  dump("A_c", interior(A.c)); dump("A_w", interior(A.w)); dump("A_e", interior(A.e));
+  dump("A_s", interior(A.s)); dump("A_n", interior(A.n)); dump("A_b", interior(A.b));
+  dump("A_t", interior(A.t)); dump("A_ci", interior(A.ci));
+  std::vector<double> dV;
+  for_vijk(p, i, j, k) dV.push_back(p.dV(i, j, k));
+  dump("dV", dV);
+  dump("b", interior(pr.fnew));
+  // operators on a seeded vector
+  const size_t N = (size_t)cs.nx * cs.ny * cs.nz;
+  std::vector<double> xr = seeded(7, N);
+  Scalar x(*w.d), y(*w.d);
+  neumann(x);
+  set_interior(x, xr);
+  dump("x_rand", xr);
+  y = A * x;
+  dump("Ax_rand", interior(y));
+  y = pr.fnew - A * x;
+  dump("r_rand", interior(y));
+  std::cout << "DOT " << x.dot(pr.fnew) << " " << x.dot_voldiv_avg(pr.fnew) << "\n";
+}
+
+void run_krylov(World &w, Krylov *sol, const char *tag, const std::vector<int> &ks,
+                int maxit_full, double rr) {
+  CasePressure &pr = *w.pr;
+  for (int k : ks) {
+    Scalar x(*w.d);
+    neumann(x);
+    x = 0.0;
+    std::cout << "RUN " << tag << " " << k << "\n";
+    bool ok = sol->solve(pr.A, x, pr.fnew, MinIter(1), MaxIter(k), tag, ResRat(rr),
+                         ResTol(boil::femto), 1.0, -1, true);
+    std::cout << "RET " << tag << " " << k << " " << ok << "\n";
+    dump(std::string(tag) + "_x" + std::to_string(k), interior(x));
+  }
+  Scalar x(*w.d);
+  neumann(x);
+  x = 0.0;
+  std::cout << "RUN " << tag << " full\n";
+  bool ok = sol->solve(pr.A, x, pr.fnew, MinIter(1), MaxIter(maxit_full), tag,
+                       ResRat(rr), ResTol(boil::femto), 1.0, -1, true);
+  std::cout << "RET " << tag << " full " << ok << "\n";
+  dump(std::string(tag) + "_xfull", interior(x));
+}
+
+void dump_ac(World &w) {
+  CasePressure &pr = *w.pr;
+  AC ac(&pr);
+  std::cout << "NLEV " << ac.nlevels << "\n";
+  for (int l = 1; l < ac.nlevels; l++) ac.coarsen_system(*ac.L[l - 1], *ac.L[l]);
+  for (int l = 0; l < ac.nlevels; l++) {
+    Centered &L = *ac.L[l];
+    std::cout << "LEVEL " << l << " " << L.ei() - L.si() + 1 << " " << L.ej() - L.sj() + 1
+              << " " << L.ek() - L.sk() + 1 << "\n";
+    std::string s = "L" + std::to_string(l) + "_";
+    if (l > 0) {
+      dump(s + "c", interior(L.A.c)); dump(s + "w", interior(L.A.w));
+      dump(s + "e", interior(L.A.e)); dump(s + "s", interior(L.A.s));
+      dump(s + "n", interior(L.A.n)); dump(s + "b", interior(L.A.b));
+      dump(s + "t", interior(L.A.t));
+    }
+    dumpi(s + "aflag", interior_i(L.aflag));
+  }
+  // restriction of level 0 -> 1 with a seeded phi
+  if (ac.nlevels > 1) {
+    Centered &h = *ac.L[0];
+    Centered &H = *ac.L[1];
+    std::vector<double> ph = seeded(11, interior(h.phi).size());
+    std::vector<double> ph0 = interior(h.phi);
+    set_interior(h.phi, ph);
+    ac.restriction(h, H);
+    dump("restrict_phi", ph);
+    dump("restrict_fH", interior(H.fnew));
+    set_interior(h.phi, ph0);
+    std::vector<double> pH = seeded(13, interior(H.phi).size());
+    set_interior(H.phi, pH);
+    h.phi = 0.0;
+    ac.interpolation(H, h);
+    dump("interp_phiH", pH);
+    dump("interp_phih", interior(h.phi));
+    H.phi = 0.0;
+    H.fnew = 0.0;
+    h.phi = 0.0;
+  }
+  // V-cycles: phi after c cycles from phi = 0, and a converged history
+  for (int c = 1; c <= 3; c++) {
+    pr.phi = 0.0;
+    ac.max_cycles(c);
+    std::cout << "ACRUN " << c << "\n";
+    int ncyc = 0;
+    bool ok = ac.vcycle(ResRat(1e-30), &ncyc);
+    std::cout << "ACRET " << c << " " << ok << " " << ncyc << "\n";
+    dump("ac_phi" + std::to_string(c), interior(pr.phi));
+  }
+  pr.phi = 0.0;
+  ac.max_cycles(20);
+  std::cout << "ACRUN full\n";
+  int ncyc = 0;
+  bool ok = ac.vcycle(ResRat(1e-8), &ncyc);
+  std::cout << "ACRET full " << ok << " " << ncyc << "\n";
+  dump("ac_phifull", interior(pr.phi));
+}
+
+Case parse_case(char **a, int nx, int ny, int nz) {
+  Case cs;
+  cs.nx = nx; cs.ny = ny; cs.nz = nz;
+  cs.nsph = std::atoi(a[0]);
+  cs.rmin = std::atof(a[1]);
+  cs.rmax = std::atof(a[2]);
+  cs.rho_in = std::atof(a[3]);
+  cs.rho_out = std::atof(a[4]);
+  cs.seed_sph = std::strtoull(a[5], nullptr, 10);
+  cs.seed_vel = std::strtoull(a[6], nullptr, 10);
+  return cs;
+}
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  std::cout << std::setprecision(17);
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: ref_driver case|bench-cg|bench-ac ...\n");
+    return 2;
+  }
+  std::string mode = argv[1];
+  if (mode == "case") {
+    if (argc < 13) { std::fprintf(stderr, "case: too few arguments\n"); return 2; }
+    outdir = argv[2];
+    Case cs = parse_case(argv + 6, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
+    if (argc >= 16) {
+      cs.centred = true;
+      cs.cx = std::atof(argv[13]); cs.cy = std::atof(argv[14]); cs.cz = std::atof(argv[15]);
+    }
+    World w = build_world(cs);
+    dump_case(w, cs);
+    run_krylov(w, w.solver, "cg", {1, 2, 3, 5, 8, 13, 21}, 2000, 1e-8);
+    BiCGS bi(*w.d, Prec::di());
+    run_krylov(w, &bi, "bicgs", {1, 2, 3, 5, 8}, 2000, 1e-8);
+    Jacobi jac(*w.d);
+    {
+      Scalar x(*w.d);
+      neumann(x);
+      x = 0.0;
+      bool ok = jac.solve(w.pr->A, x, w.pr->fnew, MinIter(1), MaxIter(5), "jacobi",
+                          ResRat(1e-30), ResTol(0.0), 1.0, -1, true);
+      std::cout << "RET jacobi 5 " << ok << "\n";
+      dump("jacobi_x5", interior(x));
+    }
+    dump_ac(w);
+    return 0;
+  }
+  if (mode == "bench-cg" || mode == "bench-ac") {
+    if (argc < 11) { std::fprintf(stderr, "bench: too few arguments\n"); return 2; }
+    int n = std::atoi(argv[2]);
+    Case cs = parse_case(argv + 3, n, n, n);
+    int lim = std::atoi(argv[10]);
+    double t0 = now();
+    World w = build_world(cs);
+    double t1 = now();
+    Scalar x(*w.d);
+    neumann(x);
+    x = 0.0;
+    if (mode == "bench-cg") {
+      double ts = now();
+      bool ok = w.solver->solve(w.pr->A, x, w.pr->fnew, MinIter(1), MaxIter(lim), "cg",
+                                ResRat(1e-8), ResTol(boil::femto), 1.0, -1, true);
+      double te = now();
+      std::cout << "BENCH cg n=" << n << " ok=" << ok << " setup_s=" << (t1 - t0)
+                << " solve_s=" << (te - ts) << "\n";
+    } else {
+      AC ac(w.pr);
+      ac.max_cycles(lim);
+      w.pr->phi = 0.0;
+      int ncyc = 0;
+      double ts = now();
+      bool ok = ac.vcycle(ResRat(1e-8), &ncyc);
+      double te = now();
+      std::cout << "BENCH ac n=" << n << " ok=" << ok << " cycles=" << ncyc
+                << " setup_s=" << (t1 - t0) << " solve_s=" << (te - ts) << "\n";
+    }
+    return 0;
+  }
+  std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
+  return 2;
+}
