@@ -1,0 +1,182 @@
+/*
+ * psb.h -- C ABI of the MI355X pressure-Poisson solver (libpsb.so).
+ *
+ * Drop-in for the solve path of PSI-BOIL: the 7-diagonal Matrix that
+ * Pressure::discretize builds (Src/Matrix/matrix.h:23-54), solved by
+ *   Linear::solve(A, x, b, MinIter, MaxIter, name, ResRat, ResTol, scale,
+ *                 stalecount, precform)        Src/Solver/Linear/linear.h:46-56
+ * with CG or BiCGS and diagonal preconditioning (Src/Solver/Linear/Krylov/
+ * cg.cpp:10-182, bicgs.cpp:9-229, Src/Solver/Preconditioner/
+ * preconditioner_form.cpp:4-13), or with the additive-correction multigrid
+ * (Src/Solver/Additive/) as a CG preconditioner.
+ *
+ * Arrays: plain pointers, cell order [i][j][k] with k fastest, like the
+ * reference's Scalar storage (Src/Field/Scalar/scalar.cpp:77-93).  Host
+ * arrays may carry the reference's ghost padding (psb_layout.ghost = 3, its
+ * BW, Src/Global/global_buffer_width.cpp:4); device arrays are always the
+ * compact interior.  Every entry point returns a psb_status; none throws.
+ * A context is not thread-safe: calls on one context are serialised by the
+ * caller.  Every call is synchronous with respect to host buffers.
+ */
+#ifndef PSB_H
+#define PSB_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct psb_ctx psb_ctx;
+
+/* Status codes.  The reference reports failure only as `false` from
+ * solve() plus a log line; the negative codes have no counterpart there. */
+enum psb_status {
+  PSB_OK = 0,          /* converged (solve() returned true)                  */
+  PSB_MAXITER = 1,     /* MaxIter reached (solve() returned false)           */
+  PSB_STALE = 2,       /* stalecount window without improvement (cg.cpp:138) */
+  PSB_BREAKDOWN = 3,   /* BiCGS rho/omega/s breakdown (bicgs.cpp:89,134,188) */
+  PSB_DIVERGED = 4,    /* refinement or cycle stopped reducing the residual  */
+  PSB_NAN = 5,         /* non-finite residual                                */
+  PSB_EINVAL = -1,     /* bad argument, shape or unsupported matrix           */
+  PSB_EHIP = -2,       /* HIP runtime error                                  */
+  PSB_ENOMEM = -3,     /* device allocation failed                           */
+  PSB_ENOTREADY = -4   /* psb_set_matrix not called                          */
+};
+
+enum psb_method { PSB_CG = 0, PSB_BICGS = 1 };
+enum psb_precond { PSB_PREC_DIAG = 0, PSB_PREC_AC = 1 };
+enum psb_precision { PSB_FP64 = 0, PSB_MIXED = 1 };
+enum psb_norm {
+  PSB_NORM_REF = 0,   /* sqrt(sum (r/dV)^2)/N/scale, cg.cpp:49,121 with
+                         Scalar::dot_voldiv_avg (scalar.h:317-326)          */
+  PSB_NORM_L2REL = 1  /* ||r||_2 compared with res_rat * ||r_0||_2          */
+};
+
+typedef struct {
+  int nx, ny, nz;   /* interior cells                                        */
+  int ghost;        /* ghost cells on each side of host arrays (0 or 3)      */
+} psb_layout;
+
+/* Host arrays of the seven diagonals (positive off-diagonals; (Ax)_P =
+ * c x_P - sum a_nb x_nb).  Couplings across the domain boundary must be 0,
+ * as Centered::create_system_bnd leaves them (centered_system_bnd.cpp:19-61).
+ * If e/n/t mirror w/s/b of the neighbour bit for bit (the pressure matrix,
+ * pressure_discretize.cpp:24-57), only c, w, s, b are stored on the device. */
+typedef struct {
+  const double *c, *w, *e, *s, *n, *b, *t;
+} psb_matrix7;
+
+/* Stopping criteria and method selection: MinIter, MaxIter, ResRat, ResTol,
+ * scale, stalecount, precform of Linear::solve, plus the choices the
+ * reference makes at construction time (CG/BiCGS class, Prec::di()). */
+typedef struct {
+  int min_iter;       /* MinIter  (Src/Ravioli/maxiter.h:31, default 1)      */
+  int max_iter;       /* MaxIter  (maxiter.h:17, default 20)                 */
+  double res_rat;     /* ResRat   (resrat.h:26, default 0.01)                */
+  double res_tol;     /* ResTol   (restol.h:26, default 1e-15)               */
+  double scale;       /* scale    (linear.h:53, default 1.0)                 */
+  int stalecount;     /* stalecount (linear.h:54, default -1 = off)          */
+  int precform;       /* precform (linear.h:55); 1/c is formed on the fly    */
+  int method;         /* psb_method                                          */
+  int precond;        /* psb_precond                                         */
+  int precision;      /* psb_precision                                       */
+  int norm;           /* psb_norm                                            */
+  /* additive-correction preconditioner (PSB_PREC_AC)                       */
+  int ac_pre, ac_post;   /* damped Jacobi sweeps before / after (2, 2)      */
+  double ac_omega;       /* Jacobi damping (0.9)                            */
+  double ac_cfac;        /* coarse-correction factor (2.0)                  */
+  /* iterative refinement (PSB_MIXED)                                       */
+  double inner_rtol;     /* inner FP32 solve: relative L2 reduction (1e-4)   */
+  int max_outer;         /* refinement steps (30)                            */
+  int graph_chunk;       /* iterations per captured graph launch (8)         */
+} psb_stop;
+
+typedef struct {
+  int iterations;    /* reference count i+1 (cg.cpp:178); inner total (IR)   */
+  int outer;         /* refinement steps (PSB_MIXED), else 0                */
+  int converged;     /* solve()'s return value                              */
+  int status;        /* psb_status                                          */
+  double res0, res;  /* initial / final residual in the chosen norm         */
+  double rel_l2;     /* true ||b - A x||_2 / ||b||_2 at exit (FP64)          */
+} psb_result;
+
+/* Fill *s with the reference's defaults (CG, diagonal, FP64, reference
+ * norm, MinIter 1, MaxIter 20, ResRat 0.01, ResTol 1e-15, scale 1). */
+void psb_stop_defaults(psb_stop *s);
+const char *psb_status_string(int status);
+
+/* Linear(const Domain&, const Prec&) / CG(...) / BiCGS(...) construction:
+ * Src/Solver/Linear/linear.cpp:13-35, Krylov/cg.h:17-18.  Allocates device
+ * buffers for the grid and a HIP stream.                                     */
+int psb_create(psb_ctx **ctx, const psb_layout *layout);
+int psb_destroy(psb_ctx *ctx);
+
+/* Cell sizes per direction (Domain::dxc/dyc/dzc, domain.h:81-88), used for
+ * dV = dxc*dyc*dzc in the reference norm (domain_surfaces_volumes.cpp:62-63).
+ * NULL pointers mean unit cells.                                            */
+int psb_set_geometry(psb_ctx *ctx, const double *dxc, const double *dyc,
+                     const double *dzc);
+
+/* Upload A (host arrays in the context's layout), build the FP32 copy and
+ * the AC hierarchy when needed.  Replaces holding a Matrix& across calls.    */
+int psb_set_matrix(psb_ctx *ctx, const psb_matrix7 *A);
+
+/* Off cells (1 = solid, Body::off_p) for the AC hierarchy's active flag
+ * (Centered::set_active_flag, additive_coarsen_flag.cpp:4-41).  Optional.   */
+int psb_set_active(psb_ctx *ctx, const unsigned char *off);
+
+/* Linear::solve (linear.h:46-56): x (in: initial guess, out: solution) and b
+ * are host arrays in the context's layout.                                  */
+int psb_solve(psb_ctx *ctx, double *x, const double *b, const psb_stop *stop,
+              psb_result *res);
+
+/* Same on device-resident compact arrays (hipMalloc'd by the caller). */
+int psb_solve_device(psb_ctx *ctx, double *d_x, const double *d_b,
+                     const psb_stop *stop, psb_result *res);
+
+/* Scalar::operator=(A_x) / (y_m_A_x) (scalar_operators.cpp:22-72) on host
+ * arrays: y = A x, or y = b - A x when b != NULL.  FP64, bit-exact with the
+ * reference. */
+int psb_apply(psb_ctx *ctx, const double *x, const double *b, double *y);
+
+/* Residual history of the last FP64 Krylov solve in the reference norm:
+ * hist[0] = res0, hist[i] = res after iteration i.  Returns the count.     */
+int psb_history(psb_ctx *ctx, double *hist, int cap);
+
+/* Synthetic cases of the benchmark: variable-density pressure matrix on a
+ * uniform (or z-stretched) grid with spheres and an optional solid cylinder,
+ * RHS = divergence of a seeded face-velocity field shifted to zero mean.
+ * Same arithmetic as Pressure::discretize / update_rhs
+ * (pressure_discretize.cpp:11-151, pressure_update_rhs.cpp:16-139).         */
+typedef struct {
+  int nx, ny, nz;
+  int nsph;                       /* random spheres (ignored if centred)     */
+  double rmin, rmax;              /* radius range (absolute length)          */
+  double rho_in, rho_out;         /* density inside / outside spheres        */
+  unsigned long long seed_sph, seed_vel;
+  int centred;                    /* 1: one sphere at (cx,cy,cz), r = rmin  */
+  double cx, cy, cz;
+  double stretch_z;               /* tanh clustering in z (0 = uniform)      */
+  double lz;                      /* z extent when stretched (else n*h)      */
+  int cylinder;                   /* solid cylinder along y                  */
+  double cyl_d;                   /* cylinder diameter / x extent            */
+} psb_case;
+
+/* Writes the seven compact arrays, b, dV, the off mask (may be NULL) and the
+ * per-direction cell sizes dxc[nx], dyc[ny], dzc[nz] (may be NULL).         */
+int psb_generate(const psb_case *cs, double *c, double *w, double *e,
+                 double *s, double *n, double *b_, double *t, double *rhs,
+                 unsigned char *off, double *dxc, double *dyc, double *dzc);
+
+/* Benchmark hook: average device time (ms) of `reps` back-to-back launches
+ * of one kernel on the context's stream, bracketed by HIP events.  kernel:
+ * 0 FP64 SpMV, 1 FP32 SpMV, 2 FP32 damped Jacobi sweep.
+ * *bytes receives the algorithmic bytes one launch moves.                   */
+int psb_time_kernel(psb_ctx *ctx, int kernel, int reps, double *avg_ms,
+                    double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

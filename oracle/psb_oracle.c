@@ -976,14 +976,17 @@ void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
   double *rH = ac->f[l + 1], *eH = ac->phi[l + 1];
   const size_t NH = (size_t)nX * nY * nZ;
   const int *fl = ac->aflag[l + 1];
+  const int *ff = ac->aflag[l]; /* solid fine cells take no part in R = P^T */
   for (int I = 0; I < nX; I++)
     for (int J = 0; J < nY; J++)
       for (int K = 0; K < nZ; K++) {
         double s = 0.0;
         for (int i = 0; i < CI; i++)
           for (int j = 0; j < CJ; j++)
-            for (int k = 0; k < CK; k++)
-              s += t[IDX(I * CI + i, J * CJ + j, K * CK + k, ny, nz)];
+            for (int k = 0; k < CK; k++) {
+              size_t P = IDX(I * CI + i, J * CJ + j, K * CK + k, ny, nz);
+              if (ff[P]) s += t[P];
+            }
         rH[IDX(I, J, K, nY, nZ)] = s;
       }
   double sum = 0.0;
@@ -1005,8 +1008,10 @@ void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
         double v = cfac * eH[IDX(I, J, K, nY, nZ)];
         for (int i = 0; i < CI; i++)
           for (int j = 0; j < CJ; j++)
-            for (int k = 0; k < CK; k++)
-              z[IDX(I * CI + i, J * CJ + j, K * CK + k, ny, nz)] += v;
+            for (int k = 0; k < CK; k++) {
+              size_t P = IDX(I * CI + i, J * CJ + j, K * CK + k, ny, nz);
+              if (ff[P]) z[P] += v;
+            }
       }
   for (int s = 0; s < nu2; s++) jacobi_sweep(A, r, z, t, omega);
 }

@@ -1,0 +1,309 @@
+// psb_device.h -- device-side building blocks of the MI355X pressure solver:
+// grid geometry, the 7-point stencil gather (4-array symmetric or 7-array
+// storage), the deterministic grid reduction and the solver state that the
+// fused kernels share.  Compiled for gfx950 only (wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace psb {
+
+constexpr int kBlock = 256;      // threads per block: 4 wave64 per workgroup
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxLevels = 24;
+constexpr int kStaleMax = 64;
+
+// Interior grid, cell P = (i*ny + j)*nz + k, k fastest (the reference's
+// [i][j][k] order without ghosts).  Sizes fit 32 bits (N < 2^31).
+struct Grid {
+  int nx, ny, nz;
+  int sI;   // ny*nz
+  int N;    // nx*ny*nz
+};
+
+// Device view of a matrix.  SYM: only c, w, s, b are stored and
+// e_P = w_{P+I}, n_P = s_{P+J}, t_P = b_{P+K} (zero on the east/north/top
+// walls).  Otherwise e, n, t are read.
+template <typename T>
+struct DMat {
+  const T *c, *w, *s, *b, *e, *n, *t;
+};
+
+template <typename T, int V>
+struct alignas(sizeof(T) * V) VecN {
+  T v[V];
+};
+
+template <typename T, int V>
+__device__ __forceinline__ VecN<T, V> ldv(const T *p) {
+  return *reinterpret_cast<const VecN<T, V> *>(p);
+}
+template <typename T, int V>
+__device__ __forceinline__ void stv(T *p, const VecN<T, V> &x) {
+  *reinterpret_cast<VecN<T, V> *>(p) = x;
+}
+template <typename T, int V>
+__device__ __forceinline__ VecN<T, V> zerov() {
+  VecN<T, V> r;
+#pragma unroll
+  for (int q = 0; q < V; q++) r.v[q] = T(0);
+  return r;
+}
+
+// Shared solver state: Krylov scalars, stopping data and reduction
+// bookkeeping.  One instance per precision per context, in device memory.
+struct DevState {
+  double rho, alpha, beta, omega, pq, rv;      // Krylov scalars
+  double res0, res, rsq0, rsq;                 // norms
+  double res_rat, res_tol, scale, ntot;        // stopping parameters
+  int mini, maxit, stalecount, norm;           // norm: 0 reference, 1 L2
+  int iter, done, converged, reason;           // progress (reason as oracle)
+  int report_iters, pad0;
+  double *hist;                                // residual per iteration
+  int hist_cap, pad1;
+  double stale[kStaleMax];
+  double mean[kMaxLevels];                     // restriction mean per level
+  double msum[kMaxLevels];
+  double rv_ss, rv_ts, rv_tt;                  // BiCGS dots
+};
+
+// ---------------------------------------------------------------------------
+// reductions: wave64 butterfly, then LDS across the 4 waves, then one
+// partial per block; the last block to arrive sums the partials in a fixed
+// order, so results do not depend on scheduling.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <int NR>
+__device__ __forceinline__ void block_sum(double (&v)[NR], double (*sm)[NR]) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < NR; r++) v[r] = wave_sum(v[r]);
+  if (lane == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; r++) sm[wid][r] = v[r];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    double t = 0.0;
+    for (int q = 0; q < kWaves; q++) t += sm[q][r];
+    v[r] = t;
+  }
+  __syncthreads();
+}
+
+// Returns true in every thread of the last block; out[] then holds the grid
+// totals.  partial: gridDim.x*NR doubles; counter: one unsigned, zero on
+// entry, zero again on exit.
+template <int NR>
+__device__ bool grid_sum(double (&v)[NR], double *partial, unsigned *counter,
+                         double (&out)[NR]) {
+  __shared__ double sm[kWaves][NR];
+  __shared__ int last;
+  block_sum<NR>(v, sm);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NR; r++) partial[(size_t)blockIdx.x * NR + r] = v[r];
+    __threadfence();
+    unsigned t = atomicAdd(counter, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  double acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) acc[r] = 0.0;
+  for (unsigned q = threadIdx.x; q < gridDim.x; q += blockDim.x) {
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+      acc[r] += __hip_atomic_load(&partial[(size_t)q * NR + r], __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+  }
+  block_sum<NR>(acc, sm);
+#pragma unroll
+  for (int r = 0; r < NR; r++) out[r] = acc[r];
+  if (threadIdx.x == 0) atomicExch(counter, 0u);
+  return true;
+}
+
+// Block -> work mapping.  Consecutive block ids land on different XCDs
+// (round-robin over the 8 XCDs of MI355X); remapping gives each XCD a
+// contiguous slab of planes so the i-1/i+1 neighbours of a tile sit in the
+// same L2.
+__device__ __forceinline__ int xcd_block() {
+  int nb = gridDim.x, b = blockIdx.x;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// 7-point stencil gather.  For cell vector P (V consecutive k), returns
+//   cx = c*x_P,  s1 = aW xW + aE xE,  s2 = aS xS + aN xN,  s3 = aB xB + aT xT
+// so that (A x)_P = ((cx - s1) - s2) - s3 and (b - A x)_P =
+// (((b - cx) + s1) + s2) + s3 round exactly like the reference's four passes
+// (Src/Field/Scalar/scalar_operators.cpp:22-72).  Neighbours outside the
+// domain contribute 0 (their coefficient is 0 on a Neumann wall).
+// X provides vec(P) and at(Q) for the operand (a plain array or a value
+// formed on the fly, e.g. z + beta p).
+// ---------------------------------------------------------------------------
+template <typename T, int V>
+struct Gath {
+  VecN<T, V> x, c, cx, s1, s2, s3;
+};
+
+template <typename T, int V, bool SYM, class X>
+__device__ __forceinline__ Gath<T, V> gather(const Grid &g, const DMat<T> &A,
+                                             const X &X_, int P, int i, int j,
+                                             int k) {
+  Gath<T, V> G;
+  VecN<T, V> xw = zerov<T, V>(), xe = zerov<T, V>(), xs = zerov<T, V>(),
+             xn = zerov<T, V>();
+  VecN<T, V> aw = ldv<T, V>(A.w + P), as = ldv<T, V>(A.s + P),
+             ab = ldv<T, V>(A.b + P);
+  VecN<T, V> ae = zerov<T, V>(), an = zerov<T, V>(), at;
+  G.c = ldv<T, V>(A.c + P);
+  G.x = X_.vec(P);
+  if (i > 0) xw = X_.vec(P - g.sI);
+  if (i < g.nx - 1) {
+    xe = X_.vec(P + g.sI);
+    ae = SYM ? ldv<T, V>(A.w + P + g.sI) : ldv<T, V>(A.e + P);
+  } else if (!SYM) {
+    ae = ldv<T, V>(A.e + P);
+  }
+  if (j > 0) xs = X_.vec(P - g.nz);
+  if (j < g.ny - 1) {
+    xn = X_.vec(P + g.nz);
+    an = SYM ? ldv<T, V>(A.s + P + g.nz) : ldv<T, V>(A.n + P);
+  } else if (!SYM) {
+    an = ldv<T, V>(A.n + P);
+  }
+  const bool has_t = (k + V) < g.nz;
+  T xb0 = (k > 0) ? X_.at(P - 1) : T(0);
+  T xtV = has_t ? X_.at(P + V) : T(0);
+  if (SYM) {
+    T btV = has_t ? A.b[P + V] : T(0);
+#pragma unroll
+    for (int q = 0; q < V; q++) at.v[q] = (q < V - 1) ? ab.v[q + 1] : btV;
+  } else {
+    at = ldv<T, V>(A.t + P);
+  }
+#pragma unroll
+  for (int q = 0; q < V; q++) {
+    T xb = (q == 0) ? xb0 : G.x.v[q - 1];
+    T xt = (q == V - 1) ? xtV : G.x.v[q + 1];
+    G.cx.v[q] = G.c.v[q] * G.x.v[q];
+    G.s1.v[q] = aw.v[q] * xw.v[q] + ae.v[q] * xe.v[q];
+    G.s2.v[q] = as.v[q] * xs.v[q] + an.v[q] * xn.v[q];
+    G.s3.v[q] = ab.v[q] * xb + at.v[q] * xt;
+  }
+  return G;
+}
+
+template <typename T, int V>
+__device__ __forceinline__ VecN<T, V> apply_A(const Gath<T, V> &G) {
+  VecN<T, V> y;
+#pragma unroll
+  for (int q = 0; q < V; q++) {
+    T v = G.cx.v[q];
+    v -= G.s1.v[q];
+    v -= G.s2.v[q];
+    v -= G.s3.v[q];
+    y.v[q] = v;
+  }
+  return y;
+}
+
+template <typename T, int V>
+__device__ __forceinline__ VecN<T, V> resid(const VecN<T, V> &b,
+                                            const Gath<T, V> &G) {
+  VecN<T, V> r;
+#pragma unroll
+  for (int q = 0; q < V; q++) {
+    T v = b.v[q] - G.cx.v[q];
+    v += G.s1.v[q];
+    v += G.s2.v[q];
+    v += G.s3.v[q];
+    r.v[q] = v;
+  }
+  return r;
+}
+
+// operand: a plain array
+template <typename T, int V>
+struct ArrX {
+  const T *x;
+  __device__ __forceinline__ VecN<T, V> vec(int P) const { return ldv<T, V>(x + P); }
+  __device__ __forceinline__ T at(int Q) const { return x[Q]; }
+};
+
+// operand: z + beta*p with z = r * (1/c) (diagonal preconditioner,
+// preconditioner_solve.cpp:4-11 with Mc = 1/c, preconditioner_form.cpp:4-13)
+template <typename T, int V>
+struct DiagPX {
+  const T *r, *c, *p;
+  T beta;
+  __device__ __forceinline__ T one(T rr, T cc, T pp) const {
+    T z = rr * (T(1) / cc);
+    return z + beta * pp;
+  }
+  __device__ __forceinline__ VecN<T, V> vec(int P) const {
+    VecN<T, V> R = ldv<T, V>(r + P), C = ldv<T, V>(c + P), Q = ldv<T, V>(p + P), o;
+#pragma unroll
+    for (int q = 0; q < V; q++) o.v[q] = one(R.v[q], C.v[q], Q.v[q]);
+    return o;
+  }
+  __device__ __forceinline__ T at(int Q) const { return one(r[Q], c[Q], p[Q]); }
+};
+
+// operand: z + beta*p with z given (preconditioned form)
+template <typename T, int V>
+struct ZPX {
+  const T *z, *p;
+  T beta;
+  __device__ __forceinline__ VecN<T, V> vec(int P) const {
+    VecN<T, V> Z = ldv<T, V>(z + P), Q = ldv<T, V>(p + P), o;
+#pragma unroll
+    for (int q = 0; q < V; q++) o.v[q] = Z.v[q] + beta * Q.v[q];
+    return o;
+  }
+  __device__ __forceinline__ T at(int Q) const { return z[Q] + beta * p[Q]; }
+};
+
+// operand: e + cfac * eH[parent] (piecewise-constant prolongation,
+// additive_interpolation.cpp:12-64, scaled by the coarse-correction factor);
+// solid cells (af == 0) take no correction, so they stay decoupled
+template <typename T, int V>
+struct ProlX {
+  const T *e, *eH;
+  const uint8_t *af;
+  T cfac;
+  int nz, sI, CI, CJ, CK, nYH, nZH;
+  __device__ __forceinline__ int parent(int Q) const {
+    int i = Q / sI, rem = Q - i * sI, j = rem / nz, k = rem - j * nz;
+    return ((i / CI) * nYH + j / CJ) * nZH + k / CK;
+  }
+  __device__ __forceinline__ T at(int Q) const {
+    if (af && !af[Q]) return e[Q];
+    return e[Q] + cfac * eH[parent(Q)];
+  }
+  __device__ __forceinline__ VecN<T, V> vec(int P) const {
+    VecN<T, V> E = ldv<T, V>(e + P), o;
+    int i = P / sI, rem = P - i * sI, j = rem / nz, k = rem - j * nz;
+    int base = ((i / CI) * nYH + j / CJ) * nZH;
+#pragma unroll
+    for (int q = 0; q < V; q++) {
+      const bool act = !af || af[P + q];
+      o.v[q] = act ? E.v[q] + cfac * eH[base + (k + q) / CK] : E.v[q];
+    }
+    return o;
+  }
+};
+
+}  // namespace psb

@@ -1,0 +1,277 @@
+"""Python front end of libpsb.so, the MI355X pressure-Poisson solver.
+
+Mirrors PSI-BOIL's solve surface (Src/Solver/Linear/linear.h:46-56):
+
+    s = Solver(nx, ny, nz)             # CG(const Domain&, Prec::di())
+    s.set_matrix(A)                    # the Matrix (c, w, e, s, n, b, t)
+    x, res = s.solve(x, b, max_iter=..., res_rat=..., res_tol=...)
+
+The library must be built (psi-boil_amd/build.py); importing this module on a
+machine without it raises instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpsb.so")
+
+PSB_CG, PSB_BICGS = 0, 1
+PSB_PREC_DIAG, PSB_PREC_AC = 0, 1
+PSB_FP64, PSB_MIXED = 0, 1
+PSB_NORM_REF, PSB_NORM_L2REL = 0, 1
+
+STATUS = {0: "OK", 1: "MAXITER", 2: "STALE", 3: "BREAKDOWN", 4: "DIVERGED", 5: "NAN",
+          -1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ENOTREADY"}
+
+
+class PsbError(RuntimeError):
+    pass
+
+
+class Layout(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("ghost", C.c_int)]
+
+
+class Matrix7(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in "cwesnbt"]
+
+
+class Stop(C.Structure):
+    _fields_ = [("min_iter", C.c_int), ("max_iter", C.c_int), ("res_rat", C.c_double),
+                ("res_tol", C.c_double), ("scale", C.c_double), ("stalecount", C.c_int),
+                ("precform", C.c_int), ("method", C.c_int), ("precond", C.c_int),
+                ("precision", C.c_int), ("norm", C.c_int), ("ac_pre", C.c_int),
+                ("ac_post", C.c_int), ("ac_omega", C.c_double), ("ac_cfac", C.c_double),
+                ("inner_rtol", C.c_double), ("max_outer", C.c_int), ("graph_chunk", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("outer", C.c_int), ("converged", C.c_int),
+                ("status", C.c_int), ("res0", C.c_double), ("res", C.c_double),
+                ("rel_l2", C.c_double)]
+
+    def asdict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["status_name"] = STATUS.get(self.status, "?")
+        return d
+
+
+class Case(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nsph", C.c_int),
+                ("rmin", C.c_double), ("rmax", C.c_double), ("rho_in", C.c_double),
+                ("rho_out", C.c_double), ("seed_sph", C.c_ulonglong),
+                ("seed_vel", C.c_ulonglong), ("centred", C.c_int), ("cx", C.c_double),
+                ("cy", C.c_double), ("cz", C.c_double), ("stretch_z", C.c_double),
+                ("lz", C.c_double), ("cylinder", C.c_int), ("cyl_d", C.c_double)]
+
+
+EXPORTS = ["psb_stop_defaults", "psb_status_string", "psb_create", "psb_destroy",
+           "psb_set_geometry", "psb_set_matrix", "psb_set_active", "psb_solve",
+           "psb_solve_device", "psb_apply", "psb_history", "psb_generate",
+           "psb_time_kernel"]
+
+_lib = None
+
+
+def lib():
+    """Load libpsb.so; raises PsbError if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PsbError(f"{LIB_PATH} is missing: build it with psi-boil_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        L.psb_status_string.restype = C.c_char_p
+        for name in EXPORTS:
+            getattr(L, name)
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def stop_defaults(**kw):
+    s = Stop()
+    lib().psb_stop_defaults(C.byref(s))
+    for k, v in kw.items():
+        if not hasattr(s, k):
+            raise KeyError(k)
+        setattr(s, k, v)
+    return s
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise PsbError(f"{what}: {STATUS.get(rc, rc)} ({lib().psb_status_string(rc).decode()})")
+    return rc
+
+
+class Solver:
+    """One solver context (device buffers, hierarchy, stream, graphs)."""
+
+    def __init__(self, nx, ny, nz, ghost=0):
+        self.shape = (nx, ny, nz)
+        self.ghost = ghost
+        self._c = C.c_void_p()
+        _check(lib().psb_create(C.byref(self._c), C.byref(Layout(nx, ny, nz, ghost))),
+               "psb_create")
+
+    def close(self):
+        if self._c:
+            lib().psb_destroy(self._c)
+            self._c = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def host_size(self):
+        g = 2 * self.ghost
+        return (self.shape[0] + g) * (self.shape[1] + g) * (self.shape[2] + g)
+
+    def _host(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        if a.size != self.host_size:
+            raise ValueError(f"array of {a.size} values, layout needs {self.host_size}")
+        return a
+
+    def set_geometry(self, dxc=None, dyc=None, dzc=None):
+        if dxc is None:
+            return _check(lib().psb_set_geometry(self._c, None, None, None), "psb_set_geometry")
+        self._geo = [np.ascontiguousarray(v, dtype=np.float64) for v in (dxc, dyc, dzc)]
+        return _check(lib().psb_set_geometry(self._c, *[_dp(v) for v in self._geo]),
+                      "psb_set_geometry")
+
+    def set_active(self, off):
+        self._off = np.ascontiguousarray(off, dtype=np.uint8).ravel()
+        return _check(lib().psb_set_active(self._c, _dp(self._off)), "psb_set_active")
+
+    def set_matrix(self, A):
+        arrs = [self._host(A[k]) for k in "cwesnbt"]
+        m = Matrix7(*[a.ctypes.data for a in arrs])
+        return _check(lib().psb_set_matrix(self._c, C.byref(m)), "psb_set_matrix")
+
+    def solve(self, x, b, **stop):
+        x = np.array(self._host(x), copy=True)
+        b = self._host(b)
+        s = stop_defaults(**stop)
+        r = Result()
+        _check(lib().psb_solve(self._c, _dp(x), _dp(b), C.byref(s), C.byref(r)), "psb_solve")
+        return x, r.asdict()
+
+    def solve_device(self, d_x, d_b, **stop):
+        """d_x, d_b: device pointers (int) to compact FP64 arrays."""
+        s = stop_defaults(**stop)
+        r = Result()
+        _check(lib().psb_solve_device(self._c, C.c_void_p(d_x), C.c_void_p(d_b), C.byref(s),
+                                      C.byref(r)), "psb_solve_device")
+        return r.asdict()
+
+    def apply(self, x, b=None):
+        x = self._host(x)
+        y = np.zeros(self.host_size)
+        bp = _dp(self._host(b)) if b is not None else None
+        _check(lib().psb_apply(self._c, _dp(x), bp, _dp(y)), "psb_apply")
+        return y
+
+    def history(self, cap=100000):
+        h = np.zeros(cap)
+        n = _check(lib().psb_history(self._c, _dp(h), cap), "psb_history")
+        return h[:n]
+
+    def time_kernel(self, kernel, reps=20):
+        ms, by = C.c_double(), C.c_double()
+        _check(lib().psb_time_kernel(self._c, kernel, reps, C.byref(ms), C.byref(by)),
+               "psb_time_kernel")
+        return ms.value, by.value
+
+
+def generate(nx, ny, nz, nsph=0, rmin=0.0, rmax=0.0, rho_in=1.0, rho_out=1.0, seed_sph=0,
+             seed_vel=0, centre=None, stretch_z=0.0, lz=0.0, cylinder=False, cyl_d=0.2):
+    """Synthetic pressure system; returns dict with c..t, rhs, off, dxc, dyc, dzc."""
+    cs = Case(nx, ny, nz, nsph, rmin, rmax, rho_in, rho_out, seed_sph, seed_vel,
+              1 if centre is not None else 0, *(centre if centre is not None else (0, 0, 0)),
+              stretch_z, lz, 1 if cylinder else 0, cyl_d)
+    N = nx * ny * nz
+    out = {k: np.zeros(N) for k in "cwesnbt"}
+    out["rhs"] = np.zeros(N)
+    out["off"] = np.zeros(N, dtype=np.uint8)
+    out["dxc"], out["dyc"], out["dzc"] = np.zeros(nx), np.zeros(ny), np.zeros(nz)
+    _check(lib().psb_generate(C.byref(cs), *[_dp(out[k]) for k in "cwesnbt"], _dp(out["rhs"]),
+                              _dp(out["off"]), _dp(out["dxc"]), _dp(out["dyc"]),
+                              _dp(out["dzc"])), "psb_generate")
+    return out
+
+
+# --------------------------------------------------------------------------
+# device buffers through the HIP runtime (plumbing for tests and benchmarks)
+# --------------------------------------------------------------------------
+_hip = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipGetErrorString.restype = C.c_char_p
+    return _hip
+
+
+def _hipchk(rc, what):
+    if rc != 0:
+        raise PsbError(f"{what}: {hip().hipGetErrorString(rc).decode()}")
+
+
+class DeviceArray:
+    """A float64 device array (hipMalloc) with host copies in and out."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        self.ptr = C.c_void_p()
+        _hipchk(hip().hipMalloc(C.byref(self.ptr), C.c_size_t(8 * max(self.n, 1))), "hipMalloc")
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        d = cls(a.size)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        _hipchk(hip().hipMemcpy(self.ptr, C.c_void_p(a.ctypes.data), C.c_size_t(8 * self.n), 1),
+                "hipMemcpy H2D")
+
+    def download(self):
+        out = np.empty(self.n)
+        _hipchk(hip().hipMemcpy(C.c_void_p(out.ctypes.data), self.ptr, C.c_size_t(8 * self.n), 2),
+                "hipMemcpy D2H")
+        return out
+
+    def zero(self):
+        _hipchk(hip().hipMemset(self.ptr, 0, C.c_size_t(8 * self.n)), "hipMemset")
+
+    @property
+    def addr(self):
+        return self.ptr.value
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_synchronize():
+    _hipchk(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")

@@ -1,0 +1,109 @@
+"""Write the golden fixtures of the pressure-solve path from the reference itself.
+
+Runs oracle/_ref/ref_driver (the reference's own Pressure, CG, BiCGS, Jacobi
+and AC classes compiled from /root/reference/Src by oracle/ref.mk) on small
+synthetic cases and stores what it computed as tests/golden/<case>.npz:
+inputs (grid nodes, metrics, density, matrix, right-hand side), operator
+outputs, solver iterates after k iterations, AC level matrices, restriction /
+interpolation results and V-cycle iterates, plus the solver log lines (17
+significant digits) parsed into arrays.
+
+    python tests/golden/make_golden.py          # needs oracle/_ref built
+
+Only the outputs travel; the reference does not.
+"""
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+
+# name: (nx, ny, nz, nsph, rmin, rmax, rho_in, rho_out, seed_sph, seed_vel, centre)
+CASES = {
+    "drop12": (12, 12, 12, 1, 0.25, 0.25, 1000.0, 1.0, 1, 1001, (0.5, 0.5, 0.5)),
+    "bubbles16x12x8": (16, 12, 8, 3, 0.1, 0.25, 1.0, 1000.0, 5, 1005, None),
+    "bubbles16": (16, 16, 16, 4, 0.08, 0.2, 1.0, 1000.0, 2, 1002, None),
+}
+
+KEEP_F64 = ["xn", "yn", "zn", "dxc", "dxw", "dxe", "dyc", "dys", "dyn", "dzc", "dzb", "dzt",
+            "spheres", "rho", "A_c", "A_w", "A_e", "A_s", "A_n", "A_b", "A_t", "A_ci", "dV",
+            "b", "x_rand", "Ax_rand", "r_rand", "cg_x1", "cg_x5", "cg_xfull", "bicgs_x1",
+            "bicgs_xfull", "jacobi_x5", "restrict_phi", "restrict_fH", "interp_phiH",
+            "interp_phih", "ac_phi1", "ac_phifull"]
+
+NUM = r"([-+0-9.eE]+|nan|inf)"
+
+
+def parse_log(text):
+    info = {"cg": [], "bicgs": [], "jacobi": [], "ret": [], "levels": [], "cycles": [],
+            "acret": []}
+    for line in text.splitlines():
+        m = re.match(r"(cg|bicgs|jacobi), residual = " + NUM + ", ratio = " + NUM +
+                     ", iterations = (\d+)", line)
+        if m:
+            info[m.group(1)].append([float(m.group(2)), float(m.group(3)), int(m.group(4))])
+            continue
+        m = re.match(r"RET (\w+) (\w+) (\d)", line)
+        if m:
+            info["ret"].append([m.group(1), m.group(2), int(m.group(3))])
+            continue
+        m = re.match(r"LEVEL (\d+) (\d+) (\d+) (\d+)", line)
+        if m:
+            info["levels"].append([int(m.group(i)) for i in range(2, 5)])
+            continue
+        m = re.match(r"Cycle (\d+); res = " + NUM, line)
+        if m:
+            info["cycles"].append([int(m.group(1)), float(m.group(2))])
+            continue
+        m = re.match(r"ACRET (\w+) (\d) (\d+)", line)
+        if m:
+            info["acret"].append([m.group(1), int(m.group(2)), int(m.group(3))])
+        m = re.match(r"Initial res = " + NUM, line)
+        if m:
+            info.setdefault("initial", []).append(float(m.group(1)))
+    return info
+
+
+def run_case(name, spec):
+    nx, ny, nz, nsph, rmin, rmax, rin, rout, ss, sv, centre = spec
+    with tempfile.TemporaryDirectory() as d:
+        args = [DRIVER, "case", d, str(nx), str(ny), str(nz), str(nsph), repr(rmin), repr(rmax),
+                repr(rin), repr(rout), str(ss), str(sv)]
+        if centre is not None:
+            args += [repr(c) for c in centre]
+        out = subprocess.run(args, check=True, capture_output=True, text=True).stdout
+        arrays = {}
+        for k in KEEP_F64:
+            arrays[k] = np.fromfile(os.path.join(d, k + ".f64"))
+        info = parse_log(out)
+        for l in range(1, len(info["levels"])):
+            for c in "cwesnbt":
+                arrays[f"L{l}_{c}"] = np.fromfile(os.path.join(d, f"L{l}_{c}.f64"))
+        for l in range(len(info["levels"])):
+            arrays[f"L{l}_aflag"] = np.fromfile(os.path.join(d, f"L{l}_aflag.i32"), dtype=np.int32)
+    meta = {"name": name, "shape": [nx, ny, nz], "nsph": nsph, "rmin": rmin, "rmax": rmax,
+            "rho_in": rin, "rho_out": rout, "seed_sph": ss, "seed_vel": sv,
+            "centre": list(centre) if centre is not None else None,
+            "cg_snapshots": [1, 5], "bicgs_snapshots": [1], "info": info}
+    arrays["meta"] = np.array(json.dumps(meta))
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **arrays)
+    return path
+
+
+def main():
+    if not os.path.exists(DRIVER):
+        sys.exit("oracle/_ref/ref_driver is not built (make -f oracle/ref.mk)")
+    for name, spec in CASES.items():
+        print(run_case(name, spec))
+
+
+if __name__ == "__main__":
+    main()

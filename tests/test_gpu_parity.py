@@ -1,0 +1,208 @@
+"""Parity of the HIP path (libpsb.so, through its C ABI) with the reference's
+own outputs (golden fixtures) and with the oracle.
+
+Tolerances: SpMV / residual are bit-exact (same per-cell rounding as the
+reference's Scalar operators).  Krylov iterates differ only through the
+order of the dot-product sums (tree vs sequential), so iterates after a few
+iterations agree to 1e-12 relative, residual histories to 1e-9 relative and
+iteration counts to +-1."""
+import numpy as np
+import pytest
+
+import psb
+import psb_oracle as O
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def solver_for(g, ghost=0):
+    s = psb.Solver(*g.shape, ghost=ghost)
+    s.set_matrix(g.mat())
+    s.set_geometry(g["dxc"], g["dyc"], g["dzc"])
+    return s
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_spmv_and_residual_bit_exact_vs_reference(golden):
+    s = solver_for(golden)
+    assert np.array_equal(s.apply(golden["x_rand"]), golden["Ax_rand"])
+    assert np.array_equal(s.apply(golden["x_rand"], golden["b"]), golden["r_rand"])
+
+
+def test_cg_diag_matches_reference(golden):
+    s = solver_for(golden)
+    b = golden["b"]
+    n = b.size
+    for k in golden.meta["cg_snapshots"]:
+        x, r = s.solve(np.zeros(n), b, max_iter=k, res_rat=1e-8)
+        assert r["iterations"] == k + 1 and not r["converged"]
+        assert rel(x, golden["cg_x%d" % k]) < 1e-12
+    x, r = s.solve(np.zeros(n), b, max_iter=2000, res_rat=1e-8)
+    res, ratio, iters = golden.info["cg"][-1]
+    assert r["converged"] and r["status"] == 0
+    assert abs(r["iterations"] - iters) <= 1
+    assert rel(x, golden["cg_xfull"]) < 1e-7
+    # residual history against the oracle's (same norm, same stopping rule)
+    A = O.Mat(*golden.shape, golden.mat())
+    xo, info = O.cg(A, np.zeros(n), b, golden["dV"], maxit=2000, res_rat=1e-8)
+    h = s.history()
+    m = min(len(h), len(info["hist"]), 30)
+    np.testing.assert_allclose(h[:m], info["hist"][:m], rtol=1e-9)
+    assert r["res0"] == pytest.approx(info["res0"], rel=1e-13)
+
+
+def test_bicgs_diag_matches_reference(golden):
+    s = solver_for(golden)
+    b = golden["b"]
+    n = b.size
+    x, r = s.solve(np.zeros(n), b, max_iter=1, res_rat=1e-8, method=psb.PSB_BICGS)
+    assert rel(x, golden["bicgs_x1"]) < 1e-12
+    x, r = s.solve(np.zeros(n), b, max_iter=2000, res_rat=1e-8, method=psb.PSB_BICGS)
+    res, ratio, iters = golden.info["bicgs"][-1]
+    assert r["converged"]
+    assert abs(r["iterations"] - iters) <= 2
+    assert rel(x, golden["bicgs_xfull"]) < 1e-6
+
+
+def test_reference_ghost_layout():
+    g = load_golden("bubbles16x12x8")
+    nx, ny, nz = g.shape
+    G = 3
+
+    def pad(a):
+        out = np.full((nx + 2 * G, ny + 2 * G, nz + 2 * G), 7.0)
+        out[G:-G, G:-G, G:-G] = a.reshape(nx, ny, nz)
+        return out.ravel()
+
+    s = psb.Solver(nx, ny, nz, ghost=G)
+    s.set_matrix({k: pad(g["A_" + k]) for k in "cwesnbt"})
+    s.set_geometry(g["dxc"], g["dyc"], g["dzc"])
+    y = s.apply(pad(g["x_rand"])).reshape(nx + 2 * G, ny + 2 * G, nz + 2 * G)
+    assert np.array_equal(y[G:-G, G:-G, G:-G].ravel(), g["Ax_rand"])
+    x, r = s.solve(pad(np.zeros(nx * ny * nz)), pad(g["b"]), max_iter=5, res_rat=1e-8)
+    x = x.reshape(nx + 2 * G, ny + 2 * G, nz + 2 * G)
+    assert rel(x[G:-G, G:-G, G:-G].ravel(), g["cg_x5"]) < 1e-12
+    assert np.all(x[:G] == 7.0)  # ghosts untouched
+
+
+def test_nonsymmetric_seven_array_path_and_odd_sizes():
+    nx, ny, nz = 9, 10, 7
+    rng = np.random.default_rng(3)
+    n = nx * ny * nz
+    A = {k: rng.uniform(0.1, 1.0, n) for k in "wesnbt"}
+    g = lambda a: a.reshape(nx, ny, nz)
+    g(A["w"])[0] = 0; g(A["e"])[-1] = 0; g(A["s"])[:, 0] = 0; g(A["n"])[:, -1] = 0
+    g(A["b"])[:, :, 0] = 0; g(A["t"])[:, :, -1] = 0
+    A["c"] = sum(A[k] for k in "wesnbt") + 0.5
+    s = psb.Solver(nx, ny, nz)
+    s.set_matrix(A)
+    Am = O.Mat(nx, ny, nz, A)
+    x = rng.standard_normal(n)
+    b = rng.standard_normal(n)
+    assert np.array_equal(s.apply(x), O.spmv(Am, x))
+    assert np.array_equal(s.apply(x, b), O.resid(Am, b, x))
+    dV = np.ones(n)
+    xg, r = s.solve(np.zeros(n), b, max_iter=200, res_rat=1e-10, method=psb.PSB_BICGS)
+    xo, info = O.bicgs(Am, np.zeros(n), b, dV, maxit=200, res_rat=1e-10)
+    assert r["converged"] == info["converged"]
+    assert abs(r["iterations"] - info["iters"]) <= 2
+    assert rel(xg, xo) < 1e-8
+
+
+def test_stopping_rules_match_oracle():
+    g = load_golden("bubbles16")
+    s = solver_for(g)
+    A = O.Mat(*g.shape, g.mat())
+    b, dV, n = g["b"], g["dV"], g["b"].size
+    # MaxIter exhausted: converged false, printed count MaxIter+1 (cg.cpp:178)
+    x, r = s.solve(np.zeros(n), b, max_iter=7, res_rat=1e-12)
+    assert r["status"] == 1
+    assert r["iterations"] == 8 and not r["converged"]
+    # ResTol above the initial residual: immediate true (cg.cpp:64)
+    x, r = s.solve(np.zeros(n), b, max_iter=50, res_tol=1e6)
+    assert r["converged"] and r["iterations"] == 0 and np.all(x == 0)
+    # MinIter gates the ratio test
+    xo, info = O.cg(A, np.zeros(n), b, dV, mini=30, maxit=200, res_rat=0.5)
+    x, r = s.solve(np.zeros(n), b, min_iter=30, max_iter=200, res_rat=0.5)
+    assert r["iterations"] == info["iters"] == 30
+    # scale divides the norm
+    x, r = s.solve(np.zeros(n), b, max_iter=3, scale=4.0)
+    xo, info = O.cg(A, np.zeros(n), b, dV, maxit=3, scale=4.0)
+    assert r["res0"] == pytest.approx(info["res0"], rel=1e-13)
+    # staleness window
+    xo, info = O.cg(A, np.zeros(n), b, dV, maxit=400, res_rat=1e-30, res_tol=0.0, stalecount=3)
+    x, r = s.solve(np.zeros(n), b, max_iter=400, res_rat=1e-30, res_tol=0.0, stalecount=3)
+    assert info["reason"] == 3 and r["status"] == 2
+    assert abs(r["iterations"] - info["iters"]) <= 3
+
+
+def test_nan_input_is_reported():
+    g = load_golden("drop12")
+    s = solver_for(g)
+    b = g["b"].copy()
+    b[5] = np.nan
+    x, r = s.solve(np.zeros(b.size), b, max_iter=10)
+    assert r["status"] == 5 and not r["converged"]
+
+
+def ac_case(n, nsph=4, ratio=1000.0, seed=2, ny=None, nz=None):
+    ny, nz = ny or n, nz or n
+    d = psb.generate(n, ny, nz, nsph, 0.1, 0.25, 1.0, ratio, seed, 1000 + seed)
+    A = O.Mat(n, ny, nz, {k: d[k] for k in "cwesnbt"})
+    return d, A
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 32, 16)])
+def test_acpcg_fp64_matches_oracle(shape):
+    nx, ny, nz = shape
+    d, A = ac_case(nx, ny=ny, nz=nz)
+    b = d["rhs"]
+    n = b.size
+    s = psb.Solver(nx, ny, nz)
+    s.set_matrix(d)
+    x, r = s.solve(np.zeros(n), b, precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL,
+                   res_rat=1e-8, max_iter=200)
+    ac = O.AC(A, np.ones(n), rule=1)
+    xo, info = ac.acpcg(np.zeros(n), b, rtol=1e-8, maxit=200, nu1=2, nu2=2, omega=0.9, cfac=2.0)
+    assert r["converged"] and info["converged"]
+    assert abs(r["iterations"] - info["iters"]) <= 1
+    assert r["rel_l2"] < 1.2e-8
+    assert rel(x - x.mean(), xo - xo.mean()) < 1e-6
+
+
+def test_mixed_precision_refinement_reaches_fp64_accuracy():
+    d, A = ac_case(48, nsph=6, seed=3)
+    b = d["rhs"]
+    n = b.size
+    s = psb.Solver(48, 48, 48)
+    s.set_matrix(d)
+    x, r = s.solve(np.zeros(n), b, precond=psb.PSB_PREC_AC, precision=psb.PSB_MIXED,
+                   norm=psb.PSB_NORM_L2REL, res_rat=1e-10, max_iter=500, inner_rtol=1e-4)
+    assert r["converged"] and r["outer"] >= 2
+    assert r["rel_l2"] <= 1e-10 * 1.01
+    # against an FP64 solve to 1e-12 by the oracle
+    ac = O.AC(A, np.ones(n), rule=1)
+    xo, info = ac.acpcg(np.zeros(n), b, rtol=1e-12, maxit=400)
+    err = rel(x - x.mean(), xo - xo.mean())
+    assert err < 1e-7
+
+
+def test_acpcg_with_solid_cells_and_stretched_grid():
+    nx, ny, nz = 32, 16, 64
+    d = psb.generate(nx, ny, nz, 8, 0.01, 0.04, 1.0, 1000.0, 7, 1007, stretch_z=2.5, lz=2.0,
+                     cylinder=True, cyl_d=0.2)
+    assert d["off"].sum() > 0
+    b, n = d["rhs"], d["rhs"].size
+    s = psb.Solver(nx, ny, nz)
+    s.set_active(d["off"])
+    s.set_matrix(d)
+    s.set_geometry(d["dxc"], d["dyc"], d["dzc"])
+    x, r = s.solve(np.zeros(n), b, precond=psb.PSB_PREC_AC, precision=psb.PSB_MIXED,
+                   norm=psb.PSB_NORM_L2REL, res_rat=1e-9, max_iter=1000)
+    assert r["converged"] and r["rel_l2"] <= 1e-9 * 1.01
+    off = d["off"].astype(bool)
+    assert np.all(x[off] == 0.0)
