@@ -90,6 +90,9 @@ typedef struct {
   double inner_rtol;     /* inner FP32 solve: relative L2 reduction (1e-4)   */
   int max_outer;         /* refinement steps (30)                            */
   int graph_chunk;       /* iterations per captured graph launch (8)         */
+  int inner_adapt;       /* 1: each inner solve aims at the outer target,
+                            max(inner_rtol, 0.5 target / current residual);
+                            0: fixed inner_rtol (default)                     */
 } psb_stop;
 
 typedef struct {
@@ -140,8 +143,9 @@ int psb_solve_device(psb_ctx *ctx, double *d_x, const double *d_b,
  * reference. */
 int psb_apply(psb_ctx *ctx, const double *x, const double *b, double *y);
 
-/* Residual history of the last FP64 Krylov solve in the reference norm:
- * hist[0] = res0, hist[i] = res after iteration i.  Returns the count.     */
+/* Residual history of the last solve in its norm: hist[0] = res0, hist[i] =
+ * res after iteration i; for PSB_MIXED the FP64 residual before each
+ * refinement step.  Returns the count.                                      */
 int psb_history(psb_ctx *ctx, double *hist, int cap);
 
 /* Synthetic cases of the benchmark: variable-density pressure matrix on a

@@ -968,7 +968,11 @@ void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
   /* first sweep from zero: e = omega r / c */
   for (size_t P = 0; P < N; P++) z[P] = nu1 > 0 ? omega * r[P] / A->c[P] : 0.0;
   for (int s = 1; s < nu1; s++) jacobi_sweep(A, r, z, t, omega);
-  /* residual and restriction (block sum, inactive 0, mean removed) */
+  /* residual and restriction (block sum, inactive 0).  Unlike the
+   * reference's stand-alone cycle (additive_restriction.cpp:151-174) the
+   * mean is not removed here: the residual sums to zero up to rounding
+   * (constants are in the left null space), the Jacobi smoothers do not
+   * drift on it, and the coarsest operator G projects it out. */
   ora_resid(A, r, z, t);
   const int nX = ac->nx[l + 1], nY = ac->ny[l + 1], nZ = ac->nz[l + 1];
   const int ny = A->ny, nz = A->nz;
@@ -989,18 +993,8 @@ void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
             }
         rH[IDX(I, J, K, nY, nZ)] = s;
       }
-  double sum = 0.0;
-  long cnt = 0;
-  for (size_t Q = 0; Q < NH; Q++) {
-    if (!fl[Q]) { rH[Q] = 0.0; continue; }
-    sum += rH[Q];
-    cnt++;
-  }
-  if (cnt) {
-    sum /= (double)cnt;
-    for (size_t Q = 0; Q < NH; Q++)
-      if (fl[Q]) rH[Q] -= sum;
-  }
+  for (size_t Q = 0; Q < NH; Q++)
+    if (!fl[Q]) rH[Q] = 0.0;
   ora_ac_precond(ac, l + 1, rH, eH, nu1, nu2, omega, cfac);
   for (int I = 0; I < nX; I++)
     for (int J = 0; J < nY; J++)
@@ -1018,6 +1012,21 @@ void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
 
 static double nrm2(size_t n, const double *x) { return sqrt(ora_dot(n, x, x)); }
 
+/* z -= mean(z) over the active cells: keeps the search directions out of the
+ * constant null space of the Neumann operator */
+static void project_mean(const ora_ac *ac, double *z) {
+  const size_t N = mat_n(&ac->A[0]);
+  const int *fl = ac->aflag[0];
+  double s = 0.0;
+  long cnt = 0;
+  for (size_t P = 0; P < N; P++)
+    if (fl[P]) { s += z[P]; cnt++; }
+  if (!cnt) return;
+  s /= (double)cnt;
+  for (size_t P = 0; P < N; P++)
+    if (fl[P]) z[P] -= s;
+}
+
 int ora_acpcg(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
               int nu1, int nu2, double omega, double cfac, double *hist,
               int *iters) {
@@ -1031,6 +1040,7 @@ int ora_acpcg(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
   if (hist) hist[0] = 1.0;
   if (r0 == 0.0) { conv = 1; goto done; }
   ora_ac_precond(ac, 0, r, z, nu1, nu2, omega, cfac);
+  project_mean(ac, z);
   memcpy(p, z, N * 8);
   double rho = ora_dot(N, r, z);
   for (it = 1; it <= maxit; it++) {
@@ -1042,6 +1052,7 @@ int ora_acpcg(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
     if (hist) hist[it] = rel;
     if (rel < rtol) { conv = 1; break; }
     ora_ac_precond(ac, 0, r, z, nu1, nu2, omega, cfac);
+    project_mean(ac, z);
     double rho_new = ora_dot(N, r, z);
     double beta = rho_new / rho;
     rho = rho_new;

@@ -149,11 +149,12 @@ int  ora_ac_vcycle(ora_ac *ac, double *phi0, const double *f0, double toler,
  * cells, Pi the mean projector, A_red the matrix with its last pivot pinned. */
 int  ora_ac_coarse_dense(ora_ac *ac);
 /* One V-cycle as a preconditioner z = M^-1 r: nu1 damped-Jacobi sweeps from
- * zero, residual restriction with mean removal, recursion, piecewise-constant
+ * zero, residual restriction (block sums), recursion, piecewise-constant
  * prolongation scaled by cfac, nu2 sweeps; coarsest level z = G r.          */
 void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
                     int nu2, double omega, double cfac);
-/* PCG with the V-cycle preconditioner; stops on ||r||2/||r0||2 < rtol.
+/* PCG with the V-cycle preconditioner, its output projected to zero mean over
+ * the active cells; stops on ||r||2/||r0||2 < rtol.
  * hist[i] = relative residual after iteration i (hist[0] = 1).             */
 int  ora_acpcg(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
                int nu1, int nu2, double omega, double cfac, double *hist,

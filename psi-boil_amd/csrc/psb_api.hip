@@ -4,8 +4,10 @@
 // for mixed precision), the additive-correction hierarchy, the Krylov work
 // vectors, a HIP stream and the captured graphs of the iteration bodies.
 // The loops of Linear::solve (cg.cpp, bicgs.cpp) run on the device: each
-// graph launch performs `graph_chunk` iterations, the kernels test the
-// device-side stop flag, and the host only polls that flag between launches.
+// graph launch performs `graph_chunk` iterations, every reduction is a
+// streaming kernel plus a one-block finalize kernel that applies the scalar
+// step and the stop test, the streaming kernels test the device-side stop
+// flag, and the host only polls that flag between graph launches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,8 +17,8 @@
 #include <functional>
 #include <memory>
 #include <stdexcept>
-#include <type_traits>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/psb.h"
@@ -36,12 +38,25 @@ struct HipError {
     if (e_ != hipSuccess) throw HipError{e_};   \
   } while (0)
 
-int grid_blocks(long long nvec) {
-  long long nb = (nvec + kBlock - 1) / kBlock;
-  if (nb > 2048) nb = 2048;
+// blocks for a streaming kernel over `nwork` items: one pass of 256 threads
+// per item run, at most kMaxBlocks (8 per CU on the 256 CUs), a multiple of
+// the 8 XCDs so the XCD remap applies
+int grid_blocks(long long nwork) {
+  long long nb = (nwork + kBlock - 1) / kBlock;
+  if (nb > kMaxBlocks) nb = kMaxBlocks;
   if (nb >= 8) nb = (nb / 8) * 8;
   if (nb < 1) nb = 1;
   return (int)nb;
+}
+
+FastDiv make_div(uint32_t d) {
+  uint32_t s = 0;
+  while ((1ull << s) < d) s++;
+  FastDiv f;
+  f.d = d;
+  f.m = (uint32_t)(((1ull << (31 + s)) + d - 1) / d);
+  f.sh = 31 + s;
+  return f;
 }
 
 Grid make_grid(int nx, int ny, int nz) {
@@ -49,6 +64,8 @@ Grid make_grid(int nx, int ny, int nz) {
   g.nx = nx; g.ny = ny; g.nz = nz;
   g.sI = ny * nz;
   g.N = nx * ny * nz;
+  g.dI = make_div((uint32_t)g.sI);
+  g.dZ = make_div((uint32_t)nz);
   return g;
 }
 
@@ -74,7 +91,7 @@ struct Arena {
 template <typename T>
 struct Level {
   Grid g;
-  int CI = 1, CJ = 1, CK = 1;      // factor from the finer level
+  int CI = 1, CJ = 1, CK = 1;      // factor from the finer level (1 or 2)
   T *c = nullptr, *w = nullptr, *s = nullptr, *b = nullptr;
   T *r = nullptr;                  // right-hand side of this level
   T *e[3] = {nullptr, nullptr, nullptr};
@@ -94,7 +111,6 @@ template <typename T>
 struct KrylovVecs {
   T *x = nullptr, *r = nullptr, *q = nullptr, *p[2] = {nullptr, nullptr};
   T *v[2] = {nullptr, nullptr}, *s = nullptr, *t = nullptr, *rt = nullptr;
-  T *b = nullptr;
 };
 
 struct Graph {
@@ -125,7 +141,8 @@ struct psb_ctx {
   DevState *hst = nullptr;  // pinned host mirror
   double *hist = nullptr;
   int hist_cap = 0;
-  std::vector<double> hist_host;
+  std::vector<double> hist_host;  // refinement: FP64 residual per outer step
+  bool hist_on_host = false;
   int hist_count = 0;
   Work work{};
   KrylovVecs<double> k64;
@@ -133,10 +150,9 @@ struct psb_ctx {
   Hier<double> h64;
   Hier<float> h32;
   double *db = nullptr, *dx = nullptr;  // staging for host-array solves
-  double *dy = nullptr;                 // scratch for psb_apply / residual checks
+  double *dy = nullptr;                 // scratch for psb_apply
   double *relres = nullptr;             // 2 doubles: sum r^2, sum b^2
   Graph gr_cgdiag, gr_bicgs, gr_ac64, gr_ac32;
-  int ac_nu1 = -1, ac_nu2 = -1;
   double ac_omega = 0, ac_cfac = 0;
   std::string err;
 };
@@ -152,139 +168,147 @@ DMat<float> mat32(const psb_ctx *c) {
 }
 Metric metric(const psb_ctx *c) { return Metric{c->dxc, c->dyc, c->dzc}; }
 
-constexpr int VM64 = 2, VM32 = 4;  // 16-byte vectors
-
-// ---------------------------------------------------------------------------
-// launch helpers: V = 16 bytes / sizeof(T) when nz allows, else 1
-// ---------------------------------------------------------------------------
+// 16-byte vectors: V = 2 doubles or 4 floats when nz allows, else 1
 template <typename T>
-constexpr int vmax() { return sizeof(T) == 8 ? VM64 : VM32; }
+constexpr int vmax() { return sizeof(T) == 8 ? 2 : 4; }
+template <typename T>
+bool vec_ok(const Grid &g) { return g.nz % vmax<T>() == 0; }
+template <typename T>
+int nblocks(const Grid &g) { return grid_blocks(vec_ok<T>(g) ? g.N / vmax<T>() : g.N); }
 
-
-template <typename T, int V, bool SYM>
-void spmv_v(hipStream_t stream, const Grid &g, DMat<T> A, const T *x, T *y) {
-  hipLaunchKernelGGL((k_spmv<T, V, SYM>), dim3(grid_blocks(g.N / V)), dim3(kBlock), 0, stream,
-                     g, A, x, y);
+// calls f(std::integral_constant<int, V>) with the vector width for g
+template <typename T, typename F>
+void with_v(const Grid &g, F f) {
+  if (vec_ok<T>(g))
+    f(std::integral_constant<int, vmax<T>()>());
+  else
+    f(std::integral_constant<int, 1>());
 }
+
+template <int NR>
+void finalize(psb_ctx *C, int op, int nblk, DevState *st, DevState *st2 = nullptr,
+              double *out = nullptr) {
+  hipLaunchKernelGGL((k_finalize<NR>), dim3(1), dim3(kBlock), 0, C->stream, op, nblk,
+                     C->work.partial, st, st2, out);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
 template <typename T, bool SYM>
 void spmv(hipStream_t stream, const Grid &g, DMat<T> A, const T *x, T *y) {
-  if (g.nz % vmax<T>() == 0)
-    spmv_v<T, vmax<T>(), SYM>(stream, g, A, x, y);
-  else
-    spmv_v<T, 1, SYM>(stream, g, A, x, y);
+  with_v<T>(g, [&](auto vt) {
+    constexpr int V = decltype(vt)::value;
+    hipLaunchKernelGGL((k_spmv<T, V, SYM>), dim3(nblocks<T>(g)), dim3(kBlock), 0, stream, g, A,
+                       x, y);
+  });
 }
 
-template <typename T, int V, bool SYM>
-void resid_v(hipStream_t stream, const Grid &g, DMat<T> A, const T *b, const T *x, T *y) {
-  hipLaunchKernelGGL((k_resid<T, V, SYM>), dim3(grid_blocks(g.N / V)), dim3(kBlock), 0,
-                     stream, g, A, b, x, y);
+template <typename T, bool SYM>
+void resid_k(hipStream_t stream, const Grid &g, DMat<T> A, const T *b, const T *x, T *y) {
+  with_v<T>(g, [&](auto vt) {
+    constexpr int V = decltype(vt)::value;
+    hipLaunchKernelGGL((k_resid<T, V, SYM>), dim3(nblocks<T>(g)), dim3(kBlock), 0, stream, g, A,
+                       b, x, y);
+  });
 }
 
 template <typename T, bool SYM, int MODE>
-void start(hipStream_t stream, const Grid &g, DMat<T> A, Metric m, const T *b, const T *x,
-           T *r, T *aux, T omega, DevState *st, Work w) {
-  if (g.nz % vmax<T>() == 0) {
-    constexpr int V = vmax<T>();
-    hipLaunchKernelGGL((k_start<T, V, SYM, MODE>), dim3(grid_blocks(g.N / V)), dim3(kBlock), 0,
-                       stream, g, A, m, b, x, r, aux, omega, st, w);
-  } else {
-    hipLaunchKernelGGL((k_start<T, 1, SYM, MODE>), dim3(grid_blocks(g.N)), dim3(kBlock), 0,
-                       stream, g, A, m, b, x, r, aux, omega, st, w);
-  }
+void start(psb_ctx *C, const Grid &g, DMat<T> A, const T *b, const T *x, T *r, T *aux, T omega,
+           DevState *st) {
+  with_v<T>(g, [&](auto vt) {
+    constexpr int V = decltype(vt)::value;
+    hipLaunchKernelGGL((k_start<T, V, SYM, MODE>), dim3(nblocks<T>(g)), dim3(kBlock), 0,
+                       C->stream, g, A, metric(C), b, x, r, aux, omega, C->work);
+  });
+  finalize<3>(C, MODE == 0 ? FIN_START_DIAG : MODE == 1 ? FIN_START_AC : FIN_START_BICGS,
+              nblocks<T>(g), st);
 }
 
 template <typename T, bool SYM, bool DIAG>
-void cg_pq(hipStream_t stream, const Grid &g, DMat<T> A, const T *rz, const T *pold, T *pnew,
-           T *q, DevState *st, Work w) {
-  if (g.nz % vmax<T>() == 0) {
-    constexpr int V = vmax<T>();
-    hipLaunchKernelGGL((k_cg_pq<T, V, SYM, DIAG>), dim3(grid_blocks(g.N / V)), dim3(kBlock), 0,
-                       stream, g, A, rz, pold, pnew, q, st, w);
-  } else {
-    hipLaunchKernelGGL((k_cg_pq<T, 1, SYM, DIAG>), dim3(grid_blocks(g.N)), dim3(kBlock), 0,
-                       stream, g, A, rz, pold, pnew, q, st, w);
-  }
+void cg_pq(psb_ctx *C, const Grid &g, DMat<T> A, const T *rz, const T *pold, T *pnew, T *q,
+           const uint8_t *af, DevState *st) {
+  with_v<T>(g, [&](auto vt) {
+    constexpr int V = decltype(vt)::value;
+    hipLaunchKernelGGL((k_cg_pq<T, V, SYM, DIAG>), dim3(nblocks<T>(g)), dim3(kBlock), 0,
+                       C->stream, g, A, rz, pold, pnew, q, af, st, C->work);
+  });
+  finalize<1>(C, FIN_PQ, nblocks<T>(g), st);
 }
 
 template <typename T, bool DIAG>
-void cg_xr(hipStream_t stream, const Grid &g, const T *c, Metric m, const T *p, const T *q, T *x,
-           T *r, T *e1, T omega, DevState *st, Work w) {
-  if (g.nz % vmax<T>() == 0) {
-    constexpr int V = vmax<T>();
-    hipLaunchKernelGGL((k_cg_xr<T, V, DIAG>), dim3(grid_blocks(g.N / V)), dim3(kBlock), 0,
-                       stream, g, c, m, p, q, x, r, e1, omega, st, w);
-  } else {
-    hipLaunchKernelGGL((k_cg_xr<T, 1, DIAG>), dim3(grid_blocks(g.N)), dim3(kBlock), 0, stream,
-                       g, c, m, p, q, x, r, e1, omega, st, w);
-  }
+void cg_xr(psb_ctx *C, const Grid &g, const T *c, const T *p, const T *q, T *x, T *r, T *e1,
+           T omega, DevState *st) {
+  with_v<T>(g, [&](auto vt) {
+    constexpr int V = decltype(vt)::value;
+    hipLaunchKernelGGL((k_cg_xr<T, V, DIAG>), dim3(nblocks<T>(g)), dim3(kBlock), 0, C->stream, g,
+                       c, metric(C), p, q, x, r, e1, omega, st, C->work);
+  });
+  finalize<3>(C, DIAG ? FIN_XR_DIAG : FIN_XR_AC, nblocks<T>(g), st);
 }
 
-template <typename T, int V, bool SYM, bool PROL, bool RHO>
-void jac_v(hipStream_t stream, const Grid &g, DMat<T> A, const T *r, const T *e, T *eout,
-           ProlX<T, V> pr, T omega, DevState *st, Work w) {
-  hipLaunchKernelGGL((k_jacobi<T, V, SYM, PROL, RHO>), dim3(grid_blocks(g.N / V)),
-                     dim3(kBlock), 0, stream, g, A, r, e, eout, pr, omega, st, w);
-}
-
-// one damped Jacobi sweep (optionally with the prolongation of eH fused in)
+// one damped Jacobi sweep on level lv (optionally with the prolongation of
+// the coarse correction eH fused in; RHO: last post-sweep on level 0)
 template <typename T, bool RHO>
-void jacobi(hipStream_t stream, const Level<T> &lv, const T *e, T *eout, const Level<T> *coarse,
-            const T *eH, T cfac, T omega, DevState *st, Work w) {
+void jacobi(psb_ctx *C, const Level<T> &lv, const T *e, T *eout, const Level<T> *coarse,
+            const T *eH, T cfac, T omega, DevState *st) {
   const Grid &g = lv.g;
-  auto mk = [&](auto vtag) {
-    constexpr int V = decltype(vtag)::value;
+  with_v<T>(g, [&](auto vt) {
+    constexpr int V = decltype(vt)::value;
     ProlX<T, V> pr{};
     if (coarse) {
       pr.e = e;
       pr.eH = eH;
       pr.af = lv.aflag;
       pr.cfac = cfac;
-      pr.nz = g.nz;
-      pr.sI = g.sI;
-      pr.CI = coarse->CI;
-      pr.CJ = coarse->CJ;
-      pr.CK = coarse->CK;
+      pr.sI = coarse->CI == 2 ? 1 : 0;
+      pr.sJ = coarse->CJ == 2 ? 1 : 0;
+      pr.sK = coarse->CK == 2 ? 1 : 0;
       pr.nYH = coarse->g.ny;
       pr.nZH = coarse->g.nz;
+      hipLaunchKernelGGL((k_jacobi<T, V, true, true, RHO>), dim3(nblocks<T>(g)), dim3(kBlock), 0,
+                         C->stream, g, lv.mat(), lv.r, e, eout, pr, omega, lv.aflag, st,
+                         C->work);
+    } else {
+      hipLaunchKernelGGL((k_jacobi<T, V, true, false, RHO>), dim3(nblocks<T>(g)), dim3(kBlock),
+                         0, C->stream, g, lv.mat(), lv.r, e, eout, pr, omega, lv.aflag, st,
+                         C->work);
     }
-    return pr;
-  };
-  if (g.nz % vmax<T>() == 0) {
-    constexpr int V = vmax<T>();
-    auto pr = mk(std::integral_constant<int, V>());
-    if (coarse)
-      jac_v<T, V, true, true, RHO>(stream, g, lv.mat(), lv.r, e, eout, pr, omega, st, w);
-    else
-      jac_v<T, V, true, false, RHO>(stream, g, lv.mat(), lv.r, e, eout, pr, omega, st, w);
+  });
+  if (RHO) finalize<4>(C, FIN_RHO, nblocks<T>(g), st);
+}
+
+template <typename T, int V, int KV, int CI, int CJ, int CK>
+void restrict_v(psb_ctx *C, const Level<T> &f, const T *e, Level<T> &cL, T omega,
+                DevState *st) {
+  const int nwork = cL.g.nx * cL.g.ny * (f.g.nz / (V * KV));
+  hipLaunchKernelGGL((k_restrict<T, V, KV, true, CI, CJ, CK>), dim3(grid_blocks(nwork)),
+                     dim3(kBlock), 0, C->stream, f.g, f.mat(), f.r, e, f.aflag, cL.g, cL.c,
+                     cL.aflag, cL.r, cL.e[0], omega, st);
+}
+
+template <typename T, int V, int CI, int CJ>
+void restrict_k(psb_ctx *C, const Level<T> &f, const T *e, Level<T> &cL, T omega,
+                DevState *st) {
+  if constexpr (V == 1) {
+    if (cL.CK == 2) restrict_v<T, 1, 2, CI, CJ, 2>(C, f, e, cL, omega, st);
+    else restrict_v<T, 1, 1, CI, CJ, 1>(C, f, e, cL, omega, st);
   } else {
-    auto pr = mk(std::integral_constant<int, 1>());
-    if (coarse)
-      jac_v<T, 1, true, true, RHO>(stream, g, lv.mat(), lv.r, e, eout, pr, omega, st, w);
-    else
-      jac_v<T, 1, true, false, RHO>(stream, g, lv.mat(), lv.r, e, eout, pr, omega, st, w);
+    if (cL.CK == 2) restrict_v<T, V, 1, CI, CJ, 2>(C, f, e, cL, omega, st);
+    else restrict_v<T, V, 1, CI, CJ, 1>(C, f, e, cL, omega, st);
   }
 }
 
+// restriction of (r - A e) from level f to cL, with cL's first sweep fused
 template <typename T>
-void restrict_(hipStream_t stream, const Level<T> &f, const T *e, Level<T> &cL, int lev,
-               DevState *st, Work w) {
-  hipLaunchKernelGGL((k_restrict<T, true>), dim3(grid_blocks(cL.g.N)), dim3(kBlock), 0, stream,
-                     f.g, f.mat(), f.r, e, f.aflag, cL.g, cL.CI, cL.CJ, cL.CK, cL.aflag, cL.r, lev,
-                     st, w);
-}
-
-template <typename T>
-void coarse_first(hipStream_t stream, const Level<T> &lv, T *e1, T omega, int lev, int nu1,
-                  DevState *st) {
-  const Grid &g = lv.g;
-  if (g.nz % vmax<T>() == 0) {
-    constexpr int V = vmax<T>();
-    hipLaunchKernelGGL((k_coarse_first<T, V>), dim3(grid_blocks(g.N / V)), dim3(kBlock), 0,
-                       stream, g, lv.c, lv.aflag, lv.r, e1, omega, lev, nu1, st);
-  } else {
-    hipLaunchKernelGGL((k_coarse_first<T, 1>), dim3(grid_blocks(g.N)), dim3(kBlock), 0, stream,
-                       g, lv.c, lv.aflag, lv.r, e1, omega, lev, nu1, st);
-  }
+void restrict_(psb_ctx *C, const Level<T> &f, const T *e, Level<T> &cL, T omega, DevState *st) {
+  with_v<T>(f.g, [&](auto vt) {
+    constexpr int V = decltype(vt)::value;
+    if (cL.CI == 2 && cL.CJ == 2) restrict_k<T, V, 2, 2>(C, f, e, cL, omega, st);
+    else if (cL.CI == 2) restrict_k<T, V, 2, 1>(C, f, e, cL, omega, st);
+    else if (cL.CJ == 2) restrict_k<T, V, 1, 2>(C, f, e, cL, omega, st);
+    else restrict_k<T, V, 1, 1>(C, f, e, cL, omega, st);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -405,7 +429,23 @@ std::vector<int> level_dims(int nx, int ny, int nz) {
 }
 
 template <typename T>
-void ensure_krylov(psb_ctx *C, KrylovVecs<T> &K, bool bicgs);
+void ensure_krylov(psb_ctx *C, KrylovVecs<T> &K, bool bicgs) {
+  const int N = C->g.N;
+  if (!K.x) {
+    K.x = C->arena.alloc<T>(N);
+    K.r = C->arena.alloc<T>(N);
+    K.q = C->arena.alloc<T>(N);
+    K.p[0] = C->arena.alloc<T>(N);
+    K.p[1] = C->arena.alloc<T>(N);
+  }
+  if (bicgs && !K.s) {
+    K.v[0] = C->arena.alloc<T>(N);
+    K.v[1] = C->arena.alloc<T>(N);
+    K.s = C->arena.alloc<T>(N);
+    K.t = C->arena.alloc<T>(N);
+    K.rt = C->arena.alloc<T>(N);
+  }
+}
 
 void build_hier64(psb_ctx *C) {
   Hier<double> &H = C->h64;
@@ -414,6 +454,7 @@ void build_hier64(psb_ctx *C) {
   hipStream_t stream = C->stream;
   std::vector<int> d = level_dims(C->g.nx, C->g.ny, C->g.nz);
   const int nl = (int)d.size() / 3;
+  if (nl < 2) throw std::invalid_argument("grid too small for the multigrid preconditioner");
   H.L.resize(nl);
   for (int l = 0; l < nl; l++) {
     Level<double> &lv = H.L[l];
@@ -455,7 +496,6 @@ void build_hier64(psb_ctx *C) {
   HIPCHK(hipMemcpyAsync(b.data(), cl.b, m * 8, hipMemcpyDeviceToHost, stream));
   if (cl.aflag) HIPCHK(hipMemcpyAsync(act.data(), cl.aflag, m, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
-  if (nl < 2) throw std::invalid_argument("grid too small for the multigrid preconditioner");
   std::vector<double> G = dense_pinv(cl.g.nx, cl.g.ny, cl.g.nz, c, w, s, b, act);
   H.m = m;
   H.G = C->arena.alloc<double>((size_t)m * m);
@@ -515,60 +555,49 @@ void build_hier32(psb_ctx *C) {
 
 // ---------------------------------------------------------------------------
 // V-cycle as a preconditioner.  On entry the level's first sweep e1 is in
-// lv.e[0] (written by the CG step on level 0, by k_coarse_first below it).
+// lv.e[0] (written by the CG step on level 0, by the restriction below it).
 // Returns the buffer that holds z.
 // ---------------------------------------------------------------------------
 template <typename T>
 T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega, T cfac) {
-  hipStream_t stream = C->stream;
   Level<T> &lv = H.L[l];
   const int nl = (int)H.L.size();
   if (l == nl - 1) {
-    hipLaunchKernelGGL((k_gemv<T>), dim3((H.m + kWaves - 1) / kWaves), dim3(kBlock), 0, stream,
-                       H.m, H.G, lv.r, lv.e[0], st);
-    return lv.e[0];
+    hipLaunchKernelGGL((k_gemv<T>), dim3((H.m + kWaves - 1) / kWaves), dim3(kBlock), 0,
+                       C->stream, H.m, H.G, lv.r, lv.e[1], st);
+    return lv.e[1];
   }
-  if (l > 0) coarse_first<T>(stream, lv, lv.e[0], omega, l, nu1, st);
   int cur = 0;
   for (int s = 1; s < nu1; s++) {
-    jacobi<T, false>(stream, lv, lv.e[cur], lv.e[(cur + 1) % 3], nullptr, nullptr, T(0), omega,
-                     st, C->work);
+    jacobi<T, false>(C, lv, lv.e[cur], lv.e[(cur + 1) % 3], nullptr, nullptr, T(0), omega, st);
     cur = (cur + 1) % 3;
   }
   Level<T> &cL = H.L[l + 1];
-  restrict_<T>(stream, lv, lv.e[cur], cL, l + 1, st, C->work);
+  restrict_<T>(C, lv, lv.e[cur], cL, omega, st);
   T *zH = vcycle<T>(C, H, l + 1, st, nu1, nu2, omega, cfac);
   for (int s = 0; s < nu2; s++) {
     const int nxt = (cur + 1) % 3;
     const bool last = (s == nu2 - 1) && l == 0;
-    if (s == 0) {
-      if (last)
-        jacobi<T, true>(stream, lv, lv.e[cur], lv.e[nxt], &cL, zH, cfac, omega, st, C->work);
-      else
-        jacobi<T, false>(stream, lv, lv.e[cur], lv.e[nxt], &cL, zH, cfac, omega, st, C->work);
-    } else {
-      if (last)
-        jacobi<T, true>(stream, lv, lv.e[cur], lv.e[nxt], nullptr, nullptr, T(0), omega, st,
-                        C->work);
-      else
-        jacobi<T, false>(stream, lv, lv.e[cur], lv.e[nxt], nullptr, nullptr, T(0), omega, st,
-                         C->work);
-    }
+    const Level<T> *co = s == 0 ? &cL : nullptr;
+    const T *eH = s == 0 ? zH : nullptr;
+    if (last)
+      jacobi<T, true>(C, lv, lv.e[cur], lv.e[nxt], co, eH, cfac, omega, st);
+    else
+      jacobi<T, false>(C, lv, lv.e[cur], lv.e[nxt], co, eH, cfac, omega, st);
     cur = nxt;
   }
   return lv.e[cur];
 }
 
-// one preconditioned CG iteration: V-cycle, p/q step, x/r step
+// one preconditioned CG iteration: V-cycle (with rho), p/q step, x/r step
 template <typename T>
 void ac_iteration(psb_ctx *C, Hier<T> &H, KrylovVecs<T> &K, DevState *st, int parity, int nu1,
                   int nu2, T omega, T cfac) {
-  hipStream_t stream = C->stream;
   Level<T> &f = H.L[0];
   T *z = vcycle<T>(C, H, 0, st, nu1, nu2, omega, cfac);
   T *pold = K.p[parity], *pnew = K.p[parity ^ 1];
-  cg_pq<T, true, false>(stream, f.g, f.mat(), z, pold, pnew, K.q, st, C->work);
-  cg_xr<T, false>(stream, f.g, f.c, metric(C), pnew, K.q, K.x, K.r, f.e[0], omega, st, C->work);
+  cg_pq<T, true, false>(C, f.g, f.mat(), z, pold, pnew, K.q, f.aflag, st);
+  cg_xr<T, false>(C, f.g, f.c, pnew, K.q, K.x, K.r, f.e[0], omega, st);
 }
 
 void destroy_graph(Graph &gr) {
@@ -611,6 +640,7 @@ void init_state(psb_ctx *C, DevState *st, const psb_stop *p, int norm, double re
   h.hist = record ? C->hist : nullptr;
   h.hist_cap = record ? C->hist_cap : 0;
   HIPCHK(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, C->stream));
+  HIPCHK(hipStreamSynchronize(C->stream));
 }
 
 void ensure_hist(psb_ctx *C, int maxit) {
@@ -621,30 +651,11 @@ void ensure_hist(psb_ctx *C, int maxit) {
 }
 
 template <typename T>
-void ensure_krylov(psb_ctx *C, KrylovVecs<T> &K, bool bicgs) {
-  const int N = C->g.N;
-  if (!K.x) {
-    K.x = C->arena.alloc<T>(N);
-    K.r = C->arena.alloc<T>(N);
-    K.q = C->arena.alloc<T>(N);
-    K.p[0] = C->arena.alloc<T>(N);
-    K.p[1] = C->arena.alloc<T>(N);
-  }
-  if (bicgs && !K.s) {
-    K.v[0] = C->arena.alloc<T>(N);
-    K.v[1] = C->arena.alloc<T>(N);
-    K.s = C->arena.alloc<T>(N);
-    K.t = C->arena.alloc<T>(N);
-    K.rt = C->arena.alloc<T>(N);
-  }
-}
-
-template <typename T>
 void zero(psb_ctx *C, T *x, int n) {
   hipLaunchKernelGGL((k_fill<T>), dim3(grid_blocks(n)), dim3(kBlock), 0, C->stream, x, T(0), n);
 }
 
-void finish_result(psb_ctx *C, psb_result *R, const DevState &h, int outer) {
+void finish_result(psb_result *R, const DevState &h, int outer) {
   R->outer = outer;
   R->converged = h.converged;
   R->res0 = h.res0;
@@ -662,63 +673,64 @@ void finish_result(psb_ctx *C, psb_result *R, const DevState &h, int outer) {
 // true ||b - A x||_2 / ||b||_2 in FP64 (diagnostic, after the solve)
 double true_rel(psb_ctx *C, const double *b, const double *x) {
   const Grid &g = C->g;
-  auto go = [&](auto vtag, auto symtag) {
-    constexpr int V = decltype(vtag)::value;
+  auto go = [&](auto symtag) {
     constexpr bool S = decltype(symtag)::value;
-    hipLaunchKernelGGL((k_relres<V, S>), dim3(grid_blocks(g.N / V)), dim3(kBlock), 0, C->stream,
-                       g, mat64(C), b, x, C->relres, C->work);
+    with_v<double>(g, [&](auto vt) {
+      constexpr int V = decltype(vt)::value;
+      hipLaunchKernelGGL((k_relres<V, S>), dim3(nblocks<double>(g)), dim3(kBlock), 0, C->stream,
+                         g, mat64(C), b, x, C->work);
+    });
   };
-  const bool v2 = g.nz % VM64 == 0;
-  if (C->sym) {
-    if (v2) go(std::integral_constant<int, VM64>(), std::true_type());
-    else go(std::integral_constant<int, 1>(), std::true_type());
-  } else {
-    if (v2) go(std::integral_constant<int, VM64>(), std::false_type());
-    else go(std::integral_constant<int, 1>(), std::false_type());
-  }
+  if (C->sym) go(std::true_type());
+  else go(std::false_type());
+  finalize<2>(C, FIN_RELRES, nblocks<double>(g), C->st64, nullptr, C->relres);
   double h[2];
   HIPCHK(hipMemcpyAsync(h, C->relres, 16, hipMemcpyDeviceToHost, C->stream));
   HIPCHK(hipStreamSynchronize(C->stream));
   return h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0;
 }
 
+template <typename F>
+void run_graph(psb_ctx *C, Graph &gr, long long key, DevState *st, int maxit, F body) {
+  long long launches = 0;
+  while (!C->hst->done) {
+    capture(C, gr, key, body);
+    HIPCHK(hipGraphLaunch(gr.exec, C->stream));
+    read_state(C, st);
+    if (++launches > (long long)maxit + 2) break;
+  }
+}
+
+int chunk_of(const psb_stop *p) { return std::max(2, p->graph_chunk + (p->graph_chunk & 1)); }
+
 // ---------------------------------------------------------------------------
 // reference CG with diagonal preconditioning (cg.cpp:10-182), FP64
 // ---------------------------------------------------------------------------
 template <bool SYM>
 void solve_cg_diag(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_result *R) {
-  hipStream_t stream = C->stream;
   KrylovVecs<double> &K = C->k64;
   ensure_krylov(C, K, false);
   ensure_hist(C, p->max_iter);
   const Grid &g = C->g;
   DMat<double> A = mat64(C);
-  HIPCHK(hipMemcpyAsync(K.x, x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, stream));
+  HIPCHK(hipMemcpyAsync(K.x, x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, C->stream));
   init_state(C, C->st64, p, p->norm, p->res_rat, p->res_tol, p->max_iter);
   zero(C, K.p[0], g.N);
   zero(C, K.p[1], g.N);
-  start<double, SYM, 0>(stream, g, A, metric(C), b, K.x, K.r, nullptr, 0.0, C->st64, C->work);
+  start<double, SYM, 0>(C, g, A, b, K.x, K.r, nullptr, 0.0, C->st64);
   HIPCHK(hipGetLastError());
   read_state(C, C->st64);
-  const int chunk = std::max(2, p->graph_chunk + (p->graph_chunk & 1));
+  const int chunk = chunk_of(p);
   auto body = [&]() {
     for (int it = 0; it < chunk; it++) {
       const int par = it & 1;
-      cg_pq<double, SYM, true>(stream, g, A, K.r, K.p[par], K.p[par ^ 1], K.q, C->st64, C->work);
-      cg_xr<double, true>(stream, g, A.c, metric(C), K.p[par ^ 1], K.q, K.x, K.r, nullptr, 0.0,
-                          C->st64, C->work);
+      cg_pq<double, SYM, true>(C, g, A, K.r, K.p[par], K.p[par ^ 1], K.q, nullptr, C->st64);
+      cg_xr<double, true>(C, g, A.c, K.p[par ^ 1], K.q, K.x, K.r, nullptr, 0.0, C->st64);
     }
   };
-  const long long key = chunk * 2 + (SYM ? 1 : 0);
-  long long launches = 0;
-  while (!C->hst->done) {
-    capture(C, C->gr_cgdiag, key, body);
-    HIPCHK(hipGraphLaunch(C->gr_cgdiag.exec, stream));
-    read_state(C, C->st64);
-    if (++launches > (long long)p->max_iter + 2) break;
-  }
-  HIPCHK(hipMemcpyAsync(x, K.x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, stream));
-  finish_result(C, R, *C->hst, 0);
+  run_graph(C, C->gr_cgdiag, chunk * 2 + (SYM ? 1 : 0), C->st64, p->max_iter, body);
+  HIPCHK(hipMemcpyAsync(x, K.x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, C->stream));
+  finish_result(R, *C->hst, 0);
   C->hist_count = std::min(C->hst->reason == 7 ? 1 : C->hst->iter + 1, C->hist_cap);
 }
 
@@ -728,67 +740,54 @@ void solve_cg_diag(psb_ctx *C, double *x, const double *b, const psb_stop *p, ps
 template <bool SYM>
 void solve_bicgs_diag(psb_ctx *C, double *x, const double *b, const psb_stop *p,
                       psb_result *R) {
-  hipStream_t stream = C->stream;
   KrylovVecs<double> &K = C->k64;
   ensure_krylov(C, K, true);
   ensure_hist(C, p->max_iter);
   const Grid &g = C->g;
   DMat<double> A = mat64(C);
   init_state(C, C->st64, p, p->norm, p->res_rat, p->res_tol, p->max_iter);
-  HIPCHK(hipMemcpyAsync(K.x, x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, stream));
+  HIPCHK(hipMemcpyAsync(K.x, x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, C->stream));
   for (int q = 0; q < 2; q++) {
     zero(C, K.p[q], g.N);
     zero(C, K.v[q], g.N);
   }
-  start<double, SYM, 2>(stream, g, A, metric(C), b, K.x, K.r, K.rt, 0.0, C->st64, C->work);
+  start<double, SYM, 2>(C, g, A, b, K.x, K.r, K.rt, 0.0, C->st64);
   HIPCHK(hipGetLastError());
   read_state(C, C->st64);
-  const int chunk = std::max(2, p->graph_chunk + (p->graph_chunk & 1));
-  auto launch_dim = [&](auto V) { return grid_blocks(g.N / decltype(V)::value); };
+  const int chunk = chunk_of(p);
+  const int nb = nblocks<double>(g);
   auto body = [&]() {
     for (int it = 0; it < chunk; it++) {
       const int par = it & 1;
-      if (g.nz % VM64 == 0) {
-        constexpr int V = VM64;
-        int nb = launch_dim(std::integral_constant<int, V>());
-        hipLaunchKernelGGL((k_bi1<double, V, SYM>), dim3(nb), dim3(kBlock), 0, stream, g, A, K.r,
-                           K.rt, K.p[par], K.v[par], K.p[par ^ 1], K.v[par ^ 1], C->st64, C->work);
-        hipLaunchKernelGGL((k_bi2<double, V, SYM>), dim3(nb), dim3(kBlock), 0, stream, g, A, K.r,
-                           K.v[par ^ 1], K.s, K.t, C->st64, C->work);
-        hipLaunchKernelGGL((k_bi3<double, V>), dim3(nb), dim3(kBlock), 0, stream, g, A.c,
+      with_v<double>(g, [&](auto vt) {
+        constexpr int V = decltype(vt)::value;
+        hipLaunchKernelGGL((k_bi1<double, V, SYM>), dim3(nb), dim3(kBlock), 0, C->stream, g, A,
+                           K.r, K.rt, K.p[par], K.v[par], K.p[par ^ 1], K.v[par ^ 1], C->st64,
+                           C->work);
+        finalize<1>(C, FIN_BI1, nb, C->st64);
+        hipLaunchKernelGGL((k_bi2<double, V, SYM>), dim3(nb), dim3(kBlock), 0, C->stream, g, A,
+                           K.r, K.v[par ^ 1], K.s, K.t, C->st64, C->work);
+        finalize<3>(C, FIN_BI2, nb, C->st64);
+        hipLaunchKernelGGL((k_bi3<double, V>), dim3(nb), dim3(kBlock), 0, C->stream, g, A.c,
                            metric(C), K.p[par ^ 1], K.s, K.t, K.rt, K.x, K.r, C->st64, C->work);
-      } else {
-        int nb = grid_blocks(g.N);
-        hipLaunchKernelGGL((k_bi1<double, 1, SYM>), dim3(nb), dim3(kBlock), 0, stream, g, A, K.r,
-                           K.rt, K.p[par], K.v[par], K.p[par ^ 1], K.v[par ^ 1], C->st64, C->work);
-        hipLaunchKernelGGL((k_bi2<double, 1, SYM>), dim3(nb), dim3(kBlock), 0, stream, g, A, K.r,
-                           K.v[par ^ 1], K.s, K.t, C->st64, C->work);
-        hipLaunchKernelGGL((k_bi3<double, 1>), dim3(nb), dim3(kBlock), 0, stream, g, A.c,
-                           metric(C), K.p[par ^ 1], K.s, K.t, K.rt, K.x, K.r, C->st64, C->work);
-      }
+        finalize<3>(C, FIN_BI3, nb, C->st64);
+      });
     }
   };
-  const long long key = chunk * 2 + (SYM ? 1 : 0);
-  long long launches = 0;
-  while (!C->hst->done) {
-    capture(C, C->gr_bicgs, key, body);
-    HIPCHK(hipGraphLaunch(C->gr_bicgs.exec, stream));
-    read_state(C, C->st64);
-    if (++launches > (long long)p->max_iter + 2) break;
-  }
-  HIPCHK(hipMemcpyAsync(x, K.x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, stream));
-  finish_result(C, R, *C->hst, 0);
+  run_graph(C, C->gr_bicgs, chunk * 2 + (SYM ? 1 : 0), C->st64, p->max_iter, body);
+  HIPCHK(hipMemcpyAsync(x, K.x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, C->stream));
+  finish_result(R, *C->hst, 0);
   C->hist_count = std::min(C->hst->reason == 7 ? 1 : C->hst->iter + 1, C->hist_cap);
 }
 
 // ---------------------------------------------------------------------------
-// AC-preconditioned CG in precision T on (K.x, K.r already holding x and
-// r = b - A x, e1 in level 0's e[0], st->res0 set).  Runs to st->done.
+// AC-preconditioned CG in precision T on K.x, K.r (r = b - A x already
+// formed, e1 in level 0's e[0], st initialised).  Runs to st->done.
 // ---------------------------------------------------------------------------
 template <typename T>
 void run_acpcg(psb_ctx *C, Hier<T> &H, KrylovVecs<T> &K, DevState *st, Graph &gr,
                const psb_stop *p) {
-  const int chunk = std::max(2, p->graph_chunk + (p->graph_chunk & 1));
+  const int chunk = chunk_of(p);
   const int nu1 = p->ac_pre, nu2 = p->ac_post;
   const T omega = (T)p->ac_omega, cfac = (T)p->ac_cfac;
   auto body = [&]() {
@@ -798,13 +797,7 @@ void run_acpcg(psb_ctx *C, Hier<T> &H, KrylovVecs<T> &K, DevState *st, Graph &gr
   if (gr.key != key || C->ac_omega != p->ac_omega || C->ac_cfac != p->ac_cfac) destroy_graph(gr);
   C->ac_omega = p->ac_omega;
   C->ac_cfac = p->ac_cfac;
-  long long launches = 0;
-  while (!C->hst->done) {
-    capture(C, gr, key, body);
-    HIPCHK(hipGraphLaunch(gr.exec, C->stream));
-    read_state(C, st);
-    if (++launches > (long long)p->max_iter + 2) break;
-  }
+  run_graph(C, gr, key, st, p->max_iter, body);
 }
 
 void solve_ac64(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_result *R) {
@@ -819,19 +812,18 @@ void solve_ac64(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_r
   init_state(C, C->st64, p, p->norm, p->res_rat, p->res_tol, p->max_iter);
   zero(C, K.p[0], g.N);
   zero(C, K.p[1], g.N);
-  start<double, true, 1>(C->stream, g, mat64(C), metric(C), b, K.x, K.r, f.e[0],
-                         p->ac_omega, C->st64, C->work);
+  start<double, true, 1>(C, g, mat64(C), b, K.x, K.r, f.e[0], p->ac_omega, C->st64);
   HIPCHK(hipGetLastError());
   read_state(C, C->st64);
   run_acpcg<double>(C, C->h64, K, C->st64, C->gr_ac64, p);
   HIPCHK(hipMemcpyAsync(x, K.x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, C->stream));
-  finish_result(C, R, *C->hst, 0);
+  finish_result(R, *C->hst, 0);
   C->hist_count = std::min(C->hst->iter + 1, C->hist_cap);
 }
 
 // ---------------------------------------------------------------------------
 // mixed precision: FP64 iterative refinement around the FP32 AC-PCG
-// (r = b - A x in FP64, inner solve to inner_rtol, x += d in FP64)
+// (r = b - A x in FP64, inner solve on the centred r, x += d in FP64)
 // ---------------------------------------------------------------------------
 void solve_ir(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_result *R) {
   KrylovVecs<float> &K = C->k32;
@@ -849,57 +841,63 @@ void solve_ir(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_res
   zero(C, K.x, g.N);
   zero(C, K.p[0], g.N);
   zero(C, K.p[1], g.N);
-  HIPCHK(hipStreamSynchronize(C->stream));
   double res0 = -1.0, prev = 1e300;
   int outer = 0, inner_total = 0, status = PSB_MAXITER, conv = 0;
-  DevState h32;
-  std::memset(&h32, 0, sizeof(h32));
+  C->hist_host.clear();
+  const int nb64 = nblocks<double>(g), nb32 = nblocks<float>(g);
   for (;;) {
-    auto launch_res = [&](auto vtag) {
-      constexpr int V = decltype(vtag)::value;
-      hipLaunchKernelGGL((k_ir_resid<V, true>), dim3(grid_blocks(g.N / V)), dim3(kBlock), 0,
-                         C->stream, g, mat64(C), metric(C), b, x, K.r, f.e[0],
-                         (float)p->ac_omega, C->st64, C->st32, C->work);
-    };
-    if (g.nz % VM64 == 0)
-      launch_res(std::integral_constant<int, VM64>());
-    else
-      launch_res(std::integral_constant<int, 1>());
+    with_v<double>(g, [&](auto vt) {
+      constexpr int V = decltype(vt)::value;
+      hipLaunchKernelGGL((k_ir_resid<V, true>), dim3(nb64), dim3(kBlock), 0, C->stream, g,
+                         mat64(C), metric(C), b, x, K.r, f.aflag, C->work);
+    });
+    finalize<4>(C, FIN_IR_RESID, nb64, C->st64, C->st32);
     HIPCHK(hipGetLastError());
     DevState h64;
     HIPCHK(hipMemcpyAsync(&h64, C->st64, sizeof(h64), hipMemcpyDeviceToHost, C->stream));
     HIPCHK(hipStreamSynchronize(C->stream));
     const double res = h64.res;
+    C->hist_host.push_back(res);
     if (res0 < 0) res0 = res;
+    R->res = res;
     if (!(res == res)) { status = PSB_NAN; break; }
     if (res < p->res_tol || res < res0 * p->res_rat || res0 == 0.0) {
       conv = 1;
       status = PSB_OK;
-      R->res = res;
       break;
     }
-    if (outer >= p->max_outer) { status = PSB_MAXITER; R->res = res; break; }
-    if (outer > 0 && res > 0.9 * prev) { status = PSB_DIVERGED; R->res = res; break; }
+    if (outer >= p->max_outer) { status = PSB_MAXITER; break; }
+    if (outer > 0 && res > 0.9 * prev) { status = PSB_DIVERGED; break; }
     prev = res;
-    R->res = res;
-    // inner FP32 solve; st32 was initialised by k_ir_resid
+    // relative reduction asked of this inner solve
+    double rt = p->inner_rtol;
+    if (p->inner_adapt) {
+      const double target = std::max(p->res_tol, res0 * p->res_rat);
+      rt = std::max(p->inner_rtol, std::min(0.5, 0.5 * target / res));
+    }
+    HIPCHK(hipMemcpyAsync(&C->st32->res_rat, &rt, sizeof(double), hipMemcpyHostToDevice,
+                          C->stream));
+    // inner FP32 solve on the centred residual
+    with_v<float>(g, [&](auto vt) {
+      constexpr int V = decltype(vt)::value;
+      hipLaunchKernelGGL((k_center<V>), dim3(nb32), dim3(kBlock), 0, C->stream, g, f.c, f.aflag,
+                         K.r, f.e[0], (float)p->ac_omega, C->st32, C->work);
+    });
+    finalize<1>(C, FIN_CENTER, nb32, C->st32);
+    HIPCHK(hipGetLastError());
     read_state(C, C->st32);
     run_acpcg<float>(C, C->h32, K, C->st32, C->gr_ac32, &in);
-    h32 = *C->hst;
-    inner_total += h32.iter;
-    if (h32.reason == 8) { status = PSB_NAN; break; }
-    auto launch_up = [&](auto vtag) {
-      constexpr int V = decltype(vtag)::value;
-      hipLaunchKernelGGL((k_ir_update<V>), dim3(grid_blocks(g.N / V)), dim3(kBlock), 0,
-                         C->stream, g, x, K.x);
-    };
-    if (g.nz % VM64 == 0)
-      launch_up(std::integral_constant<int, VM64>());
-    else
-      launch_up(std::integral_constant<int, 1>());
+    inner_total += C->hst->iter;
+    if (C->hst->reason == 8) { status = PSB_NAN; break; }
+    with_v<double>(g, [&](auto vt) {
+      constexpr int V = decltype(vt)::value;
+      hipLaunchKernelGGL((k_ir_update<V>), dim3(nb64), dim3(kBlock), 0, C->stream, g, x, K.x);
+    });
     HIPCHK(hipGetLastError());
     outer++;
   }
+  C->hist_on_host = true;
+  C->hist_count = (int)C->hist_host.size();
   R->res0 = res0;
   R->outer = outer;
   R->iterations = inner_total;
@@ -908,6 +906,7 @@ void solve_ir(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_res
 }
 
 int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_result *R) {
+  C->hist_on_host = false;
   if (p->precond == PSB_PREC_AC) {
     if (p->method != PSB_CG || !C->sym || p->ac_pre < 1 || p->ac_post < 1) return PSB_EINVAL;
     if (p->precision == PSB_MIXED)
@@ -999,6 +998,7 @@ void psb_stop_defaults(psb_stop *s) {
   s->inner_rtol = 1.0e-4;
   s->max_outer = 30;
   s->graph_chunk = 8;
+  s->inner_adapt = 0;
 }
 
 const char *psb_status_string(int s) {
@@ -1031,15 +1031,13 @@ int psb_create(psb_ctx **out, const psb_layout *L) {
     C->st64 = C->base.alloc<DevState>(1);
     C->st32 = C->base.alloc<DevState>(1);
     HIPCHK(hipHostMalloc((void **)&C->hst, sizeof(DevState), hipHostMallocDefault));
-    C->work.partial = C->base.alloc<double>(2048 * 4);
-    C->work.counter = C->base.alloc<unsigned>(1);
+    std::memset(C->hst, 0, sizeof(DevState));
+    C->work.partial = C->base.alloc<double>((size_t)kMaxBlocks * 4);
     C->relres = C->base.alloc<double>(2);
     const int N = C->g.N;
     C->dx = C->base.alloc<double>(N);
     C->db = C->base.alloc<double>(N);
     C->dy = C->base.alloc<double>(N);
-    HIPCHK(hipMemsetAsync(C->work.counter, 0, sizeof(unsigned), C->stream));
-    HIPCHK(hipStreamSynchronize(C->stream));
     return PSB_OK;
   });
   if (rc != PSB_OK) {
@@ -1094,9 +1092,7 @@ int psb_set_matrix(psb_ctx *C, const psb_matrix7 *A) {
     std::vector<double> a[7];
     const double *src[7] = {A->c, A->w, A->s, A->b, A->e, A->n, A->t};
     for (int q = 0; q < 7; q++) gather_host(L, src[q], a[q]);
-    const std::vector<double> &c = a[0], &w = a[1], &s = a[2], &b = a[3], &e = a[4], &n = a[5],
-                              &t = a[6];
-    (void)c;
+    const std::vector<double> &w = a[1], &s = a[2], &b = a[3], &e = a[4], &n = a[5], &t = a[6];
     auto id = [&](int i, int j, int k) { return ((size_t)i * ny + j) * nz + k; };
     // wall couplings must already be folded into the system (zero)
     bool sym = true;
@@ -1115,6 +1111,11 @@ int psb_set_matrix(psb_ctx *C, const psb_matrix7 *A) {
     const int N = C->g.N;
     const int nstore = sym ? 4 : 7;
     // (re)allocate: a new matrix invalidates the hierarchy and graphs
+    HIPCHK(hipStreamSynchronize(C->stream));
+    destroy_graph(C->gr_cgdiag);
+    destroy_graph(C->gr_bicgs);
+    destroy_graph(C->gr_ac64);
+    destroy_graph(C->gr_ac32);
     C->arena.release();
     for (int q = 0; q < 7; q++) C->A64[q] = nullptr;
     C->A32[0] = C->A32[1] = C->A32[2] = C->A32[3] = nullptr;
@@ -1124,10 +1125,6 @@ int psb_set_matrix(psb_ctx *C, const psb_matrix7 *A) {
     C->k64 = KrylovVecs<double>();
     C->k32 = KrylovVecs<float>();
     C->hist_count = 0;
-    destroy_graph(C->gr_cgdiag);
-    destroy_graph(C->gr_bicgs);
-    destroy_graph(C->gr_ac64);
-    destroy_graph(C->gr_ac32);
     for (int q = 0; q < nstore; q++) {
       C->A64[q] = C->arena.alloc<double>(N);
       HIPCHK(hipMemcpyAsync(C->A64[q], a[q].data(), (size_t)N * 8, hipMemcpyHostToDevice,
@@ -1203,13 +1200,8 @@ int psb_apply(psb_ctx *C, const double *x, const double *b, double *y) {
     if (b) {
       gather_host(C->lay, b, hb);
       HIPCHK(hipMemcpyAsync(C->db, hb.data(), (size_t)N * 8, hipMemcpyHostToDevice, stream));
-      if (C->sym) {
-        if (C->g.nz % VM64 == 0) resid_v<double, VM64, true>(stream, C->g, mat64(C), C->db, C->dx, dy);
-        else resid_v<double, 1, true>(stream, C->g, mat64(C), C->db, C->dx, dy);
-      } else {
-        if (C->g.nz % VM64 == 0) resid_v<double, VM64, false>(stream, C->g, mat64(C), C->db, C->dx, dy);
-        else resid_v<double, 1, false>(stream, C->g, mat64(C), C->db, C->dx, dy);
-      }
+      if (C->sym) resid_k<double, true>(stream, C->g, mat64(C), C->db, C->dx, dy);
+      else resid_k<double, false>(stream, C->g, mat64(C), C->db, C->dx, dy);
     } else {
       if (C->sym) spmv<double, true>(stream, C->g, mat64(C), C->dx, dy);
       else spmv<double, false>(stream, C->g, mat64(C), C->dx, dy);
@@ -1224,6 +1216,11 @@ int psb_apply(psb_ctx *C, const double *x, const double *b, double *y) {
 
 int psb_history(psb_ctx *C, double *hist, int cap) {
   if (!C || !hist || cap < 0) return PSB_EINVAL;
+  if (C->hist_on_host) {
+    int n = std::min(cap, (int)C->hist_host.size());
+    std::copy(C->hist_host.begin(), C->hist_host.begin() + n, hist);
+    return n;
+  }
   if (!C->hist || C->hist_count <= 0) return 0;
   return guarded(C, [&]() {
     int n = std::min(cap, C->hist_count);
@@ -1239,9 +1236,6 @@ int psb_time_kernel(psb_ctx *C, int kernel, int reps, double *avg_ms, double *by
     hipStream_t stream = C->stream;
     const Grid &g = C->g;
     const double N = (double)g.N;
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
     std::function<void()> f;
     if (kernel == 0) {
       ensure_krylov(C, C->k64, false);
@@ -1258,22 +1252,22 @@ int psb_time_kernel(psb_ctx *C, int kernel, int reps, double *avg_ms, double *by
       *bytes = N * (4 * 4 + 2 * 4);
     } else if (kernel == 2) {
       build_hier32(C);
-      ensure_krylov(C, C->k32, false);
-      Level<float> &lv = C->h32.L[0];
+      Level<float> lv = C->h32.L[0];
       zero(C, lv.e[0], g.N);
-      zero(C, C->k32.r, g.N);
+      zero(C, lv.r, g.N);
       DevState h;
       std::memset(&h, 0, sizeof(h));
       HIPCHK(hipMemcpyAsync(C->st32, &h, sizeof(h), hipMemcpyHostToDevice, stream));
       f = [=]() {
-        Level<float> l2 = lv;
-        jacobi<float, false>(stream, l2, l2.e[0], l2.e[1], nullptr, nullptr, 0.0f, 0.9f, C->st32,
-                             C->work);
+        jacobi<float, false>(C, lv, lv.e[0], lv.e[1], nullptr, nullptr, 0.0f, 0.9f, C->st32);
       };
-      *bytes = N * (4 * 4 + 3 * 4);  // c,w,s,b + e, r in; e out
+      *bytes = N * (4 * 4 + 3 * 4);  // c, w, s, b, e, r in; e out
     } else {
       return PSB_EINVAL;
     }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
     f();
     HIPCHK(hipStreamSynchronize(stream));
     HIPCHK(hipEventRecord(e0, stream));

@@ -1,7 +1,8 @@
 // psb_device.h -- device-side building blocks of the MI355X pressure solver:
-// grid geometry, the 7-point stencil gather (4-array symmetric or 7-array
-// storage), the deterministic grid reduction and the solver state that the
-// fused kernels share.  Compiled for gfx950 only (wave64).
+// grid geometry with division by invariant integers, the 7-point stencil
+// gather (4-array symmetric or 7-array storage), block partial sums for the
+// two-pass reductions and the solver state the kernels share.  gfx950 only
+// (wave64).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,13 +14,32 @@ constexpr int kBlock = 256;      // threads per block: 4 wave64 per workgroup
 constexpr int kWaves = kBlock / 64;
 constexpr int kMaxLevels = 24;
 constexpr int kStaleMax = 64;
+constexpr int kMaxBlocks = 2048; // largest reduction grid (partials buffer)
+
+// n / d for 0 <= n < 2^31 with a precomputed multiplier: m = ceil(2^(31+s)/d),
+// s = ceil(log2 d); floor(n m / 2^(31+s)) = floor(n / d) because the rounding
+// error of m is below 2^(31+s)/d * d / 2^(31+s+s) <= 1/d (Granlund &
+// Montgomery, PLDI 1994).
+struct FastDiv {
+  uint32_t d, m, sh;
+  __device__ __forceinline__ int div(int n) const {
+    return (int)(((uint64_t)(uint32_t)n * m) >> sh);
+  }
+};
 
 // Interior grid, cell P = (i*ny + j)*nz + k, k fastest (the reference's
-// [i][j][k] order without ghosts).  Sizes fit 32 bits (N < 2^31).
+// [i][j][k] order without ghosts).  Sizes fit 31 bits.
 struct Grid {
   int nx, ny, nz;
   int sI;   // ny*nz
   int N;    // nx*ny*nz
+  FastDiv dI, dZ;  // division by sI and by nz
+  __device__ __forceinline__ void ijk(int P, int &i, int &j, int &k) const {
+    i = dI.div(P);
+    const int rem = P - i * sI;
+    j = dZ.div(rem);
+    k = rem - j * nz;
+  }
 };
 
 // Device view of a matrix.  SYM: only c, w, s, b are stored and
@@ -51,8 +71,9 @@ __device__ __forceinline__ VecN<T, V> zerov() {
   return r;
 }
 
-// Shared solver state: Krylov scalars, stopping data and reduction
-// bookkeeping.  One instance per precision per context, in device memory.
+// Shared solver state: Krylov scalars and stopping data.  One instance per
+// precision per context, in device memory; written only by the one-block
+// finalize kernels, read by the streaming kernels that follow.
 struct DevState {
   double rho, alpha, beta, omega, pq, rv;      // Krylov scalars
   double res0, res, rsq0, rsq;                 // norms
@@ -63,15 +84,16 @@ struct DevState {
   double *hist;                                // residual per iteration
   int hist_cap, pad1;
   double stale[kStaleMax];
-  double mean[kMaxLevels];                     // restriction mean per level
-  double msum[kMaxLevels];
+  double mean[kMaxLevels];                     // mean of the inner right-hand side
   double rv_ss, rv_ts, rv_tt;                  // BiCGS dots
+  double zmean;                                // mean of the preconditioned residual
 };
 
 // ---------------------------------------------------------------------------
-// reductions: wave64 butterfly, then LDS across the 4 waves, then one
-// partial per block; the last block to arrive sums the partials in a fixed
-// order, so results do not depend on scheduling.
+// reductions, pass 1: wave64 butterfly, LDS across the 4 waves, one partial
+// per block.  Pass 2 is a one-block kernel (k_finalize) that sums the
+// partials in a fixed order.  Two passes avoid the device-scope fence a
+// last-block-done scheme needs, which on MI355X writes back the XCD's L2.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -98,39 +120,14 @@ __device__ __forceinline__ void block_sum(double (&v)[NR], double (*sm)[NR]) {
   __syncthreads();
 }
 
-// Returns true in every thread of the last block; out[] then holds the grid
-// totals.  partial: gridDim.x*NR doubles; counter: one unsigned, zero on
-// entry, zero again on exit.
 template <int NR>
-__device__ bool grid_sum(double (&v)[NR], double *partial, unsigned *counter,
-                         double (&out)[NR]) {
+__device__ __forceinline__ void block_partial(double (&v)[NR], double *partial) {
   __shared__ double sm[kWaves][NR];
-  __shared__ int last;
   block_sum<NR>(v, sm);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int r = 0; r < NR; r++) partial[(size_t)blockIdx.x * NR + r] = v[r];
-    __threadfence();
-    unsigned t = atomicAdd(counter, 1u);
-    last = (t == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  double acc[NR];
-#pragma unroll
-  for (int r = 0; r < NR; r++) acc[r] = 0.0;
-  for (unsigned q = threadIdx.x; q < gridDim.x; q += blockDim.x) {
-#pragma unroll
-    for (int r = 0; r < NR; r++)
-      acc[r] += __hip_atomic_load(&partial[(size_t)q * NR + r], __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
-  }
-  block_sum<NR>(acc, sm);
-#pragma unroll
-  for (int r = 0; r < NR; r++) out[r] = acc[r];
-  if (threadIdx.x == 0) atomicExch(counter, 0u);
-  return true;
 }
 
 // Block -> work mapping.  Consecutive block ids land on different XCDs
@@ -150,8 +147,9 @@ __device__ __forceinline__ int xcd_block() {
 // (((b - cx) + s1) + s2) + s3 round exactly like the reference's four passes
 // (Src/Field/Scalar/scalar_operators.cpp:22-72).  Neighbours outside the
 // domain contribute 0 (their coefficient is 0 on a Neumann wall).
-// X provides vec(P) and at(Q) for the operand (a plain array or a value
-// formed on the fly, e.g. z + beta p).
+// X provides vec(P, i, j, k) and at(Q, i, j, k) for the operand (a plain
+// array or a value formed on the fly, e.g. z + beta p); the coordinates are
+// those of the first cell addressed.
 // ---------------------------------------------------------------------------
 template <typename T, int V>
 struct Gath {
@@ -169,24 +167,24 @@ __device__ __forceinline__ Gath<T, V> gather(const Grid &g, const DMat<T> &A,
              ab = ldv<T, V>(A.b + P);
   VecN<T, V> ae = zerov<T, V>(), an = zerov<T, V>(), at;
   G.c = ldv<T, V>(A.c + P);
-  G.x = X_.vec(P);
-  if (i > 0) xw = X_.vec(P - g.sI);
+  G.x = X_.vec(P, i, j, k);
+  if (i > 0) xw = X_.vec(P - g.sI, i - 1, j, k);
   if (i < g.nx - 1) {
-    xe = X_.vec(P + g.sI);
+    xe = X_.vec(P + g.sI, i + 1, j, k);
     ae = SYM ? ldv<T, V>(A.w + P + g.sI) : ldv<T, V>(A.e + P);
   } else if (!SYM) {
     ae = ldv<T, V>(A.e + P);
   }
-  if (j > 0) xs = X_.vec(P - g.nz);
+  if (j > 0) xs = X_.vec(P - g.nz, i, j - 1, k);
   if (j < g.ny - 1) {
-    xn = X_.vec(P + g.nz);
+    xn = X_.vec(P + g.nz, i, j + 1, k);
     an = SYM ? ldv<T, V>(A.s + P + g.nz) : ldv<T, V>(A.n + P);
   } else if (!SYM) {
     an = ldv<T, V>(A.n + P);
   }
   const bool has_t = (k + V) < g.nz;
-  T xb0 = (k > 0) ? X_.at(P - 1) : T(0);
-  T xtV = has_t ? X_.at(P + V) : T(0);
+  T xb0 = (k > 0) ? X_.at(P - 1, i, j, k - 1) : T(0);
+  T xtV = has_t ? X_.at(P + V, i, j, k + V) : T(0);
   if (SYM) {
     T btV = has_t ? A.b[P + V] : T(0);
 #pragma unroll
@@ -239,8 +237,10 @@ __device__ __forceinline__ VecN<T, V> resid(const VecN<T, V> &b,
 template <typename T, int V>
 struct ArrX {
   const T *x;
-  __device__ __forceinline__ VecN<T, V> vec(int P) const { return ldv<T, V>(x + P); }
-  __device__ __forceinline__ T at(int Q) const { return x[Q]; }
+  __device__ __forceinline__ VecN<T, V> vec(int P, int, int, int) const {
+    return ldv<T, V>(x + P);
+  }
+  __device__ __forceinline__ T at(int Q, int, int, int) const { return x[Q]; }
 };
 
 // operand: z + beta*p with z = r * (1/c) (diagonal preconditioner,
@@ -253,54 +253,60 @@ struct DiagPX {
     T z = rr * (T(1) / cc);
     return z + beta * pp;
   }
-  __device__ __forceinline__ VecN<T, V> vec(int P) const {
+  __device__ __forceinline__ VecN<T, V> vec(int P, int, int, int) const {
     VecN<T, V> R = ldv<T, V>(r + P), C = ldv<T, V>(c + P), Q = ldv<T, V>(p + P), o;
 #pragma unroll
     for (int q = 0; q < V; q++) o.v[q] = one(R.v[q], C.v[q], Q.v[q]);
     return o;
   }
-  __device__ __forceinline__ T at(int Q) const { return one(r[Q], c[Q], p[Q]); }
+  __device__ __forceinline__ T at(int Q, int, int, int) const { return one(r[Q], c[Q], p[Q]); }
 };
 
-// operand: z + beta*p with z given (preconditioned form)
+// operand: (z - mean z) + beta*p with z the preconditioner output, the mean
+// taken over the active cells (af == NULL: all cells active)
 template <typename T, int V>
 struct ZPX {
   const T *z, *p;
-  T beta;
-  __device__ __forceinline__ VecN<T, V> vec(int P) const {
+  const uint8_t *af;
+  T beta, zmean;
+  __device__ __forceinline__ T one(T zz, T pp, bool act) const {
+    return (act ? zz - zmean : zz) + beta * pp;
+  }
+  __device__ __forceinline__ VecN<T, V> vec(int P, int, int, int) const {
     VecN<T, V> Z = ldv<T, V>(z + P), Q = ldv<T, V>(p + P), o;
 #pragma unroll
-    for (int q = 0; q < V; q++) o.v[q] = Z.v[q] + beta * Q.v[q];
+    for (int q = 0; q < V; q++) o.v[q] = one(Z.v[q], Q.v[q], !af || af[P + q]);
     return o;
   }
-  __device__ __forceinline__ T at(int Q) const { return z[Q] + beta * p[Q]; }
+  __device__ __forceinline__ T at(int Q, int, int, int) const {
+    return one(z[Q], p[Q], !af || af[Q]);
+  }
 };
 
 // operand: e + cfac * eH[parent] (piecewise-constant prolongation,
-// additive_interpolation.cpp:12-64, scaled by the coarse-correction factor);
-// solid cells (af == 0) take no correction, so they stay decoupled
+// additive_interpolation.cpp:12-64, scaled by the coarse-correction
+// factor); solid cells (af == 0) take no correction.  The coarsening factor
+// per direction is 1 or 2 (shift sI/sJ/sK = 0 or 1).
 template <typename T, int V>
 struct ProlX {
   const T *e, *eH;
   const uint8_t *af;
   T cfac;
-  int nz, sI, CI, CJ, CK, nYH, nZH;
-  __device__ __forceinline__ int parent(int Q) const {
-    int i = Q / sI, rem = Q - i * sI, j = rem / nz, k = rem - j * nz;
-    return ((i / CI) * nYH + j / CJ) * nZH + k / CK;
+  int sI, sJ, sK, nYH, nZH;
+  __device__ __forceinline__ int parent(int i, int j, int k) const {
+    return ((i >> sI) * nYH + (j >> sJ)) * nZH + (k >> sK);
   }
-  __device__ __forceinline__ T at(int Q) const {
+  __device__ __forceinline__ T at(int Q, int i, int j, int k) const {
     if (af && !af[Q]) return e[Q];
-    return e[Q] + cfac * eH[parent(Q)];
+    return e[Q] + cfac * eH[parent(i, j, k)];
   }
-  __device__ __forceinline__ VecN<T, V> vec(int P) const {
+  __device__ __forceinline__ VecN<T, V> vec(int P, int i, int j, int k) const {
     VecN<T, V> E = ldv<T, V>(e + P), o;
-    int i = P / sI, rem = P - i * sI, j = rem / nz, k = rem - j * nz;
-    int base = ((i / CI) * nYH + j / CJ) * nZH;
+    const int base = ((i >> sI) * nYH + (j >> sJ)) * nZH;
 #pragma unroll
     for (int q = 0; q < V; q++) {
       const bool act = !af || af[P + q];
-      o.v[q] = act ? E.v[q] + cfac * eH[base + (k + q) / CK] : E.v[q];
+      o.v[q] = act ? E.v[q] + cfac * eH[base + ((k + q) >> sK)] : E.v[q];
     }
     return o;
   }

@@ -1,13 +1,14 @@
 // psb_kernels.h -- the hand-written gfx950 kernels of the pressure solver.
 //
-// Every kernel streams the interior arrays once (HBM-bound; no MFMA): a
-// block of 256 threads walks a contiguous run of V-wide cell vectors
-// (V = 16 bytes / sizeof(T)), neighbours come through L1/L2, and the
+// Every streaming kernel reads the interior arrays once (HBM-bound; no
+// MFMA): a block of 256 threads walks a contiguous run of V-wide cell
+// vectors (V = 16 bytes / sizeof(T)), neighbours come through L1/L2, and the
 // XCD-aware block remap keeps a slab's i+-1 planes in one L2.  Reductions
-// are deterministic (fixed-order last-block sums) and accumulate in FP64 for
-// both precisions.  Kernels of an iteration start with `if (st->done)
-// return;`, so a captured graph of several iterations stops doing work as
-// soon as the solver has finished.
+// are two-pass and deterministic: each streaming kernel leaves one partial
+// per block, k_finalize sums them in a fixed order in FP64 (for both
+// precisions) and applies the scalar step (alpha, beta, stop test, ...).
+// Kernels of an iteration start with `if (st->done) return;`, so a captured
+// graph of several iterations stops doing work once the solver has finished.
 #pragma once
 
 #include "psb_device.h"
@@ -24,10 +25,10 @@ struct Metric {
 };
 
 struct Work {
-  double *partial;
-  unsigned *counter;
+  double *partial;  // kMaxBlocks * 4 doubles
 };
 
+// contiguous run of V-vectors per block; P, i, j, k of the first cell
 #define PSB_TILE_BEGIN(g, V)                                                  \
   const int nvec_ = (g).N / (V);                                              \
   const int bl_ = xcd_block();                                                \
@@ -36,15 +37,472 @@ struct Work {
   const int q1_ = min(q0_ + per_, nvec_);                                     \
   for (int q_ = q0_ + (int)threadIdx.x; q_ < q1_; q_ += kBlock) {             \
     const int P = q_ * (V);                                                   \
-    const int i = P / (g).sI;                                                 \
-    const int rem_ = P - i * (g).sI;                                          \
-    const int j = rem_ / (g).nz;                                              \
-    const int k = rem_ - j * (g).nz;                                          \
-    (void)i; (void)j; (void)k;
+    int i, j, k;                                                              \
+    (g).ijk(P, i, j, k);
 #define PSB_TILE_END }
 
-// stop logic shared by the Krylov loops (cg.cpp:121-160, bicgs.cpp:174-210):
-// called by one thread after iteration st->iter with the new residual.
+// ---------------------------------------------------------------------------
+// y = A x   (Scalar::operator=(A_x), scalar_operators.cpp:49-72)
+// ---------------------------------------------------------------------------
+template <typename T, int V, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_spmv(Grid g, DMat<T> A, const T *x, T *y) {
+  ArrX<T, V> X{x};
+  PSB_TILE_BEGIN(g, V)
+  Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+  stv<T, V>(y + P, apply_A<T, V>(G));
+  PSB_TILE_END
+}
+
+// y = b - A x   (Scalar::operator=(y_m_A_x), scalar_operators.cpp:22-46)
+template <typename T, int V, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_resid(Grid g, DMat<T> A, const T *b, const T *x,
+                                                  T *y) {
+  ArrX<T, V> X{x};
+  PSB_TILE_BEGIN(g, V)
+  Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+  stv<T, V>(y + P, resid<T, V>(ldv<T, V>(b + P), G));
+  PSB_TILE_END
+}
+
+// ---------------------------------------------------------------------------
+// Krylov start: r = b - A x with its norms, and per MODE
+//   0 diagonal CG: rho = r.(r/c)        (cg.cpp:47-80)
+//   1 preconditioned CG: e1 = omega r / c into aux
+//   2 BiCGS: r~ = r into aux, rho = r~.r (bicgs.cpp:53-84)
+// partials: sum r^2/dV^2, sum r^2, rho
+// ---------------------------------------------------------------------------
+template <typename T, int V, bool SYM, int MODE>
+__global__ __launch_bounds__(kBlock) void k_start(Grid g, DMat<T> A, Metric m, const T *b,
+                                                  const T *x, T *r, T *aux, T omega, Work w) {
+  ArrX<T, V> X{x};
+  double acc[3] = {0.0, 0.0, 0.0};
+  PSB_TILE_BEGIN(g, V)
+  Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+  VecN<T, V> R = resid<T, V>(ldv<T, V>(b + P), G), E;
+  stv<T, V>(r + P, R);
+#pragma unroll
+  for (int q = 0; q < V; q++) {
+    double rd = (double)R.v[q];
+    double dv = m.dV(i, j, k + q);
+    acc[0] += rd * rd / dv / dv;
+    acc[1] += rd * rd;
+    if (MODE == 0) {
+      T z = R.v[q] * (T(1) / G.c.v[q]);
+      acc[2] += (double)(R.v[q] * z);
+    } else if (MODE == 1) {
+      E.v[q] = omega * R.v[q] / G.c.v[q];
+    } else {
+      acc[2] += rd * rd;
+    }
+  }
+  if (MODE == 1) stv<T, V>(aux + P, E);
+  if (MODE == 2) stv<T, V>(aux + P, R);
+  PSB_TILE_END
+  block_partial<3>(acc, w.partial);
+}
+
+// ---------------------------------------------------------------------------
+// CG step 1: p = z + beta p_old (z = r/c for the diagonal preconditioner,
+// else the projected preconditioner output), q = A p; partial p.q
+// (cg.cpp:75-108)
+// ---------------------------------------------------------------------------
+template <typename T, int V, bool SYM, bool DIAG>
+__global__ __launch_bounds__(kBlock) void k_cg_pq(Grid g, DMat<T> A, const T *r_or_z,
+                                                  const T *pold, T *pnew, T *q,
+                                                  const uint8_t *af, DevState *st, Work w) {
+  if (st->done) return;
+  const T beta = (T)st->beta;
+  double acc[1] = {0.0};
+  PSB_TILE_BEGIN(g, V)
+  Gath<T, V> G;
+  if (DIAG) {
+    DiagPX<T, V> X{r_or_z, A.c, pold, beta};
+    G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+  } else {
+    ZPX<T, V> X{r_or_z, pold, af, beta, (T)st->zmean};
+    G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+  }
+  VecN<T, V> Q = apply_A<T, V>(G);
+  stv<T, V>(pnew + P, G.x);
+  stv<T, V>(q + P, Q);
+#pragma unroll
+  for (int u = 0; u < V; u++) acc[0] += (double)(G.x.v[u] * Q.v[u]);
+  PSB_TILE_END
+  block_partial<1>(acc, w.partial);
+}
+
+// ---------------------------------------------------------------------------
+// CG step 2: x += alpha p, r -= alpha q (cg.cpp:114-115); partials of the
+// residual norms and, in diagonal mode, rho_next = r.(r/c); preconditioned
+// mode writes e1 = omega r / c, the first Jacobi sweep of the next V-cycle.
+// ---------------------------------------------------------------------------
+template <typename T, int V, bool DIAG>
+__global__ __launch_bounds__(kBlock) void k_cg_xr(Grid g, const T *c, Metric m, const T *p,
+                                                  const T *q, T *x, T *r, T *e1, T omega,
+                                                  DevState *st, Work w) {
+  if (st->done) return;
+  const T alpha = (T)st->alpha;
+  double acc[3] = {0.0, 0.0, 0.0};  // voldiv, r^2, rho_next
+  PSB_TILE_BEGIN(g, V)
+  VecN<T, V> X = ldv<T, V>(x + P), R = ldv<T, V>(r + P), Pp = ldv<T, V>(p + P),
+             Q = ldv<T, V>(q + P), C = ldv<T, V>(c + P), E;
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    X.v[u] = X.v[u] + alpha * Pp.v[u];
+    R.v[u] = R.v[u] - alpha * Q.v[u];
+    double rd = (double)R.v[u];
+    acc[1] += rd * rd;
+    if (DIAG) {
+      double dv = m.dV(i, j, k + u);
+      acc[0] += rd * rd / dv / dv;
+      T z = R.v[u] * (T(1) / C.v[u]);
+      acc[2] += (double)(R.v[u] * z);
+    } else {
+      E.v[u] = omega * R.v[u] / C.v[u];
+    }
+  }
+  stv<T, V>(x + P, X);
+  stv<T, V>(r + P, R);
+  if (!DIAG) stv<T, V>(e1 + P, E);
+  PSB_TILE_END
+  block_partial<3>(acc, w.partial);
+}
+
+// ---------------------------------------------------------------------------
+// damped Jacobi sweep  e_out = e + omega (r - A e) / c, the operand either e
+// or e + cfac P eH (prolongation fused into the first post-sweep).
+// RHO (last post-sweep on level 0): partials r.z, sum z, sum r, active
+// cells, from which k_finalize forms rho = r.(z - mean z): the constants
+// span the null space of the Neumann operator, and the projection keeps the
+// search directions out of it (in FP32 a drift along it eats the digits of
+// the correction).
+// ---------------------------------------------------------------------------
+template <typename T, int V, bool SYM, bool PROL, bool RHO>
+__global__ __launch_bounds__(kBlock) void k_jacobi(Grid g, DMat<T> A, const T *r, const T *e,
+                                                   T *eout, ProlX<T, V> pr, T omega,
+                                                   const uint8_t *af, DevState *st, Work w) {
+  if (st->done) return;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  PSB_TILE_BEGIN(g, V)
+  Gath<T, V> G;
+  if (PROL) {
+    G = gather<T, V, SYM>(g, A, pr, P, i, j, k);
+  } else {
+    ArrX<T, V> X{e};
+    G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+  }
+  VecN<T, V> R = ldv<T, V>(r + P);
+  VecN<T, V> t = resid<T, V>(R, G), E;
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    E.v[u] = G.x.v[u] + omega * t.v[u] / G.c.v[u];
+    if (RHO && (!af || af[P + u])) {
+      acc[0] += (double)R.v[u] * (double)E.v[u];
+      acc[1] += (double)E.v[u];
+      acc[2] += (double)R.v[u];
+      acc[3] += 1.0;
+    }
+  }
+  stv<T, V>(eout + P, E);
+  PSB_TILE_END
+  if (RHO) block_partial<4>(acc, w.partial);
+}
+
+// ---------------------------------------------------------------------------
+// restriction for the preconditioner: rH = sum over the fine block of
+// (r - A e)  (the residual sum of additive_restriction.cpp:12-135 without the
+// reference's re-expansion; solid fine cells left out, the transpose of the
+// masked prolongation), 0 on inactive coarse cells, and the coarse level's
+// first sweep eH1 = omega rH / cH fused in.  A thread covers CI planes x CJ
+// rows x V*KV cells along k of the fine grid (whole coarse cells); fine
+// cells are summed in (i, j, k) order within each coarse cell.
+// ---------------------------------------------------------------------------
+template <typename T, int V, int KV, bool SYM, int CI, int CJ, int CK>
+__global__ __launch_bounds__(kBlock) void k_restrict(Grid g, DMat<T> A, const T *r, const T *e,
+                                                     const uint8_t *af, Grid gH, const T *cH,
+                                                     const uint8_t *afH, T *rH, T *eH1,
+                                                     T omega, DevState *st) {
+  if (st->done) return;
+  constexpr int KE = V * KV;        // fine cells along k per thread
+  constexpr int NC = KE / CK;       // coarse cells per thread
+  ArrX<T, V> X{e};
+  const int nkv = g.nz / KE;
+  const int nwork = gH.nx * gH.ny * nkv;
+  const int b = xcd_block();
+  for (int q = b * kBlock + threadIdx.x; q < nwork; q += gridDim.x * kBlock) {
+    const int IJ = q / nkv, kv = q - IJ * nkv;
+    const int I = IJ / gH.ny, J = IJ - I * gH.ny;
+    T s[NC];
+#pragma unroll
+    for (int u = 0; u < NC; u++) s[u] = T(0);
+#pragma unroll
+    for (int a = 0; a < CI; a++) {
+#pragma unroll
+      for (int bb = 0; bb < CJ; bb++) {
+        const int i = I * CI + a, j = J * CJ + bb;
+        T t[KE];
+#pragma unroll
+        for (int h = 0; h < KV; h++) {
+          const int k = kv * KE + h * V;
+          const int P = (i * g.ny + j) * g.nz + k;
+          Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+          VecN<T, V> R = resid<T, V>(ldv<T, V>(r + P), G);
+#pragma unroll
+          for (int u = 0; u < V; u++) t[h * V + u] = (!af || af[P + u]) ? R.v[u] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < NC; u++)
+#pragma unroll
+          for (int cc = 0; cc < CK; cc++) s[u] += t[u * CK + cc];
+      }
+    }
+    const int Q0 = (I * gH.ny + J) * gH.nz + kv * NC;
+#pragma unroll
+    for (int u = 0; u < NC; u++) {
+      const bool act = !afH || afH[Q0 + u];
+      const T v = act ? s[u] : T(0);
+      rH[Q0 + u] = v;
+      eH1[Q0 + u] = omega * v / cH[Q0 + u];
+    }
+  }
+}
+
+// coarsest level: z = G r, G the dense projected pseudo-inverse (FP64).
+// One wave per row; r is read in T and promoted.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gemv(int m, const double *Gm, const T *r, T *z,
+                                                 DevState *st) {
+  if (st->done) return;
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (row >= m) return;
+  const double *g = Gm + (size_t)row * m;
+  double s = 0.0;
+  for (int c = lane; c < m; c += 64) s += g[c] * (double)r[c];
+  s = wave_sum(s);
+  if (lane == 0) z[row] = (T)s;
+}
+
+// ---------------------------------------------------------------------------
+// iterative refinement: r32 = (float)(b - A x) computed in FP64; partials
+// of the FP64 norms and of the mean of r32 over the active cells
+// ---------------------------------------------------------------------------
+template <int V, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_ir_resid(Grid g, DMat<double> A, Metric m,
+                                                     const double *b, const double *x,
+                                                     float *r32, const uint8_t *af, Work w) {
+  ArrX<double, V> X{x};
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};  // r64^2, r64^2/dV^2, sum r32, active cells
+  PSB_TILE_BEGIN(g, V)
+  Gath<double, V> G = gather<double, V, SYM>(g, A, X, P, i, j, k);
+  VecN<double, V> R = resid<double, V>(ldv<double, V>(b + P), G);
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    float rf = (float)R.v[u];
+    r32[P + u] = rf;
+    double dv = m.dV(i, j, k + u);
+    acc[0] += R.v[u] * R.v[u];
+    acc[1] += R.v[u] * R.v[u] / dv / dv;
+    if (!af || af[P + u]) {
+      acc[2] += (double)rf;
+      acc[3] += 1.0;
+    }
+  }
+  PSB_TILE_END
+  block_partial<4>(acc, w.partial);
+}
+
+// start of the inner FP32 solve: the right-hand side loses its mean over
+// the active cells (the part no correction can remove: it lies outside the
+// range of the singular Neumann operator), e1 = omega r / c; partial ||r||^2
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_center(Grid g, const float *c, const uint8_t *af,
+                                                   float *r, float *e1, float omega,
+                                                   const DevState *st, Work w) {
+  const float mean = (float)st->mean[0];
+  double acc[1] = {0.0};
+  PSB_TILE_BEGIN(g, V)
+  VecN<float, V> R = ldv<float, V>(r + P), Cc = ldv<float, V>(c + P), E;
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    if (!af || af[P + u]) R.v[u] = R.v[u] - mean;
+    E.v[u] = omega * R.v[u] / Cc.v[u];
+    acc[0] += (double)R.v[u] * (double)R.v[u];
+  }
+  stv<float, V>(r + P, R);
+  stv<float, V>(e1 + P, E);
+  PSB_TILE_END
+  block_partial<1>(acc, w.partial);
+}
+
+// x64 += (double) d32, d32 = 0
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_ir_update(Grid g, double *x, float *d) {
+  PSB_TILE_BEGIN(g, V)
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    x[P + u] = x[P + u] + (double)d[P + u];
+    d[P + u] = 0.0f;
+  }
+  PSB_TILE_END
+}
+
+// true relative residual ||b - A x|| / ||b|| (FP64 diagnostic after a solve)
+template <int V, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_relres(Grid g, DMat<double> A, const double *b,
+                                                   const double *x, Work w) {
+  ArrX<double, V> X{x};
+  double acc[2] = {0.0, 0.0};
+  PSB_TILE_BEGIN(g, V)
+  Gath<double, V> G = gather<double, V, SYM>(g, A, X, P, i, j, k);
+  VecN<double, V> B = ldv<double, V>(b + P), R = resid<double, V>(B, G);
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    acc[0] += R.v[u] * R.v[u];
+    acc[1] += B.v[u] * B.v[u];
+  }
+  PSB_TILE_END
+  block_partial<2>(acc, w.partial);
+}
+
+// ---------------------------------------------------------------------------
+// BiCGS (bicgs.cpp:79-216) in three passes per iteration.
+// B1: p = r + beta (p_old - omega v_old) formed as the reference does
+//     (p = r + beta p; p -= (beta omega) v), p^ = p / c (Diagonal), v = A p^;
+//     partial r~.v
+// ---------------------------------------------------------------------------
+template <typename T, int V>
+struct BiPX {
+  const T *r, *p, *v, *c;
+  T beta, bo;
+  __device__ __forceinline__ T pnew(T rr, T pp, T vv) const {
+    T t = rr + beta * pp;
+    return t - bo * vv;
+  }
+  __device__ __forceinline__ VecN<T, V> vec(int P, int, int, int) const {
+    VecN<T, V> R = ldv<T, V>(r + P), Pp = ldv<T, V>(p + P), Vv = ldv<T, V>(v + P),
+               C = ldv<T, V>(c + P), o;
+#pragma unroll
+    for (int u = 0; u < V; u++) o.v[u] = pnew(R.v[u], Pp.v[u], Vv.v[u]) * (T(1) / C.v[u]);
+    return o;
+  }
+  __device__ __forceinline__ T at(int Q, int, int, int) const {
+    return pnew(r[Q], p[Q], v[Q]) * (T(1) / c[Q]);
+  }
+};
+
+template <typename T, int V, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_bi1(Grid g, DMat<T> A, const T *r, const T *rt,
+                                                const T *pold, const T *vold, T *pnew,
+                                                T *vnew, DevState *st, Work w) {
+  if (st->done) return;
+  const bool first = st->iter == 0;
+  const T beta = first ? T(0) : (T)st->beta;
+  const T bo = first ? T(0) : (T)(st->beta * st->omega);
+  BiPX<T, V> X{r, pold, vold, A.c, beta, bo};
+  double acc[1] = {0.0};
+  PSB_TILE_BEGIN(g, V)
+  Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+  VecN<T, V> Vv = apply_A<T, V>(G), Rt = ldv<T, V>(rt + P), R = ldv<T, V>(r + P),
+             Pp = ldv<T, V>(pold + P), Vo = ldv<T, V>(vold + P), Pn;
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    Pn.v[u] = first ? R.v[u] : X.pnew(R.v[u], Pp.v[u], Vo.v[u]);
+    acc[0] += (double)(Rt.v[u] * Vv.v[u]);
+  }
+  stv<T, V>(pnew + P, Pn);
+  stv<T, V>(vnew + P, Vv);
+  PSB_TILE_END
+  block_partial<1>(acc, w.partial);
+}
+
+// B2: s = r - alpha v, s^ = s / c, t = A s^; partials s.s, t.s, t.t
+template <typename T, int V>
+struct BiSX {
+  const T *r, *v, *c;
+  T alpha;
+  __device__ __forceinline__ T s1(T rr, T vv) const { return rr - alpha * vv; }
+  __device__ __forceinline__ VecN<T, V> vec(int P, int, int, int) const {
+    VecN<T, V> R = ldv<T, V>(r + P), Vv = ldv<T, V>(v + P), C = ldv<T, V>(c + P), o;
+#pragma unroll
+    for (int u = 0; u < V; u++) o.v[u] = s1(R.v[u], Vv.v[u]) * (T(1) / C.v[u]);
+    return o;
+  }
+  __device__ __forceinline__ T at(int Q, int, int, int) const {
+    return s1(r[Q], v[Q]) * (T(1) / c[Q]);
+  }
+};
+
+template <typename T, int V, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_bi2(Grid g, DMat<T> A, const T *r, const T *v,
+                                                T *s, T *t, DevState *st, Work w) {
+  if (st->done) return;
+  BiSX<T, V> X{r, v, A.c, (T)st->alpha};
+  double acc[3] = {0.0, 0.0, 0.0};  // ss, ts, tt
+  PSB_TILE_BEGIN(g, V)
+  Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+  VecN<T, V> Tt = apply_A<T, V>(G), R = ldv<T, V>(r + P), Vv = ldv<T, V>(v + P), S;
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    S.v[u] = X.s1(R.v[u], Vv.v[u]);
+    acc[0] += (double)(S.v[u] * S.v[u]);
+    acc[1] += (double)(Tt.v[u] * S.v[u]);
+    acc[2] += (double)(Tt.v[u] * Tt.v[u]);
+  }
+  stv<T, V>(s + P, S);
+  stv<T, V>(t + P, Tt);
+  PSB_TILE_END
+  block_partial<3>(acc, w.partial);
+}
+
+// B3: x += alpha p^; x += omega s^; r = s - omega t; partials of the norms
+// and rho_next = r~.r.  With a small s (bicgs.cpp:134-138) only x += alpha p^.
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void k_bi3(Grid g, const T *c, Metric m, const T *p,
+                                                const T *s, const T *t, const T *rt, T *x,
+                                                T *r, DevState *st, Work w) {
+  if (st->done) return;
+  const bool small_s = sqrt(st->rv_ss) < st->res_tol;
+  const T alpha = (T)st->alpha, omega = (T)st->omega;
+  double acc[3] = {0.0, 0.0, 0.0};  // voldiv, r^2, rho_next
+  PSB_TILE_BEGIN(g, V)
+  VecN<T, V> X = ldv<T, V>(x + P), Pp = ldv<T, V>(p + P), C = ldv<T, V>(c + P);
+  if (small_s) {
+#pragma unroll
+    for (int u = 0; u < V; u++) X.v[u] = X.v[u] + alpha * (Pp.v[u] * (T(1) / C.v[u]));
+    stv<T, V>(x + P, X);
+  } else {
+    VecN<T, V> S = ldv<T, V>(s + P), Tt = ldv<T, V>(t + P), Rt = ldv<T, V>(rt + P), R;
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+      T mc = T(1) / C.v[u];
+      X.v[u] = X.v[u] + alpha * (Pp.v[u] * mc);
+      X.v[u] = X.v[u] + omega * (S.v[u] * mc);
+      R.v[u] = S.v[u] - omega * Tt.v[u];
+      double rd = (double)R.v[u];
+      double dv = m.dV(i, j, k + u);
+      acc[0] += rd * rd / dv / dv;
+      acc[1] += rd * rd;
+      acc[2] += (double)(Rt.v[u] * R.v[u]);
+    }
+    stv<T, V>(x + P, X);
+    stv<T, V>(r + P, R);
+  }
+  PSB_TILE_END
+  block_partial<3>(acc, w.partial);
+}
+
+// ---------------------------------------------------------------------------
+// pass 2 of the reductions: one block sums the partials of the kernel
+// before it (fixed order) and applies the scalar step
+// ---------------------------------------------------------------------------
+enum FinOp {
+  FIN_START_DIAG, FIN_START_AC, FIN_START_BICGS, FIN_PQ, FIN_XR_DIAG, FIN_XR_AC, FIN_RHO,
+  FIN_IR_RESID, FIN_CENTER, FIN_RELRES, FIN_BI1, FIN_BI2, FIN_BI3
+};
+
+// stop logic shared by the Krylov loops (cg.cpp:121-160, bicgs.cpp:174-210),
+// after iteration st->iter with the new residual
 __device__ inline void krylov_stop(DevState *st, double res, bool small_omega) {
   const int i = st->iter;
   st->res = res;
@@ -89,533 +547,135 @@ __device__ inline double ref_norm(const DevState *st, double sum_voldiv) {
   return sqrt(d) / st->scale;
 }
 
-// ---------------------------------------------------------------------------
-// y = A x   (Scalar::operator=(A_x), scalar_operators.cpp:49-72)
-// ---------------------------------------------------------------------------
-template <typename T, int V, bool SYM>
-__global__ __launch_bounds__(kBlock) void k_spmv(Grid g, DMat<T> A, const T *x, T *y) {
-  ArrX<T, V> X{x};
-  PSB_TILE_BEGIN(g, V)
-  Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
-  stv<T, V>(y + P, apply_A<T, V>(G));
-  PSB_TILE_END
+__device__ inline void krylov_init(DevState *st, double res, double rsq, double rho) {
+  st->res0 = res;
+  st->res = res;
+  st->rsq0 = rsq;
+  st->rsq = rsq;
+  st->rho = rho;
+  st->beta = 0.0;
+  st->zmean = 0.0;
+  st->iter = 0;
+  st->report_iters = 0;
+  st->converged = 0;
+  st->reason = 0;
+  st->done = 0;
+  if (st->hist && st->hist_cap > 0) st->hist[0] = res;
+  for (int q = 0; q < st->stalecount && q < kStaleMax; q++) st->stale[q] = 1.0e24;
+  if (st->stalecount > 0) st->stale[st->stalecount - 1] = res;
 }
 
-// y = b - A x   (Scalar::operator=(y_m_A_x), scalar_operators.cpp:22-46)
-template <typename T, int V, bool SYM>
-__global__ __launch_bounds__(kBlock) void k_resid(Grid g, DMat<T> A, const T *b, const T *x,
-                                                  T *y) {
-  ArrX<T, V> X{x};
-  PSB_TILE_BEGIN(g, V)
-  Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
-  stv<T, V>(y + P, resid<T, V>(ldv<T, V>(b + P), G));
-  PSB_TILE_END
-}
-
-// ---------------------------------------------------------------------------
-// Krylov start: r = b - A x, res0, rho0 and (AC) e1 = omega r / c.
-// mode 0: diagonal CG (rho = r.(r/c)), 1: preconditioned CG (e1),
-// 2: BiCGS (r~ = r, rho = r~.r).
-// ---------------------------------------------------------------------------
-template <typename T, int V, bool SYM, int MODE>
-__global__ __launch_bounds__(kBlock) void k_start(Grid g, DMat<T> A, Metric m, const T *b,
-                                                  const T *x, T *r, T *aux, T omega,
-                                                  DevState *st, Work w) {
-  ArrX<T, V> X{x};
-  double acc[3] = {0.0, 0.0, 0.0};  // sum r^2/dV^2, sum r^2, rho
-  PSB_TILE_BEGIN(g, V)
-  Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
-  VecN<T, V> R = resid<T, V>(ldv<T, V>(b + P), G), E;
-  stv<T, V>(r + P, R);
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_finalize(int op, int nblk, const double *partial,
+                                                     DevState *st, DevState *st2, double *out) {
+  if (op >= FIN_PQ && op <= FIN_RHO && st->done) return;
+  if (op >= FIN_BI1 && st->done) return;
+  __shared__ double sm[kWaves][NR];
+  double acc[NR];
 #pragma unroll
-  for (int q = 0; q < V; q++) {
-    double rd = (double)R.v[q];
-    double dv = m.dV(i, j, k + q);
-    acc[0] += rd * rd / dv / dv;
-    acc[1] += rd * rd;
-    if (MODE == 0) {
-      T z = R.v[q] * (T(1) / G.c.v[q]);
-      acc[2] += (double)(R.v[q] * z);
-    } else if (MODE == 1) {
-      E.v[q] = omega * R.v[q] / G.c.v[q];
-    } else {
-      acc[2] += rd * rd;
-    }
-  }
-  if (MODE == 1) stv<T, V>(aux + P, E);
-  if (MODE == 2) stv<T, V>(aux + P, R);
-  PSB_TILE_END
-  double tot[3];
-  if (grid_sum<3>(acc, w.partial, w.counter, tot) && threadIdx.x == 0) {
-    double res = st->norm == 0 ? ref_norm(st, tot[0]) : sqrt(tot[1]);
-    st->res0 = res;
-    st->res = res;
-    st->rsq0 = tot[1];
-    st->rsq = tot[1];
-    st->rho = tot[2];
-    st->beta = 0.0;
-    st->iter = 0;
-    st->report_iters = 0;
-    st->converged = 0;
-    st->reason = 0;
-    st->done = 0;
-    if (st->hist && st->hist_cap > 0) st->hist[0] = res;
-    for (int q = 0; q < st->stalecount && q < kStaleMax; q++) st->stale[q] = 1.0e24;
-    if (st->stalecount > 0) st->stale[st->stalecount - 1] = res;
-    if (res < st->res_tol) {  // cg.cpp:64 early return
-      st->done = 1;
-      st->converged = 1;
-      st->reason = 7;
-    }
-    if (!(res == res)) {
-      st->done = 1;
-      st->reason = 8;
-    }
-    if (MODE == 2 && !st->done && sqrt(fabs(tot[2])) < st->res_tol) {
-      st->done = 1;  // bicgs.cpp:89 at i = 0
-      st->reason = 4;
-      st->report_iters = 1;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// CG step 1: p = z + beta p_old (z = r/c for the diagonal preconditioner,
-// else the preconditioner output), q = A p, alpha = rho / (p.q).
-// cg.cpp:75-108
-// ---------------------------------------------------------------------------
-template <typename T, int V, bool SYM, bool DIAG>
-__global__ __launch_bounds__(kBlock) void k_cg_pq(Grid g, DMat<T> A, const T *r_or_z,
-                                                  const T *pold, T *pnew, T *q,
-                                                  DevState *st, Work w) {
-  if (st->done) return;
-  const T beta = (T)st->beta;
-  double acc[1] = {0.0};
-  PSB_TILE_BEGIN(g, V)
-  Gath<T, V> G;
-  if (DIAG) {
-    DiagPX<T, V> X{r_or_z, A.c, pold, beta};
-    G = gather<T, V, SYM>(g, A, X, P, i, j, k);
-  } else {
-    ZPX<T, V> X{r_or_z, pold, beta};
-    G = gather<T, V, SYM>(g, A, X, P, i, j, k);
-  }
-  VecN<T, V> Q = apply_A<T, V>(G);
-  stv<T, V>(pnew + P, G.x);
-  stv<T, V>(q + P, Q);
+  for (int r = 0; r < NR; r++) acc[r] = 0.0;
+  for (int q = threadIdx.x; q < nblk; q += kBlock) {
 #pragma unroll
-  for (int u = 0; u < V; u++) acc[0] += (double)(G.x.v[u] * Q.v[u]);
-  PSB_TILE_END
-  double tot[1];
-  if (grid_sum<1>(acc, w.partial, w.counter, tot) && threadIdx.x == 0) {
-    st->pq = tot[0];
-    st->alpha = st->rho / tot[0];
+    for (int r = 0; r < NR; r++) acc[r] += partial[(size_t)q * NR + r];
   }
-}
-
-// ---------------------------------------------------------------------------
-// CG step 2: x += alpha p, r -= alpha q, residual norm and stop test
-// (cg.cpp:114-160); diagonal mode also forms rho_next = r.(r/c) and beta;
-// preconditioned mode writes e1 = omega r / c, the first Jacobi sweep of the
-// next V-cycle.
-// ---------------------------------------------------------------------------
-template <typename T, int V, bool DIAG>
-__global__ __launch_bounds__(kBlock) void k_cg_xr(Grid g, const T *c, Metric m, const T *p,
-                                                  const T *q, T *x, T *r, T *e1, T omega,
-                                                  DevState *st, Work w) {
-  if (st->done) return;
-  const T alpha = (T)st->alpha;
-  double acc[3] = {0.0, 0.0, 0.0};  // voldiv, r^2, rho_next
-  PSB_TILE_BEGIN(g, V)
-  VecN<T, V> X = ldv<T, V>(x + P), R = ldv<T, V>(r + P), Pp = ldv<T, V>(p + P),
-             Q = ldv<T, V>(q + P), C = ldv<T, V>(c + P), E;
-#pragma unroll
-  for (int u = 0; u < V; u++) {
-    X.v[u] = X.v[u] + alpha * Pp.v[u];
-    R.v[u] = R.v[u] - alpha * Q.v[u];
-    double rd = (double)R.v[u];
-    double dv = m.dV(i, j, k + u);
-    acc[0] += rd * rd / dv / dv;
-    acc[1] += rd * rd;
-    if (DIAG) {
-      T z = R.v[u] * (T(1) / C.v[u]);
-      acc[2] += (double)(R.v[u] * z);
-    } else {
-      E.v[u] = omega * R.v[u] / C.v[u];
-    }
-  }
-  stv<T, V>(x + P, X);
-  stv<T, V>(r + P, R);
-  if (!DIAG) stv<T, V>(e1 + P, E);
-  PSB_TILE_END
-  double tot[3];
-  if (grid_sum<3>(acc, w.partial, w.counter, tot) && threadIdx.x == 0) {
-    double res = st->norm == 0 ? ref_norm(st, tot[0]) : sqrt(tot[1]);
-    st->rsq = tot[1];
-    krylov_stop(st, res, false);
-    if (DIAG && !st->done) {
-      st->beta = tot[2] / st->rho;  // cg.cpp:93
-      st->rho = tot[2];
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// damped Jacobi sweep  e_out = e + omega (r - A e) / c, with the operand
-// either e itself or e + cfac P eH (prolongation fused into the first post
-// sweep).  RHO: also form rho = r.z for the CG step that follows (last post
-// sweep on the fine level).
-// ---------------------------------------------------------------------------
-template <typename T, int V, bool SYM, bool PROL, bool RHO>
-__global__ __launch_bounds__(kBlock) void k_jacobi(Grid g, DMat<T> A, const T *r, const T *e,
-                                                   T *eout, ProlX<T, V> pr, T omega,
-                                                   DevState *st, Work w) {
-  if (st->done) return;
-  double acc[1] = {0.0};
-  PSB_TILE_BEGIN(g, V)
-  Gath<T, V> G;
-  if (PROL) {
-    G = gather<T, V, SYM>(g, A, pr, P, i, j, k);
-  } else {
-    ArrX<T, V> X{e};
-    G = gather<T, V, SYM>(g, A, X, P, i, j, k);
-  }
-  VecN<T, V> R = ldv<T, V>(r + P);
-  VecN<T, V> t = resid<T, V>(R, G), E;
-#pragma unroll
-  for (int u = 0; u < V; u++) {
-    E.v[u] = G.x.v[u] + omega * t.v[u] / G.c.v[u];
-    if (RHO) acc[0] += (double)(R.v[u] * E.v[u]);
-  }
-  stv<T, V>(eout + P, E);
-  PSB_TILE_END
-  if (RHO) {
-    double tot[1];
-    if (grid_sum<1>(acc, w.partial, w.counter, tot) && threadIdx.x == 0) {
-      st->beta = st->iter == 0 ? 0.0 : tot[0] / st->rho;
-      st->rho = tot[0];
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// restriction for the preconditioner: rH[I] = sum over the fine block of
-// (r - A e)  (the residual sum of additive_restriction.cpp:12-135 without
-// the reference's re-expansion; solid fine cells left out, the transpose of
-// the masked prolongation), 0 on inactive coarse cells, and the sum
-// over active cells for the mean removal of additive_restriction.cpp:151-174.
-// One thread per coarse cell.
-// ---------------------------------------------------------------------------
-template <typename T, bool SYM>
-__global__ __launch_bounds__(kBlock) void k_restrict(Grid g, DMat<T> A, const T *r, const T *e,
-                                                     const uint8_t *aflag, Grid gH, int CI,
-                                                     int CJ, int CK, const uint8_t *aflagH,
-                                                     T *rH, int lev, DevState *st, Work w) {
-  if (st->done) return;
-  ArrX<T, 1> X{e};
-  double acc[2] = {0.0, 0.0};
-  const int b = xcd_block();
-  for (int Q = b * kBlock + threadIdx.x; Q < gH.N; Q += gridDim.x * kBlock) {
-    const int I = Q / gH.sI, remQ = Q - I * gH.sI, J = remQ / gH.nz, K = remQ - J * gH.nz;
-    T s = T(0);
-    for (int a = 0; a < CI; a++)
-      for (int bb = 0; bb < CJ; bb++)
-        for (int cc = 0; cc < CK; cc++) {
-          const int i = I * CI + a, j = J * CJ + bb, k = K * CK + cc;
-          const int P = (i * g.ny + j) * g.nz + k;
-          if (aflag && !aflag[P]) continue;
-          Gath<T, 1> G = gather<T, 1, SYM>(g, A, X, P, i, j, k);
-          s += resid<T, 1>(ldv<T, 1>(r + P), G).v[0];
-        }
-    const bool act = aflagH ? aflagH[Q] != 0 : true;
-    if (!act) s = T(0);
-    rH[Q] = s;
-    if (act) {
-      acc[0] += (double)s;
-      acc[1] += 1.0;
-    }
-  }
-  double tot[2];
-  if (grid_sum<2>(acc, w.partial, w.counter, tot) && threadIdx.x == 0) {
-    st->msum[lev] = tot[0];
-    st->mean[lev] = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
-  }
-}
-
-// first sweep on a coarse level: rH -= mean (active cells), e1 = omega rH / c
-template <typename T, int V>
-__global__ __launch_bounds__(kBlock) void k_coarse_first(Grid g, const T *c, const uint8_t *aflag,
-                                                         T *r, T *e1, T omega, int lev,
-                                                         int nu1, DevState *st) {
-  if (st->done) return;
-  const T mean = (T)st->mean[lev];
-  PSB_TILE_BEGIN(g, V)
-  VecN<T, V> R = ldv<T, V>(r + P), C = ldv<T, V>(c + P), E;
-#pragma unroll
-  for (int u = 0; u < V; u++) {
-    if (!aflag || aflag[P + u]) R.v[u] = R.v[u] - mean;
-    E.v[u] = nu1 > 0 ? omega * R.v[u] / C.v[u] : T(0);
-  }
-  stv<T, V>(r + P, R);
-  stv<T, V>(e1 + P, E);
-  PSB_TILE_END
-}
-
-// coarsest level: z = G r, G the dense projected pseudo-inverse (FP64).
-// One wave per row; r is read in T and promoted.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_gemv(int m, const double *Gm, const T *r, T *z,
-                                                 DevState *st) {
-  if (st->done) return;
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * kWaves + (threadIdx.x >> 6);
-  if (row >= m) return;
-  const double *g = Gm + (size_t)row * m;
-  double s = 0.0;
-  for (int c = lane; c < m; c += 64) s += g[c] * (double)r[c];
-  s = wave_sum(s);
-  if (lane == 0) z[row] = (T)s;
-}
-
-// ---------------------------------------------------------------------------
-// iterative refinement: r32 = (float)(b - A x) in FP64 with its FP64 norm,
-// the inner solve's first Jacobi sweep e1 = omega r / c, and the FP32 norm
-// that starts the inner solve.
-// ---------------------------------------------------------------------------
-template <int V, bool SYM>
-__global__ __launch_bounds__(kBlock) void k_ir_resid(Grid g, DMat<double> A, Metric m,
-                                                     const double *b, const double *x,
-                                                     float *r32, float *e1, float omega,
-                                                     DevState *st64, DevState *st32, Work w) {
-  ArrX<double, V> X{x};
-  double acc[3] = {0.0, 0.0, 0.0};  // r64^2, r32^2, r64^2/dV^2
-  PSB_TILE_BEGIN(g, V)
-  Gath<double, V> G = gather<double, V, SYM>(g, A, X, P, i, j, k);
-  VecN<double, V> R = resid<double, V>(ldv<double, V>(b + P), G);
-#pragma unroll
-  for (int u = 0; u < V; u++) {
-    float rf = (float)R.v[u];
-    float cf = (float)G.c.v[u];
-    r32[P + u] = rf;
-    e1[P + u] = omega * rf / cf;
-    double dv = m.dV(i, j, k + u);
-    acc[0] += R.v[u] * R.v[u];
-    acc[1] += (double)rf * (double)rf;
-    acc[2] += R.v[u] * R.v[u] / dv / dv;
-  }
-  PSB_TILE_END
-  double tot[3];
-  if (grid_sum<3>(acc, w.partial, w.counter, tot) && threadIdx.x == 0) {
-    st64->rsq = tot[0];
-    st64->res = st64->norm == 0 ? ref_norm(st64, tot[2]) : sqrt(tot[0]);
-    double r0 = sqrt(tot[1]);
-    st32->res0 = r0;
-    st32->res = r0;
-    st32->rsq0 = tot[1];
-    st32->rsq = tot[1];
-    st32->beta = 0.0;
-    st32->iter = 0;
-    st32->report_iters = 0;
-    st32->converged = 0;
-    st32->reason = 0;
-    st32->done = (r0 == 0.0) ? 1 : 0;
-    if (st32->hist && st32->hist_cap > 0) st32->hist[0] = r0;
-  }
-}
-
-// x64 += (double) d32, d32 = 0
-template <int V>
-__global__ __launch_bounds__(kBlock) void k_ir_update(Grid g, double *x, float *d) {
-  PSB_TILE_BEGIN(g, V)
-#pragma unroll
-  for (int u = 0; u < V; u++) {
-    x[P + u] = x[P + u] + (double)d[P + u];
-    d[P + u] = 0.0f;
-  }
-  PSB_TILE_END
-}
-
-// ---------------------------------------------------------------------------
-// BiCGS (bicgs.cpp:79-216) in three passes per iteration.
-// B1: p = r + beta (p_old - omega v_old) formed as the reference does
-//     (p = r + beta p; p -= (beta omega) v), p^ = p / c (Diagonal), v = A p^,
-//     alpha = rho / (r~.v).
-// ---------------------------------------------------------------------------
-template <typename T, int V>
-struct BiPX {
-  const T *r, *p, *v, *c;
-  T beta, bo;
-  __device__ __forceinline__ T pnew(T rr, T pp, T vv) const {
-    T t = rr + beta * pp;
-    return t - bo * vv;
-  }
-  __device__ __forceinline__ T one(int Q) const {
-    return pnew(r[Q], p[Q], v[Q]) * (T(1) / c[Q]);
-  }
-  __device__ __forceinline__ VecN<T, V> vec(int P) const {
-    VecN<T, V> R = ldv<T, V>(r + P), Pp = ldv<T, V>(p + P), Vv = ldv<T, V>(v + P),
-               C = ldv<T, V>(c + P), o;
-#pragma unroll
-    for (int u = 0; u < V; u++) o.v[u] = pnew(R.v[u], Pp.v[u], Vv.v[u]) * (T(1) / C.v[u]);
-    return o;
-  }
-  __device__ __forceinline__ T at(int Q) const { return one(Q); }
-};
-
-template <typename T, int V, bool SYM>
-__global__ __launch_bounds__(kBlock) void k_bi1(Grid g, DMat<T> A, const T *r, const T *rt,
-                                                const T *pold, const T *vold, T *pnew,
-                                                T *vnew, DevState *st, Work w) {
-  if (st->done) return;
-  const bool first = st->iter == 0;
-  const T beta = first ? T(0) : (T)st->beta;
-  const T bo = first ? T(0) : (T)(st->beta * st->omega);
-  BiPX<T, V> X{r, pold, vold, A.c, beta, bo};
-  double acc[1] = {0.0};
-  PSB_TILE_BEGIN(g, V)
-  Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
-  VecN<T, V> Vv = apply_A<T, V>(G), Rt = ldv<T, V>(rt + P), R = ldv<T, V>(r + P),
-             Pp = ldv<T, V>(pold + P), Vo = ldv<T, V>(vold + P), Pn;
-#pragma unroll
-  for (int u = 0; u < V; u++) {
-    Pn.v[u] = first ? R.v[u] : X.pnew(R.v[u], Pp.v[u], Vo.v[u]);
-    acc[0] += (double)(Rt.v[u] * Vv.v[u]);
-  }
-  stv<T, V>(pnew + P, Pn);
-  stv<T, V>(vnew + P, Vv);
-  PSB_TILE_END
-  double tot[1];
-  if (grid_sum<1>(acc, w.partial, w.counter, tot) && threadIdx.x == 0) {
-    st->rv = tot[0];
-    st->alpha = st->rho / tot[0];
-  }
-}
-
-// B2: s = r - alpha v, s^ = s / c, t = A s^; sums s.s, t.s, t.t; omega.
-template <typename T, int V>
-struct BiSX {
-  const T *r, *v, *c;
-  T alpha;
-  __device__ __forceinline__ T s1(T rr, T vv) const { return rr - alpha * vv; }
-  __device__ __forceinline__ VecN<T, V> vec(int P) const {
-    VecN<T, V> R = ldv<T, V>(r + P), Vv = ldv<T, V>(v + P), C = ldv<T, V>(c + P), o;
-#pragma unroll
-    for (int u = 0; u < V; u++) o.v[u] = s1(R.v[u], Vv.v[u]) * (T(1) / C.v[u]);
-    return o;
-  }
-  __device__ __forceinline__ T at(int Q) const { return s1(r[Q], v[Q]) * (T(1) / c[Q]); }
-};
-
-template <typename T, int V, bool SYM>
-__global__ __launch_bounds__(kBlock) void k_bi2(Grid g, DMat<T> A, const T *r, const T *v,
-                                                T *s, T *t, DevState *st, Work w) {
-  if (st->done) return;
-  BiSX<T, V> X{r, v, A.c, (T)st->alpha};
-  double acc[3] = {0.0, 0.0, 0.0};  // ss, ts, tt
-  PSB_TILE_BEGIN(g, V)
-  Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
-  VecN<T, V> Tt = apply_A<T, V>(G), R = ldv<T, V>(r + P), Vv = ldv<T, V>(v + P), S;
-#pragma unroll
-  for (int u = 0; u < V; u++) {
-    S.v[u] = X.s1(R.v[u], Vv.v[u]);
-    acc[0] += (double)(S.v[u] * S.v[u]);
-    acc[1] += (double)(Tt.v[u] * S.v[u]);
-    acc[2] += (double)(Tt.v[u] * Tt.v[u]);
-  }
-  stv<T, V>(s + P, S);
-  stv<T, V>(t + P, Tt);
-  PSB_TILE_END
-  double tot[3];
-  if (grid_sum<3>(acc, w.partial, w.counter, tot) && threadIdx.x == 0) {
-    st->rv_ss = tot[0];
-    st->rv_ts = tot[1];
-    st->rv_tt = tot[2];
-    st->omega = tot[1] / tot[2];
-  }
-}
-
-// B3: x += alpha p^; x += omega s^; r = s - omega t; res; rho_next = r~.r;
-// stop tests in the reference's order, including the small-s exit
-// (bicgs.cpp:134-138) where only x += alpha p^ is applied.
-template <typename T, int V>
-__global__ __launch_bounds__(kBlock) void k_bi3(Grid g, const T *c, Metric m, const T *p,
-                                                const T *s, const T *t, const T *rt, T *x,
-                                                T *r, DevState *st, Work w) {
-  if (st->done) return;
-  const bool small_s = sqrt(st->rv_ss) < st->res_tol;
-  const T alpha = (T)st->alpha, omega = (T)st->omega;
-  double acc[3] = {0.0, 0.0, 0.0};  // voldiv, r^2, rho_next
-  PSB_TILE_BEGIN(g, V)
-  VecN<T, V> X = ldv<T, V>(x + P), Pp = ldv<T, V>(p + P), C = ldv<T, V>(c + P);
-  if (small_s) {
-#pragma unroll
-    for (int u = 0; u < V; u++) X.v[u] = X.v[u] + alpha * (Pp.v[u] * (T(1) / C.v[u]));
-    stv<T, V>(x + P, X);
-  } else {
-    VecN<T, V> S = ldv<T, V>(s + P), Tt = ldv<T, V>(t + P), Rt = ldv<T, V>(rt + P), R;
-#pragma unroll
-    for (int u = 0; u < V; u++) {
-      T mc = T(1) / C.v[u];
-      X.v[u] = X.v[u] + alpha * (Pp.v[u] * mc);
-      X.v[u] = X.v[u] + omega * (S.v[u] * mc);
-      R.v[u] = S.v[u] - omega * Tt.v[u];
-      double rd = (double)R.v[u];
-      double dv = m.dV(i, j, k + u);
-      acc[0] += rd * rd / dv / dv;
-      acc[1] += rd * rd;
-      acc[2] += (double)(Rt.v[u] * R.v[u]);
-    }
-    stv<T, V>(x + P, X);
-    stv<T, V>(r + P, R);
-  }
-  PSB_TILE_END
-  double tot[3];
-  if (grid_sum<3>(acc, w.partial, w.counter, tot) && threadIdx.x == 0) {
-    if (small_s) {
-      st->done = 1;
-      st->converged = 0;
-      st->reason = 5;
-      st->report_iters = st->iter + 1;
-      return;
-    }
-    double res = st->norm == 0 ? ref_norm(st, tot[0]) : sqrt(tot[1]);
-    st->rsq = tot[1];
-    const bool small_om = st->omega >= -st->res_tol && st->omega <= st->res_tol;
-    krylov_stop(st, res, small_om);
-    if (!st->done) {
-      // top of the next iteration: rho = r~.r, breakdown test (bicgs.cpp:84-89)
-      double rho = tot[2];
-      if (sqrt(fabs(rho)) < st->res_tol) {
+  block_sum<NR>(acc, sm);
+  if (threadIdx.x != 0) return;
+  const double *tot = acc;
+  switch (op) {
+    case FIN_START_DIAG:
+    case FIN_START_AC:
+    case FIN_START_BICGS: {
+      const double res = st->norm == 0 ? ref_norm(st, tot[0]) : sqrt(tot[1]);
+      krylov_init(st, res, tot[1], tot[2]);
+      if (res < st->res_tol) {  // cg.cpp:64, bicgs.cpp:70 early return
         st->done = 1;
-        st->reason = 4;
-        st->report_iters = st->iter + 1;
-      } else {
-        st->beta = (rho / st->rho) * (st->alpha / st->omega);  // bicgs.cpp:102
-        st->rho = rho;
+        st->converged = 1;
+        st->reason = 7;
       }
+      if (!(res == res)) {
+        st->done = 1;
+        st->reason = 8;
+      }
+      if (op == FIN_START_BICGS && !st->done && sqrt(fabs(tot[2])) < st->res_tol) {
+        st->done = 1;  // bicgs.cpp:89 at i = 0
+        st->reason = 4;
+        st->report_iters = 1;
+      }
+      break;
     }
-  }
-}
-
-// true relative residual ||b - A x|| / ||b|| (FP64 diagnostic after a solve)
-template <int V, bool SYM>
-__global__ __launch_bounds__(kBlock) void k_relres(Grid g, DMat<double> A, const double *b,
-                                                   const double *x, double *out, Work w) {
-  ArrX<double, V> X{x};
-  double acc[2] = {0.0, 0.0};
-  PSB_TILE_BEGIN(g, V)
-  Gath<double, V> G = gather<double, V, SYM>(g, A, X, P, i, j, k);
-  VecN<double, V> B = ldv<double, V>(b + P), R = resid<double, V>(B, G);
-#pragma unroll
-  for (int u = 0; u < V; u++) {
-    acc[0] += R.v[u] * R.v[u];
-    acc[1] += B.v[u] * B.v[u];
-  }
-  PSB_TILE_END
-  double tot[2];
-  if (grid_sum<2>(acc, w.partial, w.counter, tot) && threadIdx.x == 0) {
-    out[0] = tot[0];
-    out[1] = tot[1];
+    case FIN_PQ:
+      st->pq = tot[0];
+      st->alpha = st->rho / tot[0];  // cg.cpp:107-108
+      break;
+    case FIN_XR_DIAG:
+    case FIN_XR_AC: {
+      const double res = st->norm == 0 ? ref_norm(st, tot[0]) : sqrt(tot[1]);
+      st->rsq = tot[1];
+      krylov_stop(st, res, false);
+      if (op == FIN_XR_DIAG && !st->done) {
+        st->beta = tot[2] / st->rho;  // cg.cpp:93
+        st->rho = tot[2];
+      }
+      break;
+    }
+    case FIN_RHO: {
+      const double m = tot[3] > 0.0 ? tot[1] / tot[3] : 0.0;
+      const double rho = tot[0] - m * tot[2];
+      st->zmean = m;
+      st->beta = st->iter == 0 ? 0.0 : rho / st->rho;
+      st->rho = rho;
+      break;
+    }
+    case FIN_IR_RESID:  // st: FP64 outer state, st2: FP32 inner state
+      st->rsq = tot[0];
+      st->res = st->norm == 0 ? ref_norm(st, tot[1]) : sqrt(tot[0]);
+      st2->mean[0] = tot[3] > 0.0 ? tot[2] / tot[3] : 0.0;
+      break;
+    case FIN_CENTER: {
+      krylov_init(st, sqrt(tot[0]), tot[0], 0.0);
+      if (tot[0] == 0.0) st->done = 1;
+      break;
+    }
+    case FIN_RELRES:
+      out[0] = tot[0];
+      out[1] = tot[1];
+      break;
+    case FIN_BI1:
+      st->rv = tot[0];
+      st->alpha = st->rho / tot[0];  // bicgs.cpp:122-123
+      break;
+    case FIN_BI2:
+      st->rv_ss = tot[0];
+      st->rv_ts = tot[1];
+      st->rv_tt = tot[2];
+      st->omega = tot[1] / tot[2];  // bicgs.cpp:155-157
+      break;
+    case FIN_BI3: {
+      if (sqrt(st->rv_ss) < st->res_tol) {  // bicgs.cpp:134-138
+        st->done = 1;
+        st->converged = 0;
+        st->reason = 5;
+        st->report_iters = st->iter + 1;
+        break;
+      }
+      const double res = st->norm == 0 ? ref_norm(st, tot[0]) : sqrt(tot[1]);
+      st->rsq = tot[1];
+      const bool small_om = st->omega >= -st->res_tol && st->omega <= st->res_tol;
+      krylov_stop(st, res, small_om);
+      if (!st->done) {
+        // top of the next iteration: rho = r~.r, breakdown test (bicgs.cpp:84-89)
+        const double rho = tot[2];
+        if (sqrt(fabs(rho)) < st->res_tol) {
+          st->done = 1;
+          st->reason = 4;
+          st->report_iters = st->iter + 1;
+        } else {
+          st->beta = (rho / st->rho) * (st->alpha / st->omega);  // bicgs.cpp:102
+          st->rho = rho;
+        }
+      }
+      break;
+    }
   }
 }
 
@@ -640,20 +700,27 @@ __global__ void k_to_f32(const double *a, float *b, int n) {
 __global__ void k_galerkin(Grid g, DMat<double> A, Grid gH, int CI, int CJ, int CK,
                            double *cH, double *wH, double *sH, double *bH) {
   for (int Q = blockIdx.x * blockDim.x + threadIdx.x; Q < gH.N; Q += gridDim.x * blockDim.x) {
-    const int I = Q / gH.sI, remQ = Q - I * gH.sI, J = remQ / gH.nz, K = remQ - J * gH.nz;
+    int I, J, K;
+    gH.ijk(Q, I, J, K);
     auto F = [&](int a, int b, int c) {
       return ((I * CI + a) * g.ny + (J * CJ + b)) * g.nz + (K * CK + c);
     };
-    auto e_ = [&](int P) { return (P / g.sI) < g.nx - 1 ? A.w[P + g.sI] : 0.0; };
-    auto n_ = [&](int P) { return ((P % g.sI) / g.nz) < g.ny - 1 ? A.s[P + g.nz] : 0.0; };
-    auto t_ = [&](int P) { return (P % g.nz) < g.nz - 1 ? A.b[P + 1] : 0.0; };
+    auto e_ = [&](int a, int b, int c) {
+      return I * CI + a < g.nx - 1 ? A.w[F(a, b, c) + g.sI] : 0.0;
+    };
+    auto n_ = [&](int a, int b, int c) {
+      return J * CJ + b < g.ny - 1 ? A.s[F(a, b, c) + g.nz] : 0.0;
+    };
+    auto t_ = [&](int a, int b, int c) {
+      return K * CK + c < g.nz - 1 ? A.b[F(a, b, c) + 1] : 0.0;
+    };
     double c = 0.0;
     for (int a = 0; a < CI; a++)
       for (int b = 0; b < CJ; b++)
         for (int cc = 0; cc < CK; cc++) c += A.c[F(a, b, cc)];
     for (int b = 0; b < CJ; b++)
       for (int cc = 0; cc < CK; cc++) {
-        for (int a = 0; a < CI - 1; a++) c -= e_(F(a, b, cc));
+        for (int a = 0; a < CI - 1; a++) c -= e_(a, b, cc);
         for (int a = 1; a < CI; a++) c -= A.w[F(a, b, cc)];
       }
     double w = 0.0;
@@ -661,7 +728,7 @@ __global__ void k_galerkin(Grid g, DMat<double> A, Grid gH, int CI, int CJ, int 
       for (int cc = 0; cc < CK; cc++) w += A.w[F(0, b, cc)];
     for (int a = 0; a < CI; a++)
       for (int cc = 0; cc < CK; cc++) {
-        for (int b = 0; b < CJ - 1; b++) c -= n_(F(a, b, cc));
+        for (int b = 0; b < CJ - 1; b++) c -= n_(a, b, cc);
         for (int b = 1; b < CJ; b++) c -= A.s[F(a, b, cc)];
       }
     double s = 0.0;
@@ -669,7 +736,7 @@ __global__ void k_galerkin(Grid g, DMat<double> A, Grid gH, int CI, int CJ, int 
       for (int cc = 0; cc < CK; cc++) s += A.s[F(a, 0, cc)];
     for (int a = 0; a < CI; a++)
       for (int b = 0; b < CJ; b++) {
-        for (int cc = 0; cc < CK - 1; cc++) c -= t_(F(a, b, cc));
+        for (int cc = 0; cc < CK - 1; cc++) c -= t_(a, b, cc);
         for (int cc = 1; cc < CK; cc++) c -= A.b[F(a, b, cc)];
       }
     double bb = 0.0;
@@ -686,7 +753,8 @@ __global__ void k_galerkin(Grid g, DMat<double> A, Grid gH, int CI, int CJ, int 
 __global__ void k_coarsen_flag(Grid g, const uint8_t *f, Grid gH, int CI, int CJ, int CK,
                                uint8_t *fH) {
   for (int Q = blockIdx.x * blockDim.x + threadIdx.x; Q < gH.N; Q += gridDim.x * blockDim.x) {
-    const int I = Q / gH.sI, remQ = Q - I * gH.sI, J = remQ / gH.nz, K = remQ - J * gH.nz;
+    int I, J, K;
+    gH.ijk(Q, I, J, K);
     int fl = 0;
     for (int a = 0; a < CI; a++)
       for (int b = 0; b < CJ; b++)

@@ -44,7 +44,8 @@ class Stop(C.Structure):
                 ("precform", C.c_int), ("method", C.c_int), ("precond", C.c_int),
                 ("precision", C.c_int), ("norm", C.c_int), ("ac_pre", C.c_int),
                 ("ac_post", C.c_int), ("ac_omega", C.c_double), ("ac_cfac", C.c_double),
-                ("inner_rtol", C.c_double), ("max_outer", C.c_int), ("graph_chunk", C.c_int)]
+                ("inner_rtol", C.c_double), ("max_outer", C.c_int), ("graph_chunk", C.c_int),
+                ("inner_adapt", C.c_int)]
 
 
 class Result(C.Structure):

