@@ -49,6 +49,17 @@ def timed_solves(s, dx, db, steps, warmup, **stop):
     return (t1 - t0) / steps, res
 
 
+def measured_traffic(n):
+    """HBM bytes per launch of the FP32 Jacobi sweep at 256^3 from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a
+    plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", "jacobi_spmv_256_hbm_traffic.md")
+    if n != 256 or not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch k_jacobi<float,4> (\d+)", open(path).read())
+    return int(m.group(1)) if m else None
+
+
 def cpu_baseline(n, iters_full, sample_iters):
     """Reference CG with diagonal preconditioning (cg.cpp, built from the
     reference sources into oracle/_ref) on one host core: a bounded sample of
@@ -81,11 +92,14 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="only launch the roofline kernels (FP32 Jacobi sweep, FP32/FP64 SpMV) "
+                         "and a plain copy, for hardware-counter runs")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
-              "(the path would shard by 3-D blocks with halo exchange; see the design notes)",
-              file=sys.stderr)
+              "(the solver has no multi-GPU path: it would shard by 3-D blocks with a "
+              "one-cell halo exchange and all-reduced scalars)", file=sys.stderr)
         sys.exit(2)
     n = a.n
     log(f"[bench] generating {n}^3 case")
@@ -93,6 +107,12 @@ def main():
     s = psb.Solver(n, n, n)
     s.set_matrix(d)
     s.set_geometry(d["dxc"], d["dyc"], d["dzc"])
+    if a.profile_kernels:
+        for kern, name in ((2, "jacobi_fp32"), (1, "spmv_fp32"), (0, "spmv_fp64"), (3, "copy_fp32")):
+            ms, by = s.time_kernel(kern, reps=10)
+            log(f"[bench] {name}: {ms * 1e3:.1f} us/launch, {by / 1e6:.1f} MB algorithmic, "
+                f"{by / ms / 1e6:.0f} GB/s")
+        return
     db = psb.DeviceArray.from_host(d["rhs"])
     dx = psb.DeviceArray(n ** 3)
     common = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=1e-8, res_tol=0.0,
@@ -102,7 +122,7 @@ def main():
     log(f"[bench] fp64 {t64 * 1e3:.2f} ms/solve {r64}")
     log("[bench] mixed AC-PCG + IR")
     tmx, rmx = timed_solves(s, dx, db, a.steps, a.warmup, precision=psb.PSB_MIXED,
-                            inner_rtol=1e-4, **common)
+                            inner_rtol=1e-6, inner_adapt=1, **common)
     log(f"[bench] mixed {tmx * 1e3:.2f} ms/solve {rmx}")
     x_mixed = dx.download()
     # FP64 reference solution to 1e-12 for the error figure (untimed)
@@ -118,7 +138,7 @@ def main():
     gbs = lambda by, ms: by / (ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": "k_jacobi<float,4> (fine-level damped Jacobi sweep)",
             "achieved": round(gbs(by_j, ms_j), 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs(by_j, ms_j) / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(gbs(by_j, ms_j) / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n),
             "avg_launch_ms": ms_j, "bytes_per_launch": by_j,
             "spmv_fp32_GBs": round(gbs(by_s32, ms_s32), 1),
             "spmv_fp64_GBs": round(gbs(by_s64, ms_s64), 1)}

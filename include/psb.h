@@ -126,7 +126,8 @@ int psb_set_geometry(psb_ctx *ctx, const double *dxc, const double *dyc,
 int psb_set_matrix(psb_ctx *ctx, const psb_matrix7 *A);
 
 /* Off cells (1 = solid, Body::off_p) for the AC hierarchy's active flag
- * (Centered::set_active_flag, additive_coarsen_flag.cpp:4-41).  Optional.   */
+ * (Centered::set_active_flag, additive_coarsen_flag.cpp:4-41).  Optional;
+ * takes effect at the next psb_set_matrix.                                  */
 int psb_set_active(psb_ctx *ctx, const unsigned char *off);
 
 /* Linear::solve (linear.h:46-56): x (in: initial guess, out: solution) and b
@@ -175,7 +176,8 @@ int psb_generate(const psb_case *cs, double *c, double *w, double *e,
 
 /* Benchmark hook: average device time (ms) of `reps` back-to-back launches
  * of one kernel on the context's stream, bracketed by HIP events.  kernel:
- * 0 FP64 SpMV, 1 FP32 SpMV, 2 FP32 damped Jacobi sweep.
+ * 0 FP64 SpMV, 1 FP32 SpMV, 2 FP32 damped Jacobi sweep, 3 plain FP32 copy
+ * of the grid (reference stream for hardware byte counters).
  * *bytes receives the algorithmic bytes one launch moves.                   */
 int psb_time_kernel(psb_ctx *ctx, int kernel, int reps, double *avg_ms,
                     double *bytes);

@@ -1146,7 +1146,7 @@ int psb_set_matrix(psb_ctx *C, const psb_matrix7 *A) {
 
 int psb_set_active(psb_ctx *C, const unsigned char *off) {
   if (!C) return PSB_EINVAL;
-  if (C->have_matrix) return PSB_EINVAL;  // call before psb_set_matrix
+  // takes effect at the next psb_set_matrix, which rebuilds the hierarchy
   if (!off) {
     C->off.clear();
   } else {
@@ -1262,6 +1262,16 @@ int psb_time_kernel(psb_ctx *C, int kernel, int reps, double *avg_ms, double *by
         jacobi<float, false>(C, lv, lv.e[0], lv.e[1], nullptr, nullptr, 0.0f, 0.9f, C->st32);
       };
       *bytes = N * (4 * 4 + 3 * 4);  // c, w, s, b, e, r in; e out
+    } else if (kernel == 3) {
+      ensure32(C);
+      ensure_krylov(C, C->k32, false);
+      float *x = C->k32.p[0], *y = C->k32.q;
+      zero(C, x, g.N);
+      const int n = g.N;
+      f = [=]() {
+        hipLaunchKernelGGL(k_copy, dim3(grid_blocks(n / 4)), dim3(kBlock), 0, stream, x, y, n);
+      };
+      *bytes = N * (4 + 4);
     } else {
       return PSB_EINVAL;
     }

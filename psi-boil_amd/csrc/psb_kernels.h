@@ -687,6 +687,16 @@ __global__ void k_fill(T *x, T v, int n) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) x[q] = v;
 }
 
+// plain copy of n floats (n % 4 == 0 for the vector part): the reference
+// stream for calibrating hardware byte counters
+__global__ __launch_bounds__(kBlock) void k_copy(const float *x, float *y, int n) {
+  const int nv = n / 4;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nv; q += gridDim.x * blockDim.x)
+    stv<float, 4>(y + 4 * q, ldv<float, 4>(x + 4 * q));
+  for (int q = 4 * nv + blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+    y[q] = x[q];
+}
+
 __global__ void k_to_f32(const double *a, float *b, int n) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
     b[q] = (float)a[q];
