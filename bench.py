@@ -120,9 +120,9 @@ def main():
     log("[bench] FP64 AC-PCG")
     t64, r64 = timed_solves(s, dx, db, a.steps, a.warmup, precision=psb.PSB_FP64, **common)
     log(f"[bench] fp64 {t64 * 1e3:.2f} ms/solve {r64}")
-    log("[bench] mixed AC-PCG + IR")
+    log("[bench] mixed AC-PCG, FP32 iterations with FP64 reliable updates")
     tmx, rmx = timed_solves(s, dx, db, a.steps, a.warmup, precision=psb.PSB_MIXED,
-                            inner_rtol=1e-6, inner_adapt=1, **common)
+                            inner_rtol=1e-2, inner_adapt=2, **common)
     log(f"[bench] mixed {tmx * 1e3:.2f} ms/solve {rmx}")
     x_mixed = dx.download()
     # FP64 reference solution to 1e-12 for the error figure (untimed)
@@ -162,17 +162,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 inner / fp64 refinement",
+        "dtype": "fp32 iterations / fp64 residual updates",
         "data": "synthetic (seeded bubbles, seeded face velocities)",
         "config": {"workload": f"pressure Poisson {n}^3, 16 bubbles r 0.03-0.12, density "
                                "ratio 1000, Neumann walls; AC-preconditioned CG (2+2 damped "
-                               "Jacobi, 2x coarsening to 8^3, exact FP64 coarsest) to 1e-8",
+                               "Jacobi, 2x coarsening to 8^3, exact FP64 coarsest) to 1e-8; "
+                               "FP32 CG with an FP64 true-residual update per 1e-2 drop",
                    "grid": [n, n, n], "parallelism": "single GPU"},
         "fp64_solves_per_s": round(1.0 / t64, 3),
         "fp64_ms_per_solve": round(t64 * 1e3, 3),
         "mixed_over_fp64": round(t64 / tmx, 3),
-        "iterations": {"fp64": r64["iterations"], "mixed_inner": rmx["iterations"],
-                       "mixed_outer": rmx["outer"]},
+        "iterations": {"fp64": r64["iterations"], "mixed": rmx["iterations"],
+                       "mixed_fp64_updates": rmx["outer"]},
         "rel_residual": {"fp64": r64["rel_l2"], "mixed": rmx["rel_l2"]},
         "error_vs_fp64_1e-12": err,
         "roofline": roof,

@@ -87,17 +87,23 @@ typedef struct {
   double ac_omega;       /* Jacobi damping (0.9)                            */
   double ac_cfac;        /* coarse-correction factor (2.0)                  */
   /* iterative refinement (PSB_MIXED)                                       */
-  double inner_rtol;     /* inner FP32 solve: relative L2 reduction (1e-4)   */
+  double inner_rtol;     /* inner FP32 solve: relative L2 reduction (1e-4);
+                            with inner_adapt = 2 the drop between updates   */
   int max_outer;         /* refinement steps (30)                            */
   int graph_chunk;       /* iterations per captured graph launch (8)         */
-  int inner_adapt;       /* 1: each inner solve aims at the outer target,
-                            max(inner_rtol, 0.5 target / current residual);
-                            0: fixed inner_rtol (default)                     */
+  int inner_adapt;       /* 0: restarted refinement, fixed inner_rtol (default);
+                            1: restarted, each inner solve aims at the outer
+                            target, max(inner_rtol, 0.5 target / current);
+                            2: one FP32 CG, never restarted, with reliable
+                            updates: whenever its recursive residual falls
+                            by inner_rtol below the last true one, x += d in
+                            FP64 and r = b - A x (FP64) replaces it; the stop
+                            tests use that true residual in the chosen norm */
 } psb_stop;
 
 typedef struct {
   int iterations;    /* reference count i+1 (cg.cpp:178); inner total (IR)   */
-  int outer;         /* refinement steps (PSB_MIXED), else 0                */
+  int outer;         /* refinement steps / FP64 updates (PSB_MIXED), else 0 */
   int converged;     /* solve()'s return value                              */
   int status;        /* psb_status                                          */
   double res0, res;  /* initial / final residual in the chosen norm         */
@@ -146,7 +152,8 @@ int psb_apply(psb_ctx *ctx, const double *x, const double *b, double *y);
 
 /* Residual history of the last solve in its norm: hist[0] = res0, hist[i] =
  * res after iteration i; for PSB_MIXED the FP64 residual before each
- * refinement step.  Returns the count.                                      */
+ * refinement step (after each update with inner_adapt = 2).  Returns the
+ * count.                                                                    */
 int psb_history(psb_ctx *ctx, double *hist, int cap);
 
 /* Synthetic cases of the benchmark: variable-density pressure matrix on a

@@ -117,6 +117,7 @@ struct Graph {
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   long long key = -1;
+  const void *px = nullptr, *pb = nullptr;  // caller arrays baked into the graph
 };
 
 }  // namespace
@@ -152,7 +153,7 @@ struct psb_ctx {
   double *db = nullptr, *dx = nullptr;  // staging for host-array solves
   double *dy = nullptr;                 // scratch for psb_apply
   double *relres = nullptr;             // 2 doubles: sum r^2, sum b^2
-  Graph gr_cgdiag, gr_bicgs, gr_ac64, gr_ac32;
+  Graph gr_cgdiag, gr_bicgs, gr_ac64, gr_ac32, gr_ru;
   double ac_omega = 0, ac_cfac = 0;
   std::string err;
 };
@@ -238,13 +239,13 @@ void cg_pq(psb_ctx *C, const Grid &g, DMat<T> A, const T *rz, const T *pold, T *
 
 template <typename T, bool DIAG>
 void cg_xr(psb_ctx *C, const Grid &g, const T *c, const T *p, const T *q, T *x, T *r, T *e1,
-           T omega, DevState *st) {
+           T omega, DevState *st, int op = -1) {
   with_v<T>(g, [&](auto vt) {
     constexpr int V = decltype(vt)::value;
     hipLaunchKernelGGL((k_cg_xr<T, V, DIAG>), dim3(nblocks<T>(g)), dim3(kBlock), 0, C->stream, g,
                        c, metric(C), p, q, x, r, e1, omega, st, C->work);
   });
-  finalize<3>(C, DIAG ? FIN_XR_DIAG : FIN_XR_AC, nblocks<T>(g), st);
+  finalize<3>(C, op >= 0 ? op : DIAG ? FIN_XR_DIAG : FIN_XR_AC, nblocks<T>(g), st);
 }
 
 // one damped Jacobi sweep on level lv (optionally with the prolongation of
@@ -592,12 +593,12 @@ T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega
 // one preconditioned CG iteration: V-cycle (with rho), p/q step, x/r step
 template <typename T>
 void ac_iteration(psb_ctx *C, Hier<T> &H, KrylovVecs<T> &K, DevState *st, int parity, int nu1,
-                  int nu2, T omega, T cfac) {
+                  int nu2, T omega, T cfac, int xr_op = -1) {
   Level<T> &f = H.L[0];
   T *z = vcycle<T>(C, H, 0, st, nu1, nu2, omega, cfac);
   T *pold = K.p[parity], *pnew = K.p[parity ^ 1];
   cg_pq<T, true, false>(C, f.g, f.mat(), z, pold, pnew, K.q, f.aflag, st);
-  cg_xr<T, false>(C, f.g, f.c, pnew, K.q, K.x, K.r, f.e[0], omega, st);
+  cg_xr<T, false>(C, f.g, f.c, pnew, K.q, K.x, K.r, f.e[0], omega, st, xr_op);
 }
 
 void destroy_graph(Graph &gr) {
@@ -606,6 +607,17 @@ void destroy_graph(Graph &gr) {
   gr.exec = nullptr;
   gr.graph = nullptr;
   gr.key = -1;
+  gr.px = gr.pb = nullptr;
+}
+
+// every captured graph holds device pointers of the matrix, hierarchy and
+// geometry: any change to those drops them
+void destroy_graphs(psb_ctx *C) {
+  destroy_graph(C->gr_cgdiag);
+  destroy_graph(C->gr_bicgs);
+  destroy_graph(C->gr_ac64);
+  destroy_graph(C->gr_ac32);
+  destroy_graph(C->gr_ru);
 }
 
 template <typename F>
@@ -905,11 +917,95 @@ void solve_ir(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_res
   R->status = status;
 }
 
+// ---------------------------------------------------------------------------
+// mixed precision with reliable updates: one FP32 AC-PCG run that is never
+// restarted.  Its iterate d accumulates the correction since the last
+// refresh; whenever its recursive residual has fallen by inner_rtol below the
+// last true residual (or reaches the target), x += d in FP64 and the true
+// residual b - A x, rounded to FP32, replaces the recursive one.  The search
+// direction and rho carry over, so CG keeps its Krylov space instead of
+// rebuilding it after every refinement step.
+// (H. A. van der Vorst and Q. Ye, "Residual replacement strategies for
+// Krylov subspace iterative methods", SIAM J. Sci. Comput. 22 (2000).)
+// ---------------------------------------------------------------------------
+void ru_resid(psb_ctx *C, const double *b, const double *x, bool gated, int op) {
+  const Grid &g = C->g;
+  Level<float> &f = C->h32.L[0];
+  KrylovVecs<float> &K = C->k32;
+  const float omega = (float)C->ac_omega;
+  with_v<double>(g, [&](auto vt) {
+    constexpr int V = decltype(vt)::value;
+    hipLaunchKernelGGL((k_ru_resid<V, true>), dim3(nblocks<double>(g)), dim3(kBlock), 0,
+                       C->stream, g, mat64(C), metric(C), b, x, f.c, K.r, f.e[0], omega, C->st32,
+                       gated, C->work);
+  });
+  finalize<3>(C, op, nblocks<double>(g), C->st32);
+}
+
+void ru_xupdate(psb_ctx *C, double *x, bool gated) {
+  const Grid &g = C->g;
+  with_v<double>(g, [&](auto vt) {
+    constexpr int V = decltype(vt)::value;
+    hipLaunchKernelGGL((k_ru_xupdate<V>), dim3(nblocks<double>(g)), dim3(kBlock), 0, C->stream,
+                       g, x, C->k32.x, C->st32, gated);
+  });
+}
+
+void solve_ru(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_result *R) {
+  KrylovVecs<float> &K = C->k32;
+  ensure_krylov(C, K, false);
+  build_hier32(C);
+  ensure_hist(C, p->max_iter);
+  const Grid &g = C->g;
+  const int chunk = chunk_of(p);
+  const int nu1 = p->ac_pre, nu2 = p->ac_post;
+  const float omega = (float)p->ac_omega, cfac = (float)p->ac_cfac;
+  const long long key = (long long)chunk + ((long long)nu1 << 16) + ((long long)nu2 << 24);
+  Graph &gr = C->gr_ru;
+  if (gr.key != key || gr.px != x || gr.pb != b || C->ac_omega != p->ac_omega ||
+      C->ac_cfac != p->ac_cfac)
+    destroy_graph(gr);
+  C->ac_omega = p->ac_omega;
+  C->ac_cfac = p->ac_cfac;
+  init_state(C, C->st32, p, p->norm, p->res_rat, p->res_tol, p->max_iter);
+  HIPCHK(hipMemcpyAsync(&C->st32->delta, &p->inner_rtol, sizeof(double), hipMemcpyHostToDevice,
+                        C->stream));
+  zero(C, K.x, g.N);
+  zero(C, K.p[0], g.N);
+  zero(C, K.p[1], g.N);
+  ru_resid(C, b, x, false, FIN_RU_START);
+  HIPCHK(hipGetLastError());
+  read_state(C, C->st32);
+  auto body = [&]() {
+    for (int it = 0; it < chunk; it++) {
+      ac_iteration<float>(C, C->h32, K, C->st32, it & 1, nu1, nu2, omega, cfac, FIN_XR_RU);
+      ru_xupdate(C, x, true);
+      ru_resid(C, b, x, true, FIN_RU);
+    }
+  };
+  run_graph(C, gr, key, C->st32, p->max_iter, body);
+  gr.px = x;
+  gr.pb = b;
+  // fold in what the last FP32 iterations added since the last refresh
+  ru_xupdate(C, x, false);
+  DevState h = *C->hst;
+  if (!h.converged && h.reason != 8) {  // report the final true residual
+    ru_resid(C, b, x, false, FIN_RU_END);
+    read_state(C, C->st32);
+    h.res_true = C->hst->res_true;
+  }
+  finish_result(R, h, h.outer);
+  R->res = h.res_true;
+  C->hist_count = std::min(h.outer + 1, C->hist_cap);
+}
+
 int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_result *R) {
   C->hist_on_host = false;
   if (p->precond == PSB_PREC_AC) {
     if (p->method != PSB_CG || !C->sym || p->ac_pre < 1 || p->ac_post < 1) return PSB_EINVAL;
-    if (p->precision == PSB_MIXED)
+    if (p->precision == PSB_MIXED && p->inner_adapt == 2)
+      solve_ru(C, x, b, p, R);
+    else if (p->precision == PSB_MIXED)
       solve_ir(C, x, b, p, R);
     else
       solve_ac64(C, x, b, p, R);
@@ -1032,7 +1128,7 @@ int psb_create(psb_ctx **out, const psb_layout *L) {
     C->st32 = C->base.alloc<DevState>(1);
     HIPCHK(hipHostMalloc((void **)&C->hst, sizeof(DevState), hipHostMallocDefault));
     std::memset(C->hst, 0, sizeof(DevState));
-    C->work.partial = C->base.alloc<double>((size_t)kMaxBlocks * 4);
+    C->work.partial = C->base.alloc<double>((size_t)kMaxBlocks * kMaxPartials);
     C->relres = C->base.alloc<double>(2);
     const int N = C->g.N;
     C->dx = C->base.alloc<double>(N);
@@ -1051,10 +1147,7 @@ int psb_create(psb_ctx **out, const psb_layout *L) {
 int psb_destroy(psb_ctx *C) {
   if (!C) return PSB_OK;
   if (C->stream) (void)hipStreamSynchronize(C->stream);
-  destroy_graph(C->gr_cgdiag);
-  destroy_graph(C->gr_bicgs);
-  destroy_graph(C->gr_ac64);
-  destroy_graph(C->gr_ac32);
+  destroy_graphs(C);
   C->arena.release();
   C->base.release();
   if (C->hst) (void)hipHostFree(C->hst);
@@ -1068,6 +1161,8 @@ int psb_set_geometry(psb_ctx *C, const double *dxc, const double *dyc, const dou
   if ((dxc == nullptr) != (dyc == nullptr) || (dxc == nullptr) != (dzc == nullptr))
     return PSB_EINVAL;
   return guarded(C, [&]() {
+    HIPCHK(hipStreamSynchronize(C->stream));
+    destroy_graphs(C);
     if (!dxc) {
       C->dxc = C->dyc = C->dzc = nullptr;
       return PSB_OK;
@@ -1112,10 +1207,7 @@ int psb_set_matrix(psb_ctx *C, const psb_matrix7 *A) {
     const int nstore = sym ? 4 : 7;
     // (re)allocate: a new matrix invalidates the hierarchy and graphs
     HIPCHK(hipStreamSynchronize(C->stream));
-    destroy_graph(C->gr_cgdiag);
-    destroy_graph(C->gr_bicgs);
-    destroy_graph(C->gr_ac64);
-    destroy_graph(C->gr_ac32);
+    destroy_graphs(C);
     C->arena.release();
     for (int q = 0; q < 7; q++) C->A64[q] = nullptr;
     C->A32[0] = C->A32[1] = C->A32[2] = C->A32[3] = nullptr;

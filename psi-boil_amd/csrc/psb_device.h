@@ -15,6 +15,7 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kMaxLevels = 24;
 constexpr int kStaleMax = 64;
 constexpr int kMaxBlocks = 2048; // largest reduction grid (partials buffer)
+constexpr int kMaxPartials = 8;  // partial sums per block
 
 // n / d for 0 <= n < 2^31 with a precomputed multiplier: m = ceil(2^(31+s)/d),
 // s = ceil(log2 d); floor(n m / 2^(31+s)) = floor(n / d) because the rounding
@@ -87,6 +88,10 @@ struct DevState {
   double mean[kMaxLevels];                     // mean of the inner right-hand side
   double rv_ss, rv_ts, rv_tt;                  // BiCGS dots
   double zmean;                                // mean of the preconditioned residual
+  // reliable updates (FP32 CG inside FP64 refinement without restarts)
+  int update, outer;                           // refresh pending; refreshes done
+  double rlast, delta;                         // true residual at the last refresh
+  double res_true;                             // last true (FP64) residual
 };
 
 // ---------------------------------------------------------------------------

@@ -151,10 +151,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_xr(Grid g, const T *c, Metric m, 
     X.v[u] = X.v[u] + alpha * Pp.v[u];
     R.v[u] = R.v[u] - alpha * Q.v[u];
     double rd = (double)R.v[u];
+    double dv = m.dV(i, j, k + u);
+    acc[0] += rd * rd / dv / dv;
     acc[1] += rd * rd;
     if (DIAG) {
-      double dv = m.dV(i, j, k + u);
-      acc[0] += rd * rd / dv / dv;
       T z = R.v[u] * (T(1) / C.v[u]);
       acc[2] += (double)(R.v[u] * z);
     } else {
@@ -335,6 +335,57 @@ __global__ __launch_bounds__(kBlock) void k_center(Grid g, const float *c, const
   block_partial<1>(acc, w.partial);
 }
 
+// ---------------------------------------------------------------------------
+// reliable updates: when the FP32 CG's recursive residual has dropped by
+// delta since the last refresh (st->update set by k_finalize), the
+// accumulated FP32 correction moves into the FP64 solution and the true
+// residual b - A x replaces the recursive one; CG then continues with its
+// search direction (no restart).  Both kernels do nothing unless a refresh
+// is pending (gated = true) or they start the solve (gated = false).
+// ---------------------------------------------------------------------------
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_ru_xupdate(Grid g, double *x, float *d,
+                                                       const DevState *st, bool gated) {
+  if (gated && !st->update) return;
+  PSB_TILE_BEGIN(g, V)
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    x[P + u] = x[P + u] + (double)d[P + u];
+    d[P + u] = 0.0f;
+  }
+  PSB_TILE_END
+}
+
+// r32 = (float)(b - A x) in FP64, e1 = omega r32 / c32; partials of the FP64
+// norms (L2 and reference) and of the FP32 residual
+template <int V, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_ru_resid(Grid g, DMat<double> A, Metric m,
+                                                     const double *b, const double *x,
+                                                     const float *c32, float *r32, float *e1,
+                                                     float omega, const DevState *st,
+                                                     bool gated, Work w) {
+  if (gated && !st->update) return;
+  ArrX<double, V> X{x};
+  double acc[3] = {0.0, 0.0, 0.0};  // r64^2, r64^2/dV^2, r32^2
+  PSB_TILE_BEGIN(g, V)
+  Gath<double, V> G = gather<double, V, SYM>(g, A, X, P, i, j, k);
+  VecN<double, V> R = resid<double, V>(ldv<double, V>(b + P), G);
+  VecN<float, V> Cf = ldv<float, V>(c32 + P), Rf, E;
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    Rf.v[u] = (float)R.v[u];
+    E.v[u] = omega * Rf.v[u] / Cf.v[u];
+    double dv = m.dV(i, j, k + u);
+    acc[0] += R.v[u] * R.v[u];
+    acc[1] += R.v[u] * R.v[u] / dv / dv;
+    acc[2] += (double)Rf.v[u] * (double)Rf.v[u];
+  }
+  stv<float, V>(r32 + P, Rf);
+  stv<float, V>(e1 + P, E);
+  PSB_TILE_END
+  block_partial<3>(acc, w.partial);
+}
+
 // x64 += (double) d32, d32 = 0
 template <int V>
 __global__ __launch_bounds__(kBlock) void k_ir_update(Grid g, double *x, float *d) {
@@ -498,7 +549,8 @@ __global__ __launch_bounds__(kBlock) void k_bi3(Grid g, const T *c, Metric m, co
 // ---------------------------------------------------------------------------
 enum FinOp {
   FIN_START_DIAG, FIN_START_AC, FIN_START_BICGS, FIN_PQ, FIN_XR_DIAG, FIN_XR_AC, FIN_RHO,
-  FIN_IR_RESID, FIN_CENTER, FIN_RELRES, FIN_BI1, FIN_BI2, FIN_BI3
+  FIN_XR_RU, FIN_IR_RESID, FIN_CENTER, FIN_RELRES, FIN_BI1, FIN_BI2, FIN_BI3, FIN_RU_START,
+  FIN_RU, FIN_RU_END
 };
 
 // stop logic shared by the Krylov loops (cg.cpp:121-160, bicgs.cpp:174-210),
@@ -568,8 +620,17 @@ __device__ inline void krylov_init(DevState *st, double res, double rsq, double 
 template <int NR>
 __global__ __launch_bounds__(kBlock) void k_finalize(int op, int nblk, const double *partial,
                                                      DevState *st, DevState *st2, double *out) {
-  if (op >= FIN_PQ && op <= FIN_RHO && st->done) return;
-  if (op >= FIN_BI1 && st->done) return;
+  switch (op) {  // steps of a finished loop (or a refresh not pending) do nothing
+    case FIN_PQ: case FIN_XR_DIAG: case FIN_XR_AC: case FIN_RHO: case FIN_XR_RU:
+    case FIN_BI1: case FIN_BI2: case FIN_BI3:
+      if (st->done) return;
+      break;
+    case FIN_RU:
+      if (st->done || !st->update) return;
+      break;
+    default:
+      break;
+  }
   __shared__ double sm[kWaves][NR];
   double acc[NR];
 #pragma unroll
@@ -618,6 +679,56 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int op, int nblk, const dou
       }
       break;
     }
+    case FIN_XR_RU: {  // FP32 step inside the refinement: refresh or stop
+      const double res = sqrt(tot[1]);
+      const int i = st->iter;
+      st->rsq = tot[1];
+      st->res = res;
+      st->iter = i + 1;
+      if (!(res == res) || res > 1e300) {
+        st->done = 1;
+        st->reason = 8;
+        st->report_iters = i + 1;
+      } else if (res < st->rlast * st->delta || res < sqrt(st->rsq0) * st->res_rat) {
+        st->update = 1;
+      } else if (i + 1 >= st->maxit) {
+        st->done = 1;
+        st->report_iters = st->maxit + 1;
+      }
+      break;
+    }
+    case FIN_RU_START:
+    case FIN_RU: {  // true residual after a refresh (or at the start)
+      const double rl2 = sqrt(tot[0]);
+      const double res = st->norm == 0 ? ref_norm(st, tot[1]) : rl2;
+      if (op == FIN_RU_START) {
+        krylov_init(st, res, tot[0], 0.0);
+        st->outer = 0;
+      } else {
+        st->outer += 1;
+        if (st->hist && st->outer < st->hist_cap) st->hist[st->outer] = res;
+      }
+      st->res_true = res;
+      st->rlast = rl2;
+      st->update = 0;
+      st->res = sqrt(tot[2]);
+      if (!(res == res)) {
+        st->done = 1;
+        st->reason = 8;
+      } else if (res < st->res_tol || res < st->res0 * st->res_rat || st->res0 == 0.0) {
+        st->done = 1;
+        st->converged = 1;
+        st->reason = 2;
+        st->report_iters = st->iter;
+      } else if (st->iter >= st->maxit) {
+        st->done = 1;
+        st->report_iters = st->maxit + 1;
+      }
+      break;
+    }
+    case FIN_RU_END:  // true residual at exit without convergence
+      st->res_true = st->norm == 0 ? ref_norm(st, tot[1]) : sqrt(tot[0]);
+      break;
     case FIN_RHO: {
       const double m = tot[3] > 0.0 ? tot[1] / tot[3] : 0.0;
       const double rho = tot[0] - m * tot[2];

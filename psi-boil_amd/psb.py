@@ -21,6 +21,7 @@ PSB_CG, PSB_BICGS = 0, 1
 PSB_PREC_DIAG, PSB_PREC_AC = 0, 1
 PSB_FP64, PSB_MIXED = 0, 1
 PSB_NORM_REF, PSB_NORM_L2REL = 0, 1
+PSB_OK, PSB_MAXITER, PSB_STALE, PSB_BREAKDOWN, PSB_DIVERGED, PSB_NAN = 0, 1, 2, 3, 4, 5
 
 STATUS = {0: "OK", 1: "MAXITER", 2: "STALE", 3: "BREAKDOWN", 4: "DIVERGED", 5: "NAN",
           -1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ENOTREADY"}

@@ -202,7 +202,74 @@ def test_mixed_precision_refinement_reaches_fp64_accuracy():
     assert err < 1e-7
 
 
-def test_acpcg_with_solid_cells_and_stretched_grid():
+def test_reliable_update_mixed_matches_fp64_iterations():
+    """FP32 AC-PCG with reliable updates: the FP64 solution to the FP64
+    accuracy, in as many iterations as the FP64 AC-PCG (+-2)."""
+    d, A = ac_case(48, nsph=6, seed=3)
+    b = d["rhs"]
+    n = b.size
+    s = psb.Solver(48, 48, 48)
+    s.set_matrix(d)
+    kw = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=1e-10, max_iter=500)
+    _, r64 = s.solve(np.zeros(n), b, **kw)
+    x, r = s.solve(np.zeros(n), b, precision=psb.PSB_MIXED, inner_adapt=2, inner_rtol=1e-2,
+                   **kw)
+    assert r["converged"] and r["status"] == psb.PSB_OK and r["outer"] >= 3
+    assert abs(r["iterations"] - r64["iterations"]) <= 2
+    assert r["rel_l2"] <= 1e-10 * 1.01
+    # the reported residual is the true one of the returned x
+    assert abs(r["res"] / r["res0"] - r["rel_l2"]) <= 1e-6 * r["rel_l2"]
+    h = s.history()
+    assert len(h) == r["outer"] + 1 and h[0] == r["res0"] and h[-1] == r["res"]
+    # each update after a 1e-2 drop of the recursive residual, the last one
+    # when it reaches the target
+    assert np.all(h[1:-1] < h[:-2] * 0.02) and h[-1] < h[-2]
+    ac = O.AC(A, np.ones(n), rule=1)
+    xo, info = ac.acpcg(np.zeros(n), b, rtol=1e-12, maxit=400)
+    assert rel(x - x.mean(), xo - xo.mean()) < 1e-7
+
+
+def test_reliable_update_max_iter_reports_true_residual():
+    d, _ = ac_case(32, nsph=4, seed=5)
+    b = d["rhs"]
+    n = b.size
+    s = psb.Solver(32, 32, 32)
+    s.set_matrix(d)
+    x, r = s.solve(np.zeros(n), b, precond=psb.PSB_PREC_AC, precision=psb.PSB_MIXED,
+                   inner_adapt=2, inner_rtol=1e-3, norm=psb.PSB_NORM_L2REL, res_rat=1e-12,
+                   max_iter=5)
+    assert not r["converged"] and r["status"] == psb.PSB_MAXITER
+    assert r["iterations"] == 6  # reference count maxit + 1 (cg.cpp:178)
+    assert abs(r["res"] / r["res0"] - r["rel_l2"]) <= 1e-6 * r["rel_l2"]
+
+
+@pytest.mark.parametrize("precision", [psb.PSB_FP64, psb.PSB_MIXED])
+def test_ac_with_reference_norm(precision):
+    """PSB_NORM_REF with the AC preconditioner on a stretched grid: the
+    reported residual is sqrt(sum (r/dV)^2)/N of the returned x."""
+    nx, ny, nz = 32, 16, 48
+    d = psb.generate(nx, ny, nz, 6, 0.02, 0.06, 1.0, 1000.0, 11, 1011, stretch_z=2.0, lz=1.5)
+    A = O.Mat(nx, ny, nz, {k: d[k] for k in "cwesnbt"})
+    b, n = d["rhs"], d["rhs"].size
+    s = psb.Solver(nx, ny, nz)
+    s.set_matrix(d)
+    s.set_geometry(d["dxc"], d["dyc"], d["dzc"])
+    x, r = s.solve(np.zeros(n), b, precond=psb.PSB_PREC_AC, precision=precision,
+                   inner_adapt=2, inner_rtol=1e-2, norm=psb.PSB_NORM_REF, res_rat=1e-9,
+                   res_tol=0.0, max_iter=500)
+    assert r["converged"]
+    dV = (d["dxc"][:, None, None] * d["dyc"][None, :, None] * d["dzc"][None, None, :]).ravel()
+    rr = O.resid(A, b, x)
+    ref = np.sqrt(np.sum((rr / dV) ** 2)) / n
+    ref0 = np.sqrt(np.sum((b / dV) ** 2)) / n
+    assert abs(r["res0"] - ref0) <= 1e-12 * ref0
+    tol = 1e-9 if precision == psb.PSB_MIXED else 1e-4  # FP64: recursive residual
+    assert abs(r["res"] - ref) <= tol * ref
+    assert r["res"] < 1e-9 * r["res0"]
+
+
+@pytest.mark.parametrize("adapt,rtol", [(0, 1e-4), (2, 1e-2)])
+def test_acpcg_with_solid_cells_and_stretched_grid(adapt, rtol):
     nx, ny, nz = 32, 16, 64
     d = psb.generate(nx, ny, nz, 8, 0.01, 0.04, 1.0, 1000.0, 7, 1007, stretch_z=2.5, lz=2.0,
                      cylinder=True, cyl_d=0.2)
@@ -213,7 +280,8 @@ def test_acpcg_with_solid_cells_and_stretched_grid():
     s.set_matrix(d)
     s.set_geometry(d["dxc"], d["dyc"], d["dzc"])
     x, r = s.solve(np.zeros(n), b, precond=psb.PSB_PREC_AC, precision=psb.PSB_MIXED,
-                   norm=psb.PSB_NORM_L2REL, res_rat=1e-9, max_iter=1000)
+                   norm=psb.PSB_NORM_L2REL, res_rat=1e-9, max_iter=1000, inner_adapt=adapt,
+                   inner_rtol=rtol)
     assert r["converged"] and r["rel_l2"] <= 1e-9 * 1.01
     off = d["off"].astype(bool)
     assert np.all(x[off] == 0.0)
