@@ -10,6 +10,14 @@ FP64 iterative refinement (the value), and the plain FP64 solve beside it.
 Matrix and right-hand side are resident in HBM before the timed region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+
+`--case batch` solves 96 seeded systems of uneven sizes as one batch and
+beside it one after another.  `--case 512` and `--case cyl` print the figures
+of the two other single-system
+workloads instead (512^3 with 64 bubbles to 1e-10; 384x384x768 z-stretched
+grid with a solid cylinder and 128 bubbles to 1e-9): solve times for FP64,
+restarted refinement (FP32 inner to 1e-4) and reliable updates, the final
+relative residuals, and the error against an FP64 solve to 1e-12.
 """
 import argparse
 import json
@@ -84,6 +92,141 @@ def cpu_baseline(n, iters_full, sample_iters):
             "s_per_iteration": s_iter, "iterations_full": iters_full}
 
 
+EXTRA = {
+    "512": dict(grid=(512, 512, 512), gen=dict(nsph=64, rmin=0.02, rmax=0.08, rho_in=1.0,
+                                                rho_out=1000.0, seed_sph=3, seed_vel=1003),
+                tol=1e-10,
+                desc="pressure Poisson 512^3, 64 bubbles r 0.02-0.08, density ratio 1000, "
+                     "Neumann walls; AC-PCG to 1e-10"),
+    "cyl": dict(grid=(384, 384, 768), gen=dict(nsph=128, rmin=0.01, rmax=0.04, rho_in=1.0,
+                                                rho_out=1000.0, seed_sph=5, seed_vel=1005,
+                                                stretch_z=2.5, lz=2.0, cylinder=True, cyl_d=0.2),
+                tol=1e-9,
+                desc="pressure Poisson 384x384x768 on [0,1]x[0,1]x[0,2], tanh z-stretching "
+                     "2.5, solid cylinder d 0.2 along y, 128 bubbles r 0.01-0.04, density "
+                     "ratio 1000; AC-PCG to 1e-9"),
+}
+
+
+def extra_case(a):
+    """Figures for the 512^3 and the stretched-cylinder workloads."""
+    E = EXTRA[a.case]
+    nx, ny, nz = E["grid"]
+    log(f"[bench] generating {a.case} case {nx}x{ny}x{nz}")
+    d = psb.generate(nx, ny, nz, **E["gen"])
+    s = psb.Solver(nx, ny, nz)
+    if d["off"].any():
+        s.set_active(d["off"])
+    s.set_matrix(d)
+    s.set_geometry(d["dxc"], d["dyc"], d["dzc"])
+    db = psb.DeviceArray.from_host(d["rhs"])
+    dx = psb.DeviceArray(nx * ny * nz)
+    del d
+    common = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=E["tol"],
+                  res_tol=0.0, max_iter=2000)
+    runs = {"fp64": dict(precision=psb.PSB_FP64),
+            "mixed_restarted_1e-4": dict(precision=psb.PSB_MIXED, inner_rtol=1e-4,
+                                         inner_adapt=0, max_outer=60),
+            "mixed_reliable_1e-2": dict(precision=psb.PSB_MIXED, inner_rtol=1e-2,
+                                        inner_adapt=2)}
+    dx.zero()
+    r12 = s.solve_device(dx.addr, db.addr, precision=psb.PSB_FP64,
+                         **dict(common, res_rat=1e-12, max_iter=4000))
+    x_ref = dx.download()
+    x_ref -= x_ref.mean()
+    log(f"[bench] fp64 to 1e-12 {r12}")
+    out = {"workload": E["desc"], "grid": list(E["grid"]), "tolerance": E["tol"],
+           "fp64_1e-12": {"iterations": r12["iterations"], "rel_l2": r12["rel_l2"]}}
+    for name, kw in runs.items():
+        t, r = timed_solves(s, dx, db, a.steps, a.warmup, **dict(common, **kw))
+        x = dx.download()
+        x -= x.mean()
+        err = float(np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref))
+        log(f"[bench] {name} {t * 1e3:.2f} ms/solve {r} err {err:.2e}")
+        out[name] = {"ms_per_solve": round(t * 1e3, 3), "solves_per_s": round(1.0 / t, 3),
+                     "iterations": r["iterations"], "outer": r["outer"],
+                     "status": r["status_name"], "rel_l2": r["rel_l2"],
+                     "error_vs_fp64_1e-12": err}
+    out["mixed_over_fp64"] = round(out["fp64"]["ms_per_solve"] /
+                                   out["mixed_reliable_1e-2"]["ms_per_solve"], 3)
+    print(json.dumps(out), flush=True)
+
+
+def batch_case(a):
+    """96 systems of seeded sizes solved as one batch, against the same
+    systems solved one after another with the same method (same contexts)."""
+    cases = psb.batch_cases(4, a.nsys)
+    shapes = [(c["nx"], c["ny"], c["nz"]) for c in cases]
+    ncell = sum(x * y * z for x, y, z in shapes)
+    log(f"[bench] generating {len(cases)} systems, {ncell / 1e6:.1f} M cells")
+    B = psb.Batch(shapes)
+    dbs, dxs = [], []
+    for q, c in enumerate(cases):
+        d = psb.generate(**c)
+        B.systems[q].set_matrix(d)
+        B.systems[q].set_geometry(d["dxc"], d["dyc"], d["dzc"])
+        dbs.append(psb.DeviceArray.from_host(d["rhs"]))
+        dxs.append(psb.DeviceArray(d["rhs"].size))
+    stop = dict(res_rat=1e-8, max_iter=1000)
+    px, pb = [x.addr for x in dxs], [b.addr for b in dbs]
+
+    def run_batch():
+        for x in dxs:
+            x.zero()
+        return B.solve_device(px, pb, **stop)
+
+    log("[bench] batched solves (plan build and autotuning in the warm-up)")
+    for _ in range(max(1, a.warmup)):
+        res, rc = run_batch()
+    psb.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        res, rc = run_batch()
+    psb.device_synchronize()
+    tb = (time.perf_counter() - t0) / a.steps
+    xb = [x.download() for x in dxs]
+    log(f"[bench] batch {tb * 1e3:.2f} ms rc {rc}")
+    # the same systems one after another (same contexts, same method)
+    kw = dict(psb.BATCH_STOP, **stop)
+    for q in range(len(cases)):
+        dxs[q].zero()
+        B.systems[q].solve_device(dxs[q].addr, dbs[q].addr, **kw)
+    psb.device_synchronize()
+    t0 = time.perf_counter()
+    seq = []
+    for q in range(len(cases)):
+        dxs[q].zero()
+        seq.append(B.systems[q].solve_device(dxs[q].addr, dbs[q].addr, **kw))
+    psb.device_synchronize()
+    ts = time.perf_counter() - t0
+    log(f"[bench] sequential {ts * 1e3:.2f} ms")
+    # error against FP64 solves to 1e-12
+    errs = []
+    for q in range(len(cases)):
+        dxs[q].zero()
+        B.systems[q].solve_device(dxs[q].addr, dbs[q].addr, precond=psb.PSB_PREC_AC,
+                                  norm=psb.PSB_NORM_L2REL, res_rat=1e-12, res_tol=0.0,
+                                  max_iter=2000)
+        xr = dxs[q].download()
+        xr -= xr.mean()
+        xq = xb[q] - xb[q].mean()
+        errs.append(float(np.linalg.norm(xq - xr) / np.linalg.norm(xr)))
+    its = [r["iterations"] for r in res]
+    out = {"workload": f"{len(cases)} pressure systems, dims from {{32,48,64,96,128}} per "
+                       "direction (seed 4), 1-8 bubbles r 0.05-0.2 of the shortest side, "
+                       "density ratio 10-1000; FP32 AC-PCG with FP64 reliable updates to 1e-8",
+           "systems": len(cases), "cells": ncell,
+           "batched_ms": round(tb * 1e3, 3), "batched_systems_per_s": round(len(cases) / tb, 2),
+           "sequential_ms": round(ts * 1e3, 3),
+           "batched_over_sequential": round(ts / tb, 3),
+           "converged": sum(r["converged"] for r in res), "status": rc,
+           "max_rel_l2": max(r["rel_l2"] for r in res),
+           "iterations_min_mean_max": [min(its), sum(its) / len(its), max(its)],
+           "max_error_vs_fp64_1e-12": max(errs),
+           "plan_level_class_vpt": B.plan()}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +235,9 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--nsys", type=int, default=96, help="systems of --case batch")
+    ap.add_argument("--case", choices=sorted(EXTRA) + ["batch"], default=None,
+                    help="figures of another single-system workload instead of the bench line")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch the roofline kernels (FP32 Jacobi sweep, FP32/FP64 SpMV) "
                          "and a plain copy, for hardware-counter runs")
@@ -101,6 +247,12 @@ def main():
               "(the solver has no multi-GPU path: it would shard by 3-D blocks with a "
               "one-cell halo exchange and all-reduced scalars)", file=sys.stderr)
         sys.exit(2)
+    if a.case == "batch":
+        batch_case(a)
+        return
+    if a.case:
+        extra_case(a)
+        return
     n = a.n
     log(f"[bench] generating {n}^3 case")
     d = psb.generate(n, n, n, **CASE)

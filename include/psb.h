@@ -181,6 +181,51 @@ int psb_generate(const psb_case *cs, double *c, double *w, double *e,
                  double *s, double *n, double *b_, double *t, double *rhs,
                  unsigned char *off, double *dxc, double *dyc, double *dzc);
 
+/* Seeded batch of cases (dims from {32, 48, 64, 96, 128} per direction,
+ * 1-8 bubbles of radius 0.05-0.2 of the shortest side, density ratio
+ * 10-1000, own seeds); fills out[0..nsys).                                  */
+int psb_batch_cases(unsigned long long seed, int nsys, psb_case *out);
+
+/* ------------------------------------------------------------------------
+ * Batched solves.  No counterpart in the reference, whose Linear::solve
+ * takes one system: many independent pressure systems of different grid
+ * sizes, each set up through its own context, solved together -- every
+ * phase of an iteration is one launch over all systems, the iteration
+ * sequence is one captured graph, each system stops on its own criteria.
+ * The method is that of psb_solve with PSB_PREC_AC, PSB_CG, PSB_MIXED and
+ * inner_adapt = 2 (FP32 AC-PCG with FP64 reliable updates); the coarsening
+ * continues while a direction stays >= 4 cells.
+ * ------------------------------------------------------------------------ */
+typedef struct psb_batch psb_batch;
+
+int psb_batch_create(psb_batch **bt, int nsys, const psb_layout *layouts);
+int psb_batch_destroy(psb_batch *bt);
+
+/* The context of system `sys` (owned by the batch): give it its geometry,
+ * off mask and matrix with psb_set_geometry / psb_set_active /
+ * psb_set_matrix.  The next batched solve rebuilds its plan if any system
+ * changed.  NULL for a bad index.                                          */
+psb_ctx *psb_batch_system(psb_batch *bt, int sys);
+
+/* Work per thread of the streaming kernels: 0 = chosen per (level, size
+ * class) by timing the Jacobi sweep of each class with 1, 2, 4 and 8 cell
+ * vectors per thread when the plan is built (default); > 0 = that value.    */
+int psb_batch_set_tuning(psb_batch *bt, int vpt);
+
+/* The autotuner's choices: (level, log2 cells, vectors per thread) triples;
+ * returns their count.                                                      */
+int psb_batch_plan(psb_batch *bt, int *triples, int cap);
+
+/* x[s] (in: initial guess, out: solution) and b[s] are host arrays in
+ * system s's layout; res[s] its result.  One psb_stop for all.  Returns
+ * PSB_OK when every system converged, else the first other status.          */
+int psb_batch_solve(psb_batch *bt, double *const *x, const double *const *b,
+                    const psb_stop *stop, psb_result *res);
+/* Same on device-resident compact arrays (hipMalloc'd by the caller).      */
+int psb_batch_solve_device(psb_batch *bt, double *const *d_x,
+                           const double *const *d_b, const psb_stop *stop,
+                           psb_result *res);
+
 /* Benchmark hook: average device time (ms) of `reps` back-to-back launches
  * of one kernel on the context's stream, bracketed by HIP events.  kernel:
  * 0 FP64 SpMV, 1 FP32 SpMV, 2 FP32 damped Jacobi sweep, 3 plain FP32 copy

@@ -526,6 +526,20 @@ int ora_levels(int rule, int nx, int ny, int nz, int maxlev, int *lx, int *ly,
     int n[3] = {lx[L - 1], ly[L - 1], lz[L - 1]};
     int m[3];
     int any = 0;
+    if (rule == 3) {
+      /* rule 3: halve x and y while they stay >= 8, then z */
+      int xy = 0;
+      for (int d = 0; d < 2; d++) {
+        m[d] = n[d];
+        if (n[d] % 2 == 0 && n[d] / 2 >= 8) { m[d] = n[d] / 2; xy = 1; }
+      }
+      m[2] = n[2];
+      if (!xy && n[2] % 2 == 0 && n[2] / 2 >= 8) { m[2] = n[2] / 2; xy = 1; }
+      if (!xy) break;
+      lx[L] = m[0]; ly[L] = m[1]; lz[L] = m[2];
+      L++;
+      continue;
+    }
     for (int d = 0; d < 3; d++) {
       m[d] = n[d];
       if (rule == 0) {
@@ -533,7 +547,9 @@ int ora_levels(int rule, int nx, int ny, int nz, int maxlev, int *lx, int *ly,
         int f = ora_smallest_factor(n[d]);
         if (n[d] % f == 0 && n[d] > 4 && n[d] != f) { m[d] = n[d] / f; any = 1; }
       } else {
-        if (n[d] % 2 == 0 && n[d] / 2 >= 8) { m[d] = n[d] / 2; any = 1; }
+        /* rule 1: halve while >= 8; rule 2: halve while >= 4 */
+        const int lo = rule == 2 ? 4 : 8;
+        if (n[d] % 2 == 0 && n[d] / 2 >= lo) { m[d] = n[d] / 2; any = 1; }
       }
     }
     if (!any) break;
@@ -951,8 +967,49 @@ static void jacobi_sweep(const ora_mat *A, const double *r, double *e, double *t
   for (size_t P = 0; P < N; P++) e[P] += omega * t[P] / A->c[P];
 }
 
+/* z-line Jacobi: e += omega T^-1 (r - A e), T the tridiagonal part of A
+ * along k (diagonal c, couplings -b) of each (i, j) column, solved by the
+ * Thomas algorithm (forward elimination, back substitution). */
+static void zline_solve(const ora_mat *A, const double *t, double *d, double *y) {
+  const int nx = A->nx, ny = A->ny, nz = A->nz;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      const size_t P0 = IDX(i, j, 0, ny, nz);
+      double piv = A->c[P0];
+      y[0] = t[P0];
+      d[P0] = piv;
+      for (int k = 1; k < nz; k++) {
+        const double a = A->b[P0 + k];
+        const double l = a / piv;
+        piv = A->c[P0 + k] - l * a;
+        d[P0 + k] = piv;
+        y[k] = t[P0 + k] + l * y[k - 1];
+      }
+      /* back substitution, solution into d */
+      double xn = y[nz - 1] / d[P0 + nz - 1];
+      d[P0 + nz - 1] = xn;
+      for (int k = nz - 2; k >= 0; k--) {
+        xn = (y[k] + A->b[P0 + k + 1] * xn) / d[P0 + k];
+        d[P0 + k] = xn;
+      }
+    }
+}
+
+static void zline_sweep(const ora_mat *A, const double *r, double *e, double *t,
+                        double *d, double *y, double omega) {
+  size_t N = mat_n(A);
+  ora_resid(A, r, e, t);
+  zline_solve(A, t, d, y);
+  for (size_t P = 0; P < N; P++) e[P] += omega * d[P];
+}
+
 void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
                     int nu2, double omega, double cfac) {
+  ora_ac_precond_s(ac, l, r, z, nu1, nu2, omega, cfac, 0);
+}
+
+void ora_ac_precond_s(ora_ac *ac, int l, const double *r, double *z, int nu1,
+                      int nu2, double omega, double cfac, int smoother) {
   const ora_mat *A = &ac->A[l];
   const size_t N = mat_n(A);
   if (l == ac->nlev - 1) {
@@ -965,9 +1022,22 @@ void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
     return;
   }
   double *t = ac->res[l];
-  /* first sweep from zero: e = omega r / c */
-  for (size_t P = 0; P < N; P++) z[P] = nu1 > 0 ? omega * r[P] / A->c[P] : 0.0;
-  for (int s = 1; s < nu1; s++) jacobi_sweep(A, r, z, t, omega);
+  double *ld = NULL, *ly = NULL;
+  if (smoother == 1) {
+    ld = dalloc(N);
+    ly = dalloc((size_t)A->nz);
+  }
+  /* first sweep from zero: e = omega r / c (point) or omega T^-1 r (z-line) */
+  if (smoother == 1 && nu1 > 0) {
+    zline_solve(A, r, ld, ly);
+    for (size_t P = 0; P < N; P++) z[P] = omega * ld[P];
+  } else {
+    for (size_t P = 0; P < N; P++) z[P] = nu1 > 0 ? omega * r[P] / A->c[P] : 0.0;
+  }
+  for (int s = 1; s < nu1; s++) {
+    if (smoother == 1) zline_sweep(A, r, z, t, ld, ly, omega);
+    else jacobi_sweep(A, r, z, t, omega);
+  }
   /* residual and restriction (block sum, inactive 0).  Unlike the
    * reference's stand-alone cycle (additive_restriction.cpp:151-174) the
    * mean is not removed here: the residual sums to zero up to rounding
@@ -995,7 +1065,7 @@ void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
       }
   for (size_t Q = 0; Q < NH; Q++)
     if (!fl[Q]) rH[Q] = 0.0;
-  ora_ac_precond(ac, l + 1, rH, eH, nu1, nu2, omega, cfac);
+  ora_ac_precond_s(ac, l + 1, rH, eH, nu1, nu2, omega, cfac, smoother);
   for (int I = 0; I < nX; I++)
     for (int J = 0; J < nY; J++)
       for (int K = 0; K < nZ; K++) {
@@ -1007,7 +1077,12 @@ void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
               if (ff[P]) z[P] += v;
             }
       }
-  for (int s = 0; s < nu2; s++) jacobi_sweep(A, r, z, t, omega);
+  for (int s = 0; s < nu2; s++) {
+    if (smoother == 1) zline_sweep(A, r, z, t, ld, ly, omega);
+    else jacobi_sweep(A, r, z, t, omega);
+  }
+  free(ld);
+  free(ly);
 }
 
 static double nrm2(size_t n, const double *x) { return sqrt(ora_dot(n, x, x)); }
@@ -1030,6 +1105,12 @@ static void project_mean(const ora_ac *ac, double *z) {
 int ora_acpcg(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
               int nu1, int nu2, double omega, double cfac, double *hist,
               int *iters) {
+  return ora_acpcg_s(ac, x, b, rtol, maxit, nu1, nu2, omega, cfac, 0, hist, iters);
+}
+
+int ora_acpcg_s(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
+                int nu1, int nu2, double omega, double cfac, int smoother,
+                double *hist, int *iters) {
   const ora_mat *A = &ac->A[0];
   const size_t N = mat_n(A);
   if (!ac->G) ora_ac_coarse_dense(ac);
@@ -1039,7 +1120,7 @@ int ora_acpcg(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
   int it = 0, conv = 0;
   if (hist) hist[0] = 1.0;
   if (r0 == 0.0) { conv = 1; goto done; }
-  ora_ac_precond(ac, 0, r, z, nu1, nu2, omega, cfac);
+  ora_ac_precond_s(ac, 0, r, z, nu1, nu2, omega, cfac, smoother);
   project_mean(ac, z);
   memcpy(p, z, N * 8);
   double rho = ora_dot(N, r, z);
@@ -1051,7 +1132,7 @@ int ora_acpcg(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
     double rel = nrm2(N, r) / r0;
     if (hist) hist[it] = rel;
     if (rel < rtol) { conv = 1; break; }
-    ora_ac_precond(ac, 0, r, z, nu1, nu2, omega, cfac);
+    ora_ac_precond_s(ac, 0, r, z, nu1, nu2, omega, cfac, smoother);
     project_mean(ac, z);
     double rho_new = ora_dot(N, r, z);
     double beta = rho_new / rho;

@@ -107,8 +107,9 @@ int ora_jacobi(const ora_mat *A, double *x, const double *b, int mini,
 
 /* ---- additive-correction multigrid -------------------------------------- */
 int  ora_smallest_factor(int n);                 /* Src/Global/global_factor.cpp */
-/* Level sizes by Src/Domain/domain_coarsen.cpp:6-112 (rule 0) or by the
- * benchmark rule "halve a direction while it stays >= 8" (rule 1).
+/* Level sizes by Src/Domain/domain_coarsen.cpp:6-112 (rule 0), by the
+ * benchmark rule "halve a direction while it stays >= 8" (rule 1) or by
+ * "halve while it stays >= 4" (rule 2, the batched solver's hierarchy).
  * Returns the number of levels; nl{x,y,z}[0] = n{x,y,z}.                    */
 int  ora_levels(int rule, int nx, int ny, int nz, int maxlev,
                 int *lx, int *ly, int *lz);
@@ -153,12 +154,20 @@ int  ora_ac_coarse_dense(ora_ac *ac);
  * prolongation scaled by cfac, nu2 sweeps; coarsest level z = G r.          */
 void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
                     int nu2, double omega, double cfac);
+/* Same with a choice of smoother: 0 point Jacobi (as above), 1 z-line
+ * Jacobi, e += omega T^-1 (r - A e) with T the tridiagonal part of A along
+ * k of each (i, j) column (Thomas algorithm).                              */
+void ora_ac_precond_s(ora_ac *ac, int l, const double *r, double *z, int nu1,
+                      int nu2, double omega, double cfac, int smoother);
 /* PCG with the V-cycle preconditioner, its output projected to zero mean over
  * the active cells; stops on ||r||2/||r0||2 < rtol.
  * hist[i] = relative residual after iteration i (hist[0] = 1).             */
 int  ora_acpcg(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
                int nu1, int nu2, double omega, double cfac, double *hist,
                int *iters);
+int  ora_acpcg_s(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
+                 int nu1, int nu2, double omega, double cfac, int smoother,
+                 double *hist, int *iters);
 /* Diagonal PCG with the same relative-2-norm stopping rule. */
 int  ora_pcg_rel(const ora_mat *A, double *x, const double *b, double rtol,
                  int maxit, double *hist, int *iters);

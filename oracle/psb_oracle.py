@@ -282,21 +282,24 @@ class AC:
         m = lib().ora_ac_coarse_dense(C.byref(self.ac))
         return np.ctypeslib.as_array(self.ac.G, shape=(m * m,)).reshape(m, m).copy()
 
-    def precond(self, r, nu1=2, nu2=2, omega=0.8, cfac=1.0):
+    def precond(self, r, nu1=2, nu2=2, omega=0.8, cfac=1.0, smoother=0):
         if not self.ac.G:
             self.coarse_dense()
         z = np.zeros(r.size)
-        lib().ora_ac_precond(C.byref(self.ac), 0, _p(np.ascontiguousarray(r, dtype=np.float64)),
-                             _p(z), nu1, nu2, C.c_double(omega), C.c_double(cfac))
+        lib().ora_ac_precond_s(C.byref(self.ac), 0,
+                               _p(np.ascontiguousarray(r, dtype=np.float64)), _p(z), nu1, nu2,
+                               C.c_double(omega), C.c_double(cfac), smoother)
         return z
 
-    def acpcg(self, x, b, rtol=1e-8, maxit=500, nu1=2, nu2=2, omega=0.8, cfac=1.0):
+    def acpcg(self, x, b, rtol=1e-8, maxit=500, nu1=2, nu2=2, omega=0.8, cfac=1.0,
+              smoother=0):
         x = np.array(x, dtype=np.float64, copy=True)
         hist = np.full(maxit + 1, np.nan)
         it = C.c_int(0)
-        ok = lib().ora_acpcg(C.byref(self.ac), _p(x), _p(np.ascontiguousarray(b, dtype=np.float64)),
-                             C.c_double(rtol), maxit, nu1, nu2, C.c_double(omega),
-                             C.c_double(cfac), _p(hist), C.byref(it))
+        ok = lib().ora_acpcg_s(C.byref(self.ac), _p(x),
+                               _p(np.ascontiguousarray(b, dtype=np.float64)),
+                               C.c_double(rtol), maxit, nu1, nu2, C.c_double(omega),
+                               C.c_double(cfac), smoother, _p(hist), C.byref(it))
         return x, dict(converged=bool(ok), iters=it.value, hist=hist[:it.value + 1])
 
 

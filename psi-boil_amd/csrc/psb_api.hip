@@ -21,144 +21,17 @@
 #include <type_traits>
 #include <vector>
 
-#include "../../include/psb.h"
-#include "psb_kernels.h"
+
+#include "psb_internal.h"
 
 using namespace psb;
 
-namespace {
+using namespace psb::host;
 
-struct HipError {
-  hipError_t e;
-};
 
-#define HIPCHK(x)                               \
-  do {                                          \
-    hipError_t e_ = (x);                        \
-    if (e_ != hipSuccess) throw HipError{e_};   \
-  } while (0)
 
-// blocks for a streaming kernel over `nwork` items: one pass of 256 threads
-// per item run, at most kMaxBlocks (8 per CU on the 256 CUs), a multiple of
-// the 8 XCDs so the XCD remap applies
-int grid_blocks(long long nwork) {
-  long long nb = (nwork + kBlock - 1) / kBlock;
-  if (nb > kMaxBlocks) nb = kMaxBlocks;
-  if (nb >= 8) nb = (nb / 8) * 8;
-  if (nb < 1) nb = 1;
-  return (int)nb;
-}
-
-FastDiv make_div(uint32_t d) {
-  uint32_t s = 0;
-  while ((1ull << s) < d) s++;
-  FastDiv f;
-  f.d = d;
-  f.m = (uint32_t)(((1ull << (31 + s)) + d - 1) / d);
-  f.sh = 31 + s;
-  return f;
-}
-
-Grid make_grid(int nx, int ny, int nz) {
-  Grid g;
-  g.nx = nx; g.ny = ny; g.nz = nz;
-  g.sI = ny * nz;
-  g.N = nx * ny * nz;
-  g.dI = make_div((uint32_t)g.sI);
-  g.dZ = make_div((uint32_t)nz);
-  return g;
-}
-
-// device allocations owned by a context
-struct Arena {
-  std::vector<void *> ptrs;
-  template <typename T>
-  T *alloc(size_t n) {
-    void *p = nullptr;
-    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
-    bytes = (bytes + 255) & ~size_t(255);
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) throw HipError{e};
-    ptrs.push_back(p);
-    return static_cast<T *>(p);
-  }
-  void release() {
-    for (void *p : ptrs) (void)hipFree(p);
-    ptrs.clear();
-  }
-};
-
-template <typename T>
-struct Level {
-  Grid g;
-  int CI = 1, CJ = 1, CK = 1;      // factor from the finer level (1 or 2)
-  T *c = nullptr, *w = nullptr, *s = nullptr, *b = nullptr;
-  T *r = nullptr;                  // right-hand side of this level
-  T *e[3] = {nullptr, nullptr, nullptr};
-  uint8_t *aflag = nullptr;        // NULL: all active
-  DMat<T> mat() const { return DMat<T>{c, w, s, b, nullptr, nullptr, nullptr}; }
-};
-
-template <typename T>
-struct Hier {
-  std::vector<Level<T>> L;
-  double *G = nullptr;  // dense coarsest operator (FP64)
-  int m = 0;
-  bool built = false;
-};
-
-template <typename T>
-struct KrylovVecs {
-  T *x = nullptr, *r = nullptr, *q = nullptr, *p[2] = {nullptr, nullptr};
-  T *v[2] = {nullptr, nullptr}, *s = nullptr, *t = nullptr, *rt = nullptr;
-};
-
-struct Graph {
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
-  long long key = -1;
-  const void *px = nullptr, *pb = nullptr;  // caller arrays baked into the graph
-};
-
-}  // namespace
-
-struct psb_ctx {
-  psb_layout lay{};
-  Grid g{};
-  hipStream_t stream = nullptr;
-  Arena base;   // lives as long as the context
-  Arena arena;  // matrix, hierarchy and work vectors: rebuilt by set_matrix
-  // matrix
-  bool have_matrix = false, sym = true;
-  double *A64[7] = {};   // c w s b e n t
-  float *A32[4] = {};
-  bool have32 = false;
-  std::vector<uint8_t> off;  // host copy of the off mask (may be empty)
-  uint8_t *aflag0 = nullptr;
-  // geometry
-  double *dxc = nullptr, *dyc = nullptr, *dzc = nullptr;
-  // solver state
-  DevState *st64 = nullptr, *st32 = nullptr;
-  DevState *hst = nullptr;  // pinned host mirror
-  double *hist = nullptr;
-  int hist_cap = 0;
-  std::vector<double> hist_host;  // refinement: FP64 residual per outer step
-  bool hist_on_host = false;
-  int hist_count = 0;
-  Work work{};
-  KrylovVecs<double> k64;
-  KrylovVecs<float> k32;
-  Hier<double> h64;
-  Hier<float> h32;
-  double *db = nullptr, *dx = nullptr;  // staging for host-array solves
-  double *dy = nullptr;                 // scratch for psb_apply
-  double *relres = nullptr;             // 2 doubles: sum r^2, sum b^2
-  Graph gr_cgdiag, gr_bicgs, gr_ac64, gr_ac32, gr_ru;
-  double ac_omega = 0, ac_cfac = 0;
-  std::string err;
-};
-
-namespace {
+namespace psb {
+namespace host {
 
 DMat<double> mat64(const psb_ctx *c) {
   return DMat<double>{c->A64[0], c->A64[1], c->A64[2], c->A64[3],
@@ -407,8 +280,9 @@ std::vector<double> dense_pinv(int nx, int ny, int nz, const std::vector<double>
   return G;
 }
 
-// level sizes: halve a direction while it stays >= 8 (the benchmark's rule)
-std::vector<int> level_dims(int nx, int ny, int nz) {
+// level sizes: halve a direction while it stays >= min_dim (8: the
+// benchmark's rule down to 8^3)
+std::vector<int> level_dims(int nx, int ny, int nz, int min_dim) {
   std::vector<int> d = {nx, ny, nz};
   while ((int)d.size() / 3 < kMaxLevels) {
     int n[3] = {d[d.size() - 3], d[d.size() - 2], d[d.size() - 1]};
@@ -416,7 +290,7 @@ std::vector<int> level_dims(int nx, int ny, int nz) {
     bool any = false;
     for (int q = 0; q < 3; q++) {
       m[q] = n[q];
-      if (n[q] % 2 == 0 && n[q] / 2 >= 8) {
+      if (n[q] % 2 == 0 && n[q] / 2 >= min_dim) {
         m[q] = n[q] / 2;
         any = true;
       }
@@ -448,12 +322,15 @@ void ensure_krylov(psb_ctx *C, KrylovVecs<T> &K, bool bicgs) {
   }
 }
 
+template void ensure_krylov<float>(psb_ctx *, KrylovVecs<float> &, bool);
+template void ensure_krylov<double>(psb_ctx *, KrylovVecs<double> &, bool);
+
 void build_hier64(psb_ctx *C) {
   Hier<double> &H = C->h64;
   if (H.built) return;
   ensure_krylov(C, C->k64, false);
   hipStream_t stream = C->stream;
-  std::vector<int> d = level_dims(C->g.nx, C->g.ny, C->g.nz);
+  std::vector<int> d = level_dims(C->g.nx, C->g.ny, C->g.nz, C->coarse_min);
   const int nl = (int)d.size() / 3;
   if (nl < 2) throw std::invalid_argument("grid too small for the multigrid preconditioner");
   H.L.resize(nl);
@@ -1052,22 +929,9 @@ void scatter_host(const psb_layout &L, const std::vector<double> &src, double *d
     }
 }
 
-template <typename F>
-int guarded(psb_ctx *C, F f) {
-  try {
-    return f();
-  } catch (const HipError &e) {
-    if (C) C->err = hipGetErrorString(e.e);
-    return e.e == hipErrorOutOfMemory ? PSB_ENOMEM : PSB_EHIP;
-  } catch (const std::bad_alloc &) {
-    return PSB_ENOMEM;
-  } catch (const std::invalid_argument &e) {
-    if (C) C->err = e.what();
-    return PSB_EINVAL;
-  }
-}
 
-}  // namespace
+}  // namespace host
+}  // namespace psb
 
 // ===========================================================================
 // C ABI
@@ -1163,6 +1027,7 @@ int psb_set_geometry(psb_ctx *C, const double *dxc, const double *dyc, const dou
   return guarded(C, [&]() {
     HIPCHK(hipStreamSynchronize(C->stream));
     destroy_graphs(C);
+    C->version++;
     if (!dxc) {
       C->dxc = C->dyc = C->dzc = nullptr;
       return PSB_OK;
@@ -1232,6 +1097,7 @@ int psb_set_matrix(psb_ctx *C, const psb_matrix7 *A) {
     HIPCHK(hipStreamSynchronize(C->stream));
     C->sym = sym;
     C->have_matrix = true;
+    C->version++;
     return PSB_OK;
   });
 }
@@ -1249,6 +1115,7 @@ int psb_set_active(psb_ctx *C, const unsigned char *off) {
       for (int j = 0; j < L.ny; j++)
         for (int k = 0; k < L.nz; k++) C->off[q++] = off[host_index(L, i, j, k)];
   }
+  C->version++;
   return PSB_OK;
 }
 

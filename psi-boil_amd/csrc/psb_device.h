@@ -145,6 +145,25 @@ __device__ __forceinline__ int xcd_block() {
   return b;
 }
 
+// the blocks a streaming kernel body works with: block bl of nb (the whole
+// grid for one system; one system's segment of a batched grid)
+struct Span {
+  int bl, nb;
+};
+__device__ __forceinline__ Span grid_span() { return Span{xcd_block(), (int)gridDim.x}; }
+
+// Batched launches: one grid covers many systems.  Segment q owns blocks
+// [b0, b0 + nb) of the grid (b0 and nb multiples of 8, so the XCD remap
+// applies inside the segment); bmap[blockIdx.x] names the segment.
+struct BSeg {
+  int sys, b0, nb, pad;
+};
+__device__ __forceinline__ Span seg_span(const BSeg &s) {
+  int b = (int)blockIdx.x - s.b0;
+  if ((s.nb & 7) == 0) b = (b & 7) * (s.nb >> 3) + (b >> 3);
+  return Span{b, s.nb};
+}
+
 // ---------------------------------------------------------------------------
 // 7-point stencil gather.  For cell vector P (V consecutive k), returns
 //   cx = c*x_P,  s1 = aW xW + aE xE,  s2 = aS xS + aN xN,  s3 = aB xB + aT xT

@@ -316,3 +316,32 @@ extern "C" int psb_generate(const psb_case *cs, double *c, double *w, double *e,
   if (dzc_out) std::memcpy(dzc_out, Z.dxc.data(), nz * sizeof(double));
   return PSB_OK;
 }
+
+// Batch of pressure cases: per system, each dimension uniform in
+// {32, 48, 64, 96, 128}, 1-8 bubbles with radius uniform in 0.05-0.2 of the
+// shortest side, density ratio uniform in 10-1000 (bubbles of density 1 in
+// liquid of density `ratio`), and its own seeds for the bubbles and the
+// face velocities.  Draws come from one splitmix64 stream of `seed`.
+extern "C" int psb_batch_cases(unsigned long long seed, int nsys, psb_case *out) {
+  if (nsys < 0 || (nsys > 0 && !out)) return PSB_EINVAL;
+  static const int dims[5] = {32, 48, 64, 96, 128};
+  uint64_t q = 0;
+  for (int s = 0; s < nsys; s++) {
+    psb_case &c = out[s];
+    std::memset(&c, 0, sizeof(c));
+    c.nx = dims[std::min(4, (int)(5.0 * u01_at(seed, q++)))];
+    c.ny = dims[std::min(4, (int)(5.0 * u01_at(seed, q++)))];
+    c.nz = dims[std::min(4, (int)(5.0 * u01_at(seed, q++)))];
+    c.nsph = 1 + std::min(7, (int)(8.0 * u01_at(seed, q++)));
+    const int nmax = std::max(c.nx, std::max(c.ny, c.nz));
+    const int nmin = std::min(c.nx, std::min(c.ny, c.nz));
+    const double side = (double)nmin / nmax;  // h = 1 / max(n)
+    c.rmin = 0.05 * side;
+    c.rmax = 0.2 * side;
+    c.rho_in = 1.0;
+    c.rho_out = 10.0 + 990.0 * u01_at(seed, q++);
+    c.seed_sph = mix64(seed + 0x1000 + 2 * (uint64_t)s);
+    c.seed_vel = mix64(seed + 0x1001 + 2 * (uint64_t)s);
+  }
+  return PSB_OK;
+}

@@ -1,0 +1,194 @@
+// psb_internal.h -- host-side types shared by the single-system solver
+// (psb_api.hip) and the batched solver (psb_batch.hip): device arenas, the
+// additive-correction hierarchy, Krylov work vectors, captured graphs and
+// the context behind psb_ctx.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/psb.h"
+#include "psb_kernels.h"
+
+namespace psb {
+namespace host {
+
+struct HipError {
+  hipError_t e;
+};
+
+#define HIPCHK(x)                               \
+  do {                                          \
+    hipError_t e_ = (x);                        \
+    if (e_ != hipSuccess) throw HipError{e_};   \
+  } while (0)
+
+// blocks for a streaming kernel over `nwork` items: one pass of 256 threads
+// per item run, at most kMaxBlocks (8 per CU on the 256 CUs), a multiple of
+// the 8 XCDs so the XCD remap applies
+inline int grid_blocks(long long nwork) {
+  long long nb = (nwork + kBlock - 1) / kBlock;
+  if (nb > kMaxBlocks) nb = kMaxBlocks;
+  if (nb >= 8) nb = (nb / 8) * 8;
+  if (nb < 1) nb = 1;
+  return (int)nb;
+}
+
+inline FastDiv make_div(uint32_t d) {
+  uint32_t s = 0;
+  while ((1ull << s) < d) s++;
+  FastDiv f;
+  f.d = d;
+  f.m = (uint32_t)(((1ull << (31 + s)) + d - 1) / d);
+  f.sh = 31 + s;
+  return f;
+}
+
+inline Grid make_grid(int nx, int ny, int nz) {
+  Grid g;
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.sI = ny * nz;
+  g.N = nx * ny * nz;
+  g.dI = make_div((uint32_t)g.sI);
+  g.dZ = make_div((uint32_t)nz);
+  return g;
+}
+
+// device allocations owned by a context
+struct Arena {
+  std::vector<void *> ptrs;
+  template <typename T>
+  T *alloc(size_t n) {
+    void *p = nullptr;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    bytes = (bytes + 255) & ~size_t(255);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) throw HipError{e};
+    ptrs.push_back(p);
+    return static_cast<T *>(p);
+  }
+  void release() {
+    for (void *p : ptrs) (void)hipFree(p);
+    ptrs.clear();
+  }
+};
+
+template <typename T>
+struct Level {
+  Grid g;
+  int CI = 1, CJ = 1, CK = 1;      // factor from the finer level (1 or 2)
+  T *c = nullptr, *w = nullptr, *s = nullptr, *b = nullptr;
+  T *r = nullptr;                  // right-hand side of this level
+  T *e[3] = {nullptr, nullptr, nullptr};
+  uint8_t *aflag = nullptr;        // NULL: all active
+  DMat<T> mat() const { return DMat<T>{c, w, s, b, nullptr, nullptr, nullptr}; }
+};
+
+template <typename T>
+struct Hier {
+  std::vector<Level<T>> L;
+  double *G = nullptr;  // dense coarsest operator (FP64)
+  int m = 0;
+  bool built = false;
+};
+
+template <typename T>
+struct KrylovVecs {
+  T *x = nullptr, *r = nullptr, *q = nullptr, *p[2] = {nullptr, nullptr};
+  T *v[2] = {nullptr, nullptr}, *s = nullptr, *t = nullptr, *rt = nullptr;
+};
+
+struct Graph {
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  long long key = -1;
+  const void *px = nullptr, *pb = nullptr;  // caller arrays baked into the graph
+};
+
+}  // namespace host
+}  // namespace psb
+
+struct psb_ctx {
+  using Grid = psb::Grid;
+  using DevState = psb::DevState;
+  using Work = psb::Work;
+  using Arena = psb::host::Arena;
+  using Graph = psb::host::Graph;
+  template <typename T>
+  using Hier = psb::host::Hier<T>;
+  template <typename T>
+  using KrylovVecs = psb::host::KrylovVecs<T>;
+  psb_layout lay{};
+  Grid g{};
+  hipStream_t stream = nullptr;
+  Arena base;   // lives as long as the context
+  Arena arena;  // matrix, hierarchy and work vectors: rebuilt by set_matrix
+  // matrix
+  bool have_matrix = false, sym = true;
+  double *A64[7] = {};   // c w s b e n t
+  float *A32[4] = {};
+  bool have32 = false;
+  std::vector<uint8_t> off;  // host copy of the off mask (may be empty)
+  uint8_t *aflag0 = nullptr;
+  // geometry
+  double *dxc = nullptr, *dyc = nullptr, *dzc = nullptr;
+  // solver state
+  DevState *st64 = nullptr, *st32 = nullptr;
+  DevState *hst = nullptr;  // pinned host mirror
+  double *hist = nullptr;
+  int hist_cap = 0;
+  std::vector<double> hist_host;  // refinement: FP64 residual per outer step
+  bool hist_on_host = false;
+  int hist_count = 0;
+  Work work{};
+  KrylovVecs<double> k64;
+  KrylovVecs<float> k32;
+  Hier<double> h64;
+  Hier<float> h32;
+  double *db = nullptr, *dx = nullptr;  // staging for host-array solves
+  double *dy = nullptr;                 // scratch for psb_apply
+  double *relres = nullptr;             // 2 doubles: sum r^2, sum b^2
+  Graph gr_cgdiag, gr_bicgs, gr_ac64, gr_ac32, gr_ru;
+  double ac_omega = 0, ac_cfac = 0;
+  int coarse_min = 8;  // coarsening halves a direction while it stays >= this
+  int version = 0;     // bumped by every change of matrix, geometry or off mask
+  std::string err;
+};
+
+namespace psb {
+namespace host {
+
+DMat<double> mat64(const psb_ctx *c);
+Metric metric(const psb_ctx *c);
+std::vector<int> level_dims(int nx, int ny, int nz, int min_dim);
+template <typename T>
+void ensure_krylov(psb_ctx *C, KrylovVecs<T> &K, bool bicgs);
+void build_hier32(psb_ctx *C);
+void destroy_graph(Graph &gr);
+void finish_result(psb_result *R, const DevState &h, int outer);
+void gather_host(const psb_layout &L, const double *src, std::vector<double> &dst);
+void scatter_host(const psb_layout &L, const std::vector<double> &src, double *dst);
+
+// runs f() and maps exceptions to status codes (no exception crosses the ABI)
+template <typename F>
+int guarded(psb_ctx *C, F f) {
+  try {
+    return f();
+  } catch (const HipError &e) {
+    if (C) C->err = hipGetErrorString(e.e);
+    return e.e == hipErrorOutOfMemory ? PSB_ENOMEM : PSB_EHIP;
+  } catch (const std::bad_alloc &) {
+    return PSB_ENOMEM;
+  } catch (const std::invalid_argument &e) {
+    if (C) C->err = e.what();
+    return PSB_EINVAL;
+  }
+}
+
+}  // namespace host
+}  // namespace psb

@@ -28,17 +28,21 @@ struct Work {
   double *partial;  // kMaxBlocks * 4 doubles
 };
 
-// contiguous run of V-vectors per block; P, i, j, k of the first cell
-#define PSB_TILE_BEGIN(g, V)                                                  \
+// contiguous run of V-vectors per block of the span; P, i, j, k of the
+// first cell.  PSB_TILE_BEGIN spans the whole grid.
+#define PSB_SPAN_BEGIN(g, V, sp)                                              \
   const int nvec_ = (g).N / (V);                                              \
-  const int bl_ = xcd_block();                                                \
-  const int per_ = (nvec_ + (int)gridDim.x - 1) / (int)gridDim.x;             \
+  const int bl_ = (sp).bl;                                                    \
+  const int per_ = (nvec_ + (sp).nb - 1) / (sp).nb;                           \
   const int q0_ = bl_ * per_;                                                 \
   const int q1_ = min(q0_ + per_, nvec_);                                     \
   for (int q_ = q0_ + (int)threadIdx.x; q_ < q1_; q_ += kBlock) {             \
     const int P = q_ * (V);                                                   \
     int i, j, k;                                                              \
     (g).ijk(P, i, j, k);
+#define PSB_TILE_BEGIN(g, V)                                                  \
+  const Span tsp_ = grid_span();                                              \
+  PSB_SPAN_BEGIN(g, V, tsp_)
 #define PSB_TILE_END }
 
 // ---------------------------------------------------------------------------
@@ -107,13 +111,12 @@ __global__ __launch_bounds__(kBlock) void k_start(Grid g, DMat<T> A, Metric m, c
 // (cg.cpp:75-108)
 // ---------------------------------------------------------------------------
 template <typename T, int V, bool SYM, bool DIAG>
-__global__ __launch_bounds__(kBlock) void k_cg_pq(Grid g, DMat<T> A, const T *r_or_z,
-                                                  const T *pold, T *pnew, T *q,
-                                                  const uint8_t *af, DevState *st, Work w) {
-  if (st->done) return;
+__device__ __forceinline__ void cg_pq_body(Span sp, const Grid &g, const DMat<T> &A,
+                                           const T *r_or_z, const T *pold, T *pnew, T *q,
+                                           const uint8_t *af, const DevState *st, Work w) {
   const T beta = (T)st->beta;
   double acc[1] = {0.0};
-  PSB_TILE_BEGIN(g, V)
+  PSB_SPAN_BEGIN(g, V, sp)
   Gath<T, V> G;
   if (DIAG) {
     DiagPX<T, V> X{r_or_z, A.c, pold, beta};
@@ -131,19 +134,26 @@ __global__ __launch_bounds__(kBlock) void k_cg_pq(Grid g, DMat<T> A, const T *r_
   block_partial<1>(acc, w.partial);
 }
 
+template <typename T, int V, bool SYM, bool DIAG>
+__global__ __launch_bounds__(kBlock) void k_cg_pq(Grid g, DMat<T> A, const T *r_or_z,
+                                                  const T *pold, T *pnew, T *q,
+                                                  const uint8_t *af, DevState *st, Work w) {
+  if (st->done) return;
+  cg_pq_body<T, V, SYM, DIAG>(grid_span(), g, A, r_or_z, pold, pnew, q, af, st, w);
+}
+
 // ---------------------------------------------------------------------------
 // CG step 2: x += alpha p, r -= alpha q (cg.cpp:114-115); partials of the
 // residual norms and, in diagonal mode, rho_next = r.(r/c); preconditioned
 // mode writes e1 = omega r / c, the first Jacobi sweep of the next V-cycle.
 // ---------------------------------------------------------------------------
 template <typename T, int V, bool DIAG>
-__global__ __launch_bounds__(kBlock) void k_cg_xr(Grid g, const T *c, Metric m, const T *p,
-                                                  const T *q, T *x, T *r, T *e1, T omega,
-                                                  DevState *st, Work w) {
-  if (st->done) return;
+__device__ __forceinline__ void cg_xr_body(Span sp, const Grid &g, const T *c, const Metric &m,
+                                           const T *p, const T *q, T *x, T *r, T *e1, T omega,
+                                           const DevState *st, Work w) {
   const T alpha = (T)st->alpha;
   double acc[3] = {0.0, 0.0, 0.0};  // voldiv, r^2, rho_next
-  PSB_TILE_BEGIN(g, V)
+  PSB_SPAN_BEGIN(g, V, sp)
   VecN<T, V> X = ldv<T, V>(x + P), R = ldv<T, V>(r + P), Pp = ldv<T, V>(p + P),
              Q = ldv<T, V>(q + P), C = ldv<T, V>(c + P), E;
 #pragma unroll
@@ -168,6 +178,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_xr(Grid g, const T *c, Metric m, 
   block_partial<3>(acc, w.partial);
 }
 
+template <typename T, int V, bool DIAG>
+__global__ __launch_bounds__(kBlock) void k_cg_xr(Grid g, const T *c, Metric m, const T *p,
+                                                  const T *q, T *x, T *r, T *e1, T omega,
+                                                  DevState *st, Work w) {
+  if (st->done) return;
+  cg_xr_body<T, V, DIAG>(grid_span(), g, c, m, p, q, x, r, e1, omega, st, w);
+}
+
 // ---------------------------------------------------------------------------
 // damped Jacobi sweep  e_out = e + omega (r - A e) / c, the operand either e
 // or e + cfac P eH (prolongation fused into the first post-sweep).
@@ -178,12 +196,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_xr(Grid g, const T *c, Metric m, 
 // the correction).
 // ---------------------------------------------------------------------------
 template <typename T, int V, bool SYM, bool PROL, bool RHO>
-__global__ __launch_bounds__(kBlock) void k_jacobi(Grid g, DMat<T> A, const T *r, const T *e,
-                                                   T *eout, ProlX<T, V> pr, T omega,
-                                                   const uint8_t *af, DevState *st, Work w) {
-  if (st->done) return;
+__device__ __forceinline__ void jacobi_body(Span sp, const Grid &g, const DMat<T> &A,
+                                            const T *r, const T *e, T *eout,
+                                            const ProlX<T, V> &pr, T omega, const uint8_t *af,
+                                            Work w) {
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  PSB_TILE_BEGIN(g, V)
+  PSB_SPAN_BEGIN(g, V, sp)
   Gath<T, V> G;
   if (PROL) {
     G = gather<T, V, SYM>(g, A, pr, P, i, j, k);
@@ -208,6 +226,14 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(Grid g, DMat<T> A, const T *r
   if (RHO) block_partial<4>(acc, w.partial);
 }
 
+template <typename T, int V, bool SYM, bool PROL, bool RHO>
+__global__ __launch_bounds__(kBlock) void k_jacobi(Grid g, DMat<T> A, const T *r, const T *e,
+                                                   T *eout, ProlX<T, V> pr, T omega,
+                                                   const uint8_t *af, DevState *st, Work w) {
+  if (st->done) return;
+  jacobi_body<T, V, SYM, PROL, RHO>(grid_span(), g, A, r, e, eout, pr, omega, af, w);
+}
+
 // ---------------------------------------------------------------------------
 // restriction for the preconditioner: rH = sum over the fine block of
 // (r - A e)  (the residual sum of additive_restriction.cpp:12-135 without the
@@ -218,18 +244,16 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(Grid g, DMat<T> A, const T *r
 // cells are summed in (i, j, k) order within each coarse cell.
 // ---------------------------------------------------------------------------
 template <typename T, int V, int KV, bool SYM, int CI, int CJ, int CK>
-__global__ __launch_bounds__(kBlock) void k_restrict(Grid g, DMat<T> A, const T *r, const T *e,
-                                                     const uint8_t *af, Grid gH, const T *cH,
-                                                     const uint8_t *afH, T *rH, T *eH1,
-                                                     T omega, DevState *st) {
-  if (st->done) return;
+__device__ __forceinline__ void restrict_body(Span sp, const Grid &g, const DMat<T> &A,
+                                              const T *r, const T *e, const uint8_t *af,
+                                              const Grid &gH, const T *cH, const uint8_t *afH,
+                                              T *rH, T *eH1, T omega) {
   constexpr int KE = V * KV;        // fine cells along k per thread
   constexpr int NC = KE / CK;       // coarse cells per thread
   ArrX<T, V> X{e};
   const int nkv = g.nz / KE;
   const int nwork = gH.nx * gH.ny * nkv;
-  const int b = xcd_block();
-  for (int q = b * kBlock + threadIdx.x; q < nwork; q += gridDim.x * kBlock) {
+  for (int q = sp.bl * kBlock + threadIdx.x; q < nwork; q += sp.nb * kBlock) {
     const int IJ = q / nkv, kv = q - IJ * nkv;
     const int I = IJ / gH.ny, J = IJ - I * gH.ny;
     T s[NC];
@@ -267,20 +291,35 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid g, DMat<T> A, const T 
   }
 }
 
+template <typename T, int V, int KV, bool SYM, int CI, int CJ, int CK>
+__global__ __launch_bounds__(kBlock) void k_restrict(Grid g, DMat<T> A, const T *r, const T *e,
+                                                     const uint8_t *af, Grid gH, const T *cH,
+                                                     const uint8_t *afH, T *rH, T *eH1,
+                                                     T omega, DevState *st) {
+  if (st->done) return;
+  restrict_body<T, V, KV, SYM, CI, CJ, CK>(grid_span(), g, A, r, e, af, gH, cH, afH, rH, eH1,
+                                           omega);
+}
+
 // coarsest level: z = G r, G the dense projected pseudo-inverse (FP64).
 // One wave per row; r is read in T and promoted.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_gemv(int m, const double *Gm, const T *r, T *z,
-                                                 DevState *st) {
-  if (st->done) return;
+__device__ __forceinline__ void gemv_body(int bl, int m, const double *Gm, const T *r, T *z) {
   const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int row = bl * kWaves + (threadIdx.x >> 6);
   if (row >= m) return;
   const double *g = Gm + (size_t)row * m;
   double s = 0.0;
   for (int c = lane; c < m; c += 64) s += g[c] * (double)r[c];
   s = wave_sum(s);
   if (lane == 0) z[row] = (T)s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gemv(int m, const double *Gm, const T *r, T *z,
+                                                 DevState *st) {
+  if (st->done) return;
+  gemv_body<T>(blockIdx.x, m, Gm, r, z);
 }
 
 // ---------------------------------------------------------------------------
@@ -344,10 +383,8 @@ __global__ __launch_bounds__(kBlock) void k_center(Grid g, const float *c, const
 // is pending (gated = true) or they start the solve (gated = false).
 // ---------------------------------------------------------------------------
 template <int V>
-__global__ __launch_bounds__(kBlock) void k_ru_xupdate(Grid g, double *x, float *d,
-                                                       const DevState *st, bool gated) {
-  if (gated && !st->update) return;
-  PSB_TILE_BEGIN(g, V)
+__device__ __forceinline__ void ru_xupdate_body(Span sp, const Grid &g, double *x, float *d) {
+  PSB_SPAN_BEGIN(g, V, sp)
 #pragma unroll
   for (int u = 0; u < V; u++) {
     x[P + u] = x[P + u] + (double)d[P + u];
@@ -356,18 +393,23 @@ __global__ __launch_bounds__(kBlock) void k_ru_xupdate(Grid g, double *x, float 
   PSB_TILE_END
 }
 
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_ru_xupdate(Grid g, double *x, float *d,
+                                                       const DevState *st, bool gated) {
+  if (gated && !st->update) return;
+  ru_xupdate_body<V>(grid_span(), g, x, d);
+}
+
 // r32 = (float)(b - A x) in FP64, e1 = omega r32 / c32; partials of the FP64
 // norms (L2 and reference) and of the FP32 residual
 template <int V, bool SYM>
-__global__ __launch_bounds__(kBlock) void k_ru_resid(Grid g, DMat<double> A, Metric m,
-                                                     const double *b, const double *x,
-                                                     const float *c32, float *r32, float *e1,
-                                                     float omega, const DevState *st,
-                                                     bool gated, Work w) {
-  if (gated && !st->update) return;
+__device__ __forceinline__ void ru_resid_body(Span sp, const Grid &g, const DMat<double> &A,
+                                              const Metric &m, const double *b, const double *x,
+                                              const float *c32, float *r32, float *e1,
+                                              float omega, Work w) {
   ArrX<double, V> X{x};
   double acc[3] = {0.0, 0.0, 0.0};  // r64^2, r64^2/dV^2, r32^2
-  PSB_TILE_BEGIN(g, V)
+  PSB_SPAN_BEGIN(g, V, sp)
   Gath<double, V> G = gather<double, V, SYM>(g, A, X, P, i, j, k);
   VecN<double, V> R = resid<double, V>(ldv<double, V>(b + P), G);
   VecN<float, V> Cf = ldv<float, V>(c32 + P), Rf, E;
@@ -386,6 +428,16 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid(Grid g, DMat<double> A, Met
   block_partial<3>(acc, w.partial);
 }
 
+template <int V, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_ru_resid(Grid g, DMat<double> A, Metric m,
+                                                     const double *b, const double *x,
+                                                     const float *c32, float *r32, float *e1,
+                                                     float omega, const DevState *st,
+                                                     bool gated, Work w) {
+  if (gated && !st->update) return;
+  ru_resid_body<V, SYM>(grid_span(), g, A, m, b, x, c32, r32, e1, omega, w);
+}
+
 // x64 += (double) d32, d32 = 0
 template <int V>
 __global__ __launch_bounds__(kBlock) void k_ir_update(Grid g, double *x, float *d) {
@@ -400,11 +452,11 @@ __global__ __launch_bounds__(kBlock) void k_ir_update(Grid g, double *x, float *
 
 // true relative residual ||b - A x|| / ||b|| (FP64 diagnostic after a solve)
 template <int V, bool SYM>
-__global__ __launch_bounds__(kBlock) void k_relres(Grid g, DMat<double> A, const double *b,
-                                                   const double *x, Work w) {
+__device__ __forceinline__ void relres_body(Span sp, const Grid &g, const DMat<double> &A,
+                                            const double *b, const double *x, Work w) {
   ArrX<double, V> X{x};
   double acc[2] = {0.0, 0.0};
-  PSB_TILE_BEGIN(g, V)
+  PSB_SPAN_BEGIN(g, V, sp)
   Gath<double, V> G = gather<double, V, SYM>(g, A, X, P, i, j, k);
   VecN<double, V> B = ldv<double, V>(b + P), R = resid<double, V>(B, G);
 #pragma unroll
@@ -414,6 +466,12 @@ __global__ __launch_bounds__(kBlock) void k_relres(Grid g, DMat<double> A, const
   }
   PSB_TILE_END
   block_partial<2>(acc, w.partial);
+}
+
+template <int V, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_relres(Grid g, DMat<double> A, const double *b,
+                                                   const double *x, Work w) {
+  relres_body<V, SYM>(grid_span(), g, A, b, x, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -617,31 +675,36 @@ __device__ inline void krylov_init(DevState *st, double res, double rsq, double 
   if (st->stalecount > 0) st->stale[st->stalecount - 1] = res;
 }
 
-template <int NR>
-__global__ __launch_bounds__(kBlock) void k_finalize(int op, int nblk, const double *partial,
-                                                     DevState *st, DevState *st2, double *out) {
-  switch (op) {  // steps of a finished loop (or a refresh not pending) do nothing
+// steps of a finished loop (or a refresh not pending) do nothing
+__device__ inline bool fin_skip(int op, const DevState *st) {
+  switch (op) {
     case FIN_PQ: case FIN_XR_DIAG: case FIN_XR_AC: case FIN_RHO: case FIN_XR_RU:
     case FIN_BI1: case FIN_BI2: case FIN_BI3:
-      if (st->done) return;
-      break;
+      return st->done != 0;
     case FIN_RU:
-      if (st->done || !st->update) return;
-      break;
+      return st->done || !st->update;
     default:
-      break;
+      return false;
   }
+}
+
+// sum of the partials of blocks [b0, b0 + nblk) in a fixed order (all
+// threads of the block receive the totals)
+template <int NR>
+__device__ inline void fin_sum(int b0, int nblk, const double *partial, double (&acc)[NR]) {
   __shared__ double sm[kWaves][NR];
-  double acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; r++) acc[r] = 0.0;
   for (int q = threadIdx.x; q < nblk; q += kBlock) {
 #pragma unroll
-    for (int r = 0; r < NR; r++) acc[r] += partial[(size_t)q * NR + r];
+    for (int r = 0; r < NR; r++) acc[r] += partial[(size_t)(b0 + q) * NR + r];
   }
   block_sum<NR>(acc, sm);
-  if (threadIdx.x != 0) return;
-  const double *tot = acc;
+}
+
+// the scalar step of op on the totals (one thread)
+__device__ inline void fin_apply(int op, const double *tot, DevState *st, DevState *st2,
+                                 double *out) {
   switch (op) {
     case FIN_START_DIAG:
     case FIN_START_AC:
@@ -790,6 +853,15 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int op, int nblk, const dou
   }
 }
 
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_finalize(int op, int nblk, const double *partial,
+                                                     DevState *st, DevState *st2, double *out) {
+  if (fin_skip(op, st)) return;
+  double acc[NR];
+  fin_sum<NR>(0, nblk, partial, acc);
+  if (threadIdx.x == 0) fin_apply(op, acc, st, st2, out);
+}
+
 // ---------------------------------------------------------------------------
 // set-up kernels
 // ---------------------------------------------------------------------------
@@ -800,7 +872,7 @@ __global__ void k_fill(T *x, T v, int n) {
 
 // plain copy of n floats (n % 4 == 0 for the vector part): the reference
 // stream for calibrating hardware byte counters
-__global__ __launch_bounds__(kBlock) void k_copy(const float *x, float *y, int n) {
+static __global__ __launch_bounds__(kBlock) void k_copy(const float *x, float *y, int n) {
   const int nv = n / 4;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nv; q += gridDim.x * blockDim.x)
     stv<float, 4>(y + 4 * q, ldv<float, 4>(x + 4 * q));
@@ -808,7 +880,7 @@ __global__ __launch_bounds__(kBlock) void k_copy(const float *x, float *y, int n
     y[q] = x[q];
 }
 
-__global__ void k_to_f32(const double *a, float *b, int n) {
+static __global__ void k_to_f32(const double *a, float *b, int n) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
     b[q] = (float)a[q];
 }
@@ -818,7 +890,7 @@ __global__ void k_to_f32(const double *a, float *b, int n) {
 // additive_coarsen_system.cpp:103-219 (c summed, internal e/w per (j,k),
 // n/s per (i,k), t/b per (i,j); external couplings summed over the faces).
 // The fine matrix is symmetric 4-array; so is the coarse one.
-__global__ void k_galerkin(Grid g, DMat<double> A, Grid gH, int CI, int CJ, int CK,
+static __global__ void k_galerkin(Grid g, DMat<double> A, Grid gH, int CI, int CJ, int CK,
                            double *cH, double *wH, double *sH, double *bH) {
   for (int Q = blockIdx.x * blockDim.x + threadIdx.x; Q < gH.N; Q += gridDim.x * blockDim.x) {
     int I, J, K;
@@ -871,7 +943,7 @@ __global__ void k_galerkin(Grid g, DMat<double> A, Grid gH, int CI, int CJ, int 
 }
 
 // active flag of a coarse cell: OR over its block (additive_coarsen_flag.cpp)
-__global__ void k_coarsen_flag(Grid g, const uint8_t *f, Grid gH, int CI, int CJ, int CK,
+static __global__ void k_coarsen_flag(Grid g, const uint8_t *f, Grid gH, int CI, int CJ, int CK,
                                uint8_t *fH) {
   for (int Q = blockIdx.x * blockDim.x + threadIdx.x; Q < gH.N; Q += gridDim.x * blockDim.x) {
     int I, J, K;

@@ -72,7 +72,9 @@ class Case(C.Structure):
 EXPORTS = ["psb_stop_defaults", "psb_status_string", "psb_create", "psb_destroy",
            "psb_set_geometry", "psb_set_matrix", "psb_set_active", "psb_solve",
            "psb_solve_device", "psb_apply", "psb_history", "psb_generate",
-           "psb_time_kernel"]
+           "psb_time_kernel", "psb_batch_cases", "psb_batch_create", "psb_batch_destroy",
+           "psb_batch_system", "psb_batch_set_tuning", "psb_batch_plan", "psb_batch_solve",
+           "psb_batch_solve_device"]
 
 _lib = None
 
@@ -85,6 +87,7 @@ def lib():
             raise PsbError(f"{LIB_PATH} is missing: build it with psi-boil_amd/build.py")
         L = C.CDLL(LIB_PATH)
         L.psb_status_string.restype = C.c_char_p
+        L.psb_batch_system.restype = C.c_void_p
         for name in EXPORTS:
             getattr(L, name)
         _lib = L
@@ -114,17 +117,21 @@ def _check(rc, what):
 class Solver:
     """One solver context (device buffers, hierarchy, stream, graphs)."""
 
-    def __init__(self, nx, ny, nz, ghost=0):
+    def __init__(self, nx, ny, nz, ghost=0, _ctx=None):
         self.shape = (nx, ny, nz)
         self.ghost = ghost
+        self._owned = _ctx is None
+        if _ctx is not None:  # a context owned by a Batch
+            self._c = C.c_void_p(_ctx)
+            return
         self._c = C.c_void_p()
         _check(lib().psb_create(C.byref(self._c), C.byref(Layout(nx, ny, nz, ghost))),
                "psb_create")
 
     def close(self):
-        if self._c:
+        if self._c and self._owned:
             lib().psb_destroy(self._c)
-            self._c = C.c_void_p()
+        self._c = C.c_void_p()
 
     def __del__(self):
         try:
@@ -277,3 +284,83 @@ class DeviceArray:
 
 def device_synchronize():
     _hipchk(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+# --------------------------------------------------------------------------
+# batched solves
+# --------------------------------------------------------------------------
+def batch_cases(seed, nsys):
+    """The seeded batch of cases (list of dicts of generate() arguments)."""
+    arr = (Case * nsys)()
+    _check(lib().psb_batch_cases(C.c_ulonglong(seed), nsys, arr), "psb_batch_cases")
+    out = []
+    for c in arr:
+        out.append(dict(nx=c.nx, ny=c.ny, nz=c.nz, nsph=c.nsph, rmin=c.rmin, rmax=c.rmax,
+                        rho_in=c.rho_in, rho_out=c.rho_out, seed_sph=c.seed_sph,
+                        seed_vel=c.seed_vel))
+    return out
+
+
+class Batch:
+    """Many systems solved together (psb_batch_*).  systems[s] is a Solver view
+    of system s's context for set_geometry / set_active / set_matrix."""
+
+    def __init__(self, shapes, ghost=0):
+        self.shapes = [tuple(sh) for sh in shapes]
+        lays = (Layout * len(shapes))(*[Layout(*sh, ghost) for sh in self.shapes])
+        self._b = C.c_void_p()
+        _check(lib().psb_batch_create(C.byref(self._b), len(shapes), lays), "psb_batch_create")
+        self.systems = [Solver(*sh, ghost=ghost,
+                               _ctx=lib().psb_batch_system(self._b, q))
+                        for q, sh in enumerate(self.shapes)]
+
+    def close(self):
+        if self._b:
+            for s in self.systems:
+                s.close()
+            lib().psb_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, vpt=0):
+        return _check(lib().psb_batch_set_tuning(self._b, vpt), "psb_batch_set_tuning")
+
+    def plan(self, cap=1024):
+        t = (C.c_int * (3 * cap))()
+        n = _check(lib().psb_batch_plan(self._b, t, cap), "psb_batch_plan")
+        return [tuple(t[3 * q:3 * q + 3]) for q in range(min(n, cap))]
+
+    def _results(self, R):
+        return [r.asdict() for r in R]
+
+    def solve(self, xs, bs, **stop):
+        n = len(self.shapes)
+        xs = [np.array(s._host(x), copy=True) for s, x in zip(self.systems, xs)]
+        bs = [s._host(b) for s, b in zip(self.systems, bs)]
+        px = (C.c_void_p * n)(*[x.ctypes.data for x in xs])
+        pb = (C.c_void_p * n)(*[b.ctypes.data for b in bs])
+        st = stop_defaults(**dict(BATCH_STOP, **stop))
+        R = (Result * n)()
+        rc = _check(lib().psb_batch_solve(self._b, px, pb, C.byref(st), R), "psb_batch_solve")
+        return xs, self._results(R), rc
+
+    def solve_device(self, d_xs, d_bs, **stop):
+        """d_xs, d_bs: device pointers (int) to compact FP64 arrays per system."""
+        n = len(self.shapes)
+        px = (C.c_void_p * n)(*d_xs)
+        pb = (C.c_void_p * n)(*d_bs)
+        st = stop_defaults(**dict(BATCH_STOP, **stop))
+        R = (Result * n)()
+        rc = _check(lib().psb_batch_solve_device(self._b, px, pb, C.byref(st), R),
+                    "psb_batch_solve_device")
+        return self._results(R), rc
+
+
+# the batched path's method: FP32 AC-PCG with FP64 reliable updates
+BATCH_STOP = dict(precond=PSB_PREC_AC, method=PSB_CG, precision=PSB_MIXED, inner_adapt=2,
+                  inner_rtol=1e-2, norm=PSB_NORM_L2REL, res_tol=0.0)

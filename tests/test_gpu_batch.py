@@ -1,0 +1,132 @@
+"""Batched solves (psb_batch_*, through the C ABI) against the oracle's FP64
+AC-PCG on the same hierarchy (halve while >= 4) and against the library's
+own single-system solves.
+
+Tolerances: the batched method is the FP32 AC-PCG with FP64 reliable
+updates; each system reaches ||b - A x|| <= tol ||b|| (checked in FP64), its
+iteration count equals that of the same method solving the system alone
+(+-1: only the block partition of the dot products differs), stays within 6
+of the oracle's FP64 AC-PCG on the same hierarchy (the FP32 preconditioner
+costs a few iterations on some shapes), and its solution agrees with the
+oracle's FP64 solve to 1e-12 within 1e-6 relative (after removing the mean,
+the Neumann null space)."""
+import numpy as np
+import pytest
+
+import psb
+import psb_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(32, 32, 32), (48, 32, 16), (16, 48, 32), (32, 16, 64), (24, 24, 24)]
+
+
+def make_batch(shapes, seed=11):
+    B = psb.Batch(shapes)
+    data = []
+    for q, (nx, ny, nz) in enumerate(shapes):
+        side = min(nx, ny, nz) / max(nx, ny, nz)
+        d = psb.generate(nx, ny, nz, 1 + q % 4, 0.05 * side, 0.2 * side, 1.0,
+                         [1000.0, 10.0, 300.0][q % 3], seed + q, 1000 + seed + q)
+        B.systems[q].set_matrix(d)
+        B.systems[q].set_geometry(d["dxc"], d["dyc"], d["dzc"])
+        data.append(d)
+    return B, data
+
+
+def demean(x):
+    return x - x.mean()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_batched_solve_matches_oracle_per_system():
+    B, data = make_batch(SHAPES)
+    xs, res, rc = B.solve([np.zeros(d["rhs"].size) for d in data], [d["rhs"] for d in data],
+                          res_rat=1e-8, max_iter=300)
+    assert rc == psb.PSB_OK
+    for q, ((nx, ny, nz), d, x, r) in enumerate(zip(SHAPES, data, xs, res)):
+        A = O.Mat(nx, ny, nz, {k: d[k] for k in "cwesnbt"})
+        n = d["rhs"].size
+        ac = O.AC(A, np.ones(n), rule=2)
+        _, i8 = ac.acpcg(np.zeros(n), d["rhs"], rtol=1e-8, maxit=300, nu1=2, nu2=2,
+                         omega=0.9, cfac=2.0)
+        xo, i12 = ac.acpcg(np.zeros(n), d["rhs"], rtol=1e-12, maxit=600, nu1=2, nu2=2,
+                           omega=0.9, cfac=2.0)
+        _, r1 = B.systems[q].solve(np.zeros(n), d["rhs"],
+                                   **dict(psb.BATCH_STOP, res_rat=1e-8, max_iter=300))
+        assert r["converged"] and r["status"] == psb.PSB_OK, r
+        assert r["rel_l2"] <= 1e-8 * 1.01
+        assert abs(r["res"] / r["res0"] - r["rel_l2"]) <= 1e-6 * r["rel_l2"]
+        assert abs(r["iterations"] - r1["iterations"]) <= 1, (r["iterations"], r1["iterations"])
+        assert 0 <= r["iterations"] - i8["iters"] + 1 <= 7, (r["iterations"], i8["iters"])
+        assert rel(demean(x), demean(xo)) < 1e-6
+
+
+def test_batched_per_system_stopping():
+    shapes = SHAPES[:3]
+    B, data = make_batch(shapes, seed=21)
+    bs = [d["rhs"] for d in data]
+    bs[1] = np.zeros_like(bs[1])  # converged before the first iteration
+    xs, res, rc = B.solve([np.zeros(b.size) for b in bs], bs, res_rat=1e-8, max_iter=300)
+    assert rc == psb.PSB_OK
+    assert res[1]["converged"] and res[1]["iterations"] == 0 and np.all(xs[1] == 0.0)
+    assert res[0]["iterations"] > 5 and res[2]["iterations"] > 5
+    # a cap on the iterations stops every system with MAXITER (count maxit + 1)
+    bs[1] = data[1]["rhs"]
+    xs, res, rc = B.solve([np.zeros(b.size) for b in bs], bs, res_rat=1e-12, max_iter=3)
+    assert rc == psb.PSB_MAXITER
+    for r in res:
+        assert r["status"] == psb.PSB_MAXITER and r["iterations"] == 4
+        assert abs(r["res"] / r["res0"] - r["rel_l2"]) <= 1e-6 * r["rel_l2"]
+
+
+def test_batched_tuning_variants_agree():
+    B, data = make_batch(SHAPES, seed=31)
+    bs = [d["rhs"] for d in data]
+    runs = []
+    for vpt in (1, 8, 0):
+        B.set_tuning(vpt)
+        xs, res, rc = B.solve([np.zeros(b.size) for b in bs], bs, res_rat=1e-8, max_iter=300)
+        assert rc == psb.PSB_OK
+        runs.append((xs, res))
+    plan = B.plan()
+    assert len(plan) > 0 and all(v in (1, 2, 4, 8) for _, _, v in plan)
+    for xs, res in runs[1:]:
+        for q in range(len(SHAPES)):
+            assert abs(res[q]["iterations"] - runs[0][1][q]["iterations"]) <= 1
+            assert rel(demean(xs[q]), demean(runs[0][0][q])) < 1e-7
+
+
+def test_batched_odd_sizes_and_device_arrays():
+    shapes = [(24, 20, 30), (16, 16, 17), (20, 12, 36)]  # nz not a multiple of 4: V = 1
+    B, data = make_batch(shapes, seed=41)
+    bs = [d["rhs"] for d in data]
+    xs, res, rc = B.solve([np.zeros(b.size) for b in bs], bs, res_rat=1e-8, max_iter=300)
+    assert rc == psb.PSB_OK and all(r["rel_l2"] <= 1e-8 * 1.01 for r in res)
+    dxs = [psb.DeviceArray(b.size) for b in bs]
+    dbs = [psb.DeviceArray.from_host(b) for b in bs]
+    for a in dxs:
+        a.zero()
+    res2, rc2 = B.solve_device([a.addr for a in dxs], [a.addr for a in dbs], res_rat=1e-8,
+                               max_iter=300)
+    assert rc2 == psb.PSB_OK
+    for q in range(len(shapes)):
+        assert res2[q]["iterations"] == res[q]["iterations"]
+        assert np.array_equal(dxs[q].download(), xs[q])
+
+
+def test_batched_plan_follows_matrix_changes():
+    B, data = make_batch(SHAPES[:2], seed=51)
+    bs = [d["rhs"] for d in data]
+    B.solve([np.zeros(b.size) for b in bs], bs, res_rat=1e-8, max_iter=300)
+    nx, ny, nz = SHAPES[0]
+    d = psb.generate(nx, ny, nz, 3, 0.05, 0.2, 1.0, 500.0, 99, 1099)
+    B.systems[0].set_matrix(d)
+    bs[0] = d["rhs"]
+    xs, res, rc = B.solve([np.zeros(b.size) for b in bs], bs, res_rat=1e-8, max_iter=300)
+    assert rc == psb.PSB_OK and res[0]["rel_l2"] <= 1e-8 * 1.01
+    A = O.Mat(nx, ny, nz, {k: d[k] for k in "cwesnbt"})
+    assert np.linalg.norm(O.resid(A, d["rhs"], xs[0])) <= 1.01e-8 * np.linalg.norm(d["rhs"])
