@@ -64,7 +64,8 @@ def measured_traffic(n):
     path = os.path.join(ROOT, "profiles", "jacobi_spmv_256_hbm_traffic.md")
     if n != 256 or not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch k_jacobi<float,4> (\d+)", open(path).read())
+    m = re.search(r"traffic_bytes_per_launch k_jacobi<float,4,fp16,flux> (\d+)",
+                  open(path).read())
     return int(m.group(1)) if m else None
 
 
@@ -101,10 +102,11 @@ EXTRA = {
     "cyl": dict(grid=(384, 384, 768), gen=dict(nsph=128, rmin=0.01, rmax=0.04, rho_in=1.0,
                                                 rho_out=1000.0, seed_sph=5, seed_vel=1005,
                                                 stretch_z=2.5, lz=2.0, cylinder=True, cyl_d=0.2),
-                tol=1e-9,
+                tol=1e-9, stop=dict(ac_coarsen=1, ac_cfac=1.0),
                 desc="pressure Poisson 384x384x768 on [0,1]x[0,1]x[0,2], tanh z-stretching "
                      "2.5, solid cylinder d 0.2 along y, 128 bubbles r 0.01-0.04, density "
-                     "ratio 1000; AC-PCG to 1e-9"),
+                     "ratio 1000; AC-PCG to 1e-9 (levels halve x and y before z, coarse "
+                     "correction factor 1)"),
 }
 
 
@@ -123,7 +125,7 @@ def extra_case(a):
     dx = psb.DeviceArray(nx * ny * nz)
     del d
     common = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=E["tol"],
-                  res_tol=0.0, max_iter=2000)
+                  res_tol=0.0, max_iter=2000, **E.get("stop", {}))
     runs = {"fp64": dict(precision=psb.PSB_FP64),
             "mixed_restarted_1e-4": dict(precision=psb.PSB_MIXED, inner_rtol=1e-4,
                                          inner_adapt=0, max_outer=60),
@@ -260,6 +262,11 @@ def main():
     s.set_matrix(d)
     s.set_geometry(d["dxc"], d["dyc"], d["dzc"])
     if a.profile_kernels:
+        # the mixed solve's hierarchy (FP16 smoother copies) for the Jacobi sweep
+        dx0 = psb.DeviceArray(n ** 3)
+        db0 = psb.DeviceArray.from_host(d["rhs"])
+        s.solve_device(dx0.addr, db0.addr, precond=psb.PSB_PREC_AC, precision=psb.PSB_MIXED,
+                       inner_adapt=2, inner_rtol=1e-2, ac_store=1, max_iter=1)
         for kern, name in ((2, "jacobi_fp32"), (1, "spmv_fp32"), (0, "spmv_fp64"), (3, "copy_fp32")):
             ms, by = s.time_kernel(kern, reps=10)
             log(f"[bench] {name}: {ms * 1e3:.1f} us/launch, {by / 1e6:.1f} MB algorithmic, "
@@ -274,7 +281,7 @@ def main():
     log(f"[bench] fp64 {t64 * 1e3:.2f} ms/solve {r64}")
     log("[bench] mixed AC-PCG, FP32 iterations with FP64 reliable updates")
     tmx, rmx = timed_solves(s, dx, db, a.steps, a.warmup, precision=psb.PSB_MIXED,
-                            inner_rtol=1e-2, inner_adapt=2, **common)
+                            inner_rtol=1e-2, inner_adapt=2, ac_store=1, **common)
     log(f"[bench] mixed {tmx * 1e3:.2f} ms/solve {rmx}")
     x_mixed = dx.download()
     # FP64 reference solution to 1e-12 for the error figure (untimed)
@@ -288,7 +295,8 @@ def main():
     ms_s32, by_s32 = s.time_kernel(1, reps=50)
     ms_s64, by_s64 = s.time_kernel(0, reps=50)
     gbs = lambda by, ms: by / (ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": "k_jacobi<float,4> (fine-level damped Jacobi sweep)",
+    roof = {"bound": "hbm", "kernel": "k_jacobi<float,4,...,_Float16,flux> (fine-level "
+                                      "damped Jacobi sweep, FP16 matrix copy)",
             "achieved": round(gbs(by_j, ms_j), 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs(by_j, ms_j) / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n),
             "avg_launch_ms": ms_j, "bytes_per_launch": by_j,
@@ -314,12 +322,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 iterations / fp64 residual updates",
+        "dtype": "fp32 iterations / fp64 residual updates (fp16 smoother matrix)",
         "data": "synthetic (seeded bubbles, seeded face velocities)",
         "config": {"workload": f"pressure Poisson {n}^3, 16 bubbles r 0.03-0.12, density "
                                "ratio 1000, Neumann walls; AC-preconditioned CG (2+2 damped "
                                "Jacobi, 2x coarsening to 8^3, exact FP64 coarsest) to 1e-8; "
-                               "FP32 CG with an FP64 true-residual update per 1e-2 drop",
+                               "FP32 CG with an FP64 true-residual update per 1e-2 drop, "
+                               "FP16 copy of the level matrices in the smoother",
                    "grid": [n, n, n], "parallelism": "single GPU"},
         "fp64_solves_per_s": round(1.0 / t64, 3),
         "fp64_ms_per_solve": round(t64 * 1e3, 3),

@@ -99,6 +99,14 @@ typedef struct {
                             by inner_rtol below the last true one, x += d in
                             FP64 and r = b - A x (FP64) replaces it; the stop
                             tests use that true residual in the chosen norm */
+  int ac_coarsen;        /* AC levels: 0 halve every direction that stays
+                            >= 8 (default); 1 halve x and y first, z only
+                            once x and y stop (grids stretched in z)       */
+  int ac_store;          /* PSB_MIXED smoother matrix: 0 FP32 (default);
+                            1 FP16 copy scaled by a power of two per level
+                            (half the smoother's matrix bytes; the CG
+                            operator and the residual updates keep FP32 /
+                            FP64)                                          */
 } psb_stop;
 
 typedef struct {
@@ -228,8 +236,10 @@ int psb_batch_solve_device(psb_batch *bt, double *const *d_x,
 
 /* Benchmark hook: average device time (ms) of `reps` back-to-back launches
  * of one kernel on the context's stream, bracketed by HIP events.  kernel:
- * 0 FP64 SpMV, 1 FP32 SpMV, 2 FP32 damped Jacobi sweep, 3 plain FP32 copy
- * of the grid (reference stream for hardware byte counters).
+ * 0 FP64 SpMV, 1 FP32 SpMV, 2 FP32 damped Jacobi sweep on the fine level as
+ * the last mixed-precision solve set it up (flux form; FP16 matrix copy with
+ * ac_store = 1), 3 plain FP32 copy of the grid (reference stream for
+ * hardware byte counters).
  * *bytes receives the algorithmic bytes one launch moves.                   */
 int psb_time_kernel(psb_ctx *ctx, int kernel, int reps, double *avg_ms,
                     double *bytes);

@@ -18,11 +18,17 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
 
 #include "psb_internal.h"
+
+// FP32 levels evaluate the stencil in flux form (set to 0 for the plain form)
+#ifndef PSB_FLUX
+#define PSB_FLUX 1
+#endif
 
 using namespace psb;
 
@@ -99,12 +105,12 @@ void start(psb_ctx *C, const Grid &g, DMat<T> A, const T *b, const T *x, T *r, T
               nblocks<T>(g), st);
 }
 
-template <typename T, bool SYM, bool DIAG>
+template <typename T, bool SYM, bool DIAG, bool FLUX = false>
 void cg_pq(psb_ctx *C, const Grid &g, DMat<T> A, const T *rz, const T *pold, T *pnew, T *q,
            const uint8_t *af, DevState *st) {
   with_v<T>(g, [&](auto vt) {
     constexpr int V = decltype(vt)::value;
-    hipLaunchKernelGGL((k_cg_pq<T, V, SYM, DIAG>), dim3(nblocks<T>(g)), dim3(kBlock), 0,
+    hipLaunchKernelGGL((k_cg_pq<T, V, SYM, DIAG, FLUX>), dim3(nblocks<T>(g)), dim3(kBlock), 0,
                        C->stream, g, A, rz, pold, pnew, q, af, st, C->work);
   });
   finalize<1>(C, FIN_PQ, nblocks<T>(g), st);
@@ -123,9 +129,13 @@ void cg_xr(psb_ctx *C, const Grid &g, const T *c, const T *p, const T *q, T *x, 
 
 // one damped Jacobi sweep on level lv (optionally with the prolongation of
 // the coarse correction eH fused in; RHO: last post-sweep on level 0)
-template <typename T, bool RHO>
-void jacobi(psb_ctx *C, const Level<T> &lv, const T *e, T *eout, const Level<T> *coarse,
-            const T *eH, T cfac, T omega, DevState *st) {
+// one damped Jacobi sweep on level lv (optionally with the prolongation of
+// the coarse correction eH fused in; RHO: last post-sweep on level 0, its
+// output multiplied by zscale).  With a half-precision copy of the level
+// matrix (lv.c16, scaled by lv.scale) the sweep reads that copy.
+template <typename T, bool RHO, typename M, bool FLUX>
+void jacobi_m(psb_ctx *C, const Level<T> &lv, DMat<M> A, const T *e, T *eout,
+              const Level<T> *coarse, const T *eH, T cfac, T omega, T zscale, DevState *st) {
   const Grid &g = lv.g;
   with_v<T>(g, [&](auto vt) {
     constexpr int V = decltype(vt)::value;
@@ -140,22 +150,65 @@ void jacobi(psb_ctx *C, const Level<T> &lv, const T *e, T *eout, const Level<T> 
       pr.sK = coarse->CK == 2 ? 1 : 0;
       pr.nYH = coarse->g.ny;
       pr.nZH = coarse->g.nz;
-      hipLaunchKernelGGL((k_jacobi<T, V, true, true, RHO>), dim3(nblocks<T>(g)), dim3(kBlock), 0,
-                         C->stream, g, lv.mat(), lv.r, e, eout, pr, omega, lv.aflag, st,
-                         C->work);
+      hipLaunchKernelGGL((k_jacobi<T, V, true, true, RHO, M, FLUX>), dim3(nblocks<T>(g)),
+                         dim3(kBlock), 0, C->stream, g, A, lv.r, e, eout, pr, omega, lv.aflag,
+                         st, C->work, zscale);
     } else {
-      hipLaunchKernelGGL((k_jacobi<T, V, true, false, RHO>), dim3(nblocks<T>(g)), dim3(kBlock),
-                         0, C->stream, g, lv.mat(), lv.r, e, eout, pr, omega, lv.aflag, st,
-                         C->work);
+      hipLaunchKernelGGL((k_jacobi<T, V, true, false, RHO, M, FLUX>), dim3(nblocks<T>(g)),
+                         dim3(kBlock), 0, C->stream, g, A, lv.r, e, eout, pr, omega, lv.aflag,
+                         st, C->work, zscale);
     }
   });
   if (RHO) finalize<4>(C, FIN_RHO, nblocks<T>(g), st);
+}
+
+template <typename T, bool RHO>
+void jacobi(psb_ctx *C, const Level<T> &lv, const T *e, T *eout, const Level<T> *coarse,
+            const T *eH, T cfac, T omega, DevState *st, T zscale = T(1)) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (lv.c16 && lv.d) {
+      jacobi_m<T, RHO, _Float16, true>(C, lv, lv.mat16(), e, eout, coarse, eH, cfac, omega,
+                                       zscale, st);
+      return;
+    }
+    if (lv.c16) {
+      jacobi_m<T, RHO, _Float16, false>(C, lv, lv.mat16(), e, eout, coarse, eH, cfac, omega,
+                                        zscale, st);
+      return;
+    }
+    if (lv.d) {
+      jacobi_m<T, RHO, T, true>(C, lv, lv.matflux(), e, eout, coarse, eH, cfac, omega, zscale,
+                                st);
+      return;
+    }
+  }
+  jacobi_m<T, RHO, T, false>(C, lv, lv.mat(), e, eout, coarse, eH, cfac, omega, zscale, st);
 }
 
 template <typename T, int V, int KV, int CI, int CJ, int CK>
 void restrict_v(psb_ctx *C, const Level<T> &f, const T *e, Level<T> &cL, T omega,
                 DevState *st) {
   const int nwork = cL.g.nx * cL.g.ny * (f.g.nz / (V * KV));
+  if constexpr (std::is_same<T, float>::value) {
+    if (f.c16 && f.d) {
+      hipLaunchKernelGGL((k_restrict<T, V, KV, true, CI, CJ, CK, _Float16, true>),
+                         dim3(grid_blocks(nwork)), dim3(kBlock), 0, C->stream, f.g, f.mat16(),
+                         f.r, e, f.aflag, cL.g, cL.c, cL.aflag, cL.r, cL.e[0], omega, st);
+      return;
+    }
+    if (f.c16) {
+      hipLaunchKernelGGL((k_restrict<T, V, KV, true, CI, CJ, CK, _Float16>),
+                         dim3(grid_blocks(nwork)), dim3(kBlock), 0, C->stream, f.g, f.mat16(),
+                         f.r, e, f.aflag, cL.g, cL.c, cL.aflag, cL.r, cL.e[0], omega, st);
+      return;
+    }
+    if (f.d) {
+      hipLaunchKernelGGL((k_restrict<T, V, KV, true, CI, CJ, CK, T, true>),
+                         dim3(grid_blocks(nwork)), dim3(kBlock), 0, C->stream, f.g, f.matflux(),
+                         f.r, e, f.aflag, cL.g, cL.c, cL.aflag, cL.r, cL.e[0], omega, st);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_restrict<T, V, KV, true, CI, CJ, CK>), dim3(grid_blocks(nwork)),
                      dim3(kBlock), 0, C->stream, f.g, f.mat(), f.r, e, f.aflag, cL.g, cL.c,
                      cL.aflag, cL.r, cL.e[0], omega, st);
@@ -215,46 +268,64 @@ std::vector<double> dense_pinv(int nx, int ny, int nz, const std::vector<double>
       }
   std::vector<uint8_t> pin(m, 0);
   for (int P = 0; P < m; P++) pin[P] = act[P] ? 0 : 1;
-  // right-looking Cholesky, lower triangle in place
-  for (int j = 0; j < m; j++) {
-    if (pin[j]) continue;
-    double d = M[(size_t)j * m + j];
-    if (d <= 1e-10 * std::fabs(c[j])) {
-      pin[j] = 1;
+  // row-by-row Cholesky (lower triangle in place, rows contiguous); an
+  // unknown whose pivot vanishes is pinned: its row and column drop out
+  for (int i = 0; i < m; i++) {
+    double *Li = &M[(size_t)i * m];
+    if (pin[i]) {
+      for (int j = 0; j <= i; j++) Li[j] = 0.0;
       continue;
     }
-    double L = std::sqrt(d);
-    M[(size_t)j * m + j] = L;
-    for (int i = j + 1; i < m; i++)
-      if (!pin[i]) M[(size_t)i * m + j] /= L;
-    for (int k2 = j + 1; k2 < m; k2++) {
-      if (pin[k2]) continue;
-      double lk = M[(size_t)k2 * m + j];
-      if (lk == 0.0) continue;
-      for (int i = k2; i < m; i++)
-        if (!pin[i]) M[(size_t)i * m + k2] -= M[(size_t)i * m + j] * lk;
+    for (int j = 0; j <= i; j++) {
+      if (pin[j]) {
+        Li[j] = 0.0;
+        continue;
+      }
+      const double *Lj = &M[(size_t)j * m];
+      double sm = Li[j];
+      for (int k2 = 0; k2 < j; k2++) sm -= Li[k2] * Lj[k2];
+      if (j < i) {
+        Li[j] = sm / Lj[j];
+      } else if (sm <= 1e-10 * std::fabs(c[i])) {
+        pin[i] = 1;
+        for (int q = 0; q <= i; q++) Li[q] = 0.0;
+      } else {
+        Li[i] = std::sqrt(sm);
+      }
     }
   }
-  std::vector<double> B((size_t)m * m, 0.0), y(m);
-  for (int col = 0; col < m; col++) {
-    if (pin[col]) continue;
-    for (int i = 0; i < m; i++) y[i] = (i == col) ? 1.0 : 0.0;
-    for (int i = 0; i < m; i++) {
-      if (pin[i]) { y[i] = 0.0; continue; }
-      double sm = y[i];
-      for (int k2 = 0; k2 < i; k2++)
-        if (!pin[k2]) sm -= M[(size_t)i * m + k2] * y[k2];
-      y[i] = sm / M[(size_t)i * m + i];
+  // columns of the inverse: two triangular solves each, independent, on
+  // the host's threads (the coarsest level of a stretched grid can hold
+  // 12^3 = 1728 cells)
+  std::vector<double> B((size_t)m * m, 0.0);
+  auto columns = [&](int c0, int c1) {
+    std::vector<double> y(m);
+    for (int col = c0; col < c1; col++) {
+      if (pin[col]) continue;
+      for (int i = 0; i < m; i++) y[i] = (i == col) ? 1.0 : 0.0;
+      for (int i = 0; i < m; i++) {
+        if (pin[i]) { y[i] = 0.0; continue; }
+        double sm = y[i];
+        for (int k2 = 0; k2 < i; k2++)
+          if (!pin[k2]) sm -= M[(size_t)i * m + k2] * y[k2];
+        y[i] = sm / M[(size_t)i * m + i];
+      }
+      // L^T y = y by columns of L^T (rows of L, contiguous)
+      for (int i = m - 1; i >= 0; i--) {
+        if (pin[i]) { y[i] = 0.0; continue; }
+        const double *Li = &M[(size_t)i * m];
+        y[i] /= Li[i];
+        const double yi = y[i];
+        for (int k2 = 0; k2 < i; k2++) y[k2] -= Li[k2] * yi;
+      }
+      for (int i = 0; i < m; i++) B[(size_t)i * m + col] = y[i];
     }
-    for (int i = m - 1; i >= 0; i--) {
-      if (pin[i]) { y[i] = 0.0; continue; }
-      double sm = y[i];
-      for (int k2 = i + 1; k2 < m; k2++)
-        if (!pin[k2]) sm -= M[(size_t)k2 * m + i] * y[k2];
-      y[i] = sm / M[(size_t)i * m + i];
-    }
-    for (int i = 0; i < m; i++) B[(size_t)i * m + col] = y[i];
-  }
+  };
+  const int nt = m < 256 ? 1 : (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; t++)
+    pool.emplace_back(columns, (int)((long long)m * t / nt), (int)((long long)m * (t + 1) / nt));
+  for (auto &th : pool) th.join();
   int na = 0;
   for (int P = 0; P < m; P++) na += act[P] ? 1 : 0;
   std::vector<double> rm(m, 0.0), cm(m, 0.0);
@@ -281,8 +352,10 @@ std::vector<double> dense_pinv(int nx, int ny, int nz, const std::vector<double>
 }
 
 // level sizes: halve a direction while it stays >= min_dim (8: the
-// benchmark's rule down to 8^3)
-std::vector<int> level_dims(int nx, int ny, int nz, int min_dim) {
+// benchmark's rule down to 8^3).  rule 1 halves x and y first and z only once
+// x and y stop (semi-coarsening for grids stretched in z, where the
+// couplings in x and y dominate away from the walls).
+std::vector<int> level_dims(int nx, int ny, int nz, int min_dim, int rule) {
   std::vector<int> d = {nx, ny, nz};
   while ((int)d.size() / 3 < kMaxLevels) {
     int n[3] = {d[d.size() - 3], d[d.size() - 2], d[d.size() - 1]};
@@ -290,6 +363,7 @@ std::vector<int> level_dims(int nx, int ny, int nz, int min_dim) {
     bool any = false;
     for (int q = 0; q < 3; q++) {
       m[q] = n[q];
+      if (rule == 1 && q == 2 && any) continue;
       if (n[q] % 2 == 0 && n[q] / 2 >= min_dim) {
         m[q] = n[q] / 2;
         any = true;
@@ -330,7 +404,7 @@ void build_hier64(psb_ctx *C) {
   if (H.built) return;
   ensure_krylov(C, C->k64, false);
   hipStream_t stream = C->stream;
-  std::vector<int> d = level_dims(C->g.nx, C->g.ny, C->g.nz, C->coarse_min);
+  std::vector<int> d = level_dims(C->g.nx, C->g.ny, C->g.nz, C->coarse_min, C->coarse_rule);
   const int nl = (int)d.size() / 3;
   if (nl < 2) throw std::invalid_argument("grid too small for the multigrid preconditioner");
   H.L.resize(nl);
@@ -348,19 +422,19 @@ void build_hier64(psb_ctx *C) {
       lv.r = C->k64.r;
       lv.aflag = C->aflag0;
     } else {
-      lv.c = C->arena.alloc<double>(N); lv.w = C->arena.alloc<double>(N);
-      lv.s = C->arena.alloc<double>(N); lv.b = C->arena.alloc<double>(N);
-      lv.r = C->arena.alloc<double>(N);
+      lv.c = C->harena.alloc<double>(N); lv.w = C->harena.alloc<double>(N);
+      lv.s = C->harena.alloc<double>(N); lv.b = C->harena.alloc<double>(N);
+      lv.r = C->harena.alloc<double>(N);
       Level<double> &f = H.L[l - 1];
       hipLaunchKernelGGL(k_galerkin, dim3(grid_blocks(N)), dim3(kBlock), 0, stream, f.g,
                          f.mat(), lv.g, lv.CI, lv.CJ, lv.CK, lv.c, lv.w, lv.s, lv.b);
       if (f.aflag) {
-        lv.aflag = C->arena.alloc<uint8_t>(N);
+        lv.aflag = C->harena.alloc<uint8_t>(N);
         hipLaunchKernelGGL(k_coarsen_flag, dim3(grid_blocks(N)), dim3(kBlock), 0, stream, f.g,
                            f.aflag, lv.g, lv.CI, lv.CJ, lv.CK, lv.aflag);
       }
     }
-    for (int q = 0; q < 3; q++) lv.e[q] = C->arena.alloc<double>(N);
+    for (int q = 0; q < 3; q++) lv.e[q] = C->harena.alloc<double>(N);
   }
   HIPCHK(hipGetLastError());
   // coarsest dense operator
@@ -376,7 +450,7 @@ void build_hier64(psb_ctx *C) {
   HIPCHK(hipStreamSynchronize(stream));
   std::vector<double> G = dense_pinv(cl.g.nx, cl.g.ny, cl.g.nz, c, w, s, b, act);
   H.m = m;
-  H.G = C->arena.alloc<double>((size_t)m * m);
+  H.G = C->harena.alloc<double>((size_t)m * m);
   HIPCHK(hipMemcpyAsync(H.G, G.data(), (size_t)m * m * 8, hipMemcpyHostToDevice, stream));
   HIPCHK(hipStreamSynchronize(stream));
   H.built = true;
@@ -417,13 +491,44 @@ void build_hier32(psb_ctx *C) {
       float **dst[4] = {&lv.c, &lv.w, &lv.s, &lv.b};
       const double *src[4] = {s.c, s.w, s.s, s.b};
       for (int q = 0; q < 4; q++) {
-        *dst[q] = C->arena.alloc<float>(N);
+        *dst[q] = C->harena.alloc<float>(N);
         hipLaunchKernelGGL(k_to_f32, dim3(grid_blocks(N)), dim3(kBlock), 0, C->stream, src[q],
                            *dst[q], N);
       }
-      lv.r = C->arena.alloc<float>(N);
+      lv.r = C->harena.alloc<float>(N);
     }
-    for (int q = 0; q < 3; q++) lv.e[q] = C->arena.alloc<float>(N);
+    for (int q = 0; q < 3; q++) lv.e[q] = C->harena.alloc<float>(N);
+#if PSB_FLUX
+    // flux-form diagonal for the FP32 stencils
+    lv.d = C->harena.alloc<float>(N);
+    hipLaunchKernelGGL(k_flux_d<float>, dim3(grid_blocks(N)), dim3(kBlock), 0, C->stream, lv.g,
+                       s.mat(), lv.d, 1.0);
+#endif
+    // half-precision copy for the smoother on every level but the coarsest
+    if (C->store_half && l < nl - 1) {
+      const int nb = grid_blocks(N);
+      hipLaunchKernelGGL(k_blockmax, dim3(nb), dim3(kBlock), 0, C->stream, s.c, N,
+                         C->work.partial);
+      std::vector<double> bm(nb);
+      HIPCHK(hipMemcpyAsync(bm.data(), C->work.partial, nb * 8, hipMemcpyDeviceToHost,
+                            C->stream));
+      HIPCHK(hipStreamSynchronize(C->stream));
+      const double cmax = *std::max_element(bm.begin(), bm.end());
+      // largest diagonal near 2^14: far from the FP16 limit (65504), with the
+      // smallest couplings (density ratio 1000) still normal numbers
+      lv.scale = cmax > 0.0 ? std::ldexp(1.0, 14 - (int)std::ceil(std::log2(cmax))) : 1.0;
+      _Float16 **dst[4] = {&lv.c16, &lv.w16, &lv.s16, &lv.b16};
+      const double *src[4] = {s.c, s.w, s.s, s.b};
+      for (int q = 0; q < 4; q++) {
+        *dst[q] = C->harena.alloc<_Float16>(N);
+        if (q == 0 && lv.d)  // flux form: scale * d
+          hipLaunchKernelGGL(k_flux_d<_Float16>, dim3(grid_blocks(N)), dim3(kBlock), 0,
+                             C->stream, lv.g, s.mat(), *dst[q], lv.scale);
+        else
+          hipLaunchKernelGGL(k_to_f16, dim3(grid_blocks(N)), dim3(kBlock), 0, C->stream,
+                             src[q], *dst[q], lv.scale, N, q == 0 ? 1 : 0);
+      }
+    }
   }
   H.G = H64.G;
   H.m = H64.m;
@@ -451,17 +556,21 @@ T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega
     cur = (cur + 1) % 3;
   }
   Level<T> &cL = H.L[l + 1];
-  restrict_<T>(C, lv, lv.e[cur], cL, omega, st);
+  // corrections on a level with a scaled half-precision matrix are held
+  // divided by its scale: the coarse level's first sweep, the prolongation
+  // and the preconditioner's output carry the ratios
+  restrict_<T>(C, lv, lv.e[cur], cL, T(omega / cL.scale), st);
   T *zH = vcycle<T>(C, H, l + 1, st, nu1, nu2, omega, cfac);
+  const T cfac_l = T(cfac * (cL.scale / lv.scale));
   for (int s = 0; s < nu2; s++) {
     const int nxt = (cur + 1) % 3;
     const bool last = (s == nu2 - 1) && l == 0;
     const Level<T> *co = s == 0 ? &cL : nullptr;
     const T *eH = s == 0 ? zH : nullptr;
     if (last)
-      jacobi<T, true>(C, lv, lv.e[cur], lv.e[nxt], co, eH, cfac, omega, st);
+      jacobi<T, true>(C, lv, lv.e[cur], lv.e[nxt], co, eH, cfac_l, omega, st, T(lv.scale));
     else
-      jacobi<T, false>(C, lv, lv.e[cur], lv.e[nxt], co, eH, cfac, omega, st);
+      jacobi<T, false>(C, lv, lv.e[cur], lv.e[nxt], co, eH, cfac_l, omega, st);
     cur = nxt;
   }
   return lv.e[cur];
@@ -474,8 +583,11 @@ void ac_iteration(psb_ctx *C, Hier<T> &H, KrylovVecs<T> &K, DevState *st, int pa
   Level<T> &f = H.L[0];
   T *z = vcycle<T>(C, H, 0, st, nu1, nu2, omega, cfac);
   T *pold = K.p[parity], *pnew = K.p[parity ^ 1];
-  cg_pq<T, true, false>(C, f.g, f.mat(), z, pold, pnew, K.q, f.aflag, st);
-  cg_xr<T, false>(C, f.g, f.c, pnew, K.q, K.x, K.r, f.e[0], omega, st, xr_op);
+  if (f.d)
+    cg_pq<T, true, false, true>(C, f.g, f.matflux(), z, pold, pnew, K.q, f.aflag, st);
+  else
+    cg_pq<T, true, false>(C, f.g, f.mat(), z, pold, pnew, K.q, f.aflag, st);
+  cg_xr<T, false>(C, f.g, f.c, pnew, K.q, K.x, K.r, f.e[0], T(omega / f.scale), st, xr_op);
 }
 
 void destroy_graph(Graph &gr) {
@@ -496,6 +608,25 @@ void destroy_graphs(psb_ctx *C) {
   destroy_graph(C->gr_ac32);
   destroy_graph(C->gr_ru);
 }
+
+// drop the hierarchy (and the graphs that use it): next AC solve rebuilds it
+void reset_hier(psb_ctx *C) {
+  HIPCHK(hipStreamSynchronize(C->stream));
+  destroy_graphs(C);
+  C->harena.release();
+  C->h64 = Hier<double>();
+  C->h32 = Hier<float>();
+}
+
+// the hierarchy for the coarsening rule `rule` and the smoother's matrix
+// storage (half: FP16 copies on the FP32 levels)
+void ensure_rule(psb_ctx *C, int rule, bool half) {
+  if (rule == C->coarse_rule && half == C->store_half) return;
+  reset_hier(C);
+  C->coarse_rule = rule;
+  C->store_half = half;
+}
+
 
 template <typename F>
 void capture(psb_ctx *C, Graph &gr, long long key, F body) {
@@ -770,7 +901,7 @@ void solve_ir(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_res
     with_v<float>(g, [&](auto vt) {
       constexpr int V = decltype(vt)::value;
       hipLaunchKernelGGL((k_center<V>), dim3(nb32), dim3(kBlock), 0, C->stream, g, f.c, f.aflag,
-                         K.r, f.e[0], (float)p->ac_omega, C->st32, C->work);
+                         K.r, f.e[0], (float)(p->ac_omega / f.scale), C->st32, C->work);
     });
     finalize<1>(C, FIN_CENTER, nb32, C->st32);
     HIPCHK(hipGetLastError());
@@ -809,7 +940,7 @@ void ru_resid(psb_ctx *C, const double *b, const double *x, bool gated, int op) 
   const Grid &g = C->g;
   Level<float> &f = C->h32.L[0];
   KrylovVecs<float> &K = C->k32;
-  const float omega = (float)C->ac_omega;
+  const float omega = (float)(C->ac_omega / f.scale);
   with_v<double>(g, [&](auto vt) {
     constexpr int V = decltype(vt)::value;
     hipLaunchKernelGGL((k_ru_resid<V, true>), dim3(nblocks<double>(g)), dim3(kBlock), 0,
@@ -880,6 +1011,10 @@ int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_
   C->hist_on_host = false;
   if (p->precond == PSB_PREC_AC) {
     if (p->method != PSB_CG || !C->sym || p->ac_pre < 1 || p->ac_post < 1) return PSB_EINVAL;
+    if (p->ac_coarsen < 0 || p->ac_coarsen > 1) return PSB_EINVAL;
+    if (p->ac_store < 0 || p->ac_store > 1) return PSB_EINVAL;
+    // the storage choice concerns only the FP32 levels: FP64 solves keep it
+    ensure_rule(C, p->ac_coarsen, p->precision == PSB_MIXED ? p->ac_store == 1 : C->store_half);
     if (p->precision == PSB_MIXED && p->inner_adapt == 2)
       solve_ru(C, x, b, p, R);
     else if (p->precision == PSB_MIXED)
@@ -959,6 +1094,8 @@ void psb_stop_defaults(psb_stop *s) {
   s->max_outer = 30;
   s->graph_chunk = 8;
   s->inner_adapt = 0;
+  s->ac_coarsen = 0;
+  s->ac_store = 0;
 }
 
 const char *psb_status_string(int s) {
@@ -1013,6 +1150,7 @@ int psb_destroy(psb_ctx *C) {
   if (C->stream) (void)hipStreamSynchronize(C->stream);
   destroy_graphs(C);
   C->arena.release();
+  C->harena.release();
   C->base.release();
   if (C->hst) (void)hipHostFree(C->hst);
   if (C->stream) (void)hipStreamDestroy(C->stream);
@@ -1074,6 +1212,7 @@ int psb_set_matrix(psb_ctx *C, const psb_matrix7 *A) {
     HIPCHK(hipStreamSynchronize(C->stream));
     destroy_graphs(C);
     C->arena.release();
+    C->harena.release();
     for (int q = 0; q < 7; q++) C->A64[q] = nullptr;
     C->A32[0] = C->A32[1] = C->A32[2] = C->A32[3] = nullptr;
     C->have32 = false;
@@ -1220,7 +1359,9 @@ int psb_time_kernel(psb_ctx *C, int kernel, int reps, double *avg_ms, double *by
       f = [=]() {
         jacobi<float, false>(C, lv, lv.e[0], lv.e[1], nullptr, nullptr, 0.0f, 0.9f, C->st32);
       };
-      *bytes = N * (4 * 4 + 3 * 4);  // c, w, s, b, e, r in; e out
+      // the level-0 matrix as the last mixed solve stored it (4 arrays, FP32
+      // or FP16), e and r in, e out
+      *bytes = N * (4 * (lv.c16 ? 2 : 4) + 3 * 4);
     } else if (kernel == 3) {
       ensure32(C);
       ensure_krylov(C, C->k32, false);

@@ -411,6 +411,8 @@ void build(psb_batch *B) {
       BLev<float> &D = B->lev[l][s];
       D.g = L.g;
       D.c = L.c; D.w = L.w; D.s = L.s; D.b = L.b;
+      D.d = L.d;
+      if (!L.d) throw std::invalid_argument("batched solves need the flux-form levels");
       D.r = L.r;
       for (int q = 0; q < 3; q++) D.e[q] = L.e[q];
       D.af = L.aflag;

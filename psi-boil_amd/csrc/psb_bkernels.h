@@ -15,6 +15,7 @@ template <typename T>
 struct BLev {
   Grid g;
   const T *c, *w, *s, *b;
+  const T *d;         // flux-form diagonal (c minus the couplings)
   T *r;
   T *e[3];
   const uint8_t *af;
@@ -23,6 +24,10 @@ struct BLev {
   const double *G;    // coarsest level: dense projected pseudo-inverse
   __device__ __forceinline__ DMat<T> mat() const {
     return DMat<T>{c, w, s, b, nullptr, nullptr, nullptr};
+  }
+  // the stencil kernels read the flux form: (d, w, s, b)
+  __device__ __forceinline__ DMat<T> matflux() const {
+    return DMat<T>{d, w, s, b, nullptr, nullptr, nullptr};
   }
 };
 
@@ -66,8 +71,8 @@ __global__ __launch_bounds__(kBlock) void kb_jacobi(const BSeg *segs, const uint
     pr.nYH = H.g.ny;
     pr.nZH = H.g.nz;
   }
-  jacobi_body<T, V, true, PROL, RHO>(sp, L.g, L.mat(), L.r, L.e[cur], L.e[nxt], pr, omega, L.af,
-                                     w);
+  jacobi_body<T, V, true, PROL, RHO, T, true>(sp, L.g, L.matflux(), L.r, L.e[cur], L.e[nxt], pr,
+                                              omega, L.af, w);
 }
 
 // restriction of level lev into levH (all systems of the launch share the
@@ -80,8 +85,8 @@ __global__ __launch_bounds__(kBlock) void kb_restrict(const BSeg *segs, const ui
   if (F.st->done) return;
   const BLev<T> &L = lev[sg_.sys];
   const BLev<T> &H = levH[sg_.sys];
-  restrict_body<T, V, KV, true, CI, CJ, CK>(sp, L.g, L.mat(), L.r, L.e[cur], L.af, H.g, H.c,
-                                            H.af, H.r, H.e[0], omega);
+  restrict_body<T, V, KV, true, CI, CJ, CK, T, true>(sp, L.g, L.matflux(), L.r, L.e[cur], L.af,
+                                                     H.g, H.c, H.af, H.r, H.e[0], omega);
 }
 
 // coarsest level: e[out] = G r
@@ -103,8 +108,8 @@ __global__ __launch_bounds__(kBlock) void kb_cg_pq(const BSeg *segs, const uint1
   PSB_BSEG_BEGIN
   if (F.st->done) return;
   const BLev<float> &L = lev0[sg_.sys];
-  cg_pq_body<float, V, true, false>(sp, F.g, L.mat(), L.e[zi], F.p[par], F.p[par ^ 1], F.q, L.af,
-                                    F.st, w);
+  cg_pq_body<float, V, true, false, true>(sp, F.g, L.matflux(), L.e[zi], F.p[par], F.p[par ^ 1],
+                                          F.q, L.af, F.st, w);
 }
 
 // CG x/r step (x: the FP32 correction d), first sweep of the next V-cycle

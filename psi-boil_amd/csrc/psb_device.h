@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace psb {
 
 constexpr int kBlock = 256;      // threads per block: 4 wave64 per workgroup
@@ -64,6 +66,20 @@ template <typename T, int V>
 __device__ __forceinline__ void stv(T *p, const VecN<T, V> &x) {
   *reinterpret_cast<VecN<T, V> *>(p) = x;
 }
+// V elements of a matrix stored in M (float, double or _Float16), as T
+template <typename T, typename M, int V>
+__device__ __forceinline__ VecN<T, V> ldm(const M *p) {
+  if constexpr (std::is_same<T, M>::value) {
+    return ldv<T, V>(p);
+  } else {
+    const VecN<M, V> m = ldv<M, V>(p);
+    VecN<T, V> r;
+#pragma unroll
+    for (int q = 0; q < V; q++) r.v[q] = (T)m.v[q];
+    return r;
+  }
+}
+
 template <typename T, int V>
 __device__ __forceinline__ VecN<T, V> zerov() {
   VecN<T, V> r;
@@ -180,50 +196,66 @@ struct Gath {
   VecN<T, V> x, c, cx, s1, s2, s3;
 };
 
-template <typename T, int V, bool SYM, class X>
-__device__ __forceinline__ Gath<T, V> gather(const Grid &g, const DMat<T> &A,
+// FLUX: A.c holds d = c - (sum of the six couplings) (zero on a Neumann
+// row of the pressure matrix) and the terms are formed as
+//   cx = d x_P,  s1 = aW (xW - x_P) + aE (xE - x_P), ...
+// so that (A x)_P = d x_P + sum a (x_P - x_nb): for smooth x no large terms
+// cancel, which keeps FP32 accurate on stiff (stretched, high-contrast)
+// systems.  G.c is then d + sum a, the divisor of the Jacobi sweep.
+template <typename T, int V, bool SYM, bool FLUX = false, class X, typename M>
+__device__ __forceinline__ Gath<T, V> gather(const Grid &g, const DMat<M> &A,
                                              const X &X_, int P, int i, int j,
                                              int k) {
   Gath<T, V> G;
   VecN<T, V> xw = zerov<T, V>(), xe = zerov<T, V>(), xs = zerov<T, V>(),
              xn = zerov<T, V>();
-  VecN<T, V> aw = ldv<T, V>(A.w + P), as = ldv<T, V>(A.s + P),
-             ab = ldv<T, V>(A.b + P);
+  VecN<T, V> aw = ldm<T, M, V>(A.w + P), as = ldm<T, M, V>(A.s + P),
+             ab = ldm<T, M, V>(A.b + P);
   VecN<T, V> ae = zerov<T, V>(), an = zerov<T, V>(), at;
-  G.c = ldv<T, V>(A.c + P);
+  G.c = ldm<T, M, V>(A.c + P);
   G.x = X_.vec(P, i, j, k);
   if (i > 0) xw = X_.vec(P - g.sI, i - 1, j, k);
   if (i < g.nx - 1) {
     xe = X_.vec(P + g.sI, i + 1, j, k);
-    ae = SYM ? ldv<T, V>(A.w + P + g.sI) : ldv<T, V>(A.e + P);
+    ae = SYM ? ldm<T, M, V>(A.w + P + g.sI) : ldm<T, M, V>(A.e + P);
   } else if (!SYM) {
-    ae = ldv<T, V>(A.e + P);
+    ae = ldm<T, M, V>(A.e + P);
   }
   if (j > 0) xs = X_.vec(P - g.nz, i, j - 1, k);
   if (j < g.ny - 1) {
     xn = X_.vec(P + g.nz, i, j + 1, k);
-    an = SYM ? ldv<T, V>(A.s + P + g.nz) : ldv<T, V>(A.n + P);
+    an = SYM ? ldm<T, M, V>(A.s + P + g.nz) : ldm<T, M, V>(A.n + P);
   } else if (!SYM) {
-    an = ldv<T, V>(A.n + P);
+    an = ldm<T, M, V>(A.n + P);
   }
   const bool has_t = (k + V) < g.nz;
   T xb0 = (k > 0) ? X_.at(P - 1, i, j, k - 1) : T(0);
   T xtV = has_t ? X_.at(P + V, i, j, k + V) : T(0);
   if (SYM) {
-    T btV = has_t ? A.b[P + V] : T(0);
+    T btV = has_t ? (T)A.b[P + V] : T(0);
 #pragma unroll
     for (int q = 0; q < V; q++) at.v[q] = (q < V - 1) ? ab.v[q + 1] : btV;
   } else {
-    at = ldv<T, V>(A.t + P);
+    at = ldm<T, M, V>(A.t + P);
   }
 #pragma unroll
   for (int q = 0; q < V; q++) {
     T xb = (q == 0) ? xb0 : G.x.v[q - 1];
     T xt = (q == V - 1) ? xtV : G.x.v[q + 1];
-    G.cx.v[q] = G.c.v[q] * G.x.v[q];
-    G.s1.v[q] = aw.v[q] * xw.v[q] + ae.v[q] * xe.v[q];
-    G.s2.v[q] = as.v[q] * xs.v[q] + an.v[q] * xn.v[q];
-    G.s3.v[q] = ab.v[q] * xb + at.v[q] * xt;
+    if constexpr (FLUX) {
+      const T x = G.x.v[q];
+      G.cx.v[q] = G.c.v[q] * x;
+      G.s1.v[q] = aw.v[q] * (xw.v[q] - x) + ae.v[q] * (xe.v[q] - x);
+      G.s2.v[q] = as.v[q] * (xs.v[q] - x) + an.v[q] * (xn.v[q] - x);
+      G.s3.v[q] = ab.v[q] * (xb - x) + at.v[q] * (xt - x);
+      const T cc = G.c.v[q] + (((aw.v[q] + ae.v[q]) + (as.v[q] + an.v[q])) + (ab.v[q] + at.v[q]));
+      G.c.v[q] = cc > T(0) ? cc : T(1);
+    } else {
+      G.cx.v[q] = G.c.v[q] * G.x.v[q];
+      G.s1.v[q] = aw.v[q] * xw.v[q] + ae.v[q] * xe.v[q];
+      G.s2.v[q] = as.v[q] * xs.v[q] + an.v[q] * xn.v[q];
+      G.s3.v[q] = ab.v[q] * xb + at.v[q] * xt;
+    }
   }
   return G;
 }

@@ -86,7 +86,19 @@ struct Level {
   T *r = nullptr;                  // right-hand side of this level
   T *e[3] = {nullptr, nullptr, nullptr};
   uint8_t *aflag = nullptr;        // NULL: all active
+  // half-precision copy scale*(c, w, s, b) for the smoother (FP32 levels
+  // with PSB_STORE_HALF; NULL otherwise).  The sweeps on it compute
+  // corrections scaled by 1/scale.
+  _Float16 *c16 = nullptr, *w16 = nullptr, *s16 = nullptr, *b16 = nullptr;
+  double scale = 1.0;
+  // flux form (FP32 levels): d = c - sum of couplings; the stencil kernels
+  // then read (d, w, s, b), and c16 holds scale * d
+  T *d = nullptr;
   DMat<T> mat() const { return DMat<T>{c, w, s, b, nullptr, nullptr, nullptr}; }
+  DMat<T> matflux() const { return DMat<T>{d, w, s, b, nullptr, nullptr, nullptr}; }
+  DMat<_Float16> mat16() const {
+    return DMat<_Float16>{c16, w16, s16, b16, nullptr, nullptr, nullptr};
+  }
 };
 
 template <typename T>
@@ -127,7 +139,8 @@ struct psb_ctx {
   Grid g{};
   hipStream_t stream = nullptr;
   Arena base;   // lives as long as the context
-  Arena arena;  // matrix, hierarchy and work vectors: rebuilt by set_matrix
+  Arena arena;   // matrix and work vectors: rebuilt by set_matrix
+  Arena harena;  // the coarse levels of the hierarchy: rebuilt with it
   // matrix
   bool have_matrix = false, sym = true;
   double *A64[7] = {};   // c w s b e n t
@@ -156,6 +169,8 @@ struct psb_ctx {
   Graph gr_cgdiag, gr_bicgs, gr_ac64, gr_ac32, gr_ru;
   double ac_omega = 0, ac_cfac = 0;
   int coarse_min = 8;  // coarsening halves a direction while it stays >= this
+  int coarse_rule = 0; // 0 all directions at once, 1 x and y before z
+  bool store_half = false;  // FP16 copies of the FP32 levels for the smoother
   int version = 0;     // bumped by every change of matrix, geometry or off mask
   std::string err;
 };
@@ -165,7 +180,7 @@ namespace host {
 
 DMat<double> mat64(const psb_ctx *c);
 Metric metric(const psb_ctx *c);
-std::vector<int> level_dims(int nx, int ny, int nz, int min_dim);
+std::vector<int> level_dims(int nx, int ny, int nz, int min_dim, int rule);
 template <typename T>
 void ensure_krylov(psb_ctx *C, KrylovVecs<T> &K, bool bicgs);
 void build_hier32(psb_ctx *C);

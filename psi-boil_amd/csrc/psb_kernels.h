@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_start(Grid g, DMat<T> A, Metric m, c
 // else the projected preconditioner output), q = A p; partial p.q
 // (cg.cpp:75-108)
 // ---------------------------------------------------------------------------
-template <typename T, int V, bool SYM, bool DIAG>
+template <typename T, int V, bool SYM, bool DIAG, bool FLUX = false>
 __device__ __forceinline__ void cg_pq_body(Span sp, const Grid &g, const DMat<T> &A,
                                            const T *r_or_z, const T *pold, T *pnew, T *q,
                                            const uint8_t *af, const DevState *st, Work w) {
@@ -123,7 +123,7 @@ __device__ __forceinline__ void cg_pq_body(Span sp, const Grid &g, const DMat<T>
     G = gather<T, V, SYM>(g, A, X, P, i, j, k);
   } else {
     ZPX<T, V> X{r_or_z, pold, af, beta, (T)st->zmean};
-    G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+    G = gather<T, V, SYM, FLUX>(g, A, X, P, i, j, k);
   }
   VecN<T, V> Q = apply_A<T, V>(G);
   stv<T, V>(pnew + P, G.x);
@@ -134,12 +134,12 @@ __device__ __forceinline__ void cg_pq_body(Span sp, const Grid &g, const DMat<T>
   block_partial<1>(acc, w.partial);
 }
 
-template <typename T, int V, bool SYM, bool DIAG>
+template <typename T, int V, bool SYM, bool DIAG, bool FLUX = false>
 __global__ __launch_bounds__(kBlock) void k_cg_pq(Grid g, DMat<T> A, const T *r_or_z,
                                                   const T *pold, T *pnew, T *q,
                                                   const uint8_t *af, DevState *st, Work w) {
   if (st->done) return;
-  cg_pq_body<T, V, SYM, DIAG>(grid_span(), g, A, r_or_z, pold, pnew, q, af, st, w);
+  cg_pq_body<T, V, SYM, DIAG, FLUX>(grid_span(), g, A, r_or_z, pold, pnew, q, af, st, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -195,25 +195,26 @@ __global__ __launch_bounds__(kBlock) void k_cg_xr(Grid g, const T *c, Metric m, 
 // search directions out of it (in FP32 a drift along it eats the digits of
 // the correction).
 // ---------------------------------------------------------------------------
-template <typename T, int V, bool SYM, bool PROL, bool RHO>
-__device__ __forceinline__ void jacobi_body(Span sp, const Grid &g, const DMat<T> &A,
+template <typename T, int V, bool SYM, bool PROL, bool RHO, typename M = T, bool FLUX = false>
+__device__ __forceinline__ void jacobi_body(Span sp, const Grid &g, const DMat<M> &A,
                                             const T *r, const T *e, T *eout,
                                             const ProlX<T, V> &pr, T omega, const uint8_t *af,
-                                            Work w) {
+                                            Work w, T zscale = T(1)) {
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   PSB_SPAN_BEGIN(g, V, sp)
   Gath<T, V> G;
   if (PROL) {
-    G = gather<T, V, SYM>(g, A, pr, P, i, j, k);
+    G = gather<T, V, SYM, FLUX>(g, A, pr, P, i, j, k);
   } else {
     ArrX<T, V> X{e};
-    G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+    G = gather<T, V, SYM, FLUX>(g, A, X, P, i, j, k);
   }
   VecN<T, V> R = ldv<T, V>(r + P);
   VecN<T, V> t = resid<T, V>(R, G), E;
 #pragma unroll
   for (int u = 0; u < V; u++) {
     E.v[u] = G.x.v[u] + omega * t.v[u] / G.c.v[u];
+    if (RHO) E.v[u] = zscale * E.v[u];
     if (RHO && (!af || af[P + u])) {
       acc[0] += (double)R.v[u] * (double)E.v[u];
       acc[1] += (double)E.v[u];
@@ -226,12 +227,14 @@ __device__ __forceinline__ void jacobi_body(Span sp, const Grid &g, const DMat<T
   if (RHO) block_partial<4>(acc, w.partial);
 }
 
-template <typename T, int V, bool SYM, bool PROL, bool RHO>
-__global__ __launch_bounds__(kBlock) void k_jacobi(Grid g, DMat<T> A, const T *r, const T *e,
+template <typename T, int V, bool SYM, bool PROL, bool RHO, typename M = T, bool FLUX = false>
+__global__ __launch_bounds__(kBlock) void k_jacobi(Grid g, DMat<M> A, const T *r, const T *e,
                                                    T *eout, ProlX<T, V> pr, T omega,
-                                                   const uint8_t *af, DevState *st, Work w) {
+                                                   const uint8_t *af, DevState *st, Work w,
+                                                   T zscale) {
   if (st->done) return;
-  jacobi_body<T, V, SYM, PROL, RHO>(grid_span(), g, A, r, e, eout, pr, omega, af, w);
+  jacobi_body<T, V, SYM, PROL, RHO, M, FLUX>(grid_span(), g, A, r, e, eout, pr, omega, af, w,
+                                             zscale);
 }
 
 // ---------------------------------------------------------------------------
@@ -243,8 +246,9 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(Grid g, DMat<T> A, const T *r
 // rows x V*KV cells along k of the fine grid (whole coarse cells); fine
 // cells are summed in (i, j, k) order within each coarse cell.
 // ---------------------------------------------------------------------------
-template <typename T, int V, int KV, bool SYM, int CI, int CJ, int CK>
-__device__ __forceinline__ void restrict_body(Span sp, const Grid &g, const DMat<T> &A,
+template <typename T, int V, int KV, bool SYM, int CI, int CJ, int CK, typename M = T,
+          bool FLUX = false>
+__device__ __forceinline__ void restrict_body(Span sp, const Grid &g, const DMat<M> &A,
                                               const T *r, const T *e, const uint8_t *af,
                                               const Grid &gH, const T *cH, const uint8_t *afH,
                                               T *rH, T *eH1, T omega) {
@@ -269,7 +273,7 @@ __device__ __forceinline__ void restrict_body(Span sp, const Grid &g, const DMat
         for (int h = 0; h < KV; h++) {
           const int k = kv * KE + h * V;
           const int P = (i * g.ny + j) * g.nz + k;
-          Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+          Gath<T, V> G = gather<T, V, SYM, FLUX>(g, A, X, P, i, j, k);
           VecN<T, V> R = resid<T, V>(ldv<T, V>(r + P), G);
 #pragma unroll
           for (int u = 0; u < V; u++) t[h * V + u] = (!af || af[P + u]) ? R.v[u] : T(0);
@@ -291,14 +295,15 @@ __device__ __forceinline__ void restrict_body(Span sp, const Grid &g, const DMat
   }
 }
 
-template <typename T, int V, int KV, bool SYM, int CI, int CJ, int CK>
-__global__ __launch_bounds__(kBlock) void k_restrict(Grid g, DMat<T> A, const T *r, const T *e,
+template <typename T, int V, int KV, bool SYM, int CI, int CJ, int CK, typename M = T,
+          bool FLUX = false>
+__global__ __launch_bounds__(kBlock) void k_restrict(Grid g, DMat<M> A, const T *r, const T *e,
                                                      const uint8_t *af, Grid gH, const T *cH,
                                                      const uint8_t *afH, T *rH, T *eH1,
                                                      T omega, DevState *st) {
   if (st->done) return;
-  restrict_body<T, V, KV, SYM, CI, CJ, CK>(grid_span(), g, A, r, e, af, gH, cH, afH, rH, eH1,
-                                           omega);
+  restrict_body<T, V, KV, SYM, CI, CJ, CK, M, FLUX>(grid_span(), g, A, r, e, af, gH, cH, afH,
+                                                    rH, eH1, omega);
 }
 
 // coarsest level: z = G r, G the dense projected pseudo-inverse (FP64).
@@ -878,6 +883,48 @@ static __global__ __launch_bounds__(kBlock) void k_copy(const float *x, float *y
     stv<float, 4>(y + 4 * q, ldv<float, 4>(x + 4 * q));
   for (int q = 4 * nv + blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
     y[q] = x[q];
+}
+
+// b = (half)(scale * a): the smoother's copy of a level matrix.  A positive
+// diagonal (pos = 1) never rounds to zero: the solid cells' c (scaled by
+// 1e-12) is raised to the smallest normal half, 2^-14.
+static __global__ void k_to_f16(const double *a, _Float16 *b, double scale, int n, int pos) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    float v = (float)(scale * a[q]);
+    if (pos && a[q] > 0.0 && v < 6.103515625e-05f) v = 6.103515625e-05f;
+    b[q] = (_Float16)v;
+  }
+}
+
+// d = c - (w + e + s + n + b + t) of a symmetric 4-array FP64 level matrix
+// (e, n, t read from the neighbours), rounded to T: the diagonal of the
+// flux form
+template <typename T>
+__global__ void k_flux_d(Grid g, DMat<double> A, T *d, double scale) {
+  for (int P = blockIdx.x * blockDim.x + threadIdx.x; P < g.N; P += gridDim.x * blockDim.x) {
+    int i, j, k;
+    g.ijk(P, i, j, k);
+    double s = A.w[P] + (i < g.nx - 1 ? A.w[P + g.sI] : 0.0);
+    s += A.s[P] + (j < g.ny - 1 ? A.s[P + g.nz] : 0.0);
+    s += A.b[P] + (k < g.nz - 1 ? A.b[P + 1] : 0.0);
+    d[P] = (T)(float)(scale * (A.c[P] - s));
+  }
+}
+
+// per-block maximum of a (n >= 1) into out[blockIdx.x]
+static __global__ __launch_bounds__(kBlock) void k_blockmax(const double *a, int n,
+                                                            double *out) {
+  __shared__ double sm[kBlock];
+  double m = -1.0e308;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+    m = fmax(m, a[q]);
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
 }
 
 static __global__ void k_to_f32(const double *a, float *b, int n) {

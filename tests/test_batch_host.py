@@ -37,7 +37,7 @@ def test_batched_abi_rejects_bad_arguments_without_a_device():
     assert L.psb_batch_cases(C.c_ulonglong(1), -1, None) == -1
 
 
-def test_oracle_level_rule_of_the_batched_hierarchy():
+def test_oracle_level_rules_of_the_batched_and_semi_coarsened_hierarchies():
     # rule 2: halve a direction while it stays >= 4
     assert O.levels(2, 48, 32, 96) == [(48, 32, 96), (24, 16, 48), (12, 8, 24), (6, 4, 12),
                                        (6, 4, 6)]
@@ -45,3 +45,7 @@ def test_oracle_level_rule_of_the_batched_hierarchy():
     assert O.levels(2, 30, 8, 8) == [(30, 8, 8), (15, 4, 4)]
     # rule 1 stops at 8
     assert O.levels(1, 48, 32, 96)[-1] == (12, 8, 12)
+    # rule 3: x and y first (while >= 8), then z
+    assert O.levels(3, 32, 32, 64) == [(32, 32, 64), (16, 16, 64), (8, 8, 64), (8, 8, 32),
+                                       (8, 8, 16), (8, 8, 8)]
+    assert O.levels(3, 16, 48, 12) == [(16, 48, 12), (8, 24, 12), (8, 12, 12)]

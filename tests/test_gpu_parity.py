@@ -268,6 +268,34 @@ def test_ac_with_reference_norm(precision):
     assert r["res"] < 1e-9 * r["res0"]
 
 
+def test_semi_coarsened_hierarchy_matches_oracle():
+    """ac_coarsen = 1 (x and y halved before z) on a z-stretched grid: FP64
+    AC-PCG against the oracle on the same levels (its rule 3)."""
+    nx, ny, nz = 32, 16, 48
+    d = psb.generate(nx, ny, nz, 6, 0.02, 0.06, 1.0, 1000.0, 13, 1013, stretch_z=2.5, lz=2.0)
+    A = O.Mat(nx, ny, nz, {k: d[k] for k in "cwesnbt"})
+    b, n = d["rhs"], d["rhs"].size
+    s = psb.Solver(nx, ny, nz)
+    s.set_matrix(d)
+    kw = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=1e-8, res_tol=0.0,
+              max_iter=400, ac_cfac=1.0)
+    x, r = s.solve(np.zeros(n), b, ac_coarsen=1, **kw)
+    ac = O.AC(A, np.ones(n), rule=3)
+    xo, info = ac.acpcg(np.zeros(n), b, rtol=1e-8, maxit=400, nu1=2, nu2=2, omega=0.9, cfac=1.0)
+    assert r["converged"] and info["converged"]
+    assert abs(r["iterations"] - info["iters"]) <= 1, (r["iterations"], info["iters"])
+    assert rel(x - x.mean(), xo - xo.mean()) < 1e-6
+    # switching back rebuilds the isotropic hierarchy (oracle rule 1)
+    _, r0 = s.solve(np.zeros(n), b, ac_coarsen=0, **kw)
+    _, i1 = O.AC(A, np.ones(n), rule=1).acpcg(np.zeros(n), b, rtol=1e-8, maxit=400, nu1=2,
+                                              nu2=2, omega=0.9, cfac=1.0)
+    assert abs(r0["iterations"] - i1["iters"]) <= 1, (r0["iterations"], i1["iters"])
+    # and the mixed-precision solve on the semi-coarsened levels
+    _, rm = s.solve(np.zeros(n), b, ac_coarsen=1, precision=psb.PSB_MIXED, inner_adapt=2,
+                    inner_rtol=1e-2, **kw)
+    assert rm["converged"] and rm["rel_l2"] <= 1e-8 * 1.01
+
+
 @pytest.mark.parametrize("adapt,rtol", [(0, 1e-4), (2, 1e-2)])
 def test_acpcg_with_solid_cells_and_stretched_grid(adapt, rtol):
     nx, ny, nz = 32, 16, 64
