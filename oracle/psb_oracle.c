@@ -34,6 +34,10 @@
  *   ora_pcg_rel                  not in the reference: dense pseudo-inverse
  *                                solve on tiny grids, symmetry of M^-1
  *                                (u.M^-1 v = v.M^-1 u), A-orthogonality.
+ *   ora_zline_solve,
+ *   ora_ac_precond_s             not in the reference: per-column dense
+ *                                tridiagonal solves; symmetry of M^-1 with
+ *                                the z-line smoother.
  */
 #include "psb_oracle.h"
 
@@ -993,6 +997,12 @@ static void zline_solve(const ora_mat *A, const double *t, double *d, double *y)
         d[P0 + k] = xn;
       }
     }
+}
+
+void ora_zline_solve(const ora_mat *A, const double *t, double *x) {
+  double *y = dalloc((size_t)A->nz);
+  zline_solve(A, t, x, y);
+  free(y);
 }
 
 static void zline_sweep(const ora_mat *A, const double *r, double *e, double *t,

@@ -159,6 +159,8 @@ void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
  * k of each (i, j) column (Thomas algorithm).                              */
 void ora_ac_precond_s(ora_ac *ac, int l, const double *r, double *z, int nu1,
                       int nu2, double omega, double cfac, int smoother);
+/* x = T^-1 t column by column (the z-line smoother's solve).               */
+void ora_zline_solve(const ora_mat *A, const double *t, double *x);
 /* PCG with the V-cycle preconditioner, its output projected to zero mean over
  * the active cells; stops on ||r||2/||r0||2 < rtol.
  * hist[i] = relative residual after iteration i (hist[0] = 1).             */

@@ -171,6 +171,13 @@ def resid(A, b, x):
     return r
 
 
+def zline_solve(A, t):
+    """x = T^-1 t with T the tridiagonal (c, -b) part of A along k, per column."""
+    x = np.zeros(A.n)
+    lib().ora_zline_solve(C.byref(A.s), _p(np.ascontiguousarray(t, dtype=np.float64)), _p(x))
+    return x
+
+
 def dot(x, y):
     return lib().ora_dot(C.c_size_t(x.size), _p(x), _p(y))
 

@@ -317,3 +317,39 @@ def test_planted_mistake_is_caught():
     A["s"][100] = -A["s"][100]
     Am = O.Mat(*g.shape, A)
     assert not np.array_equal(O.spmv(Am, g["x_rand"]), g["Ax_rand"])
+
+
+def test_zline_solve_against_dense_tridiagonal_solves():
+    """The z-line smoother's column solve equals a dense solve of each
+    column's tridiagonal block (diagonal c, off-diagonals -b)."""
+    rng = np.random.default_rng(7)
+    nx, ny, nz = 3, 4, 9
+    n = nx * ny * nz
+    arr = {k: np.zeros(n) for k in "cwesnbt"}
+    b = rng.uniform(0.1, 1.0, n).reshape(nx, ny, nz)
+    b[:, :, 0] = 0.0
+    arr["b"] = b.ravel()
+    tb = np.zeros((nx, ny, nz))
+    tb[:, :, :-1] = b[:, :, 1:]
+    arr["t"] = tb.ravel()
+    arr["c"] = (b + tb + rng.uniform(0.5, 2.0, (nx, ny, nz))).ravel()
+    A = O.Mat(nx, ny, nz, arr)
+    t = rng.standard_normal(n)
+    x = O.zline_solve(A, t).reshape(nx, ny, nz)
+    c3, t3 = arr["c"].reshape(nx, ny, nz), t.reshape(nx, ny, nz)
+    for i in range(nx):
+        for j in range(ny):
+            T = np.diag(c3[i, j]) - np.diag(b[i, j, 1:], -1) - np.diag(b[i, j, 1:], 1)
+            assert np.allclose(x[i, j], np.linalg.solve(T, t3[i, j]), rtol=1e-12, atol=1e-14)
+
+
+def test_zline_preconditioner_is_symmetric():
+    A, _, _, _ = O.sphere_case(16, 3, 0.1, 0.25, 1.0, 100.0, 4, 1004)
+    ac = O.AC(A, np.ones(A.n), rule=1)
+    rng = np.random.default_rng(3)
+    u, v = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    u -= u.mean()
+    v -= v.mean()
+    mu = ac.precond(u, omega=0.9, cfac=2.0, smoother=1)
+    mv = ac.precond(v, omega=0.9, cfac=2.0, smoother=1)
+    assert abs(u @ mv - v @ mu) <= 1e-10 * abs(u @ mv)
