@@ -69,28 +69,44 @@ def measured_traffic(n):
     return int(m.group(1)) if m else None
 
 
-def cpu_baseline(n, iters_full, sample_iters):
-    """Reference CG with diagonal preconditioning (cg.cpp, built from the
-    reference sources into oracle/_ref) on one host core: a bounded sample of
-    `sample_iters` iterations on the same 256^3 system, scaled to a full solve
-    with the iteration count the same algorithm needs on this system."""
+def ref_driver(mode, n, limit):
+    """Run the reference build's driver (oracle/_ref) on the bench case; the
+    solve time it prints, or None."""
     drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
     if not os.path.exists(drv):
         return None
-    args = [drv, "bench-cg", str(n), str(CASE["nsph"]), repr(CASE["rmin"]), repr(CASE["rmax"]),
+    args = [drv, mode, str(n), str(CASE["nsph"]), repr(CASE["rmin"]), repr(CASE["rmax"]),
             repr(CASE["rho_in"]), repr(CASE["rho_out"]), str(CASE["seed_sph"]),
-            str(CASE["seed_vel"]), str(sample_iters)]
-    out = subprocess.run(args, capture_output=True, text=True, timeout=600).stdout
+            str(CASE["seed_vel"]), str(limit)]
+    out = subprocess.run(args, capture_output=True, text=True, timeout=900).stdout
     m = re.search(r"solve_s=([0-9.eE+-]+)", out)
-    if not m:
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(n, cg_iters_full, cg_sample):
+    """The reference's own solvers, built from its sources into oracle/_ref,
+    on one host core, on bounded samples of the same 256^3 system:
+      - CG with the diagonal preconditioner (the baseline `value`, the
+        reference method that reaches 1e-8 here): `cg_sample` iterations,
+        scaled by the iterations it needs to 1e-8 (counted by the GPU's
+        parity-tested CG-di in the same run);
+      - one cycle of its production multigrid, AC::vcycle with CG smoothers
+        (reported beside it: on this system it stops after a few cycles,
+        its residual growing again, far above 1e-8)."""
+    t_cg = ref_driver("bench-cg", n, cg_sample)
+    if t_cg is None:
         return None
-    s_iter = float(m.group(1)) / sample_iters
-    return {"value": 1.0 / (s_iter * iters_full), "unit": "solves/s", "cores": 1,
-            "kind": "reference",
-            "sample": f"reference CG-di (Src/Solver/Linear/Krylov/cg.cpp) {sample_iters} iterations "
-                      f"at {n}^3, {s_iter:.3f} s/iteration on 1 core, scaled by the "
-                      f"{iters_full} iterations CG-di needs on this system",
-            "s_per_iteration": s_iter, "iterations_full": iters_full}
+    s_iter = t_cg / cg_sample
+    out = {"value": 1.0 / (s_iter * cg_iters_full), "unit": "solves/s", "cores": 1,
+           "kind": "reference",
+           "sample": f"reference CG-di (Src/Solver/Linear/Krylov/cg.cpp) {cg_sample} iterations "
+                     f"at {n}^3, {s_iter:.3f} s/iteration on 1 core, scaled by the "
+                     f"{cg_iters_full} iterations CG-di needs on this system",
+           "s_per_iteration": s_iter, "iterations_full": cg_iters_full}
+    t_cyc = ref_driver("bench-ac", n, 1)
+    if t_cyc is not None:
+        out["ac_vcycle_s_per_cycle"] = t_cyc
+    return out
 
 
 EXTRA = {
@@ -302,6 +318,17 @@ def main():
             "avg_launch_ms": ms_j, "bytes_per_launch": by_j,
             "spmv_fp32_GBs": round(gbs(by_s32, ms_s32), 1),
             "spmv_fp64_GBs": round(gbs(by_s64, ms_s64), 1)}
+    # the reference's own multigrid (AC::vcycle) on the GPU: cycles and time
+    log("[bench] reference AC::vcycle on the GPU")
+    dx.zero()
+    rv = s.ac_vcycle_device(dx.addr, db.addr, res_rat=1e-8, max_cycles=50)
+    psb.device_synchronize()
+    t0 = time.perf_counter()
+    dx.zero()
+    rv = s.ac_vcycle_device(dx.addr, db.addr, res_rat=1e-8, max_cycles=50)
+    psb.device_synchronize()
+    t_vc = time.perf_counter() - t0
+    log(f"[bench] AC::vcycle {t_vc * 1e3:.1f} ms {rv}")
     cpu = None
     if not a.no_cpu:
         log("[bench] FP64 CG-di iteration count for the CPU baseline scale")
@@ -309,7 +336,7 @@ def main():
         rdi = s.solve_device(dx.addr, db.addr, precision=psb.PSB_FP64, norm=psb.PSB_NORM_L2REL,
                              res_rat=1e-8, res_tol=0.0, max_iter=100000, graph_chunk=64)
         log(f"[bench] CG-di {rdi}")
-        log("[bench] reference CPU sample")
+        log("[bench] reference CPU samples")
         cpu = cpu_baseline(n, rdi["iterations"], a.cpu_iters)
     out = {
         "metric": "pressure solves/s to 1e-8 rel. residual at 256^3; %HBM roofline; mixed vs FP64",
@@ -337,6 +364,9 @@ def main():
                        "mixed_fp64_updates": rmx["outer"]},
         "rel_residual": {"fp64": r64["rel_l2"], "mixed": rmx["rel_l2"]},
         "error_vs_fp64_1e-12": err,
+        "reference_ac_vcycle_gpu": {"ms": round(t_vc * 1e3, 2), "cycles": rv["iterations"],
+                                    "status": rv["status_name"],
+                                    "rel_residual": rv["rel_l2"]},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -189,6 +189,39 @@ int psb_generate(const psb_case *cs, double *c, double *w, double *e,
                  double *s, double *n, double *b_, double *t, double *rhs,
                  unsigned char *off, double *dxc, double *dyc, double *dzc);
 
+/* ------------------------------------------------------------------------
+ * The reference's own multigrid solve: AC::cycle with the V loop
+ * (Src/Solver/Additive/additive.h:30-95 vcycle(ResTol, ResRat),
+ * additive_cycle.cpp:4-99, additive_recursive.cpp:4-34) on the GPU in FP64:
+ * its levels (smallest-prime-factor coarsening, domain_coarsen.cpp), its
+ * Galerkin operators, CG smoothing with the diagonal preconditioner on every
+ * level, restriction with mean removal, interpolation, and its stopping and
+ * divergence rules (additive_cycle_functions.cpp:12-87).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+  double res_tol;     /* ResTol toler (< 0: off, as vcycle(ResRat))          */
+  double res_rat;     /* ResRat factor (< 0: off, as vcycle(ResTol))         */
+  int max_cycles;     /* AC::max_cycles (20, additive.cpp:11)                */
+  int min_cycles;     /* AC::min_cycles (0)                                  */
+  int max_iter[3];    /* AC::max_iter_array: CG iterations before / (unused)
+                         / after the coarse correction (20, 20, 20)          */
+  double dt;          /* time step: residual = sqrt(dot_voldiv_avg) * dt
+                         (additive_residual.cpp:19-29); smoothers use 1/dt
+                         as their scale (additive_cycle_functions.cpp:94)   */
+} psb_cycle;
+
+void psb_cycle_defaults(psb_cycle *c);
+
+/* x (in: initial guess, out: solution), b: host arrays in the context's
+ * layout.  res->iterations = cycles, res->converged = the bool cycle()
+ * returns; PSB_DIVERGED when a cycle did not reduce the residual (x then
+ * holds the previous cycle's phi, as the reference restores it).
+ * psb_history gives the residual before and after every cycle.            */
+int psb_ac_vcycle(psb_ctx *ctx, double *x, const double *b, const psb_cycle *c,
+                  psb_result *res);
+int psb_ac_vcycle_device(psb_ctx *ctx, double *d_x, const double *d_b,
+                         const psb_cycle *c, psb_result *res);
+
 /* Seeded batch of cases (dims from {32, 48, 64, 96, 128} per direction,
  * 1-8 bubbles of radius 0.05-0.2 of the shortest side, density ratio
  * 10-1000, own seeds); fills out[0..nsys).                                  */

@@ -95,11 +95,12 @@ void resid_k(hipStream_t stream, const Grid &g, DMat<T> A, const T *b, const T *
 
 template <typename T, bool SYM, int MODE>
 void start(psb_ctx *C, const Grid &g, DMat<T> A, const T *b, const T *x, T *r, T *aux, T omega,
-           DevState *st) {
+           DevState *st, const Metric *mt = nullptr) {
+  const Metric m = mt ? *mt : metric(C);
   with_v<T>(g, [&](auto vt) {
     constexpr int V = decltype(vt)::value;
     hipLaunchKernelGGL((k_start<T, V, SYM, MODE>), dim3(nblocks<T>(g)), dim3(kBlock), 0,
-                       C->stream, g, A, metric(C), b, x, r, aux, omega, C->work);
+                       C->stream, g, A, m, b, x, r, aux, omega, C->work);
   });
   finalize<3>(C, MODE == 0 ? FIN_START_DIAG : MODE == 1 ? FIN_START_AC : FIN_START_BICGS,
               nblocks<T>(g), st);
@@ -118,11 +119,12 @@ void cg_pq(psb_ctx *C, const Grid &g, DMat<T> A, const T *rz, const T *pold, T *
 
 template <typename T, bool DIAG>
 void cg_xr(psb_ctx *C, const Grid &g, const T *c, const T *p, const T *q, T *x, T *r, T *e1,
-           T omega, DevState *st, int op = -1) {
+           T omega, DevState *st, int op = -1, const Metric *mt = nullptr) {
+  const Metric m = mt ? *mt : metric(C);
   with_v<T>(g, [&](auto vt) {
     constexpr int V = decltype(vt)::value;
     hipLaunchKernelGGL((k_cg_xr<T, V, DIAG>), dim3(nblocks<T>(g)), dim3(kBlock), 0, C->stream, g,
-                       c, metric(C), p, q, x, r, e1, omega, st, C->work);
+                       c, m, p, q, x, r, e1, omega, st, C->work);
   });
   finalize<3>(C, op >= 0 ? op : DIAG ? FIN_XR_DIAG : FIN_XR_AC, nblocks<T>(g), st);
 }
@@ -354,7 +356,14 @@ std::vector<double> dense_pinv(int nx, int ny, int nz, const std::vector<double>
 // level sizes: halve a direction while it stays >= min_dim (8: the
 // benchmark's rule down to 8^3).  rule 1 halves x and y first and z only once
 // x and y stop (semi-coarsening for grids stretched in z, where the
-// couplings in x and y dominate away from the walls).
+// couplings in x and y dominate away from the walls).  rule 2 is the
+// reference's own coarsening (used by its AC cycle).
+int smallest_factor(int n) {
+  for (int f = 2; f * f <= n; f++)
+    if (n % f == 0) return f;
+  return n;
+}
+
 std::vector<int> level_dims(int nx, int ny, int nz, int min_dim, int rule) {
   std::vector<int> d = {nx, ny, nz};
   while ((int)d.size() / 3 < kMaxLevels) {
@@ -363,6 +372,17 @@ std::vector<int> level_dims(int nx, int ny, int nz, int min_dim, int rule) {
     bool any = false;
     for (int q = 0; q < 3; q++) {
       m[q] = n[q];
+      if (rule == 2) {
+        // the reference's rule (domain_coarsen.cpp:33-71 with factor(),
+        // global_factor.cpp:2-19): divide by the smallest prime factor while
+        // the size exceeds 4 and is not itself that factor
+        const int f = smallest_factor(n[q]);
+        if (n[q] > 4 && n[q] != f) {
+          m[q] = n[q] / f;
+          any = true;
+        }
+        continue;
+      }
       if (rule == 1 && q == 2 && any) continue;
       if (n[q] % 2 == 0 && n[q] / 2 >= min_dim) {
         m[q] = n[q] / 2;
@@ -596,6 +616,7 @@ void destroy_graph(Graph &gr) {
   gr.exec = nullptr;
   gr.graph = nullptr;
   gr.key = -1;
+  gr.fkey = 0.0;
   gr.px = gr.pb = nullptr;
 }
 
@@ -607,6 +628,7 @@ void destroy_graphs(psb_ctx *C) {
   destroy_graph(C->gr_ac64);
   destroy_graph(C->gr_ac32);
   destroy_graph(C->gr_ru);
+  destroy_graph(C->href.gr);
 }
 
 // drop the hierarchy (and the graphs that use it): next AC solve rebuilds it
@@ -616,6 +638,7 @@ void reset_hier(psb_ctx *C) {
   C->harena.release();
   C->h64 = Hier<double>();
   C->h32 = Hier<float>();
+  C->href = RefHier();
 }
 
 // the hierarchy for the coarsening rule `rule` and the smoother's matrix
@@ -1007,6 +1030,217 @@ void solve_ru(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_res
   C->hist_count = std::min(h.outer + 1, C->hist_cap);
 }
 
+// ---------------------------------------------------------------------------
+// the reference's AC::cycle with the V loop (additive_cycle.cpp:4-99,
+// additive_recursive.cpp:4-34): on every level a CG with the diagonal
+// preconditioner smooths (MaxIter mv[0] before and mv[2] after the coarse
+// correction, ResRat and ResTol atto as AC's defaults, additive.cpp:16-17),
+// the coarsest level gets 100 CG iterations, the residual is restricted with
+// the mean over active cells removed, the correction interpolated; the coarse
+// levels' phi persists between cycles of one call.
+// ---------------------------------------------------------------------------
+constexpr double kAtto = 1.0e-18;  // Src/Global/global_constants.cpp:19
+
+void build_ref(psb_ctx *C) {
+  RefHier &H = C->href;
+  if (H.built) return;
+  if (!C->sym) throw std::invalid_argument("the AC cycle needs a symmetric matrix");
+  hipStream_t stream = C->stream;
+  std::vector<int> d = level_dims(C->g.nx, C->g.ny, C->g.nz, 0, 2);
+  const int nl = (int)d.size() / 3;
+  if (nl < 2) throw std::invalid_argument("grid too small for the AC cycle");
+  H.L.assign(nl, Level<double>());
+  H.M.assign(nl, Metric{nullptr, nullptr, nullptr});
+  H.K.assign(nl, std::vector<double *>(4, nullptr));
+  std::vector<double> hx = C->gdx, hy = C->gdy, hz = C->gdz;
+  if (hx.empty()) {
+    hx.assign(C->g.nx, 1.0);
+    hy.assign(C->g.ny, 1.0);
+    hz.assign(C->g.nz, 1.0);
+  }
+  for (int l = 0; l < nl; l++) {
+    Level<double> &lv = H.L[l];
+    lv.g = make_grid(d[3 * l], d[3 * l + 1], d[3 * l + 2]);
+    const size_t N = lv.g.N;
+    if (l == 0) {
+      lv.c = C->A64[0]; lv.w = C->A64[1]; lv.s = C->A64[2]; lv.b = C->A64[3];
+      lv.aflag = C->aflag0;
+    } else {
+      Level<double> &f = H.L[l - 1];
+      lv.CI = f.g.nx / lv.g.nx;
+      lv.CJ = f.g.ny / lv.g.ny;
+      lv.CK = f.g.nz / lv.g.nz;
+      lv.c = C->harena.alloc<double>(N); lv.w = C->harena.alloc<double>(N);
+      lv.s = C->harena.alloc<double>(N); lv.b = C->harena.alloc<double>(N);
+      hipLaunchKernelGGL(k_galerkin, dim3(grid_blocks(N)), dim3(kBlock), 0, stream, f.g,
+                         f.mat(), lv.g, lv.CI, lv.CJ, lv.CK, lv.c, lv.w, lv.s, lv.b);
+      if (f.aflag) {
+        lv.aflag = C->harena.alloc<uint8_t>(N);
+        hipLaunchKernelGGL(k_coarsen_flag, dim3(grid_blocks(N)), dim3(kBlock), 0, stream, f.g,
+                           f.aflag, lv.g, lv.CI, lv.CJ, lv.CK, lv.aflag);
+      }
+      // coarse cell sizes: sums over the block (the reference's coarse grid
+      // spans the same nodes)
+      std::vector<double> cx(lv.g.nx, 0.0), cy(lv.g.ny, 0.0), cz(lv.g.nz, 0.0);
+      for (int i = 0; i < f.g.nx; i++) cx[i / lv.CI] += hx[i];
+      for (int j = 0; j < f.g.ny; j++) cy[j / lv.CJ] += hy[j];
+      for (int k = 0; k < f.g.nz; k++) cz[k / lv.CK] += hz[k];
+      hx = cx;
+      hy = cy;
+      hz = cz;
+    }
+    double *mx = C->harena.alloc<double>(lv.g.nx), *my = C->harena.alloc<double>(lv.g.ny),
+           *mz = C->harena.alloc<double>(lv.g.nz);
+    HIPCHK(hipMemcpyAsync(mx, hx.data(), hx.size() * 8, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(my, hy.data(), hy.size() * 8, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(mz, hz.data(), hz.size() * 8, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));  // hx, hy, hz change in the next pass
+    H.M[l] = Metric{mx, my, mz};
+    lv.r = C->harena.alloc<double>(N);     // f of this level
+    lv.e[0] = C->harena.alloc<double>(N);  // phi
+    if (l == 0) lv.e[1] = C->harena.alloc<double>(N);  // phi of the previous cycle
+    for (int q = 0; q < 4; q++) H.K[l][q] = C->harena.alloc<double>(N);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(stream));
+  H.built = true;
+}
+
+// Linear::solve of one level by CG with the diagonal preconditioner
+// (call_smoother / call_solver): phi is improved in place
+void ref_cg(psb_ctx *C, int l, int maxit, double res_rat, double res_tol, double scale) {
+  RefHier &H = C->href;
+  const Level<double> &lv = H.L[l];
+  const Grid &g = lv.g;
+  double *r = H.K[l][0], *p0 = H.K[l][1], *p1 = H.K[l][2], *q = H.K[l][3];
+  DevState *st = C->st64;
+  hipLaunchKernelGGL(k_stinit, dim3(1), dim3(64), 0, C->stream, st, maxit, res_rat, res_tol,
+                     scale, (double)g.N);
+  zero(C, p0, g.N);
+  zero(C, p1, g.N);
+  start<double, true, 0>(C, g, lv.mat(), lv.r, lv.e[0], r, nullptr, 0.0, st, &H.M[l]);
+  for (int it = 0; it < maxit; it++) {
+    double *pold = (it & 1) ? p1 : p0, *pnew = (it & 1) ? p0 : p1;
+    cg_pq<double, true, true>(C, g, lv.mat(), r, pold, pnew, q, nullptr, st);
+    cg_xr<double, true>(C, g, lv.c, pnew, q, lv.e[0], r, nullptr, 0.0, st, -1, &H.M[l]);
+  }
+}
+
+void ref_vcycle(psb_ctx *C, int l, const int mv[3], double scale) {
+  RefHier &H = C->href;
+  const int nl = (int)H.L.size();
+  auto interp = [&](int lc) {  // level lc into lc - 1
+    const Level<double> &cL = H.L[lc], &f = H.L[lc - 1];
+    hipLaunchKernelGGL(k_ac_interp, dim3(grid_blocks(f.g.N)), dim3(kBlock), 0, C->stream, f.g,
+                       f.e[0], cL.g, cL.e[0], cL.CI, cL.CJ, cL.CK);
+  };
+  if (l != nl - 1) {
+    ref_cg(C, l, mv[0], kAtto, kAtto, scale);
+    const Level<double> &f = H.L[l], &cL = H.L[l + 1];
+    const int nb = grid_blocks(cL.g.N);
+    hipLaunchKernelGGL(k_ac_restrict, dim3(nb), dim3(kBlock), 0, C->stream, f.g, f.mat(), f.e[0],
+                       f.r, cL.g, cL.CI, cL.CJ, cL.CK, cL.aflag, cL.r, C->work);
+    finalize<2>(C, FIN_ACMEAN, nb, C->st64);
+    hipLaunchKernelGGL(k_ac_submean, dim3(nb), dim3(kBlock), 0, C->stream, cL.g, cL.r, cL.aflag,
+                       C->st64);
+    ref_vcycle(C, l + 1, mv, scale);
+    ref_cg(C, l, mv[2], kAtto, kAtto, scale);
+    if (l > 0) interp(l);
+  } else {
+    ref_cg(C, l, 100, -1.0, kAtto, scale);
+    interp(l);
+  }
+}
+
+// AC::residual (additive_residual.cpp:19-29) of level 0 into C->relres[0]:
+// sqrt(dot_voldiv_avg(f - A phi)) * dt
+double ref_residual(psb_ctx *C) {
+  RefHier &H = C->href;
+  const Level<double> &lv = H.L[0];
+  const Grid &g = lv.g;
+  with_v<double>(g, [&](auto vt) {
+    constexpr int V = decltype(vt)::value;
+    hipLaunchKernelGGL((k_start<double, V, true, 0>), dim3(nblocks<double>(g)), dim3(kBlock), 0,
+                       C->stream, g, lv.mat(), H.M[0], lv.r, lv.e[0], H.K[0][0], nullptr, 0.0,
+                       C->work);
+  });
+  finalize<3>(C, FIN_ACRES, nblocks<double>(g), C->st32, nullptr, C->relres);
+  double v = 0.0;
+  HIPCHK(hipMemcpyAsync(&v, C->relres, 8, hipMemcpyDeviceToHost, C->stream));
+  HIPCHK(hipStreamSynchronize(C->stream));
+  return v;
+}
+
+int ac_vcycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_result *R) {
+  if (P->max_cycles < 0 || P->min_cycles < 0 || P->dt <= 0.0 || P->max_iter[0] < 1 ||
+      P->max_iter[2] < 1)
+    return PSB_EINVAL;
+  build_ref(C);
+  RefHier &H = C->href;
+  const Level<double> &L0 = H.L[0];
+  const size_t N0 = L0.g.N;
+  HIPCHK(hipMemcpyAsync(L0.e[0], x, N0 * 8, hipMemcpyDeviceToDevice, C->stream));
+  HIPCHK(hipMemcpyAsync(L0.r, b, N0 * 8, hipMemcpyDeviceToDevice, C->stream));
+  // the residual norm's state: N of level 0, scale 1/dt
+  psb_stop s;
+  psb_stop_defaults(&s);
+  s.scale = 1.0 / P->dt;
+  init_state(C, C->st32, &s, 0, 0.0, 0.0, 1, false);
+  const double toler = P->res_tol, factor = P->res_rat, dti = 1.0 / P->dt;
+  C->hist_host.clear();
+  const double res_0 = ref_residual(C);
+  C->hist_host.push_back(res_0);
+  HIPCHK(hipMemcpyAsync(L0.e[1], L0.e[0], N0 * 8, hipMemcpyDeviceToDevice, C->stream));
+  int ncyc = 0, conv = 0, status = PSB_MAXITER;
+  double res = res_0;
+  if ((res_0 < toler && P->min_cycles == 0) || res_0 == 0.0) {
+    conv = 1;
+    status = PSB_OK;
+  } else {
+    for (int l = 1; l < (int)H.L.size(); l++) {
+      zero(C, H.L[l].e[0], H.L[l].g.N);
+      zero(C, H.L[l].r, H.L[l].g.N);
+    }
+    double res_m1 = ref_residual(C);
+    const int mv[3] = {P->max_iter[0], P->max_iter[1], P->max_iter[2]};
+    const long long key = (long long)mv[0] + ((long long)mv[2] << 20);
+    // dt enters the level solves' scale: part of the captured graph
+    if (H.gr.key != key || H.gr.fkey != dti) destroy_graph(H.gr);
+    for (int c = 1; c <= P->max_cycles; c++) {
+      capture(C, H.gr, key, [&]() { ref_vcycle(C, 0, mv, dti); });
+      H.gr.fkey = dti;
+      HIPCHK(hipGraphLaunch(H.gr.exec, C->stream));
+      const double res1 = ref_residual(C);
+      C->hist_host.push_back(res1);
+      res = res1;
+      ncyc = c;
+      if (c >= P->min_cycles && (res1 / res_0 < factor || res1 < toler)) {
+        conv = 1;
+        status = PSB_OK;
+        break;
+      }
+      if (!(c < P->min_cycles) && res1 >= res_m1) {  // diverging: keep the previous phi
+        HIPCHK(hipMemcpyAsync(L0.e[0], L0.e[1], N0 * 8, hipMemcpyDeviceToDevice, C->stream));
+        status = PSB_DIVERGED;
+        break;
+      }
+      res_m1 = res1;
+      HIPCHK(hipMemcpyAsync(L0.e[1], L0.e[0], N0 * 8, hipMemcpyDeviceToDevice, C->stream));
+    }
+  }
+  HIPCHK(hipMemcpyAsync(x, L0.e[0], N0 * 8, hipMemcpyDeviceToDevice, C->stream));
+  HIPCHK(hipStreamSynchronize(C->stream));
+  C->hist_on_host = true;
+  C->hist_count = (int)C->hist_host.size();
+  R->iterations = ncyc;
+  R->outer = 0;
+  R->converged = conv;
+  R->status = status;
+  R->res0 = res_0;
+  R->res = res;
+  return status;
+}
+
 int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_result *R) {
   C->hist_on_host = false;
   if (p->precond == PSB_PREC_AC) {
@@ -1166,10 +1400,17 @@ int psb_set_geometry(psb_ctx *C, const double *dxc, const double *dyc, const dou
     HIPCHK(hipStreamSynchronize(C->stream));
     destroy_graphs(C);
     C->version++;
+    reset_hier(C);  // the reference cycle's coarse metrics derive from these
     if (!dxc) {
       C->dxc = C->dyc = C->dzc = nullptr;
+      C->gdx.clear();
+      C->gdy.clear();
+      C->gdz.clear();
       return PSB_OK;
     }
+    C->gdx.assign(dxc, dxc + C->g.nx);
+    C->gdy.assign(dyc, dyc + C->g.ny);
+    C->gdz.assign(dzc, dzc + C->g.nz);
     if (!C->dxc) {
       C->dxc = C->base.alloc<double>(C->g.nx);
       C->dyc = C->base.alloc<double>(C->g.ny);
@@ -1218,6 +1459,7 @@ int psb_set_matrix(psb_ctx *C, const psb_matrix7 *A) {
     C->have32 = false;
     C->h64 = Hier<double>();
     C->h32 = Hier<float>();
+    C->href = RefHier();
     C->k64 = KrylovVecs<double>();
     C->k32 = KrylovVecs<float>();
     C->hist_count = 0;
@@ -1309,6 +1551,47 @@ int psb_apply(psb_ctx *C, const double *x, const double *b, double *y) {
     HIPCHK(hipStreamSynchronize(stream));
     scatter_host(C->lay, hy, y);
     return PSB_OK;
+  });
+}
+
+void psb_cycle_defaults(psb_cycle *c) {
+  std::memset(c, 0, sizeof(*c));
+  c->res_tol = -1.0;
+  c->res_rat = -1.0;
+  c->max_cycles = 20;  // additive.cpp:11-19
+  c->min_cycles = 0;
+  c->max_iter[0] = c->max_iter[1] = c->max_iter[2] = 20;
+  c->dt = 1.0;
+}
+
+int psb_ac_vcycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_result *R) {
+  if (!C || !x || !b || !P || !R) return PSB_EINVAL;
+  if (!C->have_matrix) return PSB_ENOTREADY;
+  return guarded(C, [&]() {
+    const psb_layout &L = C->lay;
+    std::vector<double> hx, hb;
+    gather_host(L, x, hx);
+    gather_host(L, b, hb);
+    const size_t N = C->g.N;
+    HIPCHK(hipMemcpy(C->dx, hx.data(), N * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(C->db, hb.data(), N * 8, hipMemcpyHostToDevice));
+    int rc = ac_vcycle(C, C->dx, C->db, P, R);
+    if (rc < 0) return rc;
+    R->rel_l2 = true_rel(C, C->db, C->dx);
+    HIPCHK(hipMemcpy(hx.data(), C->dx, N * 8, hipMemcpyDeviceToHost));
+    scatter_host(L, hx, x);
+    return rc;
+  });
+}
+
+int psb_ac_vcycle_device(psb_ctx *C, double *x, const double *b, const psb_cycle *P,
+                         psb_result *R) {
+  if (!C || !x || !b || !P || !R) return PSB_EINVAL;
+  if (!C->have_matrix) return PSB_ENOTREADY;
+  return guarded(C, [&]() {
+    int rc = ac_vcycle(C, x, b, P, R);
+    if (rc >= 0) R->rel_l2 = true_rel(C, b, x);
+    return rc;
   });
 }
 

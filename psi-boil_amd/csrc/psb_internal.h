@@ -119,7 +119,20 @@ struct Graph {
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   long long key = -1;
+  double fkey = 0.0;                         // a floating-point argument baked in
   const void *px = nullptr, *pb = nullptr;  // caller arrays baked into the graph
+};
+
+// the reference's AC cycle: its own levels (smallest-prime-factor
+// coarsening), each with phi (e[0]), the right-hand side f (r) and the
+// work vectors of that level's CG (r, p0, p1, q); level 0's e[1] holds the
+// previous cycle's phi for the divergence restore
+struct RefHier {
+  std::vector<Level<double>> L;
+  std::vector<Metric> M;
+  std::vector<std::vector<double *>> K;
+  bool built = false;
+  Graph gr;
 };
 
 }  // namespace host
@@ -172,6 +185,8 @@ struct psb_ctx {
   int coarse_rule = 0; // 0 all directions at once, 1 x and y before z
   bool store_half = false;  // FP16 copies of the FP32 levels for the smoother
   int version = 0;     // bumped by every change of matrix, geometry or off mask
+  psb::host::RefHier href;             // the reference's AC cycle
+  std::vector<double> gdx, gdy, gdz;   // host copies of the cell sizes (empty: unit)
   std::string err;
 };
 

@@ -613,7 +613,7 @@ __global__ __launch_bounds__(kBlock) void k_bi3(Grid g, const T *c, Metric m, co
 enum FinOp {
   FIN_START_DIAG, FIN_START_AC, FIN_START_BICGS, FIN_PQ, FIN_XR_DIAG, FIN_XR_AC, FIN_RHO,
   FIN_XR_RU, FIN_IR_RESID, FIN_CENTER, FIN_RELRES, FIN_BI1, FIN_BI2, FIN_BI3, FIN_RU_START,
-  FIN_RU, FIN_RU_END
+  FIN_RU, FIN_RU_END, FIN_ACMEAN, FIN_ACRES
 };
 
 // stop logic shared by the Krylov loops (cg.cpp:121-160, bicgs.cpp:174-210),
@@ -794,6 +794,12 @@ __device__ inline void fin_apply(int op, const double *tot, DevState *st, DevSta
       }
       break;
     }
+    case FIN_ACMEAN:  // mean of the restricted right-hand side over active cells
+      st->mean[0] = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
+      break;
+    case FIN_ACRES:  // AC::residual: sqrt(dot_voldiv_avg(r, r)) * dt (scale = 1/dt)
+      out[0] = ref_norm(st, tot[0]);
+      break;
     case FIN_RU_END:  // true residual at exit without convergence
       st->res_true = st->norm == 0 ? ref_norm(st, tot[1]) : sqrt(tot[0]);
       break;
@@ -865,6 +871,115 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int op, int nblk, const dou
   double acc[NR];
   fin_sum<NR>(0, nblk, partial, acc);
   if (threadIdx.x == 0) fin_apply(op, acc, st, st2, out);
+}
+
+// ---------------------------------------------------------------------------
+// the reference's additive-correction cycle (AC::v_cycle,
+// additive_recursive.cpp:4-34): its restriction, interpolation and the
+// re-initialisation of the Krylov state between the level solves
+// ---------------------------------------------------------------------------
+
+// AC::restriction (additive_restriction.cpp:12-177) for any coarsening
+// factors: fH = sum_block f + external couplings * phi_outside - D phi, with
+// D = c - internal couplings, in the reference's order of operations; 0 on
+// inactive coarse cells.  Partials: sum fH and active count (for the mean).
+static __global__ __launch_bounds__(kBlock) void k_ac_restrict(Grid g, DMat<double> A,
+                                                        const double *phi, const double *f,
+                                                        Grid gH, int CI, int CJ, int CK,
+                                                        const uint8_t *afH, double *fH,
+                                                        Work w) {
+  double acc[2] = {0.0, 0.0};
+  for (int Q = blockIdx.x * blockDim.x + threadIdx.x; Q < gH.N; Q += gridDim.x * blockDim.x) {
+    int I, J, K;
+    gH.ijk(Q, I, J, K);
+    const int i0 = I * CI, j0 = J * CJ, k0 = K * CK;
+    auto F = [&](int a, int b, int c) { return ((i0 + a) * g.ny + (j0 + b)) * g.nz + (k0 + c); };
+    auto inside = [&](int a, int b, int c) {
+      return i0 + a >= 0 && i0 + a < g.nx && j0 + b >= 0 && j0 + b < g.ny && k0 + c >= 0 &&
+             k0 + c < g.nz;
+    };
+    auto PH = [&](int a, int b, int c) { return inside(a, b, c) ? phi[F(a, b, c)] : 0.0; };
+    // e, n, t of a cell (symmetric storage: the neighbour's w, s, b; 0 at walls)
+    auto e_ = [&](int a, int b, int c) { return i0 + a < g.nx - 1 ? A.w[F(a, b, c) + g.sI] : 0.0; };
+    auto n_ = [&](int a, int b, int c) { return j0 + b < g.ny - 1 ? A.s[F(a, b, c) + g.nz] : 0.0; };
+    auto t_ = [&](int a, int b, int c) { return k0 + c < g.nz - 1 ? A.b[F(a, b, c) + 1] : 0.0; };
+    double s = 0.0;
+    for (int a = 0; a < CI; a++)
+      for (int b = 0; b < CJ; b++)
+        for (int c = 0; c < CK; c++) s += f[F(a, b, c)];
+    for (int b = 0; b < CJ; b++)
+      for (int c = 0; c < CK; c++) {
+        s += e_(CI - 1, b, c) * PH(CI, b, c);
+        s += A.w[F(0, b, c)] * PH(-1, b, c);
+      }
+    for (int a = 0; a < CI; a++)
+      for (int c = 0; c < CK; c++) {
+        s += n_(a, CJ - 1, c) * PH(a, CJ, c);
+        s += A.s[F(a, 0, c)] * PH(a, -1, c);
+      }
+    for (int a = 0; a < CI; a++)
+      for (int b = 0; b < CJ; b++) {
+        s += t_(a, b, CK - 1) * PH(a, b, CK);
+        s += A.b[F(a, b, 0)] * PH(a, b, -1);
+      }
+    for (int a = 0; a < CI; a++)
+      for (int b = 0; b < CJ; b++)
+        for (int c = 0; c < CK; c++) {
+          // D of the cell: c minus its couplings inside the block, in the
+          // reference's order (e, w, n, s, t, b)
+          double d = A.c[F(a, b, c)];
+          if (a < CI - 1) d -= e_(a, b, c);
+          if (a > 0) d -= A.w[F(a, b, c)];
+          if (b < CJ - 1) d -= n_(a, b, c);
+          if (b > 0) d -= A.s[F(a, b, c)];
+          if (c < CK - 1) d -= t_(a, b, c);
+          if (c > 0) d -= A.b[F(a, b, c)];
+          s -= d * phi[F(a, b, c)];
+        }
+    const bool act = !afH || afH[Q];
+    fH[Q] = act ? s : 0.0;
+    if (act) {
+      acc[0] += s;
+      acc[1] += 1.0;
+    }
+  }
+  block_partial<2>(acc, w.partial);
+}
+
+// mean removal of the restricted right-hand side (additive_restriction.cpp:151-174)
+static __global__ void k_ac_submean(Grid gH, double *fH, const uint8_t *afH, const DevState *st) {
+  const double m = st->mean[0];
+  for (int Q = blockIdx.x * blockDim.x + threadIdx.x; Q < gH.N; Q += gridDim.x * blockDim.x)
+    if (!afH || afH[Q]) fH[Q] = fH[Q] - m;
+}
+
+// AC::interpolation (additive_interpolation.cpp:12-64): phi_h += phi_H
+static __global__ void k_ac_interp(Grid g, double *phi, Grid gH, const double *phiH, int CI, int CJ,
+                            int CK) {
+  for (int P = blockIdx.x * blockDim.x + threadIdx.x; P < g.N; P += gridDim.x * blockDim.x) {
+    int i, j, k;
+    g.ijk(P, i, j, k);
+    phi[P] = phi[P] + phiH[((i / CI) * gH.ny + (j / CJ)) * gH.nz + (k / CK)];
+  }
+}
+
+// stopping parameters of the next level solve (Linear::solve arguments of
+// AC::call_smoother / call_solver); the start kernel's finalize resets the
+// Krylov scalars
+static __global__ void k_stinit(DevState *st, int maxit, double res_rat, double res_tol, double scale,
+                         double ntot) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  st->res_rat = res_rat;
+  st->res_tol = res_tol;
+  st->scale = scale;
+  st->ntot = ntot;
+  st->mini = 1;
+  st->maxit = maxit;
+  st->stalecount = -1;
+  st->norm = 0;
+  st->hist = nullptr;
+  st->hist_cap = 0;
+  st->done = 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -61,6 +61,11 @@ class Result(C.Structure):
         return d
 
 
+class Cycle(C.Structure):
+    _fields_ = [("res_tol", C.c_double), ("res_rat", C.c_double), ("max_cycles", C.c_int),
+                ("min_cycles", C.c_int), ("max_iter", C.c_int * 3), ("dt", C.c_double)]
+
+
 class Case(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nsph", C.c_int),
                 ("rmin", C.c_double), ("rmax", C.c_double), ("rho_in", C.c_double),
@@ -75,7 +80,8 @@ EXPORTS = ["psb_stop_defaults", "psb_status_string", "psb_create", "psb_destroy"
            "psb_solve_device", "psb_apply", "psb_history", "psb_generate",
            "psb_time_kernel", "psb_batch_cases", "psb_batch_create", "psb_batch_destroy",
            "psb_batch_system", "psb_batch_set_tuning", "psb_batch_plan", "psb_batch_solve",
-           "psb_batch_solve_device"]
+           "psb_batch_solve_device", "psb_cycle_defaults", "psb_ac_vcycle",
+           "psb_ac_vcycle_device"]
 
 _lib = None
 
@@ -181,6 +187,35 @@ class Solver:
         r = Result()
         _check(lib().psb_solve_device(self._c, C.c_void_p(d_x), C.c_void_p(d_b), C.byref(s),
                                       C.byref(r)), "psb_solve_device")
+        return r.asdict()
+
+    def ac_vcycle(self, x, b, **cyc):
+        """The reference's AC::vcycle (CG smoothers on its own levels)."""
+        x = np.array(self._host(x), copy=True)
+        b = self._host(b)
+        c = Cycle()
+        lib().psb_cycle_defaults(C.byref(c))
+        for k, v in cyc.items():
+            if k == "max_iter":
+                c.max_iter[:] = list(v)
+            else:
+                setattr(c, k, v)
+        r = Result()
+        _check(lib().psb_ac_vcycle(self._c, _dp(x), _dp(b), C.byref(c), C.byref(r)),
+               "psb_ac_vcycle")
+        return x, r.asdict()
+
+    def ac_vcycle_device(self, d_x, d_b, **cyc):
+        c = Cycle()
+        lib().psb_cycle_defaults(C.byref(c))
+        for k, v in cyc.items():
+            if k == "max_iter":
+                c.max_iter[:] = list(v)
+            else:
+                setattr(c, k, v)
+        r = Result()
+        _check(lib().psb_ac_vcycle_device(self._c, C.c_void_p(d_x), C.c_void_p(d_b),
+                                          C.byref(c), C.byref(r)), "psb_ac_vcycle_device")
         return r.asdict()
 
     def apply(self, x, b=None):

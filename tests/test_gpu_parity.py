@@ -313,3 +313,56 @@ def test_acpcg_with_solid_cells_and_stretched_grid(adapt, rtol):
     assert r["converged"] and r["rel_l2"] <= 1e-9 * 1.01
     off = d["off"].astype(bool)
     assert np.all(x[off] == 0.0)
+
+
+def test_reference_ac_vcycle_matches_reference(golden):
+    """The reference's AC::vcycle (CG smoothers on its own levels) on the
+    GPU against the reference's own results: one cycle, and the full solve to
+    ResRat 1e-8 (cycles, convergence, per-cycle residuals, solution).  Only
+    the order of the dot-product sums differs (tree vs sequential), so
+    iterates agree to rounding amplified by 20-100 CG iterations per level.
+    The constant component is compared apart: 100 CG iterations on the
+    coarsest (singular, Neumann) level run far past convergence and drift
+    along the constants by an amount set by rounding alone (0.13 on a field
+    of size 30 in the 16^3 case); the residual does not see it."""
+    s = solver_for(golden)
+    b = golden["b"]
+    n = b.size
+    x1, r1 = s.ac_vcycle(np.zeros(n), b, res_rat=1e-30, max_cycles=1)
+    assert r1["iterations"] == 1
+    ref1 = golden["ac_phi1"]
+    assert rel(x1 - x1.mean(), ref1 - ref1.mean()) < 1e-8
+    x, r = s.ac_vcycle(np.zeros(n), b, res_rat=1e-8, max_cycles=20)
+    ref_full = [q for q in golden.info["acret"] if q[0] == "full"][0]
+    assert r["iterations"] == ref_full[2] and bool(r["converged"]) == bool(ref_full[1])
+    h = s.history()
+    cyc = golden.info["cycles"][-ref_full[2]:]
+    for c, res in cyc:
+        assert h[c] == pytest.approx(res, rel=1e-7)
+    reff = golden["ac_phifull"]
+    assert rel(x - x.mean(), reff - reff.mean()) < 1e-7
+    if r["converged"]:
+        assert r["status"] == psb.PSB_OK and r["res"] < 1e-8 * r["res0"]
+
+
+def test_reference_ac_vcycle_on_stretched_grid_with_solids_matches_oracle():
+    nx, ny, nz = 24, 12, 18          # 18 = 2 * 3 * 3: levels by factors 2, 3
+    d = psb.generate(nx, ny, nz, 4, 0.02, 0.06, 1.0, 100.0, 17, 1017, stretch_z=1.5, lz=1.5,
+                     cylinder=True, cyl_d=0.4)
+    assert d["off"].sum() > 0
+    A = O.Mat(nx, ny, nz, {k: d[k] for k in "cwesnbt"})
+    dV = (d["dxc"][:, None, None] * d["dyc"][None, :, None] * d["dzc"][None, None, :]).ravel()
+    b, n = d["rhs"], d["rhs"].size
+    s = psb.Solver(nx, ny, nz)
+    s.set_active(d["off"])
+    s.set_matrix(d)
+    s.set_geometry(d["dxc"], d["dyc"], d["dzc"])
+    ac = O.AC(A, dV, off=d["off"], rule=0)
+    assert O.levels(0, nx, ny, nz) == [(24, 12, 18), (12, 6, 9), (6, 3, 3), (3, 3, 3)]
+    xo, io = ac.vcycle(np.zeros(n), b, factor=1e-6, max_cyc=20, dt=0.5)
+    x, r = s.ac_vcycle(np.zeros(n), b, res_rat=1e-6, max_cycles=20, dt=0.5)
+    assert r["iterations"] == io["ncyc"] and bool(r["converged"]) == io["converged"]
+    h = s.history()
+    assert np.allclose(h[:len(io["hist"])], io["hist"], rtol=1e-6)
+    fl = d["off"] == 0
+    assert rel(x[fl] - x[fl].mean(), xo[fl] - xo[fl].mean()) < 1e-6
