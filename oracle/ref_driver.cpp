@@ -10,6 +10,7 @@
 //   ref_driver case  OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel [cx cy cz]
 //   ref_driver bench-cg nx nsph rmin rmax rho_in rho_out seed_sph seed_vel maxit
 //   ref_driver bench-ac nx nsph rmin rmax rho_in rho_out seed_sph seed_vel cycles
+//   (bench modes: optional cx cy cz after the limit for one centred sphere)
 //
 // Private and protected members of the reference's classes (AC::L,
 // Centered::A, ...) are reached by widening access before its headers are
@@ -404,6 +405,10 @@ int main(int argc, char **argv) {
     int n = std::atoi(argv[2]);
     Case cs = parse_case(argv + 3, n, n, n);
     int lim = std::atoi(argv[10]);
+    if (argc >= 14) {  // one sphere centred at (cx, cy, cz), radius rmin
+      cs.centred = true;
+      cs.cx = std::atof(argv[11]); cs.cy = std::atof(argv[12]); cs.cz = std::atof(argv[13]);
+    }
     double t0 = now();
     World w = build_world(cs);
     double t1 = now();
