@@ -144,6 +144,36 @@ int psb_set_matrix(psb_ctx *ctx, const psb_matrix7 *A);
  * takes effect at the next psb_set_matrix.                                  */
 int psb_set_active(psb_ctx *ctx, const unsigned char *off);
 
+/* ------------------------------------------------------------------------
+ * Matrix and right-hand side built on the device (the solve's producers):
+ * Pressure::discretize (pressure_discretize.cpp:11-151, with the wall
+ * correction of centered_system_bnd.cpp:19-61) and Pressure::update_rhs
+ * (pressure_update_rhs.cpp:16-139, dt = 1).  Only the density (one array)
+ * crosses PCIe instead of seven coefficient arrays; the matrix is the one
+ * psb_set_matrix would receive from the reference, bit for bit.
+ * ------------------------------------------------------------------------ */
+typedef struct {   /* per direction, as Domain::dxc/dxw/dxe (domain.h:82-90) */
+  const double *dxc, *dxw, *dxe;  /* cell size, distance to the west / east centre */
+  const double *dyc, *dys, *dyn;
+  const double *dzc, *dzb, *dzt;
+} psb_metrics;
+
+/* rho: density with one ghost layer, (nx+2)(ny+2)(nz+2) values, [i][j][k]
+ * with k fastest (the reference's colour field with Neumann ghosts); off:
+ * solid cells (compact interior order, 1 = solid) or NULL.  Replaces
+ * psb_set_matrix, psb_set_active and psb_set_geometry (dxc, dyc, dzc).      */
+int psb_assemble_pressure(psb_ctx *ctx, const double *rho, const psb_metrics *m,
+                          const unsigned char *off);
+int psb_assemble_pressure_device(psb_ctx *ctx, const double *d_rho,
+                                 const psb_metrics *m, const unsigned char *off);
+
+/* After psb_assemble_pressure*: d_b = (u_w - u_e) dSx + (v_s - v_n) dSy +
+ * (w_b - w_t) dSz from the device face velocities u ((nx+1) ny nz), v
+ * (nx (ny+1) nz), w (nx ny (nz+1)), 0 on solid cells; zero_mean: minus the
+ * mean over the fluid cells (the benchmark's compatible right-hand side).    */
+int psb_update_rhs_device(psb_ctx *ctx, const double *d_u, const double *d_v,
+                          const double *d_w, int zero_mean, double *d_b);
+
 /* Linear::solve (linear.h:46-56): x (in: initial guess, out: solution) and b
  * are host arrays in the context's layout.                                  */
 int psb_solve(psb_ctx *ctx, double *x, const double *b, const psb_stop *stop,

@@ -1469,6 +1469,8 @@ int psb_set_matrix(psb_ctx *C, const psb_matrix7 *A) {
                             C->stream));
     }
     C->aflag0 = nullptr;
+    C->metrics = DevMetrics{};
+    C->off_dev = nullptr;
     if (!C->off.empty()) {
       std::vector<uint8_t> fl(N);
       for (int q = 0; q < N; q++) fl[q] = C->off[q] ? 0 : 1;
@@ -1480,6 +1482,110 @@ int psb_set_matrix(psb_ctx *C, const psb_matrix7 *A) {
     C->have_matrix = true;
     C->version++;
     return PSB_OK;
+  });
+}
+
+namespace {
+
+// drop everything built from the previous matrix (graphs, hierarchies,
+// Krylov vectors, the matrix itself)
+void reset_matrix(psb_ctx *C) {
+  HIPCHK(hipStreamSynchronize(C->stream));
+  destroy_graphs(C);
+  C->arena.release();
+  C->harena.release();
+  for (int q = 0; q < 7; q++) C->A64[q] = nullptr;
+  C->A32[0] = C->A32[1] = C->A32[2] = C->A32[3] = nullptr;
+  C->have32 = false;
+  C->h64 = Hier<double>();
+  C->h32 = Hier<float>();
+  C->href = RefHier();
+  C->k64 = KrylovVecs<double>();
+  C->k32 = KrylovVecs<float>();
+  C->hist_count = 0;
+  C->have_matrix = false;
+}
+
+int assemble_pressure(psb_ctx *C, const double *rho, bool rho_dev, const psb_metrics *m,
+                      const unsigned char *off) {
+  const double *arrs[9] = {m->dxc, m->dxw, m->dxe, m->dyc, m->dys, m->dyn, m->dzc, m->dzb, m->dzt};
+  for (const double *a : arrs)
+    if (!a) return PSB_EINVAL;
+  const Grid &g = C->g;
+  const int len[9] = {g.nx, g.nx, g.nx, g.ny, g.ny, g.ny, g.nz, g.nz, g.nz};
+  int rc = psb_set_geometry(C, m->dxc, m->dyc, m->dzc);
+  if (rc != PSB_OK) return rc;
+  reset_matrix(C);
+  double *dm[9];
+  for (int q = 0; q < 9; q++) {
+    dm[q] = C->arena.alloc<double>(len[q]);
+    HIPCHK(hipMemcpyAsync(dm[q], arrs[q], len[q] * 8, hipMemcpyHostToDevice, C->stream));
+  }
+  const DevMetrics M{dm[0], dm[1], dm[2], dm[3], dm[4], dm[5], dm[6], dm[7], dm[8]};
+  const size_t NG = (size_t)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+  const double *drho = rho;
+  if (!rho_dev) {
+    double *t = C->arena.alloc<double>(NG);
+    HIPCHK(hipMemcpyAsync(t, rho, NG * 8, hipMemcpyHostToDevice, C->stream));
+    drho = t;
+  }
+  uint8_t *doff = nullptr;
+  if (off) {
+    C->off.assign(off, off + g.N);
+    doff = C->arena.alloc<uint8_t>(g.N);
+    HIPCHK(hipMemcpyAsync(doff, C->off.data(), g.N, hipMemcpyHostToDevice, C->stream));
+    std::vector<uint8_t> fl(g.N);
+    for (int q = 0; q < g.N; q++) fl[q] = C->off[q] ? 0 : 1;
+    C->aflag0 = C->arena.alloc<uint8_t>(g.N);
+    HIPCHK(hipMemcpyAsync(C->aflag0, fl.data(), g.N, hipMemcpyHostToDevice, C->stream));
+  } else {
+    C->off.clear();
+    C->aflag0 = nullptr;
+  }
+  for (int q = 0; q < 4; q++) C->A64[q] = C->arena.alloc<double>(g.N);
+  hipLaunchKernelGGL(k_discretize, dim3(grid_blocks(g.N)), dim3(kBlock), 0, C->stream, g, M,
+                     drho, doff, C->A64[0], C->A64[1], C->A64[2], C->A64[3]);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(C->stream));  // host sources above are the caller's or local
+  C->metrics = M;
+  C->off_dev = doff;
+  C->sym = true;
+  C->have_matrix = true;
+  C->version++;
+  return PSB_OK;
+}
+
+}  // namespace
+
+int psb_assemble_pressure(psb_ctx *C, const double *rho, const psb_metrics *m,
+                          const unsigned char *off) {
+  if (!C || !rho || !m) return PSB_EINVAL;
+  return guarded(C, [&]() { return assemble_pressure(C, rho, false, m, off); });
+}
+
+int psb_assemble_pressure_device(psb_ctx *C, const double *d_rho, const psb_metrics *m,
+                                 const unsigned char *off) {
+  if (!C || !d_rho || !m) return PSB_EINVAL;
+  return guarded(C, [&]() { return assemble_pressure(C, d_rho, true, m, off); });
+}
+
+int psb_update_rhs_device(psb_ctx *C, const double *d_u, const double *d_v, const double *d_w,
+                          int zero_mean, double *d_b) {
+  if (!C || !d_u || !d_v || !d_w || !d_b) return PSB_EINVAL;
+  if (!C->have_matrix || !C->metrics.dxc) return PSB_ENOTREADY;
+  return guarded(C, [&]() {
+    const Grid &g = C->g;
+    const int nb = grid_blocks(g.N);
+    hipLaunchKernelGGL(k_update_rhs, dim3(nb), dim3(kBlock), 0, C->stream, g, C->metrics, d_u,
+                       d_v, d_w, C->off_dev, d_b, C->work);
+    if (zero_mean) {
+      finalize<2>(C, FIN_ACMEAN, nb, C->st32);
+      hipLaunchKernelGGL(k_rhs_center, dim3(nb), dim3(kBlock), 0, C->stream, g, d_b,
+                         C->off_dev, C->st32);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return (int)PSB_OK;
   });
 }
 

@@ -186,6 +186,8 @@ struct psb_ctx {
   bool store_half = false;  // FP16 copies of the FP32 levels for the smoother
   int version = 0;     // bumped by every change of matrix, geometry or off mask
   psb::host::RefHier href;             // the reference's AC cycle
+  psb::DevMetrics metrics{};           // metrics of an assembled matrix (device)
+  uint8_t *off_dev = nullptr;          // solid cells of an assembled matrix (device)
   std::vector<double> gdx, gdy, gdz;   // host copies of the cell sizes (empty: unit)
   std::string err;
 };

@@ -983,6 +983,110 @@ static __global__ void k_stinit(DevState *st, int maxit, double res_rat, double 
 }
 
 // ---------------------------------------------------------------------------
+// Pressure::discretize and Pressure::update_rhs on the device
+// ---------------------------------------------------------------------------
+struct DevMetrics {  // per direction: cell size, distance to the lower / upper centre
+  const double *dxc, *dxw, *dxe, *dyc, *dys, *dyn, *dzc, *dzb, *dzt;
+};
+
+// pressure_discretize.cpp:24-148 with centered_system_bnd.cpp:27-61, one
+// cell per thread: the six couplings dS / (d (rho_P + rho_nb)/2) from the
+// density with its ghost layer, c their sum, couplings to solid cells
+// zeroed on both sides (c of a fluid cell re-summed, of a solid cell kept),
+// the wall couplings moved out of c (imin, imax, jmin, jmax, kmin, kmax),
+// c of solid cells scaled by 1e-12.  The same operations in the same order
+// as the reference, so the matrix is bit-identical; stored symmetric
+// (c, w, s, b: e, n, t are the neighbours' w, s, b).
+static __global__ void k_discretize(Grid g, DevMetrics m, const double *rho, const uint8_t *off,
+                                    double *C, double *W, double *S, double *B) {
+  const int gy = g.ny + 2, gz = g.nz + 2;
+  for (int P = blockIdx.x * blockDim.x + threadIdx.x; P < g.N; P += gridDim.x * blockDim.x) {
+    int i, j, k;
+    g.ijk(P, i, j, k);
+    auto R = [&](int a, int b, int c) {
+      return rho[((size_t)(a + 1) * gy + (b + 1)) * gz + (c + 1)];
+    };
+    const double rp = R(i, j, k);
+    const double a_x = m.dyc[j] * m.dzc[k];
+    const double a_y = m.dxc[i] * m.dzc[k];
+    const double a_z = m.dxc[i] * m.dyc[j];
+    double w = a_x / m.dxw[i] / (0.5 * (rp + R(i - 1, j, k)));
+    double e = a_x / m.dxe[i] / (0.5 * (R(i + 1, j, k) + rp));
+    double s = a_y / m.dys[j] / (0.5 * (rp + R(i, j - 1, k)));
+    double n = a_y / m.dyn[j] / (0.5 * (R(i, j + 1, k) + rp));
+    double b = a_z / m.dzb[k] / (0.5 * (rp + R(i, j, k - 1)));
+    double t = a_z / m.dzt[k] / (0.5 * (R(i, j, k + 1) + rp));
+    double c = w + e + s + n + b + t;
+    if (off) {
+      if (off[P]) {
+        w = e = s = n = b = t = 0.0;
+      } else {
+        if (i > 0 && off[P - g.sI]) w = 0.0;
+        if (i < g.nx - 1 && off[P + g.sI]) e = 0.0;
+        if (j > 0 && off[P - g.nz]) s = 0.0;
+        if (j < g.ny - 1 && off[P + g.nz]) n = 0.0;
+        if (k > 0 && off[P - 1]) b = 0.0;
+        if (k < g.nz - 1 && off[P + 1]) t = 0.0;
+        c = w + e + s + n + b + t;
+      }
+    }
+    if (i == 0) { c -= w; w = 0.0; }
+    if (i == g.nx - 1) { c -= e; }
+    if (j == 0) { c -= s; s = 0.0; }
+    if (j == g.ny - 1) { c -= n; }
+    if (k == 0) { c -= b; b = 0.0; }
+    if (k == g.nz - 1) { c -= t; }
+    if (off && off[P]) c *= 1.0e-12;  // Src/Global/global_constants.cpp:17 (pico)
+    C[P] = c;
+    W[P] = w;
+    S[P] = s;
+    B[P] = b;
+  }
+}
+
+// pressure_update_rhs.cpp:27-52 with dt = 1: b = (u_w - u_e) dSx + (v_s -
+// v_n) dSy + (w_b - w_t) dSz from the staggered face velocities (u of
+// (nx+1) x ny x nz faces, v of nx x (ny+1) x nz, w of nx x ny x (nz+1)); 0 on
+// solid cells.  Partials: sum b and fluid cells (for the optional zero mean).
+static __global__ __launch_bounds__(kBlock) void k_update_rhs(Grid g, DevMetrics m,
+                                                              const double *u, const double *v,
+                                                              const double *wv,
+                                                              const uint8_t *off, double *bout,
+                                                              Work w) {
+  double acc[2] = {0.0, 0.0};
+  for (int P = blockIdx.x * blockDim.x + threadIdx.x; P < g.N; P += gridDim.x * blockDim.x) {
+    int i, j, k;
+    g.ijk(P, i, j, k);
+    const double a_x = m.dyc[j] * m.dzc[k] * 1.0;
+    const double a_y = m.dxc[i] * m.dzc[k] * 1.0;
+    const double a_z = m.dxc[i] * m.dyc[j] * 1.0;
+    const size_t ny = g.ny, nz = g.nz;
+    double f = 0.0;
+    f += a_x * u[((size_t)i * ny + j) * nz + k];
+    f -= a_x * u[((size_t)(i + 1) * ny + j) * nz + k];
+    f += a_y * v[((size_t)i * (ny + 1) + j) * nz + k];
+    f -= a_y * v[((size_t)i * (ny + 1) + j + 1) * nz + k];
+    f += a_z * wv[((size_t)i * ny + j) * (nz + 1) + k];
+    f -= a_z * wv[((size_t)i * ny + j) * (nz + 1) + k + 1];
+    const bool solid = off && off[P];
+    if (solid) f = 0.0;
+    bout[P] = f;
+    if (!solid) {
+      acc[0] += f;
+      acc[1] += 1.0;
+    }
+  }
+  block_partial<2>(acc, w.partial);
+}
+
+// b -= mean over the fluid cells (the benchmark's zero-mean right-hand side)
+static __global__ void k_rhs_center(Grid g, double *b, const uint8_t *off, const DevState *st) {
+  const double mean = st->mean[0];
+  for (int P = blockIdx.x * blockDim.x + threadIdx.x; P < g.N; P += gridDim.x * blockDim.x)
+    if (!(off && off[P])) b[P] = b[P] - mean;
+}
+
+// ---------------------------------------------------------------------------
 // set-up kernels
 // ---------------------------------------------------------------------------
 template <typename T>

@@ -61,6 +61,11 @@ class Result(C.Structure):
         return d
 
 
+class Metrics(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("dxc", "dxw", "dxe", "dyc", "dys", "dyn", "dzc",
+                                          "dzb", "dzt")]
+
+
 class Cycle(C.Structure):
     _fields_ = [("res_tol", C.c_double), ("res_rat", C.c_double), ("max_cycles", C.c_int),
                 ("min_cycles", C.c_int), ("max_iter", C.c_int * 3), ("dt", C.c_double)]
@@ -81,7 +86,8 @@ EXPORTS = ["psb_stop_defaults", "psb_status_string", "psb_create", "psb_destroy"
            "psb_time_kernel", "psb_batch_cases", "psb_batch_create", "psb_batch_destroy",
            "psb_batch_system", "psb_batch_set_tuning", "psb_batch_plan", "psb_batch_solve",
            "psb_batch_solve_device", "psb_cycle_defaults", "psb_ac_vcycle",
-           "psb_ac_vcycle_device"]
+           "psb_ac_vcycle_device", "psb_assemble_pressure", "psb_assemble_pressure_device",
+           "psb_update_rhs_device"]
 
 _lib = None
 
@@ -188,6 +194,24 @@ class Solver:
         _check(lib().psb_solve_device(self._c, C.c_void_p(d_x), C.c_void_p(d_b), C.byref(s),
                                       C.byref(r)), "psb_solve_device")
         return r.asdict()
+
+    def assemble_pressure(self, rho, mx, my, mz, off=None):
+        """Pressure::discretize on the device.  rho with one ghost layer;
+        mx = (dxc, dxw, dxe), my = (dyc, dys, dyn), mz = (dzc, dzb, dzt)."""
+        self._met = [np.ascontiguousarray(a, dtype=np.float64) for a in (*mx, *my, *mz)]
+        m = Metrics(*[a.ctypes.data for a in self._met])
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        offp = None
+        if off is not None:
+            self._off = np.ascontiguousarray(off, dtype=np.uint8).ravel()
+            offp = _dp(self._off)
+        return _check(lib().psb_assemble_pressure(self._c, _dp(rho), C.byref(m), offp),
+                      "psb_assemble_pressure")
+
+    def update_rhs_device(self, d_u, d_v, d_w, d_b, zero_mean=True):
+        return _check(lib().psb_update_rhs_device(self._c, C.c_void_p(d_u), C.c_void_p(d_v),
+                                                  C.c_void_p(d_w), int(zero_mean),
+                                                  C.c_void_p(d_b)), "psb_update_rhs_device")
 
     def ac_vcycle(self, x, b, **cyc):
         """The reference's AC::vcycle (CG smoothers on its own levels)."""
