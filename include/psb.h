@@ -107,6 +107,10 @@ typedef struct {
                             (half the smoother's matrix bytes; the CG
                             operator and the residual updates keep FP32 /
                             FP64)                                          */
+  int ac_fuse;           /* PSB_MIXED: 1 runs the two post-sweeps of a level
+                            in one pass (2.5-D marching tile) where the
+                            level's shape allows; 0 one kernel per sweep
+                            (default: faster on MI355X today; same z)      */
 } psb_stop;
 
 typedef struct {
@@ -302,8 +306,10 @@ int psb_batch_solve_device(psb_batch *bt, double *const *d_x,
  * 0 FP64 SpMV, 1 FP32 SpMV, 2 FP32 damped Jacobi sweep on the fine level as
  * the last mixed-precision solve set it up (flux form; FP16 matrix copy with
  * ac_store = 1), 3 plain FP32 copy of the grid (reference stream for
- * hardware byte counters).
- * *bytes receives the algorithmic bytes one launch moves.                   */
+ * hardware byte counters), 4 the fine level's two post-sweeps in one pass
+ * (ac_fuse = 1 form), 5 the same two sweeps as two kernels.
+ * *bytes receives the algorithmic bytes one launch moves (for 4 and 5:
+ * those of one sweep).                                                      */
 int psb_time_kernel(psb_ctx *ctx, int kernel, int reps, double *avg_ms,
                     double *bytes);
 

@@ -23,6 +23,7 @@
 #include <vector>
 
 
+#include "psb_fused.h"
 #include "psb_internal.h"
 
 // FP32 levels evaluate the stencil in flux form (set to 0 for the plain form)
@@ -185,6 +186,53 @@ void jacobi(psb_ctx *C, const Level<T> &lv, const T *e, T *eout, const Level<T> 
     }
   }
   jacobi_m<T, RHO, T, false>(C, lv, lv.mat(), e, eout, coarse, eH, cfac, omega, zscale, st);
+}
+
+// the two post-sweeps of level lv in one pass (k_post2): the first with the
+// prolongation of eH from `coarse`, the second writing eout (and the rho
+// partials when RHO).  False when the level's shape does not suit it.
+template <bool RHO, typename M, bool FLUX>
+void post2_m(psb_ctx *C, const Level<float> &lv, DMat<M> A, const float *e, float *eout,
+             const Level<float> &coarse, const float *eH, float cfac, float omega, float zscale,
+             DevState *st, const FuseShape &f) {
+  ProlX<float, 4> pr{};
+  pr.e = e;
+  pr.eH = eH;
+  pr.af = lv.aflag;
+  pr.cfac = cfac;
+  pr.sI = coarse.CI == 2 ? 1 : 0;
+  pr.sJ = coarse.CJ == 2 ? 1 : 0;
+  pr.sK = coarse.CK == 2 ? 1 : 0;
+  pr.nYH = coarse.g.ny;
+  pr.nZH = coarse.g.nz;
+  hipLaunchKernelGGL((k_post2<M, FLUX, RHO>), dim3(f.nblocks), dim3(kBlock), 0, C->stream, lv.g,
+                     A, lv.r, eout, pr, omega, lv.aflag, st, C->work, zscale, f.R, f.TI,
+                     f.ntiles);
+  if (RHO) finalize<4>(C, FIN_RHO, f.nblocks, st);
+}
+
+template <typename T, bool RHO>
+bool post2(psb_ctx *C, const Level<T> &lv, const T *e, T *eout, const Level<T> &coarse,
+           const T *eH, T cfac, T omega, T zscale, DevState *st) {
+  if constexpr (std::is_same<T, float>::value) {
+    FuseShape f;
+    if (!C->fuse || !fuse_shape(lv.g, f)) return false;
+    if (lv.c16 && lv.d)
+      post2_m<RHO, _Float16, true>(C, lv, lv.mat16(), e, eout, coarse, eH, cfac, omega, zscale,
+                                   st, f);
+    else if (lv.c16)
+      post2_m<RHO, _Float16, false>(C, lv, lv.mat16(), e, eout, coarse, eH, cfac, omega, zscale,
+                                    st, f);
+    else if (lv.d)
+      post2_m<RHO, float, true>(C, lv, lv.matflux(), e, eout, coarse, eH, cfac, omega, zscale,
+                                st, f);
+    else
+      post2_m<RHO, float, false>(C, lv, lv.mat(), e, eout, coarse, eH, cfac, omega, zscale, st,
+                                 f);
+    return true;
+  } else {
+    return false;
+  }
 }
 
 template <typename T, int V, int KV, int CI, int CJ, int CK>
@@ -582,6 +630,14 @@ T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega
   restrict_<T>(C, lv, lv.e[cur], cL, T(omega / cL.scale), st);
   T *zH = vcycle<T>(C, H, l + 1, st, nu1, nu2, omega, cfac);
   const T cfac_l = T(cfac * (cL.scale / lv.scale));
+  if (nu2 == 2) {  // both post-sweeps in one pass where the level suits it
+    const int out = (cur + 2) % 3;
+    const bool done = l == 0 ? post2<T, true>(C, lv, lv.e[cur], lv.e[out], cL, zH, cfac_l, omega,
+                                              T(lv.scale), st)
+                             : post2<T, false>(C, lv, lv.e[cur], lv.e[out], cL, zH, cfac_l,
+                                               omega, T(1), st);
+    if (done) return lv.e[out];
+  }
   for (int s = 0; s < nu2; s++) {
     const int nxt = (cur + 1) % 3;
     const bool last = (s == nu2 - 1) && l == 0;
@@ -1249,6 +1305,10 @@ int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_
     if (p->ac_store < 0 || p->ac_store > 1) return PSB_EINVAL;
     // the storage choice concerns only the FP32 levels: FP64 solves keep it
     ensure_rule(C, p->ac_coarsen, p->precision == PSB_MIXED ? p->ac_store == 1 : C->store_half);
+    if ((p->ac_fuse != 0) != C->fuse) {  // the captured graphs hold the other kernels
+      destroy_graphs(C);
+      C->fuse = p->ac_fuse != 0;
+    }
     if (p->precision == PSB_MIXED && p->inner_adapt == 2)
       solve_ru(C, x, b, p, R);
     else if (p->precision == PSB_MIXED)
@@ -1330,6 +1390,7 @@ void psb_stop_defaults(psb_stop *s) {
   s->inner_adapt = 0;
   s->ac_coarsen = 0;
   s->ac_store = 0;
+  s->ac_fuse = 0;
 }
 
 const char *psb_status_string(int s) {
@@ -1751,6 +1812,30 @@ int psb_time_kernel(psb_ctx *C, int kernel, int reps, double *avg_ms, double *by
       // the level-0 matrix as the last mixed solve stored it (4 arrays, FP32
       // or FP16), e and r in, e out
       *bytes = N * (4 * (lv.c16 ? 2 : 4) + 3 * 4);
+    } else if (kernel == 4 || kernel == 5) {
+      // the two post-sweeps of the fine level: 4 in one pass (k_post2), 5 as
+      // two kernels; both as the last mixed solve's hierarchy stores the level
+      build_hier32(C);
+      Level<float> lv = C->h32.L[0];
+      const Level<float> cL = C->h32.L[1];
+      zero(C, lv.e[0], g.N);
+      zero(C, lv.r, g.N);
+      zero(C, cL.e[0], cL.g.N);
+      DevState h;
+      std::memset(&h, 0, sizeof(h));
+      HIPCHK(hipMemcpyAsync(C->st32, &h, sizeof(h), hipMemcpyHostToDevice, stream));
+      const bool fused = kernel == 4;
+      f = [=]() {
+        const bool keep = C->fuse;
+        C->fuse = true;  // time the fused pass whatever the last solve selected
+        const bool done = fused && post2<float, false>(C, lv, lv.e[0], lv.e[2], cL, cL.e[0], 2.0f,
+                                                       0.9f, 1.0f, C->st32);
+        C->fuse = keep;
+        if (done) return;
+        jacobi<float, false>(C, lv, lv.e[0], lv.e[1], &cL, cL.e[0], 2.0f, 0.9f, C->st32);
+        jacobi<float, false>(C, lv, lv.e[1], lv.e[2], nullptr, nullptr, 0.0f, 0.9f, C->st32);
+      };
+      *bytes = N * (4 * (lv.c16 ? 2 : 4) + 3 * 4);  // one sweep's algorithmic bytes
     } else if (kernel == 3) {
       ensure32(C);
       ensure_krylov(C, C->k32, false);

@@ -356,14 +356,36 @@ struct ProlX {
     if (af && !af[Q]) return e[Q];
     return e[Q] + cfac * eH[parent(i, j, k)];
   }
+  // one vector load of e, one of the V (sK = 0) or V / 2 (sK = 1) parents
+  // and one of the V flags: k is a multiple of V and so are nz and the
+  // coarse row length, so every load is aligned to its size
+  static_assert(V == 1 || V == 2 || V == 4, "flags of a vector are read as one word");
   __device__ __forceinline__ VecN<T, V> vec(int P, int i, int j, int k) const {
     VecN<T, V> E = ldv<T, V>(e + P), o;
     const int base = ((i >> sI) * nYH + (j >> sJ)) * nZH;
+    T h[V];
+    if constexpr (V >= 2) {
+      if (sK) {
+        const VecN<T, V / 2> H = ldv<T, V / 2>(eH + base + (k >> 1));
 #pragma unroll
-    for (int q = 0; q < V; q++) {
-      const bool act = !af || af[P + q];
-      o.v[q] = act ? E.v[q] + cfac * eH[base + ((k + q) >> sK)] : E.v[q];
+        for (int q = 0; q < V; q++) h[q] = H.v[q >> 1];
+      } else {
+        const VecN<T, V> H = ldv<T, V>(eH + base + k);
+#pragma unroll
+        for (int q = 0; q < V; q++) h[q] = H.v[q];
+      }
+    } else {
+      h[0] = eH[base + (k >> sK)];
     }
+    uint32_t act = 0xffffffffu;
+    if (af) {
+      if constexpr (V == 4) act = *reinterpret_cast<const uint32_t *>(af + P);
+      else if constexpr (V == 2) act = *reinterpret_cast<const uint16_t *>(af + P);
+      else act = af[P];
+    }
+#pragma unroll
+    for (int q = 0; q < V; q++)
+      o.v[q] = ((act >> (8 * q)) & 0xffu) ? E.v[q] + cfac * h[q] : E.v[q];
     return o;
   }
 };

@@ -184,6 +184,7 @@ struct psb_ctx {
   int coarse_min = 8;  // coarsening halves a direction while it stays >= this
   int coarse_rule = 0; // 0 all directions at once, 1 x and y before z
   bool store_half = false;  // FP16 copies of the FP32 levels for the smoother
+  bool fuse = false;        // two post-sweeps in one pass (k_post2) where it applies
   int version = 0;     // bumped by every change of matrix, geometry or off mask
   psb::host::RefHier href;             // the reference's AC cycle
   psb::DevMetrics metrics{};           // metrics of an assembled matrix (device)

@@ -386,3 +386,23 @@ def test_cg_diag_64_drop_matches_oracle():
     assert abs(r["iterations"] - info["iters"]) <= 1, (r["iterations"], info["iters"])
     assert r["res"] < 1e-8 * r["res0"]
     assert rel(x, xo) < 1e-8
+
+
+@pytest.mark.parametrize("store", [0, 1])
+def test_fused_post_smoothing_gives_the_same_solve(store):
+    """The two post-sweeps in one pass (ac_fuse = 1) compute the same z as
+    one kernel per sweep; only the block partition of the rho sums differs,
+    so iteration counts agree and the solutions to rounding."""
+    d, A = ac_case(64, nsph=6, seed=9)
+    b = d["rhs"]
+    n = b.size
+    s = psb.Solver(64, 64, 64)
+    s.set_matrix(d)
+    kw = dict(precond=psb.PSB_PREC_AC, precision=psb.PSB_MIXED, inner_adapt=2, inner_rtol=1e-2,
+              norm=psb.PSB_NORM_L2REL, res_rat=1e-9, res_tol=0.0, max_iter=300, ac_store=store)
+    x1, r1 = s.solve(np.zeros(n), b, ac_fuse=1, **kw)
+    x0, r0 = s.solve(np.zeros(n), b, ac_fuse=0, **kw)
+    assert r1["converged"] and r0["converged"]
+    assert abs(r1["iterations"] - r0["iterations"]) <= 1
+    assert r1["rel_l2"] <= 1e-9 * 1.01
+    assert rel(x1 - x1.mean(), x0 - x0.mean()) < 1e-7
