@@ -178,6 +178,22 @@ int psb_assemble_pressure_device(psb_ctx *ctx, const double *d_rho,
 int psb_update_rhs_device(psb_ctx *ctx, const double *d_u, const double *d_v,
                           const double *d_w, int zero_mean, double *d_b);
 
+/* Momentum::project (Src/Equation/Staggered/Momentum/momentum_project.cpp:
+ * 10-76), the step after the pressure solve, on device arrays after
+ * psb_assemble_pressure*: the interior faces of u, v, w (layout as in
+ * psb_update_rhs_device; wall faces unchanged) take the correction
+ * (p_nb - p_P) dt / (rho_face d) with d = dxw / dys / dzb.  d_rho: the
+ * density the matrix was assembled from (ghosted layout); d_p: the solution
+ * of psb_solve_device, used in place.  Without solid cells the rounding of
+ * the reference's plain form, with them its body form (faces between two
+ * fluid cells only).  mask: bit 0 u, bit 1 v, bit 2 w (the reference's
+ * ifull / jfull / kfull); arrays of unselected components may be NULL.
+ * The ghost-face update (bnd_update_nooutlet, exchange_all) is the
+ * caller's.                                                                 */
+int psb_project_device(psb_ctx *ctx, const double *d_rho, const double *d_p,
+                       double dt, int mask, double *d_u, double *d_v,
+                       double *d_w);
+
 /* Linear::solve (linear.h:46-56): x (in: initial guess, out: solution) and b
  * are host arrays in the context's layout.                                  */
 int psb_solve(psb_ctx *ctx, double *x, const double *b, const psb_stop *stop,

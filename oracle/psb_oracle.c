@@ -29,6 +29,11 @@
  *   ora_discretize, ora_rhs      reference Pressure::discretize / update_rhs
  *                                on the same grid metrics and density
  *                                (fixtures); closed form on uniform grids.
+ *   ora_project                  the identity rhs(project(u, p)) = rhs(u) -
+ *                                dt A p on every fluid cell for random p,
+ *                                with ora_rhs / ora_discretize / ora_spmv on
+ *                                grids stretched in x, y and z; divergence
+ *                                after a converged solve = its residual.
  *   ora_ac_coarse_dense,
  *   ora_ac_precond, ora_acpcg,
  *   ora_pcg_rel                  not in the reference: dense pseudo-inverse
@@ -269,6 +274,53 @@ void ora_rhs(int nx, int ny, int nz, const double *dxc, const double *dyc,
         if (!(off && off[P])) b[P] -= mean;
     }
   }
+}
+
+/* Momentum::project (Src/Equation/Staggered/Momentum/momentum_project.cpp:
+ * 10-76): the interior faces (Staggered::set_indices drops the wall faces,
+ * staggered_set_indices.cpp:9-19) take the pressure-gradient correction with
+ * the face density 0.5 (rho_P + rho_nb) (property_value.cpp:18-32) and the
+ * centre distance dxw / dys / dzb (Vector::dxc_staggered, vector.h:388).
+ * Without a body (off == NULL) u += (p_W - p_P) / rho * dt / dxw (:29-33);
+ * with one, only faces between two fluid cells, as (p_W - p_P) * dt / rho /
+ * dxw (:50-55).  mask: bit 0 u, bit 1 v, bit 2 w (ifull / jfull / kfull).   */
+void ora_project(int nx, int ny, int nz, const double *dxw, const double *dys,
+                 const double *dzb, const double *rho, const uint8_t *off,
+                 const double *p, double dt, int mask, double *u, double *v,
+                 double *w) {
+  const int gy = ny + 2, gz = nz + 2;
+#define RHO(i, j, k) rho[((size_t)((i) + 1) * gy + ((j) + 1)) * gz + ((k) + 1)]
+  if (mask & 1)
+    for (int i = 1; i < nx; i++)
+      for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz; k++) {
+          size_t P = IDX(i, j, k, ny, nz), W = IDX(i - 1, j, k, ny, nz);
+          double r = 0.5 * (RHO(i, j, k) + RHO(i - 1, j, k));
+          double *f = &u[((size_t)i * ny + j) * nz + k];
+          if (!off) *f += (p[W] - p[P]) / r * dt / dxw[i];
+          else if (!off[P] && !off[W]) *f += (p[W] - p[P]) * dt / r / dxw[i];
+        }
+  if (mask & 2)
+    for (int i = 0; i < nx; i++)
+      for (int j = 1; j < ny; j++)
+        for (int k = 0; k < nz; k++) {
+          size_t P = IDX(i, j, k, ny, nz), S = IDX(i, j - 1, k, ny, nz);
+          double r = 0.5 * (RHO(i, j, k) + RHO(i, j - 1, k));
+          double *f = &v[((size_t)i * (ny + 1) + j) * nz + k];
+          if (!off) *f += (p[S] - p[P]) / r * dt / dys[j];
+          else if (!off[P] && !off[S]) *f += (p[S] - p[P]) * dt / r / dys[j];
+        }
+  if (mask & 4)
+    for (int i = 0; i < nx; i++)
+      for (int j = 0; j < ny; j++)
+        for (int k = 1; k < nz; k++) {
+          size_t P = IDX(i, j, k, ny, nz), B = IDX(i, j, k - 1, ny, nz);
+          double r = 0.5 * (RHO(i, j, k) + RHO(i, j, k - 1));
+          double *f = &w[((size_t)i * ny + j) * (nz + 1) + k];
+          if (!off) *f += (p[B] - p[P]) / r * dt / dzb[k];
+          else if (!off[P] && !off[B]) *f += (p[B] - p[P]) * dt / r / dzb[k];
+        }
+#undef RHO
 }
 
 /* ------------------------------------------------------------------------ */

@@ -88,6 +88,15 @@ void ora_rhs(int nx, int ny, int nz, const double *dxc, const double *dyc,
 void ora_face_velocity(uint64_t seed, int nx, int ny, int nz,
                        double *u, double *v, double *w);
 
+/* Momentum::project (momentum_project.cpp:10-76) on the interior faces of
+ * u, v, w (layout as ora_rhs) with the pressure p and the ghosted density
+ * of ora_discretize; off != NULL selects the body form (faces between two
+ * fluid cells only).  mask: bit 0 u, bit 1 v, bit 2 w.                      */
+void ora_project(int nx, int ny, int nz, const double *dxw, const double *dys,
+                 const double *dzb, const double *rho, const uint8_t *off,
+                 const double *p, double dt, int mask, double *u, double *v,
+                 double *w);
+
 /* ---- vector kernels ----------------------------------------------------- */
 void   ora_spmv(const ora_mat *A, const double *x, double *y);
 void   ora_resid(const ora_mat *A, const double *b, const double *x, double *r);

@@ -158,6 +158,22 @@ def rhs(dxc, dyc, dzc, u, v, w, off=None, zero_mean=True):
     return b
 
 
+def project(mx, my, mz, rho, p, u, v, w, dt=1.0, off=None, mask=7):
+    """Momentum::project on copies of the face velocities; mx = (dxc, dxw, dxe)."""
+    nx, ny, nz = len(mx[0]), len(my[0]), len(mz[0])
+    u, v, w = (np.array(a, dtype=np.float64) for a in (u, v, w))
+    offp = None
+    if off is not None:
+        off = np.ascontiguousarray(off, dtype=np.uint8).ravel()
+        offp = _p(off)
+    lib().ora_project(nx, ny, nz, _p(np.ascontiguousarray(mx[1], dtype=np.float64)),
+                      _p(np.ascontiguousarray(my[1], dtype=np.float64)),
+                      _p(np.ascontiguousarray(mz[1], dtype=np.float64)), _p(rho), offp,
+                      _p(np.ascontiguousarray(p, dtype=np.float64)), C.c_double(dt), int(mask),
+                      _p(u), _p(v), _p(w))
+    return u, v, w
+
+
 def spmv(A, x):
     y = np.zeros(A.n)
     lib().ora_spmv(C.byref(A.s), _p(np.ascontiguousarray(x, dtype=np.float64)), _p(y))

@@ -1444,9 +1444,9 @@ int psb_destroy(psb_ctx *C) {
   if (!C) return PSB_OK;
   if (C->stream) (void)hipStreamSynchronize(C->stream);
   destroy_graphs(C);
-  C->arena.release();
-  C->harena.release();
-  C->base.release();
+  C->arena.free_all();
+  C->harena.free_all();
+  C->base.free_all();
   if (C->hst) (void)hipHostFree(C->hst);
   if (C->stream) (void)hipStreamDestroy(C->stream);
   delete C;
@@ -1644,6 +1644,21 @@ int psb_update_rhs_device(psb_ctx *C, const double *d_u, const double *d_v, cons
       hipLaunchKernelGGL(k_rhs_center, dim3(nb), dim3(kBlock), 0, C->stream, g, d_b,
                          C->off_dev, C->st32);
     }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return (int)PSB_OK;
+  });
+}
+
+int psb_project_device(psb_ctx *C, const double *d_rho, const double *d_p, double dt, int mask,
+                       double *d_u, double *d_v, double *d_w) {
+  if (!C || !d_rho || !d_p || mask < 0 || mask > 7) return PSB_EINVAL;
+  if (((mask & 1) && !d_u) || ((mask & 2) && !d_v) || ((mask & 4) && !d_w)) return PSB_EINVAL;
+  if (!C->have_matrix || !C->metrics.dxc) return PSB_ENOTREADY;
+  return guarded(C, [&]() {
+    const Grid &g = C->g;
+    hipLaunchKernelGGL(k_project, dim3(grid_blocks(g.N)), dim3(kBlock), 0, C->stream, g,
+                       C->metrics, d_rho, C->off_dev, d_p, dt, mask, d_u, d_v, d_w);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(C->stream));
     return (int)PSB_OK;

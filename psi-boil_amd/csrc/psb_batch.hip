@@ -607,8 +607,8 @@ int psb_batch_destroy(psb_batch *B) {
   if (!B) return PSB_OK;
   if (B->stream) (void)hipStreamSynchronize(B->stream);
   destroy_graph(B->graph);
-  B->arena.release();
-  B->base.release();
+  B->arena.free_all();
+  B->base.free_all();
   if (B->hst) (void)hipHostFree(B->hst);
   if (B->hflags) (void)hipHostFree(B->hflags);
   if (B->stream) (void)hipStreamDestroy(B->stream);

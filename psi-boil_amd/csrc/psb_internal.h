@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -60,21 +61,52 @@ inline Grid make_grid(int nx, int ny, int nz) {
 }
 
 // device allocations owned by a context
+// Device allocations of a context.  release() keeps the blocks for reuse:
+// a matrix that changes every time step (two-phase flow) is re-assembled and
+// its hierarchy rebuilt with the same block sizes, which then cost no
+// hipMalloc / hipFree (each of those synchronizes the device and maps or
+// unmaps pages).  Blocks left unused for a whole cycle between two releases
+// are freed at the second; free_all() returns everything.
 struct Arena {
-  std::vector<void *> ptrs;
+  std::vector<std::pair<void *, size_t>> ptrs;  // live blocks
+  std::multimap<size_t, void *> spare;          // released, kept for reuse
   template <typename T>
   T *alloc(size_t n) {
-    void *p = nullptr;
     size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
     bytes = (bytes + 255) & ~size_t(255);
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) throw HipError{e};
-    ptrs.push_back(p);
+    void *p = nullptr;
+    auto it = spare.find(bytes);
+    if (it != spare.end()) {
+      p = it->second;
+      spare.erase(it);
+    } else {
+      hipError_t e = hipMalloc(&p, bytes);
+      if (e != hipSuccess && !spare.empty()) {
+        (void)hipGetLastError();
+        purge();
+        e = hipMalloc(&p, bytes);
+      }
+      if (e != hipSuccess) throw HipError{e};
+    }
+    ptrs.emplace_back(p, bytes);
     return static_cast<T *>(p);
   }
+  // every caller has synchronized its stream; the device-wide wait covers
+  // work of other streams on the blocks (hipFree used to imply it)
   void release() {
-    for (void *p : ptrs) (void)hipFree(p);
+    if (ptrs.empty()) return;
+    (void)hipDeviceSynchronize();
+    purge();
+    for (auto &q : ptrs) spare.emplace(q.second, q.first);
     ptrs.clear();
+  }
+  void purge() {
+    for (auto &q : spare) (void)hipFree(q.second);
+    spare.clear();
+  }
+  void free_all() {
+    release();
+    purge();
   }
 };
 

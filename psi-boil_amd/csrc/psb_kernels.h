@@ -1086,6 +1086,53 @@ static __global__ void k_rhs_center(Grid g, double *b, const uint8_t *off, const
     if (!(off && off[P])) b[P] = b[P] - mean;
 }
 
+// Momentum::project (momentum_project.cpp:10-76), one cell per thread: the
+// cell's west, south and bottom faces when they are interior faces (the wall
+// faces stay, staggered_set_indices.cpp:9-19) take
+// (p_nb - p_P) / rho_f * dt / d with rho_f = (rho_P + rho_nb) / 2
+// (property_value.cpp:18-32) and d the centre distance (dxw, dys, dzb).
+// With solid cells (off) the body form: faces between two fluid cells only,
+// rounded as (p_nb - p_P) * dt / rho_f / d (:50-70).  mask bits 0/1/2 select
+// u, v, w (Momentum's ifull / jfull / kfull).  64 B per cell: p, rho, and
+// the three face arrays read and written.
+static __global__ __launch_bounds__(kBlock) void k_project(Grid g, DevMetrics m,
+                                                           const double *rho, const uint8_t *off,
+                                                           const double *p, double dt, int mask,
+                                                           double *u, double *v, double *wv) {
+  const int gy = g.ny + 2, gz = g.nz + 2;
+  const size_t ny = g.ny, nz = g.nz;
+  for (int P = blockIdx.x * blockDim.x + threadIdx.x; P < g.N; P += gridDim.x * blockDim.x) {
+    int i, j, k;
+    g.ijk(P, i, j, k);
+    auto R = [&](int a, int b, int c) {
+      return rho[((size_t)(a + 1) * gy + (b + 1)) * gz + (c + 1)];
+    };
+    const double pp = p[P], rp = R(i, j, k);
+    const bool fluid = !(off && off[P]);
+    if ((mask & 1) && i > 0) {
+      const int Q = P - g.sI;
+      const double rf = 0.5 * (rp + R(i - 1, j, k));
+      double &f = u[P];  // face i of (nx+1) x ny x nz: same index as the cell
+      if (!off) f += (p[Q] - pp) / rf * dt / m.dxw[i];
+      else if (fluid && !off[Q]) f += (p[Q] - pp) * dt / rf / m.dxw[i];
+    }
+    if ((mask & 2) && j > 0) {
+      const int Q = P - g.nz;
+      const double rf = 0.5 * (rp + R(i, j - 1, k));
+      double &f = v[((size_t)i * (ny + 1) + j) * nz + k];
+      if (!off) f += (p[Q] - pp) / rf * dt / m.dys[j];
+      else if (fluid && !off[Q]) f += (p[Q] - pp) * dt / rf / m.dys[j];
+    }
+    if ((mask & 4) && k > 0) {
+      const int Q = P - 1;
+      const double rf = 0.5 * (rp + R(i, j, k - 1));
+      double &f = wv[((size_t)i * ny + j) * (nz + 1) + k];
+      if (!off) f += (p[Q] - pp) / rf * dt / m.dzb[k];
+      else if (fluid && !off[Q]) f += (p[Q] - pp) * dt / rf / m.dzb[k];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // set-up kernels
 // ---------------------------------------------------------------------------

@@ -87,7 +87,7 @@ EXPORTS = ["psb_stop_defaults", "psb_status_string", "psb_create", "psb_destroy"
            "psb_batch_system", "psb_batch_set_tuning", "psb_batch_plan", "psb_batch_solve",
            "psb_batch_solve_device", "psb_cycle_defaults", "psb_ac_vcycle",
            "psb_ac_vcycle_device", "psb_assemble_pressure", "psb_assemble_pressure_device",
-           "psb_update_rhs_device"]
+           "psb_update_rhs_device", "psb_project_device"]
 
 _lib = None
 
@@ -208,10 +208,28 @@ class Solver:
         return _check(lib().psb_assemble_pressure(self._c, _dp(rho), C.byref(m), offp),
                       "psb_assemble_pressure")
 
+    def assemble_pressure_device(self, d_rho, mx, my, mz, off=None):
+        """As assemble_pressure with the ghosted density already on the device."""
+        self._met = [np.ascontiguousarray(a, dtype=np.float64) for a in (*mx, *my, *mz)]
+        m = Metrics(*[a.ctypes.data for a in self._met])
+        offp = None
+        if off is not None:
+            self._off = np.ascontiguousarray(off, dtype=np.uint8).ravel()
+            offp = _dp(self._off)
+        return _check(lib().psb_assemble_pressure_device(self._c, C.c_void_p(d_rho), C.byref(m),
+                                                         offp), "psb_assemble_pressure_device")
+
     def update_rhs_device(self, d_u, d_v, d_w, d_b, zero_mean=True):
         return _check(lib().psb_update_rhs_device(self._c, C.c_void_p(d_u), C.c_void_p(d_v),
                                                   C.c_void_p(d_w), int(zero_mean),
                                                   C.c_void_p(d_b)), "psb_update_rhs_device")
+
+    def project_device(self, d_rho, d_p, d_u, d_v, d_w, dt=1.0, mask=7):
+        """Momentum::project on device face velocities (in place)."""
+        return _check(lib().psb_project_device(self._c, C.c_void_p(d_rho), C.c_void_p(d_p),
+                                               C.c_double(dt), int(mask), C.c_void_p(d_u),
+                                               C.c_void_p(d_v), C.c_void_p(d_w)),
+                      "psb_project_device")
 
     def ac_vcycle(self, x, b, **cyc):
         """The reference's AC::vcycle (CG smoothers on its own levels)."""
@@ -340,6 +358,13 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def device_copy(dst, src):
+    """dst <- src, two DeviceArrays of the same length (device to device)."""
+    if dst.n != src.n:
+        raise PsbError("device_copy: lengths differ")
+    _hipchk(hip().hipMemcpy(dst.ptr, src.ptr, C.c_size_t(8 * src.n), 3), "hipMemcpy D2D")
 
 
 def device_synchronize():
