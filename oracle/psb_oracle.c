@@ -24,8 +24,12 @@
  *   ora_interpolation            reference level matrices / restricted RHS
  *                                (fixtures) and dense P^T A P, P^T(f-A phi)
  *                                with explicit P on 8^3 -> 4^3.
- *   ora_ac_vcycle                reference AC::vcycle residual history
- *                                (fixtures).
+ *   ora_ac_vcycle, ora_ac_cycle  reference AC::vcycle residual history and
+ *                                phi (fixtures); the F, W and full-cycle
+ *                                shapes, linf control, divergence and
+ *                                min-cycle controls: residual histories
+ *                                and phi of the reference's own AC::cycle
+ *                                (cycles fixtures), bit for bit.
  *   ora_discretize, ora_rhs      reference Pressure::discretize / update_rhs
  *                                on the same grid metrics and density
  *                                (fixtures); closed form on uniform grids.
@@ -841,50 +845,147 @@ void ora_interpolation(const ora_ac *ac, int l) {
       }
 }
 
-/* AC::residual (additive_residual.cpp:19-29): sqrt(dot_voldiv_avg) * dt */
-static double ac_residual(ora_ac *ac, double dt) {
+/* AC::residual (additive_residual.cpp:19-29): sqrt(dot_voldiv_avg) * dt,
+ * and (linf != NULL) max |r / dV| * dt (Scalar::max_abs_voldiv,
+ * scalar_minmax.cpp:155-165) */
+static double ac_residual2(ora_ac *ac, double dt, double *linf) {
   const ora_mat *A = &ac->A[0];
   size_t N = mat_n(A);
   ora_resid(A, ac->f[0], ac->phi[0], ac->res[0]);
-  return sqrt(ora_dot_voldiv_avg(N, ac->res[0], ac->res[0], ac->dV[0])) * dt;
+  const double r2 = sqrt(ora_dot_voldiv_avg(N, ac->res[0], ac->res[0], ac->dV[0])) * dt;
+  if (linf) {
+    double vmax = fabs(ac->res[0][0] / ac->dV[0][0]);
+    for (size_t P = 0; P < N; P++)
+      if (fabs(ac->res[0][P] / ac->dV[0][P]) > vmax) vmax = fabs(ac->res[0][P] / ac->dV[0][P]);
+    *linf = vmax * dt;
+  }
+  return r2;
+}
+static double ac_residual(ora_ac *ac, double dt) { return ac_residual2(ac, dt, NULL); }
+
+/* Linear::solve of one level (call_smoother / call_solver,
+ * additive_cycle_functions.cpp:90-107): CG with the diagonal preconditioner;
+ * returns whether it met its tolerance */
+static int ac_cg(ora_ac *ac, int l, int mi, double rr, double rt, double scale) {
+  return ora_cg(&ac->A[l], ac->phi[l], ac->f[l], ac->dV[l], 1, mi, rr, rt, scale, -1,
+                NULL, NULL);
 }
 
-static void ac_cg(ora_ac *ac, int l, int mi, double rr, double rt, double scale) {
-  ora_cg(&ac->A[l], ac->phi[l], ac->f[l], ac->dV[l], 1, mi, rr, rt, scale, -1,
-         NULL, NULL);
+/* the recursive cycles of additive_recursive.cpp:4-166 with AC's level
+ * tolerances resrat / restol (atto by default, additive.cpp:16-17) and the
+ * coarsest call MaxIter 100, ResRat -1 */
+typedef struct {
+  const int *mv;
+  double rr, rt, scale;
+} cyc_par;
+
+static void ac_coarsest(ora_ac *ac, int l, const cyc_par *q) {
+  ac_cg(ac, l, 100, -1.0, q->rt, q->scale);
+  ora_interpolation(ac, l);
 }
 
-/* additive_recursive.cpp:4-34 with AC's defaults resrat = restol = atto
- * (additive.cpp:16-17) and the coarsest call MaxIter 100, ResRat -1. */
-static void ac_v_cycle(ora_ac *ac, int l, const int mv[3], double scale) {
+static void ac_v(ora_ac *ac, int l, const cyc_par *q) {
   if (l != ac->nlev - 1) {
-    ac_cg(ac, l, mv[0], ATTO, ATTO, scale);
+    ac_cg(ac, l, q->mv[0], q->rr, q->rt, q->scale);
     ora_restriction(ac, l);
-    ac_v_cycle(ac, l + 1, mv, scale);
-    ac_cg(ac, l, mv[2], ATTO, ATTO, scale);
+    ac_v(ac, l + 1, q);
+    ac_cg(ac, l, q->mv[2], q->rr, q->rt, q->scale);
     if (l > 0) ora_interpolation(ac, l);
   } else {
-    ac_cg(ac, l, 100, -1.0, ATTO, scale);
-    ora_interpolation(ac, l);
+    ac_coarsest(ac, l, q);
   }
 }
 
-/* AC::cycle(Cycle::none(), Cycle::V(), ...) : additive_cycle.cpp:4-99 with
- * init_cycles / converged from additive_cycle_functions.cpp:12-87 (stop_if_div
- * and priority_min_cyc true, use_linf false: additive.cpp:11-19). */
-int ora_ac_vcycle(ora_ac *ac, double *phi0, const double *f0, double toler,
-                  double factor, int max_cyc, int min_cyc, const int mv[3],
-                  double dt, double *hist, int *ncyc) {
+/* f_cycle (:37-79) and w_cycle (:82-124): smooth, restrict, recurse (F: an
+ * F-cycle, W: a W-cycle), re-smooth with mv[1], restrict again, recurse on
+ * the way back (F: a V-cycle, W: a W-cycle) with the coarse phi kept from
+ * the first descent, post-smooth, prolongate */
+static void ac_fw(ora_ac *ac, int l, const cyc_par *q, int w) {
+  if (l != ac->nlev - 1) {
+    ac_cg(ac, l, q->mv[0], q->rr, q->rt, q->scale);
+    ora_restriction(ac, l);
+    ac_fw(ac, l + 1, q, w);
+    ac_cg(ac, l, q->mv[1], q->rr, q->rt, q->scale);
+    ora_restriction(ac, l);
+    if (w) ac_fw(ac, l + 1, q, 1);
+    else ac_v(ac, l + 1, q);
+    ac_cg(ac, l, q->mv[2], q->rr, q->rt, q->scale);
+    if (l > 0) ora_interpolation(ac, l);
+  } else {
+    ac_coarsest(ac, l, q);
+  }
+}
+
+static void ac_loop(ora_ac *ac, int l, int kind, const cyc_par *q) {
+  if (kind == ORA_CYC_V) ac_v(ac, l, q);
+  else ac_fw(ac, l, q, kind == ORA_CYC_W);
+}
+
+/* full_cycle (:127-163): restrict, recurse to the coarsest level, then run
+ * the chosen cycle from each level on the way back */
+static void ac_full(ora_ac *ac, int l, int kind, const cyc_par *q) {
+  if (l != ac->nlev - 1) {
+    ora_restriction(ac, l);
+    ac_full(ac, l + 1, kind, q);
+    ac_loop(ac, l, kind, q);
+  } else {
+    ac_coarsest(ac, l, q);
+  }
+}
+
+void ora_cycle_defaults(ora_cycle *P) {
+  P->toler = -1.0;
+  P->factor = -1.0;
+  P->max_cyc = 20;
+  P->min_cyc = 0;
+  P->mv[0] = P->mv[1] = P->mv[2] = 20;
+  P->dt = 1.0;
+  P->init = ORA_CYC_NONE;
+  P->loop = ORA_CYC_V;
+  P->stop_if_div = 1;
+  P->priority_min_cyc = 1;
+  P->use_linf = 0;
+  P->resrat = ATTO;
+  P->restol = ATTO;
+}
+
+/* AC::converged (additive_cycle_functions.cpp:49-87) */
+static int ac_converged(ora_ac *ac, const ora_cycle *P, int cycle, double *res1,
+                        double *linf1, double res_beg, double linf_beg, double res0,
+                        double linf0, const double *fold) {
+  *res1 = ac_residual2(ac, P->dt, linf1);
+  const double rc = P->use_linf ? *linf1 : *res1;
+  const double r0c = P->use_linf ? linf0 : res0;
+  const double rbc = P->use_linf ? linf_beg : res_beg;
+  if (cycle >= P->min_cyc && (rc / rbc < P->factor || rc < P->toler)) return 0;
+  int check = P->stop_if_div;
+  if (P->priority_min_cyc && cycle < P->min_cyc) check = 0;
+  if (check && rc >= r0c) {
+    memcpy(ac->phi[0], fold, mat_n(&ac->A[0]) * 8);
+    return 1;
+  }
+  return 2;
+}
+
+/* AC::cycle (additive_cycle.cpp:4-99) with init_cycles
+ * (additive_cycle_functions.cpp:12-46).  hist / hist_linf (max_cyc + 2
+ * entries): [0] the initial residual, then one per evaluation (the
+ * full-cycle initialisation's first when init is set). */
+int ora_ac_cycle(ora_ac *ac, double *phi0, const double *f0, const ora_cycle *P,
+                 double *hist, double *hist_linf, int *nhist, int *ncyc) {
   const size_t N0 = mat_n(&ac->A[0]);
-  const double dti = 1.0 / dt;
+  const cyc_par q = {P->mv, P->resrat, P->restol, 1.0 / P->dt * 1.0};
   double *fold = dalloc(N0);
-  int ret = 0;
+  int ret = 0, nh = 0;
   memcpy(ac->phi[0], phi0, N0 * 8);
   memcpy(ac->f[0], f0, N0 * 8);
-  double res_0 = ac_residual(ac, dt);
-  if (hist) hist[0] = res_0;
+  double linf_0 = 0.0;
+  const double res_0 = ac_residual2(ac, P->dt, &linf_0);
+#define PUSH(a, b) do { if (hist) hist[nh] = (a); if (hist_linf) hist_linf[nh] = (b); nh++; } while (0)
+  PUSH(res_0, linf_0);
   memcpy(fold, ac->phi[0], N0 * 8);
-  if ((res_0 < toler && min_cyc == 0) || res_0 == 0.0) {
+  const double r0c = P->use_linf ? linf_0 : res_0;
+  if ((r0c < P->toler && P->min_cyc == 0) || res_0 == 0.0) {
     if (ncyc) *ncyc = 0;
     ret = 1;
     goto done;
@@ -894,31 +995,53 @@ int ora_ac_vcycle(ora_ac *ac, double *phi0, const double *f0, double toler,
     memset(ac->phi[l], 0, N * 8);
     memset(ac->f[l], 0, N * 8);
   }
-  double res_m1 = ac_residual(ac, dt);
-  for (int c = 1; c <= max_cyc; c++) {
-    ac_v_cycle(ac, 0, mv, dti * 1.0);
-    double res1 = ac_residual(ac, dt);
-    if (hist) hist[c] = res1;
-    if (c >= min_cyc && (res1 / res_0 < factor || res1 < toler)) {
+  double res_m0 = 0.0, linf_m0 = 0.0, res_m1, linf_m1;
+  if (P->init != ORA_CYC_NONE) {
+    ac_full(ac, 0, P->init, &q);
+    res_m1 = ac_residual2(ac, P->dt, &linf_m1);
+    /* the verdict is not used (additive_cycle.cpp:40-42): with the
+     * divergence check on, the equal residuals restore fold */
+    (void)ac_converged(ac, P, 0, &res_m0, &linf_m0, res_0, linf_0, res_m1, linf_m1, fold);
+    PUSH(res_m0, linf_m0);
+  } else {
+    res_m1 = ac_residual2(ac, P->dt, &linf_m1);
+  }
+  if (ncyc) *ncyc = P->max_cyc;
+  for (int c = 1; c <= P->max_cyc; c++) {
+    ac_loop(ac, 0, P->loop, &q);
+    const int con = ac_converged(ac, P, c, &res_m0, &linf_m0, res_0, linf_0, res_m1, linf_m1,
+                                 fold);
+    PUSH(res_m0, linf_m0);
+    res_m1 = res_m0;  /* linf_m1 keeps its value (additive_cycle.cpp:78) */
+    if (con == 0 || con == 1) {
       if (ncyc) *ncyc = c;
-      ret = 1;
+      ret = con == 0;
       goto done;
     }
-    if (!(c < min_cyc) && res1 >= res_m1) {
-      memcpy(ac->phi[0], fold, N0 * 8);
-      if (ncyc) *ncyc = c;
-      ret = 0;
-      goto done;
-    }
-    res_m1 = res1;
     memcpy(fold, ac->phi[0], N0 * 8);
-    if (c == max_cyc && ncyc) *ncyc = max_cyc;
   }
   ret = 0;
+#undef PUSH
 done:
+  if (nhist) *nhist = nh;
   memcpy(phi0, ac->phi[0], N0 * 8);
   free(fold);
   return ret;
+}
+
+/* AC::vcycle: the V loop with the constructor's controls */
+int ora_ac_vcycle(ora_ac *ac, double *phi0, const double *f0, double toler,
+                  double factor, int max_cyc, int min_cyc, const int mv[3],
+                  double dt, double *hist, int *ncyc) {
+  ora_cycle P;
+  ora_cycle_defaults(&P);
+  P.toler = toler;
+  P.factor = factor;
+  P.max_cyc = max_cyc;
+  P.min_cyc = min_cyc;
+  for (int k = 0; k < 3; k++) P.mv[k] = mv[k];
+  P.dt = dt;
+  return ora_ac_cycle(ac, phi0, f0, &P, hist, NULL, NULL, ncyc);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -154,6 +154,25 @@ int  ora_ac_vcycle(ora_ac *ac, double *phi0, const double *f0, double toler,
                    double factor, int max_cyc, int min_cyc, const int mv[3],
                    double dt, double *hist, int *ncyc);
 
+/* AC::cycle (additive_cycle.cpp:4-99): the loop's cycle shape, an optional
+ * full-cycle initialisation (additive_recursive.cpp:127-163) and the
+ * controls of additive.h:97-142.  Cycle::flex is not restated: its
+ * recursion ends only when a level's smoother meets its ResRat. */
+enum { ORA_CYC_NONE = -1, ORA_CYC_V = 0, ORA_CYC_F = 1, ORA_CYC_W = 2 };
+typedef struct {
+  double toler, factor;          /* ResTol, ResRat of the loop               */
+  int max_cyc, min_cyc;
+  int mv[3];                     /* MaxIter pre / re- / post-smoothing       */
+  double dt;
+  int init, loop;                /* ORA_CYC_*                                */
+  int stop_if_div, priority_min_cyc, use_linf;
+  double resrat, restol;         /* level solves (AC::resrat / restol)       */
+} ora_cycle;
+void ora_cycle_defaults(ora_cycle *P);
+/* hist, hist_linf: max_cyc + 2 entries; *nhist receives the count.          */
+int  ora_ac_cycle(ora_ac *ac, double *phi0, const double *f0, const ora_cycle *P,
+                  double *hist, double *hist_linf, int *nhist, int *ncyc);
+
 /* ---- the benchmark's composition (not in the reference) ----------------- */
 /* Dense operator of the coarsest level: G = Pi * (A_red)^-1 * Pi over active
  * cells, Pi the mean projector, A_red the matrix with its last pivot pinned. */

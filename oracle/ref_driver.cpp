@@ -8,6 +8,7 @@
 // turns small ones into committed fixtures.
 //
 //   ref_driver case  OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel [cx cy cz]
+//   ref_driver cycles OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel [cx cy cz]
 //   ref_driver bench-cg nx nsph rmin rmax rho_in rho_out seed_sph seed_vel maxit
 //   ref_driver bench-ac nx nsph rmin rmax rho_in rho_out seed_sph seed_vel cycles
 //   (bench modes: optional cx cy cz after the limit for one centred sphere)
@@ -348,6 +349,71 @@ void dump_ac(World &w) {
   dump("ac_phifull", interior(pr.phi));
 }
 
+// AC::cycle with the other cycle shapes and controls (additive_cycle.cpp,
+// additive_recursive.cpp:37-166): each run starts from phi = 0 with the
+// constructor's settings, is framed by ACRUN / ACRET lines (the cycle log
+// lines in between) and leaves phi in <tag>_phi.f64
+void dump_cycles(World &w) {
+  CasePressure &pr = *w.pr;
+  struct Run {
+    std::string tag;
+    int kind;  // 0 V, 1 F, 2 W; +10: full-cycle (V) initialisation
+    double rr;
+    int maxc;
+    bool linf, stopdiv, prio;
+    int minc;
+    double smooth_rr;  // AC::resrat of the level solves (atto by default)
+    std::array<int, 3> mv;
+  };
+  const double A = boil::atto;
+  const std::vector<Run> runs = {
+      {"F1", 1, 1e-30, 1, false, true, true, 0, A, {20, 20, 20}},
+      {"F2", 1, 1e-30, 2, false, true, true, 0, A, {20, 20, 20}},
+      {"Ffull", 1, 1e-8, 20, false, true, true, 0, A, {20, 20, 20}},
+      {"W1", 2, 1e-30, 1, false, true, true, 0, A, {20, 20, 20}},
+      {"W2", 2, 1e-30, 2, false, true, true, 0, A, {20, 20, 20}},
+      {"Wfull", 2, 1e-8, 20, false, true, true, 0, A, {20, 20, 20}},
+      {"FMV2", 10, 1e-30, 2, false, true, true, 0, A, {20, 20, 20}},
+      {"FMVfull", 10, 1e-8, 20, false, true, true, 0, A, {20, 20, 20}},
+      {"FMF2", 11, 1e-30, 2, false, true, true, 0, A, {20, 20, 20}},
+      {"FMW2", 12, 1e-30, 2, false, true, true, 0, A, {20, 20, 20}},
+      {"Vlinf", 0, 1e-8, 20, true, true, true, 0, A, {20, 20, 20}},
+      {"Vdiv", 0, 1e-30, 20, false, true, true, 0, A, {20, 20, 20}},
+      {"Vnodiv", 0, 1e-30, 20, false, false, true, 0, A, {20, 20, 20}},
+      {"Vmin", 0, 1e-30, 20, false, true, true, 12, A, {20, 20, 20}},
+      {"Vminnoprio", 0, 1e-30, 20, false, true, false, 12, A, {20, 20, 20}},
+      {"Vlinfdiv", 0, 1e-30, 8, true, true, true, 0, A, {1, 1, 1}},
+      {"Vmv", 0, 1e-8, 20, false, true, true, 0, A, {3, 20, 5}},
+      {"Vsmooth", 0, 1e-8, 20, false, true, true, 0, 1e-2, {20, 20, 20}},
+  };
+  // Cycle::flex is left out: its recursion (additive_recursive.cpp:167-200)
+  // ends only when a level's smoother meets its ResRat, and once a level's
+  // residual reaches round-off it alternates between two levels for good
+  // (this driver never returned from it on these cases)
+  for (const Run &r : runs) {
+    AC ac(&pr);
+    pr.phi = 0.0;
+    ac.max_cycles(r.maxc);
+    ac.min_cycles(r.minc);
+    ac.use_linf_error(r.linf);
+    ac.stop_if_diverging(r.stopdiv);
+    ac.priority_min_cyc = r.prio;
+    ac.resrat(ResRat(r.smooth_rr));
+    std::array<MaxIter, 3> mv = {MaxIter(r.mv[0]), MaxIter(r.mv[1]), MaxIter(r.mv[2])};
+    ac.max_iter_array(mv);
+    std::cout << "ACRUN " << r.tag << " " << r.kind << " " << r.rr << " " << r.maxc << " "
+              << r.linf << " " << r.stopdiv << " " << r.prio << " " << r.minc << " "
+              << r.smooth_rr << " " << r.mv[0] << " " << r.mv[1] << " " << r.mv[2] << std::endl;
+    int ncyc = 0;
+    const Cycle loops[3] = {Cycle::V(), Cycle::F(), Cycle::W()};
+    const Cycle init = r.kind >= 10 ? Cycle::V() : Cycle::none();
+    bool ok = ac.cycle(init, loops[r.kind % 10], ResTol(-1.0), ResRat(r.rr), ac.mv_def,
+                       ac.ms_def, &ncyc);
+    std::cout << "ACRET " << r.tag << " " << ok << " " << ncyc << std::endl;
+    dump(r.tag + "_phi", interior(pr.phi));
+  }
+}
+
 Case parse_case(char **a, int nx, int ny, int nz) {
   Case cs;
   cs.nx = nx; cs.ny = ny; cs.nz = nz;
@@ -398,6 +464,18 @@ int main(int argc, char **argv) {
       dump("jacobi_x5", interior(x));
     }
     dump_ac(w);
+    return 0;
+  }
+  if (mode == "cycles") {
+    if (argc < 13) { std::fprintf(stderr, "cycles: too few arguments\n"); return 2; }
+    outdir = argv[2];
+    Case cs = parse_case(argv + 6, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
+    if (argc >= 16) {
+      cs.centred = true;
+      cs.cx = std::atof(argv[13]); cs.cy = std::atof(argv[14]); cs.cz = std::atof(argv[15]);
+    }
+    World w = build_world(cs);
+    dump_cycles(w);
     return 0;
   }
   if (mode == "bench-cg" || mode == "bench-ac") {

@@ -318,22 +318,27 @@ std::vector<double> dense_pinv(int nx, int ny, int nz, const std::vector<double>
       }
   std::vector<uint8_t> pin(m, 0);
   for (int P = 0; P < m; P++) pin[P] = act[P] ? 0 : 1;
+  // the operator couples P to P +- 1, +- nz, +- ny nz: its Cholesky factor
+  // stays inside that band (bw below the diagonal), so the loops skip the
+  // entries outside it, which are exact zeros (same sums, same rounding)
+  const int bw = nx > 1 ? ny * nz : (ny > 1 ? nz : 1);
   // row-by-row Cholesky (lower triangle in place, rows contiguous); an
   // unknown whose pivot vanishes is pinned: its row and column drop out
   for (int i = 0; i < m; i++) {
     double *Li = &M[(size_t)i * m];
+    const int lo_i = std::max(0, i - bw);
     if (pin[i]) {
       for (int j = 0; j <= i; j++) Li[j] = 0.0;
       continue;
     }
-    for (int j = 0; j <= i; j++) {
+    for (int j = lo_i; j <= i; j++) {
       if (pin[j]) {
         Li[j] = 0.0;
         continue;
       }
       const double *Lj = &M[(size_t)j * m];
       double sm = Li[j];
-      for (int k2 = 0; k2 < j; k2++) sm -= Li[k2] * Lj[k2];
+      for (int k2 = std::max(lo_i, j - bw); k2 < j; k2++) sm -= Li[k2] * Lj[k2];
       if (j < i) {
         Li[j] = sm / Lj[j];
       } else if (sm <= 1e-10 * std::fabs(c[i])) {
@@ -344,8 +349,9 @@ std::vector<double> dense_pinv(int nx, int ny, int nz, const std::vector<double>
       }
     }
   }
-  // columns of the inverse: two triangular solves each, independent, on
-  // the host's threads (the coarsest level of a stretched grid can hold
+  // columns of the inverse: two banded triangular solves each (the forward
+  // one starts at the column's own row: y is zero above it), independent,
+  // on the host's threads (the coarsest level of a stretched grid can hold
   // 12^3 = 1728 cells)
   std::vector<double> B((size_t)m * m, 0.0);
   auto columns = [&](int c0, int c1) {
@@ -353,10 +359,10 @@ std::vector<double> dense_pinv(int nx, int ny, int nz, const std::vector<double>
     for (int col = c0; col < c1; col++) {
       if (pin[col]) continue;
       for (int i = 0; i < m; i++) y[i] = (i == col) ? 1.0 : 0.0;
-      for (int i = 0; i < m; i++) {
+      for (int i = col; i < m; i++) {
         if (pin[i]) { y[i] = 0.0; continue; }
         double sm = y[i];
-        for (int k2 = 0; k2 < i; k2++)
+        for (int k2 = std::max(col, i - bw); k2 < i; k2++)
           if (!pin[k2]) sm -= M[(size_t)i * m + k2] * y[k2];
         y[i] = sm / M[(size_t)i * m + i];
       }
@@ -366,7 +372,7 @@ std::vector<double> dense_pinv(int nx, int ny, int nz, const std::vector<double>
         const double *Li = &M[(size_t)i * m];
         y[i] /= Li[i];
         const double yi = y[i];
-        for (int k2 = 0; k2 < i; k2++) y[k2] -= Li[k2] * yi;
+        for (int k2 = std::max(0, i - bw); k2 < i; k2++) y[k2] -= Li[k2] * yi;
       }
       for (int i = 0; i < m; i++) B[(size_t)i * m + col] = y[i];
     }

@@ -13,6 +13,7 @@ for sub in ("oracle", "psi-boil_amd"):
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 GOLDEN_CASES = ["drop12", "bubbles16x12x8", "bubbles16"]
+CYCLE_CASES = ["bubbles16x12x8", "bubbles16"]  # <case>_cycles.npz: AC::cycle runs
 
 
 def pytest_configure(config):
@@ -45,3 +46,20 @@ def golden(request):
 
 def load_golden(name):
     return Golden(name)
+
+
+def load_cycles(name):
+    """(phi of each run by tag, the runs' settings and logs) of <name>_cycles.npz."""
+    z = np.load(os.path.join(GOLDEN, name + "_cycles.npz"), allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    return {k: z[k] for k in z.files if k != "meta"}, meta["runs"]
+
+
+def cycle_kwargs(run):
+    """The keyword arguments of the oracle's AC.cycle for one reference run."""
+    shapes = ["V", "F", "W"]
+    return dict(loop=shapes[run["kind"] % 10], init="V" if run["kind"] >= 10 else None,
+                toler=-1.0, factor=run["factor"], max_cyc=run["max_cyc"],
+                min_cyc=run["min_cyc"], mv=tuple(run["mv"]), stop_if_div=run["stop_if_div"],
+                priority_min_cyc=run["priority_min_cyc"], use_linf=run["use_linf"],
+                resrat=run["resrat"])

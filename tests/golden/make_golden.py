@@ -8,7 +8,11 @@ outputs, solver iterates after k iterations, AC level matrices, restriction /
 interpolation results and V-cycle iterates, plus the solver log lines (17
 significant digits) parsed into arrays.
 
-    python tests/golden/make_golden.py          # needs oracle/_ref built
+    python tests/golden/make_golden.py [cases] [cycles]   # needs oracle/_ref built
+
+"cycles" writes <case>_cycles.npz: the reference's AC::cycle with the F, W
+and full-cycle shapes and its controls (ref_driver cycles), phi after each
+run and the run's residual log.
 
 Only the outputs travel; the reference does not.
 """
@@ -98,11 +102,65 @@ def run_case(name, spec):
     return path
 
 
+# cases of the AC::cycle runs (ref_driver cycles): F, W, full-cycle starts,
+# linf control, divergence / min-cycle controls, level tolerances
+CYCLE_CASES = ["bubbles16x12x8", "bubbles16"]
+
+
+def parse_cycles(text):
+    """ACRUN header (the run's settings), the cycle log lines, ACRET."""
+    runs, cur = [], None
+    for line in text.splitlines():
+        f = line.split()
+        if line.startswith("ACRUN "):
+            cur = {"tag": f[1], "kind": int(f[2]), "factor": float(f[3]), "max_cyc": int(f[4]),
+                   "use_linf": bool(int(f[5])), "stop_if_div": bool(int(f[6])),
+                   "priority_min_cyc": bool(int(f[7])), "min_cyc": int(f[8]),
+                   "resrat": float(f[9]), "mv": [int(f[10]), int(f[11]), int(f[12])],
+                   "initial": None, "cycles": []}
+            runs.append(cur)
+            continue
+        m = re.match(r"Initial res = " + NUM + " ; " + NUM, line)
+        if m and cur is not None:
+            cur["initial"] = [float(m.group(1)), float(m.group(2))]
+            continue
+        m = re.match(r"Cycle (\d+); res = " + NUM + " ; " + NUM, line)
+        if m and cur is not None:
+            cur["cycles"].append([int(m.group(1)), float(m.group(2)), float(m.group(3))])
+            continue
+        m = re.match(r"ACRET (\w+) (\d) (\d+)", line)
+        if m and cur is not None:
+            cur["converged"], cur["ncyc"] = bool(int(m.group(2))), int(m.group(3))
+    return runs
+
+
+def run_cycles(name):
+    nx, ny, nz, nsph, rmin, rmax, rin, rout, ss, sv, centre = CASES[name]
+    with tempfile.TemporaryDirectory() as d:
+        args = [DRIVER, "cycles", d, str(nx), str(ny), str(nz), str(nsph), repr(rmin),
+                repr(rmax), repr(rin), repr(rout), str(ss), str(sv)]
+        if centre is not None:
+            args += [repr(c) for c in centre]
+        out = subprocess.run(args, check=True, capture_output=True, text=True).stdout
+        runs = parse_cycles(out)
+        arrays = {r["tag"] + "_phi": np.fromfile(os.path.join(d, r["tag"] + "_phi.f64"))
+                  for r in runs}
+    arrays["meta"] = np.array(json.dumps({"name": name, "runs": runs}))
+    path = os.path.join(HERE, name + "_cycles.npz")
+    np.savez_compressed(path, **arrays)
+    return path
+
+
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("oracle/_ref/ref_driver is not built (make -f oracle/ref.mk)")
-    for name, spec in CASES.items():
-        print(run_case(name, spec))
+    what = sys.argv[1:] or ["cases", "cycles"]
+    if "cases" in what:
+        for name, spec in CASES.items():
+            print(run_case(name, spec))
+    if "cycles" in what:
+        for name in CYCLE_CASES:
+            print(run_cycles(name))
 
 
 if __name__ == "__main__":

@@ -1,0 +1,60 @@
+"""AC::cycle in the oracle against the reference's own build: the F and W
+cycles, full-cycle starts (fullcycle), the linf control with the reference's
+stale previous-linf value, the divergence check with and without
+priority_min_cyc, min_cycles, per-stage MaxIter and the level solves'
+ResRat -- phi after each run bit for bit, every logged residual (17
+significant digits) and the returned cycle count.  CPU only."""
+import numpy as np
+import pytest
+
+import psb_oracle as O
+from conftest import CYCLE_CASES, cycle_kwargs, load_cycles, load_golden
+
+
+def _runs():
+    out = []
+    for case in CYCLE_CASES:
+        _, runs = load_cycles(case)
+        out += [(case, r["tag"]) for r in runs]
+    return out
+
+
+@pytest.fixture(scope="module")
+def systems():
+    res = {}
+    for case in CYCLE_CASES:
+        g = load_golden(case)
+        res[case] = (g, load_cycles(case))
+    return res
+
+
+@pytest.mark.parametrize("case,tag", _runs())
+def test_cycle_matches_reference(systems, case, tag):
+    g, (phis, runs) = systems[case]
+    run = [r for r in runs if r["tag"] == tag][0]
+    A = O.Mat(*g.shape, g.mat())
+    ac = O.AC(A, g["dV"], rule=0)
+    phi, info = ac.cycle(np.zeros(A.n), g["b"], **cycle_kwargs(run))
+    assert np.array_equal(phi, phis[tag + "_phi"])
+    assert info["ncyc"] == run["ncyc"] and info["converged"] == run["converged"]
+    assert info["hist"][0] == pytest.approx(run["initial"][0], rel=1e-15)
+    assert info["hist_linf"][0] == pytest.approx(run["initial"][1], rel=1e-15)
+    assert len(info["hist"]) == 1 + len(run["cycles"])
+    for q, (c, res, linf) in enumerate(run["cycles"]):
+        assert info["hist"][q + 1] == pytest.approx(res, rel=1e-15), (c, q)
+        assert info["hist_linf"][q + 1] == pytest.approx(linf, rel=1e-15), (c, q)
+
+
+def test_fixtures_exercise_the_controls():
+    """The runs cover what they are meant to: a divergence stop, a run that
+    goes on past it without the check, min_cycles deferring it, a full-cycle
+    start whose result the divergence check discards, F and W shapes that
+    converge, and the linf control."""
+    _, runs = load_cycles("bubbles16x12x8")
+    r = {x["tag"]: x for x in runs}
+    assert r["Vdiv"]["ncyc"] < r["Vdiv"]["max_cyc"] and not r["Vdiv"]["converged"]
+    assert r["Vnodiv"]["ncyc"] == r["Vnodiv"]["max_cyc"]
+    assert r["Vmin"]["ncyc"] >= r["Vmin"]["min_cyc"]
+    assert r["Ffull"]["converged"] and r["Wfull"]["converged"]
+    assert r["FMVfull"]["converged"] and r["FMVfull"]["cycles"][0][0] == 0
+    assert r["Vlinf"]["converged"] and r["Vlinf"]["use_linf"]
