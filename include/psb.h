@@ -240,24 +240,42 @@ int psb_generate(const psb_case *cs, double *c, double *w, double *e,
                  unsigned char *off, double *dxc, double *dyc, double *dzc);
 
 /* ------------------------------------------------------------------------
- * The reference's own multigrid solve: AC::cycle with the V loop
- * (Src/Solver/Additive/additive.h:30-95 vcycle(ResTol, ResRat),
- * additive_cycle.cpp:4-99, additive_recursive.cpp:4-34) on the GPU in FP64:
- * its levels (smallest-prime-factor coarsening, domain_coarsen.cpp), its
- * Galerkin operators, CG smoothing with the diagonal preconditioner on every
- * level, restriction with mean removal, interpolation, and its stopping and
- * divergence rules (additive_cycle_functions.cpp:12-87).
+ * The reference's own multigrid solve: AC::cycle (Src/Solver/Additive/
+ * additive.h:30-95 vcycle / fcycle / wcycle / fullcycle, additive_cycle.cpp:
+ * 4-99, additive_recursive.cpp:4-163) on the GPU in FP64: its levels
+ * (smallest-prime-factor coarsening, domain_coarsen.cpp), its Galerkin
+ * operators, CG smoothing with the diagonal preconditioner on every level,
+ * restriction with mean removal, interpolation, the V, F and W cycles, the
+ * full-cycle start, and its stopping, divergence, min-cycle and linf
+ * controls (additive_cycle_functions.cpp:12-87, additive.h:97-142).
+ * Cycle::flex (additive_recursive.cpp:167-200) is not offered: its
+ * recursion ends only when a level's smoother meets its ResRat, and once a
+ * level's residual reaches round-off it alternates between two levels for
+ * good (so it does with the reference's own build on these cases).
  * ------------------------------------------------------------------------ */
+enum { PSB_CYCLE_NONE = -1, PSB_CYCLE_V = 0, PSB_CYCLE_F = 1, PSB_CYCLE_W = 2 };
+
 typedef struct {
   double res_tol;     /* ResTol toler (< 0: off, as vcycle(ResRat))          */
   double res_rat;     /* ResRat factor (< 0: off, as vcycle(ResTol))         */
   int max_cycles;     /* AC::max_cycles (20, additive.cpp:11)                */
   int min_cycles;     /* AC::min_cycles (0)                                  */
-  int max_iter[3];    /* AC::max_iter_array: CG iterations before / (unused)
-                         / after the coarse correction (20, 20, 20)          */
+  int max_iter[3];    /* AC::max_iter_array: CG iterations before the coarse
+                         correction / re-smoothing between the two descents
+                         of F and W / after it (20, 20, 20)                  */
   double dt;          /* time step: residual = sqrt(dot_voldiv_avg) * dt
                          (additive_residual.cpp:19-29); smoothers use 1/dt
                          as their scale (additive_cycle_functions.cpp:94)   */
+  int loop;           /* the loop's cycle: PSB_CYCLE_V (default), _F, _W     */
+  int init;           /* PSB_CYCLE_NONE (default), or the shape of a full
+                         cycle run before the loop (fullcycle(loop) uses V) */
+  int stop_if_div;    /* AC::stop_if_diverging (1)                           */
+  int priority_min_cyc; /* AC::priority_min_cycle (1)                        */
+  int use_linf;       /* AC::use_linf_error (0): the stop tests, res0, res
+                         and the history use max |r / dV| * dt              */
+  double smooth_res_rat; /* AC::resrat: ResRat of the level solves (1e-18)   */
+  double smooth_res_tol; /* AC::restol: ResTol of the level and coarsest
+                            solves (1e-18)                                   */
 } psb_cycle;
 
 void psb_cycle_defaults(psb_cycle *c);
@@ -266,7 +284,8 @@ void psb_cycle_defaults(psb_cycle *c);
  * layout.  res->iterations = cycles, res->converged = the bool cycle()
  * returns; PSB_DIVERGED when a cycle did not reduce the residual (x then
  * holds the previous cycle's phi, as the reference restores it).
- * psb_history gives the residual before and after every cycle.            */
+ * psb_history gives the residual before the first cycle, after the
+ * full-cycle start (when init is set) and after every cycle.              */
 int psb_ac_vcycle(psb_ctx *ctx, double *x, const double *b, const psb_cycle *c,
                   psb_result *res);
 int psb_ac_vcycle_device(psb_ctx *ctx, double *d_x, const double *d_b,

@@ -374,7 +374,23 @@ void ora_resid(const ora_mat *A, const double *b, const double *x, double *r) {
 }
 
 /* Src/Field/Scalar/scalar.h:296-298 */
+/* Summation order of the dot products: 0 (default) sequential as the
+ * reference; b > 0 sums blocks of b terms and then the block sums in order,
+ * as a blocked parallel reduction does.  Test infrastructure: measures how
+ * much an algorithm's result depends on the order of its sums. */
+static size_t sum_block = 0;
+void ora_set_sum_block(int b) { sum_block = b > 0 ? (size_t)b : 0; }
+
 double ora_dot(size_t n, const double *x, const double *y) {
+  if (sum_block) {
+    double d = 0.0;
+    for (size_t q = 0; q < n; q += sum_block) {
+      double t = 0.0;
+      for (size_t P = q; P < n && P < q + sum_block; P++) t += x[P] * y[P];
+      d += t;
+    }
+    return d;
+  }
   double d = 0.0;
   for (size_t P = 0; P < n; P++) d += x[P] * y[P];
   return d;
@@ -384,7 +400,15 @@ double ora_dot(size_t n, const double *x, const double *y) {
 double ora_dot_voldiv_avg(size_t n, const double *x, const double *y,
                           const double *dV) {
   double d = 0.0;
-  for (size_t P = 0; P < n; P++) d += x[P] * y[P] / dV[P] / dV[P];
+  if (sum_block) {
+    for (size_t q = 0; q < n; q += sum_block) {
+      double t = 0.0;
+      for (size_t P = q; P < n && P < q + sum_block; P++) t += x[P] * y[P] / dV[P] / dV[P];
+      d += t;
+    }
+  } else {
+    for (size_t P = 0; P < n; P++) d += x[P] * y[P] / dV[P] / dV[P];
+  }
   return d / (double)n / (double)n;
 }
 

@@ -97,6 +97,11 @@ void ora_project(int nx, int ny, int nz, const double *dxw, const double *dys,
                  const double *p, double dt, int mask, double *u, double *v,
                  double *w);
 
+/* Summation order of ora_dot / ora_dot_voldiv_avg: 0 (default) sequential
+ * like the reference, b > 0 blocks of b terms then the block sums (test
+ * infrastructure: sensitivity of a result to the order of its sums).        */
+void ora_set_sum_block(int b);
+
 /* ---- vector kernels ----------------------------------------------------- */
 void   ora_spmv(const ora_mat *A, const double *x, double *y);
 void   ora_resid(const ora_mat *A, const double *b, const double *x, double *r);

@@ -182,6 +182,11 @@ def project(mx, my, mz, rho, p, u, v, w, dt=1.0, off=None, mask=7):
     return u, v, w
 
 
+def set_sum_block(b):
+    """0: sequential sums (the reference's order); b > 0: blocks of b terms."""
+    lib().ora_set_sum_block(int(b))
+
+
 def spmv(A, x):
     y = np.zeros(A.n)
     lib().ora_spmv(C.byref(A.s), _p(np.ascontiguousarray(x, dtype=np.float64)), _p(y))

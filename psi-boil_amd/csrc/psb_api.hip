@@ -691,6 +691,7 @@ void destroy_graphs(psb_ctx *C) {
   destroy_graph(C->gr_ac32);
   destroy_graph(C->gr_ru);
   destroy_graph(C->href.gr);
+  destroy_graph(C->href.gfull);
 }
 
 // drop the hierarchy (and the graphs that use it): next AC solve rebuilds it
@@ -1188,35 +1189,90 @@ void ref_cg(psb_ctx *C, int l, int maxit, double res_rat, double res_tol, double
   }
 }
 
-void ref_vcycle(psb_ctx *C, int l, const int mv[3], double scale) {
+// the level-solve parameters of one cycle: MaxIter per stage (pre,
+// re-smoothing, post), AC::resrat / restol, the smoothers' scale 1/dt
+struct RefPar {
+  int mv[3];
+  double rr, rt, scale;
+};
+
+void ref_interp(psb_ctx *C, int lc) {  // level lc into lc - 1 (h.phi += H.phi)
   RefHier &H = C->href;
-  const int nl = (int)H.L.size();
-  auto interp = [&](int lc) {  // level lc into lc - 1
-    const Level<double> &cL = H.L[lc], &f = H.L[lc - 1];
-    hipLaunchKernelGGL(k_ac_interp, dim3(grid_blocks(f.g.N)), dim3(kBlock), 0, C->stream, f.g,
-                       f.e[0], cL.g, cL.e[0], cL.CI, cL.CJ, cL.CK);
-  };
-  if (l != nl - 1) {
-    ref_cg(C, l, mv[0], kAtto, kAtto, scale);
-    const Level<double> &f = H.L[l], &cL = H.L[l + 1];
-    const int nb = grid_blocks(cL.g.N);
-    hipLaunchKernelGGL(k_ac_restrict, dim3(nb), dim3(kBlock), 0, C->stream, f.g, f.mat(), f.e[0],
-                       f.r, cL.g, cL.CI, cL.CJ, cL.CK, cL.aflag, cL.r, C->work);
-    finalize<2>(C, FIN_ACMEAN, nb, C->st64);
-    hipLaunchKernelGGL(k_ac_submean, dim3(nb), dim3(kBlock), 0, C->stream, cL.g, cL.r, cL.aflag,
-                       C->st64);
-    ref_vcycle(C, l + 1, mv, scale);
-    ref_cg(C, l, mv[2], kAtto, kAtto, scale);
-    if (l > 0) interp(l);
+  const Level<double> &cL = H.L[lc], &f = H.L[lc - 1];
+  hipLaunchKernelGGL(k_ac_interp, dim3(grid_blocks(f.g.N)), dim3(kBlock), 0, C->stream, f.g,
+                     f.e[0], cL.g, cL.e[0], cL.CI, cL.CJ, cL.CK);
+}
+
+void ref_restrict(psb_ctx *C, int l) {  // residual of level l into l + 1, mean removed
+  RefHier &H = C->href;
+  const Level<double> &f = H.L[l], &cL = H.L[l + 1];
+  const int nb = grid_blocks(cL.g.N);
+  hipLaunchKernelGGL(k_ac_restrict, dim3(nb), dim3(kBlock), 0, C->stream, f.g, f.mat(), f.e[0],
+                     f.r, cL.g, cL.CI, cL.CJ, cL.CK, cL.aflag, cL.r, C->work);
+  finalize<2>(C, FIN_ACMEAN, nb, C->st64);
+  hipLaunchKernelGGL(k_ac_submean, dim3(nb), dim3(kBlock), 0, C->stream, cL.g, cL.r, cL.aflag,
+                     C->st64);
+}
+
+void ref_coarsest(psb_ctx *C, int l, const RefPar &q) {  // MaxIter 100, ResRat -1
+  ref_cg(C, l, 100, -1.0, q.rt, q.scale);
+  ref_interp(C, l);
+}
+
+// v_cycle (additive_recursive.cpp:4-34)
+void ref_v(psb_ctx *C, int l, const RefPar &q) {
+  if (l != (int)C->href.L.size() - 1) {
+    ref_cg(C, l, q.mv[0], q.rr, q.rt, q.scale);
+    ref_restrict(C, l);
+    ref_v(C, l + 1, q);
+    ref_cg(C, l, q.mv[2], q.rr, q.rt, q.scale);
+    if (l > 0) ref_interp(C, l);
   } else {
-    ref_cg(C, l, 100, -1.0, kAtto, scale);
-    interp(l);
+    ref_coarsest(C, l, q);
   }
 }
 
-// AC::residual (additive_residual.cpp:19-29) of level 0 into C->relres[0]:
-// sqrt(dot_voldiv_avg(f - A phi)) * dt
-double ref_residual(psb_ctx *C) {
+// f_cycle (:37-79) and w_cycle (:82-124): after the first descent (an F-
+// or a W-cycle) the level is smoothed again (MaxIter mv[1]), its residual
+// restricted once more and a second descent made (F: a V-cycle, W: a
+// W-cycle) on the coarse phi the first one left
+void ref_fw(psb_ctx *C, int l, const RefPar &q, bool w) {
+  if (l != (int)C->href.L.size() - 1) {
+    ref_cg(C, l, q.mv[0], q.rr, q.rt, q.scale);
+    ref_restrict(C, l);
+    ref_fw(C, l + 1, q, w);
+    ref_cg(C, l, q.mv[1], q.rr, q.rt, q.scale);
+    ref_restrict(C, l);
+    if (w) ref_fw(C, l + 1, q, true);
+    else ref_v(C, l + 1, q);
+    ref_cg(C, l, q.mv[2], q.rr, q.rt, q.scale);
+    if (l > 0) ref_interp(C, l);
+  } else {
+    ref_coarsest(C, l, q);
+  }
+}
+
+void ref_loop(psb_ctx *C, int l, int shape, const RefPar &q) {
+  if (shape == PSB_CYCLE_V) ref_v(C, l, q);
+  else ref_fw(C, l, q, shape == PSB_CYCLE_W);
+}
+
+// full_cycle (:127-163): restrict down to the coarsest level, then the
+// chosen cycle from every level on the way back
+void ref_full(psb_ctx *C, int l, int shape, const RefPar &q) {
+  if (l != (int)C->href.L.size() - 1) {
+    ref_restrict(C, l);
+    ref_full(C, l + 1, shape, q);
+    ref_loop(C, l, shape, q);
+  } else {
+    ref_coarsest(C, l, q);
+  }
+}
+
+// AC::residual (additive_residual.cpp:19-29) of level 0: sqrt(dot_voldiv_avg
+// (f - A phi)) * dt, and max |f - A phi| / dV * dt (Scalar::max_abs_voldiv,
+// scalar_minmax.cpp:155-165) into *linf
+double ref_residual(psb_ctx *C, double *linf) {
   RefHier &H = C->href;
   const Level<double> &lv = H.L[0];
   const Grid &g = lv.g;
@@ -1227,35 +1283,52 @@ double ref_residual(psb_ctx *C) {
                        C->work);
   });
   finalize<3>(C, FIN_ACRES, nblocks<double>(g), C->st32, nullptr, C->relres);
-  double v = 0.0;
-  HIPCHK(hipMemcpyAsync(&v, C->relres, 8, hipMemcpyDeviceToHost, C->stream));
+  const int nb = grid_blocks(g.N);
+  hipLaunchKernelGGL(k_absmax_voldiv, dim3(nb), dim3(kBlock), 0, C->stream, g, H.M[0],
+                     H.K[0][0], C->work.partial);
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kBlock), 0, C->stream, C->work.partial, nb,
+                     C->relres + 1);
+  double v[2] = {0.0, 0.0};
+  HIPCHK(hipMemcpyAsync(v, C->relres, 16, hipMemcpyDeviceToHost, C->stream));
   HIPCHK(hipStreamSynchronize(C->stream));
-  return v;
+  *linf = v[1] * C->ref_dt;
+  return v[0];
 }
 
-int ac_vcycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_result *R) {
+// AC::cycle (additive_cycle.cpp:4-99) with init_cycles and converged
+// (additive_cycle_functions.cpp:12-87).  The cycles run as captured graphs
+// (one for the loop's cycle, one for the full-cycle start); the stop tests
+// read the residual on the host after each, as the reference does.
+int ac_cycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_result *R) {
   if (P->max_cycles < 0 || P->min_cycles < 0 || P->dt <= 0.0 || P->max_iter[0] < 1 ||
-      P->max_iter[2] < 1)
+      P->max_iter[1] < 1 || P->max_iter[2] < 1)
     return PSB_EINVAL;
+  if (P->loop < PSB_CYCLE_V || P->loop > PSB_CYCLE_W) return PSB_EINVAL;
+  if (P->init < PSB_CYCLE_NONE || P->init > PSB_CYCLE_W) return PSB_EINVAL;
   build_ref(C);
   RefHier &H = C->href;
   const Level<double> &L0 = H.L[0];
   const size_t N0 = L0.g.N;
   HIPCHK(hipMemcpyAsync(L0.e[0], x, N0 * 8, hipMemcpyDeviceToDevice, C->stream));
   HIPCHK(hipMemcpyAsync(L0.r, b, N0 * 8, hipMemcpyDeviceToDevice, C->stream));
-  // the residual norm's state: N of level 0, scale 1/dt
+  // the residual norm's state: N of level 0 and dt
   psb_stop s;
   psb_stop_defaults(&s);
-  s.scale = 1.0 / P->dt;
+  s.scale = P->dt;  // AC::residual multiplies by dt
+  C->ref_dt = P->dt;
   init_state(C, C->st32, &s, 0, 0.0, 0.0, 1, false);
   const double toler = P->res_tol, factor = P->res_rat, dti = 1.0 / P->dt;
+  const bool linf = P->use_linf != 0;
   C->hist_host.clear();
-  const double res_0 = ref_residual(C);
-  C->hist_host.push_back(res_0);
+  double linf_0 = 0.0;
+  const double res_0 = ref_residual(C, &linf_0);
+  C->hist_host.push_back(linf ? linf_0 : res_0);
+  // fold: the last "good" phi (L0.e[1])
   HIPCHK(hipMemcpyAsync(L0.e[1], L0.e[0], N0 * 8, hipMemcpyDeviceToDevice, C->stream));
   int ncyc = 0, conv = 0, status = PSB_MAXITER;
-  double res = res_0;
-  if ((res_0 < toler && P->min_cycles == 0) || res_0 == 0.0) {
+  double res_out = linf ? linf_0 : res_0;
+  const double r0c = linf ? linf_0 : res_0;
+  if ((r0c < toler && P->min_cycles == 0) || res_0 == 0.0) {
     conv = 1;
     status = PSB_OK;
   } else {
@@ -1263,30 +1336,66 @@ int ac_vcycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_re
       zero(C, H.L[l].e[0], H.L[l].g.N);
       zero(C, H.L[l].r, H.L[l].g.N);
     }
-    double res_m1 = ref_residual(C);
-    const int mv[3] = {P->max_iter[0], P->max_iter[1], P->max_iter[2]};
-    const long long key = (long long)mv[0] + ((long long)mv[2] << 20);
-    // dt enters the level solves' scale: part of the captured graph
-    if (H.gr.key != key || H.gr.fkey != dti) destroy_graph(H.gr);
+    const RefPar q{{P->max_iter[0], P->max_iter[1], P->max_iter[2]}, P->smooth_res_rat,
+                   P->smooth_res_tol, dti};
+    // the graphs hold the cycle's shape and MaxIter values; the tolerances
+    // and the scale are kernel arguments compared separately
+    const long long key = (long long)q.mv[0] + ((long long)q.mv[1] << 16) +
+                          ((long long)q.mv[2] << 32) + ((long long)P->loop << 48);
+    const long long fkey = key + ((long long)(P->init + 1) << 52);
+    if (H.gr.key != key || H.gr.fkey != dti || H.grr != q.rr || H.grt != q.rt) {
+      destroy_graph(H.gr);
+      destroy_graph(H.gfull);
+    }
+    H.gr.fkey = dti;
+    H.grr = q.rr;
+    H.grt = q.rt;
+    // AC::converged: 0 converged, 1 diverging (fold restored), 2 go on
+    auto converged = [&](int cycle, double &res1, double &linf1, double res0,
+                         double linf0) {
+      res1 = ref_residual(C, &linf1);
+      const double rc = linf ? linf1 : res1;
+      const double prev = linf ? linf0 : res0;
+      const double beg = linf ? linf_0 : res_0;
+      if (cycle >= P->min_cycles && (rc / beg < factor || rc < toler)) return 0;
+      bool check = P->stop_if_div != 0;
+      if (P->priority_min_cyc && cycle < P->min_cycles) check = false;
+      if (check && rc >= prev) {
+        HIPCHK(hipMemcpyAsync(L0.e[0], L0.e[1], N0 * 8, hipMemcpyDeviceToDevice, C->stream));
+        return 1;
+      }
+      return 2;
+    };
+    double res_m0 = 0.0, linf_m0 = 0.0, res_m1 = 0.0, linf_m1 = 0.0;
+    if (P->init != PSB_CYCLE_NONE) {
+      capture(C, H.gfull, fkey, [&]() { ref_full(C, 0, P->init, q); });
+      HIPCHK(hipGraphLaunch(H.gfull.exec, C->stream));
+      res_m1 = ref_residual(C, &linf_m1);
+      // its verdict is not used (additive_cycle.cpp:40-42); with the
+      // divergence check on, the equal residuals restore fold
+      (void)converged(0, res_m0, linf_m0, res_m1, linf_m1);
+      C->hist_host.push_back(linf ? linf_m0 : res_m0);
+      res_out = linf ? linf_m0 : res_m0;
+    } else {
+      res_m1 = ref_residual(C, &linf_m1);
+    }
     for (int c = 1; c <= P->max_cycles; c++) {
-      capture(C, H.gr, key, [&]() { ref_vcycle(C, 0, mv, dti); });
-      H.gr.fkey = dti;
+      capture(C, H.gr, key, [&]() { ref_loop(C, 0, P->loop, q); });
       HIPCHK(hipGraphLaunch(H.gr.exec, C->stream));
-      const double res1 = ref_residual(C);
-      C->hist_host.push_back(res1);
-      res = res1;
+      const int con = converged(c, res_m0, linf_m0, res_m1, linf_m1);
+      C->hist_host.push_back(linf ? linf_m0 : res_m0);
+      res_out = linf ? linf_m0 : res_m0;
+      res_m1 = res_m0;  // linf_m1 keeps its value (additive_cycle.cpp:78)
       ncyc = c;
-      if (c >= P->min_cycles && (res1 / res_0 < factor || res1 < toler)) {
+      if (con == 0) {
         conv = 1;
         status = PSB_OK;
         break;
       }
-      if (!(c < P->min_cycles) && res1 >= res_m1) {  // diverging: keep the previous phi
-        HIPCHK(hipMemcpyAsync(L0.e[0], L0.e[1], N0 * 8, hipMemcpyDeviceToDevice, C->stream));
+      if (con == 1) {
         status = PSB_DIVERGED;
         break;
       }
-      res_m1 = res1;
       HIPCHK(hipMemcpyAsync(L0.e[1], L0.e[0], N0 * 8, hipMemcpyDeviceToDevice, C->stream));
     }
   }
@@ -1298,8 +1407,8 @@ int ac_vcycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_re
   R->outer = 0;
   R->converged = conv;
   R->status = status;
-  R->res0 = res_0;
-  R->res = res;
+  R->res0 = r0c;
+  R->res = res_out;
   return status;
 }
 
@@ -1746,10 +1855,17 @@ void psb_cycle_defaults(psb_cycle *c) {
   std::memset(c, 0, sizeof(*c));
   c->res_tol = -1.0;
   c->res_rat = -1.0;
-  c->max_cycles = 20;  // additive.cpp:11-19
+  c->max_cycles = 20;  // additive.cpp:11-22
   c->min_cycles = 0;
   c->max_iter[0] = c->max_iter[1] = c->max_iter[2] = 20;
   c->dt = 1.0;
+  c->loop = PSB_CYCLE_V;
+  c->init = PSB_CYCLE_NONE;
+  c->stop_if_div = 1;
+  c->priority_min_cyc = 1;
+  c->use_linf = 0;
+  c->smooth_res_rat = kAtto;
+  c->smooth_res_tol = kAtto;
 }
 
 int psb_ac_vcycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_result *R) {
@@ -1763,7 +1879,7 @@ int psb_ac_vcycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, ps
     const size_t N = C->g.N;
     HIPCHK(hipMemcpy(C->dx, hx.data(), N * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(C->db, hb.data(), N * 8, hipMemcpyHostToDevice));
-    int rc = ac_vcycle(C, C->dx, C->db, P, R);
+    int rc = ac_cycle(C, C->dx, C->db, P, R);
     if (rc < 0) return rc;
     R->rel_l2 = true_rel(C, C->db, C->dx);
     HIPCHK(hipMemcpy(hx.data(), C->dx, N * 8, hipMemcpyDeviceToHost));
@@ -1777,7 +1893,7 @@ int psb_ac_vcycle_device(psb_ctx *C, double *x, const double *b, const psb_cycle
   if (!C || !x || !b || !P || !R) return PSB_EINVAL;
   if (!C->have_matrix) return PSB_ENOTREADY;
   return guarded(C, [&]() {
-    int rc = ac_vcycle(C, x, b, P, R);
+    int rc = ac_cycle(C, x, b, P, R);
     if (rc >= 0) R->rel_l2 = true_rel(C, b, x);
     return rc;
   });

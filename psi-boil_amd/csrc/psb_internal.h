@@ -164,7 +164,9 @@ struct RefHier {
   std::vector<Metric> M;
   std::vector<std::vector<double *>> K;
   bool built = false;
-  Graph gr;
+  Graph gr;              // one cycle of the loop
+  Graph gfull;           // the full-cycle start
+  double grr = 0.0, grt = 0.0;  // level tolerances the graphs were captured with
 };
 
 }  // namespace host
@@ -211,6 +213,7 @@ struct psb_ctx {
   double *db = nullptr, *dx = nullptr;  // staging for host-array solves
   double *dy = nullptr;                 // scratch for psb_apply
   double *relres = nullptr;             // 2 doubles: sum r^2, sum b^2
+  double ref_dt = 1.0;                  // dt of the running AC cycle (its residual)
   Graph gr_cgdiag, gr_bicgs, gr_ac64, gr_ac32, gr_ru;
   double ac_omega = 0, ac_cfac = 0;
   int coarse_min = 8;  // coarsening halves a direction while it stays >= this

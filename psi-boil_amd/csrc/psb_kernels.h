@@ -797,9 +797,12 @@ __device__ inline void fin_apply(int op, const double *tot, DevState *st, DevSta
     case FIN_ACMEAN:  // mean of the restricted right-hand side over active cells
       st->mean[0] = tot[1] > 0.0 ? tot[0] / tot[1] : 0.0;
       break;
-    case FIN_ACRES:  // AC::residual: sqrt(dot_voldiv_avg(r, r)) * dt (scale = 1/dt)
-      out[0] = ref_norm(st, tot[0]);
+    case FIN_ACRES: {  // AC::residual: sqrt(dot_voldiv_avg(r, r)) * dt (scale = dt)
+      double d = tot[0] / st->ntot;
+      d = d / st->ntot;
+      out[0] = sqrt(d) * st->scale;
       break;
+    }
     case FIN_RU_END:  // true residual at exit without convergence
       st->res_true = st->norm == 0 ? ref_norm(st, tot[1]) : sqrt(tot[0]);
       break;
@@ -1175,6 +1178,41 @@ __global__ void k_flux_d(Grid g, DMat<double> A, T *d, double scale) {
     s += A.b[P] + (k < g.nz - 1 ? A.b[P + 1] : 0.0);
     d[P] = (T)(float)(scale * (A.c[P] - s));
   }
+}
+
+// per-block max |r / dV| (Scalar::max_abs_voldiv, scalar_minmax.cpp:155-165;
+// the maximum does not depend on the order) into out[blockIdx.x]
+static __global__ __launch_bounds__(kBlock) void k_absmax_voldiv(Grid g, Metric m,
+                                                                 const double *r, double *out) {
+  __shared__ double sm[kBlock];
+  double v = 0.0;
+  for (int P = blockIdx.x * blockDim.x + threadIdx.x; P < g.N; P += gridDim.x * blockDim.x) {
+    int i, j, k;
+    g.ijk(P, i, j, k);
+    v = fmax(v, fabs(r[P] / m.dV(i, j, k)));
+  }
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
+}
+
+// one block: out[0] = max of a[0..n)
+static __global__ __launch_bounds__(kBlock) void k_max_final(const double *a, int n,
+                                                             double *out) {
+  __shared__ double sm[kBlock];
+  double v = 0.0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) v = fmax(v, a[q]);
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sm[0];
 }
 
 // per-block maximum of a (n >= 1) into out[blockIdx.x]

@@ -66,9 +66,29 @@ class Metrics(C.Structure):
                                           "dzb", "dzt")]
 
 
+PSB_CYCLE_NONE, PSB_CYCLE_V, PSB_CYCLE_F, PSB_CYCLE_W = -1, 0, 1, 2
+
+
 class Cycle(C.Structure):
     _fields_ = [("res_tol", C.c_double), ("res_rat", C.c_double), ("max_cycles", C.c_int),
-                ("min_cycles", C.c_int), ("max_iter", C.c_int * 3), ("dt", C.c_double)]
+                ("min_cycles", C.c_int), ("max_iter", C.c_int * 3), ("dt", C.c_double),
+                ("loop", C.c_int), ("init", C.c_int), ("stop_if_div", C.c_int),
+                ("priority_min_cyc", C.c_int), ("use_linf", C.c_int),
+                ("smooth_res_rat", C.c_double), ("smooth_res_tol", C.c_double)]
+
+
+def _cycle(**cyc):
+    c = Cycle()
+    lib().psb_cycle_defaults(C.byref(c))
+    shapes = {None: PSB_CYCLE_NONE, "V": PSB_CYCLE_V, "F": PSB_CYCLE_F, "W": PSB_CYCLE_W}
+    for k, v in cyc.items():
+        if k == "max_iter":
+            c.max_iter[:] = list(v)
+        elif k in ("loop", "init") and (v is None or isinstance(v, str)):
+            setattr(c, k, shapes[v])
+        else:
+            setattr(c, k, v)
+    return c
 
 
 class Case(C.Structure):
@@ -232,29 +252,18 @@ class Solver:
                       "psb_project_device")
 
     def ac_vcycle(self, x, b, **cyc):
-        """The reference's AC::vcycle (CG smoothers on its own levels)."""
+        """The reference's AC::cycle (CG smoothers on its own levels); loop and
+        init may be given as "V", "F", "W" (init None: no full-cycle start)."""
         x = np.array(self._host(x), copy=True)
         b = self._host(b)
-        c = Cycle()
-        lib().psb_cycle_defaults(C.byref(c))
-        for k, v in cyc.items():
-            if k == "max_iter":
-                c.max_iter[:] = list(v)
-            else:
-                setattr(c, k, v)
+        c = _cycle(**cyc)
         r = Result()
         _check(lib().psb_ac_vcycle(self._c, _dp(x), _dp(b), C.byref(c), C.byref(r)),
                "psb_ac_vcycle")
         return x, r.asdict()
 
     def ac_vcycle_device(self, d_x, d_b, **cyc):
-        c = Cycle()
-        lib().psb_cycle_defaults(C.byref(c))
-        for k, v in cyc.items():
-            if k == "max_iter":
-                c.max_iter[:] = list(v)
-            else:
-                setattr(c, k, v)
+        c = _cycle(**cyc)
         r = Result()
         _check(lib().psb_ac_vcycle_device(self._c, C.c_void_p(d_x), C.c_void_p(d_b),
                                           C.byref(c), C.byref(r)), "psb_ac_vcycle_device")
