@@ -194,6 +194,28 @@ int psb_project_device(psb_ctx *ctx, const double *d_rho, const double *d_p,
                        double dt, int mask, double *d_u, double *d_v,
                        double *d_w);
 
+/* ------------------------------------------------------------------------
+ * Checkpoints of a cell field in the reference's .bck format
+ * (Scalar::save / Scalar::load, Src/Field/Scalar/scalar_save.cpp:20-32,
+ * scalar_load.cpp:29-39): the padded extents as three ints, then the padded
+ * array (bw ghost layers per side; the reference's BW is 3) as doubles,
+ * [i][j][k] with k fastest.  A warm start across time steps or runs reads
+ * x back from the file the reference (or psb_save_bck) wrote.
+ * ------------------------------------------------------------------------ */
+/* name_file (Src/Global/global_name_file.cpp:4-32): name_p<rank>_<step>.bck
+ * (rank or step < 0: left out).                                             */
+int psb_bck_name(const char *name, int step, int rank, char *out, int cap);
+/* x: compact interior nx*ny*nz (host); the ghost layers written hold the
+ * nearest interior value.                                                   */
+int psb_save_bck(const char *path, int nx, int ny, int nz, int bw,
+                 const double *x);
+/* Reads a .bck of interior nx*ny*nz with any ghost width (returned in *bw,
+ * may be NULL) into the compact x.  PSB_EINVAL on a size mismatch.          */
+int psb_load_bck(const char *path, int nx, int ny, int nz, double *x, int *bw);
+/* The same for a device array of the context's grid (ghost width 3 on save). */
+int psb_save_bck_device(psb_ctx *ctx, const char *path, const double *d_x);
+int psb_load_bck_device(psb_ctx *ctx, const char *path, double *d_x);
+
 /* Linear::solve (linear.h:46-56): x (in: initial guess, out: solution) and b
  * are host arrays in the context's layout.                                  */
 int psb_solve(psb_ctx *ctx, double *x, const double *b, const psb_stop *stop,

@@ -9,6 +9,7 @@
 //
 //   ref_driver case  OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel [cx cy cz]
 //   ref_driver cycles OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel [cx cy cz]
+//   ref_driver bck OUT IN.bck nx ny nz   (Scalar::load of IN, Scalar::save of a seeded field)
 //   ref_driver bench-cg nx nsph rmin rmax rho_in rho_out seed_sph seed_vel maxit
 //   ref_driver bench-ac nx nsph rmin rmax rho_in rho_out seed_sph seed_vel cycles
 //   (bench modes: optional cx cy cz after the limit for one centred sphere)
@@ -476,6 +477,31 @@ int main(int argc, char **argv) {
     }
     World w = build_world(cs);
     dump_cycles(w);
+    return 0;
+  }
+  if (mode == "bck") {
+    if (argc < 7) { std::fprintf(stderr, "bck: too few arguments\n"); return 2; }
+    outdir = argv[2];
+    Case cs;
+    cs.nx = std::atoi(argv[4]); cs.ny = std::atoi(argv[5]); cs.nz = std::atoi(argv[6]);
+    cs.nsph = 0; cs.rmin = cs.rmax = 0.1; cs.rho_in = cs.rho_out = 1.0;
+    cs.seed_sph = cs.seed_vel = 1;
+    World w = build_world(cs);
+    Scalar s(*w.d);
+    {
+      std::ifstream in(argv[3], std::ios::binary);
+      s.load(in);
+    }
+    dump("bck_loaded", interior(s));
+    // a seeded field, ghosts included, written by the reference
+    std::vector<double> full = seeded(17, (size_t)s.ni() * s.nj() * s.nk());
+    std::memcpy(s.val[0][0], full.data(), full.size() * sizeof(double));
+    {
+      std::ofstream out(outdir + "/ref.bck", std::ios::binary);
+      s.save(out);
+    }
+    dump("bck_saved_interior", interior(s));
+    std::cout << "BCKNAME " << name_file("press", ".bck", 12, boil::cart.iam()) << "\n";
     return 0;
   }
   if (mode == "bench-cg" || mode == "bench-ac") {

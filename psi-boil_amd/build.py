@@ -17,7 +17,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
          "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["psb_api.hip", "psb_batch.hip", "psb_gen.cpp"]
+SOURCES = ["psb_api.hip", "psb_batch.hip", "psb_gen.cpp", "psb_io.cpp"]
 
 
 def build(verbose=False):

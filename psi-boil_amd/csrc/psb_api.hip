@@ -1780,6 +1780,26 @@ int psb_project_device(psb_ctx *C, const double *d_rho, const double *d_p, doubl
   });
 }
 
+int psb_save_bck_device(psb_ctx *C, const char *path, const double *d_x) {
+  if (!C || !path || !d_x) return PSB_EINVAL;
+  return guarded(C, [&]() {
+    std::vector<double> h(C->g.N);
+    HIPCHK(hipMemcpy(h.data(), d_x, h.size() * 8, hipMemcpyDeviceToHost));
+    return psb_save_bck(path, C->g.nx, C->g.ny, C->g.nz, 3, h.data());
+  });
+}
+
+int psb_load_bck_device(psb_ctx *C, const char *path, double *d_x) {
+  if (!C || !path || !d_x) return PSB_EINVAL;
+  return guarded(C, [&]() {
+    std::vector<double> h(C->g.N);
+    const int rc = psb_load_bck(path, C->g.nx, C->g.ny, C->g.nz, h.data(), nullptr);
+    if (rc != PSB_OK) return rc;
+    HIPCHK(hipMemcpy(d_x, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    return (int)PSB_OK;
+  });
+}
+
 int psb_set_active(psb_ctx *C, const unsigned char *off) {
   if (!C) return PSB_EINVAL;
   // takes effect at the next psb_set_matrix, which rebuilds the hierarchy

@@ -107,7 +107,8 @@ EXPORTS = ["psb_stop_defaults", "psb_status_string", "psb_create", "psb_destroy"
            "psb_batch_system", "psb_batch_set_tuning", "psb_batch_plan", "psb_batch_solve",
            "psb_batch_solve_device", "psb_cycle_defaults", "psb_ac_vcycle",
            "psb_ac_vcycle_device", "psb_assemble_pressure", "psb_assemble_pressure_device",
-           "psb_update_rhs_device", "psb_project_device"]
+           "psb_update_rhs_device", "psb_project_device", "psb_bck_name", "psb_save_bck",
+           "psb_load_bck", "psb_save_bck_device", "psb_load_bck_device"]
 
 _lib = None
 
@@ -251,6 +252,14 @@ class Solver:
                                                C.c_void_p(d_v), C.c_void_p(d_w)),
                       "psb_project_device")
 
+    def save_bck_device(self, path, d_x):
+        return _check(lib().psb_save_bck_device(self._c, path.encode(), C.c_void_p(d_x)),
+                      "psb_save_bck_device")
+
+    def load_bck_device(self, path, d_x):
+        return _check(lib().psb_load_bck_device(self._c, path.encode(), C.c_void_p(d_x)),
+                      "psb_load_bck_device")
+
     def ac_vcycle(self, x, b, **cyc):
         """The reference's AC::cycle (CG smoothers on its own levels); loop and
         init may be given as "V", "F", "W" (init None: no full-cycle start)."""
@@ -367,6 +376,27 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def bck_name(name, step=-1, rank=-1):
+    """name_file(name, ".bck", step, rank) of the reference."""
+    buf = C.create_string_buffer(4096)
+    _check(lib().psb_bck_name(name.encode(), int(step), int(rank), buf, 4096), "psb_bck_name")
+    return buf.value.decode()
+
+
+def save_bck(path, x, shape, bw=3):
+    """Scalar::save of a compact interior field x of the given (nx, ny, nz)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    _check(lib().psb_save_bck(path.encode(), *shape, int(bw), _dp(x)), "psb_save_bck")
+
+
+def load_bck(path, shape):
+    """Scalar::load into a compact interior field; returns (x, ghost width)."""
+    x = np.zeros(int(np.prod(shape)))
+    bw = C.c_int(0)
+    _check(lib().psb_load_bck(path.encode(), *shape, _dp(x), C.byref(bw)), "psb_load_bck")
+    return x, bw.value
 
 
 def device_copy(dst, src):

@@ -120,3 +120,27 @@ def test_solve_then_project_leaves_the_residual_as_divergence_at_128():
     nb = np.linalg.norm(b)
     assert np.linalg.norm(div - res) <= 1e-12 * nb
     assert np.linalg.norm(div) <= 1.0e-10 * nb * 1.01
+
+
+def test_checkpoint_warm_start_on_the_device(tmp_path):
+    """A converged pressure saved as .bck from the device and loaded into a
+    fresh device array starts the next solve converged (0 iterations)."""
+    n = 32
+    d = psb.generate(n, n, n, 4, 0.05, 0.15, 1.0, 1000.0, 3, 1003)
+    s = psb.Solver(n, n, n)
+    s.set_matrix(d)
+    s.set_geometry(d["dxc"], d["dyc"], d["dzc"])
+    db, dx = psb.DeviceArray.from_host(d["rhs"]), psb.DeviceArray(n ** 3)
+    dx.zero()
+    stop = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=0.0, res_tol=1e-9,
+                max_iter=200)
+    r = s.solve_device(dx.addr, db.addr, **stop)
+    assert r["converged"]
+    path = str(tmp_path / psb.bck_name("press", 7, 0))
+    s.save_bck_device(path, dx.addr)
+    dy = psb.DeviceArray(n ** 3)
+    dy.zero()
+    s.load_bck_device(path, dy.addr)
+    assert np.array_equal(dy.download(), dx.download())
+    r2 = s.solve_device(dy.addr, db.addr, **stop)
+    assert r2["converged"] and r2["iterations"] <= 1
