@@ -9,7 +9,7 @@ V-cycle (2+2 damped Jacobi, coarsest 8^3 exact FP64): FP32 inner solve inside
 FP64 iterative refinement (the value), and the plain FP64 solve beside it.
 Matrix and right-hand side are resident in HBM before the timed region.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu] [--no-ref-cycle]
 
 `--case batch` solves 96 seeded systems of uneven sizes as one batch and
 beside it one after another.  `--case 512` and `--case cyl` print the figures
@@ -348,6 +348,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ref-cycle", action="store_true",
+                    help="leave out the reference AC::vcycle timing (rocprofv3 kernel tracing "
+                         "faults on its ~1500-node captured graph)")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--nsys", type=int, default=96, help="systems of --case batch")
     ap.add_argument("--case", choices=sorted(EXTRA) + ["batch", "drop64"], default=None,
@@ -418,16 +421,20 @@ def main():
             "spmv_fp32_GBs": round(gbs(by_s32, ms_s32), 1),
             "spmv_fp64_GBs": round(gbs(by_s64, ms_s64), 1)}
     # the reference's own multigrid (AC::vcycle) on the GPU: cycles and time
-    log("[bench] reference AC::vcycle on the GPU")
-    dx.zero()
-    rv = s.ac_vcycle_device(dx.addr, db.addr, res_rat=1e-8, max_cycles=50)
-    psb.device_synchronize()
-    t0 = time.perf_counter()
-    dx.zero()
-    rv = s.ac_vcycle_device(dx.addr, db.addr, res_rat=1e-8, max_cycles=50)
-    psb.device_synchronize()
-    t_vc = time.perf_counter() - t0
-    log(f"[bench] AC::vcycle {t_vc * 1e3:.1f} ms {rv}")
+    ref_cycle = None
+    if not a.no_ref_cycle:
+        log("[bench] reference AC::vcycle on the GPU")
+        dx.zero()
+        rv = s.ac_vcycle_device(dx.addr, db.addr, res_rat=1e-8, max_cycles=50)
+        psb.device_synchronize()
+        t0 = time.perf_counter()
+        dx.zero()
+        rv = s.ac_vcycle_device(dx.addr, db.addr, res_rat=1e-8, max_cycles=50)
+        psb.device_synchronize()
+        t_vc = time.perf_counter() - t0
+        log(f"[bench] AC::vcycle {t_vc * 1e3:.1f} ms {rv}")
+        ref_cycle = {"ms": round(t_vc * 1e3, 2), "cycles": rv["iterations"],
+                     "status": rv["status_name"], "rel_residual": rv["rel_l2"]}
     log("[bench] pressure phase of a time step (assembly, rhs, solve, projection)")
     step = pressure_step(n, 3)
     log(f"[bench] pressure step {step}")
@@ -466,9 +473,7 @@ def main():
                        "mixed_fp64_updates": rmx["outer"]},
         "rel_residual": {"fp64": r64["rel_l2"], "mixed": rmx["rel_l2"]},
         "error_vs_fp64_1e-12": err,
-        "reference_ac_vcycle_gpu": {"ms": round(t_vc * 1e3, 2), "cycles": rv["iterations"],
-                                    "status": rv["status_name"],
-                                    "rel_residual": rv["rel_l2"]},
+        "reference_ac_vcycle_gpu": ref_cycle,
         "pressure_step_gpu": step,
         "roofline": roof,
         "cpu_baseline": cpu,
