@@ -24,6 +24,7 @@
 
 
 #include "psb_fused.h"
+#include "psb_march.h"
 #include "psb_internal.h"
 
 // FP32 levels evaluate the stencil in flux form (set to 0 for the plain form)
@@ -57,6 +58,11 @@ bool vec_ok(const Grid &g) { return g.nz % vmax<T>() == 0; }
 template <typename T>
 int nblocks(const Grid &g) { return grid_blocks(vec_ok<T>(g) ? g.N / vmax<T>() : g.N); }
 
+// stencil kernels on levels of at least kMarchMin cells march through the
+// i planes (psb_march.h); smaller levels use the span kernels
+constexpr bool kMarch = true;
+constexpr int kMarchMin = 1 << 15;
+
 // calls f(std::integral_constant<int, V>) with the vector width for g
 template <typename T, typename F>
 void with_v(const Grid &g, F f) {
@@ -80,6 +86,12 @@ template <typename T, bool SYM>
 void spmv(hipStream_t stream, const Grid &g, DMat<T> A, const T *x, T *y) {
   with_v<T>(g, [&](auto vt) {
     constexpr int V = decltype(vt)::value;
+    MarchShape m;
+    if (kMarch && V > 1 && g.N >= kMarchMin && march_ok<V>(g, m)) {
+      hipLaunchKernelGGL((k_spmv_march<T, V, SYM>), dim3(m.nblocks), dim3(kBlock), 0, stream, g,
+                         A, x, y, m.TI, m.ntiles);
+      return;
+    }
     hipLaunchKernelGGL((k_spmv<T, V, SYM>), dim3(nblocks<T>(g)), dim3(kBlock), 0, stream, g, A,
                        x, y);
   });
@@ -110,12 +122,21 @@ void start(psb_ctx *C, const Grid &g, DMat<T> A, const T *b, const T *x, T *r, T
 template <typename T, bool SYM, bool DIAG, bool FLUX = false>
 void cg_pq(psb_ctx *C, const Grid &g, DMat<T> A, const T *rz, const T *pold, T *pnew, T *q,
            const uint8_t *af, DevState *st) {
+  int nb = nblocks<T>(g);
   with_v<T>(g, [&](auto vt) {
     constexpr int V = decltype(vt)::value;
+    MarchShape m;
+    if (kMarch && V > 1 && g.N >= kMarchMin && march_ok<V>(g, m)) {
+      nb = m.nblocks;
+      hipLaunchKernelGGL((k_cg_pq_march<T, V, SYM, DIAG, FLUX>), dim3(m.nblocks), dim3(kBlock),
+                         0, C->stream, g, A, rz, pold, pnew, q, af, st, C->work, m.TI,
+                         m.ntiles);
+      return;
+    }
     hipLaunchKernelGGL((k_cg_pq<T, V, SYM, DIAG, FLUX>), dim3(nblocks<T>(g)), dim3(kBlock), 0,
                        C->stream, g, A, rz, pold, pnew, q, af, st, C->work);
   });
-  finalize<1>(C, FIN_PQ, nblocks<T>(g), st);
+  finalize<1>(C, FIN_PQ, nb, st);
 }
 
 template <typename T, bool DIAG>
@@ -140,6 +161,7 @@ template <typename T, bool RHO, typename M, bool FLUX>
 void jacobi_m(psb_ctx *C, const Level<T> &lv, DMat<M> A, const T *e, T *eout,
               const Level<T> *coarse, const T *eH, T cfac, T omega, T zscale, DevState *st) {
   const Grid &g = lv.g;
+  int nb = nblocks<T>(g);
   with_v<T>(g, [&](auto vt) {
     constexpr int V = decltype(vt)::value;
     ProlX<T, V> pr{};
@@ -153,6 +175,21 @@ void jacobi_m(psb_ctx *C, const Level<T> &lv, DMat<M> A, const T *e, T *eout,
       pr.sK = coarse->CK == 2 ? 1 : 0;
       pr.nYH = coarse->g.ny;
       pr.nZH = coarse->g.nz;
+    }
+    MarchShape m;
+    if (kMarch && V > 1 && g.N >= kMarchMin && march_ok<V>(g, m)) {
+      nb = m.nblocks;
+      if (coarse)
+        hipLaunchKernelGGL((k_jacobi_march<T, V, true, RHO, M, FLUX>), dim3(m.nblocks),
+                           dim3(kBlock), 0, C->stream, g, A, lv.r, e, eout, pr, omega, lv.aflag,
+                           st, C->work, zscale, m.TI, m.ntiles);
+      else
+        hipLaunchKernelGGL((k_jacobi_march<T, V, false, RHO, M, FLUX>), dim3(m.nblocks),
+                           dim3(kBlock), 0, C->stream, g, A, lv.r, e, eout, pr, omega, lv.aflag,
+                           st, C->work, zscale, m.TI, m.ntiles);
+      return;
+    }
+    if (coarse) {
       hipLaunchKernelGGL((k_jacobi<T, V, true, true, RHO, M, FLUX>), dim3(nblocks<T>(g)),
                          dim3(kBlock), 0, C->stream, g, A, lv.r, e, eout, pr, omega, lv.aflag,
                          st, C->work, zscale);
@@ -162,7 +199,7 @@ void jacobi_m(psb_ctx *C, const Level<T> &lv, DMat<M> A, const T *e, T *eout,
                          st, C->work, zscale);
     }
   });
-  if (RHO) finalize<4>(C, FIN_RHO, nblocks<T>(g), st);
+  if (RHO) finalize<4>(C, FIN_RHO, nb, st);
 }
 
 template <typename T, bool RHO>

@@ -1,0 +1,182 @@
+// psb_march.h -- stencil kernels that march through the i planes.
+//
+// A block owns a tile of kBlock cell vectors of one (j, k) plane and walks
+// TI consecutive planes; each thread keeps its own column's operand at
+// planes i-1, i, i+1 in registers, so the i-neighbours are loaded once
+// instead of three times and every plane of the operand streams from HBM
+// once per tile (the j and k neighbours come from the L1 / L2 copies the
+// neighbouring threads and tiles load).  The arithmetic is that of the
+// span kernels (same gather, same order): results are bit-identical.
+#pragma once
+
+#include "psb_kernels.h"
+
+namespace psb {
+
+// the operand with the thread's own column at planes i-1 / i / i+1 held in
+// registers; other points read through the base operand
+template <typename T, int V, class B>
+struct MarchX {
+  B base;
+  VecN<T, V> prev, cur, next;
+  int i, j;
+  __device__ __forceinline__ VecN<T, V> vec(int P, int ii, int jj, int kk) const {
+    if (ii < i) return prev;
+    if (ii > i) return next;
+    if (jj == j) return cur;
+    return base.vec(P, ii, jj, kk);
+  }
+  __device__ __forceinline__ T at(int Q, int ii, int jj, int kk) const {
+    return base.at(Q, ii, jj, kk);
+  }
+};
+
+// launch geometry: tiles of kBlock vectors per plane, slabs of TI planes,
+// about kMaxBlocks blocks in all
+struct MarchShape {
+  int TI, ntiles, nblocks;
+};
+template <int V>
+inline MarchShape march_shape(const Grid &g) {
+  MarchShape m;
+  const long long nvp = (long long)g.ny * (g.nz / V);
+  m.ntiles = (int)((nvp + kBlock - 1) / kBlock);
+  long long TI = ((long long)g.nx * m.ntiles + kMaxBlocks - 1) / kMaxBlocks;
+  if (TI < 1) TI = 1;
+  // the reductions keep one partial per block: at most kMaxBlocks blocks
+  while (TI < g.nx && m.ntiles * ((g.nx + TI - 1) / TI) > kMaxBlocks) TI++;
+  m.TI = (int)TI;
+  m.nblocks = m.ntiles * (int)((g.nx + TI - 1) / TI);
+  return m;
+}
+// the march applies when the vectors tile the rows and the grid fits the
+// reduction buffer (a plane of more than kMaxBlocks tiles does not)
+template <int V>
+inline bool march_ok(const Grid &g, MarchShape &m) {
+  if (g.nz % V != 0) return false;
+  m = march_shape<V>(g);
+  return m.nblocks <= kMaxBlocks;
+}
+
+// the (j, k) of this thread's vector and its slab; act_ false past the
+// plane's last vector (such threads skip the march but stay for the
+// block's reductions)
+#define PSB_MARCH_BEGIN(g, V, TI, ntiles)                                     \
+  const int wi_ = xcd_block();                                                \
+  const int slab_ = wi_ / (ntiles), tile_ = wi_ - slab_ * (ntiles);           \
+  const int nv_ = (g).nz / (V);                                               \
+  const int q_ = tile_ * kBlock + (int)threadIdx.x;                          \
+  const bool act_ = q_ < (g).ny * nv_;                                        \
+  const int j = act_ ? q_ / nv_ : 0, k = act_ ? (q_ - j * nv_) * (V) : 0;     \
+  const int i0_ = slab_ * (TI), i1_ = act_ ? min(i0_ + (TI), (g).nx) : i0_;
+
+// y = A x
+template <typename T, int V, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_spmv_march(Grid g, DMat<T> A, const T *x, T *y,
+                                                       int TI, int ntiles) {
+  PSB_MARCH_BEGIN(g, V, TI, ntiles)
+  if (!act_) return;
+  int P = (i0_ * g.ny + j) * g.nz + k;
+  VecN<T, V> prev = i0_ > 0 ? ldv<T, V>(x + P - g.sI) : zerov<T, V>();
+  VecN<T, V> cur = ldv<T, V>(x + P);
+  for (int i = i0_; i < i1_; i++, P += g.sI) {
+    const VecN<T, V> next = (i + 1 < g.nx) ? ldv<T, V>(x + P + g.sI) : zerov<T, V>();
+    MarchX<T, V, ArrX<T, V>> X{ArrX<T, V>{x}, prev, cur, next, i, j};
+    Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+    stv<T, V>(y + P, apply_A<T, V>(G));
+    prev = cur;
+    cur = next;
+  }
+}
+
+// damped Jacobi sweep e_out = e + omega (r - A e) / c, the operand e or,
+// with PROL, e + cfac P eH (the first post-sweep); RHO: the last post-sweep
+// of level 0 (output times zscale, partials r.z, sum z, sum r, active
+// cells per block).  The same arithmetic as k_jacobi.
+template <typename T, int V, bool PROL, bool RHO, typename M, bool FLUX>
+__global__ __launch_bounds__(kBlock) void k_jacobi_march(Grid g, DMat<M> A, const T *r,
+                                                         const T *e, T *eout, ProlX<T, V> pr,
+                                                         T omega, const uint8_t *af,
+                                                         const DevState *st, Work w, T zscale,
+                                                         int TI, int ntiles) {
+  if (st->done) return;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  PSB_MARCH_BEGIN(g, V, TI, ntiles)
+  auto opv = [&](int Q, int ii) {
+    if constexpr (PROL) return pr.vec(Q, ii, j, k);
+    else return ldv<T, V>(e + Q);
+  };
+  int P = (i0_ * g.ny + j) * g.nz + k;
+  VecN<T, V> prev = zerov<T, V>(), cur = zerov<T, V>();
+  if (act_) {
+    if (i0_ > 0) prev = opv(P - g.sI, i0_ - 1);
+    cur = opv(P, i0_);
+  }
+  for (int i = i0_; i < i1_; i++, P += g.sI) {
+    const VecN<T, V> next = (i + 1 < g.nx) ? opv(P + g.sI, i + 1) : zerov<T, V>();
+    Gath<T, V> G;
+    if constexpr (PROL) {
+      MarchX<T, V, ProlX<T, V>> X{pr, prev, cur, next, i, j};
+      G = gather<T, V, true, FLUX>(g, A, X, P, i, j, k);
+    } else {
+      MarchX<T, V, ArrX<T, V>> X{ArrX<T, V>{e}, prev, cur, next, i, j};
+      G = gather<T, V, true, FLUX>(g, A, X, P, i, j, k);
+    }
+    const VecN<T, V> R = ldv<T, V>(r + P);
+    const VecN<T, V> t = resid<T, V>(R, G);
+    VecN<T, V> E;
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+      E.v[u] = G.x.v[u] + omega * t.v[u] / G.c.v[u];
+      if (RHO) E.v[u] = zscale * E.v[u];
+      if (RHO && (!af || af[P + u])) {
+        acc[0] += (double)R.v[u] * (double)E.v[u];
+        acc[1] += (double)E.v[u];
+        acc[2] += (double)R.v[u];
+        acc[3] += 1.0;
+      }
+    }
+    stv<T, V>(eout + P, E);
+    prev = cur;
+    cur = next;
+  }
+  if (RHO) block_partial<4>(acc, w.partial);
+}
+
+// CG step 1 (cg.cpp:75-108): p = z + beta p_old with z = r/c (DIAG) or the
+// projected preconditioner output, q = A p, partial p.q -- as k_cg_pq
+template <typename T, int V, bool SYM, bool DIAG, bool FLUX>
+__global__ __launch_bounds__(kBlock) void k_cg_pq_march(Grid g, DMat<T> A, const T *r_or_z,
+                                                        const T *pold, T *pnew, T *q,
+                                                        const uint8_t *af, DevState *st,
+                                                        Work w, int TI, int ntiles) {
+  if (st->done) return;
+  const T beta = (T)st->beta;
+  double acc[1] = {0.0};
+  PSB_MARCH_BEGIN(g, V, TI, ntiles)
+  using B = typename std::conditional<DIAG, DiagPX<T, V>, ZPX<T, V>>::type;
+  B base;
+  if constexpr (DIAG) base = DiagPX<T, V>{r_or_z, A.c, pold, beta};
+  else base = ZPX<T, V>{r_or_z, pold, af, beta, (T)st->zmean};
+  int P = (i0_ * g.ny + j) * g.nz + k;
+  VecN<T, V> prev = zerov<T, V>(), cur = zerov<T, V>();
+  if (act_) {
+    if (i0_ > 0) prev = base.vec(P - g.sI, i0_ - 1, j, k);
+    cur = base.vec(P, i0_, j, k);
+  }
+  for (int i = i0_; i < i1_; i++, P += g.sI) {
+    const VecN<T, V> next = (i + 1 < g.nx) ? base.vec(P + g.sI, i + 1, j, k) : zerov<T, V>();
+    MarchX<T, V, B> X{base, prev, cur, next, i, j};
+    Gath<T, V> G = gather<T, V, SYM, FLUX>(g, A, X, P, i, j, k);
+    VecN<T, V> Q = apply_A<T, V>(G);
+    stv<T, V>(pnew + P, G.x);
+    stv<T, V>(q + P, Q);
+#pragma unroll
+    for (int u = 0; u < V; u++) acc[0] += (double)(G.x.v[u] * Q.v[u]);
+    prev = cur;
+    cur = next;
+  }
+  block_partial<1>(acc, w.partial);
+}
+
+}  // namespace psb
