@@ -61,6 +61,11 @@ int nblocks(const Grid &g) { return grid_blocks(vec_ok<T>(g) ? g.N / vmax<T>() :
 // stencil kernels on levels of at least kMarchMin cells march through the
 // i planes (psb_march.h); smaller levels use the span kernels
 constexpr bool kMarch = true;
+// the first post-sweep (prolongation fused in) marches in FP64; in FP32 with
+// the FP16 matrix the span kernel measured faster (28.0 against 28.5 ms per
+// 256^3 solve; FP64: 56.4 marching against 59.0 ms)
+template <typename T>
+constexpr bool march_prol() { return std::is_same<T, double>::value; }
 constexpr int kMarchMin = 1 << 15;
 
 // calls f(std::integral_constant<int, V>) with the vector width for g
@@ -177,7 +182,7 @@ void jacobi_m(psb_ctx *C, const Level<T> &lv, DMat<M> A, const T *e, T *eout,
       pr.nZH = coarse->g.nz;
     }
     MarchShape m;
-    if (kMarch && V > 1 && g.N >= kMarchMin && march_ok<V>(g, m)) {
+    if (kMarch && V > 1 && g.N >= kMarchMin && (!coarse || march_prol<T>()) && march_ok<V>(g, m)) {
       nb = m.nblocks;
       if (coarse)
         hipLaunchKernelGGL((k_jacobi_march<T, V, true, RHO, M, FLUX>), dim3(m.nblocks),
