@@ -126,7 +126,7 @@ def measured_traffic(n):
     path = os.path.join(ROOT, "profiles", "jacobi_spmv_256_hbm_traffic.md")
     if n != 256 or not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch k_jacobi<float,4,fp16,flux> (\d+)",
+    m = re.search(r"traffic_bytes_per_launch k_jacobi_march<float,4,fp16,flux> (\d+)",
                   open(path).read())
     return int(m.group(1)) if m else None
 
@@ -413,8 +413,8 @@ def main():
     ms_s32, by_s32 = s.time_kernel(1, reps=50)
     ms_s64, by_s64 = s.time_kernel(0, reps=50)
     gbs = lambda by, ms: by / (ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": "k_jacobi<float,4,...,_Float16,flux> (fine-level "
-                                      "damped Jacobi sweep, FP16 matrix copy)",
+    roof = {"bound": "hbm", "kernel": "k_jacobi_march<float,4,...,_Float16,flux> (fine-level "
+                                      "damped Jacobi sweep, FP16 matrix copy, marching)",
             "achieved": round(gbs(by_j, ms_j), 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs(by_j, ms_j) / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n),
             "avg_launch_ms": ms_j, "bytes_per_launch": by_j,

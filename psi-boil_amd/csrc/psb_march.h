@@ -36,12 +36,19 @@ struct MarchX {
 struct MarchShape {
   int TI, ntiles, nblocks;
 };
+#ifndef PSB_MARCH_BLOCKS
+#define PSB_MARCH_BLOCKS 1024
+#endif
+// target grid (<= kMaxBlocks): 1024 blocks, i.e. slabs of 16 planes at
+// 256^3, measured best against 512 and 2048 (27.1 / 29.4 / 27.8 ms per mixed
+// solve, 55.0 / 56.3 / 56.3 ms per FP64 solve)
+constexpr int kMarchBlocks = PSB_MARCH_BLOCKS;
 template <int V>
 inline MarchShape march_shape(const Grid &g) {
   MarchShape m;
   const long long nvp = (long long)g.ny * (g.nz / V);
   m.ntiles = (int)((nvp + kBlock - 1) / kBlock);
-  long long TI = ((long long)g.nx * m.ntiles + kMaxBlocks - 1) / kMaxBlocks;
+  long long TI = ((long long)g.nx * m.ntiles + kMarchBlocks - 1) / kMarchBlocks;
   if (TI < 1) TI = 1;
   // the reductions keep one partial per block: at most kMaxBlocks blocks
   while (TI < g.nx && m.ntiles * ((g.nx + TI - 1) / TI) > kMaxBlocks) TI++;
