@@ -277,33 +277,44 @@ bool post2(psb_ctx *C, const Level<T> &lv, const T *e, T *eout, const Level<T> &
   }
 }
 
+template <typename T, int V, int KV, int CI, int CJ, int CK, typename M, bool FLUX>
+void restrict_launch(psb_ctx *C, const Level<T> &f, DMat<M> A, const T *e, Level<T> &cL,
+                     T omega, DevState *st) {
+  if constexpr (CI == 2 && KV == 1 && V > 1) {
+    if (kMarch && f.g.N >= kMarchMin && f.g.nx == 2 * cL.g.nx) {
+      const MarchShape m = march_shape_n(cL.g.nx, (long long)cL.g.ny * (f.g.nz / V));
+      if (m.nblocks <= kMaxBlocks) {
+        hipLaunchKernelGGL((k_restrict_march<T, V, true, CJ, CK, M, FLUX>), dim3(m.nblocks),
+                           dim3(kBlock), 0, C->stream, f.g, A, f.r, e, f.aflag, cL.g, cL.c,
+                           cL.aflag, cL.r, cL.e[0], omega, st, m.TI, m.ntiles);
+        return;
+      }
+    }
+  }
+  const int nwork = cL.g.nx * cL.g.ny * (f.g.nz / (V * KV));
+  hipLaunchKernelGGL((k_restrict<T, V, KV, true, CI, CJ, CK, M, FLUX>), dim3(grid_blocks(nwork)),
+                     dim3(kBlock), 0, C->stream, f.g, A, f.r, e, f.aflag, cL.g, cL.c, cL.aflag,
+                     cL.r, cL.e[0], omega, st);
+}
+
 template <typename T, int V, int KV, int CI, int CJ, int CK>
 void restrict_v(psb_ctx *C, const Level<T> &f, const T *e, Level<T> &cL, T omega,
                 DevState *st) {
-  const int nwork = cL.g.nx * cL.g.ny * (f.g.nz / (V * KV));
   if constexpr (std::is_same<T, float>::value) {
     if (f.c16 && f.d) {
-      hipLaunchKernelGGL((k_restrict<T, V, KV, true, CI, CJ, CK, _Float16, true>),
-                         dim3(grid_blocks(nwork)), dim3(kBlock), 0, C->stream, f.g, f.mat16(),
-                         f.r, e, f.aflag, cL.g, cL.c, cL.aflag, cL.r, cL.e[0], omega, st);
+      restrict_launch<T, V, KV, CI, CJ, CK, _Float16, true>(C, f, f.mat16(), e, cL, omega, st);
       return;
     }
     if (f.c16) {
-      hipLaunchKernelGGL((k_restrict<T, V, KV, true, CI, CJ, CK, _Float16>),
-                         dim3(grid_blocks(nwork)), dim3(kBlock), 0, C->stream, f.g, f.mat16(),
-                         f.r, e, f.aflag, cL.g, cL.c, cL.aflag, cL.r, cL.e[0], omega, st);
+      restrict_launch<T, V, KV, CI, CJ, CK, _Float16, false>(C, f, f.mat16(), e, cL, omega, st);
       return;
     }
     if (f.d) {
-      hipLaunchKernelGGL((k_restrict<T, V, KV, true, CI, CJ, CK, T, true>),
-                         dim3(grid_blocks(nwork)), dim3(kBlock), 0, C->stream, f.g, f.matflux(),
-                         f.r, e, f.aflag, cL.g, cL.c, cL.aflag, cL.r, cL.e[0], omega, st);
+      restrict_launch<T, V, KV, CI, CJ, CK, T, true>(C, f, f.matflux(), e, cL, omega, st);
       return;
     }
   }
-  hipLaunchKernelGGL((k_restrict<T, V, KV, true, CI, CJ, CK>), dim3(grid_blocks(nwork)),
-                     dim3(kBlock), 0, C->stream, f.g, f.mat(), f.r, e, f.aflag, cL.g, cL.c,
-                     cL.aflag, cL.r, cL.e[0], omega, st);
+  restrict_launch<T, V, KV, CI, CJ, CK, T, false>(C, f, f.mat(), e, cL, omega, st);
 }
 
 template <typename T, int V, int CI, int CJ>

@@ -43,18 +43,21 @@ struct MarchShape {
 // 256^3, measured best against 512 and 2048 (27.1 / 29.4 / 27.8 ms per mixed
 // solve, 55.0 / 56.3 / 56.3 ms per FP64 solve)
 constexpr int kMarchBlocks = PSB_MARCH_BLOCKS;
-template <int V>
-inline MarchShape march_shape(const Grid &g) {
+// nplanes planes of nvp thread items each
+inline MarchShape march_shape_n(int nplanes, long long nvp) {
   MarchShape m;
-  const long long nvp = (long long)g.ny * (g.nz / V);
   m.ntiles = (int)((nvp + kBlock - 1) / kBlock);
-  long long TI = ((long long)g.nx * m.ntiles + kMarchBlocks - 1) / kMarchBlocks;
+  long long TI = ((long long)nplanes * m.ntiles + kMarchBlocks - 1) / kMarchBlocks;
   if (TI < 1) TI = 1;
   // the reductions keep one partial per block: at most kMaxBlocks blocks
-  while (TI < g.nx && m.ntiles * ((g.nx + TI - 1) / TI) > kMaxBlocks) TI++;
+  while (TI < nplanes && m.ntiles * ((nplanes + TI - 1) / TI) > kMaxBlocks) TI++;
   m.TI = (int)TI;
-  m.nblocks = m.ntiles * (int)((g.nx + TI - 1) / TI);
+  m.nblocks = m.ntiles * (int)((nplanes + TI - 1) / TI);
   return m;
+}
+template <int V>
+inline MarchShape march_shape(const Grid &g) {
+  return march_shape_n(g.nx, (long long)g.ny * (g.nz / V));
 }
 // the march applies when the vectors tile the rows and the grid fits the
 // reduction buffer (a plane of more than kMaxBlocks tiles does not)
@@ -184,6 +187,84 @@ __global__ __launch_bounds__(kBlock) void k_cg_pq_march(Grid g, DMat<T> A, const
     cur = next;
   }
   block_partial<1>(acc, w.partial);
+}
+
+// restriction (as k_restrict, CI = 2, one vector of V fine cells along k per
+// thread and row): a thread owns a coarse (J, k-vector) column and walks a
+// slab of coarse planes; per coarse plane it needs the operand at fine
+// planes 2I-1 .. 2I+2 of its CJ rows, of which 2I-1 and 2I are the previous
+// step's 2I+1 and 2I+2.  The sums run in the same order as k_restrict.
+template <typename T, int V, bool SYM, int CJ, int CK, typename M, bool FLUX>
+__global__ __launch_bounds__(kBlock) void k_restrict_march(Grid g, DMat<M> A, const T *r,
+                                                           const T *e, const uint8_t *af,
+                                                           Grid gH, const T *cH,
+                                                           const uint8_t *afH, T *rH, T *eH1,
+                                                           T omega, DevState *st, int TI,
+                                                           int ntiles) {
+  if (st->done) return;
+  constexpr int NC = V / CK;  // coarse cells per thread
+  const int wi = xcd_block();
+  const int slab = wi / ntiles, tile = wi - slab * ntiles;
+  const int nkv = g.nz / V;
+  const int q = tile * kBlock + (int)threadIdx.x;
+  if (q >= gH.ny * nkv) return;
+  const int J = q / nkv, kv = q - J * nkv;
+  const int k = kv * V;
+  const int I0 = slab * TI, I1 = min(I0 + TI, gH.nx);
+  ArrX<T, V> X{e};
+  VecN<T, V> ca[CJ], cb[CJ];  // operand at fine planes 2I-1 and 2I, per row
+#pragma unroll
+  for (int bb = 0; bb < CJ; bb++) {
+    const int j = J * CJ + bb;
+    const int P = (2 * I0 * g.ny + j) * g.nz + k;
+    ca[bb] = I0 > 0 ? ldv<T, V>(e + P - g.sI) : zerov<T, V>();
+    cb[bb] = ldv<T, V>(e + P);
+  }
+  for (int I = I0; I < I1; I++) {
+    VecN<T, V> cc[CJ], cd[CJ];  // fine planes 2I+1 and 2I+2
+#pragma unroll
+    for (int bb = 0; bb < CJ; bb++) {
+      const int j = J * CJ + bb;
+      const int P = ((2 * I + 1) * g.ny + j) * g.nz + k;
+      cc[bb] = ldv<T, V>(e + P);
+      cd[bb] = (2 * I + 2 < g.nx) ? ldv<T, V>(e + P + g.sI) : zerov<T, V>();
+    }
+    T s[NC];
+#pragma unroll
+    for (int u = 0; u < NC; u++) s[u] = T(0);
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+#pragma unroll
+      for (int bb = 0; bb < CJ; bb++) {
+        const int i = 2 * I + a, j = J * CJ + bb;
+        const int P = (i * g.ny + j) * g.nz + k;
+        MarchX<T, V, ArrX<T, V>> Xm{X, a == 0 ? ca[bb] : cb[bb], a == 0 ? cb[bb] : cc[bb],
+                                    a == 0 ? cc[bb] : cd[bb], i, j};
+        Gath<T, V> G = gather<T, V, SYM, FLUX>(g, A, Xm, P, i, j, k);
+        VecN<T, V> R = resid<T, V>(ldv<T, V>(r + P), G);
+        T t[V];
+#pragma unroll
+        for (int u = 0; u < V; u++) t[u] = (!af || af[P + u]) ? R.v[u] : T(0);
+#pragma unroll
+        for (int u = 0; u < NC; u++)
+#pragma unroll
+          for (int c2 = 0; c2 < CK; c2++) s[u] += t[u * CK + c2];
+      }
+    }
+    const int Q0 = (I * gH.ny + J) * gH.nz + kv * NC;
+#pragma unroll
+    for (int u = 0; u < NC; u++) {
+      const bool act = !afH || afH[Q0 + u];
+      const T v = act ? s[u] : T(0);
+      rH[Q0 + u] = v;
+      eH1[Q0 + u] = omega * v / cH[Q0 + u];
+    }
+#pragma unroll
+    for (int bb = 0; bb < CJ; bb++) {
+      ca[bb] = cc[bb];
+      cb[bb] = cd[bb];
+    }
+  }
 }
 
 }  // namespace psb
