@@ -360,9 +360,9 @@ def main():
                          "and a plain copy, for hardware-counter runs")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
-              "(the solver has no multi-GPU path: it would shard by 3-D blocks with a "
-              "one-cell halo exchange and all-reduced scalars)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the solver has "
+              "no multi-GPU path (it would shard by 3-D blocks with a one-cell halo exchange "
+              "and all-reduced scalars)", file=sys.stderr)
         sys.exit(2)
     if a.case == "batch":
         batch_case(a)
