@@ -102,11 +102,14 @@ typedef struct {
   int ac_coarsen;        /* AC levels: 0 halve every direction that stays
                             >= 8 (default); 1 halve x and y first, z only
                             once x and y stop (grids stretched in z)       */
-  int ac_store;          /* PSB_MIXED smoother matrix: 0 FP32 (default);
+  int ac_store;          /* PSB_MIXED level matrices: 0 FP32 (default);
                             1 FP16 copy scaled by a power of two per level
-                            (half the smoother's matrix bytes; the CG
+                            for the smoother (half its matrix bytes; the CG
                             operator and the residual updates keep FP32 /
-                            FP64)                                          */
+                            FP64); 2 the same copy also for the FP32 CG
+                            operator on level 0 (its q = A p reads 8 instead
+                            of 16 B per cell; the FP64 reliable updates
+                            still measure the true residual)                */
   int ac_fuse;           /* PSB_MIXED: 1 runs the two post-sweeps of a level
                             in one pass (2.5-D marching tile) where the
                             level's shape allows; 0 one kernel per sweep

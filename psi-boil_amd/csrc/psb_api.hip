@@ -711,6 +711,25 @@ T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega
   return lv.e[cur];
 }
 
+// the CG p/q step on the level-0 FP16 copy (psb_stop.ac_store = 2): q =
+// (scale A16) p / scale with FP32 arithmetic; false where it does not apply
+template <typename T>
+bool cg_pq16(psb_ctx *C, const Level<T> &f, const T *z, const T *pold, T *pnew, T *q,
+             DevState *st) {
+  if constexpr (std::is_same<T, float>::value) {
+    MarchShape m;
+    if (!C->cg_half || !f.c16 || !f.d || !vec_ok<float>(f.g) || !march_ok<4>(f.g, m))
+      return false;
+    hipLaunchKernelGGL((k_cg_pq_march<float, 4, true, false, true, _Float16>), dim3(m.nblocks),
+                       dim3(kBlock), 0, C->stream, f.g, f.mat16(), z, pold, pnew, q, f.aflag, st,
+                       C->work, m.TI, m.ntiles, (float)(1.0 / f.scale));
+    finalize<1>(C, FIN_PQ, m.nblocks, st);
+    return true;
+  } else {
+    return false;
+  }
+}
+
 // one preconditioned CG iteration: V-cycle (with rho), p/q step, x/r step
 template <typename T>
 void ac_iteration(psb_ctx *C, Hier<T> &H, KrylovVecs<T> &K, DevState *st, int parity, int nu1,
@@ -718,10 +737,12 @@ void ac_iteration(psb_ctx *C, Hier<T> &H, KrylovVecs<T> &K, DevState *st, int pa
   Level<T> &f = H.L[0];
   T *z = vcycle<T>(C, H, 0, st, nu1, nu2, omega, cfac);
   T *pold = K.p[parity], *pnew = K.p[parity ^ 1];
-  if (f.d)
-    cg_pq<T, true, false, true>(C, f.g, f.matflux(), z, pold, pnew, K.q, f.aflag, st);
-  else
-    cg_pq<T, true, false>(C, f.g, f.mat(), z, pold, pnew, K.q, f.aflag, st);
+  if (!cg_pq16<T>(C, f, z, pold, pnew, K.q, st)) {
+    if (f.d)
+      cg_pq<T, true, false, true>(C, f.g, f.matflux(), z, pold, pnew, K.q, f.aflag, st);
+    else
+      cg_pq<T, true, false>(C, f.g, f.mat(), z, pold, pnew, K.q, f.aflag, st);
+  }
   cg_xr<T, false>(C, f.g, f.c, pnew, K.q, K.x, K.r, f.e[0], T(omega / f.scale), st, xr_op);
 }
 
@@ -1470,9 +1491,14 @@ int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_
   if (p->precond == PSB_PREC_AC) {
     if (p->method != PSB_CG || !C->sym || p->ac_pre < 1 || p->ac_post < 1) return PSB_EINVAL;
     if (p->ac_coarsen < 0 || p->ac_coarsen > 1) return PSB_EINVAL;
-    if (p->ac_store < 0 || p->ac_store > 1) return PSB_EINVAL;
+    if (p->ac_store < 0 || p->ac_store > 2) return PSB_EINVAL;
     // the storage choice concerns only the FP32 levels: FP64 solves keep it
-    ensure_rule(C, p->ac_coarsen, p->precision == PSB_MIXED ? p->ac_store == 1 : C->store_half);
+    ensure_rule(C, p->ac_coarsen, p->precision == PSB_MIXED ? p->ac_store >= 1 : C->store_half);
+    const bool cg16 = p->precision == PSB_MIXED && p->ac_store == 2;
+    if (cg16 != C->cg_half) {  // the captured graphs hold the other p/q kernel
+      destroy_graphs(C);
+      C->cg_half = cg16;
+    }
     if ((p->ac_fuse != 0) != C->fuse) {  // the captured graphs hold the other kernels
       destroy_graphs(C);
       C->fuse = p->ac_fuse != 0;

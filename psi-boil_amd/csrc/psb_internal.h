@@ -220,6 +220,7 @@ struct psb_ctx {
   int coarse_rule = 0; // 0 all directions at once, 1 x and y before z
   bool store_half = false;  // FP16 copies of the FP32 levels for the smoother
   bool fuse = false;        // two post-sweeps in one pass (k_post2) where it applies
+  bool cg_half = false;     // the FP32 CG operator reads the level-0 FP16 copy too
   int version = 0;     // bumped by every change of matrix, geometry or off mask
   psb::host::RefHier href;             // the reference's AC cycle
   psb::DevMetrics metrics{};           // metrics of an assembled matrix (device)

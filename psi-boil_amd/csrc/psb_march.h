@@ -155,18 +155,22 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_march(Grid g, DMat<M> A, cons
 
 // CG step 1 (cg.cpp:75-108): p = z + beta p_old with z = r/c (DIAG) or the
 // projected preconditioner output, q = A p, partial p.q -- as k_cg_pq
-template <typename T, int V, bool SYM, bool DIAG, bool FLUX>
-__global__ __launch_bounds__(kBlock) void k_cg_pq_march(Grid g, DMat<T> A, const T *r_or_z,
+// M = _Float16: the level's FP16 copy scale * (d, w, s, b) (flux form, not
+// with DIAG); q is the product times qscale = 1 / scale (a power of two)
+template <typename T, int V, bool SYM, bool DIAG, bool FLUX, typename M = T>
+__global__ __launch_bounds__(kBlock) void k_cg_pq_march(Grid g, DMat<M> A, const T *r_or_z,
                                                         const T *pold, T *pnew, T *q,
                                                         const uint8_t *af, DevState *st,
-                                                        Work w, int TI, int ntiles) {
+                                                        Work w, int TI, int ntiles,
+                                                        T qscale = T(1)) {
+  static_assert(!DIAG || std::is_same<T, M>::value, "the diagonal operand divides by A.c");
   if (st->done) return;
   const T beta = (T)st->beta;
   double acc[1] = {0.0};
   PSB_MARCH_BEGIN(g, V, TI, ntiles)
   using B = typename std::conditional<DIAG, DiagPX<T, V>, ZPX<T, V>>::type;
   B base;
-  if constexpr (DIAG) base = DiagPX<T, V>{r_or_z, A.c, pold, beta};
+  if constexpr (DIAG) base = DiagPX<T, V>{r_or_z, (const T *)A.c, pold, beta};
   else base = ZPX<T, V>{r_or_z, pold, af, beta, (T)st->zmean};
   int P = (i0_ * g.ny + j) * g.nz + k;
   VecN<T, V> prev = zerov<T, V>(), cur = zerov<T, V>();
@@ -179,6 +183,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_pq_march(Grid g, DMat<T> A, const
     MarchX<T, V, B> X{base, prev, cur, next, i, j};
     Gath<T, V> G = gather<T, V, SYM, FLUX>(g, A, X, P, i, j, k);
     VecN<T, V> Q = apply_A<T, V>(G);
+    if constexpr (!std::is_same<T, M>::value) {
+#pragma unroll
+      for (int u = 0; u < V; u++) Q.v[u] = Q.v[u] * qscale;
+    }
     stv<T, V>(pnew + P, G.x);
     stv<T, V>(q + P, Q);
 #pragma unroll

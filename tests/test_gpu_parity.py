@@ -406,3 +406,25 @@ def test_fused_post_smoothing_gives_the_same_solve(store):
     assert abs(r1["iterations"] - r0["iterations"]) <= 1
     assert r1["rel_l2"] <= 1e-9 * 1.01
     assert rel(x1 - x1.mean(), x0 - x0.mean()) < 1e-7
+
+
+@pytest.mark.parametrize("ac_store", [1, 2])
+def test_fp16_level_copies_keep_the_iterations(ac_store):
+    """FP16 copies of the level matrices for the smoother (ac_store = 1) and
+    also for the FP32 CG operator (2): the reliable updates still measure the
+    true FP64 residual, which meets the tolerance, in about the iterations of
+    the FP32 copies (the same solve with ac_store = 0)."""
+    n = 48
+    d = psb.generate(n, n, n, 6, 0.05, 0.15, 1.0, 1000.0, 41, 1041)
+    s = psb.Solver(n, n, n)
+    s.set_matrix(d)
+    s.set_geometry(d["dxc"], d["dyc"], d["dzc"])
+    b = d["rhs"]
+    kw = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=1e-9, res_tol=0.0,
+              max_iter=300, precision=psb.PSB_MIXED, inner_adapt=2, inner_rtol=1e-2)
+    x0, r0 = s.solve(np.zeros(b.size), b, ac_store=0, **kw)
+    x1, r1 = s.solve(np.zeros(b.size), b, ac_store=ac_store, **kw)
+    assert r0["converged"] and r1["converged"]
+    assert np.linalg.norm(s.apply(x1, b)) <= 1e-9 * np.linalg.norm(b) * 1.0001
+    assert abs(r1["iterations"] - r0["iterations"]) <= 3
+    assert rel(x1 - x1.mean(), x0 - x0.mean()) < 1e-6
