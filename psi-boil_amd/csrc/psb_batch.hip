@@ -57,7 +57,8 @@ struct psb_batch {
   double *partial = nullptr;
   bool built = false;
   int tune = 0;               // 0: autotune, > 0: fixed vectors per thread
-  int V = 1, V2 = 1;          // FP32 / FP64 vector widths (common to all systems)
+  int V = 1, V2 = 1;          // FP32 / FP64 vector widths of the fine level (all systems)
+  std::vector<int> Vl;        // FP32 vector width per level (4 where every system's nz allows)
   int nlev = 0;
   std::vector<int> last;      // coarsest level per system
   std::vector<BFine> fine;
@@ -138,8 +139,8 @@ void build_lists(psb_batch *B) {
     for (int s = 0; s < ns; s++) {
       if (l < B->last[s]) {
         const BLev<float> &L = B->lev[l][s], &H = B->lev[l + 1][s];
-        if (l > 0) B->smoothL[l].segs.push_back(make_seg(s, nvec(L.g, B->V), B->vpt[l][s]));
-        const int KE = B->V == 1 ? (H.sK ? 2 : 1) : B->V;
+        if (l > 0) B->smoothL[l].segs.push_back(make_seg(s, nvec(L.g, B->Vl[l]), B->vpt[l][s]));
+        const int KE = B->Vl[l] == 1 ? (H.sK ? 2 : 1) : B->Vl[l];
         const long long nw = (long long)H.g.nx * H.g.ny * (L.g.nz / KE);
         groups[combo_of(H)].segs.push_back(make_seg(s, nw, B->vpt[l][s]));
       } else if (l == B->last[s]) {
@@ -159,6 +160,31 @@ void build_lists(psb_batch *B) {
   B->partial = B->arena.alloc<double>(need);
 }
 
+// levels of at least kMarchMin cells with 4-wide vectors march (the Jacobi
+// sweeps, and on level 0 the CG p/q step): slabs of TI planes sized so that
+// the system's segment holds about one tile-slab per block
+constexpr int kBatchMarchMin = 1 << 15;
+
+void march_tables(psb_batch *B) {
+  for (int l = 0; l < B->nlev; l++) {
+    const SegList &L = l == 0 ? B->fineL : B->smoothL[l];
+    for (const BSeg &sg : L.segs) {
+      BLev<float> &D = B->lev[l][sg.sys];
+      D.TI = D.ntiles = 0;
+      if (B->Vl[l] != 4 || D.g.N < kBatchMarchMin || D.g.nz % 4 != 0) continue;
+      const long long nvp = (long long)D.g.ny * (D.g.nz / 4);
+      const int ntiles = (int)((nvp + kBlock - 1) / kBlock);
+      int TI = (int)(((long long)D.g.nx * ntiles + sg.nb - 1) / sg.nb);
+      if (TI < 1) TI = 1;
+      D.TI = TI;
+      D.ntiles = ntiles;
+    }
+    HIPCHK(hipMemcpyAsync(B->d_lev[l], B->lev[l].data(), sizeof(BLev<float>) * B->nsys,
+                          hipMemcpyHostToDevice, B->stream));
+  }
+  HIPCHK(hipStreamSynchronize(B->stream));
+}
+
 // ---------------------------------------------------------------------------
 // launchers over a list
 // ---------------------------------------------------------------------------
@@ -169,7 +195,7 @@ void l_jacobi(psb_batch *B, int l, int cur, int nxt, int outH) {
   const Work w{B->partial};
   const BLev<float> *lh = l + 1 < B->nlev ? B->d_lev[l + 1] : nullptr;
   const float cfac = (float)B->cfac, omega = (float)B->omega;
-  if (B->V == 4)
+  if (B->Vl[l] == 4)
     hipLaunchKernelGGL((kb_jacobi<float, 4, PROL, RHO>), dim3(L.nblocks), dim3(kBlock), 0,
                        B->stream, L.d_segs, L.d_bmap, B->d_fine, B->d_lev[l], lh, cur, nxt, outH,
                        cfac, omega, w);
@@ -188,7 +214,7 @@ void l_restrict_v(psb_batch *B, const SegList &L, int l, int cur) {
 
 template <int CI, int CJ, int CK>
 void l_restrict_c(psb_batch *B, const SegList &L, int l, int cur) {
-  if (B->V == 4) {
+  if (B->Vl[l] == 4) {
     l_restrict_v<4, 1, CI, CJ, CK>(B, L, l, cur);
   } else {
     if constexpr (CK == 2) l_restrict_v<1, 2, CI, CJ, CK>(B, L, l, cur);
@@ -312,14 +338,14 @@ bool stale(psb_batch *B) {
 // time the level-l Jacobi sweep of the systems in `who` with `vpt`
 double time_class(psb_batch *B, int l, const std::vector<int> &who, int vpt) {
   SegList L;
-  for (int s : who) L.segs.push_back(make_seg(s, nvec(B->lev[l][s].g, B->V), vpt));
+  for (int s : who) L.segs.push_back(make_seg(s, nvec(B->lev[l][s].g, B->Vl[l]), vpt));
   upload_list(B, L);
   const BLev<float> *lv = B->d_lev[l];
   const float omega = 0.9f;
   auto launch = [&]() {
     ProlX<float, 4> dummy{};
     (void)dummy;
-    if (B->V == 4)
+    if (B->Vl[l] == 4)
       hipLaunchKernelGGL((kb_jacobi<float, 4, false, false>), dim3(L.nblocks), dim3(kBlock), 0,
                          B->stream, L.d_segs, L.d_bmap, B->d_fine, lv, lv, 0, 1, 0, 0.0f, omega,
                          Work{B->partial});
@@ -390,15 +416,20 @@ void build(psb_batch *B) {
   }
   B->nlev = 0;
   B->last.assign(ns, 0);
-  bool v4 = true, v2 = true;
+  bool v2 = true;
   for (int s = 0; s < ns; s++) {
     const Hier<float> &H = B->sys[s]->h32;
     B->last[s] = (int)H.L.size() - 1;
     B->nlev = std::max(B->nlev, (int)H.L.size());
-    for (const Level<float> &L : H.L) v4 = v4 && L.g.nz % 4 == 0;
     v2 = v2 && B->sys[s]->g.nz % 2 == 0;
   }
-  B->V = v4 ? 4 : 1;
+  B->Vl.assign(B->nlev, 4);
+  for (int s = 0; s < ns; s++) {
+    const Hier<float> &H = B->sys[s]->h32;
+    for (int l = 0; l < (int)H.L.size(); l++)
+      if (H.L[l].g.nz % 4 != 0) B->Vl[l] = 1;
+  }
+  B->V = B->Vl[0];
   B->V2 = v2 ? 2 : 1;
   // tables
   B->lev.assign(B->nlev, std::vector<BLev<float>>(ns));
@@ -449,6 +480,7 @@ void build(psb_batch *B) {
   HIPCHK(hipStreamSynchronize(B->stream));
   if (B->tune == 0) autotune(B);
   build_lists(B);
+  march_tables(B);
   HIPCHK(hipStreamSynchronize(B->stream));
   for (int s = 0; s < ns; s++) B->seen[s] = B->sys[s]->version;
   B->built = true;

@@ -7,6 +7,7 @@
 #pragma once
 
 #include "psb_kernels.h"
+#include "psb_march.h"
 
 namespace psb {
 
@@ -22,6 +23,8 @@ struct BLev {
   int sI, sJ, sK;     // coarsening shift (0/1) per direction from the finer level
   int m;              // coarsest level: dense operator size (0 otherwise)
   const double *G;    // coarsest level: dense projected pseudo-inverse
+  int TI, ntiles;     // TI > 0: the Jacobi sweeps (and on level 0 the CG p/q
+                      // step) march, slabs of TI planes, ntiles tiles a plane
   __device__ __forceinline__ DMat<T> mat() const {
     return DMat<T>{c, w, s, b, nullptr, nullptr, nullptr};
   }
@@ -71,6 +74,14 @@ __global__ __launch_bounds__(kBlock) void kb_jacobi(const BSeg *segs, const uint
     pr.nYH = H.g.ny;
     pr.nZH = H.g.nz;
   }
+  if constexpr (!PROL && V > 1) {
+    if (L.TI > 0) {
+      jacobi_march_body<T, V, false, RHO, T, true>(sp, L.g, L.matflux(), L.r, L.e[cur],
+                                                   L.e[nxt], pr, omega, L.af, w, T(1), L.TI,
+                                                   L.ntiles);
+      return;
+    }
+  }
   jacobi_body<T, V, true, PROL, RHO, T, true>(sp, L.g, L.matflux(), L.r, L.e[cur], L.e[nxt], pr,
                                               omega, L.af, w);
 }
@@ -108,6 +119,13 @@ __global__ __launch_bounds__(kBlock) void kb_cg_pq(const BSeg *segs, const uint1
   PSB_BSEG_BEGIN
   if (F.st->done) return;
   const BLev<float> &L = lev0[sg_.sys];
+  if constexpr (V > 1) {
+    if (L.TI > 0) {
+      cg_pq_march_body<float, V, true, true>(sp, F.g, L.matflux(), L.e[zi], F.p[par],
+                                             F.p[par ^ 1], F.q, L.af, F.st, w, L.TI, L.ntiles);
+      return;
+    }
+  }
   cg_pq_body<float, V, true, false, true>(sp, F.g, L.matflux(), L.e[zi], F.p[par], F.p[par ^ 1],
                                           F.q, L.af, F.st, w);
 }

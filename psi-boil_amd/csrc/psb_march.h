@@ -103,26 +103,25 @@ __global__ __launch_bounds__(kBlock) void k_spmv_march(Grid g, DMat<T> A, const 
 // with PROL, e + cfac P eH (the first post-sweep); RHO: the last post-sweep
 // of level 0 (output times zscale, partials r.z, sum z, sum r, active
 // cells per block).  The same arithmetic as k_jacobi.
+// one item: tile `tile` of slab `slab` (TI planes)
 template <typename T, int V, bool PROL, bool RHO, typename M, bool FLUX>
-__global__ __launch_bounds__(kBlock) void k_jacobi_march(Grid g, DMat<M> A, const T *r,
-                                                         const T *e, T *eout, ProlX<T, V> pr,
-                                                         T omega, const uint8_t *af,
-                                                         const DevState *st, Work w, T zscale,
-                                                         int TI, int ntiles) {
-  if (st->done) return;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  PSB_MARCH_BEGIN(g, V, TI, ntiles)
+__device__ __forceinline__ void jacobi_march_item(const Grid &g, const DMat<M> &A, const T *r,
+                                                  const T *e, T *eout, const ProlX<T, V> &pr,
+                                                  T omega, const uint8_t *af, T zscale, int TI,
+                                                  int slab, int tile, double (&acc)[4]) {
+  const int nv = g.nz / V;
+  const int q = tile * kBlock + (int)threadIdx.x;
+  if (q >= g.ny * nv) return;
+  const int j = q / nv, k = (q - j * nv) * V;
+  const int i0 = slab * TI, i1 = min(i0 + TI, g.nx);
   auto opv = [&](int Q, int ii) {
     if constexpr (PROL) return pr.vec(Q, ii, j, k);
     else return ldv<T, V>(e + Q);
   };
-  int P = (i0_ * g.ny + j) * g.nz + k;
-  VecN<T, V> prev = zerov<T, V>(), cur = zerov<T, V>();
-  if (act_) {
-    if (i0_ > 0) prev = opv(P - g.sI, i0_ - 1);
-    cur = opv(P, i0_);
-  }
-  for (int i = i0_; i < i1_; i++, P += g.sI) {
+  int P = (i0 * g.ny + j) * g.nz + k;
+  VecN<T, V> prev = i0 > 0 ? opv(P - g.sI, i0 - 1) : zerov<T, V>();
+  VecN<T, V> cur = opv(P, i0);
+  for (int i = i0; i < i1; i++, P += g.sI) {
     const VecN<T, V> next = (i + 1 < g.nx) ? opv(P + g.sI, i + 1) : zerov<T, V>();
     Gath<T, V> G;
     if constexpr (PROL) {
@@ -150,35 +149,55 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_march(Grid g, DMat<M> A, cons
     prev = cur;
     cur = next;
   }
+}
+
+// all items of a grid of ntiles x slabs, block bl of nb taking items
+// bl, bl + nb, ... (one item per block in the single-system launch)
+template <typename T, int V, bool PROL, bool RHO, typename M, bool FLUX>
+__device__ __forceinline__ void jacobi_march_body(Span sp, const Grid &g, const DMat<M> &A,
+                                                  const T *r, const T *e, T *eout,
+                                                  const ProlX<T, V> &pr, T omega,
+                                                  const uint8_t *af, Work w, T zscale, int TI,
+                                                  int ntiles) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int items = ntiles * ((g.nx + TI - 1) / TI);
+  for (int wi = sp.bl; wi < items; wi += sp.nb)
+    jacobi_march_item<T, V, PROL, RHO, M, FLUX>(g, A, r, e, eout, pr, omega, af, zscale, TI,
+                                                wi / ntiles, wi % ntiles, acc);
   if (RHO) block_partial<4>(acc, w.partial);
+}
+
+// damped Jacobi sweep e_out = e + omega (r - A e) / c, the operand e or,
+// with PROL, e + cfac P eH (the first post-sweep); RHO: the last post-sweep
+// of level 0 (output times zscale, partials r.z, sum z, sum r, active
+// cells per block).  The same arithmetic as k_jacobi.
+template <typename T, int V, bool PROL, bool RHO, typename M, bool FLUX>
+__global__ __launch_bounds__(kBlock) void k_jacobi_march(Grid g, DMat<M> A, const T *r,
+                                                         const T *e, T *eout, ProlX<T, V> pr,
+                                                         T omega, const uint8_t *af,
+                                                         const DevState *st, Work w, T zscale,
+                                                         int TI, int ntiles) {
+  if (st->done) return;
+  jacobi_march_body<T, V, PROL, RHO, M, FLUX>(grid_span(), g, A, r, e, eout, pr, omega, af, w,
+                                              zscale, TI, ntiles);
 }
 
 // CG step 1 (cg.cpp:75-108): p = z + beta p_old with z = r/c (DIAG) or the
 // projected preconditioner output, q = A p, partial p.q -- as k_cg_pq
-// M = _Float16: the level's FP16 copy scale * (d, w, s, b) (flux form, not
-// with DIAG); q is the product times qscale = 1 / scale (a power of two)
-template <typename T, int V, bool SYM, bool DIAG, bool FLUX, typename M = T>
-__global__ __launch_bounds__(kBlock) void k_cg_pq_march(Grid g, DMat<M> A, const T *r_or_z,
-                                                        const T *pold, T *pnew, T *q,
-                                                        const uint8_t *af, DevState *st,
-                                                        Work w, int TI, int ntiles,
-                                                        T qscale = T(1)) {
-  static_assert(!DIAG || std::is_same<T, M>::value, "the diagonal operand divides by A.c");
-  if (st->done) return;
-  const T beta = (T)st->beta;
-  double acc[1] = {0.0};
-  PSB_MARCH_BEGIN(g, V, TI, ntiles)
-  using B = typename std::conditional<DIAG, DiagPX<T, V>, ZPX<T, V>>::type;
-  B base;
-  if constexpr (DIAG) base = DiagPX<T, V>{r_or_z, (const T *)A.c, pold, beta};
-  else base = ZPX<T, V>{r_or_z, pold, af, beta, (T)st->zmean};
-  int P = (i0_ * g.ny + j) * g.nz + k;
-  VecN<T, V> prev = zerov<T, V>(), cur = zerov<T, V>();
-  if (act_) {
-    if (i0_ > 0) prev = base.vec(P - g.sI, i0_ - 1, j, k);
-    cur = base.vec(P, i0_, j, k);
-  }
-  for (int i = i0_; i < i1_; i++, P += g.sI) {
+// CG step 1 item (cg.cpp:75-108): p = z + beta p_old, q = A p, p.q
+template <typename T, int V, bool SYM, bool FLUX, typename M, class B>
+__device__ __forceinline__ void cg_pq_march_item(const Grid &g, const DMat<M> &A, const B &base,
+                                                 T *pnew, T *q, T qscale, int TI, int slab,
+                                                 int tile, double (&acc)[1]) {
+  const int nv = g.nz / V;
+  const int qi = tile * kBlock + (int)threadIdx.x;
+  if (qi >= g.ny * nv) return;
+  const int j = qi / nv, k = (qi - j * nv) * V;
+  const int i0 = slab * TI, i1 = min(i0 + TI, g.nx);
+  int P = (i0 * g.ny + j) * g.nz + k;
+  VecN<T, V> prev = i0 > 0 ? base.vec(P - g.sI, i0 - 1, j, k) : zerov<T, V>();
+  VecN<T, V> cur = base.vec(P, i0, j, k);
+  for (int i = i0; i < i1; i++, P += g.sI) {
     const VecN<T, V> next = (i + 1 < g.nx) ? base.vec(P + g.sI, i + 1, j, k) : zerov<T, V>();
     MarchX<T, V, B> X{base, prev, cur, next, i, j};
     Gath<T, V> G = gather<T, V, SYM, FLUX>(g, A, X, P, i, j, k);
@@ -194,7 +213,47 @@ __global__ __launch_bounds__(kBlock) void k_cg_pq_march(Grid g, DMat<M> A, const
     prev = cur;
     cur = next;
   }
+}
+
+// the projected-z form (the preconditioned CG), over the items of block bl
+template <typename T, int V, bool SYM, bool FLUX, typename M = T>
+__device__ __forceinline__ void cg_pq_march_body(Span sp, const Grid &g, const DMat<M> &A,
+                                                 const T *z, const T *pold, T *pnew, T *q,
+                                                 const uint8_t *af, const DevState *st, Work w,
+                                                 int TI, int ntiles, T qscale = T(1)) {
+  double acc[1] = {0.0};
+  const ZPX<T, V> base{z, pold, af, (T)st->beta, (T)st->zmean};
+  const int items = ntiles * ((g.nx + TI - 1) / TI);
+  for (int wi = sp.bl; wi < items; wi += sp.nb)
+    cg_pq_march_item<T, V, SYM, FLUX, M>(g, A, base, pnew, q, qscale, TI, wi / ntiles,
+                                         wi % ntiles, acc);
   block_partial<1>(acc, w.partial);
+}
+
+// CG step 1 (as k_cg_pq): z = r/c (DIAG) or the projected preconditioner
+// output; M = _Float16: the level's FP16 copy scale * (d, w, s, b) (flux
+// form, not with DIAG), q = the product times qscale = 1 / scale
+template <typename T, int V, bool SYM, bool DIAG, bool FLUX, typename M = T>
+__global__ __launch_bounds__(kBlock) void k_cg_pq_march(Grid g, DMat<M> A, const T *r_or_z,
+                                                        const T *pold, T *pnew, T *q,
+                                                        const uint8_t *af, DevState *st,
+                                                        Work w, int TI, int ntiles,
+                                                        T qscale = T(1)) {
+  static_assert(!DIAG || std::is_same<T, M>::value, "the diagonal operand divides by A.c");
+  if (st->done) return;
+  if constexpr (DIAG) {
+    double acc[1] = {0.0};
+    const DiagPX<T, V> base{r_or_z, (const T *)A.c, pold, (T)st->beta};
+    const Span sp = grid_span();
+    const int items = ntiles * ((g.nx + TI - 1) / TI);
+    for (int wi = sp.bl; wi < items; wi += sp.nb)
+      cg_pq_march_item<T, V, SYM, FLUX, M>(g, A, base, pnew, q, qscale, TI, wi / ntiles,
+                                           wi % ntiles, acc);
+    block_partial<1>(acc, w.partial);
+  } else {
+    cg_pq_march_body<T, V, SYM, FLUX, M>(grid_span(), g, A, r_or_z, pold, pnew, q, af, st, w,
+                                         TI, ntiles, qscale);
+  }
 }
 
 // restriction (as k_restrict, CI = 2, one vector of V fine cells along k per
