@@ -76,7 +76,8 @@ typedef struct {
   double res_rat;     /* ResRat   (resrat.h:26, default 0.01)                */
   double res_tol;     /* ResTol   (restol.h:26, default 1e-15)               */
   double scale;       /* scale    (linear.h:53, default 1.0)                 */
-  int stalecount;     /* stalecount (linear.h:54, default -1 = off)          */
+  int stalecount;     /* stalecount (linear.h:54, default -1 = off; at most
+                         64, larger values are PSB_EINVAL)                   */
   int precform;       /* precform (linear.h:55); 1/c is formed on the fly    */
   int method;         /* psb_method                                          */
   int precond;        /* psb_precond                                         */
@@ -301,6 +302,10 @@ typedef struct {
   double smooth_res_rat; /* AC::resrat: ResRat of the level solves (1e-18)   */
   double smooth_res_tol; /* AC::restol: ResTol of the level and coarsest
                             solves (1e-18)                                   */
+  int max_stale[3];   /* AC::stale_iter_array: stalecount of the pre-, re-
+                         and post-smoothing CG solves (-1, -1, -1 = off,
+                         additive.cpp:19; at most 64); the coarsest solve
+                         runs without one (additive_recursive.cpp:27)       */
 } psb_cycle;
 
 void psb_cycle_defaults(psb_cycle *c);

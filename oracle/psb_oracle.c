@@ -890,8 +890,8 @@ static double ac_residual(ora_ac *ac, double dt) { return ac_residual2(ac, dt, N
 /* Linear::solve of one level (call_smoother / call_solver,
  * additive_cycle_functions.cpp:90-107): CG with the diagonal preconditioner;
  * returns whether it met its tolerance */
-static int ac_cg(ora_ac *ac, int l, int mi, double rr, double rt, double scale) {
-  return ora_cg(&ac->A[l], ac->phi[l], ac->f[l], ac->dV[l], 1, mi, rr, rt, scale, -1,
+static int ac_cg(ora_ac *ac, int l, int mi, double rr, double rt, double scale, int ms) {
+  return ora_cg(&ac->A[l], ac->phi[l], ac->f[l], ac->dV[l], 1, mi, rr, rt, scale, ms,
                 NULL, NULL);
 }
 
@@ -899,21 +899,21 @@ static int ac_cg(ora_ac *ac, int l, int mi, double rr, double rt, double scale) 
  * tolerances resrat / restol (atto by default, additive.cpp:16-17) and the
  * coarsest call MaxIter 100, ResRat -1 */
 typedef struct {
-  const int *mv;
+  const int *mv, *ms;  /* MaxIter and stale counts: pre / re- / post-smoothing */
   double rr, rt, scale;
 } cyc_par;
 
 static void ac_coarsest(ora_ac *ac, int l, const cyc_par *q) {
-  ac_cg(ac, l, 100, -1.0, q->rt, q->scale);
+  ac_cg(ac, l, 100, -1.0, q->rt, q->scale, -1);
   ora_interpolation(ac, l);
 }
 
 static void ac_v(ora_ac *ac, int l, const cyc_par *q) {
   if (l != ac->nlev - 1) {
-    ac_cg(ac, l, q->mv[0], q->rr, q->rt, q->scale);
+    ac_cg(ac, l, q->mv[0], q->rr, q->rt, q->scale, q->ms[0]);
     ora_restriction(ac, l);
     ac_v(ac, l + 1, q);
-    ac_cg(ac, l, q->mv[2], q->rr, q->rt, q->scale);
+    ac_cg(ac, l, q->mv[2], q->rr, q->rt, q->scale, q->ms[2]);
     if (l > 0) ora_interpolation(ac, l);
   } else {
     ac_coarsest(ac, l, q);
@@ -926,14 +926,14 @@ static void ac_v(ora_ac *ac, int l, const cyc_par *q) {
  * the first descent, post-smooth, prolongate */
 static void ac_fw(ora_ac *ac, int l, const cyc_par *q, int w) {
   if (l != ac->nlev - 1) {
-    ac_cg(ac, l, q->mv[0], q->rr, q->rt, q->scale);
+    ac_cg(ac, l, q->mv[0], q->rr, q->rt, q->scale, q->ms[0]);
     ora_restriction(ac, l);
     ac_fw(ac, l + 1, q, w);
-    ac_cg(ac, l, q->mv[1], q->rr, q->rt, q->scale);
+    ac_cg(ac, l, q->mv[1], q->rr, q->rt, q->scale, q->ms[1]);
     ora_restriction(ac, l);
     if (w) ac_fw(ac, l + 1, q, 1);
     else ac_v(ac, l + 1, q);
-    ac_cg(ac, l, q->mv[2], q->rr, q->rt, q->scale);
+    ac_cg(ac, l, q->mv[2], q->rr, q->rt, q->scale, q->ms[2]);
     if (l > 0) ora_interpolation(ac, l);
   } else {
     ac_coarsest(ac, l, q);
@@ -963,6 +963,7 @@ void ora_cycle_defaults(ora_cycle *P) {
   P->max_cyc = 20;
   P->min_cyc = 0;
   P->mv[0] = P->mv[1] = P->mv[2] = 20;
+  P->ms[0] = P->ms[1] = P->ms[2] = -1;
   P->dt = 1.0;
   P->init = ORA_CYC_NONE;
   P->loop = ORA_CYC_V;
@@ -998,7 +999,7 @@ static int ac_converged(ora_ac *ac, const ora_cycle *P, int cycle, double *res1,
 int ora_ac_cycle(ora_ac *ac, double *phi0, const double *f0, const ora_cycle *P,
                  double *hist, double *hist_linf, int *nhist, int *ncyc) {
   const size_t N0 = mat_n(&ac->A[0]);
-  const cyc_par q = {P->mv, P->resrat, P->restol, 1.0 / P->dt * 1.0};
+  const cyc_par q = {P->mv, P->ms, P->resrat, P->restol, 1.0 / P->dt * 1.0};
   double *fold = dalloc(N0);
   int ret = 0, nh = 0;
   memcpy(ac->phi[0], phi0, N0 * 8);

@@ -168,6 +168,7 @@ typedef struct {
   double toler, factor;          /* ResTol, ResRat of the loop               */
   int max_cyc, min_cyc;
   int mv[3];                     /* MaxIter pre / re- / post-smoothing       */
+  int ms[3];                     /* stale counts of those solves (-1: off)   */
   double dt;
   int init, loop;                /* ORA_CYC_*                                */
   int stop_if_div, priority_min_cyc, use_linf;

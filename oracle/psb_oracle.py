@@ -34,7 +34,8 @@ MAXL = 24
 
 class OraCycle(C.Structure):
     _fields_ = [("toler", C.c_double), ("factor", C.c_double), ("max_cyc", C.c_int),
-                ("min_cyc", C.c_int), ("mv", C.c_int * 3), ("dt", C.c_double),
+                ("min_cyc", C.c_int), ("mv", C.c_int * 3), ("ms", C.c_int * 3),
+                ("dt", C.c_double),
                 ("init", C.c_int), ("loop", C.c_int), ("stop_if_div", C.c_int),
                 ("priority_min_cyc", C.c_int), ("use_linf", C.c_int), ("resrat", C.c_double),
                 ("restol", C.c_double)]
@@ -316,12 +317,13 @@ class AC:
 
     def cycle(self, phi0, f0, loop="V", init=None, toler=-1.0, factor=1e-8, max_cyc=20,
               min_cyc=0, mv=(20, 20, 20), dt=1.0, stop_if_div=True, priority_min_cyc=True,
-              use_linf=False, resrat=1e-18, restol=1e-18):
+              use_linf=False, resrat=1e-18, restol=1e-18, ms=(-1, -1, -1)):
         """AC::cycle(init, loop, ...) with CG-di level solves (additive_cycle.cpp);
         loop / init in "V", "F", "W" (init None: no full-cycle start)."""
         kinds = {"V": 0, "F": 1, "W": 2, None: -1}
         P = OraCycle(C.c_double(toler), C.c_double(factor), max_cyc, min_cyc,
-                     (C.c_int * 3)(*mv), C.c_double(dt), kinds[init], kinds[loop],
+                     (C.c_int * 3)(*mv), (C.c_int * 3)(*ms), C.c_double(dt), kinds[init],
+                     kinds[loop],
                      int(stop_if_div), int(priority_min_cyc), int(use_linf),
                      C.c_double(resrat), C.c_double(restol))
         phi = np.array(phi0, dtype=np.float64, copy=True)

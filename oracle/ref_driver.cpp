@@ -365,27 +365,31 @@ void dump_cycles(World &w) {
     int minc;
     double smooth_rr;  // AC::resrat of the level solves (atto by default)
     std::array<int, 3> mv;
+    std::array<int, 3> ms;  // AC::stale_iter_array (-1: off, the default)
   };
   const double A = boil::atto;
+  const std::array<int, 3> NS = {-1, -1, -1};
   const std::vector<Run> runs = {
-      {"F1", 1, 1e-30, 1, false, true, true, 0, A, {20, 20, 20}},
-      {"F2", 1, 1e-30, 2, false, true, true, 0, A, {20, 20, 20}},
-      {"Ffull", 1, 1e-8, 20, false, true, true, 0, A, {20, 20, 20}},
-      {"W1", 2, 1e-30, 1, false, true, true, 0, A, {20, 20, 20}},
-      {"W2", 2, 1e-30, 2, false, true, true, 0, A, {20, 20, 20}},
-      {"Wfull", 2, 1e-8, 20, false, true, true, 0, A, {20, 20, 20}},
-      {"FMV2", 10, 1e-30, 2, false, true, true, 0, A, {20, 20, 20}},
-      {"FMVfull", 10, 1e-8, 20, false, true, true, 0, A, {20, 20, 20}},
-      {"FMF2", 11, 1e-30, 2, false, true, true, 0, A, {20, 20, 20}},
-      {"FMW2", 12, 1e-30, 2, false, true, true, 0, A, {20, 20, 20}},
-      {"Vlinf", 0, 1e-8, 20, true, true, true, 0, A, {20, 20, 20}},
-      {"Vdiv", 0, 1e-30, 20, false, true, true, 0, A, {20, 20, 20}},
-      {"Vnodiv", 0, 1e-30, 20, false, false, true, 0, A, {20, 20, 20}},
-      {"Vmin", 0, 1e-30, 20, false, true, true, 12, A, {20, 20, 20}},
-      {"Vminnoprio", 0, 1e-30, 20, false, true, false, 12, A, {20, 20, 20}},
-      {"Vlinfdiv", 0, 1e-30, 8, true, true, true, 0, A, {1, 1, 1}},
-      {"Vmv", 0, 1e-8, 20, false, true, true, 0, A, {3, 20, 5}},
-      {"Vsmooth", 0, 1e-8, 20, false, true, true, 0, 1e-2, {20, 20, 20}},
+      {"F1", 1, 1e-30, 1, false, true, true, 0, A, {20, 20, 20}, NS},
+      {"F2", 1, 1e-30, 2, false, true, true, 0, A, {20, 20, 20}, NS},
+      {"Ffull", 1, 1e-8, 20, false, true, true, 0, A, {20, 20, 20}, NS},
+      {"W1", 2, 1e-30, 1, false, true, true, 0, A, {20, 20, 20}, NS},
+      {"W2", 2, 1e-30, 2, false, true, true, 0, A, {20, 20, 20}, NS},
+      {"Wfull", 2, 1e-8, 20, false, true, true, 0, A, {20, 20, 20}, NS},
+      {"FMV2", 10, 1e-30, 2, false, true, true, 0, A, {20, 20, 20}, NS},
+      {"FMVfull", 10, 1e-8, 20, false, true, true, 0, A, {20, 20, 20}, NS},
+      {"FMF2", 11, 1e-30, 2, false, true, true, 0, A, {20, 20, 20}, NS},
+      {"FMW2", 12, 1e-30, 2, false, true, true, 0, A, {20, 20, 20}, NS},
+      {"Vlinf", 0, 1e-8, 20, true, true, true, 0, A, {20, 20, 20}, NS},
+      {"Vdiv", 0, 1e-30, 20, false, true, true, 0, A, {20, 20, 20}, NS},
+      {"Vnodiv", 0, 1e-30, 20, false, false, true, 0, A, {20, 20, 20}, NS},
+      {"Vmin", 0, 1e-30, 20, false, true, true, 12, A, {20, 20, 20}, NS},
+      {"Vminnoprio", 0, 1e-30, 20, false, true, false, 12, A, {20, 20, 20}, NS},
+      {"Vlinfdiv", 0, 1e-30, 8, true, true, true, 0, A, {1, 1, 1}, NS},
+      {"Vmv", 0, 1e-8, 20, false, true, true, 0, A, {3, 20, 5}, NS},
+      {"Vsmooth", 0, 1e-8, 20, false, true, true, 0, 1e-2, {20, 20, 20}, NS},
+      {"Vstale", 0, 1e-8, 20, false, true, true, 0, A, {20, 20, 20}, {3, 3, 3}},
+      {"Fstale", 1, 1e-8, 20, false, true, true, 0, A, {20, 20, 20}, {2, 4, 2}},
   };
   // Cycle::flex is left out: its recursion (additive_recursive.cpp:167-200)
   // ends only when a level's smoother meets its ResRat, and once a level's
@@ -402,9 +406,12 @@ void dump_cycles(World &w) {
     ac.resrat(ResRat(r.smooth_rr));
     std::array<MaxIter, 3> mv = {MaxIter(r.mv[0]), MaxIter(r.mv[1]), MaxIter(r.mv[2])};
     ac.max_iter_array(mv);
+    std::array<MaxIter, 3> ms = {MaxIter(r.ms[0]), MaxIter(r.ms[1]), MaxIter(r.ms[2])};
+    ac.stale_iter_array(ms);
     std::cout << "ACRUN " << r.tag << " " << r.kind << " " << r.rr << " " << r.maxc << " "
               << r.linf << " " << r.stopdiv << " " << r.prio << " " << r.minc << " "
-              << r.smooth_rr << " " << r.mv[0] << " " << r.mv[1] << " " << r.mv[2] << std::endl;
+              << r.smooth_rr << " " << r.mv[0] << " " << r.mv[1] << " " << r.mv[2] << " "
+              << r.ms[0] << " " << r.ms[1] << " " << r.ms[2] << std::endl;
     int ncyc = 0;
     const Cycle loops[3] = {Cycle::V(), Cycle::F(), Cycle::W()};
     const Cycle init = r.kind >= 10 ? Cycle::V() : Cycle::none();

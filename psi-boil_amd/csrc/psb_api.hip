@@ -815,7 +815,7 @@ void init_state(psb_ctx *C, DevState *st, const psb_stop *p, int norm, double re
   h.ntot = (double)C->g.N;
   h.mini = p->min_iter;
   h.maxit = maxit;
-  h.stalecount = std::min(p->stalecount, kStaleMax);
+  h.stalecount = p->stalecount;  // at most kStaleMax (checked by the entry points)
   h.norm = norm;
   h.hist = record ? C->hist : nullptr;
   h.hist_cap = record ? C->hist_cap : 0;
@@ -1245,14 +1245,15 @@ void build_ref(psb_ctx *C) {
 
 // Linear::solve of one level by CG with the diagonal preconditioner
 // (call_smoother / call_solver): phi is improved in place
-void ref_cg(psb_ctx *C, int l, int maxit, double res_rat, double res_tol, double scale) {
+void ref_cg(psb_ctx *C, int l, int maxit, double res_rat, double res_tol, double scale,
+            int stalecount) {
   RefHier &H = C->href;
   const Level<double> &lv = H.L[l];
   const Grid &g = lv.g;
   double *r = H.K[l][0], *p0 = H.K[l][1], *p1 = H.K[l][2], *q = H.K[l][3];
   DevState *st = C->st64;
   hipLaunchKernelGGL(k_stinit, dim3(1), dim3(64), 0, C->stream, st, maxit, res_rat, res_tol,
-                     scale, (double)g.N);
+                     scale, (double)g.N, stalecount);
   zero(C, p0, g.N);
   zero(C, p1, g.N);
   start<double, true, 0>(C, g, lv.mat(), lv.r, lv.e[0], r, nullptr, 0.0, st, &H.M[l]);
@@ -1263,10 +1264,10 @@ void ref_cg(psb_ctx *C, int l, int maxit, double res_rat, double res_tol, double
   }
 }
 
-// the level-solve parameters of one cycle: MaxIter per stage (pre,
-// re-smoothing, post), AC::resrat / restol, the smoothers' scale 1/dt
+// the level-solve parameters of one cycle: MaxIter and stalecount per stage
+// (pre, re-smoothing, post), AC::resrat / restol, the smoothers' scale 1/dt
 struct RefPar {
-  int mv[3];
+  int mv[3], ms[3];
   double rr, rt, scale;
 };
 
@@ -1289,17 +1290,17 @@ void ref_restrict(psb_ctx *C, int l) {  // residual of level l into l + 1, mean 
 }
 
 void ref_coarsest(psb_ctx *C, int l, const RefPar &q) {  // MaxIter 100, ResRat -1
-  ref_cg(C, l, 100, -1.0, q.rt, q.scale);
+  ref_cg(C, l, 100, -1.0, q.rt, q.scale, -1);
   ref_interp(C, l);
 }
 
 // v_cycle (additive_recursive.cpp:4-34)
 void ref_v(psb_ctx *C, int l, const RefPar &q) {
   if (l != (int)C->href.L.size() - 1) {
-    ref_cg(C, l, q.mv[0], q.rr, q.rt, q.scale);
+    ref_cg(C, l, q.mv[0], q.rr, q.rt, q.scale, q.ms[0]);
     ref_restrict(C, l);
     ref_v(C, l + 1, q);
-    ref_cg(C, l, q.mv[2], q.rr, q.rt, q.scale);
+    ref_cg(C, l, q.mv[2], q.rr, q.rt, q.scale, q.ms[2]);
     if (l > 0) ref_interp(C, l);
   } else {
     ref_coarsest(C, l, q);
@@ -1312,14 +1313,14 @@ void ref_v(psb_ctx *C, int l, const RefPar &q) {
 // W-cycle) on the coarse phi the first one left
 void ref_fw(psb_ctx *C, int l, const RefPar &q, bool w) {
   if (l != (int)C->href.L.size() - 1) {
-    ref_cg(C, l, q.mv[0], q.rr, q.rt, q.scale);
+    ref_cg(C, l, q.mv[0], q.rr, q.rt, q.scale, q.ms[0]);
     ref_restrict(C, l);
     ref_fw(C, l + 1, q, w);
-    ref_cg(C, l, q.mv[1], q.rr, q.rt, q.scale);
+    ref_cg(C, l, q.mv[1], q.rr, q.rt, q.scale, q.ms[1]);
     ref_restrict(C, l);
     if (w) ref_fw(C, l + 1, q, true);
     else ref_v(C, l + 1, q);
-    ref_cg(C, l, q.mv[2], q.rr, q.rt, q.scale);
+    ref_cg(C, l, q.mv[2], q.rr, q.rt, q.scale, q.ms[2]);
     if (l > 0) ref_interp(C, l);
   } else {
     ref_coarsest(C, l, q);
@@ -1379,6 +1380,8 @@ int ac_cycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_res
     return PSB_EINVAL;
   if (P->loop < PSB_CYCLE_V || P->loop > PSB_CYCLE_W) return PSB_EINVAL;
   if (P->init < PSB_CYCLE_NONE || P->init > PSB_CYCLE_W) return PSB_EINVAL;
+  for (int s = 0; s < 3; s++)
+    if (P->max_stale[s] > kStaleMax) return PSB_EINVAL;
   build_ref(C);
   RefHier &H = C->href;
   const Level<double> &L0 = H.L[0];
@@ -1410,20 +1413,23 @@ int ac_cycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_res
       zero(C, H.L[l].e[0], H.L[l].g.N);
       zero(C, H.L[l].r, H.L[l].g.N);
     }
-    const RefPar q{{P->max_iter[0], P->max_iter[1], P->max_iter[2]}, P->smooth_res_rat,
+    const RefPar q{{P->max_iter[0], P->max_iter[1], P->max_iter[2]},
+                   {P->max_stale[0], P->max_stale[1], P->max_stale[2]}, P->smooth_res_rat,
                    P->smooth_res_tol, dti};
     // the graphs hold the cycle's shape and MaxIter values; the tolerances
     // and the scale are kernel arguments compared separately
     const long long key = (long long)q.mv[0] + ((long long)q.mv[1] << 16) +
                           ((long long)q.mv[2] << 32) + ((long long)P->loop << 48);
     const long long fkey = key + ((long long)(P->init + 1) << 52);
-    if (H.gr.key != key || H.gr.fkey != dti || H.grr != q.rr || H.grt != q.rt) {
+    const bool same_ms = std::equal(q.ms, q.ms + 3, H.gms);
+    if (H.gr.key != key || H.gr.fkey != dti || H.grr != q.rr || H.grt != q.rt || !same_ms) {
       destroy_graph(H.gr);
       destroy_graph(H.gfull);
     }
     H.gr.fkey = dti;
     H.grr = q.rr;
     H.grt = q.rt;
+    std::copy(q.ms, q.ms + 3, H.gms);
     // AC::converged: 0 converged, 1 diverging (fold restored), 2 go on
     auto converged = [&](int cycle, double &res1, double &linf1, double res0,
                          double linf0) {
@@ -1899,7 +1905,7 @@ int psb_set_active(psb_ctx *C, const unsigned char *off) {
 int psb_solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_result *R) {
   if (!C || !x || !b || !p || !R) return PSB_EINVAL;
   if (!C->have_matrix) return PSB_ENOTREADY;
-  if (p->max_iter < 0 || p->scale == 0.0) return PSB_EINVAL;
+  if (p->max_iter < 0 || p->scale == 0.0 || p->stalecount > kStaleMax) return PSB_EINVAL;
   std::memset(R, 0, sizeof(*R));
   return guarded(C, [&]() { return solve_device(C, x, b, p, R); });
 }
@@ -1965,6 +1971,7 @@ void psb_cycle_defaults(psb_cycle *c) {
   c->use_linf = 0;
   c->smooth_res_rat = kAtto;
   c->smooth_res_tol = kAtto;
+  c->max_stale[0] = c->max_stale[1] = c->max_stale[2] = -1;  // additive.cpp:19
 }
 
 int psb_ac_vcycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_result *R) {

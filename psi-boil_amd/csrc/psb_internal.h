@@ -167,6 +167,7 @@ struct RefHier {
   Graph gr;              // one cycle of the loop
   Graph gfull;           // the full-cycle start
   double grr = 0.0, grt = 0.0;  // level tolerances the graphs were captured with
+  int gms[3] = {-1, -1, -1};    // and stale counts
 };
 
 }  // namespace host

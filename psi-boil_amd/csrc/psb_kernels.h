@@ -970,7 +970,7 @@ static __global__ void k_ac_interp(Grid g, double *phi, Grid gH, const double *p
 // AC::call_smoother / call_solver); the start kernel's finalize resets the
 // Krylov scalars
 static __global__ void k_stinit(DevState *st, int maxit, double res_rat, double res_tol, double scale,
-                         double ntot) {
+                         double ntot, int stalecount) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   st->res_rat = res_rat;
   st->res_tol = res_tol;
@@ -978,7 +978,7 @@ static __global__ void k_stinit(DevState *st, int maxit, double res_rat, double 
   st->ntot = ntot;
   st->mini = 1;
   st->maxit = maxit;
-  st->stalecount = -1;
+  st->stalecount = stalecount;
   st->norm = 0;
   st->hist = nullptr;
   st->hist_cap = 0;

@@ -74,7 +74,8 @@ class Cycle(C.Structure):
                 ("min_cycles", C.c_int), ("max_iter", C.c_int * 3), ("dt", C.c_double),
                 ("loop", C.c_int), ("init", C.c_int), ("stop_if_div", C.c_int),
                 ("priority_min_cyc", C.c_int), ("use_linf", C.c_int),
-                ("smooth_res_rat", C.c_double), ("smooth_res_tol", C.c_double)]
+                ("smooth_res_rat", C.c_double), ("smooth_res_tol", C.c_double),
+                ("max_stale", C.c_int * 3)]
 
 
 def _cycle(**cyc):
@@ -82,8 +83,8 @@ def _cycle(**cyc):
     lib().psb_cycle_defaults(C.byref(c))
     shapes = {None: PSB_CYCLE_NONE, "V": PSB_CYCLE_V, "F": PSB_CYCLE_F, "W": PSB_CYCLE_W}
     for k, v in cyc.items():
-        if k == "max_iter":
-            c.max_iter[:] = list(v)
+        if k in ("max_iter", "max_stale"):
+            getattr(c, k)[:] = list(v)
         elif k in ("loop", "init") and (v is None or isinstance(v, str)):
             setattr(c, k, shapes[v])
         else:

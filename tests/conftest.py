@@ -62,4 +62,4 @@ def cycle_kwargs(run):
                 toler=-1.0, factor=run["factor"], max_cyc=run["max_cyc"],
                 min_cyc=run["min_cyc"], mv=tuple(run["mv"]), stop_if_div=run["stop_if_div"],
                 priority_min_cyc=run["priority_min_cyc"], use_linf=run["use_linf"],
-                resrat=run["resrat"])
+                resrat=run["resrat"], ms=tuple(run.get("ms", (-1, -1, -1))))

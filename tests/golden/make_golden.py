@@ -117,6 +117,7 @@ def parse_cycles(text):
                    "use_linf": bool(int(f[5])), "stop_if_div": bool(int(f[6])),
                    "priority_min_cyc": bool(int(f[7])), "min_cyc": int(f[8]),
                    "resrat": float(f[9]), "mv": [int(f[10]), int(f[11]), int(f[12])],
+                   "ms": [int(f[13]), int(f[14]), int(f[15])],
                    "initial": None, "cycles": []}
             runs.append(cur)
             continue

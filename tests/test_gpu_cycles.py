@@ -2,7 +2,7 @@
 init / control fields) against the reference's own runs (cycles fixtures,
 written by its own build): F and W cycles, full-cycle starts, the linf
 control with its stale previous value, divergence and min-cycle controls,
-per-stage MaxIter and the level solves' ResRat.
+per-stage MaxIter and stale counts and the level solves' ResRat.
 
 The GPU sums dot products in a tree, the reference sequentially.  How much
 that matters is a property of each run, and it is measured: the oracle
@@ -43,7 +43,7 @@ def gpu_kwargs(run):
                 min_cycles=run["min_cyc"], max_iter=run["mv"],
                 stop_if_div=int(run["stop_if_div"]),
                 priority_min_cyc=int(run["priority_min_cyc"]), use_linf=int(run["use_linf"]),
-                smooth_res_rat=run["resrat"])
+                smooth_res_rat=run["resrat"], max_stale=run.get("ms", (-1, -1, -1)))
 
 
 _solvers = {}
@@ -140,3 +140,17 @@ def test_gpu_full_cycle_start_is_discarded_like_the_reference():
     h2 = s.history()
     assert h2[1] == h1[1]       # the same start, evaluated the same way
     assert h2[2] < 0.1 * h1[2]  # kept, it leaves the first cycle far less to do
+
+
+def test_gpu_stale_counts_beyond_the_window_are_rejected():
+    """Stale windows hold at most 64 residuals: a larger stale_iter_array
+    entry, or a larger stalecount of a Krylov solve, is an invalid argument
+    rather than a silently shortened window."""
+    g, s = solver("bubbles16x12x8")
+    b, n = g["b"], g["b"].size
+    with pytest.raises(psb.PsbError, match="EINVAL"):
+        s.ac_vcycle(np.zeros(n), b, res_rat=1e-8, max_stale=(3, 65, 3))
+    with pytest.raises(psb.PsbError, match="EINVAL"):
+        s.solve(np.zeros(n), b, stalecount=65, max_iter=10)
+    x, r = s.ac_vcycle(np.zeros(n), b, res_rat=1e-8, max_stale=(64, 64, 64))
+    assert r["converged"]

@@ -1,8 +1,8 @@
 """AC::cycle in the oracle against the reference's own build: the F and W
 cycles, full-cycle starts (fullcycle), the linf control with the reference's
 stale previous-linf value, the divergence check with and without
-priority_min_cyc, min_cycles, per-stage MaxIter and the level solves'
-ResRat -- phi after each run bit for bit, every logged residual (17
+priority_min_cyc, min_cycles, per-stage MaxIter and stale counts
+(stale_iter_array) and the level solves' ResRat -- phi after each run bit for bit, every logged residual (17
 significant digits) and the returned cycle count.  CPU only."""
 import numpy as np
 import pytest
@@ -58,3 +58,20 @@ def test_fixtures_exercise_the_controls():
     assert r["Ffull"]["converged"] and r["Wfull"]["converged"]
     assert r["FMVfull"]["converged"] and r["FMVfull"]["cycles"][0][0] == 0
     assert r["Vlinf"]["converged"] and r["Vlinf"]["use_linf"]
+
+
+@pytest.mark.parametrize("tag", ["Vstale", "Fstale"])
+def test_stale_counts_change_the_result(systems, tag):
+    """The stale-count runs differ from the same settings with the counts off,
+    so their fixtures pin where each count goes (pre-, re-, post-smoothing)."""
+    g, (phis, runs) = systems["bubbles16x12x8"]
+    run = [r for r in runs if r["tag"] == tag][0]
+    kw = cycle_kwargs(run)
+    A = O.Mat(*g.shape, g.mat())
+    kw["ms"] = (-1, -1, -1)
+    phi, _ = O.AC(A, g["dV"], rule=0).cycle(np.zeros(A.n), g["b"], **kw)
+    assert not np.array_equal(phi, phis[tag + "_phi"])
+    if tag == "Fstale":  # the re-smoothing count alone swapped with the others
+        kw["ms"] = (4, 2, 4)
+        phi, _ = O.AC(A, g["dV"], rule=0).cycle(np.zeros(A.n), g["b"], **kw)
+        assert not np.array_equal(phi, phis[tag + "_phi"])
