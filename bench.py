@@ -191,7 +191,9 @@ EXTRA = {
 def drop_case(a):
     """The plain reference method at its own size: 64^3, one centred drop,
     CG-di in FP64 to 1e-8 on the GPU, and the reference's CG::solve (its own
-    build) on one host core on the same system."""
+    build) on one host core on the same system; beside it CG with the
+    red-black Gauss-Seidel preconditioner on the GPU and the reference's CG
+    with Prec::ic2 (which it replaces) on one core."""
     n = 64
     d = psb.generate(n, n, n, 1, 0.25, 0.25, 1000.0, 1.0, 0, 1001, centre=(0.5, 0.5, 0.5))
     s = psb.Solver(n, n, n)
@@ -206,15 +208,25 @@ def drop_case(a):
                        "1e-8 (reference norm)",
            "gpu_ms_per_solve": round(t * 1e3, 3), "iterations": r["iterations"],
            "rel_residual": r["rel_l2"]}
+    t2, r2 = timed_solves(s, dx, db, a.steps, a.warmup, max_iter=1000, res_rat=1e-8,
+                          graph_chunk=32, precond=psb.PSB_PREC_SGS)
+    out["sgs_gpu_ms_per_solve"] = round(t2 * 1e3, 3)
+    out["sgs_iterations"] = r2["iterations"]
+    out["sgs_rel_residual"] = r2["rel_l2"]
     drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
     if os.path.exists(drv) and not a.no_cpu:
-        args = [drv, "bench-cg", str(n), "1", "0.25", "0.25", "1000.0", "1.0", "0", "1001", "1000",
-                "0.5", "0.5", "0.5"]
-        o = subprocess.run(args, capture_output=True, text=True, timeout=600).stdout
-        m = re.search(r"solve_s=([0-9.eE+-]+)", o)
-        if m:
-            out["reference_cpu_s_per_solve"] = float(m.group(1))
-            out["gpu_over_cpu"] = round(float(m.group(1)) / t, 1)
+        for mode, key, tg in (("bench-cg", "reference_cpu_s_per_solve", t),
+                              ("bench-ic2", "reference_ic2_cpu_s_per_solve", t2)):
+            args = [drv, mode, str(n), "1", "0.25", "0.25", "1000.0", "1.0", "0", "1001", "1000",
+                    "0.5", "0.5", "0.5"]
+            o = subprocess.run(args, capture_output=True, text=True, timeout=600).stdout
+            m = re.search(r"solve_s=([0-9.eE+-]+)", o)
+            it = re.search(r"iterations = (\d+)", o)
+            if m:
+                out[key] = float(m.group(1))
+                out[key.replace("_s_per_solve", "_over_gpu")] = round(float(m.group(1)) / tg, 1)
+            if it and mode == "bench-ic2":
+                out["reference_ic2_iterations"] = int(it.group(1))
     print(json.dumps(out), flush=True)
 
 

@@ -45,7 +45,15 @@ enum psb_status {
 };
 
 enum psb_method { PSB_CG = 0, PSB_BICGS = 1 };
-enum psb_precond { PSB_PREC_DIAG = 0, PSB_PREC_AC = 1 };
+/* PSB_PREC_DIAG: Prec::di (preconditioner_form.cpp:4-13).  PSB_PREC_AC: the
+ * additive-correction V-cycle.  PSB_PREC_SGS: red-black symmetric
+ * Gauss-Seidel, M = (D - L) D^-1 (D - U) with the even cells ((i+j+k) % 2 = 0)
+ * ordered first, the parallel replacement for Prec::ic0 / ic2 / ic3 (whose
+ * triangular sweeps, preconditioner_solve.cpp:14-110, run cell after cell);
+ * CG and FP64 only.  Not the same preconditioner: on the fixture cases CG
+ * needs about 1.5-1.8 times the iterations of Prec::ic2 and half those of
+ * Prec::di, each iteration in parallel. */
+enum psb_precond { PSB_PREC_DIAG = 0, PSB_PREC_AC = 1, PSB_PREC_SGS = 2 };
 enum psb_precision { PSB_FP64 = 0, PSB_MIXED = 1 };
 enum psb_norm {
   PSB_NORM_REF = 0,   /* sqrt(sum (r/dV)^2)/N/scale, cg.cpp:49,121 with

@@ -15,6 +15,14 @@
  *                                the reference's CG::solve / BiCGS::solve
  *                                (golden fixtures); dense pseudo-inverse
  *                                solution on tiny grids.
+ *   ora_cg_prec (IC0, IC2)       x after 1, 5 iterations and at 1e-8, counts
+ *                                and residuals of the reference's CG with
+ *                                Prec::ic0 / ic2 (prec fixtures), bit for bit.
+ *   ora_cg_prec (RBSGS),
+ *   ora_prec_apply               no counterpart in the reference: z = M^-1 r
+ *                                against M = (D - L) D^-1 (D - U) assembled
+ *                                densely in red-black order (the other order
+ *                                fails it), M = A + L D^-1 U, symmetry.
  *   ora_jacobi                   reference Jacobi::solve history (fixtures);
  *                                closed-form iteration on a diagonal matrix.
  *   ora_levels                   reference level sizes (fixtures) and the
@@ -428,18 +436,172 @@ static int stale_check(double *win, int cnt, double res) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* CG, Src/Solver/Linear/Krylov/cg.cpp:10-182 with Diagonal preconditioning  */
-/* (preconditioner_form.cpp:4-13, preconditioner_solve.cpp:4-11)             */
+/* preconditioners of CG (Src/Solver/Preconditioner, selected by            */
+/* linear.cpp:31-34): Diagonal, IncompleteCholesky0 / 2, and the red-black  */
+/* symmetric Gauss-Seidel that the library runs in place of the sequential  */
+/* incomplete-Cholesky sweeps                                               */
 /* ------------------------------------------------------------------------ */
-int ora_cg(const ora_mat *A, double *x, const double *b, const double *dV,
-           int mini, int maxit, double res_rat, double res_tol, double scale,
-           int stalecount, double *hist, ora_result *out) {
+typedef struct {
+  int kind;
+  const ora_mat *A;
+  double *Mc, *Mw, *Ms, *Mb, *Mtw, *Mts;
+} ora_prec;
+
+/* IncompleteCholesky0 / 2 :: form (preconditioner_form.cpp:16-80) on the
+ * compact interior: the factor's couplings the reference writes into ghost
+ * cells multiply ghost values of z and are left out */
+static void prec_form(ora_prec *M, const ora_mat *A, int kind) {
+  const int nx = A->nx, ny = A->ny, nz = A->nz;
   const size_t N = mat_n(A);
-  double *p = dalloc(N), *q = dalloc(N), *r = dalloc(N), *z = dalloc(N),
-         *Mc = dalloc(N);
+  memset(M, 0, sizeof(*M));
+  M->kind = kind;
+  M->A = A;
+  M->Mc = dalloc(N);
+  if (kind == ORA_PREC_DI || kind == ORA_PREC_RBSGS) {
+    for (size_t P = 0; P < N; P++) M->Mc[P] = 1.0 / A->c[P];
+    return;
+  }
+  const int ic2 = kind == ORA_PREC_IC2;
+  M->Mw = dalloc(N); M->Ms = dalloc(N); M->Mb = dalloc(N);
+  M->Mtw = dalloc(N); M->Mts = dalloc(N);
+  double *Mc = M->Mc, *Mw = M->Mw, *Ms = M->Ms, *Mb = M->Mb, *Mtw = M->Mtw, *Mts = M->Mts;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        const size_t P = IDX(i, j, k, ny, nz);
+        double sum = A->c[P];
+        sum = sum - Mw[P] * Mw[P];
+        if (ic2) sum = sum - Mtw[P] * Mtw[P];
+        sum = sum - Ms[P] * Ms[P];
+        if (ic2) sum = sum - Mts[P] * Mts[P];
+        sum = sum - Mb[P] * Mb[P];
+        sum = sqrt(sum);
+        Mc[P] = sum;
+        if (i + 1 < nx) {
+          const size_t E = IDX(i + 1, j, k, ny, nz);
+          Mw[E] = -A->w[E] / Mc[P];
+          if (ic2 && k > 0) {
+            const size_t EB = IDX(i + 1, j, k - 1, ny, nz);
+            Mtw[EB] = -Mw[EB] * Mb[P] / Mc[P];
+          }
+        }
+        if (j + 1 < ny) {
+          const size_t Nn = IDX(i, j + 1, k, ny, nz);
+          Ms[Nn] = -A->s[Nn] / Mc[P];
+          if (ic2 && k > 0) {
+            const size_t NB = IDX(i, j + 1, k - 1, ny, nz);
+            Mts[NB] = -Ms[NB] * Mb[P] / Mc[P];
+          }
+        }
+        if (k + 1 < nz) {
+          const size_t T = P + 1;
+          if (ic2)
+            Mb[T] = (-A->b[T] - Mw[T] * Mtw[P] - Ms[T] * Mts[P]) / Mc[P];
+          else
+            Mb[T] = -A->b[T] / Mc[P];
+        }
+      }
+}
+
+static void prec_free(ora_prec *M) {
+  free(M->Mc); free(M->Mw); free(M->Ms); free(M->Mb); free(M->Mtw); free(M->Mts);
+}
+
+/* one colour of the red-black symmetric Gauss-Seidel sweep on the cells with
+ * (i + j + k) % 2 == colour, in place:  z_P = z_P + (sum a_nb z_nb) / c_P,
+ * the neighbour sum grouped as (W + E) + (S + N) + (B + T) */
+static void rbsgs_colour(const ora_mat *A, double *z, int colour) {
+  const int nx = A->nx, ny = A->ny, nz = A->nz;
+  const size_t sI = (size_t)ny * nz, sJ = (size_t)nz;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        if (((i + j + k) & 1) != colour) continue;
+        const size_t P = IDX(i, j, k, ny, nz);
+        const double xw = i > 0 ? z[P - sI] : 0.0, xe = i < nx - 1 ? z[P + sI] : 0.0;
+        const double xs = j > 0 ? z[P - sJ] : 0.0, xn = j < ny - 1 ? z[P + sJ] : 0.0;
+        const double xb = k > 0 ? z[P - 1] : 0.0, xt = k < nz - 1 ? z[P + 1] : 0.0;
+        const double s1 = A->w[P] * xw + A->e[P] * xe;
+        const double s2 = A->s[P] * xs + A->n[P] * xn;
+        const double s3 = A->b[P] * xb + A->t[P] * xt;
+        z[P] = z[P] + ((s1 + s2) + s3) / A->c[P];
+      }
+}
+
+/* z = M^-1 r */
+static void prec_solve(const ora_prec *M, double *z, const double *r) {
+  const ora_mat *A = M->A;
+  const int nx = A->nx, ny = A->ny, nz = A->nz;
+  const size_t N = mat_n(A);
+  if (M->kind == ORA_PREC_DI) {  /* preconditioner_solve.cpp:4-11 */
+    for (size_t P = 0; P < N; P++) z[P] = r[P] * M->Mc[P];
+    return;
+  }
+  if (M->kind == ORA_PREC_RBSGS) {
+    /* M = (D - L) D^-1 (D - U) in red-black order (even cells first):
+     * z = D^-1 r, odd cells from the even ones, then even cells from the
+     * odd ones */
+    for (size_t P = 0; P < N; P++) z[P] = r[P] / A->c[P];
+    rbsgs_colour(A, z, 1);
+    rbsgs_colour(A, z, 0);
+    return;
+  }
+  /* IncompleteCholesky0 / 2 :: solve (preconditioner_solve.cpp:14-73) */
+  const int ic2 = M->kind == ORA_PREC_IC2;
+  const double *Mc = M->Mc, *Mw = M->Mw, *Ms = M->Ms, *Mb = M->Mb, *Mtw = M->Mtw,
+               *Mts = M->Mts;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        const size_t P = IDX(i, j, k, ny, nz);
+        double sum = r[P];
+        if (i > 0) sum = sum - Mw[P] * z[IDX(i - 1, j, k, ny, nz)];
+        if (ic2 && i > 0 && k + 1 < nz) sum = sum - Mtw[P] * z[IDX(i - 1, j, k + 1, ny, nz)];
+        if (j > 0) sum = sum - Ms[P] * z[IDX(i, j - 1, k, ny, nz)];
+        if (ic2 && j > 0 && k + 1 < nz) sum = sum - Mts[P] * z[IDX(i, j - 1, k + 1, ny, nz)];
+        if (k > 0) sum = sum - Mb[P] * z[P - 1];
+        z[P] = sum / Mc[P];
+      }
+  for (int i = nx - 1; i >= 0; i--)
+    for (int j = ny - 1; j >= 0; j--)
+      for (int k = nz - 1; k >= 0; k--) {
+        const size_t P = IDX(i, j, k, ny, nz);
+        double sum = z[P];
+        if (i + 1 < nx) {
+          const size_t E = IDX(i + 1, j, k, ny, nz);
+          sum = sum - Mw[E] * z[E];
+          if (ic2 && k > 0) sum = sum - Mtw[E - 1] * z[E - 1];
+        }
+        if (j + 1 < ny) {
+          const size_t Nn = IDX(i, j + 1, k, ny, nz);
+          sum = sum - Ms[Nn] * z[Nn];
+          if (ic2 && k > 0) sum = sum - Mts[Nn - 1] * z[Nn - 1];
+        }
+        if (k + 1 < nz) sum = sum - Mb[P + 1] * z[P + 1];
+        z[P] = sum / Mc[P];
+      }
+}
+
+void ora_prec_apply(const ora_mat *A, int kind, const double *r, double *z) {
+  ora_prec M;
+  prec_form(&M, A, kind);
+  prec_solve(&M, z, r);
+  prec_free(&M);
+}
+
+/* ------------------------------------------------------------------------ */
+/* CG, Src/Solver/Linear/Krylov/cg.cpp:10-182, with the preconditioner kind  */
+/* (Diagonal: preconditioner_form.cpp:4-13, preconditioner_solve.cpp:4-11)  */
+/* ------------------------------------------------------------------------ */
+int ora_cg_prec(const ora_mat *A, int prec, double *x, const double *b, const double *dV,
+                int mini, int maxit, double res_rat, double res_tol, double scale,
+                int stalecount, double *hist, ora_result *out) {
+  const size_t N = mat_n(A);
+  double *p = dalloc(N), *q = dalloc(N), *r = dalloc(N), *z = dalloc(N);
   double *win = stalecount > 0 ? dalloc((size_t)stalecount) : NULL;
   double alfa, beta, rho = 0.0, rho_old = 1.0, pq;
-  for (size_t P = 0; P < N; P++) Mc[P] = 1.0 / A->c[P];
+  ora_prec M;
+  prec_form(&M, A, prec);
   for (int q2 = 0; q2 < stalecount; q2++) win[q2] = UNREAL;
 
   ora_resid(A, b, x, r);
@@ -454,7 +616,7 @@ int ora_cg(const ora_mat *A, double *x, const double *b, const double *dV,
   }
   int i;
   for (i = 0; i < maxit; i++) {
-    for (size_t P = 0; P < N; P++) z[P] = r[P] * Mc[P];
+    prec_solve(&M, z, r);
     rho = ora_dot(N, r, z);
     if (i == 0) {
       memcpy(p, z, N * sizeof(double));
@@ -478,8 +640,16 @@ int ora_cg(const ora_mat *A, double *x, const double *b, const double *dV,
   R.res = res;
 done:
   if (out) *out = R;
-  free(p); free(q); free(r); free(z); free(Mc); free(win);
+  free(p); free(q); free(r); free(z); free(win);
+  prec_free(&M);
   return R.converged;
+}
+
+int ora_cg(const ora_mat *A, double *x, const double *b, const double *dV,
+           int mini, int maxit, double res_rat, double res_tol, double scale,
+           int stalecount, double *hist, ora_result *out) {
+  return ora_cg_prec(A, ORA_PREC_DI, x, b, dV, mini, maxit, res_rat, res_tol, scale,
+                     stalecount, hist, out);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -113,6 +113,14 @@ double ora_dot_voldiv_avg(size_t n, const double *x, const double *y,
 int ora_cg(const ora_mat *A, double *x, const double *b, const double *dV,
            int mini, int maxit, double res_rat, double res_tol, double scale,
            int stalecount, double *hist, ora_result *out);
+/* CG with a preconditioner: ORA_PREC_DI / _IC0 / _IC2 as Prec::di / ic0 /
+ * ic2 (preconditioner.h:16-20), ORA_PREC_RBSGS the red-black symmetric
+ * Gauss-Seidel the library runs in their place; ora_prec_apply: z = M^-1 r */
+enum { ORA_PREC_DI = 0, ORA_PREC_IC0 = 1, ORA_PREC_IC2 = 2, ORA_PREC_RBSGS = 10 };
+int ora_cg_prec(const ora_mat *A, int prec, double *x, const double *b, const double *dV,
+                int mini, int maxit, double res_rat, double res_tol, double scale,
+                int stalecount, double *hist, ora_result *out);
+void ora_prec_apply(const ora_mat *A, int kind, const double *r, double *z);
 int ora_bicgs(const ora_mat *A, double *x, const double *b, const double *dV,
               int mini, int maxit, double res_rat, double res_tol, double scale,
               int stalecount, double *hist, ora_result *out);

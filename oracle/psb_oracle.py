@@ -235,6 +235,27 @@ def cg(A, x, b, dV, **kw):
     return _krylov(lib().ora_cg, A, x, b, dV, **kw)
 
 
+PREC = {"di": 0, "ic0": 1, "ic2": 2, "rbsgs": 10}
+
+
+def cg_prec(A, prec, x, b, dV, **kw):
+    """CG::solve with Prec::di / ic0 / ic2 (preconditioner_form.cpp,
+    preconditioner_solve.cpp) or the red-black symmetric Gauss-Seidel
+    ("rbsgs") the library runs in place of the incomplete Cholesky."""
+    k = PREC[prec]
+
+    def fn(*a):
+        return lib().ora_cg_prec(a[0], k, *a[1:])
+    return _krylov(fn, A, x, b, dV, **kw)
+
+
+def prec_apply(A, prec, r):
+    z = np.zeros(A.n)
+    lib().ora_prec_apply(C.byref(A.s), PREC[prec], _p(np.ascontiguousarray(r, np.float64)),
+                         _p(z))
+    return z
+
+
 def bicgs(A, x, b, dV, **kw):
     """Reference BiCGS::solve semantics (bicgs.cpp:9-229)."""
     return _krylov(lib().ora_bicgs, A, x, b, dV, **kw)

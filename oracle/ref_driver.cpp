@@ -444,7 +444,7 @@ double now() {
 int main(int argc, char **argv) {
   std::cout << std::setprecision(17);
   if (argc < 2) {
-    std::fprintf(stderr, "usage: ref_driver case|bench-cg|bench-ac ...\n");
+    std::fprintf(stderr, "usage: ref_driver case|cycles|prec|bck|bench-cg|bench-ic2|bench-ac ...\n");
     return 2;
   }
   std::string mode = argv[1];
@@ -486,6 +486,23 @@ int main(int argc, char **argv) {
     dump_cycles(w);
     return 0;
   }
+  if (mode == "prec") {
+    // CG with the reference's incomplete-Cholesky preconditioners
+    // (Prec::ic0, Prec::ic2; linear.cpp:31-34) on the case's pressure system
+    if (argc < 13) { std::fprintf(stderr, "prec: too few arguments\n"); return 2; }
+    outdir = argv[2];
+    Case cs = parse_case(argv + 6, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
+    if (argc >= 16) {
+      cs.centred = true;
+      cs.cx = std::atof(argv[13]); cs.cy = std::atof(argv[14]); cs.cz = std::atof(argv[15]);
+    }
+    World w = build_world(cs);
+    CG ic0(*w.d, Prec::ic0());
+    run_krylov(w, &ic0, "ic0", {1, 5}, 2000, 1e-8);
+    CG ic2(*w.d, Prec::ic2());
+    run_krylov(w, &ic2, "ic2", {1, 5}, 2000, 1e-8);
+    return 0;
+  }
   if (mode == "bck") {
     if (argc < 7) { std::fprintf(stderr, "bck: too few arguments\n"); return 2; }
     outdir = argv[2];
@@ -511,7 +528,7 @@ int main(int argc, char **argv) {
     std::cout << "BCKNAME " << name_file("press", ".bck", 12, boil::cart.iam()) << "\n";
     return 0;
   }
-  if (mode == "bench-cg" || mode == "bench-ac") {
+  if (mode == "bench-cg" || mode == "bench-ac" || mode == "bench-ic2") {
     if (argc < 11) { std::fprintf(stderr, "bench: too few arguments\n"); return 2; }
     int n = std::atoi(argv[2]);
     Case cs = parse_case(argv + 3, n, n, n);
@@ -526,12 +543,15 @@ int main(int argc, char **argv) {
     Scalar x(*w.d);
     neumann(x);
     x = 0.0;
-    if (mode == "bench-cg") {
+    if (mode == "bench-cg" || mode == "bench-ic2") {
+      CG ic2(*w.d, Prec::ic2());
+      Krylov *sol = mode == "bench-cg" ? w.solver : &ic2;
+      const char *tag = mode == "bench-cg" ? "cg" : "ic2";
       double ts = now();
-      bool ok = w.solver->solve(w.pr->A, x, w.pr->fnew, MinIter(1), MaxIter(lim), "cg",
-                                ResRat(1e-8), ResTol(boil::femto), 1.0, -1, true);
+      bool ok = sol->solve(w.pr->A, x, w.pr->fnew, MinIter(1), MaxIter(lim), tag,
+                           ResRat(1e-8), ResTol(boil::femto), 1.0, -1, true);
       double te = now();
-      std::cout << "BENCH cg n=" << n << " ok=" << ok << " setup_s=" << (t1 - t0)
+      std::cout << "BENCH " << tag << " n=" << n << " ok=" << ok << " setup_s=" << (t1 - t0)
                 << " solve_s=" << (te - ts) << "\n";
     } else {
       AC ac(w.pr);

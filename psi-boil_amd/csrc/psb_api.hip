@@ -760,6 +760,7 @@ void destroy_graph(Graph &gr) {
 // geometry: any change to those drops them
 void destroy_graphs(psb_ctx *C) {
   destroy_graph(C->gr_cgdiag);
+  destroy_graph(C->gr_cgsgs);
   destroy_graph(C->gr_bicgs);
   destroy_graph(C->gr_ac64);
   destroy_graph(C->gr_ac32);
@@ -909,6 +910,59 @@ void solve_cg_diag(psb_ctx *C, double *x, const double *b, const psb_stop *p, ps
     }
   };
   run_graph(C, C->gr_cgdiag, chunk * 2 + (SYM ? 1 : 0), C->st64, p->max_iter, body);
+  HIPCHK(hipMemcpyAsync(x, K.x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, C->stream));
+  finish_result(R, *C->hst, 0);
+  C->hist_count = std::min(C->hst->reason == 7 ? 1 : C->hst->iter + 1, C->hist_cap);
+}
+
+// ---------------------------------------------------------------------------
+// CG (cg.cpp:10-182) with the red-black symmetric Gauss-Seidel preconditioner
+// in place of Prec::ic0 / ic2 / ic3, whose triangular sweeps
+// (preconditioner_solve.cpp:14-110) run cell after cell: per iteration
+// z = r / c (fused into the x/r step), the two colour sweeps (the second
+// with rho = r.z), the p/q step and the x/r step with the stop test.  FP64.
+// ---------------------------------------------------------------------------
+template <bool SYM>
+void rbsgs(psb_ctx *C, const Grid &g, DMat<double> A, double *z, const double *r, int colour,
+           bool rho, DevState *st) {
+  with_v<double>(g, [&](auto vt) {
+    constexpr int V = decltype(vt)::value;
+    if (rho)
+      hipLaunchKernelGGL((k_rbsgs<double, V, SYM, true>), dim3(nblocks<double>(g)), dim3(kBlock),
+                         0, C->stream, g, A, z, r, colour, st, C->work);
+    else
+      hipLaunchKernelGGL((k_rbsgs<double, V, SYM, false>), dim3(nblocks<double>(g)),
+                         dim3(kBlock), 0, C->stream, g, A, z, r, colour, st, C->work);
+  });
+  if (rho) finalize<4>(C, FIN_RHO, nblocks<double>(g), st);
+}
+
+template <bool SYM>
+void solve_cg_sgs(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_result *R) {
+  KrylovVecs<double> &K = C->k64;
+  ensure_krylov(C, K, false);
+  if (!K.z) K.z = C->arena.alloc<double>(C->g.N);
+  ensure_hist(C, p->max_iter);
+  const Grid &g = C->g;
+  DMat<double> A = mat64(C);
+  HIPCHK(hipMemcpyAsync(K.x, x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, C->stream));
+  init_state(C, C->st64, p, p->norm, p->res_rat, p->res_tol, p->max_iter);
+  zero(C, K.p[0], g.N);
+  zero(C, K.p[1], g.N);
+  start<double, SYM, 1>(C, g, A, b, K.x, K.r, K.z, 1.0, C->st64);  // z = r / c
+  HIPCHK(hipGetLastError());
+  read_state(C, C->st64);
+  const int chunk = chunk_of(p);
+  auto body = [&]() {
+    for (int it = 0; it < chunk; it++) {
+      const int par = it & 1;
+      rbsgs<SYM>(C, g, A, K.z, K.r, 1, false, C->st64);
+      rbsgs<SYM>(C, g, A, K.z, K.r, 0, true, C->st64);
+      cg_pq<double, SYM, false>(C, g, A, K.z, K.p[par], K.p[par ^ 1], K.q, nullptr, C->st64);
+      cg_xr<double, false>(C, g, A.c, K.p[par ^ 1], K.q, K.x, K.r, K.z, 1.0, C->st64);
+    }
+  };
+  run_graph(C, C->gr_cgsgs, chunk * 2 + (SYM ? 1 : 0), C->st64, p->max_iter, body);
   HIPCHK(hipMemcpyAsync(x, K.x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, C->stream));
   finish_result(R, *C->hst, 0);
   C->hist_count = std::min(C->hst->reason == 7 ? 1 : C->hst->iter + 1, C->hist_cap);
@@ -1515,8 +1569,12 @@ int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_
       solve_ir(C, x, b, p, R);
     else
       solve_ac64(C, x, b, p, R);
+  } else if (p->precond == PSB_PREC_SGS) {
+    if (p->precision != PSB_FP64 || p->method != PSB_CG) return PSB_EINVAL;
+    if (C->sym) solve_cg_sgs<true>(C, x, b, p, R);
+    else solve_cg_sgs<false>(C, x, b, p, R);
   } else {
-    if (p->precision != PSB_FP64) return PSB_EINVAL;
+    if (p->precond != PSB_PREC_DIAG || p->precision != PSB_FP64) return PSB_EINVAL;
     if (p->method == PSB_CG) {
       if (C->sym) solve_cg_diag<true>(C, x, b, p, R);
       else solve_cg_diag<false>(C, x, b, p, R);

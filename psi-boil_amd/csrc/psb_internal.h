@@ -145,6 +145,7 @@ template <typename T>
 struct KrylovVecs {
   T *x = nullptr, *r = nullptr, *q = nullptr, *p[2] = {nullptr, nullptr};
   T *v[2] = {nullptr, nullptr}, *s = nullptr, *t = nullptr, *rt = nullptr;
+  T *z = nullptr;  // preconditioned residual (red-black Gauss-Seidel)
 };
 
 struct Graph {
@@ -215,7 +216,7 @@ struct psb_ctx {
   double *dy = nullptr;                 // scratch for psb_apply
   double *relres = nullptr;             // 2 doubles: sum r^2, sum b^2
   double ref_dt = 1.0;                  // dt of the running AC cycle (its residual)
-  Graph gr_cgdiag, gr_bicgs, gr_ac64, gr_ac32, gr_ru;
+  Graph gr_cgdiag, gr_cgsgs, gr_bicgs, gr_ac64, gr_ac32, gr_ru;
   double ac_omega = 0, ac_cfac = 0;
   int coarse_min = 8;  // coarsening halves a direction while it stays >= this
   int coarse_rule = 0; // 0 all directions at once, 1 x and y before z

@@ -187,6 +187,41 @@ __global__ __launch_bounds__(kBlock) void k_cg_xr(Grid g, const T *c, Metric m, 
 }
 
 // ---------------------------------------------------------------------------
+// red-black symmetric Gauss-Seidel preconditioner, one colour in place: on
+// the cells with (i + j + k) % 2 == colour, z_P += (sum a_nb z_nb) / c_P.
+// Starting from z = r / c, colour 1 and then colour 0 give
+// z = (D - L)^-1 D (D - U)^-1 r in red-black order (even cells first).  The
+// lanes of the other colour are stored back unchanged; every neighbour an
+// updated cell reads is of that other colour, so no value read is one this
+// launch changes.  RHO (the second colour, z then final on every cell):
+// partials of r.z for FIN_RHO (sum z, sum r and the count left 0: no mean
+// is removed, as in the reference's CG).
+// ---------------------------------------------------------------------------
+template <typename T, int V, bool SYM, bool RHO>
+__global__ __launch_bounds__(kBlock) void k_rbsgs(Grid g, DMat<T> A, T *z, const T *r,
+                                                  int colour, DevState *st, Work w) {
+  if (st->done) return;
+  ArrX<T, V> X{z};
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  PSB_TILE_BEGIN(g, V)
+  Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+  VecN<T, V> E;
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    const bool act = ((i + j + k + u) & 1) == colour;
+    E.v[u] = act ? G.x.v[u] + ((G.s1.v[u] + G.s2.v[u]) + G.s3.v[u]) / G.c.v[u] : G.x.v[u];
+  }
+  if (RHO) {
+    VecN<T, V> R = ldv<T, V>(r + P);
+#pragma unroll
+    for (int u = 0; u < V; u++) acc[0] += (double)R.v[u] * (double)E.v[u];
+  }
+  stv<T, V>(z + P, E);
+  PSB_TILE_END
+  if (RHO) block_partial<4>(acc, w.partial);
+}
+
+// ---------------------------------------------------------------------------
 // damped Jacobi sweep  e_out = e + omega (r - A e) / c, the operand either e
 // or e + cfac P eH (prolongation fused into the first post-sweep).
 // RHO (last post-sweep on level 0): partials r.z, sum z, sum r, active

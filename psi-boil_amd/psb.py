@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpsb.so")
 
 PSB_CG, PSB_BICGS = 0, 1
-PSB_PREC_DIAG, PSB_PREC_AC = 0, 1
+PSB_PREC_DIAG, PSB_PREC_AC, PSB_PREC_SGS = 0, 1, 2
 PSB_FP64, PSB_MIXED = 0, 1
 PSB_NORM_REF, PSB_NORM_L2REL = 0, 1
 PSB_OK, PSB_MAXITER, PSB_STALE, PSB_BREAKDOWN, PSB_DIVERGED, PSB_NAN = 0, 1, 2, 3, 4, 5

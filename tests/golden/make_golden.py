@@ -8,11 +8,14 @@ outputs, solver iterates after k iterations, AC level matrices, restriction /
 interpolation results and V-cycle iterates, plus the solver log lines (17
 significant digits) parsed into arrays.
 
-    python tests/golden/make_golden.py [cases] [cycles]   # needs oracle/_ref built
+    python tests/golden/make_golden.py [cases] [cycles] [prec]   # needs oracle/_ref built
 
 "cycles" writes <case>_cycles.npz: the reference's AC::cycle with the F, W
 and full-cycle shapes and its controls (ref_driver cycles), phi after each
-run and the run's residual log.
+run and the run's residual log.  "prec" writes <case>_prec.npz: CG with the
+reference's incomplete-Cholesky preconditioners IC0 and IC2 (ref_driver
+prec), x after 1 and 5 iterations and at ResRat 1e-8, and the final log
+lines.
 
 Only the outputs travel; the reference does not.
 """
@@ -152,16 +155,48 @@ def run_cycles(name):
     return path
 
 
+def run_prec(name):
+    """CG with the reference's Prec::ic0 / Prec::ic2 (ref_driver prec): x
+    after 1 and 5 iterations and at 1e-8, the printed final residuals."""
+    nx, ny, nz, nsph, rmin, rmax, rin, rout, ss, sv, centre = CASES[name]
+    with tempfile.TemporaryDirectory() as d:
+        args = [DRIVER, "prec", d, str(nx), str(ny), str(nz), str(nsph), repr(rmin),
+                repr(rmax), repr(rin), repr(rout), str(ss), str(sv)]
+        if centre is not None:
+            args += [repr(c) for c in centre]
+        out = subprocess.run(args, check=True, capture_output=True, text=True).stdout
+        runs = {}
+        for line in out.splitlines():
+            m = re.match(r"(ic0|ic2), residual = " + NUM + ", ratio = " + NUM +
+                         r", iterations = (\d+)", line)
+            if m:
+                runs.setdefault(m.group(1), []).append(
+                    [float(m.group(2)), float(m.group(3)), int(m.group(4))])
+            m = re.match(r"RET (ic0|ic2) (\w+) (\d)", line)
+            if m:
+                runs.setdefault(m.group(1) + "_ret", []).append([m.group(2), int(m.group(3))])
+        arrays = {f"{t}_x{k}": np.fromfile(os.path.join(d, f"{t}_x{k}.f64"))
+                  for t in ("ic0", "ic2") for k in ("1", "5", "full")}
+    arrays["meta"] = np.array(json.dumps({"name": name, "snapshots": [1, 5, "full"],
+                                          "runs": runs}))
+    path = os.path.join(HERE, name + "_prec.npz")
+    np.savez_compressed(path, **arrays)
+    return path
+
+
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("oracle/_ref/ref_driver is not built (make -f oracle/ref.mk)")
-    what = sys.argv[1:] or ["cases", "cycles"]
+    what = sys.argv[1:] or ["cases", "cycles", "prec"]
     if "cases" in what:
         for name, spec in CASES.items():
             print(run_case(name, spec))
     if "cycles" in what:
         for name in CYCLE_CASES:
             print(run_cycles(name))
+    if "prec" in what:
+        for name in CASES:
+            print(run_prec(name))
 
 
 if __name__ == "__main__":
