@@ -1383,6 +1383,36 @@ static void zline_sweep(const ora_mat *A, const double *r, double *e, double *t,
   for (size_t P = 0; P < N; P++) e[P] += omega * d[P];
 }
 
+/* levels whose coarse correction is two steps of flexible CG preconditioned
+ * by the cycle below (the K-cycle of Notay and Vassilevski, Numer. Linear
+ * Algebra Appl. 15 (2008) 473-487): bit l set for level l */
+static int kcycle_mask = 0;
+void ora_set_kcycle(int mask) { kcycle_mask = mask; }
+
+/* out ~ A_l^-1 r: c = B r, v = A c, alpha1 = c.r / c.v, rt = r - alpha1 v,
+ * d = B rt, gamma = d.v, beta = d.A d, alpha2 = d.rt / (beta - gamma^2 / c.v),
+ * out = (alpha1 - gamma alpha2 / c.v) c + alpha2 d, with B the cycle on
+ * level l (the second step is always taken) */
+static void ac_kstep(ora_ac *ac, int l, const double *r, double *out, int nu1, int nu2,
+                     double omega, double cfac, int smoother) {
+  const ora_mat *A = &ac->A[l];
+  const size_t N = mat_n(A);
+  double *c = dalloc(N), *v = dalloc(N), *d = dalloc(N), *w = dalloc(N), *rt = dalloc(N);
+  ora_ac_precond_s(ac, l, r, c, nu1, nu2, omega, cfac, smoother);
+  ora_spmv(A, c, v);
+  const double rho1 = ora_dot(N, c, v);
+  const double a1 = rho1 > 0.0 ? ora_dot(N, c, r) / rho1 : 0.0;
+  for (size_t P = 0; P < N; P++) rt[P] = r[P] - a1 * v[P];
+  ora_ac_precond_s(ac, l, rt, d, nu1, nu2, omega, cfac, smoother);
+  ora_spmv(A, d, w);
+  const double gam = ora_dot(N, d, v), bet = ora_dot(N, d, w), del = ora_dot(N, d, rt);
+  const double den = rho1 > 0.0 ? bet - gam * gam / rho1 : 0.0;
+  const double a2 = den > 0.0 ? del / den : 0.0;
+  const double a = a1 - (rho1 > 0.0 ? gam * a2 / rho1 : 0.0);
+  for (size_t P = 0; P < N; P++) out[P] = a * c[P] + a2 * d[P];
+  free(c); free(v); free(d); free(w); free(rt);
+}
+
 void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
                     int nu2, double omega, double cfac) {
   ora_ac_precond_s(ac, l, r, z, nu1, nu2, omega, cfac, 0);
@@ -1445,11 +1475,17 @@ void ora_ac_precond_s(ora_ac *ac, int l, const double *r, double *z, int nu1,
       }
   for (size_t Q = 0; Q < NH; Q++)
     if (!fl[Q]) rH[Q] = 0.0;
-  ora_ac_precond_s(ac, l + 1, rH, eH, nu1, nu2, omega, cfac, smoother);
+  double cf = cfac;
+  if (l + 1 < ac->nlev - 1 && ((kcycle_mask >> (l + 1)) & 1)) {
+    ac_kstep(ac, l + 1, rH, eH, nu1, nu2, omega, cfac, smoother);
+    cf = 1.0;  /* the Krylov steps choose the scaling */
+  } else {
+    ora_ac_precond_s(ac, l + 1, rH, eH, nu1, nu2, omega, cfac, smoother);
+  }
   for (int I = 0; I < nX; I++)
     for (int J = 0; J < nY; J++)
       for (int K = 0; K < nZ; K++) {
-        double v = cfac * eH[IDX(I, J, K, nY, nZ)];
+        double v = cf * eH[IDX(I, J, K, nY, nZ)];
         for (int i = 0; i < CI; i++)
           for (int j = 0; j < CJ; j++)
             for (int k = 0; k < CK; k++) {

@@ -201,6 +201,10 @@ void ora_ac_precond(ora_ac *ac, int l, const double *r, double *z, int nu1,
  * k of each (i, j) column (Thomas algorithm).                              */
 void ora_ac_precond_s(ora_ac *ac, int l, const double *r, double *z, int nu1,
                       int nu2, double omega, double cfac, int smoother);
+/* K-cycle: levels (bit l = level l, 0 < l < coarsest) whose coarse
+ * correction in ora_ac_precond* is two flexible-CG steps preconditioned by
+ * the cycle below instead of cfac times one cycle; 0 (default): V-cycle.    */
+void ora_set_kcycle(int mask);
 /* x = T^-1 t column by column (the z-line smoother's solve).               */
 void ora_zline_solve(const ora_mat *A, const double *t, double *x);
 /* PCG with the V-cycle preconditioner, its output projected to zero mean over

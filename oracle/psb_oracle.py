@@ -183,6 +183,12 @@ def project(mx, my, mz, rho, p, u, v, w, dt=1.0, off=None, mask=7):
     return u, v, w
 
 
+def set_kcycle(mask):
+    """Levels (bit l) whose coarse correction is two flexible-CG steps
+    preconditioned by the cycle below (K-cycle); 0: the V-cycle."""
+    lib().ora_set_kcycle(int(mask))
+
+
 def set_sum_block(b):
     """0: sequential sums (the reference's order); b > 0: blocks of b terms."""
     lib().ora_set_sum_block(int(b))
