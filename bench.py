@@ -5,11 +5,14 @@ Workload: the variable-density pressure-Poisson system on a 256^3 unit cube,
 walls, right-hand side = divergence of a seeded uniform(-1,1) face-velocity
 field shifted to zero mean, x0 = 0.  A step is one complete solve to
 ||b - A x||_2 <= 1e-8 ||b||_2 by CG preconditioned with the additive-correction
-V-cycle (2+2 damped Jacobi, coarsest 8^3 exact FP64): FP32 inner solve inside
-FP64 iterative refinement (the value), and the plain FP64 solve beside it.
-Matrix and right-hand side are resident in HBM before the timed region.
+multigrid cycle (2+2 damped Jacobi, coarsest 8^3 exact FP64; on levels 2-4 the
+coarse correction is two flexible-CG steps, the K-cycle, --kcycle 0 gives the
+V-cycle): FP32 CG with FP64 residual updates (the value), and the plain FP64
+solve beside it; the V-cycle's two figures are reported too.  Matrix and
+right-hand side are resident in HBM before the timed region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu] [--no-ref-cycle]
+                    [--kcycle 0b11100]
 
 `--case batch` solves 96 seeded systems of uneven sizes as one batch and
 beside it one after another.  `--case 512` and `--case cyl` print the figures
@@ -256,6 +259,10 @@ def extra_case(a):
             "mixed_reliable_1e-2_fp16_smoother_and_cg": dict(precision=psb.PSB_MIXED,
                                                              inner_rtol=1e-2, inner_adapt=2,
                                                              ac_store=2)}
+    for km in (a.kcycle, a.kcycle << 1):  # K-cycle on the bench's levels and one lower
+        runs[f"fp64_kcycle_{km:b}"] = dict(precision=psb.PSB_FP64, ac_kcycle=km)
+        runs[f"mixed_reliable_1e-2_fp16_smoother_kcycle_{km:b}"] = dict(
+            precision=psb.PSB_MIXED, inner_rtol=1e-2, inner_adapt=2, ac_store=1, ac_kcycle=km)
     dx.zero()
     r12 = s.solve_device(dx.addr, db.addr, precision=psb.PSB_FP64,
                          **dict(common, res_rat=1e-12, max_iter=4000))
@@ -367,6 +374,8 @@ def main():
                     help="leave out the reference AC::vcycle timing (rocprofv3 kernel tracing "
                          "faults on its ~1500-node captured graph)")
     ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--kcycle", type=lambda v: int(v, 0), default=0b11100,
+                    help="K-cycle levels of the headline solves (bit l: level l; 0: V-cycle)")
     ap.add_argument("--nsys", type=int, default=96, help="systems of --case batch")
     ap.add_argument("--case", choices=sorted(EXTRA) + ["batch", "drop64"], default=None,
                     help="figures of another single-system workload instead of the bench line")
@@ -408,14 +417,20 @@ def main():
     db = psb.DeviceArray.from_host(d["rhs"])
     dx = psb.DeviceArray(n ** 3)
     common = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=1e-8, res_tol=0.0,
-                  max_iter=1000)
-    log("[bench] FP64 AC-PCG")
+                  max_iter=1000, ac_kcycle=a.kcycle)
+    mixed = dict(precision=psb.PSB_MIXED, inner_rtol=1e-2, inner_adapt=2, ac_store=1)
+    log(f"[bench] FP64 AC-PCG, K-cycle levels {a.kcycle:b}")
     t64, r64 = timed_solves(s, dx, db, a.steps, a.warmup, precision=psb.PSB_FP64, **common)
     log(f"[bench] fp64 {t64 * 1e3:.2f} ms/solve {r64}")
     x_fp64 = dx.download()
+    # the same two solves with the plain V-cycle, for comparison
+    vc = dict(common, ac_kcycle=0)
+    t64v, r64v = timed_solves(s, dx, db, a.steps, a.warmup, precision=psb.PSB_FP64, **vc)
+    tmxv, rmxv = timed_solves(s, dx, db, a.steps, a.warmup, **dict(vc, **mixed))
+    log(f"[bench] V-cycle: fp64 {t64v * 1e3:.2f} ms ({r64v['iterations']}), mixed "
+        f"{tmxv * 1e3:.2f} ms ({rmxv['iterations']})")
     log("[bench] mixed AC-PCG, FP32 iterations with FP64 reliable updates")
-    tmx, rmx = timed_solves(s, dx, db, a.steps, a.warmup, precision=psb.PSB_MIXED,
-                            inner_rtol=1e-2, inner_adapt=2, ac_store=1, **common)
+    tmx, rmx = timed_solves(s, dx, db, a.steps, a.warmup, **dict(common, **mixed))
     log(f"[bench] mixed {tmx * 1e3:.2f} ms/solve {rmx}")
     x_mixed = dx.download()
     # FP64 reference solution to 1e-12 for the error figure (untimed)
@@ -482,15 +497,21 @@ def main():
         "data": "synthetic (seeded bubbles, seeded face velocities)",
         "config": {"workload": f"pressure Poisson {n}^3, 16 bubbles r 0.03-0.12, density "
                                "ratio 1000, Neumann walls; AC-preconditioned CG (2+2 damped "
-                               "Jacobi, 2x coarsening to 8^3, exact FP64 coarsest) to 1e-8; "
-                               "FP32 CG with an FP64 true-residual update per 1e-2 drop, "
-                               "FP16 copy of the level matrices in the smoother",
+                               "Jacobi, 2x coarsening to 8^3, exact FP64 coarsest, two "
+                               "flexible-CG steps as the coarse correction of levels "
+                               f"{a.kcycle:b} (bit = level)) to 1e-8; FP32 CG with an FP64 "
+                               "true-residual update per 1e-2 drop, FP16 copy of the level "
+                               "matrices in the smoother",
                    "grid": [n, n, n], "parallelism": "single GPU"},
         "fp64_solves_per_s": round(1.0 / t64, 3),
         "fp64_ms_per_solve": round(t64 * 1e3, 3),
         "mixed_over_fp64": round(t64 / tmx, 3),
         "iterations": {"fp64": r64["iterations"], "mixed": rmx["iterations"],
                        "mixed_fp64_updates": rmx["outer"]},
+        "vcycle": {"fp64_ms_per_solve": round(t64v * 1e3, 3),
+                   "mixed_ms_per_solve": round(tmxv * 1e3, 3),
+                   "iterations": {"fp64": r64v["iterations"], "mixed": rmxv["iterations"]},
+                   "mixed_over_fp64": round(t64v / tmxv, 3)},
         "rel_residual": {"fp64": r64["rel_l2"], "mixed": rmx["rel_l2"]},
         "error_vs_fp64_1e-12": err,
         "fp64_solve_error_vs_fp64_1e-12": err64,

@@ -123,6 +123,14 @@ typedef struct {
                             in one pass (2.5-D marching tile) where the
                             level's shape allows; 0 one kernel per sweep
                             (default: faster on MI355X today; same z)      */
+  int ac_kcycle;         /* K-cycle levels, bit l for level l (level 1 is the
+                            first coarse level; bit 0 must be 0): there the
+                            coarse correction is two flexible-CG steps
+                            preconditioned by the cycle below (Notay and
+                            Vassilevski 2008) instead of cfac times one
+                            cycle.  Bits of the coarsest level or of levels
+                            the hierarchy does not have are ignored.
+                            0 (default): the V-cycle.                       */
 } psb_stop;
 
 typedef struct {

@@ -669,6 +669,75 @@ void build_hier32(psb_ctx *C) {
 // Returns the buffer that holds z.
 // ---------------------------------------------------------------------------
 template <typename T>
+T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega, T cfac);
+
+// whether level l's coarse correction is the K-cycle's two Krylov steps
+bool kcycle_level(const psb_ctx *C, int l, int nl) {
+  return l >= 1 && l < nl - 1 && ((C->kmask >> l) & 1);
+}
+
+// the K-cycle levels' work arrays, allocated before any graph capture
+template <typename T>
+void kcycle_buffers(psb_ctx *C, Hier<T> &H) {
+  const int nl = (int)H.L.size();
+  for (int l = 0; l < nl; l++) {
+    Level<T> &lv = H.L[l];
+    if (!kcycle_level(C, l, nl) || lv.kc) continue;
+    lv.kc = C->harena.alloc<T>(lv.g.N);
+    lv.kv = C->harena.alloc<T>(lv.g.N);
+    lv.kz = C->harena.alloc<T>(lv.g.N);
+    lv.kst = C->harena.alloc<double>(4);
+  }
+}
+
+// level l's correction for its right-hand side lv.r (first sweep already in
+// lv.e[0]) by two flexible-CG steps preconditioned by the cycle on l:
+// c = B r, v = s A c, alpha1 = c.r / c.v; rt = r - alpha1 v (in lv.r);
+// d = B rt; gamma = d.v, beta = d.s A d, alpha2 = d.rt / (beta - gamma^2 /
+// c.v); z = (alpha1 - gamma alpha2 / c.v) c + alpha2 d
+template <typename T>
+T *kstep(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega, T cfac) {
+  Level<T> &lv = H.L[l];
+  const Grid &g = lv.g;
+  if (!lv.kc) throw HipError{hipErrorInvalidValue};  // kcycle_buffers() was not called
+  const T s = T(lv.scale);
+  const int nb = nblocks<T>(g);
+  T *c = vcycle<T>(C, H, l, st, nu1, nu2, omega, cfac);
+  with_v<T>(g, [&](auto vt) {
+    constexpr int V = decltype(vt)::value;
+    if (lv.d)
+      hipLaunchKernelGGL((k_kc_av<T, V, true>), dim3(nb), dim3(kBlock), 0, C->stream, g,
+                         lv.matflux(), s, c, lv.r, lv.kc, lv.kv, st, C->work);
+    else
+      hipLaunchKernelGGL((k_kc_av<T, V, false>), dim3(nb), dim3(kBlock), 0, C->stream, g,
+                         lv.mat(), s, c, lv.r, lv.kc, lv.kv, st, C->work);
+  });
+  finalize<2>(C, FIN_K1, nb, st, nullptr, lv.kst);
+  with_v<T>(g, [&](auto vt) {
+    constexpr int V = decltype(vt)::value;
+    hipLaunchKernelGGL((k_kc_r<T, V>), dim3(nb), dim3(kBlock), 0, C->stream, g, lv.c, lv.kv,
+                       lv.r, lv.e[0], T(omega / lv.scale), lv.kst, st);
+  });
+  T *d = vcycle<T>(C, H, l, st, nu1, nu2, omega, cfac);
+  with_v<T>(g, [&](auto vt) {
+    constexpr int V = decltype(vt)::value;
+    if (lv.d)
+      hipLaunchKernelGGL((k_kc_dw<T, V, true>), dim3(nb), dim3(kBlock), 0, C->stream, g,
+                         lv.matflux(), s, d, lv.kv, lv.r, st, C->work);
+    else
+      hipLaunchKernelGGL((k_kc_dw<T, V, false>), dim3(nb), dim3(kBlock), 0, C->stream, g,
+                         lv.mat(), s, d, lv.kv, lv.r, st, C->work);
+  });
+  finalize<3>(C, FIN_K2, nb, st, nullptr, lv.kst);
+  with_v<T>(g, [&](auto vt) {
+    constexpr int V = decltype(vt)::value;
+    hipLaunchKernelGGL((k_kc_comb<T, V>), dim3(nb), dim3(kBlock), 0, C->stream, g, lv.kc, d,
+                       lv.kz, lv.kst, st);
+  });
+  return lv.kz;
+}
+
+template <typename T>
 T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega, T cfac) {
   Level<T> &lv = H.L[l];
   const int nl = (int)H.L.size();
@@ -687,8 +756,10 @@ T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega
   // divided by its scale: the coarse level's first sweep, the prolongation
   // and the preconditioner's output carry the ratios
   restrict_<T>(C, lv, lv.e[cur], cL, T(omega / cL.scale), st);
-  T *zH = vcycle<T>(C, H, l + 1, st, nu1, nu2, omega, cfac);
-  const T cfac_l = T(cfac * (cL.scale / lv.scale));
+  const bool kc = kcycle_level(C, l + 1, nl);  // the Krylov steps choose the scaling
+  T *zH = kc ? kstep<T>(C, H, l + 1, st, nu1, nu2, omega, cfac)
+             : vcycle<T>(C, H, l + 1, st, nu1, nu2, omega, cfac);
+  const T cfac_l = T((kc ? 1.0 : (double)cfac) * (cL.scale / lv.scale));
   if (nu2 == 2) {  // both post-sweeps in one pass where the level suits it
     const int out = (cur + 2) % 3;
     const bool done = l == 0 ? post2<T, true>(C, lv, lv.e[cur], lv.e[out], cL, zH, cfac_l, omega,
@@ -1038,6 +1109,7 @@ void solve_ac64(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_r
   KrylovVecs<double> &K = C->k64;
   ensure_krylov(C, K, false);
   build_hier64(C);
+  kcycle_buffers(C, C->h64);
   ensure_hist(C, p->max_iter);
   const Grid &g = C->g;
   Level<double> &f = C->h64.L[0];
@@ -1063,6 +1135,7 @@ void solve_ir(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_res
   KrylovVecs<float> &K = C->k32;
   ensure_krylov(C, K, false);
   build_hier32(C);
+  kcycle_buffers(C, C->h32);
   ensure_hist(C, p->max_iter);
   const Grid &g = C->g;
   Level<float> &f = C->h32.L[0];
@@ -1177,6 +1250,7 @@ void solve_ru(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_res
   KrylovVecs<float> &K = C->k32;
   ensure_krylov(C, K, false);
   build_hier32(C);
+  kcycle_buffers(C, C->h32);
   ensure_hist(C, p->max_iter);
   const Grid &g = C->g;
   const int chunk = chunk_of(p);
@@ -1563,6 +1637,11 @@ int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_
       destroy_graphs(C);
       C->fuse = p->ac_fuse != 0;
     }
+    if (p->ac_kcycle < 0 || (p->ac_kcycle & 1)) return PSB_EINVAL;
+    if (p->ac_kcycle != C->kmask) {  // the captured graphs hold the other cycle
+      destroy_graphs(C);
+      C->kmask = p->ac_kcycle;
+    }
     if (p->precision == PSB_MIXED && p->inner_adapt == 2)
       solve_ru(C, x, b, p, R);
     else if (p->precision == PSB_MIXED)
@@ -1649,6 +1728,7 @@ void psb_stop_defaults(psb_stop *s) {
   s->ac_coarsen = 0;
   s->ac_store = 0;
   s->ac_fuse = 0;
+  s->ac_kcycle = 0;
 }
 
 const char *psb_status_string(int s) {

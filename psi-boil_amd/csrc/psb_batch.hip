@@ -489,7 +489,7 @@ void build(psb_batch *B) {
 int batch_solve(psb_batch *B, double *const *x, const double *const *b, const psb_stop *p,
                 psb_result *R) {
   if (p->precond != PSB_PREC_AC || p->method != PSB_CG || p->precision != PSB_MIXED ||
-      p->inner_adapt != 2 || p->ac_pre < 1 || p->ac_post < 1)
+      p->inner_adapt != 2 || p->ac_pre < 1 || p->ac_post < 1 || p->ac_kcycle != 0)
     return PSB_EINVAL;
   if (stale(B)) build(B);
   const int ns = B->nsys;

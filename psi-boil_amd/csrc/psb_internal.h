@@ -126,6 +126,10 @@ struct Level {
   // flux form (FP32 levels): d = c - sum of couplings; the stencil kernels
   // then read (d, w, s, b), and c16 holds scale * d
   T *d = nullptr;
+  // K-cycle on this level: copy of the first cycle's output, s A of it,
+  // the combined correction, and the scalars rho1, alpha1, a, alpha2
+  T *kc = nullptr, *kv = nullptr, *kz = nullptr;
+  double *kst = nullptr;
   DMat<T> mat() const { return DMat<T>{c, w, s, b, nullptr, nullptr, nullptr}; }
   DMat<T> matflux() const { return DMat<T>{d, w, s, b, nullptr, nullptr, nullptr}; }
   DMat<_Float16> mat16() const {
@@ -223,6 +227,7 @@ struct psb_ctx {
   bool store_half = false;  // FP16 copies of the FP32 levels for the smoother
   bool fuse = false;        // two post-sweeps in one pass (k_post2) where it applies
   bool cg_half = false;     // the FP32 CG operator reads the level-0 FP16 copy too
+  int kmask = 0;            // K-cycle levels (bit l: level l) of the AC hierarchies
   int version = 0;     // bumped by every change of matrix, geometry or off mask
   psb::host::RefHier href;             // the reference's AC cycle
   psb::DevMetrics metrics{};           // metrics of an assembled matrix (device)

@@ -47,7 +47,7 @@ class Stop(C.Structure):
                 ("ac_post", C.c_int), ("ac_omega", C.c_double), ("ac_cfac", C.c_double),
                 ("inner_rtol", C.c_double), ("max_outer", C.c_int), ("graph_chunk", C.c_int),
                 ("inner_adapt", C.c_int), ("ac_coarsen", C.c_int),
-                ("ac_store", C.c_int), ("ac_fuse", C.c_int)]
+                ("ac_store", C.c_int), ("ac_fuse", C.c_int), ("ac_kcycle", C.c_int)]
 
 
 class Result(C.Structure):

@@ -1,0 +1,85 @@
+"""The K-cycle (psb_stop.ac_kcycle: two flexible-CG steps as the coarse
+correction of chosen levels) against the oracle's restatement of the same
+preconditioner: iteration counts +-1 and the solution to 1e-6 (the iterates
+differ only by the order of the dot-product sums) in FP64, and in the mixed
+solve the FP64 answer to the tolerance."""
+import numpy as np
+import pytest
+
+import psb
+import psb_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def demeaned_rel(a, b):
+    return rel(a - a.mean(), b - b.mean())
+
+
+def system(n, nsph=10, seed=5):
+    d = psb.generate(n, n, n, nsph, 0.05, 0.15, 1.0, 1000.0, seed, 1000 + seed)
+    s = psb.Solver(n, n, n)
+    s.set_matrix(d)
+    return d, s
+
+
+AC = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=1e-8, res_tol=0.0,
+          max_iter=300)
+
+
+@pytest.mark.parametrize("mask", [0b10, 0b110])
+def test_kcycle_fp64_matches_oracle(mask):
+    n = 64
+    print(f"[kcycle] fp64 {n}^3 mask {mask:b}", flush=True)
+    d, s = system(n)
+    b = d["rhs"]
+    x, r = s.solve(np.zeros(b.size), b, ac_kcycle=mask, **AC)
+    A = O.Mat(n, n, n, {k: d[k] for k in "cwesnbt"})
+    ac = O.AC(A, np.ones(b.size), rule=1)
+    try:
+        O.set_kcycle(mask)
+        xo, info = ac.acpcg(np.zeros(b.size), b, rtol=1e-8, maxit=300, nu1=2, nu2=2, omega=0.9,
+                            cfac=2.0)
+    finally:
+        O.set_kcycle(0)
+    assert r["converged"] and info["converged"]
+    assert abs(r["iterations"] - info["iters"]) <= 1, (r["iterations"], info["iters"])
+    assert r["rel_l2"] < 1.2e-8
+    assert demeaned_rel(x, xo) < 1e-6
+    h = s.history()
+    m = min(len(h), len(info["hist"]), 8)
+    np.testing.assert_allclose(h[:m] / h[0], info["hist"][:m], rtol=1e-6)
+
+
+def test_kcycle_mixed_reaches_the_fp64_answer():
+    n = 96
+    print(f"[kcycle] mixed {n}^3", flush=True)
+    d, s = system(n, nsph=16, seed=2)
+    b = d["rhs"]
+    x64, r64 = s.solve(np.zeros(b.size), b, precision=psb.PSB_FP64,
+                       **dict(AC, res_rat=1e-12))
+    xd, rd = s.solve(np.zeros(b.size), b, precision=psb.PSB_FP64, ac_kcycle=0b110, **AC)
+    xk, rk = s.solve(np.zeros(b.size), b, precision=psb.PSB_MIXED, inner_adapt=2,
+                     inner_rtol=1e-2, ac_store=1, ac_kcycle=0b110, **AC)
+    assert rd["converged"] and rk["converged"]
+    assert rk["rel_l2"] < 1.2e-8
+    # the FP32 iterations with FP16 smoother copies track the FP64 ones
+    assert abs(rk["iterations"] - rd["iterations"]) <= 2, (rk["iterations"], rd["iterations"])
+    ed, ek = demeaned_rel(xd, x64), demeaned_rel(xk, x64)
+    assert ek < 3.0 * ed + 1e-9, (ed, ek)
+
+
+def test_kcycle_arguments():
+    d, s = system(16, nsph=2)
+    b = d["rhs"]
+    for bad in (1, 0b11, -2):
+        with pytest.raises(psb.PsbError, match="EINVAL"):
+            s.solve(np.zeros(b.size), b, ac_kcycle=bad, **AC)
+    # bits beyond the hierarchy are ignored: the V-cycle's result
+    x0, r0 = s.solve(np.zeros(b.size), b, **AC)
+    x1, r1 = s.solve(np.zeros(b.size), b, ac_kcycle=1 << 20, **AC)
+    assert np.array_equal(x0, x1) and r0 == r1
