@@ -6,13 +6,13 @@ walls, right-hand side = divergence of a seeded uniform(-1,1) face-velocity
 field shifted to zero mean, x0 = 0.  A step is one complete solve to
 ||b - A x||_2 <= 1e-8 ||b||_2 by CG preconditioned with the additive-correction
 multigrid cycle (2+2 damped Jacobi, coarsest 8^3 exact FP64; on levels 2-4 the
-coarse correction is two flexible-CG steps, the K-cycle, --kcycle 0 gives the
-V-cycle): FP32 CG with FP64 residual updates (the value), and the plain FP64
+coarse correction is two flexible-CG steps, the K-cycle, on the levels of
+16^3 to 64^3 cells; --kcycle 0 gives the V-cycle): FP32 CG with FP64 residual updates (the value), and the plain FP64
 solve beside it; the V-cycle's two figures are reported too.  Matrix and
 right-hand side are resident in HBM before the timed region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu] [--no-ref-cycle]
-                    [--kcycle 0b11100]
+                    [--kcycle -1]
 
 `--case batch` solves 96 seeded systems of uneven sizes as one batch and
 beside it one after another.  `--case 512` and `--case cyl` print the figures
@@ -60,7 +60,7 @@ def timed_solves(s, dx, db, steps, warmup, **stop):
     return (t1 - t0) / steps, res
 
 
-def pressure_step(n, steps):
+def pressure_step(n, steps, kcycle=0):
     """One time step's pressure phase on the device, as a two-phase code runs
     it every step: Pressure::discretize from the density, update_rhs from the
     face velocities, the mixed solve (hierarchy rebuilt for the new matrix)
@@ -88,7 +88,7 @@ def pressure_step(n, steps):
     s = psb.Solver(n, n, n)
     stop = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=1e-8, res_tol=0.0,
                 max_iter=1000, precision=psb.PSB_MIXED, inner_rtol=1e-2, inner_adapt=2,
-                ac_store=1)
+                ac_store=1, ac_kcycle=kcycle)
     t = {"assemble": 0.0, "rhs": 0.0, "solve": 0.0, "project": 0.0}
     res = None
     for step in range(steps + 1):
@@ -259,10 +259,10 @@ def extra_case(a):
             "mixed_reliable_1e-2_fp16_smoother_and_cg": dict(precision=psb.PSB_MIXED,
                                                              inner_rtol=1e-2, inner_adapt=2,
                                                              ac_store=2)}
-    for km in (a.kcycle, a.kcycle << 1):  # K-cycle on the bench's levels and one lower
-        runs[f"fp64_kcycle_{km:b}"] = dict(precision=psb.PSB_FP64, ac_kcycle=km)
-        runs[f"mixed_reliable_1e-2_fp16_smoother_kcycle_{km:b}"] = dict(
-            precision=psb.PSB_MIXED, inner_rtol=1e-2, inner_adapt=2, ac_store=1, ac_kcycle=km)
+    # the K-cycle as the headline runs it (-1: on the levels of 16^3 to 64^3 cells)
+    runs["fp64_kcycle"] = dict(precision=psb.PSB_FP64, ac_kcycle=a.kcycle)
+    runs["mixed_reliable_1e-2_fp16_smoother_kcycle"] = dict(
+        precision=psb.PSB_MIXED, inner_rtol=1e-2, inner_adapt=2, ac_store=1, ac_kcycle=a.kcycle)
     dx.zero()
     r12 = s.solve_device(dx.addr, db.addr, precision=psb.PSB_FP64,
                          **dict(common, res_rat=1e-12, max_iter=4000))
@@ -374,8 +374,9 @@ def main():
                     help="leave out the reference AC::vcycle timing (rocprofv3 kernel tracing "
                          "faults on its ~1500-node captured graph)")
     ap.add_argument("--cpu-iters", type=int, default=20)
-    ap.add_argument("--kcycle", type=lambda v: int(v, 0), default=0b11100,
-                    help="K-cycle levels of the headline solves (bit l: level l; 0: V-cycle)")
+    ap.add_argument("--kcycle", type=lambda v: int(v, 0), default=-1,
+                    help="K-cycle levels of the solves (bit l: level l; -1: the levels of "
+                         "16^3 to 64^3 cells; 0: V-cycle)")
     ap.add_argument("--nsys", type=int, default=96, help="systems of --case batch")
     ap.add_argument("--case", choices=sorted(EXTRA) + ["batch", "drop64"], default=None,
                     help="figures of another single-system workload instead of the bench line")
@@ -419,7 +420,7 @@ def main():
     common = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=1e-8, res_tol=0.0,
                   max_iter=1000, ac_kcycle=a.kcycle)
     mixed = dict(precision=psb.PSB_MIXED, inner_rtol=1e-2, inner_adapt=2, ac_store=1)
-    log(f"[bench] FP64 AC-PCG, K-cycle levels {a.kcycle:b}")
+    log(f"[bench] FP64 AC-PCG, K-cycle levels {a.kcycle:#b}")
     t64, r64 = timed_solves(s, dx, db, a.steps, a.warmup, precision=psb.PSB_FP64, **common)
     log(f"[bench] fp64 {t64 * 1e3:.2f} ms/solve {r64}")
     x_fp64 = dx.download()
@@ -471,7 +472,7 @@ def main():
         ref_cycle = {"ms": round(t_vc * 1e3, 2), "cycles": rv["iterations"],
                      "status": rv["status_name"], "rel_residual": rv["rel_l2"]}
     log("[bench] pressure phase of a time step (assembly, rhs, solve, projection)")
-    step = pressure_step(n, 3)
+    step = pressure_step(n, 3, a.kcycle)
     log(f"[bench] pressure step {step}")
     cpu = None
     if not a.no_cpu:
@@ -498,10 +499,11 @@ def main():
         "config": {"workload": f"pressure Poisson {n}^3, 16 bubbles r 0.03-0.12, density "
                                "ratio 1000, Neumann walls; AC-preconditioned CG (2+2 damped "
                                "Jacobi, 2x coarsening to 8^3, exact FP64 coarsest, two "
-                               "flexible-CG steps as the coarse correction of levels "
-                               f"{a.kcycle:b} (bit = level)) to 1e-8; FP32 CG with an FP64 "
-                               "true-residual update per 1e-2 drop, FP16 copy of the level "
-                               "matrices in the smoother",
+                               "flexible-CG steps as the coarse correction of "
+                               + ("the levels of 64^3 to 16^3 cells" if a.kcycle == -1 else
+                                  f"levels {a.kcycle:#b} (bit = level)")
+                               + ") to 1e-8; FP32 CG with an FP64 true-residual update per "
+                               "1e-2 drop, FP16 copy of the level matrices in the smoother",
                    "grid": [n, n, n], "parallelism": "single GPU"},
         "fp64_solves_per_s": round(1.0 / t64, 3),
         "fp64_ms_per_solve": round(t64 * 1e3, 3),

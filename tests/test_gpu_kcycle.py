@@ -73,6 +73,18 @@ def test_kcycle_mixed_reaches_the_fp64_answer():
     assert ek < 3.0 * ed + 1e-9, (ed, ek)
 
 
+def test_kcycle_levels_chosen_by_size():
+    """ac_kcycle = -1 takes the levels of 16^3 to 64^3 cells between the finest
+    and the coarsest: at 128^3 (levels 128, 64, 32, 16, 8) levels 1-3."""
+    n = 128
+    print(f"[kcycle] auto {n}^3", flush=True)
+    d, s = system(n, nsph=12, seed=3)
+    b = d["rhs"]
+    xa, ra = s.solve(np.zeros(b.size), b, ac_kcycle=-1, **AC)
+    xm, rm = s.solve(np.zeros(b.size), b, ac_kcycle=0b1110, **AC)
+    assert np.array_equal(xa, xm) and ra == rm
+
+
 def test_kcycle_arguments():
     d, s = system(16, nsph=2)
     b = d["rhs"]
