@@ -184,6 +184,9 @@ EXTRA = {
                                                 rho_out=1000.0, seed_sph=5, seed_vel=1005,
                                                 stretch_z=2.5, lz=2.0, cylinder=True, cyl_d=0.2),
                 tol=1e-9, stop=dict(ac_coarsen=1, ac_cfac=1.0),
+                # the K-cycle on levels 1-4 (192x192x384 down to 24x24x48): 709 ms against
+                # 766 on levels 2-4 and 3174 on the size-chosen levels 3-4
+                kcycle=0b11110,
                 desc="pressure Poisson 384x384x768 on [0,1]x[0,1]x[0,2], tanh z-stretching "
                      "2.5, solid cylinder d 0.2 along y, 128 bubbles r 0.01-0.04, density "
                      "ratio 1000; AC-PCG to 1e-9 (levels halve x and y before z, coarse "
@@ -259,17 +262,19 @@ def extra_case(a):
             "mixed_reliable_1e-2_fp16_smoother_and_cg": dict(precision=psb.PSB_MIXED,
                                                              inner_rtol=1e-2, inner_adapt=2,
                                                              ac_store=2)}
-    # the K-cycle as the headline runs it (-1: on the levels of 16^3 to 64^3 cells)
-    runs["fp64_kcycle"] = dict(precision=psb.PSB_FP64, ac_kcycle=a.kcycle)
+    # the K-cycle (as the headline runs it, -1: on the levels of 16^3 to 64^3 cells,
+    # unless the workload names its levels)
+    km = E.get("kcycle", a.kcycle)
+    runs["fp64_kcycle"] = dict(precision=psb.PSB_FP64, ac_kcycle=km)
     runs["mixed_reliable_1e-2_fp16_smoother_kcycle"] = dict(
-        precision=psb.PSB_MIXED, inner_rtol=1e-2, inner_adapt=2, ac_store=1, ac_kcycle=a.kcycle)
+        precision=psb.PSB_MIXED, inner_rtol=1e-2, inner_adapt=2, ac_store=1, ac_kcycle=km)
     dx.zero()
     r12 = s.solve_device(dx.addr, db.addr, precision=psb.PSB_FP64,
                          **dict(common, res_rat=1e-12, max_iter=4000))
     x_ref = dx.download()
     x_ref -= x_ref.mean()
     log(f"[bench] fp64 to 1e-12 {r12}")
-    out = {"workload": E["desc"], "grid": list(E["grid"]), "tolerance": E["tol"],
+    out = {"workload": E["desc"], "grid": list(E["grid"]), "tolerance": E["tol"], "kcycle": km,
            "fp64_1e-12": {"iterations": r12["iterations"], "rel_l2": r12["rel_l2"]}}
     for name, kw in runs.items():
         t, r = timed_solves(s, dx, db, a.steps, a.warmup, **dict(common, **kw))
@@ -374,6 +379,10 @@ def main():
                     help="leave out the reference AC::vcycle timing (rocprofv3 kernel tracing "
                          "faults on its ~1500-node captured graph)")
     ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--graph-chunk", type=int, default=8,
+                    help="CG iterations per captured graph of the headline solves (rocprofv3 "
+                         "kernel tracing faults instantiating the 8-iteration K-cycle graph; "
+                         "profile with 2)")
     ap.add_argument("--kcycle", type=lambda v: int(v, 0), default=-1,
                     help="K-cycle levels of the solves (bit l: level l; -1: the levels of "
                          "16^3 to 64^3 cells; 0: V-cycle)")
@@ -418,7 +427,7 @@ def main():
     db = psb.DeviceArray.from_host(d["rhs"])
     dx = psb.DeviceArray(n ** 3)
     common = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=1e-8, res_tol=0.0,
-                  max_iter=1000, ac_kcycle=a.kcycle)
+                  max_iter=1000, ac_kcycle=a.kcycle, graph_chunk=a.graph_chunk)
     mixed = dict(precision=psb.PSB_MIXED, inner_rtol=1e-2, inner_adapt=2, ac_store=1)
     log(f"[bench] FP64 AC-PCG, K-cycle levels {a.kcycle:#b}")
     t64, r64 = timed_solves(s, dx, db, a.steps, a.warmup, precision=psb.PSB_FP64, **common)
