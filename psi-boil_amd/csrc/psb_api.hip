@@ -705,7 +705,8 @@ void kcycle_buffers(psb_ctx *C, Hier<T> &H) {
 }
 
 // level l's correction for its right-hand side lv.r (first sweep already in
-// lv.e[0]) by two flexible-CG steps preconditioned by the cycle on l:
+// lv.e[0]) by two flexible-CG steps preconditioned by the cycle on l (their
+// reductions summed by the kernels that use them, no finalize launches):
 // c = B r, v = s A c, alpha1 = c.r / c.v; rt = r - alpha1 v (in lv.r);
 // d = B rt; gamma = d.v, beta = d.s A d, alpha2 = d.rt / (beta - gamma^2 /
 // c.v); z = (alpha1 - gamma alpha2 / c.v) c + alpha2 d
@@ -726,11 +727,10 @@ T *kstep(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega,
       hipLaunchKernelGGL((k_kc_av<T, V, false>), dim3(nb), dim3(kBlock), 0, C->stream, g,
                          lv.mat(), s, c, lv.r, lv.kc, lv.kv, st, C->work);
   });
-  finalize<2>(C, FIN_K1, nb, st, nullptr, lv.kst);
-  with_v<T>(g, [&](auto vt) {
+  with_v<T>(g, [&](auto vt) {  // sums k_kc_av's partials itself
     constexpr int V = decltype(vt)::value;
     hipLaunchKernelGGL((k_kc_r<T, V>), dim3(nb), dim3(kBlock), 0, C->stream, g, lv.c, lv.kv,
-                       lv.r, lv.e[0], T(omega / lv.scale), lv.kst, st);
+                       lv.r, lv.e[0], T(omega / lv.scale), lv.kst, nb, C->work.partial, st);
   });
   T *d = vcycle<T>(C, H, l, st, nu1, nu2, omega, cfac);
   with_v<T>(g, [&](auto vt) {
@@ -742,11 +742,10 @@ T *kstep(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega,
       hipLaunchKernelGGL((k_kc_dw<T, V, false>), dim3(nb), dim3(kBlock), 0, C->stream, g,
                          lv.mat(), s, d, lv.kv, lv.r, st, C->work);
   });
-  finalize<3>(C, FIN_K2, nb, st, nullptr, lv.kst);
-  with_v<T>(g, [&](auto vt) {
+  with_v<T>(g, [&](auto vt) {  // sums k_kc_dw's partials itself
     constexpr int V = decltype(vt)::value;
     hipLaunchKernelGGL((k_kc_comb<T, V>), dim3(nb), dim3(kBlock), 0, C->stream, g, lv.kc, d,
-                       lv.kz, lv.kst, st);
+                       lv.kz, lv.kst, nb, C->work.partial, st);
   });
   return lv.kz;
 }

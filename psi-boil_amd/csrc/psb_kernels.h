@@ -222,90 +222,6 @@ __global__ __launch_bounds__(kBlock) void k_rbsgs(Grid g, DMat<T> A, T *z, const
 }
 
 // ---------------------------------------------------------------------------
-// K-cycle on a coarse level (two flexible-CG steps preconditioned by the
-// cycle below; Y. Notay and P. S. Vassilevski, Numer. Linear Algebra Appl. 15
-// (2008) 473-487).  The level's corrections are held divided by its FP16
-// scale s, so its operator here is s A (A the FP32 or FP64 level matrix).
-// k_kc_av: v = s A c, kc = c, partials c.v and c.r (FIN_K1: rho1, alpha1)
-// ---------------------------------------------------------------------------
-template <typename T, int V, bool FLUX>
-__global__ __launch_bounds__(kBlock) void k_kc_av(Grid g, DMat<T> A, T scale, const T *c,
-                                                  const T *r, T *kc, T *v, const DevState *st,
-                                                  Work w) {
-  if (st->done) return;
-  ArrX<T, V> X{c};
-  double acc[2] = {0.0, 0.0};
-  PSB_TILE_BEGIN(g, V)
-  Gath<T, V> G = gather<T, V, true, FLUX>(g, A, X, P, i, j, k);
-  VecN<T, V> Av = apply_A<T, V>(G), R = ldv<T, V>(r + P);
-#pragma unroll
-  for (int u = 0; u < V; u++) {
-    Av.v[u] = scale * Av.v[u];
-    acc[0] += (double)G.x.v[u] * (double)Av.v[u];
-    acc[1] += (double)G.x.v[u] * (double)R.v[u];
-  }
-  stv<T, V>(v + P, Av);
-  stv<T, V>(kc + P, G.x);
-  PSB_TILE_END
-  block_partial<2>(acc, w.partial);
-}
-
-// r <- r - alpha1 v, and the next cycle's first sweep e0 = omega r / c
-template <typename T, int V>
-__global__ __launch_bounds__(kBlock) void k_kc_r(Grid g, const T *c, const T *v, T *r, T *e0,
-                                                 T omega, const double *kst,
-                                                 const DevState *st) {
-  if (st->done) return;
-  const T a1 = (T)kst[1];
-  PSB_TILE_BEGIN(g, V)
-  VecN<T, V> R = ldv<T, V>(r + P), Vv = ldv<T, V>(v + P), Cc = ldv<T, V>(c + P), E;
-#pragma unroll
-  for (int u = 0; u < V; u++) {
-    R.v[u] = R.v[u] - a1 * Vv.v[u];
-    E.v[u] = omega * R.v[u] / Cc.v[u];
-  }
-  stv<T, V>(r + P, R);
-  stv<T, V>(e0 + P, E);
-  PSB_TILE_END
-}
-
-// partials d.v, d.(s A d), d.r (FIN_K2: the combination's coefficients)
-template <typename T, int V, bool FLUX>
-__global__ __launch_bounds__(kBlock) void k_kc_dw(Grid g, DMat<T> A, T scale, const T *d,
-                                                  const T *v, const T *r, const DevState *st,
-                                                  Work w) {
-  if (st->done) return;
-  ArrX<T, V> X{d};
-  double acc[3] = {0.0, 0.0, 0.0};
-  PSB_TILE_BEGIN(g, V)
-  Gath<T, V> G = gather<T, V, true, FLUX>(g, A, X, P, i, j, k);
-  VecN<T, V> Ad = apply_A<T, V>(G), Vv = ldv<T, V>(v + P), R = ldv<T, V>(r + P);
-#pragma unroll
-  for (int u = 0; u < V; u++) {
-    const double dd = (double)G.x.v[u];
-    acc[0] += dd * (double)Vv.v[u];
-    acc[1] += dd * (double)(scale * Ad.v[u]);
-    acc[2] += dd * (double)R.v[u];
-  }
-  PSB_TILE_END
-  block_partial<3>(acc, w.partial);
-}
-
-// out = a kc + alpha2 d
-template <typename T, int V>
-__global__ __launch_bounds__(kBlock) void k_kc_comb(Grid g, const T *kc, const T *d, T *out,
-                                                    const double *kst, const DevState *st) {
-  if (st->done) return;
-  const T a = (T)kst[2], a2 = (T)kst[3];
-  PSB_TILE_BEGIN(g, V)
-  VecN<T, V> Kc = ldv<T, V>(kc + P), D = ldv<T, V>(d + P), O;
-#pragma unroll
-  for (int u = 0; u < V; u++) O.v[u] = a * Kc.v[u] + a2 * D.v[u];
-  stv<T, V>(out + P, O);
-  PSB_TILE_END
-}
-
-// ---------------------------------------------------------------------------
 // damped Jacobi sweep  e_out = e + omega (r - A e) / c, the operand either e
 // or e + cfac P eH (prolongation fused into the first post-sweep).
 // RHO (last post-sweep on level 0): partials r.z, sum z, sum r, active
@@ -732,7 +648,7 @@ __global__ __launch_bounds__(kBlock) void k_bi3(Grid g, const T *c, Metric m, co
 enum FinOp {
   FIN_START_DIAG, FIN_START_AC, FIN_START_BICGS, FIN_PQ, FIN_XR_DIAG, FIN_XR_AC, FIN_RHO,
   FIN_XR_RU, FIN_IR_RESID, FIN_CENTER, FIN_RELRES, FIN_BI1, FIN_BI2, FIN_BI3, FIN_RU_START,
-  FIN_RU, FIN_RU_END, FIN_ACMEAN, FIN_ACRES, FIN_K1, FIN_K2
+  FIN_RU, FIN_RU_END, FIN_ACMEAN, FIN_ACRES
 };
 
 // stop logic shared by the Krylov loops (cg.cpp:121-160, bicgs.cpp:174-210),
@@ -803,7 +719,6 @@ __device__ inline void krylov_init(DevState *st, double res, double rsq, double 
 __device__ inline bool fin_skip(int op, const DevState *st) {
   switch (op) {
     case FIN_PQ: case FIN_XR_DIAG: case FIN_XR_AC: case FIN_RHO: case FIN_XR_RU:
-    case FIN_K1: case FIN_K2:
     case FIN_BI1: case FIN_BI2: case FIN_BI3:
       return st->done != 0;
     case FIN_RU:
@@ -948,18 +863,6 @@ __device__ inline void fin_apply(int op, const double *tot, DevState *st, DevSta
       out[0] = tot[0];
       out[1] = tot[1];
       break;
-    case FIN_K1:  // K-cycle: rho1 = c.v, alpha1 = c.r / rho1 (out: the level's scalars)
-      out[0] = tot[0];
-      out[1] = tot[0] > 0.0 ? tot[1] / tot[0] : 0.0;
-      break;
-    case FIN_K2: {  // gamma = d.v, beta = d.A d, delta = d.rt
-      const double rho1 = out[0], a1 = out[1];
-      const double den = rho1 > 0.0 ? tot[1] - tot[0] * tot[0] / rho1 : 0.0;
-      const double a2 = den > 0.0 ? tot[2] / den : 0.0;
-      out[2] = a1 - (rho1 > 0.0 ? tot[0] * a2 / rho1 : 0.0);
-      out[3] = a2;
-      break;
-    }
     case FIN_BI1:
       st->rv = tot[0];
       st->alpha = st->rho / tot[0];  // bicgs.cpp:122-123
@@ -1007,6 +910,124 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int op, int nblk, const dou
   fin_sum<NR>(0, nblk, partial, acc);
   if (threadIdx.x == 0) fin_apply(op, acc, st, st2, out);
 }
+
+// ---------------------------------------------------------------------------
+// K-cycle on a coarse level (two flexible-CG steps preconditioned by the
+// cycle below; Y. Notay and P. S. Vassilevski, Numer. Linear Algebra Appl. 15
+// (2008) 473-487).  The level's corrections are held divided by its FP16
+// scale s, so its operator here is s A (A the FP32 or FP64 level matrix).
+// k_kc_av: v = s A c, kc = c, partials c.v and c.r
+// ---------------------------------------------------------------------------
+template <typename T, int V, bool FLUX>
+__global__ __launch_bounds__(kBlock) void k_kc_av(Grid g, DMat<T> A, T scale, const T *c,
+                                                  const T *r, T *kc, T *v, const DevState *st,
+                                                  Work w) {
+  if (st->done) return;
+  ArrX<T, V> X{c};
+  double acc[2] = {0.0, 0.0};
+  PSB_TILE_BEGIN(g, V)
+  Gath<T, V> G = gather<T, V, true, FLUX>(g, A, X, P, i, j, k);
+  VecN<T, V> Av = apply_A<T, V>(G), R = ldv<T, V>(r + P);
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    Av.v[u] = scale * Av.v[u];
+    acc[0] += (double)G.x.v[u] * (double)Av.v[u];
+    acc[1] += (double)G.x.v[u] * (double)R.v[u];
+  }
+  stv<T, V>(v + P, Av);
+  stv<T, V>(kc + P, G.x);
+  PSB_TILE_END
+  block_partial<2>(acc, w.partial);
+}
+
+// r <- r - alpha1 v, and the next cycle's first sweep e0 = omega r / c.  Every
+// block sums k_kc_av's nprev partials itself (in k_finalize's order; a few
+// hundred at most on these levels) instead of a finalize launch: rho1 =
+// c.v, alpha1 = c.r / rho1, written to kst[0..1] by block 0
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void k_kc_r(Grid g, const T *c, const T *v, T *r, T *e0,
+                                                 T omega, double *kst, int nprev,
+                                                 const double *partial, const DevState *st) {
+  if (st->done) return;
+  __shared__ double a1s;
+  double tot[2];
+  fin_sum<2>(0, nprev, partial, tot);
+  if (threadIdx.x == 0) {
+    a1s = tot[0] > 0.0 ? tot[1] / tot[0] : 0.0;
+    if (blockIdx.x == 0) {
+      kst[0] = tot[0];
+      kst[1] = a1s;
+    }
+  }
+  __syncthreads();
+  const T a1 = (T)a1s;
+  PSB_TILE_BEGIN(g, V)
+  VecN<T, V> R = ldv<T, V>(r + P), Vv = ldv<T, V>(v + P), Cc = ldv<T, V>(c + P), E;
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    R.v[u] = R.v[u] - a1 * Vv.v[u];
+    E.v[u] = omega * R.v[u] / Cc.v[u];
+  }
+  stv<T, V>(r + P, R);
+  stv<T, V>(e0 + P, E);
+  PSB_TILE_END
+}
+
+// partials d.v, d.(s A d), d.r
+template <typename T, int V, bool FLUX>
+__global__ __launch_bounds__(kBlock) void k_kc_dw(Grid g, DMat<T> A, T scale, const T *d,
+                                                  const T *v, const T *r, const DevState *st,
+                                                  Work w) {
+  if (st->done) return;
+  ArrX<T, V> X{d};
+  double acc[3] = {0.0, 0.0, 0.0};
+  PSB_TILE_BEGIN(g, V)
+  Gath<T, V> G = gather<T, V, true, FLUX>(g, A, X, P, i, j, k);
+  VecN<T, V> Ad = apply_A<T, V>(G), Vv = ldv<T, V>(v + P), R = ldv<T, V>(r + P);
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    const double dd = (double)G.x.v[u];
+    acc[0] += dd * (double)Vv.v[u];
+    acc[1] += dd * (double)(scale * Ad.v[u]);
+    acc[2] += dd * (double)R.v[u];
+  }
+  PSB_TILE_END
+  block_partial<3>(acc, w.partial);
+}
+
+// out = a kc + alpha2 d, the coefficients from k_kc_dw's nprev partials
+// (gamma = d.v, beta = d.s A d, delta = d.rt) summed by every block, with
+// rho1 and alpha1 from kst: alpha2 = delta / (beta - gamma^2 / rho1),
+// a = alpha1 - gamma alpha2 / rho1 (block 0 keeps them in kst[2..3])
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void k_kc_comb(Grid g, const T *kc, const T *d, T *out,
+                                                    double *kst, int nprev,
+                                                    const double *partial, const DevState *st) {
+  if (st->done) return;
+  __shared__ double cs[2];
+  double tot[3];
+  fin_sum<3>(0, nprev, partial, tot);
+  if (threadIdx.x == 0) {
+    const double rho1 = kst[0], a1 = kst[1];
+    const double den = rho1 > 0.0 ? tot[1] - tot[0] * tot[0] / rho1 : 0.0;
+    const double a2 = den > 0.0 ? tot[2] / den : 0.0;
+    cs[0] = a1 - (rho1 > 0.0 ? tot[0] * a2 / rho1 : 0.0);
+    cs[1] = a2;
+    if (blockIdx.x == 0) {
+      kst[2] = cs[0];
+      kst[3] = cs[1];
+    }
+  }
+  __syncthreads();
+  const T a = (T)cs[0], a2 = (T)cs[1];
+  PSB_TILE_BEGIN(g, V)
+  VecN<T, V> Kc = ldv<T, V>(kc + P), D = ldv<T, V>(d + P), O;
+#pragma unroll
+  for (int u = 0; u < V; u++) O.v[u] = a * Kc.v[u] + a2 * D.v[u];
+  stv<T, V>(out + P, O);
+  PSB_TILE_END
+}
+
 
 // ---------------------------------------------------------------------------
 // the reference's additive-correction cycle (AC::v_cycle,
