@@ -130,3 +130,16 @@ def test_batched_plan_follows_matrix_changes():
     assert rc == psb.PSB_OK and res[0]["rel_l2"] <= 1e-8 * 1.01
     A = O.Mat(nx, ny, nz, {k: d[k] for k in "cwesnbt"})
     assert np.linalg.norm(O.resid(A, d["rhs"], xs[0])) <= 1.01e-8 * np.linalg.norm(d["rhs"])
+
+
+def test_batched_solve_rejects_the_kcycle():
+    """The batched solver runs the V-cycle only: a K-cycle request is an
+    invalid argument, not a silent V-cycle."""
+    B, data = make_batch(SHAPES[:2], seed=61)
+    bs = [d["rhs"] for d in data]
+    for km in (-1, 0b100):
+        with pytest.raises(psb.PsbError, match="EINVAL"):
+            B.solve([np.zeros(b.size) for b in bs], bs, res_rat=1e-8, max_iter=300,
+                    ac_kcycle=km)
+    xs, res, rc = B.solve([np.zeros(b.size) for b in bs], bs, res_rat=1e-8, max_iter=300)
+    assert rc == psb.PSB_OK
