@@ -919,13 +919,11 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int op, int nblk, const dou
 // k_kc_av: v = s A c, kc = c, partials c.v and c.r
 // ---------------------------------------------------------------------------
 template <typename T, int V, bool FLUX>
-__global__ __launch_bounds__(kBlock) void k_kc_av(Grid g, DMat<T> A, T scale, const T *c,
-                                                  const T *r, T *kc, T *v, const DevState *st,
-                                                  Work w) {
-  if (st->done) return;
+__device__ __forceinline__ void kc_av_body(Span sp, const Grid &g, const DMat<T> &A, T scale,
+                                           const T *c, const T *r, T *kc, T *v, Work w) {
   ArrX<T, V> X{c};
   double acc[2] = {0.0, 0.0};
-  PSB_TILE_BEGIN(g, V)
+  PSB_SPAN_BEGIN(g, V, sp)
   Gath<T, V> G = gather<T, V, true, FLUX>(g, A, X, P, i, j, k);
   VecN<T, V> Av = apply_A<T, V>(G), R = ldv<T, V>(r + P);
 #pragma unroll
@@ -940,15 +938,22 @@ __global__ __launch_bounds__(kBlock) void k_kc_av(Grid g, DMat<T> A, T scale, co
   block_partial<2>(acc, w.partial);
 }
 
+template <typename T, int V, bool FLUX>
+__global__ __launch_bounds__(kBlock) void k_kc_av(Grid g, DMat<T> A, T scale, const T *c,
+                                                  const T *r, T *kc, T *v, const DevState *st,
+                                                  Work w) {
+  if (st->done) return;
+  kc_av_body<T, V, FLUX>(grid_span(), g, A, scale, c, r, kc, v, w);
+}
+
 // r <- r - alpha1 v, and the next cycle's first sweep e0 = omega r / c.  Every
 // block sums k_kc_av's nprev partials itself (in k_finalize's order; a few
 // hundred at most on these levels) instead of a finalize launch: rho1 =
 // c.v, alpha1 = c.r / rho1, written to kst[0..1] by block 0
 template <typename T, int V>
-__global__ __launch_bounds__(kBlock) void k_kc_r(Grid g, const T *c, const T *v, T *r, T *e0,
-                                                 T omega, double *kst, int nprev,
-                                                 const double *partial, const DevState *st) {
-  if (st->done) return;
+__device__ __forceinline__ void kc_r_body(Span sp, const Grid &g, const T *c, const T *v, T *r,
+                                          T *e0, T omega, double *kst, int nprev,
+                                          const double *partial) {
   __shared__ double a1s;
   double tot[2];
   fin_sum<2>(0, nprev, partial, tot);
@@ -961,7 +966,7 @@ __global__ __launch_bounds__(kBlock) void k_kc_r(Grid g, const T *c, const T *v,
   }
   __syncthreads();
   const T a1 = (T)a1s;
-  PSB_TILE_BEGIN(g, V)
+  PSB_SPAN_BEGIN(g, V, sp)
   VecN<T, V> R = ldv<T, V>(r + P), Vv = ldv<T, V>(v + P), Cc = ldv<T, V>(c + P), E;
 #pragma unroll
   for (int u = 0; u < V; u++) {
@@ -973,15 +978,21 @@ __global__ __launch_bounds__(kBlock) void k_kc_r(Grid g, const T *c, const T *v,
   PSB_TILE_END
 }
 
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void k_kc_r(Grid g, const T *c, const T *v, T *r, T *e0,
+                                                 T omega, double *kst, int nprev,
+                                                 const double *partial, const DevState *st) {
+  if (st->done) return;
+  kc_r_body<T, V>(grid_span(), g, c, v, r, e0, omega, kst, nprev, partial);
+}
+
 // partials d.v, d.(s A d), d.r
 template <typename T, int V, bool FLUX>
-__global__ __launch_bounds__(kBlock) void k_kc_dw(Grid g, DMat<T> A, T scale, const T *d,
-                                                  const T *v, const T *r, const DevState *st,
-                                                  Work w) {
-  if (st->done) return;
+__device__ __forceinline__ void kc_dw_body(Span sp, const Grid &g, const DMat<T> &A, T scale,
+                                           const T *d, const T *v, const T *r, Work w) {
   ArrX<T, V> X{d};
   double acc[3] = {0.0, 0.0, 0.0};
-  PSB_TILE_BEGIN(g, V)
+  PSB_SPAN_BEGIN(g, V, sp)
   Gath<T, V> G = gather<T, V, true, FLUX>(g, A, X, P, i, j, k);
   VecN<T, V> Ad = apply_A<T, V>(G), Vv = ldv<T, V>(v + P), R = ldv<T, V>(r + P);
 #pragma unroll
@@ -995,15 +1006,22 @@ __global__ __launch_bounds__(kBlock) void k_kc_dw(Grid g, DMat<T> A, T scale, co
   block_partial<3>(acc, w.partial);
 }
 
+template <typename T, int V, bool FLUX>
+__global__ __launch_bounds__(kBlock) void k_kc_dw(Grid g, DMat<T> A, T scale, const T *d,
+                                                  const T *v, const T *r, const DevState *st,
+                                                  Work w) {
+  if (st->done) return;
+  kc_dw_body<T, V, FLUX>(grid_span(), g, A, scale, d, v, r, w);
+}
+
 // out = a kc + alpha2 d, the coefficients from k_kc_dw's nprev partials
 // (gamma = d.v, beta = d.s A d, delta = d.rt) summed by every block, with
 // rho1 and alpha1 from kst: alpha2 = delta / (beta - gamma^2 / rho1),
 // a = alpha1 - gamma alpha2 / rho1 (block 0 keeps them in kst[2..3])
 template <typename T, int V>
-__global__ __launch_bounds__(kBlock) void k_kc_comb(Grid g, const T *kc, const T *d, T *out,
-                                                    double *kst, int nprev,
-                                                    const double *partial, const DevState *st) {
-  if (st->done) return;
+__device__ __forceinline__ void kc_comb_body(Span sp, const Grid &g, const T *kc, const T *d,
+                                             T *out, double *kst, int nprev,
+                                             const double *partial) {
   __shared__ double cs[2];
   double tot[3];
   fin_sum<3>(0, nprev, partial, tot);
@@ -1020,7 +1038,7 @@ __global__ __launch_bounds__(kBlock) void k_kc_comb(Grid g, const T *kc, const T
   }
   __syncthreads();
   const T a = (T)cs[0], a2 = (T)cs[1];
-  PSB_TILE_BEGIN(g, V)
+  PSB_SPAN_BEGIN(g, V, sp)
   VecN<T, V> Kc = ldv<T, V>(kc + P), D = ldv<T, V>(d + P), O;
 #pragma unroll
   for (int u = 0; u < V; u++) O.v[u] = a * Kc.v[u] + a2 * D.v[u];
@@ -1028,6 +1046,13 @@ __global__ __launch_bounds__(kBlock) void k_kc_comb(Grid g, const T *kc, const T
   PSB_TILE_END
 }
 
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void k_kc_comb(Grid g, const T *kc, const T *d, T *out,
+                                                    double *kst, int nprev,
+                                                    const double *partial, const DevState *st) {
+  if (st->done) return;
+  kc_comb_body<T, V>(grid_span(), g, kc, d, out, kst, nprev, partial);
+}
 
 // ---------------------------------------------------------------------------
 // the reference's additive-correction cycle (AC::v_cycle,
