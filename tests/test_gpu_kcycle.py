@@ -95,3 +95,34 @@ def test_kcycle_arguments():
     x0, r0 = s.solve(np.zeros(b.size), b, **AC)
     x1, r1 = s.solve(np.zeros(b.size), b, ac_kcycle=1 << 20, **AC)
     assert np.array_equal(x0, x1) and r0 == r1
+
+
+def test_kcycle_on_semi_coarsened_levels_matches_oracle():
+    """The K-cycle on the hierarchy that halves x and y before z
+    (ac_coarsen = 1, the oracle's rule 3) of a z-stretched grid with solid
+    cells: FP64 iteration counts +-1 and solutions against the oracle."""
+    nx, ny, nz = 64, 32, 96
+    print("[kcycle] semi-coarsened", flush=True)
+    d = psb.generate(nx, ny, nz, 8, 0.02, 0.06, 1.0, 1000.0, 17, 1017, stretch_z=2.5, lz=2.0,
+                     cylinder=True, cyl_d=0.2)
+    A = O.Mat(nx, ny, nz, {k: d[k] for k in "cwesnbt"})
+    b, n = d["rhs"], d["rhs"].size
+    s = psb.Solver(nx, ny, nz)
+    if d["off"].any():
+        s.set_active(d["off"])
+    s.set_matrix(d)
+    mask = 0b110
+    x, r = s.solve(np.zeros(n), b, ac_coarsen=1, ac_cfac=1.0, ac_kcycle=mask,
+                   **dict(AC, max_iter=400))
+    off = d["off"] if d["off"].any() else None
+    ac = O.AC(A, np.ones(n), off=off, rule=3)
+    try:
+        O.set_kcycle(mask)
+        xo, info = ac.acpcg(np.zeros(n), b, rtol=1e-8, maxit=400, nu1=2, nu2=2, omega=0.9,
+                            cfac=1.0)
+    finally:
+        O.set_kcycle(0)
+    assert r["converged"] and info["converged"]
+    assert abs(r["iterations"] - info["iters"]) <= 1, (r["iterations"], info["iters"])
+    fl = d["off"] == 0
+    assert demeaned_rel(x[fl], xo[fl]) < 1e-6
