@@ -184,9 +184,6 @@ EXTRA = {
                                                 rho_out=1000.0, seed_sph=5, seed_vel=1005,
                                                 stretch_z=2.5, lz=2.0, cylinder=True, cyl_d=0.2),
                 tol=1e-9, stop=dict(ac_coarsen=1, ac_cfac=1.0),
-                # the K-cycle on levels 1-4 (192x192x384 down to 24x24x48): 709 ms against
-                # 766 on levels 2-4 and 3174 on the size-chosen levels 3-4
-                kcycle=0b11110,
                 desc="pressure Poisson 384x384x768 on [0,1]x[0,1]x[0,2], tanh z-stretching "
                      "2.5, solid cylinder d 0.2 along y, 128 bubbles r 0.01-0.04, density "
                      "ratio 1000; AC-PCG to 1e-9 (levels halve x and y before z, coarse "
@@ -262,9 +259,8 @@ def extra_case(a):
             "mixed_reliable_1e-2_fp16_smoother_and_cg": dict(precision=psb.PSB_MIXED,
                                                              inner_rtol=1e-2, inner_adapt=2,
                                                              ac_store=2)}
-    # the K-cycle (as the headline runs it, -1: on the levels of 16^3 to 64^3 cells,
-    # unless the workload names its levels)
-    km = E.get("kcycle", a.kcycle)
+    # the K-cycle as the headline runs it (-1: levels chosen from the hierarchy)
+    km = a.kcycle
     runs["fp64_kcycle"] = dict(precision=psb.PSB_FP64, ac_kcycle=km)
     runs["mixed_reliable_1e-2_fp16_smoother_kcycle"] = dict(
         precision=psb.PSB_MIXED, inner_rtol=1e-2, inner_adapt=2, ac_store=1, ac_kcycle=km)
@@ -384,8 +380,9 @@ def main():
                          "kernel tracing faults instantiating the 8-iteration K-cycle graph; "
                          "profile with 2)")
     ap.add_argument("--kcycle", type=lambda v: int(v, 0), default=-1,
-                    help="K-cycle levels of the solves (bit l: level l; -1: the levels of "
-                         "16^3 to 64^3 cells; 0: V-cycle)")
+                    help="K-cycle levels of the solves (bit l: level l; -1: chosen from the "
+                         "hierarchy, the levels of 16^3 to 64^3 cells, or the first four "
+                         "coarse levels of a semi-coarsened one; 0: V-cycle)")
     ap.add_argument("--nsys", type=int, default=96, help="systems of --case batch")
     ap.add_argument("--case", choices=sorted(EXTRA) + ["batch", "drop64"], default=None,
                     help="figures of another single-system workload instead of the bench line")

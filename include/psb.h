@@ -130,9 +130,11 @@ typedef struct {
                             Vassilevski 2008) instead of cfac times one
                             cycle.  Bits of the coarsest level or of levels
                             the hierarchy does not have are ignored.
-                            -1: chosen by size, every level between the
-                            finest and the coarsest with 16^3 to 64^3
-                            cells (the benchmark's setting).
+                            -1: chosen from the hierarchy: halving every
+                            direction (ac_coarsen = 0), every level between
+                            the finest and the coarsest with 16^3 to 64^3
+                            cells; halving x and y first (ac_coarsen = 1),
+                            levels 1-4 (the benchmark's setting).
                             0 (default): the V-cycle.                       */
 } psb_stop;
 

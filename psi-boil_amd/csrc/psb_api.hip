@@ -671,22 +671,29 @@ void build_hier32(psb_ctx *C) {
 template <typename T>
 T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega, T cfac);
 
-// K-cycle levels chosen by size (psb_stop.ac_kcycle = -1): every level but
-// the finest and the coarsest with 16^3 to 64^3 cells.  Measured on MI355X
-// (mixed solves to 1e-8): at 256^3 levels 2-4 take 16 iterations and
-// 19.1 ms against the V-cycle's 36 and 27.7 ms, with level 1 added 12 and
-// 22.4 ms; at 512^3 levels 3-5 66 iterations and 352 ms against 113 and
-// 548 ms, levels 2-5 89 and 515 ms.  Larger levels cost more per visit than
-// the iterations they save; smaller ones are a few launches of a few
-// microseconds each, doubled at every K level above them.
-constexpr int kKcycleMinCells = 1 << 12, kKcycleMaxCells = 1 << 18;
+// K-cycle levels chosen from the hierarchy (psb_stop.ac_kcycle = -1).
+// Halving every direction: every level but the finest and the coarsest with
+// 16^3 to 64^3 cells.  Measured on MI355X (mixed solves to 1e-8): at 256^3
+// levels 2-4 take 16 iterations and 19.1 ms against the V-cycle's 36 and
+// 27.7 ms, with level 1 added 12 and 22.4 ms; at 512^3 levels 3-5 66
+// iterations and 352 ms against 113 and 548 ms, levels 2-5 89 and 515 ms.
+// Larger levels cost more per visit than the iterations they save; smaller
+// ones are a few launches of a few microseconds each, doubled at every K
+// level above them.  Halving x and y before z (a z-stretched grid): the
+// first four coarse levels, where that coarsening leaves the correction
+// weakest; on the 384x384x768 cylinder case 60 iterations and 709 ms against
+// 535 and 3201 ms, and 368 iterations with the levels of the sizes above.
+constexpr int kKcycleMinCells = 1 << 12, kKcycleMaxCells = 1 << 18, kKcycleSemiLevels = 4;
 
 // whether level l's coarse correction is the K-cycle's two Krylov steps
 template <typename T>
 bool kcycle_level(const psb_ctx *C, const Hier<T> &H, int l) {
   const int nl = (int)H.L.size();
   if (l < 1 || l >= nl - 1) return false;
-  if (C->kmask == -1) return H.L[l].g.N >= kKcycleMinCells && H.L[l].g.N <= kKcycleMaxCells;
+  if (C->kmask == -1) {
+    if (C->coarse_rule == 1) return l <= kKcycleSemiLevels;
+    return H.L[l].g.N >= kKcycleMinCells && H.L[l].g.N <= kKcycleMaxCells;
+  }
   return (C->kmask >> l) & 1;
 }
 

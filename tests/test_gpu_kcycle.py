@@ -85,6 +85,19 @@ def test_kcycle_levels_chosen_by_size():
     assert np.array_equal(xa, xm) and ra == rm
 
 
+def test_kcycle_levels_of_semi_coarsened_hierarchy():
+    """ac_kcycle = -1 with ac_coarsen = 1 takes the first four coarse levels."""
+    nx, ny, nz = 64, 32, 96  # levels 64x32x96, 32x16x96, 16x8x96, 8x8x96, 8x8x48, ...
+    d = psb.generate(nx, ny, nz, 8, 0.02, 0.06, 1.0, 1000.0, 19, 1019, stretch_z=2.5, lz=2.0)
+    s = psb.Solver(nx, ny, nz)
+    s.set_matrix(d)
+    b = d["rhs"]
+    kw = dict(AC, ac_coarsen=1, ac_cfac=1.0, max_iter=400)
+    xa, ra = s.solve(np.zeros(b.size), b, ac_kcycle=-1, **kw)
+    xm, rm = s.solve(np.zeros(b.size), b, ac_kcycle=0b11110, **kw)
+    assert ra["converged"] and np.array_equal(xa, xm) and ra == rm
+
+
 def test_kcycle_arguments():
     d, s = system(16, nsph=2)
     b = d["rhs"]
