@@ -376,9 +376,8 @@ def main():
                          "faults on its ~1500-node captured graph)")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--graph-chunk", type=int, default=8,
-                    help="CG iterations per captured graph of the headline solves (rocprofv3 "
-                         "kernel tracing faults instantiating the 8-iteration K-cycle graph; "
-                         "profile with 2)")
+                    help="CG iterations per captured graph of the headline solves (smaller "
+                         "values give tracing tools smaller graphs)")
     ap.add_argument("--kcycle", type=lambda v: int(v, 0), default=-1,
                     help="K-cycle levels of the solves (bit l: level l; -1: chosen from the "
                          "hierarchy, the levels of 16^3 to 64^3 cells, or the first four "
