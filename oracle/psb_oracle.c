@@ -18,6 +18,11 @@
  *   ora_cg_prec (IC0, IC2)       x after 1, 5 iterations and at 1e-8, counts
  *                                and residuals of the reference's CG with
  *                                Prec::ic0 / ic2 (prec fixtures), bit for bit.
+ *   ora_set_kcycle (K-cycle)     no counterpart in the reference: an
+ *                                independent numpy restatement on the
+ *                                oracle's own levels, homogeneity, and the
+ *                                orthogonality of the two steps' residual to
+ *                                both search directions.
  *   ora_cg_prec (RBSGS),
  *   ora_prec_apply               no counterpart in the reference: z = M^-1 r
  *                                against M = (D - L) D^-1 (D - U) assembled
