@@ -674,8 +674,8 @@ T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega
 // K-cycle levels chosen from the hierarchy (psb_stop.ac_kcycle = -1).
 // Halving every direction: every level but the finest and the coarsest with
 // 16^3 to 64^3 cells.  Measured on MI355X (mixed solves to 1e-8): at 256^3
-// levels 2-4 take 16 iterations and 19.1 ms against the V-cycle's 36 and
-// 27.7 ms, with level 1 added 12 and 22.4 ms; at 512^3 levels 3-5 66
+// levels 2-4 take 16 iterations and 18.7 ms against the V-cycle's 36 and
+// 27.7 ms, with level 1 added 12 and 21.6 ms; at 512^3 levels 3-5 66
 // iterations and 352 ms against 113 and 548 ms, levels 2-5 89 and 515 ms.
 // Larger levels cost more per visit than the iterations they save; smaller
 // ones are a few launches of a few microseconds each, doubled at every K
