@@ -55,20 +55,23 @@ def test_kcycle_fp64_matches_oracle(mask):
     np.testing.assert_allclose(h[:m] / h[0], info["hist"][:m], rtol=1e-6)
 
 
-def test_kcycle_mixed_reaches_the_fp64_answer():
+@pytest.mark.parametrize("adapt", [2, 0])
+def test_kcycle_mixed_reaches_the_fp64_answer(adapt):
+    """With reliable updates (adapt 2) and with restarted refinement (adapt 0,
+    inner solves to 1e-4)."""
     n = 96
-    print(f"[kcycle] mixed {n}^3", flush=True)
+    print(f"[kcycle] mixed {n}^3 adapt {adapt}", flush=True)
     d, s = system(n, nsph=16, seed=2)
     b = d["rhs"]
     x64, r64 = s.solve(np.zeros(b.size), b, precision=psb.PSB_FP64,
                        **dict(AC, res_rat=1e-12))
     xd, rd = s.solve(np.zeros(b.size), b, precision=psb.PSB_FP64, ac_kcycle=0b110, **AC)
-    xk, rk = s.solve(np.zeros(b.size), b, precision=psb.PSB_MIXED, inner_adapt=2,
-                     inner_rtol=1e-2, ac_store=1, ac_kcycle=0b110, **AC)
+    xk, rk = s.solve(np.zeros(b.size), b, precision=psb.PSB_MIXED, inner_adapt=adapt,
+                     inner_rtol=1e-2 if adapt == 2 else 1e-4, ac_store=1, ac_kcycle=0b110, **AC)
     assert rd["converged"] and rk["converged"]
     assert rk["rel_l2"] < 1.2e-8
-    # the FP32 iterations with FP16 smoother copies track the FP64 ones
-    assert abs(rk["iterations"] - rd["iterations"]) <= 2, (rk["iterations"], rd["iterations"])
+    if adapt == 2:  # the FP32 iterations with FP16 smoother copies track the FP64 ones
+        assert abs(rk["iterations"] - rd["iterations"]) <= 2, (rk["iterations"], rd["iterations"])
     ed, ek = demeaned_rel(xd, x64), demeaned_rel(xk, x64)
     assert ek < 3.0 * ed + 1e-9, (ed, ek)
 
