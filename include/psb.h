@@ -77,8 +77,13 @@ typedef struct {
 
 /* Stopping criteria and method selection: MinIter, MaxIter, ResRat, ResTol,
  * scale, stalecount, precform of Linear::solve, plus the choices the
- * reference makes at construction time (CG/BiCGS class, Prec::di()). */
+ * reference makes at construction time (CG/BiCGS class, Prec::di()).
+ * Fill it with psb_stop_defaults, which sets struct_size: every entry point
+ * that takes a psb_stop returns PSB_EINVAL when struct_size is not the
+ * sizeof(psb_stop) the library was built with (a caller compiled against
+ * another version of this header). */
 typedef struct {
+  int struct_size;    /* sizeof(psb_stop), set by psb_stop_defaults          */
   int min_iter;       /* MinIter  (Src/Ravioli/maxiter.h:31, default 1)      */
   int max_iter;       /* MaxIter  (maxiter.h:17, default 20)                 */
   double res_rat;     /* ResRat   (resrat.h:26, default 0.01)                */
@@ -136,6 +141,13 @@ typedef struct {
                             cells; halving x and y first (ac_coarsen = 1),
                             levels 1-4 (the benchmark's setting).
                             0 (default): the V-cycle.                       */
+  int ac_flexible;       /* outer CG's beta: 1 flexible (Polak-Ribiere form,
+                            beta = -z_new.A p / p.A p, which keeps successive
+                            directions A-orthogonal when the preconditioner
+                            varies from one application to the next, as the
+                            K-cycle does); 0 Fletcher-Reeves, rho_new / rho
+                            (cg.cpp:93); -1 (default): flexible exactly when
+                            the K-cycle is on                               */
 } psb_stop;
 
 typedef struct {
@@ -303,6 +315,8 @@ int psb_generate(const psb_case *cs, double *c, double *w, double *e,
 enum { PSB_CYCLE_NONE = -1, PSB_CYCLE_V = 0, PSB_CYCLE_F = 1, PSB_CYCLE_W = 2 };
 
 typedef struct {
+  int struct_size;    /* sizeof(psb_cycle), set by psb_cycle_defaults; other
+                         values are PSB_EINVAL                               */
   double res_tol;     /* ResTol toler (< 0: off, as vcycle(ResRat))          */
   double res_rat;     /* ResRat factor (< 0: off, as vcycle(ResTol))         */
   int max_cycles;     /* AC::max_cycles (20, additive.cpp:11)                */

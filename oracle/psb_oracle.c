@@ -56,6 +56,13 @@
  *   ora_pcg_rel                  not in the reference: dense pseudo-inverse
  *                                solve on tiny grids, symmetry of M^-1
  *                                (u.M^-1 v = v.M^-1 u), A-orthogonality.
+ *   ora_set_flexible             not in the reference: with the flexible
+ *                                beta every new search direction is
+ *                                A-orthogonal to the last one
+ *                                (ora_set_orth_record) under the K-cycle's
+ *                                varying preconditioner, which the
+ *                                Fletcher-Reeves beta fails; with a fixed
+ *                                preconditioner both agree.
  *   ora_zline_solve,
  *   ora_ac_precond_s             not in the reference: per-column dense
  *                                tridiagonal solves; symmetry of M^-1 with
@@ -1523,6 +1530,22 @@ static void project_mean(const ora_ac *ac, double *z) {
     if (fl[P]) z[P] -= s;
 }
 
+/* the outer CG's beta: 0 Fletcher-Reeves, rho_new / rho (cg.cpp:93); 1
+ * flexible, -z.A p / p.A p (Y. Notay, "Flexible conjugate gradients", SIAM J.
+ * Sci. Comput. 22 (2000) 1444-1460), which makes each new direction
+ * A-orthogonal to the last one whatever the preconditioner did */
+static int flex_beta = 0;
+void ora_set_flexible(int f) { flex_beta = f; }
+
+/* test hook: orth[k] = |p_k+1 . A p_k| / sqrt(p_k+1 . A p_k+1  p_k . A p_k)
+ * for the directions of the next ora_acpcg_s runs (NULL: off) */
+static double *orth_buf = NULL;
+static int orth_cap = 0;
+void ora_set_orth_record(double *buf, int cap) {
+  orth_buf = buf;
+  orth_cap = cap;
+}
+
 int ora_acpcg(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
               int nu1, int nu2, double omega, double cfac, double *hist,
               int *iters) {
@@ -1547,7 +1570,8 @@ int ora_acpcg_s(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
   double rho = ora_dot(N, r, z);
   for (it = 1; it <= maxit; it++) {
     ora_spmv(A, p, q);
-    double alpha = rho / ora_dot(N, p, q);
+    const double pq = ora_dot(N, p, q);
+    double alpha = rho / pq;
     for (size_t P = 0; P < N; P++) x[P] += alpha * p[P];
     for (size_t P = 0; P < N; P++) r[P] -= alpha * q[P];
     double rel = nrm2(N, r) / r0;
@@ -1556,9 +1580,15 @@ int ora_acpcg_s(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
     ora_ac_precond_s(ac, 0, r, z, nu1, nu2, omega, cfac, smoother);
     project_mean(ac, z);
     double rho_new = ora_dot(N, r, z);
-    double beta = rho_new / rho;
+    double beta = flex_beta ? (pq != 0.0 ? -ora_dot(N, z, q) / pq : 0.0) : rho_new / rho;
     rho = rho_new;
     for (size_t P = 0; P < N; P++) p[P] = z[P] + beta * p[P];
+    if (orth_buf && it - 1 < orth_cap) {
+      double *w = dalloc(N);
+      ora_spmv(A, p, w);
+      orth_buf[it - 1] = fabs(ora_dot(N, p, q)) / sqrt(ora_dot(N, p, w) * pq);
+      free(w);
+    }
   }
   if (it > maxit) it = maxit;
 done:

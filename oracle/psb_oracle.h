@@ -205,6 +205,10 @@ void ora_ac_precond_s(ora_ac *ac, int l, const double *r, double *z, int nu1,
  * correction in ora_ac_precond* is two flexible-CG steps preconditioned by
  * the cycle below instead of cfac times one cycle; 0 (default): V-cycle.    */
 void ora_set_kcycle(int mask);
+/* outer CG beta of ora_acpcg_s: 0 Fletcher-Reeves (default), 1 flexible */
+void ora_set_flexible(int f);
+/* test hook: A-orthogonality of successive directions of ora_acpcg_s */
+void ora_set_orth_record(double *buf, int cap);
 /* x = T^-1 t column by column (the z-line smoother's solve).               */
 void ora_zline_solve(const ora_mat *A, const double *t, double *x);
 /* PCG with the V-cycle preconditioner, its output projected to zero mean over

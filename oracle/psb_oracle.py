@@ -189,6 +189,11 @@ def set_kcycle(mask):
     lib().ora_set_kcycle(int(mask))
 
 
+def set_flexible(f):
+    """Outer CG beta of AC.acpcg: False Fletcher-Reeves, True flexible."""
+    lib().ora_set_flexible(1 if f else 0)
+
+
 def set_sum_block(b):
     """0: sequential sums (the reference's order); b > 0: blocks of b terms."""
     lib().ora_set_sum_block(int(b))
@@ -377,15 +382,26 @@ class AC:
         return z
 
     def acpcg(self, x, b, rtol=1e-8, maxit=500, nu1=2, nu2=2, omega=0.8, cfac=1.0,
-              smoother=0):
+              smoother=0, orth=False):
+        """orth: also return |p_k+1.A p_k| / (|p_k+1|_A |p_k|_A) per step."""
         x = np.array(x, dtype=np.float64, copy=True)
         hist = np.full(maxit + 1, np.nan)
+        ob = np.full(maxit, np.nan)
+        if orth:
+            lib().ora_set_orth_record(_p(ob), maxit)
         it = C.c_int(0)
-        ok = lib().ora_acpcg_s(C.byref(self.ac), _p(x),
-                               _p(np.ascontiguousarray(b, dtype=np.float64)),
-                               C.c_double(rtol), maxit, nu1, nu2, C.c_double(omega),
-                               C.c_double(cfac), smoother, _p(hist), C.byref(it))
-        return x, dict(converged=bool(ok), iters=it.value, hist=hist[:it.value + 1])
+        try:
+            ok = lib().ora_acpcg_s(C.byref(self.ac), _p(x),
+                                   _p(np.ascontiguousarray(b, dtype=np.float64)),
+                                   C.c_double(rtol), maxit, nu1, nu2, C.c_double(omega),
+                                   C.c_double(cfac), smoother, _p(hist), C.byref(it))
+        finally:
+            if orth:
+                lib().ora_set_orth_record(None, 0)
+        out = dict(converged=bool(ok), iters=it.value, hist=hist[:it.value + 1])
+        if orth:
+            out["orth"] = ob[:max(0, it.value - 1)]
+        return x, out
 
 
 def pcg_rel(A, x, b, rtol=1e-8, maxit=1000):

@@ -8,6 +8,10 @@
 // turns small ones into committed fixtures.
 //
 //   ref_driver case  OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel [cx cy cz]
+//   ref_driver case-s OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel
+//              d1x dnx d1y dny d1z dnz     (the same on a non-uniform grid: the
+//              reference's Grid1D stretched from end cell sizes D1 to DN,
+//              given in units of h = 1 / max(nx, ny, nz))
 //   ref_driver cycles OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel [cx cy cz]
 //   ref_driver bck OUT IN.bck nx ny nz   (Scalar::load of IN, Scalar::save of a seeded field)
 //   ref_driver bench-cg nx nsph rmin rmax rho_in rho_out seed_sph seed_vel maxit
@@ -134,6 +138,9 @@ struct Case {
   uint64_t seed_sph, seed_vel;
   bool centred = false;
   double cx = 0, cy = 0, cz = 0;
+  // first and last cell size per direction in units of h (1, 1: uniform);
+  // Grid1D distributes the nodes in between (grid1d_distribute_nodes_inside.cpp)
+  double d1[3] = {1.0, 1.0, 1.0}, dn[3] = {1.0, 1.0, 1.0};
 };
 
 struct World {
@@ -152,9 +159,12 @@ World build_world(const Case &cs, Krylov *solver_override = nullptr) {
   World w;
   const int nmax = std::max(cs.nx, std::max(cs.ny, cs.nz));
   const double h = 1.0 / nmax;
-  w.gx = new Grid1D(Range<real>(0.0, cs.nx * h), Range<real>(h, h), cs.nx, Periodic::no());
-  w.gy = new Grid1D(Range<real>(0.0, cs.ny * h), Range<real>(h, h), cs.ny, Periodic::no());
-  w.gz = new Grid1D(Range<real>(0.0, cs.nz * h), Range<real>(h, h), cs.nz, Periodic::no());
+  w.gx = new Grid1D(Range<real>(0.0, cs.nx * h), Range<real>(cs.d1[0] * h, cs.dn[0] * h), cs.nx,
+                    Periodic::no());
+  w.gy = new Grid1D(Range<real>(0.0, cs.ny * h), Range<real>(cs.d1[1] * h, cs.dn[1] * h), cs.ny,
+                    Periodic::no());
+  w.gz = new Grid1D(Range<real>(0.0, cs.nz * h), Range<real>(cs.d1[2] * h, cs.dn[2] * h), cs.nz,
+                    Periodic::no());
   w.d = new Domain(*w.gx, *w.gy, *w.gz);
   w.time = new Times(1, 1.0);
   w.solver = solver_override ? solver_override : new CG(*w.d, Prec::di());
@@ -441,6 +451,7 @@ double now() {
 
 }  // namespace
 
+#ifndef REF_DRIVER_EMBED
 int main(int argc, char **argv) {
   std::cout << std::setprecision(17);
   if (argc < 2) {
@@ -448,11 +459,17 @@ int main(int argc, char **argv) {
     return 2;
   }
   std::string mode = argv[1];
-  if (mode == "case") {
+  if (mode == "case" || mode == "case-s") {
     if (argc < 13) { std::fprintf(stderr, "case: too few arguments\n"); return 2; }
     outdir = argv[2];
     Case cs = parse_case(argv + 6, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
-    if (argc >= 16) {
+    if (mode == "case-s") {
+      if (argc < 19) { std::fprintf(stderr, "case-s: too few arguments\n"); return 2; }
+      for (int q = 0; q < 3; q++) {
+        cs.d1[q] = std::atof(argv[13 + 2 * q]);
+        cs.dn[q] = std::atof(argv[14 + 2 * q]);
+      }
+    } else if (argc >= 16) {
       cs.centred = true;
       cs.cx = std::atof(argv[13]); cs.cy = std::atof(argv[14]); cs.cz = std::atof(argv[15]);
     }
@@ -569,3 +586,4 @@ int main(int argc, char **argv) {
   std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
   return 2;
 }
+#endif  // REF_DRIVER_EMBED

@@ -164,7 +164,8 @@ void cg_xr(psb_ctx *C, const Grid &g, const T *c, const T *p, const T *q, T *x, 
 // matrix (lv.c16, scaled by lv.scale) the sweep reads that copy.
 template <typename T, bool RHO, typename M, bool FLUX>
 void jacobi_m(psb_ctx *C, const Level<T> &lv, DMat<M> A, const T *e, T *eout,
-              const Level<T> *coarse, const T *eH, T cfac, T omega, T zscale, DevState *st) {
+              const Level<T> *coarse, const T *eH, T cfac, T omega, T zscale, DevState *st,
+              const T *qf) {
   const Grid &g = lv.g;
   int nb = nblocks<T>(g);
   with_v<T>(g, [&](auto vt) {
@@ -187,47 +188,49 @@ void jacobi_m(psb_ctx *C, const Level<T> &lv, DMat<M> A, const T *e, T *eout,
       if (coarse)
         hipLaunchKernelGGL((k_jacobi_march<T, V, true, RHO, M, FLUX>), dim3(m.nblocks),
                            dim3(kBlock), 0, C->stream, g, A, lv.r, e, eout, pr, omega, lv.aflag,
-                           st, C->work, zscale, m.TI, m.ntiles);
+                           st, C->work, zscale, m.TI, m.ntiles, qf);
       else
         hipLaunchKernelGGL((k_jacobi_march<T, V, false, RHO, M, FLUX>), dim3(m.nblocks),
                            dim3(kBlock), 0, C->stream, g, A, lv.r, e, eout, pr, omega, lv.aflag,
-                           st, C->work, zscale, m.TI, m.ntiles);
+                           st, C->work, zscale, m.TI, m.ntiles, qf);
       return;
     }
     if (coarse) {
       hipLaunchKernelGGL((k_jacobi<T, V, true, true, RHO, M, FLUX>), dim3(nblocks<T>(g)),
                          dim3(kBlock), 0, C->stream, g, A, lv.r, e, eout, pr, omega, lv.aflag,
-                         st, C->work, zscale);
+                         st, C->work, zscale, qf);
     } else {
       hipLaunchKernelGGL((k_jacobi<T, V, true, false, RHO, M, FLUX>), dim3(nblocks<T>(g)),
                          dim3(kBlock), 0, C->stream, g, A, lv.r, e, eout, pr, omega, lv.aflag,
-                         st, C->work, zscale);
+                         st, C->work, zscale, qf);
     }
   });
-  if (RHO) finalize<4>(C, FIN_RHO, nb, st);
+  if (RHO) finalize<kRhoParts>(C, qf ? FIN_RHO_FLEX : FIN_RHO, nb, st);
 }
 
+// qf (RHO only): q = A p of the last search direction for the flexible beta
 template <typename T, bool RHO>
 void jacobi(psb_ctx *C, const Level<T> &lv, const T *e, T *eout, const Level<T> *coarse,
-            const T *eH, T cfac, T omega, DevState *st, T zscale = T(1)) {
+            const T *eH, T cfac, T omega, DevState *st, T zscale = T(1),
+            const T *qf = nullptr) {
   if constexpr (std::is_same<T, float>::value) {
-    if (lv.c16 && lv.d) {
+    if (lv.h16() && lv.d) {
       jacobi_m<T, RHO, _Float16, true>(C, lv, lv.mat16(), e, eout, coarse, eH, cfac, omega,
-                                       zscale, st);
+                                       zscale, st, qf);
       return;
     }
-    if (lv.c16) {
+    if (lv.h16()) {
       jacobi_m<T, RHO, _Float16, false>(C, lv, lv.mat16(), e, eout, coarse, eH, cfac, omega,
-                                        zscale, st);
+                                        zscale, st, qf);
       return;
     }
     if (lv.d) {
       jacobi_m<T, RHO, T, true>(C, lv, lv.matflux(), e, eout, coarse, eH, cfac, omega, zscale,
-                                st);
+                                st, qf);
       return;
     }
   }
-  jacobi_m<T, RHO, T, false>(C, lv, lv.mat(), e, eout, coarse, eH, cfac, omega, zscale, st);
+  jacobi_m<T, RHO, T, false>(C, lv, lv.mat(), e, eout, coarse, eH, cfac, omega, zscale, st, qf);
 }
 
 // the two post-sweeps of level lv in one pass (k_post2): the first with the
@@ -236,7 +239,7 @@ void jacobi(psb_ctx *C, const Level<T> &lv, const T *e, T *eout, const Level<T> 
 template <bool RHO, typename M, bool FLUX>
 void post2_m(psb_ctx *C, const Level<float> &lv, DMat<M> A, const float *e, float *eout,
              const Level<float> &coarse, const float *eH, float cfac, float omega, float zscale,
-             DevState *st, const FuseShape &f) {
+             DevState *st, const FuseShape &f, const float *qf) {
   ProlX<float, 4> pr{};
   pr.e = e;
   pr.eH = eH;
@@ -249,28 +252,28 @@ void post2_m(psb_ctx *C, const Level<float> &lv, DMat<M> A, const float *e, floa
   pr.nZH = coarse.g.nz;
   hipLaunchKernelGGL((k_post2<M, FLUX, RHO>), dim3(f.nblocks), dim3(kBlock), 0, C->stream, lv.g,
                      A, lv.r, eout, pr, omega, lv.aflag, st, C->work, zscale, f.R, f.TI,
-                     f.ntiles);
-  if (RHO) finalize<4>(C, FIN_RHO, f.nblocks, st);
+                     f.ntiles, qf);
+  if (RHO) finalize<kRhoParts>(C, qf ? FIN_RHO_FLEX : FIN_RHO, f.nblocks, st);
 }
 
 template <typename T, bool RHO>
 bool post2(psb_ctx *C, const Level<T> &lv, const T *e, T *eout, const Level<T> &coarse,
-           const T *eH, T cfac, T omega, T zscale, DevState *st) {
+           const T *eH, T cfac, T omega, T zscale, DevState *st, const T *qf = nullptr) {
   if constexpr (std::is_same<T, float>::value) {
     FuseShape f;
     if (!C->fuse || !fuse_shape(lv.g, f)) return false;
-    if (lv.c16 && lv.d)
+    if (lv.h16() && lv.d)
       post2_m<RHO, _Float16, true>(C, lv, lv.mat16(), e, eout, coarse, eH, cfac, omega, zscale,
-                                   st, f);
-    else if (lv.c16)
+                                   st, f, qf);
+    else if (lv.h16())
       post2_m<RHO, _Float16, false>(C, lv, lv.mat16(), e, eout, coarse, eH, cfac, omega, zscale,
-                                    st, f);
+                                    st, f, qf);
     else if (lv.d)
       post2_m<RHO, float, true>(C, lv, lv.matflux(), e, eout, coarse, eH, cfac, omega, zscale,
-                                st, f);
+                                st, f, qf);
     else
       post2_m<RHO, float, false>(C, lv, lv.mat(), e, eout, coarse, eH, cfac, omega, zscale, st,
-                                 f);
+                                 f, qf);
     return true;
   } else {
     return false;
@@ -301,11 +304,11 @@ template <typename T, int V, int KV, int CI, int CJ, int CK>
 void restrict_v(psb_ctx *C, const Level<T> &f, const T *e, Level<T> &cL, T omega,
                 DevState *st) {
   if constexpr (std::is_same<T, float>::value) {
-    if (f.c16 && f.d) {
+    if (f.h16() && f.d) {
       restrict_launch<T, V, KV, CI, CJ, CK, _Float16, true>(C, f, f.mat16(), e, cL, omega, st);
       return;
     }
-    if (f.c16) {
+    if (f.h16()) {
       restrict_launch<T, V, KV, CI, CJ, CK, _Float16, false>(C, f, f.mat16(), e, cL, omega, st);
       return;
     }
@@ -631,8 +634,27 @@ void build_hier32(psb_ctx *C) {
     hipLaunchKernelGGL(k_flux_d<float>, dim3(grid_blocks(N)), dim3(kBlock), 0, C->stream, lv.g,
                        s.mat(), lv.d, 1.0);
 #endif
-    // half-precision copy for the smoother on every level but the coarsest
-    if (C->store_half && l < nl - 1) {
+  }
+  H.G = H64.G;
+  H.m = H64.m;
+  HIPCHK(hipGetLastError());
+  H.built = true;
+}
+
+// Whether the FP32 levels' smoother reads half-precision copies
+// scale16 * (c, w, s, b) (psb_stop.ac_store >= 1): built on every level but
+// the coarsest the first time a solve asks for them, then kept beside the
+// FP32 matrices, so that solves with and without them alternate on one
+// hierarchy (and their captured graphs stay valid).
+void set_half(psb_ctx *C, bool on) {
+  Hier<float> &H = C->h32;
+  const Hier<double> &H64 = C->h64;
+  const int nl = (int)H.L.size();
+  for (int l = 0; l < nl - 1; l++) {
+    Level<float> &lv = H.L[l];
+    const Level<double> &s = H64.L[l];
+    const int N = lv.g.N;
+    if (on && !lv.c16) {
       const int nb = grid_blocks(N);
       hipLaunchKernelGGL(k_blockmax, dim3(nb), dim3(kBlock), 0, C->stream, s.c, N,
                          C->work.partial);
@@ -643,24 +665,23 @@ void build_hier32(psb_ctx *C) {
       const double cmax = *std::max_element(bm.begin(), bm.end());
       // largest diagonal near 2^14: far from the FP16 limit (65504), with the
       // smallest couplings (density ratio 1000) still normal numbers
-      lv.scale = cmax > 0.0 ? std::ldexp(1.0, 14 - (int)std::ceil(std::log2(cmax))) : 1.0;
+      lv.scale16 = cmax > 0.0 ? std::ldexp(1.0, 14 - (int)std::ceil(std::log2(cmax))) : 1.0;
       _Float16 **dst[4] = {&lv.c16, &lv.w16, &lv.s16, &lv.b16};
       const double *src[4] = {s.c, s.w, s.s, s.b};
       for (int q = 0; q < 4; q++) {
         *dst[q] = C->harena.alloc<_Float16>(N);
         if (q == 0 && lv.d)  // flux form: scale * d
           hipLaunchKernelGGL(k_flux_d<_Float16>, dim3(grid_blocks(N)), dim3(kBlock), 0,
-                             C->stream, lv.g, s.mat(), *dst[q], lv.scale);
+                             C->stream, lv.g, s.mat(), *dst[q], lv.scale16);
         else
           hipLaunchKernelGGL(k_to_f16, dim3(grid_blocks(N)), dim3(kBlock), 0, C->stream,
-                             src[q], *dst[q], lv.scale, N, q == 0 ? 1 : 0);
+                             src[q], *dst[q], lv.scale16, N, q == 0 ? 1 : 0);
       }
+      HIPCHK(hipGetLastError());
     }
+    lv.use16 = on;
+    lv.scale = on ? lv.scale16 : 1.0;
   }
-  H.G = H64.G;
-  H.m = H64.m;
-  HIPCHK(hipGetLastError());
-  H.built = true;
 }
 
 // ---------------------------------------------------------------------------
@@ -783,7 +804,7 @@ T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega
   if (nu2 == 2) {  // both post-sweeps in one pass where the level suits it
     const int out = (cur + 2) % 3;
     const bool done = l == 0 ? post2<T, true>(C, lv, lv.e[cur], lv.e[out], cL, zH, cfac_l, omega,
-                                              T(lv.scale), st)
+                                              T(lv.scale), st, H.qf)
                              : post2<T, false>(C, lv, lv.e[cur], lv.e[out], cL, zH, cfac_l,
                                                omega, T(1), st);
     if (done) return lv.e[out];
@@ -794,7 +815,8 @@ T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega
     const Level<T> *co = s == 0 ? &cL : nullptr;
     const T *eH = s == 0 ? zH : nullptr;
     if (last)
-      jacobi<T, true>(C, lv, lv.e[cur], lv.e[nxt], co, eH, cfac_l, omega, st, T(lv.scale));
+      jacobi<T, true>(C, lv, lv.e[cur], lv.e[nxt], co, eH, cfac_l, omega, st, T(lv.scale),
+                      H.qf);
     else
       jacobi<T, false>(C, lv, lv.e[cur], lv.e[nxt], co, eH, cfac_l, omega, st);
     cur = nxt;
@@ -809,7 +831,7 @@ bool cg_pq16(psb_ctx *C, const Level<T> &f, const T *z, const T *pold, T *pnew, 
              DevState *st) {
   if constexpr (std::is_same<T, float>::value) {
     MarchShape m;
-    if (!C->cg_half || !f.c16 || !f.d || !vec_ok<float>(f.g) || !march_ok<4>(f.g, m))
+    if (!C->cg_half || !f.h16() || !f.d || !vec_ok<float>(f.g) || !march_ok<4>(f.g, m))
       return false;
     hipLaunchKernelGGL((k_cg_pq_march<float, 4, true, false, true, _Float16>), dim3(m.nblocks),
                        dim3(kBlock), 0, C->stream, f.g, f.mat16(), z, pold, pnew, q, f.aflag, st,
@@ -826,6 +848,7 @@ template <typename T>
 void ac_iteration(psb_ctx *C, Hier<T> &H, KrylovVecs<T> &K, DevState *st, int parity, int nu1,
                   int nu2, T omega, T cfac, int xr_op = -1) {
   Level<T> &f = H.L[0];
+  H.qf = C->flex ? K.q : nullptr;  // q = A p of the last direction (flexible beta)
   T *z = vcycle<T>(C, H, 0, st, nu1, nu2, omega, cfac);
   T *pold = K.p[parity], *pnew = K.p[parity ^ 1];
   if (!cg_pq16<T>(C, f, z, pold, pnew, K.q, st)) {
@@ -848,16 +871,38 @@ void destroy_graph(Graph &gr) {
 }
 
 // every captured graph holds device pointers of the matrix, hierarchy and
-// geometry: any change to those drops them
+// geometry: any change to those drops them.  The device is idle first, so
+// that no launch of a graph is in flight when it goes.
 void destroy_graphs(psb_ctx *C) {
+  if (C->stream) (void)hipStreamSynchronize(C->stream);
+  (void)hipDeviceSynchronize();
   destroy_graph(C->gr_cgdiag);
   destroy_graph(C->gr_cgsgs);
   destroy_graph(C->gr_bicgs);
-  destroy_graph(C->gr_ac64);
-  destroy_graph(C->gr_ac32);
-  destroy_graph(C->gr_ru);
+  for (auto &kv : C->gcache) destroy_graph(kv.second);
+  C->gcache.clear();
   destroy_graph(C->href.gr);
   destroy_graph(C->href.gfull);
+}
+
+// the cached graph of an AC-preconditioned solve with these settings (empty
+// if none was captured yet); the least recently used one goes when the
+// cache is full
+Graph &graph_for(psb_ctx *C, const GKey &k) {
+  auto it = C->gcache.find(k);
+  if (it == C->gcache.end()) {
+    if (C->gcache.size() >= kGraphCacheMax) {
+      auto old = C->gcache.begin();
+      for (auto q = C->gcache.begin(); q != C->gcache.end(); ++q)
+        if (q->second.used < old->second.used) old = q;
+      HIPCHK(hipStreamSynchronize(C->stream));
+      destroy_graph(old->second);
+      C->gcache.erase(old);
+    }
+    it = C->gcache.emplace(k, Graph()).first;
+  }
+  it->second.used = ++C->gclock;
+  return it->second;
 }
 
 // drop the hierarchy (and the graphs that use it): next AC solve rebuilds it
@@ -870,13 +915,12 @@ void reset_hier(psb_ctx *C) {
   C->href = RefHier();
 }
 
-// the hierarchy for the coarsening rule `rule` and the smoother's matrix
-// storage (half: FP16 copies on the FP32 levels)
-void ensure_rule(psb_ctx *C, int rule, bool half) {
-  if (rule == C->coarse_rule && half == C->store_half) return;
+// the hierarchy for the coarsening rule `rule` (the FP16 copies are added to
+// it when a solve first asks for them, set_half)
+void ensure_rule(psb_ctx *C, int rule) {
+  if (rule == C->coarse_rule) return;
   reset_hier(C);
   C->coarse_rule = rule;
-  C->store_half = half;
 }
 
 
@@ -1109,20 +1153,37 @@ void solve_bicgs_diag(psb_ctx *C, double *x, const double *b, const psb_stop *p,
 // AC-preconditioned CG in precision T on K.x, K.r (r = b - A x already
 // formed, e1 in level 0's e[0], st initialised).  Runs to st->done.
 // ---------------------------------------------------------------------------
+// the graph key of an AC-preconditioned solve with the context's current
+// settings (body: 0 FP64, 1 FP32 inner solve, 2 reliable updates)
+GKey solve_key(const psb_ctx *C, const psb_stop *p, int body, const void *x, const void *b) {
+  GKey k;
+  k.body = body;
+  k.chunk = chunk_of(p);
+  k.nu1 = p->ac_pre;
+  k.nu2 = p->ac_post;
+  k.kmask = C->kmask;
+  k.fuse = C->fuse;
+  k.cg16 = C->cg_half;
+  k.half = body == 0 ? 0 : (C->h32.L.empty() ? 0 : C->h32.L[0].use16);
+  k.flex = C->flex;
+  k.omega = p->ac_omega;
+  k.cfac = p->ac_cfac;
+  k.px = x;
+  k.pb = b;
+  return k;
+}
+
 template <typename T>
-void run_acpcg(psb_ctx *C, Hier<T> &H, KrylovVecs<T> &K, DevState *st, Graph &gr,
+void run_acpcg(psb_ctx *C, Hier<T> &H, KrylovVecs<T> &K, DevState *st, int body,
                const psb_stop *p) {
   const int chunk = chunk_of(p);
   const int nu1 = p->ac_pre, nu2 = p->ac_post;
   const T omega = (T)p->ac_omega, cfac = (T)p->ac_cfac;
-  auto body = [&]() {
+  auto f = [&]() {
     for (int it = 0; it < chunk; it++) ac_iteration<T>(C, H, K, st, it & 1, nu1, nu2, omega, cfac);
   };
-  const long long key = (long long)chunk + ((long long)nu1 << 16) + ((long long)nu2 << 24);
-  if (gr.key != key || C->ac_omega != p->ac_omega || C->ac_cfac != p->ac_cfac) destroy_graph(gr);
-  C->ac_omega = p->ac_omega;
-  C->ac_cfac = p->ac_cfac;
-  run_graph(C, gr, key, st, p->max_iter, body);
+  Graph &gr = graph_for(C, solve_key(C, p, body, nullptr, nullptr));
+  run_graph(C, gr, 0, st, p->max_iter, f);
 }
 
 void solve_ac64(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_result *R) {
@@ -1141,7 +1202,7 @@ void solve_ac64(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_r
   start<double, true, 1>(C, g, mat64(C), b, K.x, K.r, f.e[0], p->ac_omega, C->st64);
   HIPCHK(hipGetLastError());
   read_state(C, C->st64);
-  run_acpcg<double>(C, C->h64, K, C->st64, C->gr_ac64, p);
+  run_acpcg<double>(C, C->h64, K, C->st64, 0, p);
   HIPCHK(hipMemcpyAsync(x, K.x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, C->stream));
   finish_result(R, *C->hst, 0);
   C->hist_count = std::min(C->hst->iter + 1, C->hist_cap);
@@ -1213,7 +1274,7 @@ void solve_ir(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_res
     finalize<1>(C, FIN_CENTER, nb32, C->st32);
     HIPCHK(hipGetLastError());
     read_state(C, C->st32);
-    run_acpcg<float>(C, C->h32, K, C->st32, C->gr_ac32, &in);
+    run_acpcg<float>(C, C->h32, K, C->st32, 1, &in);
     inner_total += C->hst->iter;
     if (C->hst->reason == 8) { status = PSB_NAN; break; }
     with_v<double>(g, [&](auto vt) {
@@ -1276,13 +1337,7 @@ void solve_ru(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_res
   const int chunk = chunk_of(p);
   const int nu1 = p->ac_pre, nu2 = p->ac_post;
   const float omega = (float)p->ac_omega, cfac = (float)p->ac_cfac;
-  const long long key = (long long)chunk + ((long long)nu1 << 16) + ((long long)nu2 << 24);
-  Graph &gr = C->gr_ru;
-  if (gr.key != key || gr.px != x || gr.pb != b || C->ac_omega != p->ac_omega ||
-      C->ac_cfac != p->ac_cfac)
-    destroy_graph(gr);
-  C->ac_omega = p->ac_omega;
-  C->ac_cfac = p->ac_cfac;
+  Graph &gr = graph_for(C, solve_key(C, p, 2, x, b));
   init_state(C, C->st32, p, p->norm, p->res_rat, p->res_tol, p->max_iter);
   HIPCHK(hipMemcpyAsync(&C->st32->delta, &p->inner_rtol, sizeof(double), hipMemcpyHostToDevice,
                         C->stream));
@@ -1299,9 +1354,7 @@ void solve_ru(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_res
       ru_resid(C, b, x, true, FIN_RU);
     }
   };
-  run_graph(C, gr, key, C->st32, p->max_iter, body);
-  gr.px = x;
-  gr.pb = b;
+  run_graph(C, gr, 0, C->st32, p->max_iter, body);
   // fold in what the last FP32 iterations added since the last refresh
   ru_xupdate(C, x, false);
   DevState h = *C->hst;
@@ -1646,21 +1699,19 @@ int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_
     if (p->method != PSB_CG || !C->sym || p->ac_pre < 1 || p->ac_post < 1) return PSB_EINVAL;
     if (p->ac_coarsen < 0 || p->ac_coarsen > 1) return PSB_EINVAL;
     if (p->ac_store < 0 || p->ac_store > 2) return PSB_EINVAL;
-    // the storage choice concerns only the FP32 levels: FP64 solves keep it
-    ensure_rule(C, p->ac_coarsen, p->precision == PSB_MIXED ? p->ac_store >= 1 : C->store_half);
-    const bool cg16 = p->precision == PSB_MIXED && p->ac_store == 2;
-    if (cg16 != C->cg_half) {  // the captured graphs hold the other p/q kernel
-      destroy_graphs(C);
-      C->cg_half = cg16;
-    }
-    if ((p->ac_fuse != 0) != C->fuse) {  // the captured graphs hold the other kernels
-      destroy_graphs(C);
-      C->fuse = p->ac_fuse != 0;
-    }
     if (p->ac_kcycle < -1 || (p->ac_kcycle > 0 && (p->ac_kcycle & 1))) return PSB_EINVAL;
-    if (p->ac_kcycle != C->kmask) {  // the captured graphs hold the other cycle
-      destroy_graphs(C);
-      C->kmask = p->ac_kcycle;
+    if (p->ac_flexible < -1 || p->ac_flexible > 1) return PSB_EINVAL;
+    ensure_rule(C, p->ac_coarsen);
+    // settings of this solve, read while its graph is captured (the graph
+    // cache keys on them: no setting change drops a graph)
+    C->cg_half = p->precision == PSB_MIXED && p->ac_store == 2;
+    C->fuse = p->ac_fuse != 0;
+    C->kmask = p->ac_kcycle;
+    C->flex = p->ac_flexible == 1 || (p->ac_flexible == -1 && p->ac_kcycle != 0);
+    C->ac_omega = p->ac_omega;
+    if (p->precision == PSB_MIXED) {
+      build_hier32(C);
+      set_half(C, p->ac_store >= 1);
     }
     if (p->precision == PSB_MIXED && p->inner_adapt == 2)
       solve_ru(C, x, b, p, R);
@@ -1726,6 +1777,7 @@ extern "C" {
 
 void psb_stop_defaults(psb_stop *s) {
   std::memset(s, 0, sizeof(*s));
+  s->struct_size = (int)sizeof(psb_stop);
   s->min_iter = 1;
   s->max_iter = 20;
   s->res_rat = 0.01;
@@ -1749,6 +1801,7 @@ void psb_stop_defaults(psb_stop *s) {
   s->ac_store = 0;
   s->ac_fuse = 0;
   s->ac_kcycle = 0;
+  s->ac_flexible = -1;
 }
 
 const char *psb_status_string(int s) {
@@ -2061,7 +2114,7 @@ int psb_set_active(psb_ctx *C, const unsigned char *off) {
 }
 
 int psb_solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_result *R) {
-  if (!C || !x || !b || !p || !R) return PSB_EINVAL;
+  if (!C || !x || !b || !p || !R || !stop_ok(p)) return PSB_EINVAL;
   if (!C->have_matrix) return PSB_ENOTREADY;
   if (p->max_iter < 0 || p->scale == 0.0 || p->stalecount > kStaleMax) return PSB_EINVAL;
   std::memset(R, 0, sizeof(*R));
@@ -2069,7 +2122,7 @@ int psb_solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, 
 }
 
 int psb_solve(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_result *R) {
-  if (!C || !x || !b || !p || !R) return PSB_EINVAL;
+  if (!C || !x || !b || !p || !R || !stop_ok(p)) return PSB_EINVAL;
   if (!C->have_matrix) return PSB_ENOTREADY;
   return guarded(C, [&]() {
     const int N = C->g.N;
@@ -2116,6 +2169,7 @@ int psb_apply(psb_ctx *C, const double *x, const double *b, double *y) {
 
 void psb_cycle_defaults(psb_cycle *c) {
   std::memset(c, 0, sizeof(*c));
+  c->struct_size = (int)sizeof(psb_cycle);
   c->res_tol = -1.0;
   c->res_rat = -1.0;
   c->max_cycles = 20;  // additive.cpp:11-22
@@ -2133,7 +2187,7 @@ void psb_cycle_defaults(psb_cycle *c) {
 }
 
 int psb_ac_vcycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_result *R) {
-  if (!C || !x || !b || !P || !R) return PSB_EINVAL;
+  if (!C || !x || !b || !P || !R || !cycle_ok(P)) return PSB_EINVAL;
   if (!C->have_matrix) return PSB_ENOTREADY;
   return guarded(C, [&]() {
     const psb_layout &L = C->lay;
@@ -2154,7 +2208,7 @@ int psb_ac_vcycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, ps
 
 int psb_ac_vcycle_device(psb_ctx *C, double *x, const double *b, const psb_cycle *P,
                          psb_result *R) {
-  if (!C || !x || !b || !P || !R) return PSB_EINVAL;
+  if (!C || !x || !b || !P || !R || !cycle_ok(P)) return PSB_EINVAL;
   if (!C->have_matrix) return PSB_ENOTREADY;
   return guarded(C, [&]() {
     int rc = ac_cycle(C, x, b, P, R);
@@ -2212,7 +2266,7 @@ int psb_time_kernel(psb_ctx *C, int kernel, int reps, double *avg_ms, double *by
       };
       // the level-0 matrix as the last mixed solve stored it (4 arrays, FP32
       // or FP16), e and r in, e out
-      *bytes = N * (4 * (lv.c16 ? 2 : 4) + 3 * 4);
+      *bytes = N * (4 * (lv.h16() ? 2 : 4) + 3 * 4);
     } else if (kernel == 4 || kernel == 5) {
       // the two post-sweeps of the fine level: 4 in one pass (k_post2), 5 as
       // two kernels; both as the last mixed solve's hierarchy stores the level
@@ -2236,7 +2290,7 @@ int psb_time_kernel(psb_ctx *C, int kernel, int reps, double *avg_ms, double *by
         jacobi<float, false>(C, lv, lv.e[0], lv.e[1], &cL, cL.e[0], 2.0f, 0.9f, C->st32);
         jacobi<float, false>(C, lv, lv.e[1], lv.e[2], nullptr, nullptr, 0.0f, 0.9f, C->st32);
       };
-      *bytes = N * (4 * (lv.c16 ? 2 : 4) + 3 * 4);  // one sweep's algorithmic bytes
+      *bytes = N * (4 * (lv.h16() ? 2 : 4) + 3 * 4);  // one sweep's algorithmic bytes
     } else if (kernel == 3) {
       ensure32(C);
       ensure_krylov(C, C->k32, false);

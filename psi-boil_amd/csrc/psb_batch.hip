@@ -297,7 +297,7 @@ void b_vcycle(psb_batch *B, int nu1, int nu2) {
       cur = nxt;
     }
   }
-  l_finalize<4>(B, FIN_RHO);
+  l_finalize<kRhoParts>(B, FIN_RHO);
 }
 
 void b_iteration(psb_batch *B, int par, int nu1, int nu2) {
@@ -670,13 +670,13 @@ int psb_batch_plan(psb_batch *B, int *triples, int cap) {
 
 int psb_batch_solve_device(psb_batch *B, double *const *x, const double *const *b,
                            const psb_stop *p, psb_result *R) {
-  if (!B || !x || !b || !p || !R) return PSB_EINVAL;
+  if (!B || !x || !b || !p || !R || !stop_ok(p)) return PSB_EINVAL;
   return bguard(B, [&]() { return batch_solve(B, x, b, p, R); });
 }
 
 int psb_batch_solve(psb_batch *B, double *const *x, const double *const *b, const psb_stop *p,
                     psb_result *R) {
-  if (!B || !x || !b || !p || !R) return PSB_EINVAL;
+  if (!B || !x || !b || !p || !R || !stop_ok(p)) return PSB_EINVAL;
   return bguard(B, [&]() {
     const int ns = B->nsys;
     std::vector<double *> dx(ns);

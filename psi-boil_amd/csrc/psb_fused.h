@@ -62,7 +62,8 @@ template <typename M, bool FLUX, bool RHO>
 __global__ __launch_bounds__(kBlock) void k_post2(Grid g, DMat<M> A, const float *r,
                                                   float *z, ProlX<float, 4> pr, float omega,
                                                   const uint8_t *af, DevState *st, Work w,
-                                                  float zscale, int R, int TI, int ntiles) {
+                                                  float zscale, int R, int TI, int ntiles,
+                                                  const float *qf) {
   if (st->done) return;
   __shared__ float lds[kFuseCells];
   const int nz = g.nz, nv = nz / 4;
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(kBlock) void k_post2(Grid g, DMat<M> A, const float
   const int nitem = R * nv;
   // the first sweep on planes i-1, i, i+1 of the thread's items
   VecN<float, 4> prv[kFuseItems], cur[kFuseItems], nxt[kFuseItems];
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double acc[kRhoParts] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 
   // first sweep at plane ip for every item of the tile (zero outside the grid)
   auto sweep1 = [&](int ip, VecN<float, 4>(&out)[kFuseItems]) {
@@ -130,12 +131,9 @@ __global__ __launch_bounds__(kBlock) void k_post2(Grid g, DMat<M> A, const float
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         E.v[u] = G.x.v[u] + omega * t.v[u] / G.c.v[u];
-        if (RHO) E.v[u] = zscale * E.v[u];
-        if (RHO && (!af || af[P + u])) {
-          acc[0] += (double)Rv.v[u] * (double)E.v[u];
-          acc[1] += (double)E.v[u];
-          acc[2] += (double)Rv.v[u];
-          acc[3] += 1.0;
+        if (RHO) {
+          E.v[u] = zscale * E.v[u];
+          rho_acc<float>(acc, Rv.v[u], E.v[u], qf, P + u, !af || af[P + u]);
         }
       }
       stv<float, 4>(z + P, E);
@@ -149,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_post2(Grid g, DMat<M> A, const float
       cur[m] = nxt[m];
     }
   }
-  if (RHO) block_partial<4>(acc, w.partial);
+  if (RHO) block_partial<kRhoParts>(acc, w.partial);
 }
 
 // launch geometry of k_post2 for a level, or false when it does not apply

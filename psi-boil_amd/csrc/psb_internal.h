@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <map>
+#include <tuple>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -60,7 +61,6 @@ inline Grid make_grid(int nx, int ny, int nz) {
   return g;
 }
 
-// device allocations owned by a context
 // Device allocations of a context.  release() keeps the blocks for reuse:
 // a matrix that changes every time step (two-phase flow) is re-assembled and
 // its hierarchy rebuilt with the same block sizes, which then cost no
@@ -118,11 +118,14 @@ struct Level {
   T *r = nullptr;                  // right-hand side of this level
   T *e[3] = {nullptr, nullptr, nullptr};
   uint8_t *aflag = nullptr;        // NULL: all active
-  // half-precision copy scale*(c, w, s, b) for the smoother (FP32 levels
-  // with PSB_STORE_HALF; NULL otherwise).  The sweeps on it compute
-  // corrections scaled by 1/scale.
+  // half-precision copy scale16*(c, w, s, b) for the smoother (FP32 levels,
+  // built the first time a solve asks for it; NULL before).  The sweeps on
+  // it compute corrections scaled by 1/scale.
   _Float16 *c16 = nullptr, *w16 = nullptr, *s16 = nullptr, *b16 = nullptr;
-  double scale = 1.0;
+  double scale16 = 1.0;  // the copy's scale (a power of two)
+  bool use16 = false;    // this solve's smoother reads the copy
+  double scale = 1.0;    // scale16 while use16, else 1
+  bool h16() const { return c16 && use16; }
   // flux form (FP32 levels): d = c - sum of couplings; the stencil kernels
   // then read (d, w, s, b), and c16 holds scale * d
   T *d = nullptr;
@@ -143,6 +146,7 @@ struct Hier {
   double *G = nullptr;  // dense coarsest operator (FP64)
   int m = 0;
   bool built = false;
+  T *qf = nullptr;  // q = A p for the flexible beta (set while a solve is captured)
 };
 
 template <typename T>
@@ -158,7 +162,26 @@ struct Graph {
   long long key = -1;
   double fkey = 0.0;                         // a floating-point argument baked in
   const void *px = nullptr, *pb = nullptr;  // caller arrays baked into the graph
+  long long used = 0;                        // last use (cache eviction order)
 };
+
+// Everything a captured AC-preconditioned solve bakes into its graph: the
+// kernels chosen (K-cycle levels, fused post-sweeps, FP16 copies, the CG
+// operator's storage, flexible beta), the scalars passed by value (omega,
+// coarse factor) and the caller arrays.  Solves with equal keys replay one
+// graph; a solve with other settings captures its own beside it, so that
+// switching settings back and forth never destroys a graph in use.
+struct GKey {
+  int body = 0;  // 0 FP64 AC-PCG, 1 FP32 inner solve of the refinement, 2 reliable updates
+  int chunk = 0, nu1 = 0, nu2 = 0, kmask = 0, fuse = 0, cg16 = 0, half = 0, flex = 0;
+  double omega = 0.0, cfac = 0.0;
+  const void *px = nullptr, *pb = nullptr;
+  auto tie() const {
+    return std::tie(body, chunk, nu1, nu2, kmask, fuse, cg16, half, flex, omega, cfac, px, pb);
+  }
+  bool operator<(const GKey &o) const { return tie() < o.tie(); }
+};
+constexpr size_t kGraphCacheMax = 12;
 
 // the reference's AC cycle: its own levels (smallest-prime-factor
 // coarsening), each with phi (e[0]), the right-hand side f (r) and the
@@ -220,14 +243,17 @@ struct psb_ctx {
   double *dy = nullptr;                 // scratch for psb_apply
   double *relres = nullptr;             // 2 doubles: sum r^2, sum b^2
   double ref_dt = 1.0;                  // dt of the running AC cycle (its residual)
-  Graph gr_cgdiag, gr_cgsgs, gr_bicgs, gr_ac64, gr_ac32, gr_ru;
-  double ac_omega = 0, ac_cfac = 0;
+  Graph gr_cgdiag, gr_cgsgs, gr_bicgs;
+  std::map<psb::host::GKey, Graph> gcache;  // AC-preconditioned solves
+  long long gclock = 0;
+  double ac_omega = 0;      // omega of the running AC solve
   int coarse_min = 8;  // coarsening halves a direction while it stays >= this
   int coarse_rule = 0; // 0 all directions at once, 1 x and y before z
-  bool store_half = false;  // FP16 copies of the FP32 levels for the smoother
+  // settings of the running AC solve, read while its graph is captured
   bool fuse = false;        // two post-sweeps in one pass (k_post2) where it applies
   bool cg_half = false;     // the FP32 CG operator reads the level-0 FP16 copy too
   int kmask = 0;            // K-cycle levels (bit l: level l) of the AC hierarchies
+  bool flex = false;        // flexible (Polak-Ribiere) beta in the outer CG
   int version = 0;     // bumped by every change of matrix, geometry or off mask
   psb::host::RefHier href;             // the reference's AC cycle
   psb::DevMetrics metrics{};           // metrics of an assembled matrix (device)
@@ -249,6 +275,12 @@ void destroy_graph(Graph &gr);
 void finish_result(psb_result *R, const DevState &h, int outer);
 void gather_host(const psb_layout &L, const double *src, std::vector<double> &dst);
 void scatter_host(const psb_layout &L, const std::vector<double> &src, double *dst);
+
+// the caller's struct was filled for this library's layout (psb_stop_defaults
+// / psb_cycle_defaults set struct_size): a header of another version gives
+// another size, and the library would read past the caller's struct
+inline bool stop_ok(const psb_stop *p) { return p->struct_size == (int)sizeof(psb_stop); }
+inline bool cycle_ok(const psb_cycle *c) { return c->struct_size == (int)sizeof(psb_cycle); }
 
 // runs f() and maps exceptions to status codes (no exception crosses the ABI)
 template <typename F>

@@ -224,18 +224,36 @@ __global__ __launch_bounds__(kBlock) void k_rbsgs(Grid g, DMat<T> A, T *z, const
 // ---------------------------------------------------------------------------
 // damped Jacobi sweep  e_out = e + omega (r - A e) / c, the operand either e
 // or e + cfac P eH (prolongation fused into the first post-sweep).
-// RHO (last post-sweep on level 0): partials r.z, sum z, sum r, active
-// cells, from which k_finalize forms rho = r.(z - mean z): the constants
-// span the null space of the Neumann operator, and the projection keeps the
-// search directions out of it (in FP32 a drift along it eats the digits of
-// the correction).
+// RHO (last post-sweep on level 0): kRhoParts partials, r.z, sum z, sum r,
+// active cells, from which k_finalize forms rho = r.(z - mean z) (the
+// constants span the null space of the Neumann operator, and the projection
+// keeps the search directions out of it: in FP32 a drift along it eats the
+// digits of the correction), and with qf (the flexible beta) z.q and the sum
+// of q over the active cells, q = A p of the last search direction.
 // ---------------------------------------------------------------------------
+constexpr int kRhoParts = 6;
+
+template <typename T>
+__device__ __forceinline__ void rho_acc(double (&acc)[kRhoParts], T r, T z, const T *qf, int P,
+                                        bool act) {
+  if (!act) return;
+  acc[0] += (double)r * (double)z;
+  acc[1] += (double)z;
+  acc[2] += (double)r;
+  acc[3] += 1.0;
+  if (qf) {
+    const double q = (double)qf[P];
+    acc[4] += (double)z * q;
+    acc[5] += q;
+  }
+}
+
 template <typename T, int V, bool SYM, bool PROL, bool RHO, typename M = T, bool FLUX = false>
 __device__ __forceinline__ void jacobi_body(Span sp, const Grid &g, const DMat<M> &A,
                                             const T *r, const T *e, T *eout,
                                             const ProlX<T, V> &pr, T omega, const uint8_t *af,
-                                            Work w, T zscale = T(1)) {
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+                                            Work w, T zscale = T(1), const T *qf = nullptr) {
+  double acc[kRhoParts] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   PSB_SPAN_BEGIN(g, V, sp)
   Gath<T, V> G;
   if (PROL) {
@@ -249,27 +267,24 @@ __device__ __forceinline__ void jacobi_body(Span sp, const Grid &g, const DMat<M
 #pragma unroll
   for (int u = 0; u < V; u++) {
     E.v[u] = G.x.v[u] + omega * t.v[u] / G.c.v[u];
-    if (RHO) E.v[u] = zscale * E.v[u];
-    if (RHO && (!af || af[P + u])) {
-      acc[0] += (double)R.v[u] * (double)E.v[u];
-      acc[1] += (double)E.v[u];
-      acc[2] += (double)R.v[u];
-      acc[3] += 1.0;
+    if (RHO) {
+      E.v[u] = zscale * E.v[u];
+      rho_acc<T>(acc, R.v[u], E.v[u], qf, P + u, !af || af[P + u]);
     }
   }
   stv<T, V>(eout + P, E);
   PSB_TILE_END
-  if (RHO) block_partial<4>(acc, w.partial);
+  if (RHO) block_partial<kRhoParts>(acc, w.partial);
 }
 
 template <typename T, int V, bool SYM, bool PROL, bool RHO, typename M = T, bool FLUX = false>
 __global__ __launch_bounds__(kBlock) void k_jacobi(Grid g, DMat<M> A, const T *r, const T *e,
                                                    T *eout, ProlX<T, V> pr, T omega,
                                                    const uint8_t *af, DevState *st, Work w,
-                                                   T zscale) {
+                                                   T zscale, const T *qf) {
   if (st->done) return;
   jacobi_body<T, V, SYM, PROL, RHO, M, FLUX>(grid_span(), g, A, r, e, eout, pr, omega, af, w,
-                                             zscale);
+                                             zscale, qf);
 }
 
 // ---------------------------------------------------------------------------
@@ -648,7 +663,7 @@ __global__ __launch_bounds__(kBlock) void k_bi3(Grid g, const T *c, Metric m, co
 enum FinOp {
   FIN_START_DIAG, FIN_START_AC, FIN_START_BICGS, FIN_PQ, FIN_XR_DIAG, FIN_XR_AC, FIN_RHO,
   FIN_XR_RU, FIN_IR_RESID, FIN_CENTER, FIN_RELRES, FIN_BI1, FIN_BI2, FIN_BI3, FIN_RU_START,
-  FIN_RU, FIN_RU_END, FIN_ACMEAN, FIN_ACRES
+  FIN_RU, FIN_RU_END, FIN_ACMEAN, FIN_ACRES, FIN_RHO_FLEX
 };
 
 // stop logic shared by the Krylov loops (cg.cpp:121-160, bicgs.cpp:174-210),
@@ -718,7 +733,8 @@ __device__ inline void krylov_init(DevState *st, double res, double rsq, double 
 // steps of a finished loop (or a refresh not pending) do nothing
 __device__ inline bool fin_skip(int op, const DevState *st) {
   switch (op) {
-    case FIN_PQ: case FIN_XR_DIAG: case FIN_XR_AC: case FIN_RHO: case FIN_XR_RU:
+    case FIN_PQ: case FIN_XR_DIAG: case FIN_XR_AC: case FIN_RHO: case FIN_RHO_FLEX:
+    case FIN_XR_RU:
     case FIN_BI1: case FIN_BI2: case FIN_BI3:
       return st->done != 0;
     case FIN_RU:
@@ -841,11 +857,22 @@ __device__ inline void fin_apply(int op, const double *tot, DevState *st, DevSta
     case FIN_RU_END:  // true residual at exit without convergence
       st->res_true = st->norm == 0 ? ref_norm(st, tot[1]) : sqrt(tot[0]);
       break;
-    case FIN_RHO: {
+    case FIN_RHO:
+    case FIN_RHO_FLEX: {
       const double m = tot[3] > 0.0 ? tot[1] / tot[3] : 0.0;
       const double rho = tot[0] - m * tot[2];
       st->zmean = m;
-      st->beta = st->iter == 0 ? 0.0 : rho / st->rho;
+      if (st->iter == 0) {
+        st->beta = 0.0;
+      } else if (op == FIN_RHO) {
+        st->beta = rho / st->rho;  // Fletcher-Reeves, cg.cpp:93
+      } else {
+        // flexible: beta = -z'.A p / p.A p with z' = z - m on the active
+        // cells, so the new direction is A-orthogonal to the last one
+        // whatever the preconditioner did (Notay, SISC 22 (2000) 1444-1460)
+        const double zq = tot[4] - m * tot[5];
+        st->beta = st->pq != 0.0 ? -zq / st->pq : 0.0;
+      }
       st->rho = rho;
       break;
     }

@@ -108,7 +108,8 @@ template <typename T, int V, bool PROL, bool RHO, typename M, bool FLUX>
 __device__ __forceinline__ void jacobi_march_item(const Grid &g, const DMat<M> &A, const T *r,
                                                   const T *e, T *eout, const ProlX<T, V> &pr,
                                                   T omega, const uint8_t *af, T zscale, int TI,
-                                                  int slab, int tile, double (&acc)[4]) {
+                                                  int slab, int tile, const T *qf,
+                                                  double (&acc)[kRhoParts]) {
   const int nv = g.nz / V;
   const int q = tile * kBlock + (int)threadIdx.x;
   if (q >= g.ny * nv) return;
@@ -137,12 +138,9 @@ __device__ __forceinline__ void jacobi_march_item(const Grid &g, const DMat<M> &
 #pragma unroll
     for (int u = 0; u < V; u++) {
       E.v[u] = G.x.v[u] + omega * t.v[u] / G.c.v[u];
-      if (RHO) E.v[u] = zscale * E.v[u];
-      if (RHO && (!af || af[P + u])) {
-        acc[0] += (double)R.v[u] * (double)E.v[u];
-        acc[1] += (double)E.v[u];
-        acc[2] += (double)R.v[u];
-        acc[3] += 1.0;
+      if (RHO) {
+        E.v[u] = zscale * E.v[u];
+        rho_acc<T>(acc, R.v[u], E.v[u], qf, P + u, !af || af[P + u]);
       }
     }
     stv<T, V>(eout + P, E);
@@ -158,13 +156,13 @@ __device__ __forceinline__ void jacobi_march_body(Span sp, const Grid &g, const 
                                                   const T *r, const T *e, T *eout,
                                                   const ProlX<T, V> &pr, T omega,
                                                   const uint8_t *af, Work w, T zscale, int TI,
-                                                  int ntiles) {
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+                                                  int ntiles, const T *qf = nullptr) {
+  double acc[kRhoParts] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int items = ntiles * ((g.nx + TI - 1) / TI);
   for (int wi = sp.bl; wi < items; wi += sp.nb)
     jacobi_march_item<T, V, PROL, RHO, M, FLUX>(g, A, r, e, eout, pr, omega, af, zscale, TI,
-                                                wi / ntiles, wi % ntiles, acc);
-  if (RHO) block_partial<4>(acc, w.partial);
+                                                wi / ntiles, wi % ntiles, qf, acc);
+  if (RHO) block_partial<kRhoParts>(acc, w.partial);
 }
 
 // damped Jacobi sweep e_out = e + omega (r - A e) / c, the operand e or,
@@ -176,10 +174,10 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_march(Grid g, DMat<M> A, cons
                                                          const T *e, T *eout, ProlX<T, V> pr,
                                                          T omega, const uint8_t *af,
                                                          const DevState *st, Work w, T zscale,
-                                                         int TI, int ntiles) {
+                                                         int TI, int ntiles, const T *qf) {
   if (st->done) return;
   jacobi_march_body<T, V, PROL, RHO, M, FLUX>(grid_span(), g, A, r, e, eout, pr, omega, af, w,
-                                              zscale, TI, ntiles);
+                                              zscale, TI, ntiles, qf);
 }
 
 // CG step 1 (cg.cpp:75-108): p = z + beta p_old with z = r/c (DIAG) or the

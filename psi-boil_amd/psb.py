@@ -40,14 +40,15 @@ class Matrix7(C.Structure):
 
 
 class Stop(C.Structure):
-    _fields_ = [("min_iter", C.c_int), ("max_iter", C.c_int), ("res_rat", C.c_double),
-                ("res_tol", C.c_double), ("scale", C.c_double), ("stalecount", C.c_int),
-                ("precform", C.c_int), ("method", C.c_int), ("precond", C.c_int),
+    _fields_ = [("struct_size", C.c_int), ("min_iter", C.c_int), ("max_iter", C.c_int),
+                ("res_rat", C.c_double), ("res_tol", C.c_double), ("scale", C.c_double),
+                ("stalecount", C.c_int), ("precform", C.c_int), ("method", C.c_int), ("precond", C.c_int),
                 ("precision", C.c_int), ("norm", C.c_int), ("ac_pre", C.c_int),
                 ("ac_post", C.c_int), ("ac_omega", C.c_double), ("ac_cfac", C.c_double),
                 ("inner_rtol", C.c_double), ("max_outer", C.c_int), ("graph_chunk", C.c_int),
                 ("inner_adapt", C.c_int), ("ac_coarsen", C.c_int),
-                ("ac_store", C.c_int), ("ac_fuse", C.c_int), ("ac_kcycle", C.c_int)]
+                ("ac_store", C.c_int), ("ac_fuse", C.c_int), ("ac_kcycle", C.c_int),
+                ("ac_flexible", C.c_int)]
 
 
 class Result(C.Structure):
@@ -70,7 +71,8 @@ PSB_CYCLE_NONE, PSB_CYCLE_V, PSB_CYCLE_F, PSB_CYCLE_W = -1, 0, 1, 2
 
 
 class Cycle(C.Structure):
-    _fields_ = [("res_tol", C.c_double), ("res_rat", C.c_double), ("max_cycles", C.c_int),
+    _fields_ = [("struct_size", C.c_int), ("res_tol", C.c_double), ("res_rat", C.c_double),
+                ("max_cycles", C.c_int),
                 ("min_cycles", C.c_int), ("max_iter", C.c_int * 3), ("dt", C.c_double),
                 ("loop", C.c_int), ("init", C.c_int), ("stop_if_div", C.c_int),
                 ("priority_min_cyc", C.c_int), ("use_linf", C.c_int),

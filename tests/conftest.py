@@ -12,7 +12,8 @@ for sub in ("oracle", "psi-boil_amd"):
         sys.path.insert(0, p)
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-GOLDEN_CASES = ["drop12", "bubbles16x12x8", "bubbles16"]
+GOLDEN_CASES = ["drop12", "bubbles16x12x8", "bubbles16", "stretch12x10x8"]
+PREC_CASES = ["drop12", "bubbles16x12x8", "bubbles16"]  # <case>_prec.npz: IC0 / IC2 runs
 CYCLE_CASES = ["bubbles16x12x8", "bubbles16"]  # <case>_cycles.npz: AC::cycle runs
 
 

@@ -8,7 +8,7 @@ outputs, solver iterates after k iterations, AC level matrices, restriction /
 interpolation results and V-cycle iterates, plus the solver log lines (17
 significant digits) parsed into arrays.
 
-    python tests/golden/make_golden.py [cases] [cycles] [prec]   # needs oracle/_ref built
+    python tests/golden/make_golden.py [cases] [cycles] [prec] [case names]   # needs oracle/_ref built
 
 "cycles" writes <case>_cycles.npz: the reference's AC::cycle with the F, W
 and full-cycle shapes and its controls (ref_driver cycles), phi after each
@@ -37,7 +37,17 @@ CASES = {
     "drop12": (12, 12, 12, 1, 0.25, 0.25, 1000.0, 1.0, 1, 1001, (0.5, 0.5, 0.5)),
     "bubbles16x12x8": (16, 12, 8, 3, 0.1, 0.25, 1.0, 1000.0, 5, 1005, None),
     "bubbles16": (16, 16, 16, 4, 0.08, 0.2, 1.0, 1000.0, 2, 1002, None),
+    "stretch12x10x8": (12, 10, 8, 3, 0.1, 0.25, 1.0, 1000.0, 9, 1009, None),
 }
+
+# cases on a non-uniform grid (ref_driver case-s): the reference's Grid1D
+# stretched from a first to a last cell size per direction (d1x, dnx, d1y,
+# dny, d1z, dnz in units of h = 1 / max(nx, ny, nz)), so that the distance
+# between cell centres differs from the cell size (dxw != dxc)
+STRETCH = {"stretch12x10x8": (0.5, 2.0, 2.0, 0.7, 1.0, 3.0)}
+
+# cases of the incomplete-Cholesky runs (ref_driver prec)
+PREC_CASES = ["drop12", "bubbles16x12x8", "bubbles16"]
 
 KEEP_F64 = ["xn", "yn", "zn", "dxc", "dxw", "dxe", "dyc", "dys", "dyn", "dzc", "dzb", "dzt",
             "spheres", "rho", "A_c", "A_w", "A_e", "A_s", "A_n", "A_b", "A_t", "A_ci", "dV",
@@ -80,9 +90,12 @@ def parse_log(text):
 
 def run_case(name, spec):
     nx, ny, nz, nsph, rmin, rmax, rin, rout, ss, sv, centre = spec
+    stretch = STRETCH.get(name)
     with tempfile.TemporaryDirectory() as d:
-        args = [DRIVER, "case", d, str(nx), str(ny), str(nz), str(nsph), repr(rmin), repr(rmax),
-                repr(rin), repr(rout), str(ss), str(sv)]
+        args = [DRIVER, "case-s" if stretch else "case", d, str(nx), str(ny), str(nz), str(nsph),
+                repr(rmin), repr(rmax), repr(rin), repr(rout), str(ss), str(sv)]
+        if stretch is not None:
+            args += [repr(v) for v in stretch]
         if centre is not None:
             args += [repr(c) for c in centre]
         out = subprocess.run(args, check=True, capture_output=True, text=True).stdout
@@ -98,6 +111,7 @@ def run_case(name, spec):
     meta = {"name": name, "shape": [nx, ny, nz], "nsph": nsph, "rmin": rmin, "rmax": rmax,
             "rho_in": rin, "rho_out": rout, "seed_sph": ss, "seed_vel": sv,
             "centre": list(centre) if centre is not None else None,
+            "stretch": list(stretch) if stretch is not None else None,
             "cg_snapshots": [1, 5], "bicgs_snapshots": [1], "info": info}
     arrays["meta"] = np.array(json.dumps(meta))
     path = os.path.join(HERE, name + ".npz")
@@ -188,15 +202,19 @@ def main():
     if not os.path.exists(DRIVER):
         sys.exit("oracle/_ref/ref_driver is not built (make -f oracle/ref.mk)")
     what = sys.argv[1:] or ["cases", "cycles", "prec"]
+    only = [w for w in what if w in CASES]  # restrict to the cases named
     if "cases" in what:
         for name, spec in CASES.items():
-            print(run_case(name, spec))
+            if not only or name in only:
+                print(run_case(name, spec))
     if "cycles" in what:
         for name in CYCLE_CASES:
-            print(run_cycles(name))
+            if not only or name in only:
+                print(run_cycles(name))
     if "prec" in what:
-        for name in CASES:
-            print(run_prec(name))
+        for name in PREC_CASES:
+            if not only or name in only:
+                print(run_prec(name))
 
 
 if __name__ == "__main__":

@@ -37,6 +37,17 @@ def test_stop_defaults_are_the_reference_defaults():
     assert s.precond == psb.PSB_PREC_DIAG and s.precision == psb.PSB_FP64
 
 
+def test_structs_carry_their_size():
+    # psb_stop_defaults / psb_cycle_defaults write sizeof() of the library's
+    # structs: the ctypes mirrors must have the same layout, and a caller built
+    # against another header version is recognised by the entry points
+    s = psb.stop_defaults()
+    assert s.struct_size == C.sizeof(psb.Stop)
+    c = psb._cycle()
+    assert c.struct_size == C.sizeof(psb.Cycle)
+    assert s.ac_flexible == -1
+
+
 def test_status_strings():
     L = psb.lib()
     assert L.psb_status_string(0) == b"converged"

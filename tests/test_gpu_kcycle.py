@@ -31,21 +31,25 @@ AC = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=1e-8, res_to
           max_iter=300)
 
 
-@pytest.mark.parametrize("mask", [0b10, 0b110])
-def test_kcycle_fp64_matches_oracle(mask):
+@pytest.mark.parametrize("mask,flex", [(0b10, -1), (0b110, -1), (0b110, 0), (0, 1)])
+def test_kcycle_fp64_matches_oracle(mask, flex):
+    """flex: the outer CG's beta, -1 the default (flexible with the K-cycle,
+    Fletcher-Reeves with the V-cycle), 0 Fletcher-Reeves, 1 flexible."""
     n = 64
-    print(f"[kcycle] fp64 {n}^3 mask {mask:b}", flush=True)
+    print(f"[kcycle] fp64 {n}^3 mask {mask:b} flexible {flex}", flush=True)
     d, s = system(n)
     b = d["rhs"]
-    x, r = s.solve(np.zeros(b.size), b, ac_kcycle=mask, **AC)
+    x, r = s.solve(np.zeros(b.size), b, ac_kcycle=mask, ac_flexible=flex, **AC)
     A = O.Mat(n, n, n, {k: d[k] for k in "cwesnbt"})
     ac = O.AC(A, np.ones(b.size), rule=1)
     try:
         O.set_kcycle(mask)
+        O.set_flexible(flex == 1 or (flex == -1 and mask != 0))
         xo, info = ac.acpcg(np.zeros(b.size), b, rtol=1e-8, maxit=300, nu1=2, nu2=2, omega=0.9,
                             cfac=2.0)
     finally:
         O.set_kcycle(0)
+        O.set_flexible(False)
     assert r["converged"] and info["converged"]
     assert abs(r["iterations"] - info["iters"]) <= 1, (r["iterations"], info["iters"])
     assert r["rel_l2"] < 1.2e-8
@@ -134,10 +138,12 @@ def test_kcycle_on_semi_coarsened_levels_matches_oracle():
     ac = O.AC(A, np.ones(n), off=off, rule=3)
     try:
         O.set_kcycle(mask)
+        O.set_flexible(True)  # the default beta with the K-cycle
         xo, info = ac.acpcg(np.zeros(n), b, rtol=1e-8, maxit=400, nu1=2, nu2=2, omega=0.9,
                             cfac=1.0)
     finally:
         O.set_kcycle(0)
+        O.set_flexible(False)
     assert r["converged"] and info["converged"]
     assert abs(r["iterations"] - info["iters"]) <= 1, (r["iterations"], info["iters"])
     fl = d["off"] == 0

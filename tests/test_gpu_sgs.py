@@ -14,7 +14,7 @@ import pytest
 
 import psb
 import psb_oracle as O
-from conftest import GOLDEN, GOLDEN_CASES, load_golden
+from conftest import GOLDEN, PREC_CASES, load_golden
 
 pytestmark = pytest.mark.gpu
 
@@ -37,7 +37,7 @@ def solver_for(g):
 SGS = dict(precond=psb.PSB_PREC_SGS, res_tol=1e-15)
 
 
-@pytest.mark.parametrize("case", GOLDEN_CASES)
+@pytest.mark.parametrize("case", PREC_CASES)
 def test_sgs_cg_matches_oracle_and_reaches_the_ic2_answer(case):
     g = load_golden(case)
     print(f"[sgs] {case}", flush=True)

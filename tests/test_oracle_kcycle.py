@@ -153,3 +153,40 @@ def test_kcycle_preconditioned_cg_converges():
     assert i1["iters"] <= i0["iters"], (i0["iters"], i1["iters"])
     r = b - O.spmv(A, x1)
     assert np.linalg.norm(r - r.mean()) < 1e-7 * np.linalg.norm(b - b.mean())
+
+
+def _orth_run(ac, b, mask, flex):
+    kw = dict(rtol=1e-10, maxit=300, nu1=2, nu2=2, omega=0.9, cfac=2.0)
+    try:
+        O.set_kcycle(mask)
+        O.set_flexible(flex)
+        return ac.acpcg(np.zeros(b.size), b, orth=True, **kw)
+    finally:
+        O.set_kcycle(0)
+        O.set_flexible(False)
+
+
+def test_flexible_beta_keeps_directions_a_orthogonal():
+    """The K-cycle changes the preconditioner from one application to the
+    next, so the Fletcher-Reeves beta rho_new / rho no longer makes the new
+    search direction A-orthogonal to the last one; the flexible beta
+    -z.A p / p.A p does by construction.  The record is |p_k+1.A p_k| over
+    the A-norms of both directions, computed with a separate product."""
+    A, ci, dV, b = O.sphere_case(48, 8, 0.05, 0.15, 1.0, 1000.0, 7, 107)
+    ac = O.AC(A, dV, rule=1)
+    xf, inf = _orth_run(ac, b, 0b110, True)
+    xr, inr = _orth_run(ac, b, 0b110, False)
+    assert inf["converged"] and inr["converged"]
+    assert inf["orth"].size > 5 and inr["orth"].size > 5
+    assert inf["orth"].max() < 1e-9, inf["orth"].max()
+    assert inr["orth"].max() > 1e-5, inr["orth"].max()
+    # both reach the same solution
+    r = b - O.spmv(A, xf)
+    assert np.linalg.norm(r - r.mean()) < 1e-9 * np.linalg.norm(b - b.mean())
+    # with a fixed (linear, symmetric) preconditioner, the V-cycle, the two
+    # formulas agree and both keep the directions A-orthogonal to rounding
+    xv, iv = _orth_run(ac, b, 0, True)
+    xw, iw = _orth_run(ac, b, 0, False)
+    assert abs(iv["iters"] - iw["iters"]) <= 1
+    assert iv["orth"].max() < 1e-9 and iw["orth"].max() < 1e-6
+    assert np.linalg.norm((xv - xv.mean()) - (xw - xw.mean())) < 1e-8 * np.linalg.norm(xv)

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import psb_oracle as O
-from conftest import GOLDEN, GOLDEN_CASES, load_golden
+from conftest import GOLDEN, PREC_CASES, load_golden
 
 
 def load_prec(name):
@@ -21,7 +21,7 @@ def load_prec(name):
     return {k: z[k] for k in z.files if k != "meta"}, json.loads(str(z["meta"]))
 
 
-@pytest.mark.parametrize("case", GOLDEN_CASES)
+@pytest.mark.parametrize("case", PREC_CASES)
 @pytest.mark.parametrize("prec", ["ic0", "ic2"])
 def test_incomplete_cholesky_cg_matches_reference(case, prec):
     g = load_golden(case)
@@ -112,7 +112,7 @@ def test_rbsgs_without_couplings_is_the_diagonal():
     assert np.array_equal(O.prec_apply(M, "rbsgs", r), r / A["c"])
 
 
-@pytest.mark.parametrize("case", GOLDEN_CASES)
+@pytest.mark.parametrize("case", PREC_CASES)
 def test_rbsgs_cg_sits_between_diagonal_and_ic2(case):
     """On the fixture systems (density ratio 1000) CG with the red-black
     Gauss-Seidel needs about half the iterations of the diagonal
