@@ -122,16 +122,22 @@ def pressure_step(n, steps, kcycle=0):
             "divergence_after_projection_rel": float(np.linalg.norm(div) / np.linalg.norm(b))}
 
 
-def measured_traffic(n):
-    """HBM bytes per launch of the FP32 Jacobi sweep at 256^3 from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a
-    plain copy), or None."""
-    path = os.path.join(ROOT, "profiles", "jacobi_spmv_256_hbm_traffic.md")
-    if n != 256 or not os.path.exists(path):
+def measured_traffic(n, kind):
+    """HBM bytes per launch of the fine-level kernel `kind` at 256^3 from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries (calibrated against
+    a plain copy): a line "traffic_bytes_per_launch kind=<kind> <bytes>" in a
+    profiles/*hbm_traffic*.md file; None where none was measured."""
+    pdir = os.path.join(ROOT, "profiles")
+    if n != 256 or not os.path.isdir(pdir):
         return None
-    m = re.search(r"traffic_bytes_per_launch k_jacobi_march<float,4,fp16,flux> (\d+)",
-                  open(path).read())
-    return int(m.group(1)) if m else None
+    for f in sorted(os.listdir(pdir)):
+        if "hbm_traffic" not in f:
+            continue
+        m = re.search(r"traffic_bytes_per_launch kind=%s (\d+)" % re.escape(kind),
+                      open(os.path.join(pdir, f)).read())
+        if m:
+            return int(m.group(1))
+    return None
 
 
 def ref_driver(mode, n, limit):
@@ -146,6 +152,47 @@ def ref_driver(mode, n, limit):
     out = subprocess.run(args, capture_output=True, text=True, timeout=900).stdout
     m = re.search(r"solve_s=([0-9.eE+-]+)", out)
     return float(m.group(1)) if m else None
+
+
+def host_cpu():
+    """CPU model and the cores this process may use (the host the reference's
+    CPU sample runs on)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count()
+    return {"cpu_model": model, "nproc": avail, "cpu_count": os.cpu_count()}
+
+
+def in_solve_kernels(s, dx, db, stop):
+    """One solve with the same settings, launched kernel by kernel, the
+    finest level's kernels bracketed by HIP events on the solver's stream:
+    per kind the launches, average duration, algorithmic bytes per launch,
+    GB/s and fraction of the HBM peak; plus the eager solve's total."""
+    dx.zero()
+    res, st, solve_ms = s.profile_solve_device(dx.addr, db.addr, **stop)
+    out = {}
+    for k, v in st.items():
+        if not v["launches"]:
+            continue
+        avg = v["ms"] / v["launches"]
+        gbs = v["bytes"] / (avg * 1e-3) / 1e9
+        out[k] = {"launches": v["launches"], "avg_us": round(avg * 1e3, 2),
+                  "total_ms": round(v["ms"], 3), "bytes_per_launch": v["bytes"],
+                  "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    fine = sum(v["ms"] for v in st.values())
+    return {"kernels": out, "eager_solve_ms": round(solve_ms, 3),
+            "fine_level_ms": round(fine, 3),
+            "coarse_levels_reductions_gaps_ms": round(solve_ms - fine, 3),
+            "iterations": res["iterations"]}
 
 
 def cpu_baseline(n, cg_iters_full, cg_sample):
@@ -163,7 +210,7 @@ def cpu_baseline(n, cg_iters_full, cg_sample):
         return None
     s_iter = t_cg / cg_sample
     out = {"value": 1.0 / (s_iter * cg_iters_full), "unit": "solves/s", "cores": 1,
-           "kind": "reference",
+           "kind": "reference", **host_cpu(),
            "sample": f"reference CG-di (Src/Solver/Linear/Krylov/cg.cpp) {cg_sample} iterations "
                      f"at {n}^3, {s_iter:.3f} s/iteration on 1 core, scaled by the "
                      f"{cg_iters_full} iterations CG-di needs on this system",
@@ -372,8 +419,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ref-cycle", action="store_true",
-                    help="leave out the reference AC::vcycle timing (rocprofv3 kernel tracing "
-                         "faults on its ~1500-node captured graph)")
+                    help="leave out the reference AC::vcycle timing")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--graph-chunk", type=int, default=8,
                     help="CG iterations per captured graph of the headline solves (smaller "
@@ -439,28 +485,73 @@ def main():
     tmx, rmx = timed_solves(s, dx, db, a.steps, a.warmup, **dict(common, **mixed))
     log(f"[bench] mixed {tmx * 1e3:.2f} ms/solve {rmx}")
     x_mixed = dx.download()
+    # the method as first specified: FP32 smoother matrices (no FP16 copy)
+    mixed32 = dict(mixed, ac_store=0)
+    tm32, rm32 = timed_solves(s, dx, db, a.steps, a.warmup, **dict(common, **mixed32))
+    log(f"[bench] mixed, FP32 smoother {tm32 * 1e3:.2f} ms/solve {rm32}")
+    x_m32 = dx.download()
     # FP64 reference solution to 1e-12 for the error figure (untimed)
     dx.zero()
     r12 = s.solve_device(dx.addr, db.addr, precision=psb.PSB_FP64, **dict(common, res_rat=1e-12))
     x_ref = dx.download()
-    xm, xr = x_mixed - x_mixed.mean(), x_ref - x_ref.mean()
-    err = float(np.linalg.norm(xm - xr) / np.linalg.norm(xr))
+    xr = x_ref - x_ref.mean()
+
+    def err_of(x):
+        x = x - x.mean()
+        return float(np.linalg.norm(x - xr) / np.linalg.norm(xr))
+
+    err, err32 = err_of(x_mixed), err_of(x_m32)
     # the same figure for the FP64 solve to the same tolerance: what a 1e-8
     # residual allows on this system, whatever the precision
-    x64 = x_fp64 - x_fp64.mean()
-    err64 = float(np.linalg.norm(x64 - xr) / np.linalg.norm(xr))
-    # dominant kernel: FP32 damped Jacobi sweep on the fine level
+    err64 = err_of(x_fp64)
+    # the residual tolerance each path needs for a given solution error
+    # (untimed): mixed <= 1e-7, FP64 <= 1e-8
+    tol_sweep = {"mixed": [], "fp64": []}
+    for tol in (1e-8, 5e-9, 2e-9, 1e-9, 5e-10, 2e-10, 1e-10):
+        for name, kw in (("mixed", mixed), ("fp64", dict(precision=psb.PSB_FP64))):
+            dx.zero()
+            rr = s.solve_device(dx.addr, db.addr, **dict(common, res_rat=tol, **kw))
+            tol_sweep[name].append({"res_rat": tol, "iterations": rr["iterations"],
+                                    "rel_l2": rr["rel_l2"], "error": err_of(dx.download())})
+    tol_for = {name: next((t["res_rat"] for t in tol_sweep[name] if t["error"] <= lim), None)
+               for name, lim in (("mixed", 1e-7), ("fp64", 1e-8))}
+    log(f"[bench] tolerance for the error bounds {tol_for}")
+    # per-kernel durations inside the solve (the finest level's kernels
+    # bracketed by HIP events in an eager run of the same solve)
+    prof_mx = in_solve_kernels(s, dx, db, dict(common, **mixed))
+    prof_64 = in_solve_kernels(s, dx, db, dict(common, precision=psb.PSB_FP64))
+    log(f"[bench] in-solve mixed {prof_mx}")
+    log(f"[bench] in-solve fp64 {prof_64}")
+    # the dominant kernel: the fine-level kind with the largest time per solve
+    kern = max(prof_mx["kernels"].items(), key=lambda kv: kv[1]["total_ms"])
+    names = {"cg_pq": "k_cg_pq_march<float,4,SYM,-,flux> (p = z + beta p, q = A p, p.q; "
+                      "FP32 matrix)",
+             "sweep_prol": "k_jacobi<float,4,...,_Float16,flux> (first post-sweep with the "
+                           "prolongation fused; FP16 matrix copy)",
+             "sweep": "k_jacobi_march<float,4,...,_Float16,flux> (damped Jacobi sweep, FP16 "
+                      "matrix copy)",
+             "sweep_last": "k_jacobi_march<float,4,RHO,...> (last post-sweep with the rho "
+                           "partials)",
+             "restrict": "k_restrict_march<float,4,...> (residual block sums, coarse first "
+                         "sweep fused)",
+             "cg_xr": "k_cg_xr<float,4> (x, r update, norms, next first sweep)"}
+    # isolated back-to-back launches of the fine-level sweep and SpMVs
     ms_j, by_j = s.time_kernel(2, reps=50)
     ms_s32, by_s32 = s.time_kernel(1, reps=50)
     ms_s64, by_s64 = s.time_kernel(0, reps=50)
     gbs = lambda by, ms: by / (ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": "k_jacobi_march<float,4,...,_Float16,flux> (fine-level "
-                                      "damped Jacobi sweep, FP16 matrix copy, marching)",
-            "achieved": round(gbs(by_j, ms_j), 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs(by_j, ms_j) / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n),
-            "avg_launch_ms": ms_j, "bytes_per_launch": by_j,
-            "spmv_fp32_GBs": round(gbs(by_s32, ms_s32), 1),
-            "spmv_fp64_GBs": round(gbs(by_s64, ms_s64), 1)}
+    kv = kern[1]
+    roof = {"bound": "hbm", "kernel": names.get(kern[0], kern[0]), "kind": kern[0],
+            "achieved": kv["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": kv["frac"], "traffic": measured_traffic(n, kern[0]),
+            "avg_launch_ms": kv["avg_us"] * 1e-3, "bytes_per_launch": kv["bytes_per_launch"],
+            "measured": "in-solve: HIP events around each launch on the solver's stream in an "
+                        "eager run of the headline solve",
+            "in_solve_fine_kernels": prof_mx["kernels"],
+            "isolated": {"jacobi_fp16_GBs": round(gbs(by_j, ms_j), 1),
+                         "spmv_fp32_GBs": round(gbs(by_s32, ms_s32), 1),
+                         "spmv_fp64_GBs": round(gbs(by_s64, ms_s64), 1),
+                         "note": "50 back-to-back launches on the same arrays"}}
     # the reference's own multigrid (AC::vcycle) on the GPU: cycles and time
     ref_cycle = None
     if not a.no_ref_cycle:
@@ -519,9 +610,17 @@ def main():
                    "mixed_ms_per_solve": round(tmxv * 1e3, 3),
                    "iterations": {"fp64": r64v["iterations"], "mixed": rmxv["iterations"]},
                    "mixed_over_fp64": round(t64v / tmxv, 3)},
+        "fp32_smoother": {"ms_per_solve": round(tm32 * 1e3, 3),
+                          "solves_per_s": round(1.0 / tm32, 3),
+                          "mixed_over_fp64": round(t64 / tm32, 3),
+                          "iterations": rm32["iterations"], "rel_l2": rm32["rel_l2"],
+                          "error_vs_fp64_1e-12": err32},
         "rel_residual": {"fp64": r64["rel_l2"], "mixed": rmx["rel_l2"]},
         "error_vs_fp64_1e-12": err,
         "fp64_solve_error_vs_fp64_1e-12": err64,
+        "tolerance_for_error": {"mixed_error_le_1e-7": tol_for["mixed"],
+                                "fp64_error_le_1e-8": tol_for["fp64"], "sweep": tol_sweep},
+        "in_solve": {"mixed": prof_mx, "fp64": prof_64},
         "reference_ac_vcycle_gpu": ref_cycle,
         "pressure_step_gpu": step,
         "roofline": roof,

@@ -413,6 +413,37 @@ int psb_batch_solve_device(psb_batch *bt, double *const *d_x,
 int psb_time_kernel(psb_ctx *ctx, int kernel, int reps, double *avg_ms,
                     double *bytes);
 
+/* Benchmark hook: one solve exactly as psb_solve_device runs it (same
+ * kernels in the same order on the same data), but launched kernel by kernel
+ * instead of as captured graphs, each kernel on the finest level bracketed
+ * by HIP events on the context's stream.  Per kernel kind: launches, their
+ * summed duration (ms) and the algorithmic bytes one launch moves (the
+ * arrays it must read and write once: matrix in its stored precision,
+ * vectors, 1/8 of a coarse array for a restriction or prolongation).
+ * *solve_ms: the whole eager solve between two events; what the fine-level
+ * kernels leave of it is the coarse levels, the one-block reductions and
+ * launch gaps.  The gated kernels of the reliable updates count every
+ * launch, most of which return at once. */
+enum psb_kernel_kind {
+  PSB_K_SWEEP = 0,    /* damped Jacobi sweep (pre-smoothing, later post-sweeps) */
+  PSB_K_SWEEP_PROL,   /* first post-sweep, prolongation of the coarse correction fused */
+  PSB_K_SWEEP_LAST,   /* last post-sweep of the finest level, the rho partials fused */
+  PSB_K_RESTRICT,     /* residual block sums to level 1, its first sweep fused */
+  PSB_K_CG_PQ,        /* p = z + beta p, q = A p, p.q */
+  PSB_K_CG_XR,        /* x += alpha p, r -= alpha q, norms, next first sweep */
+  PSB_K_RU_RESID,     /* FP64 true residual of a reliable update (gated) */
+  PSB_K_RU_XUPDATE,   /* x += d of a reliable update (gated) */
+  PSB_K_COUNT
+};
+typedef struct {
+  int launches;
+  double ms;      /* summed over the launches */
+  double bytes;   /* algorithmic bytes of one launch */
+} psb_kernel_stat;
+int psb_profile_solve_device(psb_ctx *ctx, double *d_x, const double *d_b,
+                             const psb_stop *stop, psb_result *res,
+                             psb_kernel_stat *stats, double *solve_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1127,6 +1127,39 @@ static void ac_loop(ora_ac *ac, int l, int kind, const cyc_par *q) {
   else ac_fw(ac, l, q, kind == ORA_CYC_W);
 }
 
+/* flex_cycle (additive_recursive.cpp:161-200) from level 0, its tail
+ * recursion as a loop: smooth (MaxIter mv[0] on the way down, mv[2] on the
+ * way up); a level whose smoother met its tolerance hands its phi up
+ * (interpolation) and the finer level smooths again, level 0 ends the cycle;
+ * a level whose smoother did not restricts its residual and goes down; the
+ * coarsest level is solved (MaxIter 100) and hands up.  The reference bounds
+ * nothing (once a level's smoother cannot meet its ResRat the cycle
+ * alternates between two levels for good); here a cycle ends after
+ * max_visits level visits: returns 1 then, 0 when level 0 ended it. */
+static int ac_flex(ora_ac *ac, const cyc_par *q, int max_visits) {
+  int l = 0, up = 0;
+  for (int v = 0; v < max_visits; v++) {
+    if (l != ac->nlev - 1) {
+      const int s = up ? 2 : 0;
+      if (ac_cg(ac, l, q->mv[s], q->rr, q->rt, q->scale, q->ms[s])) {
+        if (l == 0) return 0;
+        ora_interpolation(ac, l);
+        l--;
+        up = 1;
+      } else {
+        ora_restriction(ac, l);
+        l++;
+        up = 0;
+      }
+    } else {
+      ac_coarsest(ac, l, q);
+      l--;
+      up = 0;  /* the default sign of the call (additive.h:185-187) */
+    }
+  }
+  return 1;
+}
+
 /* full_cycle (:127-163): restrict, recurse to the coarsest level, then run
  * the chosen cycle from each level on the way back */
 static void ac_full(ora_ac *ac, int l, int kind, const cyc_par *q) {
@@ -1154,6 +1187,7 @@ void ora_cycle_defaults(ora_cycle *P) {
   P->use_linf = 0;
   P->resrat = ATTO;
   P->restol = ATTO;
+  P->max_visits = 1000;
 }
 
 /* AC::converged (additive_cycle_functions.cpp:49-87) */
@@ -1215,7 +1249,17 @@ int ora_ac_cycle(ora_ac *ac, double *phi0, const double *f0, const ora_cycle *P,
   }
   if (ncyc) *ncyc = P->max_cyc;
   for (int c = 1; c <= P->max_cyc; c++) {
-    ac_loop(ac, 0, P->loop, &q);
+    if (P->loop == ORA_CYC_FLEX) {
+      if (ac_flex(ac, &q, P->max_visits)) {
+        /* no end in sight: the last cycle's phi, as after a divergence */
+        memcpy(ac->phi[0], fold, N0 * 8);
+        if (ncyc) *ncyc = c;
+        ret = -1;
+        goto done;
+      }
+    } else {
+      ac_loop(ac, 0, P->loop, &q);
+    }
     const int con = ac_converged(ac, P, c, &res_m0, &linf_m0, res_0, linf_0, res_m1, linf_m1,
                                  fold);
     PUSH(res_m0, linf_m0);

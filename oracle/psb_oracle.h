@@ -169,9 +169,12 @@ int  ora_ac_vcycle(ora_ac *ac, double *phi0, const double *f0, double toler,
 
 /* AC::cycle (additive_cycle.cpp:4-99): the loop's cycle shape, an optional
  * full-cycle initialisation (additive_recursive.cpp:127-163) and the
- * controls of additive.h:97-142.  Cycle::flex is not restated: its
- * recursion ends only when a level's smoother meets its ResRat. */
-enum { ORA_CYC_NONE = -1, ORA_CYC_V = 0, ORA_CYC_F = 1, ORA_CYC_W = 2 };
+ * controls of additive.h:97-142.  Cycle::flex (loop only) with a bound of
+ * max_visits level visits per cycle (the reference has none: its recursion
+ * ends only when a level's smoother meets its ResRat); a cycle that reaches
+ * it ends the run, ora_ac_cycle returns -1 and phi holds the previous
+ * cycle's result. */
+enum { ORA_CYC_NONE = -1, ORA_CYC_V = 0, ORA_CYC_F = 1, ORA_CYC_W = 2, ORA_CYC_FLEX = 3 };
 typedef struct {
   double toler, factor;          /* ResTol, ResRat of the loop               */
   int max_cyc, min_cyc;
@@ -181,6 +184,7 @@ typedef struct {
   int init, loop;                /* ORA_CYC_*                                */
   int stop_if_div, priority_min_cyc, use_linf;
   double resrat, restol;         /* level solves (AC::resrat / restol)       */
+  int max_visits;                /* Cycle::flex: level visits per cycle      */
 } ora_cycle;
 void ora_cycle_defaults(ora_cycle *P);
 /* hist, hist_linf: max_cyc + 2 entries; *nhist receives the count.          */

@@ -43,7 +43,7 @@ OBJS := $(patsubst $(REF)/%.cpp,$(OUT)/obj/%.o,$(SRCS))
 CFG  := $(OUT)/cfg
 INCS := -I$(CFG)/Parallel -I$(CFG)/a -I$(CFG)/a/b -I$(CFG)/a/b/c
 
-all: $(OUT)/ref_driver
+all: $(OUT)/ref_driver $(OUT)/gk_driver
 
 $(CFG)/.stamp:
 	mkdir -p $(CFG)/Parallel $(CFG)/a/b/c
@@ -64,5 +64,15 @@ $(OUT)/oracle_c.o: oracle/psb_oracle.c oracle/psb_oracle.h
 $(OUT)/ref_driver: oracle/ref_driver.cpp $(OUT)/libboil.a $(OUT)/oracle_c.o
 	$(CXX) $(CXXFLAGS) $(INCS) -I$(REF) -Ioracle $< $(OUT)/oracle_c.o \
 	  -Wl,--start-group $(OUT)/libboil.a -Wl,--end-group -o $@ -lm
+
+# the reference's callers of Linear::solve with the GpuKrylov binding
+# (include/psb_krylov.hpp) over libpsb.so (built first by psi-boil_amd/build.py);
+# the rpath finds the library from the binary's own directory
+LIBPSB := psi-boil_amd/libpsb.so
+$(OUT)/gk_driver: oracle/gk_driver.cpp oracle/ref_driver.cpp include/psb_krylov.hpp include/psb.h \
+                  $(OUT)/libboil.a $(OUT)/oracle_c.o $(LIBPSB)
+	$(CXX) $(CXXFLAGS) $(INCS) -I$(REF) -Ioracle -Iinclude $< $(OUT)/oracle_c.o \
+	  -Wl,--start-group $(OUT)/libboil.a -Wl,--end-group -Lpsi-boil_amd -lpsb \
+	  -Wl,-rpath,'$$ORIGIN/../../psi-boil_amd' -o $@ -lm
 
 .PHONY: all

@@ -155,7 +155,9 @@ struct World {
   std::vector<double> sph;
 };
 
-World build_world(const Case &cs, Krylov *solver_override = nullptr) {
+// make_solver: the solver every level of the case's Pressure holds (NULL: the
+// reference's CG with Prec::di())
+World build_world(const Case &cs, Krylov *(*make_solver)(const Domain &) = nullptr) {
   World w;
   const int nmax = std::max(cs.nx, std::max(cs.ny, cs.nz));
   const double h = 1.0 / nmax;
@@ -167,7 +169,7 @@ World build_world(const Case &cs, Krylov *solver_override = nullptr) {
                     Periodic::no());
   w.d = new Domain(*w.gx, *w.gy, *w.gz);
   w.time = new Times(1, 1.0);
-  w.solver = solver_override ? solver_override : new CG(*w.d, Prec::di());
+  w.solver = make_solver ? make_solver(*w.d) : new CG(*w.d, Prec::di());
   w.uvw = new Vector(*w.d);
   w.p = new Scalar(*w.d);
   w.f = new Scalar(*w.d);
@@ -432,6 +434,25 @@ void dump_cycles(World &w) {
   }
 }
 
+// AC::flexcycle (additive.h:54-95, additive_recursive.cpp:161-200): from phi
+// = 0, the level solves' ResRat given (the recursion climbs back only when a
+// level's smoother meets it); tag, cycles and phi as dump_cycles writes them
+void dump_flex(World &w, double smooth_rr, int maxc, const std::array<int, 3> &mvs) {
+  CasePressure &pr = *w.pr;
+  AC ac(&pr);
+  pr.phi = 0.0;
+  ac.max_cycles(maxc);
+  ac.resrat(ResRat(smooth_rr));
+  std::array<MaxIter, 3> mv = {MaxIter(mvs[0]), MaxIter(mvs[1]), MaxIter(mvs[2])};
+  ac.max_iter_array(mv);
+  std::cout << "ACRUN Flex 3 1e-08 " << maxc << " 0 1 1 0 " << smooth_rr << " " << mvs[0] << " "
+            << mvs[1] << " " << mvs[2] << " -1 -1 -1" << std::endl;
+  int ncyc = 0;
+  bool ok = ac.flexcycle(ResRat(1e-8), &ncyc);
+  std::cout << "ACRET Flex " << ok << " " << ncyc << std::endl;
+  dump("Flex_phi", interior(pr.phi));
+}
+
 Case parse_case(char **a, int nx, int ny, int nz) {
   Case cs;
   cs.nx = nx; cs.ny = ny; cs.nz = nz;
@@ -501,6 +522,17 @@ int main(int argc, char **argv) {
     }
     World w = build_world(cs);
     dump_cycles(w);
+    return 0;
+  }
+  if (mode == "flex") {
+    // ref_driver flex OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel
+    //                 smooth_rr max_cycles mv0 mv1 mv2
+    if (argc < 18) { std::fprintf(stderr, "flex: too few arguments\n"); return 2; }
+    outdir = argv[2];
+    Case cs = parse_case(argv + 6, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
+    World w = build_world(cs);
+    dump_flex(w, std::atof(argv[13]), std::atoi(argv[14]),
+              {std::atoi(argv[15]), std::atoi(argv[16]), std::atoi(argv[17])});
     return 0;
   }
   if (mode == "prec") {

@@ -84,6 +84,37 @@ void finalize(psb_ctx *C, int op, int nblk, DevState *st, DevState *st2 = nullpt
                      C->work.partial, st, st2, out);
 }
 
+// Per-kernel timing of an eager solve (psb_profile_solve_device): a kernel
+// on the finest level launched inside a scope is bracketed by two HIP
+// events on the context's stream; `per_cell` algorithmic bytes per cell.
+// Does nothing in ordinary solves or on coarse levels.
+struct ProbeScope {
+  psb_ctx *C = nullptr;
+  int kind = 0;
+  hipEvent_t e0 = nullptr;
+  ProbeScope(psb_ctx *c, int k, const Grid &g, double per_cell) {
+    if (!c->probe.on || g.N != c->g.N) return;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventRecord(e0, c->stream));
+    C = c;
+    kind = k;
+    C->probe.bytes[k] = per_cell * (double)g.N;
+  }
+  // closes the bracket (before a kernel that is not to be counted)
+  void end() {
+    if (!C) return;
+    hipEvent_t e1 = nullptr;
+    if (hipEventCreate(&e1) == hipSuccess) {
+      (void)hipEventRecord(e1, C->stream);
+      C->probe.rec.emplace_back(kind, e0, e1);
+    }
+    C = nullptr;
+  }
+  ~ProbeScope() { end(); }
+  ProbeScope(const ProbeScope &) = delete;
+  ProbeScope &operator=(const ProbeScope &) = delete;
+};
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -128,6 +159,8 @@ template <typename T, bool SYM, bool DIAG, bool FLUX = false>
 void cg_pq(psb_ctx *C, const Grid &g, DMat<T> A, const T *rz, const T *pold, T *pnew, T *q,
            const uint8_t *af, DevState *st) {
   int nb = nblocks<T>(g);
+  // the stored coefficient arrays, z (or r), p in; p, q out
+  ProbeScope ps(C, PSB_K_CG_PQ, g, ((SYM ? 4 : 7) + 4) * sizeof(T) + (af ? 1 : 0));
   with_v<T>(g, [&](auto vt) {
     constexpr int V = decltype(vt)::value;
     MarchShape m;
@@ -141,6 +174,7 @@ void cg_pq(psb_ctx *C, const Grid &g, DMat<T> A, const T *rz, const T *pold, T *
     hipLaunchKernelGGL((k_cg_pq<T, V, SYM, DIAG, FLUX>), dim3(nblocks<T>(g)), dim3(kBlock), 0,
                        C->stream, g, A, rz, pold, pnew, q, af, st, C->work);
   });
+  ps.end();
   finalize<1>(C, FIN_PQ, nb, st);
 }
 
@@ -148,11 +182,15 @@ template <typename T, bool DIAG>
 void cg_xr(psb_ctx *C, const Grid &g, const T *c, const T *p, const T *q, T *x, T *r, T *e1,
            T omega, DevState *st, int op = -1, const Metric *mt = nullptr) {
   const Metric m = mt ? *mt : metric(C);
-  with_v<T>(g, [&](auto vt) {
-    constexpr int V = decltype(vt)::value;
-    hipLaunchKernelGGL((k_cg_xr<T, V, DIAG>), dim3(nblocks<T>(g)), dim3(kBlock), 0, C->stream, g,
-                       c, m, p, q, x, r, e1, omega, st, C->work);
-  });
+  {
+    // x, r, p, q, c in; x, r (and e1) out
+    ProbeScope ps(C, PSB_K_CG_XR, g, (DIAG ? 7 : 8) * sizeof(T));
+    with_v<T>(g, [&](auto vt) {
+      constexpr int V = decltype(vt)::value;
+      hipLaunchKernelGGL((k_cg_xr<T, V, DIAG>), dim3(nblocks<T>(g)), dim3(kBlock), 0, C->stream,
+                         g, c, m, p, q, x, r, e1, omega, st, C->work);
+    });
+  }
   finalize<3>(C, op >= 0 ? op : DIAG ? FIN_XR_DIAG : FIN_XR_AC, nblocks<T>(g), st);
 }
 
@@ -168,6 +206,13 @@ void jacobi_m(psb_ctx *C, const Level<T> &lv, DMat<M> A, const T *e, T *eout,
               const T *qf) {
   const Grid &g = lv.g;
   int nb = nblocks<T>(g);
+  // four coefficient arrays, r, e in, e out; the coarse correction (one
+  // coarse value per block of cells), q (flexible beta), the active flags
+  const double per_cell =
+      4.0 * sizeof(M) + 3.0 * sizeof(T) +
+      (coarse ? (double)sizeof(T) / (coarse->CI * coarse->CJ * coarse->CK) : 0.0) +
+      (RHO && qf ? sizeof(T) : 0) + (lv.aflag && (coarse || RHO) ? 1 : 0);
+  ProbeScope ps(C, coarse ? PSB_K_SWEEP_PROL : RHO ? PSB_K_SWEEP_LAST : PSB_K_SWEEP, g, per_cell);
   with_v<T>(g, [&](auto vt) {
     constexpr int V = decltype(vt)::value;
     ProlX<T, V> pr{};
@@ -205,6 +250,7 @@ void jacobi_m(psb_ctx *C, const Level<T> &lv, DMat<M> A, const T *e, T *eout,
                          st, C->work, zscale, qf);
     }
   });
+  ps.end();
   if (RHO) finalize<kRhoParts>(C, qf ? FIN_RHO_FLEX : FIN_RHO, nb, st);
 }
 
@@ -283,6 +329,10 @@ bool post2(psb_ctx *C, const Level<T> &lv, const T *e, T *eout, const Level<T> &
 template <typename T, int V, int KV, int CI, int CJ, int CK, typename M, bool FLUX>
 void restrict_launch(psb_ctx *C, const Level<T> &f, DMat<M> A, const T *e, Level<T> &cL,
                      T omega, DevState *st) {
+  // fine: four coefficient arrays, r, e (and flags); coarse: c in, r and e out
+  ProbeScope ps(C, PSB_K_RESTRICT, f.g,
+                4.0 * sizeof(M) + 2.0 * sizeof(T) + (f.aflag ? 1 : 0) +
+                    (3.0 * sizeof(T) + (cL.aflag ? 1 : 0)) / (CI * CJ * CK));
   if constexpr (CI == 2 && KV == 1 && V > 1) {
     if (kMarch && f.g.N >= kMarchMin && f.g.nx == 2 * cL.g.nx) {
       const MarchShape m = march_shape_n(cL.g.nx, (long long)cL.g.ny * (f.g.nz / V));
@@ -732,6 +782,16 @@ void kcycle_buffers(psb_ctx *C, Hier<T> &H) {
   }
 }
 
+// the outer CG's beta for this solve: flexible when asked (ac_flexible 1),
+// or by default (-1) when some level of the hierarchy takes the K-cycle's
+// Krylov steps, i.e. when the preconditioner is not a fixed linear operator
+template <typename T>
+void set_flex(psb_ctx *C, const Hier<T> &H) {
+  bool any = false;
+  for (int l = 0; l < (int)H.L.size(); l++) any = any || kcycle_level(C, H, l);
+  C->flex = C->flex_req == 1 || (C->flex_req == -1 && any);
+}
+
 // level l's correction for its right-hand side lv.r (first sweep already in
 // lv.e[0]) by two flexible-CG steps preconditioned by the cycle on l (their
 // reductions summed by the kernels that use them, no finalize launches):
@@ -833,9 +893,12 @@ bool cg_pq16(psb_ctx *C, const Level<T> &f, const T *z, const T *pold, T *pnew, 
     MarchShape m;
     if (!C->cg_half || !f.h16() || !f.d || !vec_ok<float>(f.g) || !march_ok<4>(f.g, m))
       return false;
-    hipLaunchKernelGGL((k_cg_pq_march<float, 4, true, false, true, _Float16>), dim3(m.nblocks),
-                       dim3(kBlock), 0, C->stream, f.g, f.mat16(), z, pold, pnew, q, f.aflag, st,
-                       C->work, m.TI, m.ntiles, (float)(1.0 / f.scale));
+    {
+      ProbeScope ps(C, PSB_K_CG_PQ, f.g, 4 * 2 + 4 * 4 + (f.aflag ? 1 : 0));
+      hipLaunchKernelGGL((k_cg_pq_march<float, 4, true, false, true, _Float16>),
+                         dim3(m.nblocks), dim3(kBlock), 0, C->stream, f.g, f.mat16(), z, pold,
+                         pnew, q, f.aflag, st, C->work, m.TI, m.ntiles, (float)(1.0 / f.scale));
+    }
     finalize<1>(C, FIN_PQ, m.nblocks, st);
     return true;
   } else {
@@ -1010,8 +1073,12 @@ template <typename F>
 void run_graph(psb_ctx *C, Graph &gr, long long key, DevState *st, int maxit, F body) {
   long long launches = 0;
   while (!C->hst->done) {
-    capture(C, gr, key, body);
-    HIPCHK(hipGraphLaunch(gr.exec, C->stream));
+    if (C->probe.on) {  // the same launches, one by one (timed per kernel)
+      body();
+    } else {
+      capture(C, gr, key, body);
+      HIPCHK(hipGraphLaunch(gr.exec, C->stream));
+    }
     read_state(C, st);
     if (++launches > (long long)maxit + 2) break;
   }
@@ -1191,6 +1258,7 @@ void solve_ac64(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_r
   ensure_krylov(C, K, false);
   build_hier64(C);
   kcycle_buffers(C, C->h64);
+  set_flex(C, C->h64);
   ensure_hist(C, p->max_iter);
   const Grid &g = C->g;
   Level<double> &f = C->h64.L[0];
@@ -1217,6 +1285,7 @@ void solve_ir(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_res
   ensure_krylov(C, K, false);
   build_hier32(C);
   kcycle_buffers(C, C->h32);
+  set_flex(C, C->h32);
   ensure_hist(C, p->max_iter);
   const Grid &g = C->g;
   Level<float> &f = C->h32.L[0];
@@ -1304,22 +1373,38 @@ void solve_ir(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_res
 // (H. A. van der Vorst and Q. Ye, "Residual replacement strategies for
 // Krylov subspace iterative methods", SIAM J. Sci. Comput. 22 (2000).)
 // ---------------------------------------------------------------------------
+// in a timed eager solve: whether a gated reliable-update kernel will run
+// (the refresh flag its finalize left), read back from the device; outside
+// one (or ungated) always true
+bool probe_gate(psb_ctx *C, bool gated) {
+  if (!C->probe.on || !gated) return true;
+  read_state(C, C->st32);
+  return C->hst->update != 0 && !C->hst->done;
+}
+
 void ru_resid(psb_ctx *C, const double *b, const double *x, bool gated, int op) {
   const Grid &g = C->g;
   Level<float> &f = C->h32.L[0];
   KrylovVecs<float> &K = C->k32;
   const float omega = (float)(C->ac_omega / f.scale);
-  with_v<double>(g, [&](auto vt) {
-    constexpr int V = decltype(vt)::value;
-    hipLaunchKernelGGL((k_ru_resid<V, true>), dim3(nblocks<double>(g)), dim3(kBlock), 0,
-                       C->stream, g, mat64(C), metric(C), b, x, f.c, K.r, f.e[0], omega, C->st32,
-                       gated, C->work);
-  });
+  {
+    // FP64 c, w, s, b, x, b and FP32 c in; FP32 r and e1 out (timed only
+    // when it runs: a gated launch without a pending update returns at once)
+    ProbeScope ps(C, PSB_K_RU_RESID, probe_gate(C, gated) ? g : Grid{}, 6 * 8 + 3 * 4);
+    with_v<double>(g, [&](auto vt) {
+      constexpr int V = decltype(vt)::value;
+      hipLaunchKernelGGL((k_ru_resid<V, true>), dim3(nblocks<double>(g)), dim3(kBlock), 0,
+                         C->stream, g, mat64(C), metric(C), b, x, f.c, K.r, f.e[0], omega,
+                         C->st32, gated, C->work);
+    });
+  }
   finalize<3>(C, op, nblocks<double>(g), C->st32);
 }
 
 void ru_xupdate(psb_ctx *C, double *x, bool gated) {
   const Grid &g = C->g;
+  // x, d in and out (timed only when a refresh is pending)
+  ProbeScope ps(C, PSB_K_RU_XUPDATE, probe_gate(C, gated) ? g : Grid{}, 2 * 8 + 2 * 4);
   with_v<double>(g, [&](auto vt) {
     constexpr int V = decltype(vt)::value;
     hipLaunchKernelGGL((k_ru_xupdate<V>), dim3(nblocks<double>(g)), dim3(kBlock), 0, C->stream,
@@ -1332,6 +1417,7 @@ void solve_ru(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_res
   ensure_krylov(C, K, false);
   build_hier32(C);
   kcycle_buffers(C, C->h32);
+  set_flex(C, C->h32);
   ensure_hist(C, p->max_iter);
   const Grid &g = C->g;
   const int chunk = chunk_of(p);
@@ -1707,7 +1793,7 @@ int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_
     C->cg_half = p->precision == PSB_MIXED && p->ac_store == 2;
     C->fuse = p->ac_fuse != 0;
     C->kmask = p->ac_kcycle;
-    C->flex = p->ac_flexible == 1 || (p->ac_flexible == -1 && p->ac_kcycle != 0);
+    C->flex_req = p->ac_flexible;  // resolved against the hierarchy (set_flex)
     C->ac_omega = p->ac_omega;
     if (p->precision == PSB_MIXED) {
       build_hier32(C);
@@ -2320,6 +2406,51 @@ int psb_time_kernel(psb_ctx *C, int kernel, int reps, double *avg_ms, double *by
     (void)hipEventDestroy(e1);
     return PSB_OK;
   });
+}
+
+int psb_profile_solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p,
+                             psb_result *R, psb_kernel_stat *stats, double *solve_ms) {
+  if (!C || !x || !b || !p || !R || !stats || !solve_ms || !stop_ok(p)) return PSB_EINVAL;
+  if (!C->have_matrix) return PSB_ENOTREADY;
+  if (p->max_iter < 0 || p->scale == 0.0 || p->stalecount > kStaleMax) return PSB_EINVAL;
+  std::memset(R, 0, sizeof(*R));
+  std::memset(stats, 0, sizeof(psb_kernel_stat) * PSB_K_COUNT);
+  auto clear = [&]() {
+    for (auto &t : C->probe.rec) {
+      (void)hipEventDestroy(std::get<1>(t));
+      (void)hipEventDestroy(std::get<2>(t));
+    }
+    C->probe.rec.clear();
+    C->probe.on = false;
+  };
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  const int rc = guarded(C, [&]() {
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    clear();
+    for (double &v : C->probe.bytes) v = 0.0;
+    C->probe.on = true;
+    HIPCHK(hipEventRecord(t0, C->stream));
+    const int r = solve_device(C, x, b, p, R);
+    HIPCHK(hipEventRecord(t1, C->stream));
+    HIPCHK(hipEventSynchronize(t1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+    *solve_ms = ms;
+    for (auto &t : C->probe.rec) {
+      float k = 0.0f;
+      HIPCHK(hipEventElapsedTime(&k, std::get<1>(t), std::get<2>(t)));
+      psb_kernel_stat &S = stats[std::get<0>(t)];
+      S.launches += 1;
+      S.ms += k;
+    }
+    for (int q = 0; q < PSB_K_COUNT; q++) stats[q].bytes = C->probe.bytes[q];
+    return r;
+  });
+  clear();
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  return rc;
 }
 
 }  // extern "C"

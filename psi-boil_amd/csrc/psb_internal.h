@@ -254,7 +254,14 @@ struct psb_ctx {
   bool cg_half = false;     // the FP32 CG operator reads the level-0 FP16 copy too
   int kmask = 0;            // K-cycle levels (bit l: level l) of the AC hierarchies
   bool flex = false;        // flexible (Polak-Ribiere) beta in the outer CG
+  int flex_req = -1;        // psb_stop.ac_flexible of the running solve
   int version = 0;     // bumped by every change of matrix, geometry or off mask
+  // per-kernel timing of one eager solve (psb_profile_solve_device)
+  struct Probe {
+    bool on = false;
+    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> rec;
+    double bytes[PSB_K_COUNT] = {};
+  } probe;
   psb::host::RefHier href;             // the reference's AC cycle
   psb::DevMetrics metrics{};           // metrics of an assembled matrix (device)
   uint8_t *off_dev = nullptr;          // solid cells of an assembled matrix (device)

@@ -62,6 +62,14 @@ class Result(C.Structure):
         return d
 
 
+KERNEL_KINDS = ["sweep", "sweep_prol", "sweep_last", "restrict", "cg_pq", "cg_xr", "ru_resid",
+                "ru_xupdate"]
+
+
+class KernelStat(C.Structure):
+    _fields_ = [("launches", C.c_int), ("ms", C.c_double), ("bytes", C.c_double)]
+
+
 class Metrics(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("dxc", "dxw", "dxe", "dyc", "dys", "dyn", "dzc",
                                           "dzb", "dzt")]
@@ -111,7 +119,8 @@ EXPORTS = ["psb_stop_defaults", "psb_status_string", "psb_create", "psb_destroy"
            "psb_batch_solve_device", "psb_cycle_defaults", "psb_ac_vcycle",
            "psb_ac_vcycle_device", "psb_assemble_pressure", "psb_assemble_pressure_device",
            "psb_update_rhs_device", "psb_project_device", "psb_bck_name", "psb_save_bck",
-           "psb_load_bck", "psb_save_bck_device", "psb_load_bck_device"]
+           "psb_load_bck", "psb_save_bck_device", "psb_load_bck_device",
+           "psb_profile_solve_device"]
 
 _lib = None
 
@@ -218,6 +227,21 @@ class Solver:
         _check(lib().psb_solve_device(self._c, C.c_void_p(d_x), C.c_void_p(d_b), C.byref(s),
                                       C.byref(r)), "psb_solve_device")
         return r.asdict()
+
+    def profile_solve_device(self, d_x, d_b, **stop):
+        """One solve as solve_device runs it, launched kernel by kernel with
+        the finest level's kernels timed by HIP events: (result, {kind:
+        {launches, ms, bytes}}, solve_ms)."""
+        s = stop_defaults(**stop)
+        r = Result()
+        st = (KernelStat * len(KERNEL_KINDS))()
+        ms = C.c_double()
+        _check(lib().psb_profile_solve_device(self._c, C.c_void_p(d_x), C.c_void_p(d_b),
+                                              C.byref(s), C.byref(r), st, C.byref(ms)),
+               "psb_profile_solve_device")
+        stats = {k: {"launches": st[q].launches, "ms": st[q].ms, "bytes": st[q].bytes}
+                 for q, k in enumerate(KERNEL_KINDS)}
+        return r.asdict(), stats, ms.value
 
     def assemble_pressure(self, rho, mx, my, mz, off=None):
         """Pressure::discretize on the device.  rho with one ghost layer;

@@ -148,3 +148,46 @@ def test_kcycle_on_semi_coarsened_levels_matches_oracle():
     assert abs(r["iterations"] - info["iters"]) <= 1, (r["iterations"], info["iters"])
     fl = d["off"] == 0
     assert demeaned_rel(x[fl], xo[fl]) < 1e-6
+
+
+def test_mixed_kcycle_stretched_cylinder_against_oracle_fp64():
+    """The stretched-grid workload's method in small: a z-stretched grid
+    (tanh clustering 2.5) with a solid cylinder and bubbles, levels halving x
+    and y before z, the K-cycle on the first four coarse levels, FP32
+    iterations with FP16 smoother copies and FP64 reliable updates, to 1e-9.
+    Against the oracle's FP64 solve of the same preconditioned method to
+    1e-12: the GPU's FP64 residual meets the tolerance, the iteration count is
+    the oracle's to 1e-9 within 10 %, and the solution (fluid cells, mean
+    removed) is within 10 times the error the oracle's own FP64 solve to 1e-9
+    leaves."""
+    nx, ny, nz = 48, 32, 96
+    print("[kcycle] mixed stretched cylinder", flush=True)
+    d = psb.generate(nx, ny, nz, 12, 0.02, 0.06, 1.0, 1000.0, 23, 1023, stretch_z=2.5, lz=2.0,
+                     cylinder=True, cyl_d=0.2)
+    assert d["off"].any()
+    A = O.Mat(nx, ny, nz, {k: d[k] for k in "cwesnbt"})
+    b, n = d["rhs"], d["rhs"].size
+    s = psb.Solver(nx, ny, nz)
+    s.set_active(d["off"])
+    s.set_matrix(d)
+    x, r = s.solve(np.zeros(n), b, precision=psb.PSB_MIXED, inner_adapt=2, inner_rtol=1e-2,
+                   ac_store=1, ac_coarsen=1, ac_cfac=1.0, ac_kcycle=-1,
+                   **dict(AC, res_rat=1e-9, max_iter=600))
+    ac = O.AC(A, np.ones(n), off=d["off"], rule=3)
+    kw = dict(nu1=2, nu2=2, omega=0.9, cfac=1.0, maxit=600)
+    try:
+        O.set_kcycle(0b11110)  # the first four coarse levels, as ac_kcycle = -1 takes them
+        O.set_flexible(True)
+        x12, i12 = ac.acpcg(np.zeros(n), b, rtol=1e-12, **kw)
+        x9, i9 = ac.acpcg(np.zeros(n), b, rtol=1e-9, **kw)
+    finally:
+        O.set_kcycle(0)
+        O.set_flexible(False)
+    assert r["converged"] and i12["converged"] and i9["converged"]
+    assert r["rel_l2"] <= 1.05e-9, r
+    assert abs(r["iterations"] - i9["iters"]) <= max(2, 0.1 * i9["iters"]), (r, i9["iters"])
+    fl = d["off"] == 0
+    e_gpu = demeaned_rel(x[fl], x12[fl])
+    e_ora = demeaned_rel(x9[fl], x12[fl])
+    print(f"[kcycle] error gpu mixed {e_gpu:.2e}, oracle fp64 to 1e-9 {e_ora:.2e}", flush=True)
+    assert e_gpu <= 10.0 * e_ora + 1e-12, (e_gpu, e_ora)
