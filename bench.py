@@ -431,6 +431,9 @@ def main():
     ap.add_argument("--nsys", type=int, default=96, help="systems of --case batch")
     ap.add_argument("--case", choices=sorted(EXTRA) + ["batch", "drop64"], default=None,
                     help="figures of another single-system workload instead of the bench line")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the timed FP64 and mixed solves of the bench line and the "
+                         "in-solve kernel timing (for a kernel-trace run of the bench method)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch the roofline kernels (FP32 Jacobi sweep, FP32/FP64 SpMV) "
                          "and a plain copy, for hardware-counter runs")
@@ -477,14 +480,24 @@ def main():
     x_fp64 = dx.download()
     # the same two solves with the plain V-cycle, for comparison
     vc = dict(common, ac_kcycle=0)
-    t64v, r64v = timed_solves(s, dx, db, a.steps, a.warmup, precision=psb.PSB_FP64, **vc)
-    tmxv, rmxv = timed_solves(s, dx, db, a.steps, a.warmup, **dict(vc, **mixed))
-    log(f"[bench] V-cycle: fp64 {t64v * 1e3:.2f} ms ({r64v['iterations']}), mixed "
-        f"{tmxv * 1e3:.2f} ms ({rmxv['iterations']})")
+    if not a.headline_only:
+        t64v, r64v = timed_solves(s, dx, db, a.steps, a.warmup, precision=psb.PSB_FP64, **vc)
+        tmxv, rmxv = timed_solves(s, dx, db, a.steps, a.warmup, **dict(vc, **mixed))
+        log(f"[bench] V-cycle: fp64 {t64v * 1e3:.2f} ms ({r64v['iterations']}), mixed "
+            f"{tmxv * 1e3:.2f} ms ({rmxv['iterations']})")
     log("[bench] mixed AC-PCG, FP32 iterations with FP64 reliable updates")
     tmx, rmx = timed_solves(s, dx, db, a.steps, a.warmup, **dict(common, **mixed))
     log(f"[bench] mixed {tmx * 1e3:.2f} ms/solve {rmx}")
     x_mixed = dx.download()
+    if a.headline_only:
+        prof_mx = in_solve_kernels(s, dx, db, dict(common, **mixed))
+        log(f"[bench] in-solve mixed {prof_mx}")
+        print(json.dumps({"mixed_ms_per_solve": round(tmx * 1e3, 3),
+                          "fp64_ms_per_solve": round(t64 * 1e3, 3),
+                          "mixed_over_fp64": round(t64 / tmx, 3),
+                          "iterations": {"fp64": r64["iterations"], "mixed": rmx["iterations"]},
+                          "in_solve_mixed": prof_mx}), flush=True)
+        return
     # the method as first specified: FP32 smoother matrices (no FP16 copy)
     mixed32 = dict(mixed, ac_store=0)
     tm32, rm32 = timed_solves(s, dx, db, a.steps, a.warmup, **dict(common, **mixed32))

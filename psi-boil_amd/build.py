@@ -15,17 +15,33 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpsb.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["psb_api.hip", "psb_batch.hip", "psb_gen.cpp", "psb_io.cpp"]
+OBJDIR = os.path.join(HERE, "_build")
 
 
 def build(verbose=False):
-    cmd = [HIPCC, *FLAGS, "-o", OUT + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES] + \
-          ["-lpthread"]
+    """Compiles the sources to objects in parallel, then links the library."""
+    os.makedirs(OBJDIR, exist_ok=True)
+    objs, cmds = [], []
+    for src in SOURCES:
+        obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        cmds.append([HIPCC, *FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)])
+    procs = []
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd))
+        procs.append(subprocess.Popen(cmd))
+    bad = [c for c, p in zip(cmds, procs) if p.wait() != 0]
+    if bad:
+        raise subprocess.CalledProcessError(1, bad[0])
+    link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp", *objs,
+            "-lpthread"]
     if verbose:
-        print(" ".join(cmd))
-    subprocess.check_call(cmd)
+        print(" ".join(link))
+    subprocess.check_call(link)
     os.replace(OUT + ".tmp", OUT)
     return OUT
 
