@@ -262,6 +262,22 @@ int psb_solve(psb_ctx *ctx, double *x, const double *b, const psb_stop *stop,
 int psb_solve_device(psb_ctx *ctx, double *d_x, const double *d_b,
                      const psb_stop *stop, psb_result *res);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory during a solve (tight serving loops, code that forbids hipMalloc):
+ * psb_solve_device for CG with the diagonal preconditioner in FP64 (the only
+ * method it accepts; anything else is PSB_EINVAL) that allocates nothing.
+ * Its kernels are those of psb_solve_device, launched one by one instead of
+ * from a captured graph, and its Krylov vectors, reduction partials and
+ * solver state live in a static device array of 64 MiB per device (held by
+ * each device the library runs on until the process ends), so the system
+ * may have at most 2^21 cells (128^3); larger ones return PSB_EINVAL with a
+ * message on stderr.  x (in: initial guess, out: solution) is iterated in
+ * place.  The array is shared by every context and call on a device: calls
+ * that overlap on one device race.  Results equal psb_solve_device's bit for
+ * bit; psb_history reports no history for it.                               */
+int psb_solve_device_static(psb_ctx *ctx, double *d_x, const double *d_b,
+                            const psb_stop *stop, psb_result *res);
+
 /* Scalar::operator=(A_x) / (y_m_A_x) (scalar_operators.cpp:22-72) on host
  * arrays: y = A x, or y = b - A x when b != NULL.  FP64, bit-exact with the
  * reference. */

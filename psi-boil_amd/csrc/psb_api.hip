@@ -15,7 +15,9 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -1118,6 +1120,90 @@ void solve_cg_diag(psb_ctx *C, double *x, const double *b, const psb_stop *p, ps
 }
 
 // ---------------------------------------------------------------------------
+// CG with the diagonal preconditioner without any allocation
+// (psb_solve_device_static): the kernels of solve_cg_diag launched eagerly
+// (no graph: instantiating one allocates), iterating on the caller's x, with
+// the Krylov vectors r, p0, p1, q, the reduction partials and the solver
+// state in the file-scope device array `scratch` instead of buffers the
+// library allocates.  Each device holds one such array for the life of the
+// process, shared by every context and call on that device: calls that
+// overlap on one device race.  Every region is written before it is read
+// (the state by init_state, p0 / p1 cleared, r by the start kernel, q and
+// the partials by their producing kernels), so a call never sees the
+// previous call's contents.
+// ---------------------------------------------------------------------------
+constexpr size_t kStaticMaxCells = (size_t)1 << 21;  // 128^3 cells
+constexpr size_t kStaticStateDoubles = (sizeof(DevState) + 255) / 256 * 32;
+constexpr size_t kStaticPartialDoubles = (size_t)kMaxBlocks * kMaxPartials;
+constexpr size_t kStaticDoubles =
+    kStaticStateDoubles + kStaticPartialDoubles + 4 * kStaticMaxCells;  // 64.1 MiB
+
+}  // namespace host
+}  // namespace psb
+
+static __device__ __align__(256) double scratch[psb::host::kStaticDoubles];
+
+namespace psb {
+namespace host {
+
+// the device address of `scratch` on the current device, looked up once per
+// device (the host-side name is not a device address)
+double *static_scratch_here() {
+  static std::mutex mu;
+  static std::map<int, double *> ptrs;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = ptrs.find(dev);
+  if (it != ptrs.end()) return it->second;
+  void *ptr = nullptr;
+  HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  ptrs[dev] = static_cast<double *>(ptr);
+  return static_cast<double *>(ptr);
+}
+
+template <bool SYM>
+void solve_cg_diag_static(psb_ctx *C, double *x, const double *b, const psb_stop *p,
+                          psb_result *R) {
+  const Grid &g = C->g;
+  double *base = C->static_scratch;
+  DevState *st = reinterpret_cast<DevState *>(base);
+  double *partial = base + kStaticStateDoubles;
+  double *r = partial + kStaticPartialDoubles;
+  double *pv[2] = {r + kStaticMaxCells, r + 2 * kStaticMaxCells};
+  double *q = r + 3 * kStaticMaxCells;
+  // the launch helpers take the reduction partials from C->work
+  struct WorkSwap {
+    psb_ctx *C;
+    Work keep;
+    ~WorkSwap() { C->work = keep; }
+  } swap{C, C->work};
+  C->work.partial = partial;
+  DMat<double> A = mat64(C);
+  init_state(C, st, p, p->norm, p->res_rat, p->res_tol, p->max_iter, false);
+  zero(C, pv[0], g.N);
+  zero(C, pv[1], g.N);
+  start<double, SYM, 0>(C, g, A, b, x, r, nullptr, 0.0, st);
+  HIPCHK(hipGetLastError());
+  read_state(C, st);
+  const int chunk = chunk_of(p);
+  long long rounds = 0;
+  while (!C->hst->done) {
+    for (int it = 0; it < chunk; it++) {
+      const int par = it & 1;
+      cg_pq<double, SYM, true>(C, g, A, r, pv[par], pv[par ^ 1], q, nullptr, st);
+      cg_xr<double, true>(C, g, A.c, pv[par ^ 1], q, x, r, nullptr, 0.0, st);
+    }
+    HIPCHK(hipGetLastError());
+    read_state(C, st);
+    if (++rounds > (long long)p->max_iter + 2) break;
+  }
+  finish_result(R, *C->hst, 0);
+  C->hist_count = 0;
+  R->rel_l2 = true_rel(C, b, x);
+}
+
+// ---------------------------------------------------------------------------
 // CG (cg.cpp:10-182) with the red-black symmetric Gauss-Seidel preconditioner
 // in place of Prec::ic0 / ic2 / ic3, whose triangular sweeps
 // (preconditioner_solve.cpp:14-110) run cell after cell: per iteration
@@ -1927,6 +2013,7 @@ int psb_create(psb_ctx **out, const psb_layout *L) {
     C->dx = C->base.alloc<double>(N);
     C->db = C->base.alloc<double>(N);
     C->dy = C->base.alloc<double>(N);
+    C->static_scratch = static_scratch_here();
     return PSB_OK;
   });
   if (rc != PSB_OK) {
@@ -2205,6 +2292,29 @@ int psb_solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, 
   if (p->max_iter < 0 || p->scale == 0.0 || p->stalecount > kStaleMax) return PSB_EINVAL;
   std::memset(R, 0, sizeof(*R));
   return guarded(C, [&]() { return solve_device(C, x, b, p, R); });
+}
+
+int psb_solve_device_static(psb_ctx *C, double *x, const double *b, const psb_stop *p,
+                            psb_result *R) {
+  if (!C || !x || !b || !p || !R || !stop_ok(p)) return PSB_EINVAL;
+  if (!C->have_matrix) return PSB_ENOTREADY;
+  if (p->max_iter < 0 || p->scale == 0.0 || p->stalecount > kStaleMax) return PSB_EINVAL;
+  if (p->method != PSB_CG || p->precond != PSB_PREC_DIAG || p->precision != PSB_FP64)
+    return PSB_EINVAL;
+  if ((size_t)C->g.N > kStaticMaxCells) {
+    C->err = "psb_solve_device_static: " + std::to_string(C->g.N) +
+             " cells exceed the static scratch (at most " + std::to_string(kStaticMaxCells) +
+             " cells)";
+    std::fprintf(stderr, "%s\n", C->err.c_str());
+    return PSB_EINVAL;
+  }
+  if (!C->static_scratch) return PSB_EHIP;
+  std::memset(R, 0, sizeof(*R));
+  return guarded(C, [&]() {
+    if (C->sym) solve_cg_diag_static<true>(C, x, b, p, R);
+    else solve_cg_diag_static<false>(C, x, b, p, R);
+    return R->status;
+  });
 }
 
 int psb_solve(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_result *R) {

@@ -267,6 +267,9 @@ struct psb_ctx {
   uint8_t *off_dev = nullptr;          // solid cells of an assembled matrix (device)
   std::vector<double> gdx, gdy, gdz;   // host copies of the cell sizes (empty: unit)
   std::string err;
+  // device address of the static scratch of psb_solve_device_static on the
+  // device this context was created on (found once per device at create)
+  double *static_scratch = nullptr;
 };
 
 namespace psb {

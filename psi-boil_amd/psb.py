@@ -120,7 +120,7 @@ EXPORTS = ["psb_stop_defaults", "psb_status_string", "psb_create", "psb_destroy"
            "psb_ac_vcycle_device", "psb_assemble_pressure", "psb_assemble_pressure_device",
            "psb_update_rhs_device", "psb_project_device", "psb_bck_name", "psb_save_bck",
            "psb_load_bck", "psb_save_bck_device", "psb_load_bck_device",
-           "psb_profile_solve_device"]
+           "psb_profile_solve_device", "psb_solve_device_static"]
 
 _lib = None
 
@@ -226,6 +226,17 @@ class Solver:
         r = Result()
         _check(lib().psb_solve_device(self._c, C.c_void_p(d_x), C.c_void_p(d_b), C.byref(s),
                                       C.byref(r)), "psb_solve_device")
+        return r.asdict()
+
+    def solve_device_static(self, d_x, d_b, **stop):
+        """solve_device for CG-di in FP64 through the entry point that
+        allocates nothing (its work arrays in the library's static device
+        array; at most 2^21 cells)."""
+        s = stop_defaults(**stop)
+        r = Result()
+        _check(lib().psb_solve_device_static(self._c, C.c_void_p(d_x), C.c_void_p(d_b),
+                                             C.byref(s), C.byref(r)),
+               "psb_solve_device_static")
         return r.asdict()
 
     def profile_solve_device(self, d_x, d_b, **stop):
