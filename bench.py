@@ -434,9 +434,13 @@ def main():
     ap.add_argument("--headline-only", action="store_true",
                     help="only the timed FP64 and mixed solves of the bench line and the "
                          "in-solve kernel timing (for a kernel-trace run of the bench method)")
+    ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
+                    help="with --headline-only: only this one of the two solves (one "
+                         "precision per traced process)")
     ap.add_argument("--profile-kernels", action="store_true",
-                    help="only launch the roofline kernels (FP32 Jacobi sweep, FP32/FP64 SpMV) "
-                         "and a plain copy, for hardware-counter runs")
+                    help="only launch the roofline kernels (FP32 Jacobi sweep, FP32/FP64 SpMV, "
+                         "FP32 CG p/q step, first post-sweep) and a plain copy, for "
+                         "hardware-counter runs")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the solver has "
@@ -464,7 +468,8 @@ def main():
         db0 = psb.DeviceArray.from_host(d["rhs"])
         s.solve_device(dx0.addr, db0.addr, precond=psb.PSB_PREC_AC, precision=psb.PSB_MIXED,
                        inner_adapt=2, inner_rtol=1e-2, ac_store=1, max_iter=1)
-        for kern, name in ((2, "jacobi_fp32"), (1, "spmv_fp32"), (0, "spmv_fp64"), (3, "copy_fp32")):
+        for kern, name in ((2, "jacobi_fp32"), (1, "spmv_fp32"), (0, "spmv_fp64"),
+                           (6, "cg_pq_fp32"), (7, "sweep_prol_fp32"), (3, "copy_fp32")):
             ms, by = s.time_kernel(kern, reps=10)
             log(f"[bench] {name}: {ms * 1e3:.1f} us/launch, {by / 1e6:.1f} MB algorithmic, "
                 f"{by / ms / 1e6:.0f} GB/s")
@@ -474,6 +479,15 @@ def main():
     common = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=1e-8, res_tol=0.0,
                   max_iter=1000, ac_kcycle=a.kcycle, graph_chunk=a.graph_chunk)
     mixed = dict(precision=psb.PSB_MIXED, inner_rtol=1e-2, inner_adapt=2, ac_store=1)
+    if a.headline_only and a.only:
+        kw = dict(common, **(mixed if a.only == "mixed" else dict(precision=psb.PSB_FP64)))
+        t, r = timed_solves(s, dx, db, a.steps, a.warmup, **kw)
+        log(f"[bench] {a.only} {t * 1e3:.2f} ms/solve {r}")
+        prof = in_solve_kernels(s, dx, db, kw)
+        log(f"[bench] in-solve {a.only} {prof}")
+        print(json.dumps({a.only + "_ms_per_solve": round(t * 1e3, 3),
+                          "iterations": r["iterations"], "in_solve": prof}), flush=True)
+        return
     log(f"[bench] FP64 AC-PCG, K-cycle levels {a.kcycle:#b}")
     t64, r64 = timed_solves(s, dx, db, a.steps, a.warmup, precision=psb.PSB_FP64, **common)
     log(f"[bench] fp64 {t64 * 1e3:.2f} ms/solve {r64}")
@@ -491,12 +505,13 @@ def main():
     x_mixed = dx.download()
     if a.headline_only:
         prof_mx = in_solve_kernels(s, dx, db, dict(common, **mixed))
+        prof_64 = in_solve_kernels(s, dx, db, dict(common, precision=psb.PSB_FP64))
         log(f"[bench] in-solve mixed {prof_mx}")
         print(json.dumps({"mixed_ms_per_solve": round(tmx * 1e3, 3),
                           "fp64_ms_per_solve": round(t64 * 1e3, 3),
                           "mixed_over_fp64": round(t64 / tmx, 3),
                           "iterations": {"fp64": r64["iterations"], "mixed": rmx["iterations"]},
-                          "in_solve_mixed": prof_mx}), flush=True)
+                          "in_solve_mixed": prof_mx, "in_solve_fp64": prof_64}), flush=True)
         return
     # the method as first specified: FP32 smoother matrices (no FP16 copy)
     mixed32 = dict(mixed, ac_store=0)

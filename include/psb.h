@@ -398,13 +398,20 @@ int psb_batch_destroy(psb_batch *bt);
  * changed.  NULL for a bad index.                                          */
 psb_ctx *psb_batch_system(psb_batch *bt, int sys);
 
-/* Work per thread of the streaming kernels: 0 = chosen per (level, size
- * class) by timing the Jacobi sweep of each class with 1, 2, 4 and 8 cell
- * vectors per thread when the plan is built (default); > 0 = that value.    */
+/* Tiles of the streaming kernels: 0 = chosen per (level, size class) when
+ * the plan is built (default) by timing the Jacobi sweep of each class with
+ * every candidate: the span tile with 1, 2, 4 or 8 cell vectors per thread,
+ * and on levels of at least 2^15 cells whose rows hold whole 4-wide vectors
+ * the marching tile (256 vectors of a (j, k) plane walking a slab of 4, 8,
+ * 16 or 32 i planes).  With the environment variable PSB_TUNE_CACHE naming
+ * a file, choices for this device are read from it (JSON) and new ones
+ * written back.  > 0 = that many vectors per thread everywhere, the levels
+ * of at least 2^15 cells marching with slabs that fill their segments.     */
 int psb_batch_set_tuning(psb_batch *bt, int vpt);
 
-/* The autotuner's choices: (level, log2 cells, vectors per thread) triples;
- * returns their count.                                                      */
+/* The autotuner's choices: (level, log2 cells, choice) triples, choice > 0
+ * vectors per thread of the span tile, < 0 minus the planes per slab of the
+ * marching tile; returns their count.                                       */
 int psb_batch_plan(psb_batch *bt, int *triples, int cap);
 
 /* x[s] (in: initial guess, out: solution) and b[s] are host arrays in
@@ -423,7 +430,9 @@ int psb_batch_solve_device(psb_batch *bt, double *const *d_x,
  * the last mixed-precision solve set it up (flux form; FP16 matrix copy with
  * ac_store = 1), 3 plain FP32 copy of the grid (reference stream for
  * hardware byte counters), 4 the fine level's two post-sweeps in one pass
- * (ac_fuse = 1 form), 5 the same two sweeps as two kernels.
+ * (ac_fuse = 1 form), 5 the same two sweeps as two kernels, 6 the FP32 CG
+ * p/q step of the mixed multigrid-preconditioned solve on the fine level,
+ * 7 the fine level's first post-sweep with the prolongation fused in.
  * *bytes receives the algorithmic bytes one launch moves (for 4 and 5:
  * those of one sweep).                                                      */
 int psb_time_kernel(psb_ctx *ctx, int kernel, int reps, double *avg_ms,

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -27,6 +28,7 @@
 
 #include "psb_fused.h"
 #include "psb_march.h"
+#include "psb_lds.h"
 #include "psb_internal.h"
 
 // FP32 levels evaluate the stencil in flux form (set to 0 for the plain form)
@@ -69,6 +71,16 @@ constexpr bool kMarch = true;
 template <typename T>
 constexpr bool march_prol() { return std::is_same<T, double>::value; }
 constexpr int kMarchMin = 1 << 15;
+// fine-level kernels that run in the LDS-staged marching form (psb_lds.h),
+// per precision (psb_ctx::lds32 / lds64).  Measured at 256^3 in the solve:
+// FP32 first post-sweep 84.8 against 98.5 us (span), CG p/q step 109 against
+// 131 us (register march); the other FP32 sweeps were slower staged (97 /
+// 121 against 85 / 113 us), so they keep the register march.
+enum { kLdsSweep = 1, kLdsProl = 2, kLdsLast = 4, kLdsCgPq = 8 };
+template <typename T>
+bool lds_use(const psb_ctx *C, int kind) {
+  return ((sizeof(T) == 8 ? C->lds64 : C->lds32) & kind) != 0;
+}
 
 // calls f(std::integral_constant<int, V>) with the vector width for g
 template <typename T, typename F>
@@ -168,9 +180,14 @@ void cg_pq(psb_ctx *C, const Grid &g, DMat<T> A, const T *rz, const T *pold, T *
     MarchShape m;
     if (kMarch && V > 1 && g.N >= kMarchMin && march_ok<V>(g, m)) {
       nb = m.nblocks;
-      hipLaunchKernelGGL((k_cg_pq_march<T, V, SYM, DIAG, FLUX>), dim3(m.nblocks), dim3(kBlock),
-                         0, C->stream, g, A, rz, pold, pnew, q, af, st, C->work, m.TI,
-                         m.ntiles);
+      if (lds_use<T>(C, kLdsCgPq) && lds_ok<V>(g))
+        hipLaunchKernelGGL((k_cg_pq_lds<T, V, SYM, DIAG, FLUX>), dim3(m.nblocks), dim3(kBlock),
+                           0, C->stream, g, A, rz, pold, pnew, q, af, st, C->work, m.TI,
+                           m.ntiles, T(1));
+      else
+        hipLaunchKernelGGL((k_cg_pq_march<T, V, SYM, DIAG, FLUX>), dim3(m.nblocks),
+                           dim3(kBlock), 0, C->stream, g, A, rz, pold, pnew, q, af, st,
+                           C->work, m.TI, m.ntiles);
       return;
     }
     hipLaunchKernelGGL((k_cg_pq<T, V, SYM, DIAG, FLUX>), dim3(nblocks<T>(g)), dim3(kBlock), 0,
@@ -230,6 +247,19 @@ void jacobi_m(psb_ctx *C, const Level<T> &lv, DMat<M> A, const T *e, T *eout,
       pr.nZH = coarse->g.nz;
     }
     MarchShape m;
+    if (lds_use<T>(C, coarse ? kLdsProl : RHO ? kLdsLast : kLdsSweep) && V > 1 &&
+        g.N >= kMarchMin && lds_ok<V>(g) && march_ok<V>(g, m)) {
+      nb = m.nblocks;
+      if (coarse)
+        hipLaunchKernelGGL((k_jacobi_lds<T, V, true, RHO, M, FLUX>), dim3(m.nblocks),
+                           dim3(kBlock), 0, C->stream, g, A, lv.r, e, eout, pr, omega, lv.aflag,
+                           st, C->work, zscale, m.TI, m.ntiles, qf);
+      else
+        hipLaunchKernelGGL((k_jacobi_lds<T, V, false, RHO, M, FLUX>), dim3(m.nblocks),
+                           dim3(kBlock), 0, C->stream, g, A, lv.r, e, eout, pr, omega, lv.aflag,
+                           st, C->work, zscale, m.TI, m.ntiles, qf);
+      return;
+    }
     if (kMarch && V > 1 && g.N >= kMarchMin && (!coarse || march_prol<T>()) && march_ok<V>(g, m)) {
       nb = m.nblocks;
       if (coarse)
@@ -2014,6 +2044,15 @@ int psb_create(psb_ctx **out, const psb_layout *L) {
     C->db = C->base.alloc<double>(N);
     C->dy = C->base.alloc<double>(N);
     C->static_scratch = static_scratch_here();
+    // which fine-level kernels use the LDS-staged form (same results):
+    // PSB_NO_LDS set: none (for comparisons); PSB_LDS_MASK="m32,m64"
+    // overrides the measured defaults (bits kLdsSweep .. kLdsCgPq)
+    if (std::getenv("PSB_NO_LDS")) {
+      C->lds32 = C->lds64 = 0;
+    } else if (const char *m = std::getenv("PSB_LDS_MASK")) {
+      if (std::sscanf(m, "%d,%d", &C->lds32, &C->lds64) != 2)
+        throw std::invalid_argument("PSB_LDS_MASK is not \"m32,m64\"");
+    }
     return PSB_OK;
   });
   if (rc != PSB_OK) {
@@ -2487,6 +2526,55 @@ int psb_time_kernel(psb_ctx *C, int kernel, int reps, double *avg_ms, double *by
         jacobi<float, false>(C, lv, lv.e[1], lv.e[2], nullptr, nullptr, 0.0f, 0.9f, C->st32);
       };
       *bytes = N * (4 * (lv.h16() ? 2 : 4) + 3 * 4);  // one sweep's algorithmic bytes
+    } else if (kernel == 6) {
+      // the FP32 CG p/q step of the mixed AC solve on level 0 (flux-form FP32
+      // matrix, marching): p = z + beta p_old with beta 0, q = A p, p.q
+      build_hier32(C);
+      ensure_krylov(C, C->k32, false);
+      const Level<float> lv = C->h32.L[0];
+      const KrylovVecs<float> K = C->k32;
+      MarchShape m;
+      if (!lv.d || !vec_ok<float>(g) || !march_ok<4>(g, m)) return PSB_EINVAL;
+      zero(C, lv.e[0], g.N);
+      zero(C, K.p[0], g.N);
+      DevState h;
+      std::memset(&h, 0, sizeof(h));
+      HIPCHK(hipMemcpyAsync(C->st32, &h, sizeof(h), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      const bool lds = lds_use<float>(C, kLdsCgPq) && lds_ok<4>(g);
+      f = [=]() {
+        if (lds)
+          hipLaunchKernelGGL((k_cg_pq_lds<float, 4, true, false, true, float>), dim3(m.nblocks),
+                             dim3(kBlock), 0, stream, g, lv.matflux(), lv.e[0], K.p[0], K.p[1],
+                             K.q, lv.aflag, C->st32, C->work, m.TI, m.ntiles, 1.0f);
+        else
+          hipLaunchKernelGGL((k_cg_pq_march<float, 4, true, false, true, float>),
+                             dim3(m.nblocks), dim3(kBlock), 0, stream, g, lv.matflux(), lv.e[0],
+                             K.p[0], K.p[1], K.q, lv.aflag, C->st32, C->work, m.TI, m.ntiles,
+                             1.0f);
+      };
+      // four FP32 coefficient arrays, z and p_old in, p and q out
+      *bytes = N * (4 * 4 + 4 * 4 + (lv.aflag ? 1 : 0));
+    } else if (kernel == 7) {
+      // the first post-sweep of level 0 with the prolongation of level 1's
+      // correction fused in, as the last mixed solve stores the level
+      build_hier32(C);
+      const Level<float> lv = C->h32.L[0];
+      const Level<float> cL = C->h32.L[1];
+      zero(C, lv.e[0], g.N);
+      zero(C, lv.r, g.N);
+      zero(C, cL.e[0], cL.g.N);
+      DevState h;
+      std::memset(&h, 0, sizeof(h));
+      HIPCHK(hipMemcpyAsync(C->st32, &h, sizeof(h), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      f = [=]() {
+        jacobi<float, false>(C, lv, lv.e[0], lv.e[1], &cL, cL.e[0], 2.0f, 0.9f, C->st32);
+      };
+      // the sweep's bytes plus one coarse value per block of fine cells (and
+      // the active flags when the level has solid cells)
+      *bytes = N * (4 * (lv.h16() ? 2 : 4) + 3 * 4 + 4.0 / (cL.CI * cL.CJ * cL.CK) +
+                    (lv.aflag ? 1 : 0));
     } else if (kernel == 3) {
       ensure32(C);
       ensure_krylov(C, C->k32, false);

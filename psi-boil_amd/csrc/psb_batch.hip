@@ -12,14 +12,25 @@
 // smooth and restrict, so one V-cycle of the batch is one descent and one
 // ascent over the deepest hierarchy.
 //
-// The work per thread of each (level, size class) is picked by a small
-// autotuner that times the Jacobi sweep of that class with each candidate.
+// The tile of each (level, size class) is picked by a small autotuner that
+// times the Jacobi sweep of that class with each candidate: the span form
+// with 1, 2, 4 or 8 cell vectors per thread, or (levels of at least 2^15
+// cells with 4-wide vectors) the marching form, tiles of 256 vectors of a
+// (j, k) plane walking slabs of 4, 8, 16 or 32 i planes (the z-blocking).
+// With PSB_TUNE_CACHE naming a file, the choices are read from it when it
+// holds every class of the batch (same device) and written back after
+// tuning, as JSON.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <map>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "psb_bkernels.h"
@@ -32,6 +43,7 @@ namespace {
 
 constexpr int kCoarseMinBatch = 4;  // coarsest dims 4..7: dense operators <= 343 cells
 constexpr int kVptCand[] = {1, 2, 4, 8};
+constexpr int kTiCand[] = {4, 8, 16, 32};  // planes per slab of the marching tile
 constexpr int kVptDefault = 2;
 
 struct SegList {
@@ -66,6 +78,8 @@ struct psb_batch {
   std::vector<std::vector<BLev<float>>> lev;  // [level][system]
   std::vector<BLev<float> *> d_lev;
   std::vector<std::vector<int>> vpt;          // [level][system] vectors per thread
+  std::vector<std::vector<int>> ti;           // [level][system] planes per slab (0: span,
+                                              // -1: as many as fill the segment)
   SegList fineL;                                // all systems, fine level (FP32 vectors)
   SegList fine64L;                              // all systems, fine level (FP64 vectors)
   std::vector<SegList> smoothL;                 // [l]: systems with l < last
@@ -73,7 +87,9 @@ struct psb_batch {
   std::vector<SegList> gemvL;                   // [l]: systems with last == l
   Graph graph;
   double omega = 0, cfac = 0;
-  std::vector<int> tuned;                       // (level, class, vpt) triples
+  std::vector<int> tuned;                       // (level, class, choice) triples:
+                                                // choice > 0 vectors per thread (span),
+                                                // < 0 minus the planes per slab (march)
 };
 
 namespace {
@@ -119,14 +135,37 @@ int size_class(const Grid &g) {
   return c;
 }
 
-// launch lists for the current per-(level, system) work per thread
+constexpr int kBatchMarchMin = 1 << 15;
+
+// whether level l of system s can take the marching tile
+bool march_able(const psb_batch *B, int l, int s) {
+  const Grid &g = B->lev[l][s].g;
+  return B->Vl[l] == 4 && g.N >= kBatchMarchMin && g.nz % 4 == 0;
+}
+
+// (slab, tile) items of the marching tile with TI planes per slab
+long long march_items(const Grid &g, int TI) {
+  const long long ntiles = ((long long)g.ny * (g.nz / 4) + kBlock - 1) / kBlock;
+  return ntiles * ((g.nx + TI - 1) / TI);
+}
+
+// the segment of system s's streaming kernels on level l: one block per
+// marching item when the level marches with a chosen slab depth, else
+// vpt vectors per thread
+BSeg level_seg(const psb_batch *B, int l, int s, long long nwork) {
+  const int TI = B->ti[l][s];
+  if (TI > 0) return BSeg{s, 0, round8(march_items(B->lev[l][s].g, TI)), 0};
+  return make_seg(s, nwork, B->vpt[l][s]);
+}
+
+// launch lists for the current per-(level, system) tiles
 void build_lists(psb_batch *B) {
   const int ns = B->nsys;
   B->fineL = SegList();
   B->fine64L = SegList();
   for (int s = 0; s < ns; s++) {
     // the fine list partitions every fine-level kernel that leaves partials
-    B->fineL.segs.push_back(make_seg(s, nvec(B->fine[s].g, B->V), B->vpt[0][s]));
+    B->fineL.segs.push_back(level_seg(B, 0, s, nvec(B->fine[s].g, B->V)));
   }
   B->fine64L = B->fineL;  // same partition: partial ranges stay per system
   upload_list(B, B->fineL);
@@ -139,7 +178,7 @@ void build_lists(psb_batch *B) {
     for (int s = 0; s < ns; s++) {
       if (l < B->last[s]) {
         const BLev<float> &L = B->lev[l][s], &H = B->lev[l + 1][s];
-        if (l > 0) B->smoothL[l].segs.push_back(make_seg(s, nvec(L.g, B->Vl[l]), B->vpt[l][s]));
+        if (l > 0) B->smoothL[l].segs.push_back(level_seg(B, l, s, nvec(L.g, B->Vl[l])));
         const int KE = B->Vl[l] == 1 ? (H.sK ? 2 : 1) : B->Vl[l];
         const long long nw = (long long)H.g.nx * H.g.ny * (L.g.nz / KE);
         groups[combo_of(H)].segs.push_back(make_seg(s, nw, B->vpt[l][s]));
@@ -160,22 +199,22 @@ void build_lists(psb_batch *B) {
   B->partial = B->arena.alloc<double>(need);
 }
 
-// levels of at least kMarchMin cells with 4-wide vectors march (the Jacobi
-// sweeps, and on level 0 the CG p/q step): slabs of TI planes sized so that
-// the system's segment holds about one tile-slab per block
-constexpr int kBatchMarchMin = 1 << 15;
-
+// the marching fields of the level tables (the Jacobi sweeps, and on level
+// 0 the CG p/q step, march where TI > 0): the chosen slab depth, or (ti ==
+// -1, a fixed work per thread) slabs sized so that the system's segment
+// holds about one tile-slab per block; span where ti == 0
 void march_tables(psb_batch *B) {
   for (int l = 0; l < B->nlev; l++) {
     const SegList &L = l == 0 ? B->fineL : B->smoothL[l];
     for (const BSeg &sg : L.segs) {
       BLev<float> &D = B->lev[l][sg.sys];
       D.TI = D.ntiles = 0;
-      if (B->Vl[l] != 4 || D.g.N < kBatchMarchMin || D.g.nz % 4 != 0) continue;
+      const int want = B->ti[l][sg.sys];
+      if (want == 0 || !march_able(B, l, sg.sys)) continue;
       const long long nvp = (long long)D.g.ny * (D.g.nz / 4);
       const int ntiles = (int)((nvp + kBlock - 1) / kBlock);
-      int TI = (int)(((long long)D.g.nx * ntiles + sg.nb - 1) / sg.nb);
-      if (TI < 1) TI = 1;
+      int TI = want;
+      if (TI < 0) TI = std::max(1, (int)(((long long)D.g.nx * ntiles + sg.nb - 1) / sg.nb));
       D.TI = TI;
       D.ntiles = ntiles;
     }
@@ -335,16 +374,27 @@ bool stale(psb_batch *B) {
   return false;
 }
 
-// time the level-l Jacobi sweep of the systems in `who` with `vpt`
-double time_class(psb_batch *B, int l, const std::vector<int> &who, int vpt) {
+// time the level-l Jacobi sweep of the systems in `who` with the span tile
+// (vpt vectors per thread, TI = 0) or the marching tile (TI planes per slab)
+double time_class(psb_batch *B, int l, const std::vector<int> &who, int vpt, int TI) {
   SegList L;
-  for (int s : who) L.segs.push_back(make_seg(s, nvec(B->lev[l][s].g, B->Vl[l]), vpt));
+  for (int s : who) {
+    BLev<float> &D = B->lev[l][s];
+    if (TI > 0) {
+      D.TI = TI;
+      D.ntiles = (int)(((long long)D.g.ny * (D.g.nz / 4) + kBlock - 1) / kBlock);
+      L.segs.push_back(BSeg{s, 0, round8(march_items(D.g, TI)), 0});
+    } else {
+      D.TI = D.ntiles = 0;
+      L.segs.push_back(make_seg(s, nvec(D.g, B->Vl[l]), vpt));
+    }
+  }
+  HIPCHK(hipMemcpyAsync(B->d_lev[l], B->lev[l].data(), sizeof(BLev<float>) * B->nsys,
+                        hipMemcpyHostToDevice, B->stream));
   upload_list(B, L);
   const BLev<float> *lv = B->d_lev[l];
   const float omega = 0.9f;
   auto launch = [&]() {
-    ProlX<float, 4> dummy{};
-    (void)dummy;
     if (B->Vl[l] == 4)
       hipLaunchKernelGGL((kb_jacobi<float, 4, false, false>), dim3(L.nblocks), dim3(kBlock), 0,
                          B->stream, L.d_segs, L.d_bmap, B->d_fine, lv, lv, 0, 1, 0, 0.0f, omega,
@@ -370,6 +420,59 @@ double time_class(psb_batch *B, int l, const std::vector<int> &who, int vpt) {
   return ms / reps;
 }
 
+// ---------------------------------------------------------------------------
+// tuning cache (PSB_TUNE_CACHE): {"device": "<arch>:<CUs>", "entries": [
+// {"level": l, "class": c, "choice": v, "ms": t}, ...]} -- one entry per
+// (level, size class); choice as in psb_batch::tuned
+// ---------------------------------------------------------------------------
+std::string device_tag() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipDeviceProp_t pr;
+  HIPCHK(hipGetDeviceProperties(&pr, dev));
+  return std::string(pr.gcnArchName) + ":" + std::to_string(pr.multiProcessorCount);
+}
+
+struct TuneEntry {
+  int level, cls, choice;
+  double ms;
+};
+
+// the entries of the cache file for this device (empty if absent or for
+// another device)
+std::vector<TuneEntry> read_cache(const char *path, const std::string &tag) {
+  std::vector<TuneEntry> out;
+  std::ifstream f(path);
+  if (!f) return out;
+  std::stringstream ss;
+  ss << f.rdbuf();
+  const std::string t = ss.str();
+  const std::string dkey = "\"device\": \"" + tag + "\"";
+  if (t.find(dkey) == std::string::npos) return out;
+  size_t pos = 0;
+  while ((pos = t.find("{\"level\":", pos)) != std::string::npos) {
+    TuneEntry e{};
+    if (std::sscanf(t.c_str() + pos, "{\"level\": %d, \"class\": %d, \"choice\": %d, \"ms\": %lf",
+                    &e.level, &e.cls, &e.choice, &e.ms) == 4)
+      out.push_back(e);
+    pos++;
+  }
+  return out;
+}
+
+void write_cache(const char *path, const std::string &tag, const std::vector<TuneEntry> &es) {
+  std::ofstream f(path);
+  if (!f) return;  // a cache that cannot be written only costs the next tuning
+  f << "{\"device\": \"" << tag << "\", \"entries\": [";
+  for (size_t q = 0; q < es.size(); q++) {
+    char buf[160];
+    std::snprintf(buf, sizeof(buf), "%s\n  {\"level\": %d, \"class\": %d, \"choice\": %d, \"ms\": %.6g}",
+                  q ? "," : "", es[q].level, es[q].cls, es[q].choice, es[q].ms);
+    f << buf;
+  }
+  f << "\n]}\n";
+}
+
 void autotune(psb_batch *B) {
   B->tuned.clear();
   // the tuning sweeps need the states to read "not done"
@@ -377,27 +480,61 @@ void autotune(psb_batch *B) {
   std::memset(h.data(), 0, sizeof(DevState) * B->nsys);
   HIPCHK(hipMemcpyAsync(B->st, h.data(), sizeof(DevState) * B->nsys, hipMemcpyHostToDevice,
                         B->stream));
+  const char *cache = std::getenv("PSB_TUNE_CACHE");
+  const std::string tag = cache ? device_tag() : std::string();
+  std::vector<TuneEntry> known = cache ? read_cache(cache, tag) : std::vector<TuneEntry>();
+  std::map<std::pair<int, int>, TuneEntry> table;
+  for (const TuneEntry &e : known) table[{e.level, e.cls}] = e;
+  bool fresh = false;
   for (int l = 0; l < B->nlev; l++) {
     std::map<int, std::vector<int>> classes;
     for (int s = 0; s < B->nsys; s++)
       if (l < B->last[s]) classes[size_class(B->lev[l][s].g)].push_back(s);
     for (auto &kv : classes) {
+      auto hit = table.find({l, kv.first});
       int best = kVptDefault;
-      double tbest = 1e300;
-      for (int v : kVptCand) {
-        const double t = time_class(B, l, kv.second, v);
-        if (t < tbest) {
-          tbest = t;
-          best = v;
+      if (hit != table.end()) {
+        best = hit->second.choice;
+      } else {
+        double tbest = 1e300;
+        for (int v : kVptCand) {
+          const double t = time_class(B, l, kv.second, v, 0);
+          if (t < tbest) {
+            tbest = t;
+            best = v;
+          }
         }
+        bool marchable = true;
+        for (int s : kv.second) marchable = marchable && march_able(B, l, s);
+        if (marchable) {
+          for (int TI : kTiCand) {
+            const double t = time_class(B, l, kv.second, 0, TI);
+            if (t < tbest) {
+              tbest = t;
+              best = -TI;
+            }
+          }
+        }
+        table[{l, kv.first}] = TuneEntry{l, kv.first, best, tbest};
+        fresh = true;
       }
-      for (int s : kv.second) B->vpt[l][s] = best;
+      bool marchable = best < 0;
+      for (int s : kv.second) marchable = marchable && march_able(B, l, s);
+      for (int s : kv.second) {
+        B->vpt[l][s] = best > 0 ? best : kVptDefault;
+        B->ti[l][s] = marchable ? -best : 0;
+      }
       B->tuned.push_back(l);
       B->tuned.push_back(kv.first);
-      B->tuned.push_back(best);
+      B->tuned.push_back(marchable ? best : B->vpt[l][kv.second[0]]);
     }
   }
   HIPCHK(hipStreamSynchronize(B->stream));
+  if (cache && fresh) {
+    std::vector<TuneEntry> all;
+    for (auto &kv : table) all.push_back(kv.second);
+    write_cache(cache, tag, all);
+  }
 }
 
 void build(psb_batch *B) {
@@ -473,6 +610,9 @@ void build(psb_batch *B) {
   }
   B->d_fine = B->arena.alloc<BFine>(ns);
   B->vpt.assign(B->nlev, std::vector<int>(ns, B->tune > 0 ? B->tune : kVptDefault));
+  // a fixed work per thread: the large levels march with slabs that fill
+  // their segments; the autotuner chooses per (level, class) otherwise
+  B->ti.assign(B->nlev, std::vector<int>(ns, -1));
   // the fine tables point at the callers' x and b only at solve time
   HIPCHK(hipMemcpyAsync(B->d_fine, B->fine.data(), sizeof(BFine) * ns, hipMemcpyHostToDevice,
                         B->stream));

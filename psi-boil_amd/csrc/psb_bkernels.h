@@ -159,7 +159,8 @@ __global__ __launch_bounds__(kBlock) void kb_ru_resid(const BSeg *segs, const ui
   PSB_BSEG_BEGIN
   if (gated && !F.st->update) return;
   const BLev<float> &L = lev0[sg_.sys];
-  ru_resid_body<V, true>(sp, F.g, F.A, F.met, F.b, F.x, L.c, F.r, L.e[0], omega, w);
+  ru_resid_body<V, true>(sp, F.g, F.A, F.met, F.b, F.x, L.c, F.r, L.e[0], omega, w,
+                         F.st->norm == 0);
 }
 
 // ||b - A x||, ||b|| after the solve

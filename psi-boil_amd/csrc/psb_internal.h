@@ -270,6 +270,9 @@ struct psb_ctx {
   // device address of the static scratch of psb_solve_device_static on the
   // device this context was created on (found once per device at create)
   double *static_scratch = nullptr;
+  // fine-level kernels (bit masks of kLds* kinds, psb_api.hip) that run with
+  // the operand staged in LDS where the rows tile a block (psb_lds.h)
+  int lds32 = 2 | 8, lds64 = 1 | 2 | 4 | 8;
 };
 
 namespace psb {

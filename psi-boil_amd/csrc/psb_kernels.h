@@ -152,6 +152,9 @@ __device__ __forceinline__ void cg_xr_body(Span sp, const Grid &g, const T *c, c
                                            const T *p, const T *q, T *x, T *r, T *e1, T omega,
                                            const DevState *st, Work w) {
   const T alpha = (T)st->alpha;
+  // the reference norm's sum (dV loads, two FP64 divisions per cell) only
+  // when the solve stops on it
+  const bool vd = st->norm == 0;
   double acc[3] = {0.0, 0.0, 0.0};  // voldiv, r^2, rho_next
   PSB_SPAN_BEGIN(g, V, sp)
   VecN<T, V> X = ldv<T, V>(x + P), R = ldv<T, V>(r + P), Pp = ldv<T, V>(p + P),
@@ -161,8 +164,10 @@ __device__ __forceinline__ void cg_xr_body(Span sp, const Grid &g, const T *c, c
     X.v[u] = X.v[u] + alpha * Pp.v[u];
     R.v[u] = R.v[u] - alpha * Q.v[u];
     double rd = (double)R.v[u];
-    double dv = m.dV(i, j, k + u);
-    acc[0] += rd * rd / dv / dv;
+    if (vd) {
+      double dv = m.dV(i, j, k + u);
+      acc[0] += rd * rd / dv / dv;
+    }
     acc[1] += rd * rd;
     if (DIAG) {
       T z = R.v[u] * (T(1) / C.v[u]);
@@ -461,9 +466,9 @@ template <int V, bool SYM>
 __device__ __forceinline__ void ru_resid_body(Span sp, const Grid &g, const DMat<double> &A,
                                               const Metric &m, const double *b, const double *x,
                                               const float *c32, float *r32, float *e1,
-                                              float omega, Work w) {
+                                              float omega, Work w, bool vd) {
   ArrX<double, V> X{x};
-  double acc[3] = {0.0, 0.0, 0.0};  // r64^2, r64^2/dV^2, r32^2
+  double acc[3] = {0.0, 0.0, 0.0};  // r64^2, r64^2/dV^2 (vd: the reference norm), r32^2
   PSB_SPAN_BEGIN(g, V, sp)
   Gath<double, V> G = gather<double, V, SYM>(g, A, X, P, i, j, k);
   VecN<double, V> R = resid<double, V>(ldv<double, V>(b + P), G);
@@ -472,9 +477,11 @@ __device__ __forceinline__ void ru_resid_body(Span sp, const Grid &g, const DMat
   for (int u = 0; u < V; u++) {
     Rf.v[u] = (float)R.v[u];
     E.v[u] = omega * Rf.v[u] / Cf.v[u];
-    double dv = m.dV(i, j, k + u);
     acc[0] += R.v[u] * R.v[u];
-    acc[1] += R.v[u] * R.v[u] / dv / dv;
+    if (vd) {
+      double dv = m.dV(i, j, k + u);
+      acc[1] += R.v[u] * R.v[u] / dv / dv;
+    }
     acc[2] += (double)Rf.v[u] * (double)Rf.v[u];
   }
   stv<float, V>(r32 + P, Rf);
@@ -490,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid(Grid g, DMat<double> A, Met
                                                      float omega, const DevState *st,
                                                      bool gated, Work w) {
   if (gated && !st->update) return;
-  ru_resid_body<V, SYM>(grid_span(), g, A, m, b, x, c32, r32, e1, omega, w);
+  ru_resid_body<V, SYM>(grid_span(), g, A, m, b, x, c32, r32, e1, omega, w, st->norm == 0);
 }
 
 // x64 += (double) d32, d32 = 0
