@@ -76,7 +76,7 @@ constexpr int kMarchMin = 1 << 15;
 // FP32 first post-sweep 84.8 against 98.5 us (span), CG p/q step 109 against
 // 131 us (register march); the other FP32 sweeps were slower staged (97 /
 // 121 against 85 / 113 us), so they keep the register march.
-enum { kLdsSweep = 1, kLdsProl = 2, kLdsLast = 4, kLdsCgPq = 8 };
+enum { kLdsSweep = 1, kLdsProl = 2, kLdsLast = 4, kLdsCgPq = 8, kLdsResid = 16 };
 template <typename T>
 bool lds_use(const psb_ctx *C, int kind) {
   return ((sizeof(T) == 8 ? C->lds64 : C->lds32) & kind) != 0;
@@ -1503,18 +1503,28 @@ void ru_resid(psb_ctx *C, const double *b, const double *x, bool gated, int op) 
   Level<float> &f = C->h32.L[0];
   KrylovVecs<float> &K = C->k32;
   const float omega = (float)(C->ac_omega / f.scale);
+  int nb = nblocks<double>(g);
   {
     // FP64 c, w, s, b, x, b and FP32 c in; FP32 r and e1 out (timed only
     // when it runs: a gated launch without a pending update returns at once)
     ProbeScope ps(C, PSB_K_RU_RESID, probe_gate(C, gated) ? g : Grid{}, 6 * 8 + 3 * 4);
     with_v<double>(g, [&](auto vt) {
       constexpr int V = decltype(vt)::value;
+      MarchShape m;
+      if (lds_use<double>(C, kLdsResid) && V > 1 && g.N >= kMarchMin && lds_ok<V>(g) &&
+          march_ok<V>(g, m)) {
+        nb = m.nblocks;
+        hipLaunchKernelGGL((k_ru_resid_lds<V, true>), dim3(m.nblocks), dim3(kBlock), 0,
+                           C->stream, g, mat64(C), metric(C), b, x, f.c, K.r, f.e[0], omega,
+                           C->st32, gated, C->work, m.TI, m.ntiles);
+        return;
+      }
       hipLaunchKernelGGL((k_ru_resid<V, true>), dim3(nblocks<double>(g)), dim3(kBlock), 0,
                          C->stream, g, mat64(C), metric(C), b, x, f.c, K.r, f.e[0], omega,
                          C->st32, gated, C->work);
     });
   }
-  finalize<3>(C, op, nblocks<double>(g), C->st32);
+  finalize<4>(C, op, nb, C->st32);
 }
 
 void ru_xupdate(psb_ctx *C, double *x, bool gated) {
@@ -1564,9 +1574,14 @@ void solve_ru(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_res
     ru_resid(C, b, x, false, FIN_RU_END);
     read_state(C, C->st32);
     h.res_true = C->hst->res_true;
+    h.rl2_true = C->hst->rl2_true;
   }
   finish_result(R, h, h.outer);
   R->res = h.res_true;
+  // ||b - A x|| / ||b|| from the last FP64 residual (the x it was taken at:
+  // a converged solve stops right after a refresh, which leaves no FP32
+  // correction behind), instead of another pass over the FP64 system
+  R->rel_l2 = h.bsq > 0.0 ? h.rl2_true / std::sqrt(h.bsq) : 0.0;
   C->hist_count = std::min(h.outer + 1, C->hist_cap);
 }
 
@@ -1915,9 +1930,10 @@ int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_
       build_hier32(C);
       set_half(C, p->ac_store >= 1);
     }
-    if (p->precision == PSB_MIXED && p->inner_adapt == 2)
-      solve_ru(C, x, b, p, R);
-    else if (p->precision == PSB_MIXED)
+    if (p->precision == PSB_MIXED && p->inner_adapt == 2) {
+      solve_ru(C, x, b, p, R);  // sets rel_l2 itself
+      return R->status;
+    } else if (p->precision == PSB_MIXED)
       solve_ir(C, x, b, p, R);
     else
       solve_ac64(C, x, b, p, R);

@@ -65,7 +65,6 @@ struct psb_batch {
   Arena arena;                // plan: tables, lists, partials
   DevState *st = nullptr, *hst = nullptr;
   int *flags = nullptr, *hflags = nullptr;
-  double *relres = nullptr;
   double *partial = nullptr;
   bool built = false;
   int tune = 0;               // 0: autotune, > 0: fixed vectors per thread
@@ -292,7 +291,7 @@ void l_ru_resid(psb_batch *B, bool gated, int op) {
   else
     hipLaunchKernelGGL((kb_ru_resid<1>), dim3(L.nblocks), dim3(kBlock), 0, B->stream, L.d_segs,
                        L.d_bmap, B->d_fine, B->d_lev[0], (float)B->omega, gated, w);
-  l_finalize<3>(B, op);
+  l_finalize<4>(B, op);
 }
 
 void l_ru_xupdate(psb_batch *B, bool gated) {
@@ -600,7 +599,6 @@ void build(psb_batch *B) {
     F.p[1] = C->k32.p[1];
     F.q = C->k32.q;
     F.st = B->st + s;
-    F.relres = B->relres + 2 * s;
   }
   B->d_lev.assign(B->nlev, nullptr);
   for (int l = 0; l < B->nlev; l++) {
@@ -696,25 +694,12 @@ int batch_solve(psb_batch *B, double *const *x, const double *const *b, const ps
     HIPCHK(hipGraphLaunch(B->graph.exec, B->stream));
     if (++launches > (long long)p->max_iter + 2) break;
   }
-  // fold in the last corrections, final true residuals, ||b - A x|| / ||b||
+  // fold in the last corrections; the final true residuals give ||b - A x||
+  // (||b|| came with the first one)
   l_ru_xupdate(B, false);
   l_ru_resid(B, false, FIN_RU_END);
-  {
-    const SegList &L = B->fine64L;
-    const Work w{B->partial};
-    if (B->V2 == 2)
-      hipLaunchKernelGGL((kb_relres<2>), dim3(L.nblocks), dim3(kBlock), 0, B->stream, L.d_segs,
-                         L.d_bmap, B->d_fine, w);
-    else
-      hipLaunchKernelGGL((kb_relres<1>), dim3(L.nblocks), dim3(kBlock), 0, B->stream, L.d_segs,
-                         L.d_bmap, B->d_fine, w);
-    l_finalize<2>(B, FIN_RELRES);
-  }
   HIPCHK(hipGetLastError());
-  std::vector<double> rr(2 * ns);
   HIPCHK(hipMemcpyAsync(B->hst, B->st, sizeof(DevState) * ns, hipMemcpyDeviceToHost,
-                        B->stream));
-  HIPCHK(hipMemcpyAsync(rr.data(), B->relres, sizeof(double) * 2 * ns, hipMemcpyDeviceToHost,
                         B->stream));
   HIPCHK(hipStreamSynchronize(B->stream));
   int status = PSB_OK;
@@ -722,7 +707,7 @@ int batch_solve(psb_batch *B, double *const *x, const double *const *b, const ps
     const DevState &h = B->hst[s];
     finish_result(&R[s], h, h.outer);
     R[s].res = h.res_true;
-    R[s].rel_l2 = rr[2 * s + 1] > 0 ? std::sqrt(rr[2 * s] / rr[2 * s + 1]) : 0.0;
+    R[s].rel_l2 = h.bsq > 0.0 ? h.rl2_true / std::sqrt(h.bsq) : 0.0;
     if (status == PSB_OK && R[s].status != PSB_OK) status = R[s].status;
   }
   return status;
@@ -762,7 +747,6 @@ int psb_batch_create(psb_batch **out, int nsys, const psb_layout *layouts) {
     HIPCHK(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
     B->st = B->base.alloc<DevState>(nsys);
     B->flags = B->base.alloc<int>(nsys);
-    B->relres = B->base.alloc<double>(2 * nsys);
     HIPCHK(hipHostMalloc((void **)&B->hst, sizeof(DevState) * nsys, hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&B->hflags, sizeof(int) * nsys, hipHostMallocDefault));
     return (int)PSB_OK;

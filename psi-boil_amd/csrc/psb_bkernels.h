@@ -43,7 +43,6 @@ struct BFine {
   double *x;             // solution (FP64), updated at every reliable update
   float *d, *r, *p[2], *q;  // FP32 correction since the last update, CG vectors
   DevState *st;
-  double *relres;        // 2 doubles: sum (b - A x)^2, sum b^2
 };
 
 #define PSB_BSEG_BEGIN                                    \
@@ -163,14 +162,6 @@ __global__ __launch_bounds__(kBlock) void kb_ru_resid(const BSeg *segs, const ui
                          F.st->norm == 0);
 }
 
-// ||b - A x||, ||b|| after the solve
-template <int V>
-__global__ __launch_bounds__(kBlock) void kb_relres(const BSeg *segs, const uint16_t *bmap,
-                                                    const BFine *fine, Work w) {
-  PSB_BSEG_BEGIN
-  relres_body<V, true>(sp, F.g, F.A, F.b, F.x, w);
-}
-
 // zero the FP32 correction and both search directions
 template <int V>
 __global__ __launch_bounds__(kBlock) void kb_zero(const BSeg *segs, const uint16_t *bmap,
@@ -198,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void kb_finalize(int op, const BSeg *fsegs,
   if (!fin_skip(op, st)) {
     double acc[NR];
     fin_sum<NR>(fsegs[sys].b0, fsegs[sys].nb, partial, acc);
-    if (threadIdx.x == 0) fin_apply(op, acc, st, nullptr, F.relres);
+    if (threadIdx.x == 0) fin_apply(op, acc, st, nullptr, nullptr);
   }
   if (threadIdx.x == 0) flags[sys] = st->done;
 }

@@ -108,6 +108,7 @@ struct DevState {
   int update, outer;                           // refresh pending; refreshes done
   double rlast, delta;                         // true residual at the last refresh
   double res_true;                             // last true (FP64) residual
+  double rl2_true, bsq;                        // its L2 norm; ||b||^2 (reliable updates)
 };
 
 // ---------------------------------------------------------------------------

@@ -272,7 +272,7 @@ struct psb_ctx {
   double *static_scratch = nullptr;
   // fine-level kernels (bit masks of kLds* kinds, psb_api.hip) that run with
   // the operand staged in LDS where the rows tile a block (psb_lds.h)
-  int lds32 = 2 | 8, lds64 = 1 | 2 | 4 | 8;
+  int lds32 = 2 | 8, lds64 = 1 | 2 | 4 | 8 | 16;
 };
 
 namespace psb {

@@ -468,10 +468,12 @@ __device__ __forceinline__ void ru_resid_body(Span sp, const Grid &g, const DMat
                                               const float *c32, float *r32, float *e1,
                                               float omega, Work w, bool vd) {
   ArrX<double, V> X{x};
-  double acc[3] = {0.0, 0.0, 0.0};  // r64^2, r64^2/dV^2 (vd: the reference norm), r32^2
+  // r64^2, r64^2/dV^2 (vd: the reference norm), r32^2, b^2
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   PSB_SPAN_BEGIN(g, V, sp)
   Gath<double, V> G = gather<double, V, SYM>(g, A, X, P, i, j, k);
-  VecN<double, V> R = resid<double, V>(ldv<double, V>(b + P), G);
+  const VecN<double, V> Bv = ldv<double, V>(b + P);
+  VecN<double, V> R = resid<double, V>(Bv, G);
   VecN<float, V> Cf = ldv<float, V>(c32 + P), Rf, E;
 #pragma unroll
   for (int u = 0; u < V; u++) {
@@ -483,11 +485,12 @@ __device__ __forceinline__ void ru_resid_body(Span sp, const Grid &g, const DMat
       acc[1] += R.v[u] * R.v[u] / dv / dv;
     }
     acc[2] += (double)Rf.v[u] * (double)Rf.v[u];
+    acc[3] += Bv.v[u] * Bv.v[u];
   }
   stv<float, V>(r32 + P, Rf);
   stv<float, V>(e1 + P, E);
   PSB_TILE_END
-  block_partial<3>(acc, w.partial);
+  block_partial<4>(acc, w.partial);
 }
 
 template <int V, bool SYM>
@@ -835,6 +838,8 @@ __device__ inline void fin_apply(int op, const double *tot, DevState *st, DevSta
         if (st->hist && st->outer < st->hist_cap) st->hist[st->outer] = res;
       }
       st->res_true = res;
+      st->rl2_true = rl2;
+      if (op == FIN_RU_START) st->bsq = tot[3];
       st->rlast = rl2;
       st->update = 0;
       st->res = sqrt(tot[2]);
@@ -863,6 +868,7 @@ __device__ inline void fin_apply(int op, const double *tot, DevState *st, DevSta
     }
     case FIN_RU_END:  // true residual at exit without convergence
       st->res_true = st->norm == 0 ? ref_norm(st, tot[1]) : sqrt(tot[0]);
+      st->rl2_true = sqrt(tot[0]);
       break;
     case FIN_RHO:
     case FIN_RHO_FLEX: {

@@ -161,4 +161,45 @@ __global__ __launch_bounds__(kBlock) void k_cg_pq_lds(Grid g, DMat<M> A, const T
   block_partial<1>(acc, w.partial);
 }
 
+// reliable update of the mixed solve: r32 = (float)(b - A x) in FP64, e1 =
+// omega r32 / c32; partials r64^2, r64^2 / dV^2 (vd), r32^2, b^2.  As
+// k_ru_resid, marching with x staged in LDS.
+template <int V, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_ru_resid_lds(Grid g, DMat<double> A, Metric m,
+                                                         const double *b, const double *x,
+                                                         const float *c32, float *r32,
+                                                         float *e1, float omega,
+                                                         const DevState *st, bool gated,
+                                                         Work w, int TI, int ntiles) {
+  if (gated && !st->update) return;
+  __shared__ double buf[2][2 * kBlock * V];
+  const bool vd = st->norm == 0;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int wi = xcd_block();
+  const int slab = wi / ntiles, tile = wi - slab * ntiles;
+  auto epi = [&](int P, int i, int j, int k, const Gath<double, V> &G) {
+    const VecN<double, V> Bv = ldv<double, V>(b + P);
+    const VecN<double, V> R = resid<double, V>(Bv, G);
+    const VecN<float, V> Cf = ldv<float, V>(c32 + P);
+    VecN<float, V> Rf, E;
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+      Rf.v[u] = (float)R.v[u];
+      E.v[u] = omega * Rf.v[u] / Cf.v[u];
+      acc[0] += R.v[u] * R.v[u];
+      if (vd) {
+        const double dv = m.dV(i, j, k + u);
+        acc[1] += R.v[u] * R.v[u] / dv / dv;
+      }
+      acc[2] += (double)Rf.v[u] * (double)Rf.v[u];
+      acc[3] += Bv.v[u] * Bv.v[u];
+    }
+    stv<float, V>(r32 + P, Rf);
+    stv<float, V>(e1 + P, E);
+  };
+  lds_march_item<double, V, SYM, false, double>(g, A, ArrX<double, V>{x}, TI, slab, tile, buf,
+                                                epi);
+  block_partial<4>(acc, w.partial);
+}
+
 }  // namespace psb
