@@ -351,26 +351,38 @@ def batch_case(a):
         B.systems[q].set_geometry(d["dxc"], d["dyc"], d["dzc"])
         dbs.append(psb.DeviceArray.from_host(d["rhs"]))
         dxs.append(psb.DeviceArray(d["rhs"].size))
-    stop = dict(res_rat=1e-8, max_iter=1000)
+    # the V-cycle (measured faster for this batch: its systems are small
+    # enough that the K-cycle's extra small-level launches cost more than its
+    # iterations save), and the K-cycle on levels 1 and 2 (by level index)
+    # beside it
+    kb = 0 if a.kcycle == -1 else a.kcycle
+    stop = dict(res_rat=1e-8, max_iter=1000, ac_kcycle=kb)
     px, pb = [x.addr for x in dxs], [b.addr for b in dbs]
 
-    def run_batch():
+    def run_batch(**kw):
         for x in dxs:
             x.zero()
-        return B.solve_device(px, pb, **stop)
+        return B.solve_device(px, pb, **dict(stop, **kw))
+
+    def timed_batch(**kw):
+        for _ in range(max(1, a.warmup)):
+            res, rc = run_batch(**kw)
+        psb.device_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            res, rc = run_batch(**kw)
+        psb.device_synchronize()
+        return (time.perf_counter() - t0) / a.steps, res, rc
 
     log("[bench] batched solves (plan build and autotuning in the warm-up)")
-    for _ in range(max(1, a.warmup)):
-        res, rc = run_batch()
-    psb.device_synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        res, rc = run_batch()
-    psb.device_synchronize()
-    tb = (time.perf_counter() - t0) / a.steps
+    tk, resk, _ = timed_batch(ac_kcycle=-1)
+    itk = [r["iterations"] for r in resk]
+    log(f"[bench] batch K-cycle {tk * 1e3:.2f} ms")
+    tb, res, rc = timed_batch()
     xb = [x.download() for x in dxs]
     log(f"[bench] batch {tb * 1e3:.2f} ms rc {rc}")
-    # the same systems one after another (same contexts, same method)
+    # the same systems one after another (same contexts, same method; the
+    # batch's K-cycle levels by index)
     kw = dict(psb.BATCH_STOP, **stop)
     for q in range(len(cases)):
         dxs[q].zero()
@@ -398,7 +410,8 @@ def batch_case(a):
     its = [r["iterations"] for r in res]
     out = {"workload": f"{len(cases)} pressure systems, dims from {{32,48,64,96,128}} per "
                        "direction (seed 4), 1-8 bubbles r 0.05-0.2 of the shortest side, "
-                       "density ratio 10-1000; FP32 AC-PCG with FP64 reliable updates to 1e-8",
+                       "density ratio 10-1000; FP32 AC-PCG with FP64 reliable updates to 1e-8"
+                       + ("" if kb == 0 else f", K-cycle on levels {kb:#b} (bit = level)"),
            "systems": len(cases), "cells": ncell,
            "batched_ms": round(tb * 1e3, 3), "batched_systems_per_s": round(len(cases) / tb, 2),
            "sequential_ms": round(ts * 1e3, 3),
@@ -407,7 +420,11 @@ def batch_case(a):
            "max_rel_l2": max(r["rel_l2"] for r in res),
            "iterations_min_mean_max": [min(its), sum(its) / len(its), max(its)],
            "max_error_vs_fp64_1e-12": max(errs),
-           "plan_level_class_vpt": B.plan()}
+           "plan_level_class_choice": B.plan()}
+    out["kcycle_levels_1_2"] = {"batched_ms": round(tk * 1e3, 3),
+                                "iterations_min_mean_max": [min(itk), sum(itk) / len(itk),
+                                                            max(itk)],
+                                "converged": sum(r["converged"] for r in resk)}
     print(json.dumps(out), flush=True)
 
 

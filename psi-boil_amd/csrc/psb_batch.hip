@@ -86,6 +86,9 @@ struct psb_batch {
   std::vector<SegList> gemvL;                   // [l]: systems with last == l
   Graph graph;
   double omega = 0, cfac = 0;
+  int kmask = 0;                                // K-cycle levels of the tables (bit l: level l)
+  bool flex = false;                            // flexible beta in the outer CG (with K-cycles)
+  bool kset = false;                            // the tables carry kmask's K-cycle buffers
   std::vector<int> tuned;                       // (level, class, choice) triples:
                                                 // choice > 0 vectors per thread (span),
                                                 // < 0 minus the planes per slab (march)
@@ -194,8 +197,9 @@ void build_lists(psb_batch *B) {
       B->restrictL[l].push_back(kv);
     }
   }
-  size_t need = (size_t)B->fineL.nblocks * kMaxPartials;
-  B->partial = B->arena.alloc<double>(need);
+  int most = B->fineL.nblocks;
+  for (const SegList &L : B->smoothL) most = std::max(most, L.nblocks);
+  B->partial = B->arena.alloc<double>((size_t)most * kMaxPartials);
 }
 
 // the marching fields of the level tables (the Jacobi sweeps, and on level
@@ -233,14 +237,15 @@ void l_jacobi(psb_batch *B, int l, int cur, int nxt, int outH) {
   const Work w{B->partial};
   const BLev<float> *lh = l + 1 < B->nlev ? B->d_lev[l + 1] : nullptr;
   const float cfac = (float)B->cfac, omega = (float)B->omega;
+  const bool flex = RHO && B->flex;
   if (B->Vl[l] == 4)
     hipLaunchKernelGGL((kb_jacobi<float, 4, PROL, RHO>), dim3(L.nblocks), dim3(kBlock), 0,
                        B->stream, L.d_segs, L.d_bmap, B->d_fine, B->d_lev[l], lh, cur, nxt, outH,
-                       cfac, omega, w);
+                       cfac, omega, w, flex);
   else
     hipLaunchKernelGGL((kb_jacobi<float, 1, PROL, RHO>), dim3(L.nblocks), dim3(kBlock), 0,
                        B->stream, L.d_segs, L.d_bmap, B->d_fine, B->d_lev[l], lh, cur, nxt, outH,
-                       cfac, omega, w);
+                       cfac, omega, w, flex);
 }
 
 template <int V, int KV, int CI, int CJ, int CK>
@@ -304,38 +309,77 @@ void l_ru_xupdate(psb_batch *B, bool gated) {
                        L.d_segs, L.d_bmap, B->d_fine, gated);
 }
 
-// one V-cycle of every system (z in level 0's e[out]) and the rho step
-void b_vcycle(psb_batch *B, int nu1, int nu2) {
+// K-cycle steps of level l over the systems that smooth there
+template <typename F>
+void l_kc(psb_batch *B, int l, F launch) {
+  const SegList &L = B->smoothL[l];
+  if (L.empty()) return;
+  if (B->Vl[l] == 4) launch(std::integral_constant<int, 4>(), L);
+  else launch(std::integral_constant<int, 1>(), L);
+}
+
+void b_cycle(psb_batch *B, int l, int nu1, int nu2);
+
+// level l's correction by two flexible-CG steps preconditioned by the cycle
+// on l (as kstep in the single-system solver), in kz of the systems that
+// smooth on l; the systems whose coarsest level is l keep their dense solve
+void b_kstep(psb_batch *B, int l, int nu1, int nu2) {
   const int out = (nu1 - 1 + nu2) % 3;
-  const int cur_pre = (nu1 - 1) % 3;
-  for (int l = 0; l < B->nlev; l++) {
-    const SegList &G = B->gemvL[l];
-    if (!G.empty())
-      hipLaunchKernelGGL((kb_gemv<float>), dim3(G.nblocks), dim3(kBlock), 0, B->stream, G.d_segs,
-                         G.d_bmap, B->d_fine, B->d_lev[l], out);
-    if (B->smoothL[l].empty()) continue;
-    int cur = 0;
-    for (int s = 1; s < nu1; s++) {
-      const int nxt = (cur + 1) % 3;
-      l_jacobi<false, false>(B, l, cur, nxt, 0);
-      cur = nxt;
-    }
-    l_restrict(B, l, cur);
+  const float omega = (float)B->omega;
+  b_cycle(B, l, nu1, nu2);
+  l_kc(B, l, [&](auto vt, const SegList &L) {
+    constexpr int V = decltype(vt)::value;
+    hipLaunchKernelGGL((kb_kc_av<float, V>), dim3(L.nblocks), dim3(kBlock), 0, B->stream,
+                       L.d_segs, L.d_bmap, B->d_fine, B->d_lev[l], out, Work{B->partial});
+    hipLaunchKernelGGL((kb_kc_r<float, V>), dim3(L.nblocks), dim3(kBlock), 0, B->stream,
+                       L.d_segs, L.d_bmap, B->d_fine, B->d_lev[l], omega, B->partial);
+  });
+  b_cycle(B, l, nu1, nu2);
+  l_kc(B, l, [&](auto vt, const SegList &L) {
+    constexpr int V = decltype(vt)::value;
+    hipLaunchKernelGGL((kb_kc_dw<float, V>), dim3(L.nblocks), dim3(kBlock), 0, B->stream,
+                       L.d_segs, L.d_bmap, B->d_fine, B->d_lev[l], out, Work{B->partial});
+    hipLaunchKernelGGL((kb_kc_comb<float, V>), dim3(L.nblocks), dim3(kBlock), 0, B->stream,
+                       L.d_segs, L.d_bmap, B->d_fine, B->d_lev[l], out, B->partial);
+  });
+}
+
+// the cycle on level l of every system that reaches it (its correction in
+// e[out]; the first sweep already in e[0]): the dense solve where l is the
+// coarsest level, else pre-sweeps, restriction, the coarse correction (the
+// K-cycle's two steps on K levels) and the post-sweeps with its prolongation
+void b_cycle(psb_batch *B, int l, int nu1, int nu2) {
+  const int out = (nu1 - 1 + nu2) % 3;
+  const SegList &G = B->gemvL[l];
+  if (!G.empty())
+    hipLaunchKernelGGL((kb_gemv<float>), dim3(G.nblocks), dim3(kBlock), 0, B->stream, G.d_segs,
+                       G.d_bmap, B->d_fine, B->d_lev[l], out);
+  if (B->smoothL[l].empty()) return;
+  int cur = 0;
+  for (int s = 1; s < nu1; s++) {
+    const int nxt = (cur + 1) % 3;
+    l_jacobi<false, false>(B, l, cur, nxt, 0);
+    cur = nxt;
   }
-  for (int l = B->nlev - 1; l >= 0; l--) {
-    if (B->smoothL[l].empty()) continue;
-    int cur = cur_pre;
-    for (int s = 0; s < nu2; s++) {
-      const int nxt = (cur + 1) % 3;
-      const bool last = (s == nu2 - 1) && l == 0;
-      if (s == 0 && last) l_jacobi<true, true>(B, l, cur, nxt, out);
-      else if (s == 0) l_jacobi<true, false>(B, l, cur, nxt, out);
-      else if (last) l_jacobi<false, true>(B, l, cur, nxt, out);
-      else l_jacobi<false, false>(B, l, cur, nxt, out);
-      cur = nxt;
-    }
+  l_restrict(B, l, cur);
+  if ((B->kmask >> (l + 1)) & 1) b_kstep(B, l + 1, nu1, nu2);
+  else b_cycle(B, l + 1, nu1, nu2);
+  for (int s = 0; s < nu2; s++) {
+    const int nxt = (cur + 1) % 3;
+    const bool last = (s == nu2 - 1) && l == 0;
+    if (s == 0 && last) l_jacobi<true, true>(B, l, cur, nxt, out);
+    else if (s == 0) l_jacobi<true, false>(B, l, cur, nxt, out);
+    else if (last) l_jacobi<false, true>(B, l, cur, nxt, out);
+    else l_jacobi<false, false>(B, l, cur, nxt, out);
+    cur = nxt;
   }
-  l_finalize<kRhoParts>(B, FIN_RHO);
+}
+
+// one preconditioner application of every system (z in level 0's e[out])
+// and the rho step
+void b_vcycle(psb_batch *B, int nu1, int nu2) {
+  b_cycle(B, 0, nu1, nu2);
+  l_finalize<kRhoParts>(B, B->flex ? FIN_RHO_FLEX : FIN_RHO);
 }
 
 void b_iteration(psb_batch *B, int par, int nu1, int nu2) {
@@ -397,11 +441,11 @@ double time_class(psb_batch *B, int l, const std::vector<int> &who, int vpt, int
     if (B->Vl[l] == 4)
       hipLaunchKernelGGL((kb_jacobi<float, 4, false, false>), dim3(L.nblocks), dim3(kBlock), 0,
                          B->stream, L.d_segs, L.d_bmap, B->d_fine, lv, lv, 0, 1, 0, 0.0f, omega,
-                         Work{B->partial});
+                         Work{B->partial}, false);
     else
       hipLaunchKernelGGL((kb_jacobi<float, 1, false, false>), dim3(L.nblocks), dim3(kBlock), 0,
                          B->stream, L.d_segs, L.d_bmap, B->d_fine, lv, lv, 0, 1, 0, 0.0f, omega,
-                         Work{B->partial});
+                         Work{B->partial}, false);
   };
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
@@ -622,14 +666,60 @@ void build(psb_batch *B) {
   HIPCHK(hipStreamSynchronize(B->stream));
   for (int s = 0; s < ns; s++) B->seen[s] = B->sys[s]->version;
   B->built = true;
+  B->kset = false;  // new level tables: the K-cycle fields are set per solve
+}
+
+// K-cycle levels of a batch: by level index (the hierarchies differ in
+// depth), on every system that smooths on that level; -1 chooses levels 1
+// and 2
+constexpr int kBatchKcycleAuto = 0b110;
+
+// the level tables' K-cycle fields for `mask`: buffers from each system's
+// hierarchy arena (kept with the hierarchy), flags on the levels a system
+// smooths on
+void set_kcycle(psb_batch *B, int mask) {
+  if (B->kset && B->kmask == mask) return;
+  for (int l = 0; l < B->nlev; l++) {
+    for (int s = 0; s < B->nsys; s++) {
+      BLev<float> &D = B->lev[l][s];
+      D.kcyc = l >= 1 && ((mask >> l) & 1) && l < B->last[s];
+      if (!D.kcyc) continue;
+      Level<float> &lv = B->sys[s]->h32.L[l];
+      if (!lv.kc) {
+        psb_ctx *C = B->sys[s];
+        lv.kc = C->harena.alloc<float>(lv.g.N);
+        lv.kv = C->harena.alloc<float>(lv.g.N);
+        lv.kz = C->harena.alloc<float>(lv.g.N);
+        lv.kst = C->harena.alloc<double>(4);
+      }
+      D.kc = lv.kc;
+      D.kv = lv.kv;
+      D.kz = lv.kz;
+      D.kst = lv.kst;
+    }
+    HIPCHK(hipMemcpyAsync(B->d_lev[l], B->lev[l].data(), sizeof(BLev<float>) * B->nsys,
+                          hipMemcpyHostToDevice, B->stream));
+  }
+  HIPCHK(hipStreamSynchronize(B->stream));
+  B->kmask = mask;
+  B->kset = true;
 }
 
 int batch_solve(psb_batch *B, double *const *x, const double *const *b, const psb_stop *p,
                 psb_result *R) {
   if (p->precond != PSB_PREC_AC || p->method != PSB_CG || p->precision != PSB_MIXED ||
-      p->inner_adapt != 2 || p->ac_pre < 1 || p->ac_post < 1 || p->ac_kcycle != 0)
+      p->inner_adapt != 2 || p->ac_pre < 1 || p->ac_post < 1)
     return PSB_EINVAL;
+  if (p->ac_kcycle < -1 || (p->ac_kcycle > 0 && (p->ac_kcycle & 1))) return PSB_EINVAL;
   if (stale(B)) build(B);
+  if (p->ac_flexible < -1 || p->ac_flexible > 1) return PSB_EINVAL;
+  const int kmask = p->ac_kcycle == -1 ? kBatchKcycleAuto : p->ac_kcycle;
+  // the outer CG's beta: flexible when asked, or by default when the
+  // preconditioner varies (K-cycle)
+  const bool flex = p->ac_flexible == 1 || (p->ac_flexible == -1 && kmask != 0);
+  if (!B->kset || kmask != B->kmask || flex != B->flex) destroy_graph(B->graph);
+  set_kcycle(B, kmask);
+  B->flex = flex;
   const int ns = B->nsys;
   // point the fine tables at this call's arrays
   for (int s = 0; s < ns; s++) {

@@ -25,6 +25,9 @@ struct BLev {
   const double *G;    // coarsest level: dense projected pseudo-inverse
   int TI, ntiles;     // TI > 0: the Jacobi sweeps (and on level 0 the CG p/q
                       // step) march, slabs of TI planes, ntiles tiles a plane
+  int kcyc;           // this level's correction is the K-cycle's (in kz)
+  T *kc, *kv, *kz;    // K-cycle: first cycle's output, A of it, the correction
+  double *kst;        // K-cycle scalars rho1, alpha1, a, alpha2
   __device__ __forceinline__ DMat<T> mat() const {
     return DMat<T>{c, w, s, b, nullptr, nullptr, nullptr};
   }
@@ -50,23 +53,27 @@ struct BFine {
   const BFine &F = fine[sg_.sys];                          \
   const Span sp = seg_span(sg_);
 
-// damped Jacobi sweep on level `lev` (prolongation of levH's e[outH] fused
-// in when PROL; RHO on level 0's last post-sweep)
+// damped Jacobi sweep on level `lev` (prolongation of levH's correction
+// fused in when PROL: e[outH] times cfac, or where levH took the K-cycle its
+// kz, already scaled; RHO on level 0's last post-sweep)
 template <typename T, int V, bool PROL, bool RHO>
 __global__ __launch_bounds__(kBlock) void kb_jacobi(const BSeg *segs, const uint16_t *bmap,
                                                     const BFine *fine, const BLev<T> *lev,
                                                     const BLev<T> *levH, int cur, int nxt,
-                                                    int outH, T cfac, T omega, Work w) {
+                                                    int outH, T cfac, T omega, Work w,
+                                                    bool flex) {
   PSB_BSEG_BEGIN
   if (F.st->done) return;
   const BLev<T> &L = lev[sg_.sys];
+  // RHO with the flexible beta: the partials of z.q, q the last direction's A p
+  const T *qf = RHO && flex ? (const T *)F.q : nullptr;
   ProlX<T, V> pr{};
   if (PROL) {
     const BLev<T> &H = levH[sg_.sys];
     pr.e = L.e[cur];
-    pr.eH = H.e[outH];
+    pr.eH = H.kcyc ? H.kz : H.e[outH];
     pr.af = L.af;
-    pr.cfac = cfac;
+    pr.cfac = H.kcyc ? T(1) : cfac;
     pr.sI = H.sI;
     pr.sJ = H.sJ;
     pr.sK = H.sK;
@@ -77,12 +84,12 @@ __global__ __launch_bounds__(kBlock) void kb_jacobi(const BSeg *segs, const uint
     if (L.TI > 0) {
       jacobi_march_body<T, V, false, RHO, T, true>(sp, L.g, L.matflux(), L.r, L.e[cur],
                                                    L.e[nxt], pr, omega, L.af, w, T(1), L.TI,
-                                                   L.ntiles);
+                                                   L.ntiles, qf);
       return;
     }
   }
   jacobi_body<T, V, true, PROL, RHO, T, true>(sp, L.g, L.matflux(), L.r, L.e[cur], L.e[nxt], pr,
-                                              omega, L.af, w);
+                                              omega, L.af, w, T(1), qf);
 }
 
 // restriction of level lev into levH (all systems of the launch share the
@@ -97,6 +104,51 @@ __global__ __launch_bounds__(kBlock) void kb_restrict(const BSeg *segs, const ui
   const BLev<T> &H = levH[sg_.sys];
   restrict_body<T, V, KV, true, CI, CJ, CK, T, true>(sp, L.g, L.matflux(), L.r, L.e[cur], L.af,
                                                      H.g, H.c, H.af, H.r, H.e[0], omega);
+}
+
+// K-cycle steps on level `lev` of the systems of the list (kc_av_body and
+// the others, psb_kernels.h); each system's dot products summed over its own
+// segment of blocks
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void kb_kc_av(const BSeg *segs, const uint16_t *bmap,
+                                                   const BFine *fine, const BLev<T> *lev,
+                                                   int out, Work w) {
+  PSB_BSEG_BEGIN
+  if (F.st->done) return;
+  const BLev<T> &L = lev[sg_.sys];
+  kc_av_body<T, V, true>(sp, L.g, L.matflux(), T(1), L.e[out], L.r, L.kc, L.kv, w);
+}
+
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void kb_kc_r(const BSeg *segs, const uint16_t *bmap,
+                                                  const BFine *fine, const BLev<T> *lev,
+                                                  T omega, const double *partial) {
+  PSB_BSEG_BEGIN
+  if (F.st->done) return;
+  const BLev<T> &L = lev[sg_.sys];
+  kc_r_body<T, V>(sp, L.g, L.c, L.kv, L.r, L.e[0], omega, L.kst, sg_.nb,
+                  partial + (size_t)sg_.b0 * 2);
+}
+
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void kb_kc_dw(const BSeg *segs, const uint16_t *bmap,
+                                                   const BFine *fine, const BLev<T> *lev,
+                                                   int out, Work w) {
+  PSB_BSEG_BEGIN
+  if (F.st->done) return;
+  const BLev<T> &L = lev[sg_.sys];
+  kc_dw_body<T, V, true>(sp, L.g, L.matflux(), T(1), L.e[out], L.kv, L.r, w);
+}
+
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void kb_kc_comb(const BSeg *segs, const uint16_t *bmap,
+                                                     const BFine *fine, const BLev<T> *lev,
+                                                     int out, const double *partial) {
+  PSB_BSEG_BEGIN
+  if (F.st->done) return;
+  const BLev<T> &L = lev[sg_.sys];
+  kc_comb_body<T, V>(sp, L.g, L.kc, L.e[out], L.kz, L.kst, sg_.nb,
+                     partial + (size_t)sg_.b0 * 3);
 }
 
 // coarsest level: e[out] = G r

@@ -111,6 +111,17 @@ struct DevState {
   double rl2_true, bsq;                        // its L2 norm; ||b||^2 (reliable updates)
 };
 
+// Whether a kernel of the preconditioner on a level of n cells returns at
+// once because the solve has stopped.  Below kDoneCheckMin cells it never
+// reads the flag: the load is one more round trip to memory before the
+// operands on levels whose launches last a few microseconds, and running such
+// a launch after the stop only rewrites that level's work vectors, which
+// every later use writes before it reads them.
+constexpr int kDoneCheckMin = 1 << 19;
+__device__ __forceinline__ bool skip_done(const DevState *st, int n) {
+  return n >= kDoneCheckMin && st->done;
+}
+
 // ---------------------------------------------------------------------------
 // reductions, pass 1: wave64 butterfly, LDS across the 4 waves, one partial
 // per block.  Pass 2 is a one-block kernel (k_finalize) that sums the

@@ -7,7 +7,8 @@
 // are two-pass and deterministic: each streaming kernel leaves one partial
 // per block, k_finalize sums them in a fixed order in FP64 (for both
 // precisions) and applies the scalar step (alpha, beta, stop test, ...).
-// Kernels of an iteration start with `if (st->done) return;`, so a captured
+// Kernels of an iteration start with `if (st->done) return;` (skip_done for
+// the preconditioner's kernels), so a captured
 // graph of several iterations stops doing work once the solver has finished.
 #pragma once
 
@@ -287,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(Grid g, DMat<M> A, const T *r
                                                    T *eout, ProlX<T, V> pr, T omega,
                                                    const uint8_t *af, DevState *st, Work w,
                                                    T zscale, const T *qf) {
-  if (st->done) return;
+  if (skip_done(st, g.N)) return;
   jacobi_body<T, V, SYM, PROL, RHO, M, FLUX>(grid_span(), g, A, r, e, eout, pr, omega, af, w,
                                              zscale, qf);
 }
@@ -356,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Grid g, DMat<M> A, const T 
                                                      const uint8_t *af, Grid gH, const T *cH,
                                                      const uint8_t *afH, T *rH, T *eH1,
                                                      T omega, DevState *st) {
-  if (st->done) return;
+  if (skip_done(st, g.N)) return;
   restrict_body<T, V, KV, SYM, CI, CJ, CK, M, FLUX>(grid_span(), g, A, r, e, af, gH, cH, afH,
                                                     rH, eH1, omega);
 }
@@ -378,7 +379,7 @@ __device__ __forceinline__ void gemv_body(int bl, int m, const double *Gm, const
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gemv(int m, const double *Gm, const T *r, T *z,
                                                  DevState *st) {
-  if (st->done) return;
+  if (skip_done(st, m)) return;
   gemv_body<T>(blockIdx.x, m, Gm, r, z);
 }
 
@@ -982,14 +983,14 @@ template <typename T, int V, bool FLUX>
 __global__ __launch_bounds__(kBlock) void k_kc_av(Grid g, DMat<T> A, T scale, const T *c,
                                                   const T *r, T *kc, T *v, const DevState *st,
                                                   Work w) {
-  if (st->done) return;
+  if (skip_done(st, g.N)) return;
   kc_av_body<T, V, FLUX>(grid_span(), g, A, scale, c, r, kc, v, w);
 }
 
 // r <- r - alpha1 v, and the next cycle's first sweep e0 = omega r / c.  Every
 // block sums k_kc_av's nprev partials itself (in k_finalize's order; a few
 // hundred at most on these levels) instead of a finalize launch: rho1 =
-// c.v, alpha1 = c.r / rho1, written to kst[0..1] by block 0
+// c.v, alpha1 = c.r / rho1, written to kst[0..1] by the span's block 0
 template <typename T, int V>
 __device__ __forceinline__ void kc_r_body(Span sp, const Grid &g, const T *c, const T *v, T *r,
                                           T *e0, T omega, double *kst, int nprev,
@@ -999,7 +1000,7 @@ __device__ __forceinline__ void kc_r_body(Span sp, const Grid &g, const T *c, co
   fin_sum<2>(0, nprev, partial, tot);
   if (threadIdx.x == 0) {
     a1s = tot[0] > 0.0 ? tot[1] / tot[0] : 0.0;
-    if (blockIdx.x == 0) {
+    if (sp.bl == 0) {
       kst[0] = tot[0];
       kst[1] = a1s;
     }
@@ -1022,7 +1023,7 @@ template <typename T, int V>
 __global__ __launch_bounds__(kBlock) void k_kc_r(Grid g, const T *c, const T *v, T *r, T *e0,
                                                  T omega, double *kst, int nprev,
                                                  const double *partial, const DevState *st) {
-  if (st->done) return;
+  if (skip_done(st, g.N)) return;
   kc_r_body<T, V>(grid_span(), g, c, v, r, e0, omega, kst, nprev, partial);
 }
 
@@ -1050,14 +1051,14 @@ template <typename T, int V, bool FLUX>
 __global__ __launch_bounds__(kBlock) void k_kc_dw(Grid g, DMat<T> A, T scale, const T *d,
                                                   const T *v, const T *r, const DevState *st,
                                                   Work w) {
-  if (st->done) return;
+  if (skip_done(st, g.N)) return;
   kc_dw_body<T, V, FLUX>(grid_span(), g, A, scale, d, v, r, w);
 }
 
 // out = a kc + alpha2 d, the coefficients from k_kc_dw's nprev partials
 // (gamma = d.v, beta = d.s A d, delta = d.rt) summed by every block, with
 // rho1 and alpha1 from kst: alpha2 = delta / (beta - gamma^2 / rho1),
-// a = alpha1 - gamma alpha2 / rho1 (block 0 keeps them in kst[2..3])
+// a = alpha1 - gamma alpha2 / rho1 (the span's block 0 keeps them in kst[2..3])
 template <typename T, int V>
 __device__ __forceinline__ void kc_comb_body(Span sp, const Grid &g, const T *kc, const T *d,
                                              T *out, double *kst, int nprev,
@@ -1071,7 +1072,7 @@ __device__ __forceinline__ void kc_comb_body(Span sp, const Grid &g, const T *kc
     const double a2 = den > 0.0 ? tot[2] / den : 0.0;
     cs[0] = a1 - (rho1 > 0.0 ? tot[0] * a2 / rho1 : 0.0);
     cs[1] = a2;
-    if (blockIdx.x == 0) {
+    if (sp.bl == 0) {
       kst[2] = cs[0];
       kst[3] = cs[1];
     }
@@ -1090,7 +1091,7 @@ template <typename T, int V>
 __global__ __launch_bounds__(kBlock) void k_kc_comb(Grid g, const T *kc, const T *d, T *out,
                                                     double *kst, int nprev,
                                                     const double *partial, const DevState *st) {
-  if (st->done) return;
+  if (skip_done(st, g.N)) return;
   kc_comb_body<T, V>(grid_span(), g, kc, d, out, kst, nprev, partial);
 }
 

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_lds(Grid g, DMat<M> A, const 
                                                        T omega, const uint8_t *af,
                                                        const DevState *st, Work w, T zscale,
                                                        int TI, int ntiles, const T *qf) {
-  if (st->done) return;
+  if (skip_done(st, g.N)) return;
   __shared__ T buf[2][2 * kBlock * V];
   double acc[kRhoParts] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int wi = xcd_block();

@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_march(Grid g, DMat<M> A, cons
                                                          T omega, const uint8_t *af,
                                                          const DevState *st, Work w, T zscale,
                                                          int TI, int ntiles, const T *qf) {
-  if (st->done) return;
+  if (skip_done(st, g.N)) return;
   jacobi_march_body<T, V, PROL, RHO, M, FLUX>(grid_span(), g, A, r, e, eout, pr, omega, af, w,
                                               zscale, TI, ntiles, qf);
 }
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict_march(Grid g, DMat<M> A, co
                                                            const uint8_t *afH, T *rH, T *eH1,
                                                            T omega, DevState *st, int TI,
                                                            int ntiles) {
-  if (st->done) return;
+  if (skip_done(st, g.N)) return;
   constexpr int NC = V / CK;  // coarse cells per thread
   const int wi = xcd_block();
   const int slab = wi / ntiles, tile = wi - slab * ntiles;

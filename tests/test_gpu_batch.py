@@ -133,14 +133,29 @@ def test_batched_plan_follows_matrix_changes():
     assert np.linalg.norm(O.resid(A, d["rhs"], xs[0])) <= 1.01e-8 * np.linalg.norm(d["rhs"])
 
 
-def test_batched_solve_rejects_the_kcycle():
-    """The batched solver runs the V-cycle only: a K-cycle request is an
-    invalid argument, not a silent V-cycle."""
-    B, data = make_batch(SHAPES[:2], seed=61)
+def test_batched_kcycle_matches_single_systems():
+    """The K-cycle in the batch (levels chosen by index: bits of ac_kcycle,
+    -1 = levels 1 and 2, on every system that smooths there) gives each
+    system what the same method gives it alone: converged to the tolerance,
+    iterations within 2, solution within 1e-6; an odd mask is rejected."""
+    B, data = make_batch(SHAPES, seed=61)
     bs = [d["rhs"] for d in data]
+    with pytest.raises(psb.PsbError, match="EINVAL"):
+        B.solve([np.zeros(b.size) for b in bs], bs, res_rat=1e-8, max_iter=300, ac_kcycle=0b11)
     for km in (-1, 0b100):
-        with pytest.raises(psb.PsbError, match="EINVAL"):
-            B.solve([np.zeros(b.size) for b in bs], bs, res_rat=1e-8, max_iter=300,
-                    ac_kcycle=km)
+        print(f"[batch] kcycle {km}", flush=True)
+        xs, res, rc = B.solve([np.zeros(b.size) for b in bs], bs, res_rat=1e-8, max_iter=300,
+                              ac_kcycle=km)
+        assert rc == psb.PSB_OK
+        mask = 0b110 if km == -1 else km
+        for q, (d, x, r) in enumerate(zip(data, xs, res)):
+            assert r["converged"] and r["rel_l2"] <= 1e-8 * 1.01, (q, r)
+            x1, r1 = B.systems[q].solve(np.zeros(d["rhs"].size), d["rhs"],
+                                        **dict(psb.BATCH_STOP, res_rat=1e-8, max_iter=300,
+                                               ac_kcycle=mask))
+            assert r1["converged"], (q, r1)
+            assert abs(r["iterations"] - r1["iterations"]) <= 2, (q, r, r1)
+            assert rel(demean(x), demean(x1)) < 1e-6
+    # back to the V-cycle on the same batch
     xs, res, rc = B.solve([np.zeros(b.size) for b in bs], bs, res_rat=1e-8, max_iter=300)
     assert rc == psb.PSB_OK
