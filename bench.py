@@ -455,8 +455,8 @@ def main():
                     help="with --headline-only: only this one of the two solves (one "
                          "precision per traced process)")
     ap.add_argument("--profile-kernels", action="store_true",
-                    help="only launch the roofline kernels (FP32 Jacobi sweep, FP32/FP64 SpMV, "
-                         "FP32 CG p/q step, first post-sweep) and a plain copy, for "
+                    help="only launch the roofline kernels (FP32 Jacobi sweeps, FP32/FP64 "
+                         "SpMV, FP32 CG p/q and x/r steps) and a plain copy, for "
                          "hardware-counter runs")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -484,9 +484,11 @@ def main():
         dx0 = psb.DeviceArray(n ** 3)
         db0 = psb.DeviceArray.from_host(d["rhs"])
         s.solve_device(dx0.addr, db0.addr, precond=psb.PSB_PREC_AC, precision=psb.PSB_MIXED,
-                       inner_adapt=2, inner_rtol=1e-2, ac_store=1, max_iter=1)
+                       inner_adapt=2, inner_rtol=1e-2, ac_store=1, ac_kcycle=a.kcycle,
+                       max_iter=1)
         for kern, name in ((2, "jacobi_fp32"), (1, "spmv_fp32"), (0, "spmv_fp64"),
-                           (6, "cg_pq_fp32"), (7, "sweep_prol_fp32"), (3, "copy_fp32")):
+                           (6, "cg_pq_fp32"), (7, "sweep_prol_fp32"), (8, "sweep_last_fp32"),
+                           (9, "cg_xr_fp32"), (3, "copy_fp32")):
             ms, by = s.time_kernel(kern, reps=10)
             log(f"[bench] {name}: {ms * 1e3:.1f} us/launch, {by / 1e6:.1f} MB algorithmic, "
                 f"{by / ms / 1e6:.0f} GB/s")

@@ -323,12 +323,16 @@ int psb_generate(const psb_case *cs, double *c, double *w, double *e,
  * restriction with mean removal, interpolation, the V, F and W cycles, the
  * full-cycle start, and its stopping, divergence, min-cycle and linf
  * controls (additive_cycle_functions.cpp:12-87, additive.h:97-142).
- * Cycle::flex (additive_recursive.cpp:167-200) is not offered: its
- * recursion ends only when a level's smoother meets its ResRat, and once a
- * level's residual reaches round-off it alternates between two levels for
- * good (so it does with the reference's own build on these cases).
+ * Cycle::flex (additive_recursive.cpp:161-200, AC::flexcycle) with a bound:
+ * its recursion climbs back only when a level's smoother meets its ResRat,
+ * and where one cannot, the reference's own build goes down and up between
+ * two levels without end; here a flex cycle stops after max_visits level
+ * visits, and the run ends as after a divergence (PSB_DIVERGED, x the
+ * previous cycle's phi).  Where the reference's recursion ends, the result
+ * is the same.
  * ------------------------------------------------------------------------ */
-enum { PSB_CYCLE_NONE = -1, PSB_CYCLE_V = 0, PSB_CYCLE_F = 1, PSB_CYCLE_W = 2 };
+enum { PSB_CYCLE_NONE = -1, PSB_CYCLE_V = 0, PSB_CYCLE_F = 1, PSB_CYCLE_W = 2,
+       PSB_CYCLE_FLEX = 3 };
 
 typedef struct {
   int struct_size;    /* sizeof(psb_cycle), set by psb_cycle_defaults; other
@@ -343,7 +347,8 @@ typedef struct {
   double dt;          /* time step: residual = sqrt(dot_voldiv_avg) * dt
                          (additive_residual.cpp:19-29); smoothers use 1/dt
                          as their scale (additive_cycle_functions.cpp:94)   */
-  int loop;           /* the loop's cycle: PSB_CYCLE_V (default), _F, _W     */
+  int loop;           /* the loop's cycle: PSB_CYCLE_V (default), _F, _W,
+                         _FLEX                                              */
   int init;           /* PSB_CYCLE_NONE (default), or the shape of a full
                          cycle run before the loop (fullcycle(loop) uses V) */
   int stop_if_div;    /* AC::stop_if_diverging (1)                           */
@@ -357,6 +362,9 @@ typedef struct {
                          and post-smoothing CG solves (-1, -1, -1 = off,
                          additive.cpp:19; at most 64); the coarsest solve
                          runs without one (additive_recursive.cpp:27)       */
+  int max_visits;     /* PSB_CYCLE_FLEX: level visits per cycle before the
+                         run is ended (1000; no counterpart in the
+                         reference, whose flex recursion is unbounded)      */
 } psb_cycle;
 
 void psb_cycle_defaults(psb_cycle *c);
@@ -432,7 +440,9 @@ int psb_batch_solve_device(psb_batch *bt, double *const *d_x,
  * hardware byte counters), 4 the fine level's two post-sweeps in one pass
  * (ac_fuse = 1 form), 5 the same two sweeps as two kernels, 6 the FP32 CG
  * p/q step of the mixed multigrid-preconditioned solve on the fine level,
- * 7 the fine level's first post-sweep with the prolongation fused in.
+ * 7 the fine level's first post-sweep with the prolongation fused in, 8 its
+ * last post-sweep with the rho partials of the flexible beta, 9 the FP32 CG
+ * x/r step.
  * *bytes receives the algorithmic bytes one launch moves (for 4 and 5:
  * those of one sweep).                                                      */
 int psb_time_kernel(psb_ctx *ctx, int kernel, int reps, double *avg_ms,

@@ -1750,6 +1750,42 @@ void ref_loop(psb_ctx *C, int l, int shape, const RefPar &q) {
   else ref_fw(C, l, q, shape == PSB_CYCLE_W);
 }
 
+// flex_cycle (additive_recursive.cpp:161-200) from level 0, its tail
+// recursion as a loop driven from the host (each step depends on whether
+// the level's CG met its tolerance): smooth (MaxIter mv[0] going down,
+// mv[2] coming up); a level whose smoother converged hands its phi up
+// (interpolation) and the finer level smooths again, level 0 ends the
+// cycle; one whose smoother did not restricts its residual and goes down;
+// the coarsest level is solved and hands up.  Returns false after
+// max_visits level visits (the reference does not bound them).
+bool ref_flex(psb_ctx *C, const RefPar &q, int max_visits) {
+  const int nl = (int)C->href.L.size();
+  int l = 0;
+  bool up = false;
+  for (int v = 0; v < max_visits; v++) {
+    if (l != nl - 1) {
+      const int s = up ? 2 : 0;
+      ref_cg(C, l, q.mv[s], q.rr, q.rt, q.scale, q.ms[s]);
+      read_state(C, C->st64);
+      if (C->hst->converged) {
+        if (l == 0) return true;
+        ref_interp(C, l);
+        l--;
+        up = true;
+      } else {
+        ref_restrict(C, l);
+        l++;
+        up = false;
+      }
+    } else {
+      ref_coarsest(C, l, q);
+      l--;
+      up = false;  // the call's default sign (additive.h:185-187)
+    }
+  }
+  return false;
+}
+
 // full_cycle (:127-163): restrict down to the coarsest level, then the
 // chosen cycle from every level on the way back
 void ref_full(psb_ctx *C, int l, int shape, const RefPar &q) {
@@ -1796,7 +1832,8 @@ int ac_cycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_res
   if (P->max_cycles < 0 || P->min_cycles < 0 || P->dt <= 0.0 || P->max_iter[0] < 1 ||
       P->max_iter[1] < 1 || P->max_iter[2] < 1)
     return PSB_EINVAL;
-  if (P->loop < PSB_CYCLE_V || P->loop > PSB_CYCLE_W) return PSB_EINVAL;
+  if (P->loop < PSB_CYCLE_V || P->loop > PSB_CYCLE_FLEX) return PSB_EINVAL;
+  if (P->loop == PSB_CYCLE_FLEX && P->max_visits < 1) return PSB_EINVAL;
   if (P->init < PSB_CYCLE_NONE || P->init > PSB_CYCLE_W) return PSB_EINVAL;
   for (int s = 0; s < 3; s++)
     if (P->max_stale[s] > kStaleMax) return PSB_EINVAL;
@@ -1878,8 +1915,18 @@ int ac_cycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_res
       res_m1 = ref_residual(C, &linf_m1);
     }
     for (int c = 1; c <= P->max_cycles; c++) {
-      capture(C, H.gr, key, [&]() { ref_loop(C, 0, P->loop, q); });
-      HIPCHK(hipGraphLaunch(H.gr.exec, C->stream));
+      if (P->loop == PSB_CYCLE_FLEX) {
+        if (!ref_flex(C, q, P->max_visits)) {
+          // no end in sight: the last cycle's phi, as after a divergence
+          HIPCHK(hipMemcpyAsync(L0.e[0], L0.e[1], N0 * 8, hipMemcpyDeviceToDevice, C->stream));
+          ncyc = c;
+          status = PSB_DIVERGED;
+          break;
+        }
+      } else {
+        capture(C, H.gr, key, [&]() { ref_loop(C, 0, P->loop, q); });
+        HIPCHK(hipGraphLaunch(H.gr.exec, C->stream));
+      }
       const int con = converged(c, res_m0, linf_m0, res_m1, linf_m1);
       C->hist_host.push_back(linf ? linf_m0 : res_m0);
       res_out = linf ? linf_m0 : res_m0;
@@ -2435,6 +2482,7 @@ void psb_cycle_defaults(psb_cycle *c) {
   c->smooth_res_rat = kAtto;
   c->smooth_res_tol = kAtto;
   c->max_stale[0] = c->max_stale[1] = c->max_stale[2] = -1;  // additive.cpp:19
+  c->max_visits = 1000;
 }
 
 int psb_ac_vcycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_result *R) {
@@ -2591,6 +2639,57 @@ int psb_time_kernel(psb_ctx *C, int kernel, int reps, double *avg_ms, double *by
       // the active flags when the level has solid cells)
       *bytes = N * (4 * (lv.h16() ? 2 : 4) + 3 * 4 + 4.0 / (cL.CI * cL.CJ * cL.CK) +
                     (lv.aflag ? 1 : 0));
+    } else if (kernel == 8) {
+      // the last post-sweep of level 0 as the mixed solve runs it with the
+      // K-cycle (rho partials with the flexible beta: q read too), register
+      // march on the level's FP16 copy
+      build_hier32(C);
+      ensure_krylov(C, C->k32, false);
+      const Level<float> lv = C->h32.L[0];
+      const float *qf = C->k32.q;
+      MarchShape m;
+      if (!lv.h16() || !lv.d || !vec_ok<float>(g) || !march_ok<4>(g, m)) return PSB_EINVAL;
+      zero(C, lv.e[0], g.N);
+      zero(C, lv.r, g.N);
+      zero(C, C->k32.q, g.N);
+      DevState h;
+      std::memset(&h, 0, sizeof(h));
+      HIPCHK(hipMemcpyAsync(C->st32, &h, sizeof(h), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      const ProlX<float, 4> pr{};
+      const float zs = (float)lv.scale;
+      f = [=]() {
+        hipLaunchKernelGGL((k_jacobi_march<float, 4, false, true, _Float16, true>),
+                           dim3(m.nblocks), dim3(kBlock), 0, stream, g, lv.mat16(), lv.r,
+                           lv.e[0], lv.e[1], pr, 0.9f, lv.aflag, C->st32, C->work, zs, m.TI,
+                           m.ntiles, qf);
+      };
+      // four FP16 coefficient arrays, r, e and q in, e out
+      *bytes = N * (4 * 2 + 4 * 4 + (lv.aflag ? 1 : 0));
+    } else if (kernel == 9) {
+      // the FP32 CG x/r step of the mixed solve (x, r, p, q, c in; x, r and
+      // the next cycle's first sweep out)
+      build_hier32(C);
+      ensure_krylov(C, C->k32, false);
+      const Level<float> lv = C->h32.L[0];
+      const KrylovVecs<float> K = C->k32;
+      if (!vec_ok<float>(g)) return PSB_EINVAL;
+      zero(C, K.x, g.N);
+      zero(C, K.r, g.N);
+      zero(C, K.p[0], g.N);
+      zero(C, K.q, g.N);
+      DevState h;
+      std::memset(&h, 0, sizeof(h));
+      h.norm = 1;
+      HIPCHK(hipMemcpyAsync(C->st32, &h, sizeof(h), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      const Metric mt = metric(C);
+      f = [=]() {
+        hipLaunchKernelGGL((k_cg_xr<float, 4, false>), dim3(nblocks<float>(g)), dim3(kBlock), 0,
+                           stream, g, lv.c, mt, K.p[0], K.q, K.x, K.r, lv.e[0], 0.9f, C->st32,
+                           C->work);
+      };
+      *bytes = N * 8 * 4;
     } else if (kernel == 3) {
       ensure32(C);
       ensure_krylov(C, C->k32, false);

@@ -75,7 +75,7 @@ class Metrics(C.Structure):
                                           "dzb", "dzt")]
 
 
-PSB_CYCLE_NONE, PSB_CYCLE_V, PSB_CYCLE_F, PSB_CYCLE_W = -1, 0, 1, 2
+PSB_CYCLE_NONE, PSB_CYCLE_V, PSB_CYCLE_F, PSB_CYCLE_W, PSB_CYCLE_FLEX = -1, 0, 1, 2, 3
 
 
 class Cycle(C.Structure):
@@ -85,13 +85,14 @@ class Cycle(C.Structure):
                 ("loop", C.c_int), ("init", C.c_int), ("stop_if_div", C.c_int),
                 ("priority_min_cyc", C.c_int), ("use_linf", C.c_int),
                 ("smooth_res_rat", C.c_double), ("smooth_res_tol", C.c_double),
-                ("max_stale", C.c_int * 3)]
+                ("max_stale", C.c_int * 3), ("max_visits", C.c_int)]
 
 
 def _cycle(**cyc):
     c = Cycle()
     lib().psb_cycle_defaults(C.byref(c))
-    shapes = {None: PSB_CYCLE_NONE, "V": PSB_CYCLE_V, "F": PSB_CYCLE_F, "W": PSB_CYCLE_W}
+    shapes = {None: PSB_CYCLE_NONE, "V": PSB_CYCLE_V, "F": PSB_CYCLE_F, "W": PSB_CYCLE_W,
+              "flex": PSB_CYCLE_FLEX}
     for k, v in cyc.items():
         if k in ("max_iter", "max_stale"):
             getattr(c, k)[:] = list(v)

@@ -49,16 +49,20 @@ def load_golden(name):
     return Golden(name)
 
 
-def load_cycles(name):
-    """(phi of each run by tag, the runs' settings and logs) of <name>_cycles.npz."""
-    z = np.load(os.path.join(GOLDEN, name + "_cycles.npz"), allow_pickle=False)
+FLEX_CASES = ["bubbles16x12x8", "bubbles16"]  # <case>_flex.npz: AC::flexcycle runs
+
+
+def load_cycles(name, kind="cycles"):
+    """(phi of each run by tag, the runs' settings and logs) of <name>_<kind>.npz
+    (kind "cycles" or "flex")."""
+    z = np.load(os.path.join(GOLDEN, name + "_" + kind + ".npz"), allow_pickle=False)
     meta = json.loads(str(z["meta"]))
     return {k: z[k] for k in z.files if k != "meta"}, meta["runs"]
 
 
 def cycle_kwargs(run):
     """The keyword arguments of the oracle's AC.cycle for one reference run."""
-    shapes = ["V", "F", "W"]
+    shapes = ["V", "F", "W", "flex"]
     return dict(loop=shapes[run["kind"] % 10], init="V" if run["kind"] >= 10 else None,
                 toler=-1.0, factor=run["factor"], max_cyc=run["max_cyc"],
                 min_cyc=run["min_cyc"], mv=tuple(run["mv"]), stop_if_div=run["stop_if_div"],

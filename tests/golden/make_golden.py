@@ -8,11 +8,12 @@ outputs, solver iterates after k iterations, AC level matrices, restriction /
 interpolation results and V-cycle iterates, plus the solver log lines (17
 significant digits) parsed into arrays.
 
-    python tests/golden/make_golden.py [cases] [cycles] [prec] [case names]   # needs oracle/_ref built
+    python tests/golden/make_golden.py [cases] [cycles] [flex] [prec] [case names]   # needs oracle/_ref built
 
 "cycles" writes <case>_cycles.npz: the reference's AC::cycle with the F, W
 and full-cycle shapes and its controls (ref_driver cycles), phi after each
-run and the run's residual log.  "prec" writes <case>_prec.npz: CG with the
+run and the run's residual log.  "flex" writes <case>_flex.npz: the same for
+AC::flexcycle on the level settings where its recursion ends.  "prec" writes <case>_prec.npz: CG with the
 reference's incomplete-Cholesky preconditioners IC0 and IC2 (ref_driver
 prec), x after 1 and 5 iterations and at ResRat 1e-8, and the final log
 lines.
@@ -169,6 +170,37 @@ def run_cycles(name):
     return path
 
 
+# AC::flexcycle runs (ref_driver flex): level-solve ResRat and MaxIter for
+# which the reference's recursion ends (it does not on many settings: once a
+# level's smoother cannot meet its ResRat it alternates between two levels)
+FLEX_RUNS = {
+    "bubbles16": [(0.5, 20, (20, 20, 20)), (0.2, 20, (20, 20, 20))],
+    "bubbles16x12x8": [(0.5, 20, (10, 10, 10)), (0.2, 20, (20, 20, 20)),
+                       (0.2, 20, (10, 10, 10)), (0.1, 20, (20, 20, 20)),
+                       (0.1, 20, (10, 10, 10))],
+}
+
+
+def run_flex(name):
+    nx, ny, nz, nsph, rmin, rmax, rin, rout, ss, sv, centre = CASES[name]
+    runs, arrays = [], {}
+    for rr, maxc, mv in FLEX_RUNS[name]:
+        with tempfile.TemporaryDirectory() as d:
+            args = [DRIVER, "flex", d, str(nx), str(ny), str(nz), str(nsph), repr(rmin),
+                    repr(rmax), repr(rin), repr(rout), str(ss), str(sv), repr(rr), str(maxc),
+                    *[str(m) for m in mv]]
+            out = subprocess.run(args, check=True, capture_output=True, text=True,
+                                 timeout=120).stdout
+            run = parse_cycles(out)[0]
+            run["tag"] = f"Flex_{rr}_{mv[0]}"
+            arrays[run["tag"] + "_phi"] = np.fromfile(os.path.join(d, "Flex_phi.f64"))
+            runs.append(run)
+    arrays["meta"] = np.array(json.dumps({"name": name, "runs": runs}))
+    path = os.path.join(HERE, name + "_flex.npz")
+    np.savez_compressed(path, **arrays)
+    return path
+
+
 def run_prec(name):
     """CG with the reference's Prec::ic0 / Prec::ic2 (ref_driver prec): x
     after 1 and 5 iterations and at 1e-8, the printed final residuals."""
@@ -201,7 +233,7 @@ def run_prec(name):
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("oracle/_ref/ref_driver is not built (make -f oracle/ref.mk)")
-    what = sys.argv[1:] or ["cases", "cycles", "prec"]
+    what = sys.argv[1:] or ["cases", "cycles", "flex", "prec"]
     only = [w for w in what if w in CASES]  # restrict to the cases named
     if "cases" in what:
         for name, spec in CASES.items():
@@ -211,6 +243,10 @@ def main():
         for name in CYCLE_CASES:
             if not only or name in only:
                 print(run_cycles(name))
+    if "flex" in what:
+        for name in FLEX_RUNS:
+            if not only or name in only:
+                print(run_flex(name))
     if "prec" in what:
         for name in PREC_CASES:
             if not only or name in only:

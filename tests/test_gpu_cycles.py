@@ -2,7 +2,9 @@
 init / control fields) against the reference's own runs (cycles fixtures,
 written by its own build): F and W cycles, full-cycle starts, the linf
 control with its stale previous value, divergence and min-cycle controls,
-per-stage MaxIter and stale counts and the level solves' ResRat.
+per-stage MaxIter and stale counts and the level solves' ResRat; and the
+flex cycle (flexcycle, its recursion bounded by max_visits) on the settings
+where the reference's own recursion ends.
 
 The GPU sums dot products in a tree, the reference sequentially.  How much
 that matters is a property of each run, and it is measured: the oracle
@@ -22,18 +24,21 @@ import pytest
 
 import psb
 import psb_oracle as O
-from conftest import CYCLE_CASES, cycle_kwargs, load_cycles, load_golden
+from conftest import CYCLE_CASES, FLEX_CASES, cycle_kwargs, load_cycles, load_golden
 
 pytestmark = pytest.mark.gpu
 
-_SHAPES = ["V", "F", "W"]
+_SHAPES = ["V", "F", "W", "flex"]
 
 
 def _runs():
     out = []
     for case in CYCLE_CASES:
         _, runs = load_cycles(case)
-        out += [(case, r["tag"]) for r in runs]
+        out += [(case, "cycles", r["tag"]) for r in runs]
+    for case in FLEX_CASES:  # AC::flexcycle where the reference's recursion ends
+        _, runs = load_cycles(case, "flex")
+        out += [(case, "flex", r["tag"]) for r in runs]
     return out
 
 
@@ -83,10 +88,10 @@ def oracle_variants(g, run, col):
     return out
 
 
-@pytest.mark.parametrize("case,tag", _runs())
-def test_gpu_cycle_matches_reference(case, tag):
+@pytest.mark.parametrize("case,kind,tag", _runs())
+def test_gpu_cycle_matches_reference(case, kind, tag):
     g, s = solver(case)
-    phis, runs = load_cycles(case)
+    phis, runs = load_cycles(case, kind)
     run = [r for r in runs if r["tag"] == tag][0]
     col = 2 if run["use_linf"] else 1
     ref_hist = np.array([run["initial"][col - 1]] + [c[col] for c in run["cycles"]])
@@ -154,3 +159,25 @@ def test_gpu_stale_counts_beyond_the_window_are_rejected():
         s.solve(np.zeros(n), b, stalecount=65, max_iter=10)
     x, r = s.ac_vcycle(np.zeros(n), b, res_rat=1e-8, max_stale=(64, 64, 64))
     assert r["converged"]
+
+
+def test_gpu_flex_cycle_bound():
+    """A flex cycle whose recursion does not end (bubbles16, the level solves'
+    ResRat 1e-3: level 0's 20-iteration smoother cannot meet it, so the
+    reference goes down and up between levels for good) stops after
+    max_visits level visits and ends the run as a divergence would:
+    PSB_DIVERGED in the first cycle, phi the initial guess, the history only
+    the initial residual; as the oracle's bound."""
+    g, s = solver("bubbles16")
+    b, n = g["b"], g["b"].size
+    x0 = np.zeros(n)
+    x, r = s.ac_vcycle(x0, b, loop="flex", res_rat=1e-8, max_cycles=20, smooth_res_rat=1e-3,
+                       max_visits=400)
+    h = s.history()
+    assert r["status"] == psb.PSB_DIVERGED and not r["converged"] and r["iterations"] == 1
+    assert len(h) == 1 and r["res"] == h[0]
+    assert np.array_equal(x, x0)
+    A = O.Mat(*g.shape, g.mat())
+    xo, info = O.AC(A, g["dV"], rule=0).cycle(x0, b, loop="flex", factor=1e-8, max_cyc=20,
+                                              mv=(20, 20, 20), resrat=1e-3, max_visits=400)
+    assert info["bound"] and info["ncyc"] == 1 and np.array_equal(xo, x0)

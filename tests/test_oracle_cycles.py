@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import psb_oracle as O
-from conftest import CYCLE_CASES, cycle_kwargs, load_cycles, load_golden
+from conftest import CYCLE_CASES, FLEX_CASES, cycle_kwargs, load_cycles, load_golden
 
 
 def _runs():
@@ -75,3 +75,41 @@ def test_stale_counts_change_the_result(systems, tag):
         kw["ms"] = (4, 2, 4)
         phi, _ = O.AC(A, g["dV"], rule=0).cycle(np.zeros(A.n), g["b"], **kw)
         assert not np.array_equal(phi, phis[tag + "_phi"])
+
+
+def _flex_runs():
+    return [(case, r["tag"]) for case in FLEX_CASES for r in load_cycles(case, "flex")[1]]
+
+
+@pytest.mark.parametrize("case,tag", _flex_runs())
+def test_flex_cycle_matches_reference(case, tag):
+    """AC::flexcycle (additive_recursive.cpp:161-200) where the reference's
+    recursion ends: phi bit for bit, the logged residuals, the cycle count
+    and the return value; the bound on level visits is not reached."""
+    g = load_golden(case)
+    phis, runs = load_cycles(case, "flex")
+    run = [r for r in runs if r["tag"] == tag][0]
+    A = O.Mat(*g.shape, g.mat())
+    ac = O.AC(A, g["dV"], rule=0)
+    phi, info = ac.cycle(np.zeros(A.n), g["b"], **cycle_kwargs(run))
+    assert not info["bound"]
+    assert np.array_equal(phi, phis[tag + "_phi"])
+    assert info["ncyc"] == run["ncyc"] and info["converged"] == run["converged"]
+    assert len(info["hist"]) == 1 + len(run["cycles"])
+    for q, (c, res, linf) in enumerate(run["cycles"]):
+        assert info["hist"][q + 1] == pytest.approx(res, rel=1e-15), (c, q)
+
+
+def test_flex_cycle_bound_ends_a_run_the_reference_does_not_end():
+    """On bubbles16 with the level solves' ResRat 0.1 the reference's flex
+    recursion does not return (its smoother on one level cannot meet the
+    ratio, so it goes down and up between two levels); the oracle's bound on
+    level visits ends the run as a divergence would: phi of the last
+    completed cycle, not converged."""
+    g = load_golden("bubbles16")
+    A = O.Mat(*g.shape, g.mat())
+    ac = O.AC(A, g["dV"], rule=0)
+    phi, info = ac.cycle(np.zeros(A.n), g["b"], loop="flex", factor=1e-8, max_cyc=20,
+                         mv=(20, 20, 20), resrat=0.1, max_visits=400)
+    assert info["bound"] and not info["converged"]
+    assert np.all(np.isfinite(phi))
