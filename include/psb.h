@@ -16,7 +16,11 @@
  * BW, Src/Global/global_buffer_width.cpp:4); device arrays are always the
  * compact interior.  Every entry point returns a psb_status; none throws.
  * A context is not thread-safe: calls on one context are serialised by the
- * caller.  Every call is synchronous with respect to host buffers.
+ * caller.  Every call returns when its device work is complete (host and
+ * device buffers alike); a context's stream is a blocking one, so device
+ * arrays the caller wrote on the null stream are complete before it reads
+ * them (work on the caller's own non-blocking streams must be synchronized
+ * by the caller).
  */
 #ifndef PSB_H
 #define PSB_H

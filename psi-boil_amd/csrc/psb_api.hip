@@ -1979,6 +1979,8 @@ int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_
     }
     if (p->precision == PSB_MIXED && p->inner_adapt == 2) {
       solve_ru(C, x, b, p, R);  // sets rel_l2 itself
+      // x is final when the call returns (its last update may be in flight)
+      HIPCHK(hipStreamSynchronize(C->stream));
       return R->status;
     } else if (p->precision == PSB_MIXED)
       solve_ir(C, x, b, p, R);
@@ -2095,7 +2097,9 @@ int psb_create(psb_ctx **out, const psb_layout *L) {
   int rc = guarded(C, [&]() {
     C->lay = *L;
     C->g = make_grid(L->nx, L->ny, L->nz);
-    HIPCHK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+    // a blocking stream: work the caller queued on the null stream (a
+    // hipMemset of x, say) completes before the context's kernels read it
+    HIPCHK(hipStreamCreateWithFlags(&C->stream, hipStreamDefault));
     C->st64 = C->base.alloc<DevState>(1);
     C->st32 = C->base.alloc<DevState>(1);
     HIPCHK(hipHostMalloc((void **)&C->hst, sizeof(DevState), hipHostMallocDefault));

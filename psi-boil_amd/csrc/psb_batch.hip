@@ -834,7 +834,7 @@ int psb_batch_create(psb_batch **out, int nsys, const psb_layout *layouts) {
       if (r != PSB_OK) return r;
       B->sys[s]->coarse_min = kCoarseMinBatch;
     }
-    HIPCHK(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&B->stream, hipStreamDefault));  // orders after the null stream
     B->st = B->base.alloc<DevState>(nsys);
     B->flags = B->base.alloc<int>(nsys);
     HIPCHK(hipHostMalloc((void **)&B->hst, sizeof(DevState) * nsys, hipHostMallocDefault));

@@ -85,3 +85,29 @@ def test_struct_size_is_checked(system):
     c.struct_size += 4
     assert psb.lib().psb_ac_vcycle(s._c, psb._dp(x), psb._dp(d["rhs"]), psb.C.byref(c),
                                    psb.C.byref(r)) == -1
+
+
+def test_back_to_back_solves_on_caller_zeroed_arrays(system):
+    """Solves in a row on one context, the caller zeroing x with hipMemset
+    (null stream) between them: every solve starts from x = 0 (res0 = ||b||
+    in the L2 norm) and gives the same result.  A call returns only when its
+    last write of x is done, and the context's stream orders after the null
+    stream, so neither the memset nor the next solve races the previous
+    solve's final update."""
+    d = system
+    s = psb.Solver(N, N, N)
+    s.set_matrix(d)
+    db = psb.DeviceArray.from_host(d["rhs"])
+    dx = psb.DeviceArray(N ** 3)
+    bnorm = np.linalg.norm(d["rhs"])
+    for name, kw in (("mixed kcycle", dict(MIXED, ac_store=1, ac_kcycle=-1)),
+                     ("fp64 kcycle", dict(precision=psb.PSB_FP64, ac_kcycle=-1))):
+        runs = []
+        for _ in range(4):
+            dx.zero()
+            r = s.solve_device(dx.addr, db.addr, **dict(BASE, **kw))
+            runs.append((r, dx.download()))
+        for r, x in runs:
+            assert abs(r["res0"] - bnorm) <= 1e-12 * bnorm, (name, r["res0"], bnorm)
+            assert r["iterations"] == runs[0][0]["iterations"], name
+            assert np.array_equal(x, runs[0][1]), name
