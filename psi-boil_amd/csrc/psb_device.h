@@ -214,37 +214,58 @@ struct Gath {
 // so that (A x)_P = d x_P + sum a (x_P - x_nb): for smooth x no large terms
 // cancel, which keeps FP32 accurate on stiff (stretched, high-contrast)
 // systems.  G.c is then d + sum a, the divisor of the Jacobi sweep.
+// the operand at a j neighbour of the cell vector (a marching or LDS
+// operand overloads it: there the own column lives in registers)
+template <class X>
+__device__ __forceinline__ auto x_nbj(const X &x, int P, int ii, int jj, int kk) {
+  return x.vec(P, ii, jj, kk);
+}
+// ... and at an i neighbour (up: i + 1)
+template <class X>
+__device__ __forceinline__ auto x_nbi(const X &x, bool, int P, int ii, int jj, int kk) {
+  return x.vec(P, ii, jj, kk);
+}
+
+template <typename T, int V>
+__device__ __forceinline__ VecN<T, V> vsel(bool keep, const VecN<T, V> &a) {
+  VecN<T, V> o;
+#pragma unroll
+  for (int q = 0; q < V; q++) o.v[q] = keep ? a.v[q] : T(0);
+  return o;
+}
+
+// Every load is issued whatever the position: a neighbour outside the
+// domain is read at the cell itself (an address in range, coordinates
+// clamped the same way) and its value replaced by 0 afterwards.  Without
+// branches around them, a plane's loads leave together instead of in
+// dependent rounds, each waiting for the one before.
 template <typename T, int V, bool SYM, bool FLUX = false, class X, typename M>
 __device__ __forceinline__ Gath<T, V> gather(const Grid &g, const DMat<M> &A,
                                              const X &X_, int P, int i, int j,
                                              int k) {
   Gath<T, V> G;
-  VecN<T, V> xw = zerov<T, V>(), xe = zerov<T, V>(), xs = zerov<T, V>(),
-             xn = zerov<T, V>();
+  const bool hw = i > 0, he = i < g.nx - 1, hs = j > 0, hn = j < g.ny - 1;
+  const bool hb = k > 0, ht = (k + V) < g.nz;
+  const int Pe = he ? P + g.sI : P, Pn = hn ? P + g.nz : P;
   VecN<T, V> aw = ldm<T, M, V>(A.w + P), as = ldm<T, M, V>(A.s + P),
              ab = ldm<T, M, V>(A.b + P);
-  VecN<T, V> ae = zerov<T, V>(), an = zerov<T, V>(), at;
+  VecN<T, V> ae, an, at;
   G.c = ldm<T, M, V>(A.c + P);
   G.x = X_.vec(P, i, j, k);
-  if (i > 0) xw = X_.vec(P - g.sI, i - 1, j, k);
-  if (i < g.nx - 1) {
-    xe = X_.vec(P + g.sI, i + 1, j, k);
-    ae = SYM ? ldm<T, M, V>(A.w + P + g.sI) : ldm<T, M, V>(A.e + P);
-  } else if (!SYM) {
-    ae = ldm<T, M, V>(A.e + P);
-  }
-  if (j > 0) xs = X_.vec(P - g.nz, i, j - 1, k);
-  if (j < g.ny - 1) {
-    xn = X_.vec(P + g.nz, i, j + 1, k);
-    an = SYM ? ldm<T, M, V>(A.s + P + g.nz) : ldm<T, M, V>(A.n + P);
-  } else if (!SYM) {
-    an = ldm<T, M, V>(A.n + P);
-  }
-  const bool has_t = (k + V) < g.nz;
-  T xb0 = (k > 0) ? X_.at(P - 1, i, j, k - 1) : T(0);
-  T xtV = has_t ? X_.at(P + V, i, j, k + V) : T(0);
-  if (SYM) {
-    T btV = has_t ? (T)A.b[P + V] : T(0);
+  const VecN<T, V> xw = vsel<T, V>(hw, x_nbi(X_, false, hw ? P - g.sI : P, hw ? i - 1 : i, j, k));
+  const VecN<T, V> xe = vsel<T, V>(he, x_nbi(X_, true, Pe, he ? i + 1 : i, j, k));
+  if constexpr (SYM) ae = vsel<T, V>(he, ldm<T, M, V>(A.w + Pe));
+  else ae = ldm<T, M, V>(A.e + P);
+  const VecN<T, V> xs = vsel<T, V>(hs, x_nbj(X_, hs ? P - g.nz : P, i, hs ? j - 1 : j, k));
+  const VecN<T, V> xn = vsel<T, V>(hn, x_nbj(X_, Pn, i, hn ? j + 1 : j, k));
+  if constexpr (SYM) an = vsel<T, V>(hn, ldm<T, M, V>(A.s + Pn));
+  else an = ldm<T, M, V>(A.n + P);
+  const T xb0r = X_.at(hb ? P - 1 : P, i, j, hb ? k - 1 : k);
+  const T xtVr = X_.at(ht ? P + V : P, i, j, ht ? k + V : k);
+  const T xb0 = hb ? xb0r : T(0), xtV = ht ? xtVr : T(0);
+  if constexpr (SYM) {
+    const T btVr = (T)A.b[ht ? P + V : P];
+    const T btV = ht ? btVr : T(0);
 #pragma unroll
     for (int q = 0; q < V; q++) at.v[q] = (q < V - 1) ? ab.v[q + 1] : btV;
   } else {

@@ -53,6 +53,12 @@ struct RingX {
     return lds[(jj - j0 + 1) * nz + k];
   }
 };
+__device__ __forceinline__ VecN<float, 4> x_nbj(const RingX &x, int, int, int jj, int k) {
+  return ldv<float, 4>(x.lds + (jj - x.j0 + 1) * x.nz + k);
+}
+__device__ __forceinline__ VecN<float, 4> x_nbi(const RingX &x, bool up, int, int, int, int) {
+  return up ? x.next : x.prev;
+}
 
 // R: tile rows including the two halo rows (R * nz / 4 <= kBlock * kFuseItems);
 // TI: planes per slab; ntiles: tiles along j.  Block b works on slab

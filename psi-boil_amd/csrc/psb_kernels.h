@@ -254,6 +254,21 @@ __device__ __forceinline__ void rho_acc(double (&acc)[kRhoParts], T r, T z, cons
   }
 }
 
+// the same with q already loaded (hasq: the flexible beta's terms)
+template <typename T>
+__device__ __forceinline__ void rho_accq(double (&acc)[kRhoParts], T r, T z, bool hasq, T q,
+                                         bool act) {
+  if (!act) return;
+  acc[0] += (double)r * (double)z;
+  acc[1] += (double)z;
+  acc[2] += (double)r;
+  acc[3] += 1.0;
+  if (hasq) {
+    acc[4] += (double)z * (double)q;
+    acc[5] += (double)q;
+  }
+}
+
 template <typename T, int V, bool SYM, bool PROL, bool RHO, typename M = T, bool FLUX = false>
 __device__ __forceinline__ void jacobi_body(Span sp, const Grid &g, const DMat<M> &A,
                                             const T *r, const T *e, T *eout,

@@ -46,6 +46,14 @@ struct LdsX {
     return rows[(jj - j0 + 1) * nz + kk];
   }
 };
+template <typename T, int V>
+__device__ __forceinline__ VecN<T, V> x_nbj(const LdsX<T, V> &x, int, int, int jj, int kk) {
+  return *reinterpret_cast<const VecN<T, V> *>(x.rows + (jj - x.j0 + 1) * x.nz + kk);
+}
+template <typename T, int V>
+__device__ __forceinline__ VecN<T, V> x_nbi(const LdsX<T, V> &x, bool up, int, int, int, int) {
+  return up ? x.next : x.prev;
+}
 
 // one (slab, tile) item: stage, gather and epi(P, i, j, k, G) per plane.
 // Every thread of the block runs the same number of planes (the LDS

@@ -30,6 +30,16 @@ struct MarchX {
     return base.at(Q, ii, jj, kk);
   }
 };
+template <typename T, int V, class B>
+__device__ __forceinline__ VecN<T, V> x_nbj(const MarchX<T, V, B> &x, int P, int ii, int jj,
+                                            int kk) {
+  return x.base.vec(P, ii, jj, kk);
+}
+template <typename T, int V, class B>
+__device__ __forceinline__ VecN<T, V> x_nbi(const MarchX<T, V, B> &x, bool up, int, int, int,
+                                            int) {
+  return up ? x.next : x.prev;
+}
 
 // launch geometry: tiles of kBlock vectors per plane, slabs of TI planes,
 // about kMaxBlocks blocks in all
@@ -123,7 +133,13 @@ __device__ __forceinline__ void jacobi_march_item(const Grid &g, const DMat<M> &
   VecN<T, V> prev = i0 > 0 ? opv(P - g.sI, i0 - 1) : zerov<T, V>();
   VecN<T, V> cur = opv(P, i0);
   for (int i = i0; i < i1; i++, P += g.sI) {
-    const VecN<T, V> next = (i + 1 < g.nx) ? opv(P + g.sI, i + 1) : zerov<T, V>();
+    const bool up = i + 1 < g.nx;
+    const VecN<T, V> next = vsel<T, V>(up, opv(up ? P + g.sI : P, up ? i + 1 : i));
+    // the plane's other loads issued with the gather's (without q, r is
+    // read twice rather than under a branch)
+    const VecN<T, V> R = ldv<T, V>(r + P);
+    VecN<T, V> Qf;
+    if constexpr (RHO) Qf = ldv<T, V>((qf ? qf : r) + P);
     Gath<T, V> G;
     if constexpr (PROL) {
       MarchX<T, V, ProlX<T, V>> X{pr, prev, cur, next, i, j};
@@ -132,7 +148,6 @@ __device__ __forceinline__ void jacobi_march_item(const Grid &g, const DMat<M> &
       MarchX<T, V, ArrX<T, V>> X{ArrX<T, V>{e}, prev, cur, next, i, j};
       G = gather<T, V, true, FLUX>(g, A, X, P, i, j, k);
     }
-    const VecN<T, V> R = ldv<T, V>(r + P);
     const VecN<T, V> t = resid<T, V>(R, G);
     VecN<T, V> E;
 #pragma unroll
@@ -140,7 +155,7 @@ __device__ __forceinline__ void jacobi_march_item(const Grid &g, const DMat<M> &
       E.v[u] = G.x.v[u] + omega * t.v[u] / G.c.v[u];
       if (RHO) {
         E.v[u] = zscale * E.v[u];
-        rho_acc<T>(acc, R.v[u], E.v[u], qf, P + u, !af || af[P + u]);
+        rho_accq<T>(acc, R.v[u], E.v[u], qf != nullptr, Qf.v[u], !af || af[P + u]);
       }
     }
     stv<T, V>(eout + P, E);
@@ -170,7 +185,7 @@ __device__ __forceinline__ void jacobi_march_body(Span sp, const Grid &g, const 
 // of level 0 (output times zscale, partials r.z, sum z, sum r, active
 // cells per block).  The same arithmetic as k_jacobi.
 template <typename T, int V, bool PROL, bool RHO, typename M, bool FLUX>
-__global__ __launch_bounds__(kBlock) void k_jacobi_march(Grid g, DMat<M> A, const T *r,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_jacobi_march(Grid g, DMat<M> A, const T *r,
                                                          const T *e, T *eout, ProlX<T, V> pr,
                                                          T omega, const uint8_t *af,
                                                          const DevState *st, Work w, T zscale,
