@@ -234,43 +234,92 @@ __device__ __forceinline__ VecN<T, V> vsel(bool keep, const VecN<T, V> &a) {
   return o;
 }
 
-// Every load is issued whatever the position: a neighbour outside the
-// domain is read at the cell itself (an address in range, coordinates
-// clamped the same way) and its value replaced by 0 afterwards.  Without
-// branches around them, a plane's loads leave together instead of in
-// dependent rounds, each waiting for the one before.
+// the active flags of the N cells from Q as one word, read without a
+// branch: with no flag array (every cell active) the word is read from
+// `any` (an array of the same cells) and replaced by all-active
+template <int N, typename A>
+__device__ __forceinline__ uint32_t flag_word(const uint8_t *af, int Q, const A *any) {
+  static_assert(N == 1 || N == 2 || N == 4, "a vector's flags are read as one word");
+  const uint8_t *p = af ? af + Q : reinterpret_cast<const uint8_t *>(any + Q);
+  uint32_t w;
+  if constexpr (N == 4) w = *reinterpret_cast<const uint32_t *>(p);
+  else if constexpr (N == 2) w = *reinterpret_cast<const uint16_t *>(p);
+  else w = *p;
+  return af ? w : 0xffffffffu;
+}
+__device__ __forceinline__ bool flag_on(uint32_t w, int u) { return ((w >> (8 * u)) & 0xffu) != 0; }
+
+// A cell vector's coefficients as stored (M), loaded apart from the
+// arithmetic so that a kernel can issue them early (before a barrier, or a
+// plane ahead).  SYM: e = A.w at i + 1, n = A.s at j + 1, bt = A.b at k + V;
+// otherwise e, n, t are the arrays themselves.  Every load is issued
+// whatever the position: a neighbour outside the domain is read at the cell
+// itself (an address in range) and its value replaced by 0 when the
+// coefficients are used.  Without branches around them, a plane's loads
+// leave together instead of in dependent rounds, each waiting for the one
+// before.
+template <typename M, int V, bool SYM>
+struct Coef {
+  VecN<M, V> c, w, s, b, e, n, t;
+  M bt;
+};
+template <typename M, int V, bool SYM>
+__device__ __forceinline__ Coef<M, V, SYM> load_coef(const Grid &g, const DMat<M> &A, int P,
+                                                     int i, int j, int k) {
+  Coef<M, V, SYM> C;
+  C.c = ldv<M, V>(A.c + P);
+  C.w = ldv<M, V>(A.w + P);
+  C.s = ldv<M, V>(A.s + P);
+  C.b = ldv<M, V>(A.b + P);
+  if constexpr (SYM) {
+    C.e = ldv<M, V>(A.w + (i < g.nx - 1 ? P + g.sI : P));
+    C.n = ldv<M, V>(A.s + (j < g.ny - 1 ? P + g.nz : P));
+    C.bt = A.b[(k + V) < g.nz ? P + V : P];
+  } else {
+    C.e = ldv<M, V>(A.e + P);
+    C.n = ldv<M, V>(A.n + P);
+    C.t = ldv<M, V>(A.t + P);
+  }
+  return C;
+}
+
+template <typename T, typename M, int V>
+__device__ __forceinline__ VecN<T, V> cvtv(const VecN<M, V> &m) {
+  VecN<T, V> r;
+#pragma unroll
+  for (int q = 0; q < V; q++) r.v[q] = (T)m.v[q];
+  return r;
+}
+
+// the gather with the coefficients already loaded
 template <typename T, int V, bool SYM, bool FLUX = false, class X, typename M>
-__device__ __forceinline__ Gath<T, V> gather(const Grid &g, const DMat<M> &A,
-                                             const X &X_, int P, int i, int j,
-                                             int k) {
+__device__ __forceinline__ Gath<T, V> gather_c(const Grid &g, const Coef<M, V, SYM> &C,
+                                               const X &X_, int P, int i, int j, int k) {
   Gath<T, V> G;
   const bool hw = i > 0, he = i < g.nx - 1, hs = j > 0, hn = j < g.ny - 1;
   const bool hb = k > 0, ht = (k + V) < g.nz;
-  const int Pe = he ? P + g.sI : P, Pn = hn ? P + g.nz : P;
-  VecN<T, V> aw = ldm<T, M, V>(A.w + P), as = ldm<T, M, V>(A.s + P),
-             ab = ldm<T, M, V>(A.b + P);
+  const VecN<T, V> aw = cvtv<T, M, V>(C.w), as = cvtv<T, M, V>(C.s), ab = cvtv<T, M, V>(C.b);
   VecN<T, V> ae, an, at;
-  G.c = ldm<T, M, V>(A.c + P);
-  G.x = X_.vec(P, i, j, k);
-  const VecN<T, V> xw = vsel<T, V>(hw, x_nbi(X_, false, hw ? P - g.sI : P, hw ? i - 1 : i, j, k));
-  const VecN<T, V> xe = vsel<T, V>(he, x_nbi(X_, true, Pe, he ? i + 1 : i, j, k));
-  if constexpr (SYM) ae = vsel<T, V>(he, ldm<T, M, V>(A.w + Pe));
-  else ae = ldm<T, M, V>(A.e + P);
-  const VecN<T, V> xs = vsel<T, V>(hs, x_nbj(X_, hs ? P - g.nz : P, i, hs ? j - 1 : j, k));
-  const VecN<T, V> xn = vsel<T, V>(hn, x_nbj(X_, Pn, i, hn ? j + 1 : j, k));
-  if constexpr (SYM) an = vsel<T, V>(hn, ldm<T, M, V>(A.s + Pn));
-  else an = ldm<T, M, V>(A.n + P);
-  const T xb0r = X_.at(hb ? P - 1 : P, i, j, hb ? k - 1 : k);
-  const T xtVr = X_.at(ht ? P + V : P, i, j, ht ? k + V : k);
-  const T xb0 = hb ? xb0r : T(0), xtV = ht ? xtVr : T(0);
+  G.c = cvtv<T, M, V>(C.c);
   if constexpr (SYM) {
-    const T btVr = (T)A.b[ht ? P + V : P];
-    const T btV = ht ? btVr : T(0);
+    ae = vsel<T, V>(he, cvtv<T, M, V>(C.e));
+    an = vsel<T, V>(hn, cvtv<T, M, V>(C.n));
+    const T btV = ht ? (T)C.bt : T(0);
 #pragma unroll
     for (int q = 0; q < V; q++) at.v[q] = (q < V - 1) ? ab.v[q + 1] : btV;
   } else {
-    at = ldm<T, M, V>(A.t + P);
+    ae = cvtv<T, M, V>(C.e);
+    an = cvtv<T, M, V>(C.n);
+    at = cvtv<T, M, V>(C.t);
   }
+  G.x = X_.vec(P, i, j, k);
+  const VecN<T, V> xw = vsel<T, V>(hw, x_nbi(X_, false, hw ? P - g.sI : P, hw ? i - 1 : i, j, k));
+  const VecN<T, V> xe = vsel<T, V>(he, x_nbi(X_, true, he ? P + g.sI : P, he ? i + 1 : i, j, k));
+  const VecN<T, V> xs = vsel<T, V>(hs, x_nbj(X_, hs ? P - g.nz : P, i, hs ? j - 1 : j, k));
+  const VecN<T, V> xn = vsel<T, V>(hn, x_nbj(X_, hn ? P + g.nz : P, i, hn ? j + 1 : j, k));
+  const T xb0r = X_.at(hb ? P - 1 : P, i, j, hb ? k - 1 : k);
+  const T xtVr = X_.at(ht ? P + V : P, i, j, ht ? k + V : k);
+  const T xb0 = hb ? xb0r : T(0), xtV = ht ? xtVr : T(0);
 #pragma unroll
   for (int q = 0; q < V; q++) {
     T xb = (q == 0) ? xb0 : G.x.v[q - 1];
@@ -291,6 +340,13 @@ __device__ __forceinline__ Gath<T, V> gather(const Grid &g, const DMat<M> &A,
     }
   }
   return G;
+}
+
+template <typename T, int V, bool SYM, bool FLUX = false, class X, typename M>
+__device__ __forceinline__ Gath<T, V> gather(const Grid &g, const DMat<M> &A,
+                                             const X &X_, int P, int i, int j,
+                                             int k) {
+  return gather_c<T, V, SYM, FLUX>(g, load_coef<M, V, SYM>(g, A, P, i, j, k), X_, P, i, j, k);
 }
 
 template <typename T, int V>
@@ -361,14 +417,16 @@ struct ZPX {
   __device__ __forceinline__ T one(T zz, T pp, bool act) const {
     return (act ? zz - zmean : zz) + beta * pp;
   }
+  // the flags read without a branch (flag_word)
   __device__ __forceinline__ VecN<T, V> vec(int P, int, int, int) const {
     VecN<T, V> Z = ldv<T, V>(z + P), Q = ldv<T, V>(p + P), o;
+    const uint32_t act = flag_word<V>(af, P, z);
 #pragma unroll
-    for (int q = 0; q < V; q++) o.v[q] = one(Z.v[q], Q.v[q], !af || af[P + q]);
+    for (int q = 0; q < V; q++) o.v[q] = one(Z.v[q], Q.v[q], flag_on(act, q));
     return o;
   }
   __device__ __forceinline__ T at(int Q, int, int, int) const {
-    return one(z[Q], p[Q], !af || af[Q]);
+    return one(z[Q], p[Q], flag_on(flag_word<1>(af, Q, z), 0));
   }
 };
 
@@ -385,40 +443,33 @@ struct ProlX {
   __device__ __forceinline__ int parent(int i, int j, int k) const {
     return ((i >> sI) * nYH + (j >> sJ)) * nZH + (k >> sK);
   }
+  // the loads are issued unconditionally (no branch for a missing flag
+  // array, flag_word, or for the coarsening along k)
   __device__ __forceinline__ T at(int Q, int i, int j, int k) const {
-    if (af && !af[Q]) return e[Q];
-    return e[Q] + cfac * eH[parent(i, j, k)];
+    const T ev = e[Q], hv = eH[parent(i, j, k)];
+    return flag_on(flag_word<1>(af, Q, e), 0) ? ev + cfac * hv : ev;
   }
-  // one vector load of e, one of the V (sK = 0) or V / 2 (sK = 1) parents
+  // one vector load of e, two of the V (sK = 0) or V / 2 (sK = 1) parents
   // and one of the V flags: k is a multiple of V and so are nz and the
-  // coarse row length, so every load is aligned to its size
+  // coarse row length
   static_assert(V == 1 || V == 2 || V == 4, "flags of a vector are read as one word");
   __device__ __forceinline__ VecN<T, V> vec(int P, int i, int j, int k) const {
     VecN<T, V> E = ldv<T, V>(e + P), o;
     const int base = ((i >> sI) * nYH + (j >> sJ)) * nZH;
     T h[V];
     if constexpr (V >= 2) {
-      if (sK) {
-        const VecN<T, V / 2> H = ldv<T, V / 2>(eH + base + (k >> 1));
+      // halves at k / 2 (twice) for sK = 1, at k and k + V / 2 for sK = 0
+      const VecN<T, V / 2> H0 = ldv<T, V / 2>(eH + base + (k >> sK));
+      const VecN<T, V / 2> H1 = ldv<T, V / 2>(eH + base + (sK ? (k >> 1) : k + V / 2));
 #pragma unroll
-        for (int q = 0; q < V; q++) h[q] = H.v[q >> 1];
-      } else {
-        const VecN<T, V> H = ldv<T, V>(eH + base + k);
-#pragma unroll
-        for (int q = 0; q < V; q++) h[q] = H.v[q];
-      }
+      for (int q = 0; q < V; q++)
+        h[q] = sK ? H0.v[q >> 1] : (q < V / 2 ? H0.v[q] : H1.v[q - V / 2]);
     } else {
       h[0] = eH[base + (k >> sK)];
     }
-    uint32_t act = 0xffffffffu;
-    if (af) {
-      if constexpr (V == 4) act = *reinterpret_cast<const uint32_t *>(af + P);
-      else if constexpr (V == 2) act = *reinterpret_cast<const uint16_t *>(af + P);
-      else act = af[P];
-    }
+    const uint32_t act = flag_word<V>(af, P, e);
 #pragma unroll
-    for (int q = 0; q < V; q++)
-      o.v[q] = ((act >> (8 * q)) & 0xffu) ? E.v[q] + cfac * h[q] : E.v[q];
+    for (int q = 0; q < V; q++) o.v[q] = flag_on(act, q) ? E.v[q] + cfac * h[q] : E.v[q];
     return o;
   }
 };

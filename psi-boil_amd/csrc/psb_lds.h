@@ -55,13 +55,20 @@ __device__ __forceinline__ VecN<T, V> x_nbi(const LdsX<T, V> &x, bool up, int, i
   return up ? x.next : x.prev;
 }
 
-// one (slab, tile) item: stage, gather and epi(P, i, j, k, G) per plane.
-// Every thread of the block runs the same number of planes (the LDS
-// barriers); threads past the grid's last row only stage nothing.
-template <typename T, int V, bool SYM, bool FLUX, typename M, class B, class Epi>
+// one (slab, tile) item: stage, gather and epi(P, i, j, k, G, pre(P)) per
+// plane.  Every thread of the block runs the same number of planes (the LDS
+// barriers); threads past the grid's last row stage nothing.  A plane's
+// global loads are issued without branches (lanes with nothing to load
+// read row 0 and discard it): the next plane's operand before the barrier,
+// the coefficients and what pre(P) reads for the epilogue before it too
+// (PRE) or after it.  Measured per kernel at 256^3: the first post-sweep
+// (its operand formed from two arrays) gains from PRE, the CG step and the
+// FP64 sweep do not.
+template <bool PRE, typename T, int V, bool SYM, bool FLUX, typename M, class B, class Pre,
+          class Epi>
 __device__ __forceinline__ void lds_march_item(const Grid &g, const DMat<M> &A, const B &base,
                                                int TI, int slab, int tile, T (*buf)[2 * kBlock * V],
-                                               Epi epi) {
+                                               Pre pre, Epi epi) {
   const int nv = g.nz / V, R = kBlock / nv;
   const int t = (int)threadIdx.x;
   const int j0 = tile * R;
@@ -72,33 +79,44 @@ __device__ __forceinline__ void lds_march_item(const Grid &g, const DMat<M> &A, 
   const int hr = t < nv ? -1 : R;
   const int hj = j0 + hr, hk = (t < nv ? t : t - nv) * V;
   const bool hact = t < 2 * nv && hj >= 0 && hj < g.ny;
+  const int ja = act ? j : 0, hja = hact ? hj : 0, hka = hact ? hk : 0;
   const int i0 = slab * TI, i1 = min(i0 + TI, g.nx);
-  int P = (i0 * g.ny + j) * g.nz + k;
-  int HP = (i0 * g.ny + hj) * g.nz + hk;
-  VecN<T, V> prev = zerov<T, V>(), cur = zerov<T, V>(), hcur = zerov<T, V>();
-  if (act) {
-    if (i0 > 0) prev = base.vec(P - g.sI, i0 - 1, j, k);
-    cur = base.vec(P, i0, j, k);
-  }
-  if (hact) hcur = base.vec(HP, i0, hj, hk);
+  int P = (i0 * g.ny + ja) * g.nz + k;
+  int HP = (i0 * g.ny + hja) * g.nz + hka;
+  VecN<T, V> prev = zerov<T, V>();
+  if (i0 > 0) prev = base.vec(P - g.sI, i0 - 1, ja, k);
+  VecN<T, V> cur = base.vec(P, i0, ja, k), hcur = base.vec(HP, i0, hja, hka);
   for (int i = i0; i < i1; i++, P += g.sI, HP += g.sI) {
-    VecN<T, V> next = zerov<T, V>(), hnext = zerov<T, V>();
-    if (act && i + 1 < g.nx) next = base.vec(P + g.sI, i + 1, j, k);
-    if (hact && i + 1 < g.nx) hnext = base.vec(HP + g.sI, i + 1, hj, hk);
+    const bool up = i + 1 < g.nx;
+    const int iu = up ? i + 1 : i;
+    const VecN<T, V> next = vsel<T, V>(up, base.vec(up ? P + g.sI : P, iu, ja, k));
+    const VecN<T, V> hnext = base.vec(up ? HP + g.sI : HP, iu, hja, hka);
+    Coef<M, V, SYM> cf;
+    decltype(pre(P)) pv;
+    if constexpr (PRE) {
+      cf = load_coef<M, V, SYM>(g, A, P, i, ja, k);
+      pv = pre(P);
+    }
     T *rows = buf[i & 1];
     if (act) *reinterpret_cast<VecN<T, V> *>(rows + (r + 1) * g.nz + k) = cur;
     if (hact) *reinterpret_cast<VecN<T, V> *>(rows + (hr + 1) * g.nz + hk) = hcur;
     __syncthreads();
+    if constexpr (!PRE) {
+      cf = load_coef<M, V, SYM>(g, A, P, i, ja, k);
+      pv = pre(P);
+    }
     if (act) {
       const LdsX<T, V> X{rows, prev, cur, next, i, j, j0, g.nz};
-      const Gath<T, V> G = gather<T, V, SYM, FLUX>(g, A, X, P, i, j, k);
-      epi(P, i, j, k, G);
+      const Gath<T, V> G = gather_c<T, V, SYM, FLUX>(g, cf, X, P, i, j, k);
+      epi(P, i, j, k, G, pv);
     }
     prev = cur;
     cur = next;
     hcur = hnext;
   }
 }
+
+struct NoPre {};
 
 // damped Jacobi sweep e_out = e + omega (r - A e) / c (the operand e, or
 // with PROL e + cfac P eH); RHO: the last post-sweep of level 0 (output
@@ -114,24 +132,34 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_lds(Grid g, DMat<M> A, const 
   double acc[kRhoParts] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int wi = xcd_block();
   const int slab = wi / ntiles, tile = wi - slab * ntiles;
-  auto epi = [&](int P, int, int, int, const Gath<T, V> &G) {
-    const VecN<T, V> R = ldv<T, V>(r + P);
-    const VecN<T, V> t = resid<T, V>(R, G);
+  struct RQ {
+    VecN<T, V> R, Q;
+  };
+  // r, and q for the rho partials (without q, r read twice)
+  auto pre = [&](int P) {
+    RQ o;
+    o.R = ldv<T, V>(r + P);
+    if constexpr (RHO) o.Q = ldv<T, V>((qf ? qf : r) + P);
+    return o;
+  };
+  auto epi = [&](int P, int, int, int, const Gath<T, V> &G, const RQ &pv) {
+    const VecN<T, V> t = resid<T, V>(pv.R, G);
     VecN<T, V> E;
 #pragma unroll
     for (int u = 0; u < V; u++) {
       E.v[u] = G.x.v[u] + omega * t.v[u] / G.c.v[u];
       if (RHO) {
         E.v[u] = zscale * E.v[u];
-        rho_acc<T>(acc, R.v[u], E.v[u], qf, P + u, !af || af[P + u]);
+        rho_accq<T>(acc, pv.R.v[u], E.v[u], qf != nullptr, pv.Q.v[u], !af || af[P + u]);
       }
     }
     stv<T, V>(eout + P, E);
   };
   if constexpr (PROL)
-    lds_march_item<T, V, true, FLUX, M>(g, A, pr, TI, slab, tile, buf, epi);
+    lds_march_item<true, T, V, true, FLUX, M>(g, A, pr, TI, slab, tile, buf, pre, epi);
   else
-    lds_march_item<T, V, true, FLUX, M>(g, A, ArrX<T, V>{e}, TI, slab, tile, buf, epi);
+    lds_march_item<false, T, V, true, FLUX, M>(g, A, ArrX<T, V>{e}, TI, slab, tile, buf, pre,
+                                               epi);
   if (RHO) block_partial<kRhoParts>(acc, w.partial);
 }
 
@@ -148,7 +176,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_pq_lds(Grid g, DMat<M> A, const T
   double acc[1] = {0.0};
   const int wi = xcd_block();
   const int slab = wi / ntiles, tile = wi - slab * ntiles;
-  auto epi = [&](int P, int, int, int, const Gath<T, V> &G) {
+  auto pre = [](int) { return NoPre{}; };
+  auto epi = [&](int P, int, int, int, const Gath<T, V> &G, NoPre) {
     VecN<T, V> Q = apply_A<T, V>(G);
     if constexpr (!std::is_same<T, M>::value) {
 #pragma unroll
@@ -161,10 +190,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_pq_lds(Grid g, DMat<M> A, const T
   };
   if constexpr (DIAG) {
     const DiagPX<T, V> base{r_or_z, (const T *)A.c, pold, (T)st->beta};
-    lds_march_item<T, V, SYM, FLUX, M>(g, A, base, TI, slab, tile, buf, epi);
+    lds_march_item<false, T, V, SYM, FLUX, M>(g, A, base, TI, slab, tile, buf, pre, epi);
   } else {
     const ZPX<T, V> base{r_or_z, pold, af, (T)st->beta, (T)st->zmean};
-    lds_march_item<T, V, SYM, FLUX, M>(g, A, base, TI, slab, tile, buf, epi);
+    lds_march_item<false, T, V, SYM, FLUX, M>(g, A, base, TI, slab, tile, buf, pre, epi);
   }
   block_partial<1>(acc, w.partial);
 }
@@ -185,10 +214,15 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid_lds(Grid g, DMat<double> A,
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const int wi = xcd_block();
   const int slab = wi / ntiles, tile = wi - slab * ntiles;
-  auto epi = [&](int P, int i, int j, int k, const Gath<double, V> &G) {
-    const VecN<double, V> Bv = ldv<double, V>(b + P);
+  struct BC {
+    VecN<double, V> B;
+    VecN<float, V> C;
+  };
+  auto pre = [&](int P) { return BC{ldv<double, V>(b + P), ldv<float, V>(c32 + P)}; };
+  auto epi = [&](int P, int i, int j, int k, const Gath<double, V> &G, const BC &pv) {
+    const VecN<double, V> &Bv = pv.B;
     const VecN<double, V> R = resid<double, V>(Bv, G);
-    const VecN<float, V> Cf = ldv<float, V>(c32 + P);
+    const VecN<float, V> &Cf = pv.C;
     VecN<float, V> Rf, E;
 #pragma unroll
     for (int u = 0; u < V; u++) {
@@ -205,8 +239,8 @@ __global__ __launch_bounds__(kBlock) void k_ru_resid_lds(Grid g, DMat<double> A,
     stv<float, V>(r32 + P, Rf);
     stv<float, V>(e1 + P, E);
   };
-  lds_march_item<double, V, SYM, false, double>(g, A, ArrX<double, V>{x}, TI, slab, tile, buf,
-                                                epi);
+  lds_march_item<true, double, V, SYM, false, double>(g, A, ArrX<double, V>{x}, TI, slab, tile,
+                                                      buf, pre, epi);
   block_partial<4>(acc, w.partial);
 }
 

@@ -96,18 +96,34 @@ __global__ __launch_bounds__(kBlock) void k_spmv_march(Grid g, DMat<T> A, const 
                                                        int TI, int ntiles) {
   PSB_MARCH_BEGIN(g, V, TI, ntiles)
   if (!act_) return;
+  // software-pipelined as the Jacobi sweep: plane i + 1's coefficients and
+  // plane i + 2's operand are loaded while plane i is computed
   int P = (i0_ * g.ny + j) * g.nz + k;
   VecN<T, V> prev = i0_ > 0 ? ldv<T, V>(x + P - g.sI) : zerov<T, V>();
   VecN<T, V> cur = ldv<T, V>(x + P);
+  const bool up0 = i0_ + 1 < g.nx;
+  VecN<T, V> next = vsel<T, V>(up0, ldv<T, V>(x + (up0 ? P + g.sI : P)));
+  Coef<T, V, SYM> cf = load_coef<T, V, SYM>(g, A, P, i0_, j, k);
   for (int i = i0_; i < i1_; i++, P += g.sI) {
-    const VecN<T, V> next = (i + 1 < g.nx) ? ldv<T, V>(x + P + g.sI) : zerov<T, V>();
+    const bool in1 = i + 1 < i1_, up2 = in1 && i + 2 < g.nx;
+    const VecN<T, V> nn = vsel<T, V>(up2, ldv<T, V>(x + (up2 ? P + 2 * g.sI : P)));
+    const Coef<T, V, SYM> cfn =
+        load_coef<T, V, SYM>(g, A, in1 ? P + g.sI : P, in1 ? i + 1 : i, j, k);
     MarchX<T, V, ArrX<T, V>> X{ArrX<T, V>{x}, prev, cur, next, i, j};
-    Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+    Gath<T, V> G = gather_c<T, V, SYM>(g, cf, X, P, i, j, k);
     stv<T, V>(y + P, apply_A<T, V>(G));
     prev = cur;
     cur = next;
+    next = nn;
+    cf = cfn;
   }
 }
+
+// coefficients loaded a plane ahead, except in the last post-sweep (RHO:
+// measured the same either way at 256^3, and without it the kernel holds
+// fewer registers)
+template <bool RHO>
+constexpr bool kPipe = !RHO;
 
 // damped Jacobi sweep e_out = e + omega (r - A e) / c, the operand e or,
 // with PROL, e + cfac P eH (the first post-sweep); RHO: the last post-sweep
@@ -129,24 +145,33 @@ __device__ __forceinline__ void jacobi_march_item(const Grid &g, const DMat<M> &
     if constexpr (PROL) return pr.vec(Q, ii, j, k);
     else return ldv<T, V>(e + Q);
   };
+  // software-pipelined: plane i + 1's coefficients and plane i + 2's
+  // operand are loaded while plane i is computed (past the slab's end the
+  // loads read plane i again and are discarded)
   int P = (i0 * g.ny + j) * g.nz + k;
   VecN<T, V> prev = i0 > 0 ? opv(P - g.sI, i0 - 1) : zerov<T, V>();
   VecN<T, V> cur = opv(P, i0);
+  const bool up0 = i0 + 1 < g.nx;
+  VecN<T, V> next = vsel<T, V>(up0, opv(up0 ? P + g.sI : P, up0 ? i0 + 1 : i0));
+  Coef<M, V, true> cf;
+  if constexpr (kPipe<RHO>) cf = load_coef<M, V, true>(g, A, P, i0, j, k);
   for (int i = i0; i < i1; i++, P += g.sI) {
-    const bool up = i + 1 < g.nx;
-    const VecN<T, V> next = vsel<T, V>(up, opv(up ? P + g.sI : P, up ? i + 1 : i));
-    // the plane's other loads issued with the gather's (without q, r is
-    // read twice rather than under a branch)
+    const bool in1 = i + 1 < i1, up2 = in1 && i + 2 < g.nx;
+    const VecN<T, V> nn = vsel<T, V>(up2, opv(up2 ? P + 2 * g.sI : P, up2 ? i + 2 : i));
+    Coef<M, V, true> cfn;
+    if constexpr (kPipe<RHO>) cfn = load_coef<M, V, true>(g, A, in1 ? P + g.sI : P, in1 ? i + 1 : i, j, k);
+    else cf = load_coef<M, V, true>(g, A, P, i, j, k);
+    // (without q, r is read twice rather than under a branch)
     const VecN<T, V> R = ldv<T, V>(r + P);
     VecN<T, V> Qf;
     if constexpr (RHO) Qf = ldv<T, V>((qf ? qf : r) + P);
     Gath<T, V> G;
     if constexpr (PROL) {
       MarchX<T, V, ProlX<T, V>> X{pr, prev, cur, next, i, j};
-      G = gather<T, V, true, FLUX>(g, A, X, P, i, j, k);
+      G = gather_c<T, V, true, FLUX>(g, cf, X, P, i, j, k);
     } else {
       MarchX<T, V, ArrX<T, V>> X{ArrX<T, V>{e}, prev, cur, next, i, j};
-      G = gather<T, V, true, FLUX>(g, A, X, P, i, j, k);
+      G = gather_c<T, V, true, FLUX>(g, cf, X, P, i, j, k);
     }
     const VecN<T, V> t = resid<T, V>(R, G);
     VecN<T, V> E;
@@ -161,6 +186,8 @@ __device__ __forceinline__ void jacobi_march_item(const Grid &g, const DMat<M> &
     stv<T, V>(eout + P, E);
     prev = cur;
     cur = next;
+    next = nn;
+    if constexpr (kPipe<RHO>) cf = cfn;
   }
 }
 
@@ -211,7 +238,8 @@ __device__ __forceinline__ void cg_pq_march_item(const Grid &g, const DMat<M> &A
   VecN<T, V> prev = i0 > 0 ? base.vec(P - g.sI, i0 - 1, j, k) : zerov<T, V>();
   VecN<T, V> cur = base.vec(P, i0, j, k);
   for (int i = i0; i < i1; i++, P += g.sI) {
-    const VecN<T, V> next = (i + 1 < g.nx) ? base.vec(P + g.sI, i + 1, j, k) : zerov<T, V>();
+    const bool up = i + 1 < g.nx;
+    const VecN<T, V> next = vsel<T, V>(up, base.vec(up ? P + g.sI : P, up ? i + 1 : i, j, k));
     MarchX<T, V, B> X{base, prev, cur, next, i, j};
     Gath<T, V> G = gather<T, V, SYM, FLUX>(g, A, X, P, i, j, k);
     VecN<T, V> Q = apply_A<T, V>(G);
@@ -301,13 +329,14 @@ __global__ __launch_bounds__(kBlock) void k_restrict_march(Grid g, DMat<M> A, co
     cb[bb] = ldv<T, V>(e + P);
   }
   for (int I = I0; I < I1; I++) {
-    VecN<T, V> cc[CJ], cd[CJ];  // fine planes 2I+1 and 2I+2
+    VecN<T, V> cc[CJ], cd[CJ];  // fine planes 2I+1 and 2I+2 (past the last: zero)
+    const bool up = 2 * I + 2 < g.nx;
 #pragma unroll
     for (int bb = 0; bb < CJ; bb++) {
       const int j = J * CJ + bb;
       const int P = ((2 * I + 1) * g.ny + j) * g.nz + k;
       cc[bb] = ldv<T, V>(e + P);
-      cd[bb] = (2 * I + 2 < g.nx) ? ldv<T, V>(e + P + g.sI) : zerov<T, V>();
+      cd[bb] = vsel<T, V>(up, ldv<T, V>(e + (up ? P + g.sI : P)));
     }
     T s[NC];
 #pragma unroll
