@@ -571,14 +571,14 @@ def main():
     log(f"[bench] in-solve fp64 {prof_64}")
     # the dominant kernel: the fine-level kind with the largest time per solve
     kern = max(prof_mx["kernels"].items(), key=lambda kv: kv[1]["total_ms"])
-    names = {"cg_pq": "k_cg_pq_march<float,4,SYM,-,flux> (p = z + beta p, q = A p, p.q; "
-                      "FP32 matrix)",
-             "sweep_prol": "k_jacobi<float,4,...,_Float16,flux> (first post-sweep with the "
-                           "prolongation fused; FP16 matrix copy)",
-             "sweep": "k_jacobi_march<float,4,...,_Float16,flux> (damped Jacobi sweep, FP16 "
-                      "matrix copy)",
-             "sweep_last": "k_jacobi_march<float,4,RHO,...> (last post-sweep with the rho "
-                           "partials)",
+    names = {"cg_pq": "k_cg_pq_lds<float,4,SYM,-,flux> (p = z + beta p, q = A p, p.q; FP32 "
+                      "matrix, operand plane staged in LDS)",
+             "sweep_prol": "k_jacobi_lds<float,4,PROL,-,_Float16,flux> (first post-sweep with "
+                           "the prolongation fused; FP16 matrix copy, LDS-staged)",
+             "sweep": "k_jacobi_march<float,4,-,-,_Float16,flux> (damped Jacobi sweep, FP16 "
+                      "matrix copy, marching)",
+             "sweep_last": "k_jacobi_march<float,4,-,RHO,_Float16,flux> (last post-sweep with "
+                           "the rho partials, marching)",
              "restrict": "k_restrict_march<float,4,...> (residual block sums, coarse first "
                          "sweep fused)",
              "cg_xr": "k_cg_xr<float,4> (x, r update, norms, next first sweep)"}
