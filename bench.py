@@ -60,7 +60,7 @@ def timed_solves(s, dx, db, steps, warmup, **stop):
     return (t1 - t0) / steps, res
 
 
-def pressure_step(n, steps, kcycle=0):
+def pressure_step(n, steps, kcycle=0, cfac=2.0):
     """One time step's pressure phase on the device, as a two-phase code runs
     it every step: Pressure::discretize from the density, update_rhs from the
     face velocities, the mixed solve (hierarchy rebuilt for the new matrix)
@@ -88,7 +88,7 @@ def pressure_step(n, steps, kcycle=0):
     s = psb.Solver(n, n, n)
     stop = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=1e-8, res_tol=0.0,
                 max_iter=1000, precision=psb.PSB_MIXED, inner_rtol=1e-2, inner_adapt=2,
-                ac_store=1, ac_kcycle=kcycle)
+                ac_store=1, ac_kcycle=kcycle, ac_cfac=cfac)
     t = {"assemble": 0.0, "rhs": 0.0, "solve": 0.0, "project": 0.0}
     res = None
     for step in range(steps + 1):
@@ -445,6 +445,9 @@ def main():
                     help="K-cycle levels of the solves (bit l: level l; -1: chosen from the "
                          "hierarchy, the levels of 16^3 to 64^3 cells, or the first four "
                          "coarse levels of a semi-coarsened one; 0: V-cycle)")
+    ap.add_argument("--cfac", type=float, default=2.5,
+                    help="coarse-correction factor of the V-cycle levels (psb_stop.ac_cfac; "
+                         "at 256^3 2.2-3.5 give 15 iterations, 2.0 16, 1.5 17)")
     ap.add_argument("--nsys", type=int, default=96, help="systems of --case batch")
     ap.add_argument("--case", choices=sorted(EXTRA) + ["batch", "drop64"], default=None,
                     help="figures of another single-system workload instead of the bench line")
@@ -496,7 +499,7 @@ def main():
     db = psb.DeviceArray.from_host(d["rhs"])
     dx = psb.DeviceArray(n ** 3)
     common = dict(precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL, res_rat=1e-8, res_tol=0.0,
-                  max_iter=1000, ac_kcycle=a.kcycle, graph_chunk=a.graph_chunk)
+                  max_iter=1000, ac_kcycle=a.kcycle, ac_cfac=a.cfac, graph_chunk=a.graph_chunk)
     mixed = dict(precision=psb.PSB_MIXED, inner_rtol=1e-2, inner_adapt=2, ac_store=1)
     if a.headline_only and a.only:
         kw = dict(common, **(mixed if a.only == "mixed" else dict(precision=psb.PSB_FP64)))
@@ -615,7 +618,7 @@ def main():
         ref_cycle = {"ms": round(t_vc * 1e3, 2), "cycles": rv["iterations"],
                      "status": rv["status_name"], "rel_residual": rv["rel_l2"]}
     log("[bench] pressure phase of a time step (assembly, rhs, solve, projection)")
-    step = pressure_step(n, 3, a.kcycle)
+    step = pressure_step(n, 3, a.kcycle, a.cfac)
     log(f"[bench] pressure step {step}")
     cpu = None
     if not a.no_cpu:
@@ -641,7 +644,8 @@ def main():
         "data": "synthetic (seeded bubbles, seeded face velocities)",
         "config": {"workload": f"pressure Poisson {n}^3, 16 bubbles r 0.03-0.12, density "
                                "ratio 1000, Neumann walls; AC-preconditioned CG (2+2 damped "
-                               "Jacobi, 2x coarsening to 8^3, exact FP64 coarsest, two "
+                               f"Jacobi, omega 0.9, coarse-correction factor {a.cfac:g}, 2x "
+                               "coarsening to 8^3, exact FP64 coarsest, two "
                                "flexible-CG steps as the coarse correction of "
                                + ("the levels of 64^3 to 16^3 cells" if a.kcycle == -1 else
                                   f"levels {a.kcycle:#b} (bit = level)")

@@ -772,7 +772,8 @@ void set_half(psb_ctx *C, bool on) {
 // Returns the buffer that holds z.
 // ---------------------------------------------------------------------------
 template <typename T>
-T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega, T cfac);
+T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega, T cfac,
+          int done = 0);
 
 // K-cycle levels chosen from the hierarchy (psb_stop.ac_kcycle = -1).
 // Halving every direction: every level but the finest and the coarsest with
@@ -806,11 +807,14 @@ void kcycle_buffers(psb_ctx *C, Hier<T> &H) {
   const int nl = (int)H.L.size();
   for (int l = 0; l < nl; l++) {
     Level<T> &lv = H.L[l];
-    if (!kcycle_level(C, H, l) || lv.kc) continue;
-    lv.kc = C->harena.alloc<T>(lv.g.N);
-    lv.kv = C->harena.alloc<T>(lv.g.N);
-    lv.kz = C->harena.alloc<T>(lv.g.N);
-    lv.kst = C->harena.alloc<double>(4);
+    if (!kcycle_level(C, H, l)) continue;
+    // each one that is missing: the batched solver allocates some of them on
+    // the same levels
+    if (!lv.kc) lv.kc = C->harena.alloc<T>(lv.g.N);
+    if (!lv.kv) lv.kv = C->harena.alloc<T>(lv.g.N);
+    if (!lv.kz) lv.kz = C->harena.alloc<T>(lv.g.N);
+    if (!lv.krt) lv.krt = C->harena.alloc<T>(lv.g.N);
+    if (!lv.kst) lv.kst = C->harena.alloc<double>(4);
   }
 }
 
@@ -824,6 +828,32 @@ void set_flex(psb_ctx *C, const Hier<T> &H) {
   C->flex = C->flex_req == 1 || (C->flex_req == -1 && any);
 }
 
+// k_kc_r fused with the next cycle's second pre-sweep: rt to lv.krt, the
+// sweep's output to lv.e[1] (the level's matrix as jacobi() picks it)
+template <typename T>
+void kc_r_sweep(psb_ctx *C, const Level<T> &lv, T om0, T omega, int nprev, DevState *st) {
+  const Grid &g = lv.g;
+  const int nb = nblocks<T>(g);
+  auto launch = [&](auto mt, auto flux, const auto &A) {
+    using M = typename decltype(mt)::type;
+    constexpr bool FLUX = decltype(flux)::value;
+    with_v<T>(g, [&](auto vt) {
+      constexpr int V = decltype(vt)::value;
+      hipLaunchKernelGGL((k_kc_r_sweep<T, V, M, FLUX>), dim3(nb), dim3(kBlock), 0, C->stream, g,
+                         A, lv.c, lv.kv, lv.r, lv.krt, lv.e[1], om0, omega, lv.kst, nprev,
+                         C->work.partial, st);
+    });
+  };
+  using FT = std::true_type;
+  using FF = std::false_type;
+  if constexpr (std::is_same<T, float>::value) {
+    if (lv.h16() && lv.d) return launch(std::common_type<_Float16>(), FT(), lv.mat16());
+    if (lv.h16()) return launch(std::common_type<_Float16>(), FF(), lv.mat16());
+    if (lv.d) return launch(std::common_type<T>(), FT(), lv.matflux());
+  }
+  launch(std::common_type<T>(), FF(), lv.mat());
+}
+
 // level l's correction for its right-hand side lv.r (first sweep already in
 // lv.e[0]) by two flexible-CG steps preconditioned by the cycle on l (their
 // reductions summed by the kernels that use them, no finalize launches):
@@ -834,7 +864,8 @@ template <typename T>
 T *kstep(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega, T cfac) {
   Level<T> &lv = H.L[l];
   const Grid &g = lv.g;
-  if (!lv.kc) throw HipError{hipErrorInvalidValue};  // kcycle_buffers() was not called
+  if (!lv.kc || !lv.kv || !lv.kz || !lv.krt || !lv.kst)  // kcycle_buffers() was not called
+    throw HipError{hipErrorInvalidValue};
   const T s = T(lv.scale);
   const int nb = nblocks<T>(g);
   T *c = vcycle<T>(C, H, l, st, nu1, nu2, omega, cfac);
@@ -847,12 +878,23 @@ T *kstep(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega,
       hipLaunchKernelGGL((k_kc_av<T, V, false>), dim3(nb), dim3(kBlock), 0, C->stream, g,
                          lv.mat(), s, c, lv.r, lv.kc, lv.kv, st, C->work);
   });
-  with_v<T>(g, [&](auto vt) {  // sums k_kc_av's partials itself
-    constexpr int V = decltype(vt)::value;
-    hipLaunchKernelGGL((k_kc_r<T, V>), dim3(nb), dim3(kBlock), 0, C->stream, g, lv.c, lv.kv,
-                       lv.r, lv.e[0], T(omega / lv.scale), lv.kst, nb, C->work.partial, st);
-  });
-  T *d = vcycle<T>(C, H, l, st, nu1, nu2, omega, cfac);
+  // with two or more pre-sweeps the residual update and the next cycle's
+  // second sweep are one launch (the cycle then starts at its third sweep);
+  // the updated right-hand side lives in lv.krt while the cycle and the
+  // dot products run, lv.r keeping the level's own for the next visit
+  const bool fuse = nu1 >= 2 && C->kfuse;
+  T *r0 = lv.r;
+  if (fuse) {
+    kc_r_sweep<T>(C, lv, T(omega / lv.scale), omega, nb, st);
+    lv.r = lv.krt;
+  } else {
+    with_v<T>(g, [&](auto vt) {  // sums k_kc_av's partials itself
+      constexpr int V = decltype(vt)::value;
+      hipLaunchKernelGGL((k_kc_r<T, V>), dim3(nb), dim3(kBlock), 0, C->stream, g, lv.c, lv.kv,
+                         lv.r, lv.e[0], T(omega / lv.scale), lv.kst, nb, C->work.partial, st);
+    });
+  }
+  T *d = vcycle<T>(C, H, l, st, nu1, nu2, omega, cfac, fuse ? 1 : 0);
   with_v<T>(g, [&](auto vt) {
     constexpr int V = decltype(vt)::value;
     if (lv.d)
@@ -867,11 +909,15 @@ T *kstep(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega,
     hipLaunchKernelGGL((k_kc_comb<T, V>), dim3(nb), dim3(kBlock), 0, C->stream, g, lv.kc, d,
                        lv.kz, lv.kst, nb, C->work.partial, st);
   });
+  lv.r = r0;
   return lv.kz;
 }
 
+// done: pre-sweeps already applied beyond the first (whose result is in
+// lv.e[done])
 template <typename T>
-T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega, T cfac) {
+T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega, T cfac,
+          int done) {
   Level<T> &lv = H.L[l];
   const int nl = (int)H.L.size();
   if (l == nl - 1) {
@@ -879,8 +925,8 @@ T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega
                        C->stream, H.m, H.G, lv.r, lv.e[1], st);
     return lv.e[1];
   }
-  int cur = 0;
-  for (int s = 1; s < nu1; s++) {
+  int cur = done;
+  for (int s = 1 + done; s < nu1; s++) {
     jacobi<T, false>(C, lv, lv.e[cur], lv.e[(cur + 1) % 3], nullptr, nullptr, T(0), omega, st);
     cur = (cur + 1) % 3;
   }
@@ -2114,6 +2160,7 @@ int psb_create(psb_ctx **out, const psb_layout *L) {
     // which fine-level kernels use the LDS-staged form (same results):
     // PSB_NO_LDS set: none (for comparisons); PSB_LDS_MASK="m32,m64"
     // overrides the measured defaults (bits kLdsSweep .. kLdsCgPq)
+    if (std::getenv("PSB_NO_KFUSE")) C->kfuse = false;
     if (std::getenv("PSB_NO_LDS")) {
       C->lds32 = C->lds64 = 0;
     } else if (const char *m = std::getenv("PSB_LDS_MASK")) {

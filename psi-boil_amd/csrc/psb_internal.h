@@ -132,6 +132,7 @@ struct Level {
   // K-cycle on this level: copy of the first cycle's output, s A of it,
   // the combined correction, and the scalars rho1, alpha1, a, alpha2
   T *kc = nullptr, *kv = nullptr, *kz = nullptr;
+  T *krt = nullptr;  // the second Krylov step's right-hand side (fused kc_r + sweep)
   double *kst = nullptr;
   DMat<T> mat() const { return DMat<T>{c, w, s, b, nullptr, nullptr, nullptr}; }
   DMat<T> matflux() const { return DMat<T>{d, w, s, b, nullptr, nullptr, nullptr}; }
@@ -273,6 +274,9 @@ struct psb_ctx {
   // fine-level kernels (bit masks of kLds* kinds, psb_api.hip) that run with
   // the operand staged in LDS where the rows tile a block (psb_lds.h)
   int lds32 = 2 | 8, lds64 = 1 | 2 | 4 | 8 | 16;
+  // the K-cycle's residual update fused with the next cycle's second sweep
+  // (PSB_NO_KFUSE set: separate launches, for comparisons)
+  bool kfuse = true;
 };
 
 namespace psb {

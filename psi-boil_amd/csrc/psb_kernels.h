@@ -1072,6 +1072,61 @@ __global__ __launch_bounds__(kBlock) void k_kc_r(Grid g, const T *c, const T *v,
   kc_r_body<T, V>(grid_span(), g, c, v, r, e0, omega, kst, nprev, partial);
 }
 
+// k_kc_r fused with the second pre-sweep of the cycle that follows it (a
+// cycle with nu1 >= 2): rt = r - alpha1 v (to rt, r is read by the
+// neighbours), and e1 = e0 + omega (rt - A e0) / c with the first sweep
+// e0 = omega0 rt / c0 (as k_kc_r writes it) formed at every point the stencil
+// reads.  The same arithmetic as k_kc_r followed by k_jacobi, one launch
+// fewer.
+template <typename T, int V>
+struct KrX {
+  const T *r, *v, *c;
+  T a1, om0;
+  __device__ __forceinline__ T one(T rr, T vv, T cc) const { return om0 * (rr - a1 * vv) / cc; }
+  __device__ __forceinline__ VecN<T, V> vec(int P, int, int, int) const {
+    const VecN<T, V> R = ldv<T, V>(r + P), Vv = ldv<T, V>(v + P), C = ldv<T, V>(c + P);
+    VecN<T, V> o;
+#pragma unroll
+    for (int u = 0; u < V; u++) o.v[u] = one(R.v[u], Vv.v[u], C.v[u]);
+    return o;
+  }
+  __device__ __forceinline__ T at(int Q, int, int, int) const { return one(r[Q], v[Q], c[Q]); }
+};
+
+template <typename T, int V, typename M, bool FLUX>
+__global__ __launch_bounds__(kBlock) void k_kc_r_sweep(Grid g, DMat<M> A, const T *c,
+                                                       const T *v, const T *r, T *rt, T *e1,
+                                                       T om0, T omega, double *kst, int nprev,
+                                                       const double *partial,
+                                                       const DevState *st) {
+  if (skip_done(st, g.N)) return;
+  __shared__ double a1s;
+  double tot[2];
+  fin_sum<2>(0, nprev, partial, tot);
+  if (threadIdx.x == 0) {
+    a1s = tot[0] > 0.0 ? tot[1] / tot[0] : 0.0;
+    if (blockIdx.x == 0) {
+      kst[0] = tot[0];
+      kst[1] = a1s;
+    }
+  }
+  __syncthreads();
+  const T a1 = (T)a1s;
+  const KrX<T, V> X{r, v, c, a1, om0};
+  PSB_TILE_BEGIN(g, V)
+  const VecN<T, V> R = ldv<T, V>(r + P), Vv = ldv<T, V>(v + P);
+  const Gath<T, V> G = gather<T, V, true, FLUX>(g, A, X, P, i, j, k);
+  VecN<T, V> Rt, E;
+#pragma unroll
+  for (int u = 0; u < V; u++) Rt.v[u] = R.v[u] - a1 * Vv.v[u];
+  const VecN<T, V> t = resid<T, V>(Rt, G);
+#pragma unroll
+  for (int u = 0; u < V; u++) E.v[u] = G.x.v[u] + omega * t.v[u] / G.c.v[u];
+  stv<T, V>(rt + P, Rt);
+  stv<T, V>(e1 + P, E);
+  PSB_TILE_END
+}
+
 // partials d.v, d.(s A d), d.r
 template <typename T, int V, bool FLUX>
 __device__ __forceinline__ void kc_dw_body(Span sp, const Grid &g, const DMat<T> &A, T scale,
