@@ -248,6 +248,15 @@ __device__ __forceinline__ uint32_t flag_word(const uint8_t *af, int Q, const A 
   return af ? w : 0xffffffffu;
 }
 __device__ __forceinline__ bool flag_on(uint32_t w, int u) { return ((w >> (8 * u)) & 0xffu) != 0; }
+// v where on, +0 elsewhere, by a bit mask: the compiler keeps it a select
+// (a plain conditional it may turn into a branch around the work feeding v,
+// and the loads behind it then wait for the branch)
+template <typename T>
+__device__ __forceinline__ T keep_if(bool on, T v) {
+  using U = typename std::conditional<sizeof(T) == 8, uint64_t, uint32_t>::type;
+  const U m = U(0) - U(on);
+  return __builtin_bit_cast(T, __builtin_bit_cast(U, v) & m);
+}
 
 // A cell vector's coefficients as stored (M), loaded apart from the
 // arithmetic so that a kernel can issue them early (before a barrier, or a

@@ -41,6 +41,11 @@ struct Work {
     const int P = q_ * (V);                                                   \
     int i, j, k;                                                              \
     (g).ijk(P, i, j, k);
+// kernels that run on the small levels (a few thousand cells, a handful of
+// waves on the chip): let the compiler hold every load of a cell vector in
+// flight at once rather than keep registers for occupancy those launches
+// never reach
+#define PSB_SMALL __attribute__((amdgpu_waves_per_eu(1, 2)))
 #define PSB_TILE_BEGIN(g, V)                                                  \
   const Span tsp_ = grid_span();                                              \
   PSB_SPAN_BEGIN(g, V, tsp_)
@@ -329,7 +334,7 @@ __device__ __forceinline__ void jacobi_body(Span sp, const Grid &g, const DMat<M
 }
 
 template <typename T, int V, bool SYM, bool PROL, bool RHO, typename M = T, bool FLUX = false>
-__global__ __launch_bounds__(kBlock) void k_jacobi(Grid g, DMat<M> A, const T *r, const T *e,
+__global__ __launch_bounds__(kBlock) PSB_SMALL void k_jacobi(Grid g, DMat<M> A, const T *r, const T *e,
                                                    T *eout, ProlX<T, V> pr, T omega,
                                                    const uint8_t *af, DevState *st, Work w,
                                                    T zscale, const T *qf) {
@@ -361,6 +366,15 @@ __device__ __forceinline__ void restrict_body(Span sp, const Grid &g, const DMat
   for (int q = sp.bl * kBlock + threadIdx.x; q < nwork; q += sp.nb * kBlock) {
     const int IJ = q / nkv, kv = q - IJ * nkv;
     const int I = IJ / gH.ny, J = IJ - I * gH.ny;
+    // the coarse cells' diagonal and flags loaded with the fine cells' data
+    const int Q0 = (I * gH.ny + J) * gH.nz + kv * NC;
+    T cq[NC];
+    bool actH[NC];
+#pragma unroll
+    for (int u = 0; u < NC; u++) {
+      cq[u] = cH[Q0 + u];
+      actH[u] = flag_on(flag_word<1>(afH, Q0 + u, cH), 0);
+    }
     T s[NC];
 #pragma unroll
     for (int u = 0; u < NC; u++) s[u] = T(0);
@@ -376,8 +390,9 @@ __device__ __forceinline__ void restrict_body(Span sp, const Grid &g, const DMat
           const int P = (i * g.ny + j) * g.nz + k;
           Gath<T, V> G = gather<T, V, SYM, FLUX>(g, A, X, P, i, j, k);
           VecN<T, V> R = resid<T, V>(ldv<T, V>(r + P), G);
+          const uint32_t fw = flag_word<V>(af, P, r);
 #pragma unroll
-          for (int u = 0; u < V; u++) t[h * V + u] = (!af || af[P + u]) ? R.v[u] : T(0);
+          for (int u = 0; u < V; u++) t[h * V + u] = keep_if(flag_on(fw, u), R.v[u]);
         }
 #pragma unroll
         for (int u = 0; u < NC; u++)
@@ -385,20 +400,18 @@ __device__ __forceinline__ void restrict_body(Span sp, const Grid &g, const DMat
           for (int cc = 0; cc < CK; cc++) s[u] += t[u * CK + cc];
       }
     }
-    const int Q0 = (I * gH.ny + J) * gH.nz + kv * NC;
 #pragma unroll
     for (int u = 0; u < NC; u++) {
-      const bool act = !afH || afH[Q0 + u];
-      const T v = act ? s[u] : T(0);
+      const T v = keep_if(actH[u], s[u]);
       rH[Q0 + u] = v;
-      eH1[Q0 + u] = omega * v / cH[Q0 + u];
+      eH1[Q0 + u] = omega * v / cq[u];
     }
   }
 }
 
 template <typename T, int V, int KV, bool SYM, int CI, int CJ, int CK, typename M = T,
           bool FLUX = false>
-__global__ __launch_bounds__(kBlock) void k_restrict(Grid g, DMat<M> A, const T *r, const T *e,
+__global__ __launch_bounds__(kBlock) PSB_SMALL void k_restrict(Grid g, DMat<M> A, const T *r, const T *e,
                                                      const uint8_t *af, Grid gH, const T *cH,
                                                      const uint8_t *afH, T *rH, T *eH1,
                                                      T omega, DevState *st) {
@@ -416,13 +429,15 @@ __device__ __forceinline__ void gemv_body(int bl, int m, const double *Gm, const
   if (row >= m) return;
   const double *g = Gm + (size_t)row * m;
   double s = 0.0;
+  // unrolled so that a lane's loads leave together (same summation order)
+#pragma unroll 8
   for (int c = lane; c < m; c += 64) s += g[c] * (double)r[c];
   s = wave_sum(s);
   if (lane == 0) z[row] = (T)s;
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_gemv(int m, const double *Gm, const T *r, T *z,
+__global__ __launch_bounds__(kBlock) PSB_SMALL void k_gemv(int m, const double *Gm, const T *r, T *z,
                                                  DevState *st) {
   if (skip_done(st, m)) return;
   gemv_body<T>(blockIdx.x, m, Gm, r, z);
@@ -1025,7 +1040,7 @@ __device__ __forceinline__ void kc_av_body(Span sp, const Grid &g, const DMat<T>
 }
 
 template <typename T, int V, bool FLUX>
-__global__ __launch_bounds__(kBlock) void k_kc_av(Grid g, DMat<T> A, T scale, const T *c,
+__global__ __launch_bounds__(kBlock) PSB_SMALL void k_kc_av(Grid g, DMat<T> A, T scale, const T *c,
                                                   const T *r, T *kc, T *v, const DevState *st,
                                                   Work w) {
   if (skip_done(st, g.N)) return;
@@ -1065,7 +1080,7 @@ __device__ __forceinline__ void kc_r_body(Span sp, const Grid &g, const T *c, co
 }
 
 template <typename T, int V>
-__global__ __launch_bounds__(kBlock) void k_kc_r(Grid g, const T *c, const T *v, T *r, T *e0,
+__global__ __launch_bounds__(kBlock) PSB_SMALL void k_kc_r(Grid g, const T *c, const T *v, T *r, T *e0,
                                                  T omega, double *kst, int nprev,
                                                  const double *partial, const DevState *st) {
   if (skip_done(st, g.N)) return;
@@ -1094,7 +1109,7 @@ struct KrX {
 };
 
 template <typename T, int V, typename M, bool FLUX>
-__global__ __launch_bounds__(kBlock) void k_kc_r_sweep(Grid g, DMat<M> A, const T *c,
+__global__ __launch_bounds__(kBlock) PSB_SMALL void k_kc_r_sweep(Grid g, DMat<M> A, const T *c,
                                                        const T *v, const T *r, T *rt, T *e1,
                                                        T om0, T omega, double *kst, int nprev,
                                                        const double *partial,
@@ -1148,7 +1163,7 @@ __device__ __forceinline__ void kc_dw_body(Span sp, const Grid &g, const DMat<T>
 }
 
 template <typename T, int V, bool FLUX>
-__global__ __launch_bounds__(kBlock) void k_kc_dw(Grid g, DMat<T> A, T scale, const T *d,
+__global__ __launch_bounds__(kBlock) PSB_SMALL void k_kc_dw(Grid g, DMat<T> A, T scale, const T *d,
                                                   const T *v, const T *r, const DevState *st,
                                                   Work w) {
   if (skip_done(st, g.N)) return;
@@ -1164,10 +1179,16 @@ __device__ __forceinline__ void kc_comb_body(Span sp, const Grid &g, const T *kc
                                              T *out, double *kst, int nprev,
                                              const double *partial) {
   __shared__ double cs[2];
+  // the first vector's loads (and rho1, alpha1) issued before the partial
+  // sums, which the rest waits for anyway
+  const int nvec = g.N / V, per = (nvec + sp.nb - 1) / sp.nb;
+  const int q0 = sp.bl * per, q1 = min(q0 + per, nvec);
+  const int Pf = (q0 + (int)threadIdx.x < q1 ? q0 + (int)threadIdx.x : 0) * V;
+  VecN<T, V> Kc = ldv<T, V>(kc + Pf), D = ldv<T, V>(d + Pf);
+  const double rho1 = kst[0], a1 = kst[1];
   double tot[3];
   fin_sum<3>(0, nprev, partial, tot);
   if (threadIdx.x == 0) {
-    const double rho1 = kst[0], a1 = kst[1];
     const double den = rho1 > 0.0 ? tot[1] - tot[0] * tot[0] / rho1 : 0.0;
     const double a2 = den > 0.0 ? tot[2] / den : 0.0;
     cs[0] = a1 - (rho1 > 0.0 ? tot[0] * a2 / rho1 : 0.0);
@@ -1179,16 +1200,21 @@ __device__ __forceinline__ void kc_comb_body(Span sp, const Grid &g, const T *kc
   }
   __syncthreads();
   const T a = (T)cs[0], a2 = (T)cs[1];
-  PSB_SPAN_BEGIN(g, V, sp)
-  VecN<T, V> Kc = ldv<T, V>(kc + P), D = ldv<T, V>(d + P), O;
+  for (int qq = q0 + (int)threadIdx.x; qq < q1; qq += kBlock) {
+    const int P = qq * V;
+    if (P != Pf) {
+      Kc = ldv<T, V>(kc + P);
+      D = ldv<T, V>(d + P);
+    }
+    VecN<T, V> O;
 #pragma unroll
-  for (int u = 0; u < V; u++) O.v[u] = a * Kc.v[u] + a2 * D.v[u];
-  stv<T, V>(out + P, O);
-  PSB_TILE_END
+    for (int u = 0; u < V; u++) O.v[u] = a * Kc.v[u] + a2 * D.v[u];
+    stv<T, V>(out + P, O);
+  }
 }
 
 template <typename T, int V>
-__global__ __launch_bounds__(kBlock) void k_kc_comb(Grid g, const T *kc, const T *d, T *out,
+__global__ __launch_bounds__(kBlock) PSB_SMALL void k_kc_comb(Grid g, const T *kc, const T *d, T *out,
                                                     double *kst, int nprev,
                                                     const double *partial, const DevState *st) {
   if (skip_done(st, g.N)) return;
