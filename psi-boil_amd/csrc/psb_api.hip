@@ -70,7 +70,10 @@ constexpr bool kMarch = true;
 // 256^3 solve; FP64: 56.4 marching against 59.0 ms)
 template <typename T>
 constexpr bool march_prol() { return std::is_same<T, double>::value; }
-constexpr int kMarchMin = 1 << 15;
+#ifndef PSB_MARCH_MIN
+#define PSB_MARCH_MIN (1 << 15)
+#endif
+constexpr int kMarchMin = PSB_MARCH_MIN;
 // fine-level kernels that run in the LDS-staged marching form (psb_lds.h),
 // per precision (psb_ctx::lds32 / lds64).  Measured at 256^3 in the solve:
 // FP32 first post-sweep 84.8 against 98.5 us (span), CG p/q step 109 against
