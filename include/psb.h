@@ -369,6 +369,11 @@ typedef struct {
   int max_visits;     /* PSB_CYCLE_FLEX: level visits per cycle before the
                          run is ended (1000; no counterpart in the
                          reference, whose flex recursion is unbounded)      */
+  int level_solver;   /* the Krylov solver of every level and the coarsest:
+                         PSB_CG (default) or PSB_BICGS, both with the
+                         diagonal preconditioner -- the equation's solver
+                         that AC::call_smoother / call_solver use
+                         (additive_cycle_functions.cpp:89-107)            */
 } psb_cycle;
 
 void psb_cycle_defaults(psb_cycle *c);

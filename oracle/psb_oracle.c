@@ -1069,33 +1069,39 @@ static double ac_residual2(ora_ac *ac, double dt, double *linf) {
 }
 static double ac_residual(ora_ac *ac, double dt) { return ac_residual2(ac, dt, NULL); }
 
-/* Linear::solve of one level (call_smoother / call_solver,
- * additive_cycle_functions.cpp:90-107): CG with the diagonal preconditioner;
- * returns whether it met its tolerance */
-static int ac_cg(ora_ac *ac, int l, int mi, double rr, double rt, double scale, int ms) {
-  return ora_cg(&ac->A[l], ac->phi[l], ac->f[l], ac->dV[l], 1, mi, rr, rt, scale, ms,
-                NULL, NULL);
-}
-
 /* the recursive cycles of additive_recursive.cpp:4-166 with AC's level
  * tolerances resrat / restol (atto by default, additive.cpp:16-17) and the
  * coarsest call MaxIter 100, ResRat -1 */
 typedef struct {
   const int *mv, *ms;  /* MaxIter and stale counts: pre / re- / post-smoothing */
   double rr, rt, scale;
+  int solver;          /* 0 CG, 1 BiCGS */
 } cyc_par;
 
+/* Linear::solve of one level (call_smoother / call_solver,
+ * additive_cycle_functions.cpp:90-107): the equation's Krylov solver, CG or
+ * BiCGS with the diagonal preconditioner; returns whether it met its
+ * tolerance */
+static int ac_cg(ora_ac *ac, int l, int mi, double rr, double rt, double scale, int ms,
+                 int solver) {
+  if (solver == 1)
+    return ora_bicgs(&ac->A[l], ac->phi[l], ac->f[l], ac->dV[l], 1, mi, rr, rt, scale, ms,
+                     NULL, NULL);
+  return ora_cg(&ac->A[l], ac->phi[l], ac->f[l], ac->dV[l], 1, mi, rr, rt, scale, ms,
+                NULL, NULL);
+}
+
 static void ac_coarsest(ora_ac *ac, int l, const cyc_par *q) {
-  ac_cg(ac, l, 100, -1.0, q->rt, q->scale, -1);
+  ac_cg(ac, l, 100, -1.0, q->rt, q->scale, -1, q->solver);
   ora_interpolation(ac, l);
 }
 
 static void ac_v(ora_ac *ac, int l, const cyc_par *q) {
   if (l != ac->nlev - 1) {
-    ac_cg(ac, l, q->mv[0], q->rr, q->rt, q->scale, q->ms[0]);
+    ac_cg(ac, l, q->mv[0], q->rr, q->rt, q->scale, q->ms[0], q->solver);
     ora_restriction(ac, l);
     ac_v(ac, l + 1, q);
-    ac_cg(ac, l, q->mv[2], q->rr, q->rt, q->scale, q->ms[2]);
+    ac_cg(ac, l, q->mv[2], q->rr, q->rt, q->scale, q->ms[2], q->solver);
     if (l > 0) ora_interpolation(ac, l);
   } else {
     ac_coarsest(ac, l, q);
@@ -1108,14 +1114,14 @@ static void ac_v(ora_ac *ac, int l, const cyc_par *q) {
  * the first descent, post-smooth, prolongate */
 static void ac_fw(ora_ac *ac, int l, const cyc_par *q, int w) {
   if (l != ac->nlev - 1) {
-    ac_cg(ac, l, q->mv[0], q->rr, q->rt, q->scale, q->ms[0]);
+    ac_cg(ac, l, q->mv[0], q->rr, q->rt, q->scale, q->ms[0], q->solver);
     ora_restriction(ac, l);
     ac_fw(ac, l + 1, q, w);
-    ac_cg(ac, l, q->mv[1], q->rr, q->rt, q->scale, q->ms[1]);
+    ac_cg(ac, l, q->mv[1], q->rr, q->rt, q->scale, q->ms[1], q->solver);
     ora_restriction(ac, l);
     if (w) ac_fw(ac, l + 1, q, 1);
     else ac_v(ac, l + 1, q);
-    ac_cg(ac, l, q->mv[2], q->rr, q->rt, q->scale, q->ms[2]);
+    ac_cg(ac, l, q->mv[2], q->rr, q->rt, q->scale, q->ms[2], q->solver);
     if (l > 0) ora_interpolation(ac, l);
   } else {
     ac_coarsest(ac, l, q);
@@ -1141,7 +1147,7 @@ static int ac_flex(ora_ac *ac, const cyc_par *q, int max_visits) {
   for (int v = 0; v < max_visits; v++) {
     if (l != ac->nlev - 1) {
       const int s = up ? 2 : 0;
-      if (ac_cg(ac, l, q->mv[s], q->rr, q->rt, q->scale, q->ms[s])) {
+      if (ac_cg(ac, l, q->mv[s], q->rr, q->rt, q->scale, q->ms[s], q->solver)) {
         if (l == 0) return 0;
         ora_interpolation(ac, l);
         l--;
@@ -1188,6 +1194,7 @@ void ora_cycle_defaults(ora_cycle *P) {
   P->resrat = ATTO;
   P->restol = ATTO;
   P->max_visits = 1000;
+  P->solver = 0;
 }
 
 /* AC::converged (additive_cycle_functions.cpp:49-87) */
@@ -1215,7 +1222,7 @@ static int ac_converged(ora_ac *ac, const ora_cycle *P, int cycle, double *res1,
 int ora_ac_cycle(ora_ac *ac, double *phi0, const double *f0, const ora_cycle *P,
                  double *hist, double *hist_linf, int *nhist, int *ncyc) {
   const size_t N0 = mat_n(&ac->A[0]);
-  const cyc_par q = {P->mv, P->ms, P->resrat, P->restol, 1.0 / P->dt * 1.0};
+  const cyc_par q = {P->mv, P->ms, P->resrat, P->restol, 1.0 / P->dt * 1.0, P->solver};
   double *fold = dalloc(N0);
   int ret = 0, nh = 0;
   memcpy(ac->phi[0], phi0, N0 * 8);

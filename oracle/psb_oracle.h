@@ -185,6 +185,9 @@ typedef struct {
   int stop_if_div, priority_min_cyc, use_linf;
   double resrat, restol;         /* level solves (AC::resrat / restol)       */
   int max_visits;                /* Cycle::flex: level visits per cycle      */
+  int solver;                    /* level solves: 0 CG, 1 BiCGS (diagonal
+                                    preconditioner both, the equation's solver
+                                    of additive_cycle_functions.cpp:89-107) */
 } ora_cycle;
 void ora_cycle_defaults(ora_cycle *P);
 /* hist, hist_linf: max_cyc + 2 entries; *nhist receives the count.          */

@@ -38,7 +38,7 @@ class OraCycle(C.Structure):
                 ("dt", C.c_double),
                 ("init", C.c_int), ("loop", C.c_int), ("stop_if_div", C.c_int),
                 ("priority_min_cyc", C.c_int), ("use_linf", C.c_int), ("resrat", C.c_double),
-                ("restol", C.c_double), ("max_visits", C.c_int)]
+                ("restol", C.c_double), ("max_visits", C.c_int), ("solver", C.c_int)]
 
 
 class OraAC(C.Structure):
@@ -349,17 +349,20 @@ class AC:
 
     def cycle(self, phi0, f0, loop="V", init=None, toler=-1.0, factor=1e-8, max_cyc=20,
               min_cyc=0, mv=(20, 20, 20), dt=1.0, stop_if_div=True, priority_min_cyc=True,
-              use_linf=False, resrat=1e-18, restol=1e-18, ms=(-1, -1, -1), max_visits=1000):
-        """AC::cycle(init, loop, ...) with CG-di level solves (additive_cycle.cpp);
-        loop in "V", "F", "W", "flex", init in "V", "F", "W" or None (no
-        full-cycle start).  A flex cycle that reaches max_visits level visits
-        ends the run with status "bound" (phi: the previous cycle's)."""
+              use_linf=False, resrat=1e-18, restol=1e-18, ms=(-1, -1, -1), max_visits=1000,
+              solver="cg"):
+        """AC::cycle(init, loop, ...) with CG-di (solver "cg") or BiCGS-di
+        ("bicgs") level solves (additive_cycle.cpp); loop in "V", "F", "W",
+        "flex", init in "V", "F", "W" or None (no full-cycle start).  A flex
+        cycle that reaches max_visits level visits ends the run with status
+        "bound" (phi: the previous cycle's)."""
         kinds = {"V": 0, "F": 1, "W": 2, "flex": 3, None: -1}
         P = OraCycle(C.c_double(toler), C.c_double(factor), max_cyc, min_cyc,
                      (C.c_int * 3)(*mv), (C.c_int * 3)(*ms), C.c_double(dt), kinds[init],
                      kinds[loop],
                      int(stop_if_div), int(priority_min_cyc), int(use_linf),
-                     C.c_double(resrat), C.c_double(restol), max_visits)
+                     C.c_double(resrat), C.c_double(restol), max_visits,
+                     {"cg": 0, "bicgs": 1}[solver])
         phi = np.array(phi0, dtype=np.float64, copy=True)
         f0 = np.ascontiguousarray(f0, dtype=np.float64)
         hist = np.full(max_cyc + 2, np.nan)

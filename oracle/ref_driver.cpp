@@ -13,6 +13,7 @@
 //              reference's Grid1D stretched from end cell sizes D1 to DN,
 //              given in units of h = 1 / max(nx, ny, nz))
 //   ref_driver cycles OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel [cx cy cz]
+//   ref_driver cycles-bicgs OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel
 //   ref_driver bck OUT IN.bck nx ny nz   (Scalar::load of IN, Scalar::save of a seeded field)
 //   ref_driver bench-cg nx nsph rmin rmax rho_in rho_out seed_sph seed_vel maxit
 //   ref_driver bench-ac nx nsph rmin rmax rho_in rho_out seed_sph seed_vel cycles
@@ -366,7 +367,9 @@ void dump_ac(World &w) {
 // additive_recursive.cpp:37-166): each run starts from phi = 0 with the
 // constructor's settings, is framed by ACRUN / ACRET lines (the cycle log
 // lines in between) and leaves phi in <tag>_phi.f64
-void dump_cycles(World &w) {
+// bicgs: the world's levels hold BiCGS with Prec::di() (build_world's
+// make_solver), the runs a subset of the CG ones, tags prefixed "B"
+void dump_cycles(World &w, bool bicgs = false) {
   CasePressure &pr = *w.pr;
   struct Run {
     std::string tag;
@@ -407,7 +410,12 @@ void dump_cycles(World &w) {
   // ends only when a level's smoother meets its ResRat, and once a level's
   // residual reaches round-off it alternates between two levels for good
   // (this driver never returned from it on these cases)
-  for (const Run &r : runs) {
+  const std::vector<std::string> bicgs_tags = {"F2", "W1", "FMV2", "Vmv", "Vsmooth", "Vstale"};
+  for (Run r : runs) {
+    if (bicgs) {
+      if (std::find(bicgs_tags.begin(), bicgs_tags.end(), r.tag) == bicgs_tags.end()) continue;
+      r.tag = "B" + r.tag;
+    }
     AC ac(&pr);
     pr.phi = 0.0;
     ac.max_cycles(r.maxc);
@@ -476,7 +484,7 @@ double now() {
 int main(int argc, char **argv) {
   std::cout << std::setprecision(17);
   if (argc < 2) {
-    std::fprintf(stderr, "usage: ref_driver case|cycles|prec|bck|bench-cg|bench-ic2|bench-ac ...\n");
+    std::fprintf(stderr, "usage: ref_driver case|cycles|cycles-bicgs|flex|prec|bck|bench-cg|bench-ic2|bench-ac ...\n");
     return 2;
   }
   std::string mode = argv[1];
@@ -522,6 +530,16 @@ int main(int argc, char **argv) {
     }
     World w = build_world(cs);
     dump_cycles(w);
+    return 0;
+  }
+  if (mode == "cycles-bicgs") {
+    // ref_driver cycles-bicgs OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel:
+    // AC::cycle runs with BiCGS (Prec::di()) as every level's solver
+    if (argc < 13) { std::fprintf(stderr, "cycles-bicgs: too few arguments\n"); return 2; }
+    outdir = argv[2];
+    Case cs = parse_case(argv + 6, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
+    World w = build_world(cs, [](const Domain &d) -> Krylov * { return new BiCGS(d, Prec::di()); });
+    dump_cycles(w, true);
     return 0;
   }
   if (mode == "flex") {

@@ -1731,12 +1731,68 @@ void ref_cg(psb_ctx *C, int l, int maxit, double res_rat, double res_tol, double
   }
 }
 
+// Linear::solve of one level by BiCGS with the diagonal preconditioner
+// (bicgs.cpp:9-229): AC::call_smoother / call_solver hand every level the
+// equation's own Krylov solver (additive_cycle_functions.cpp:89-107), which
+// for the non-symmetric equations is BiCGS.  The kernels and scalar steps
+// of the single-level BiCGS solve (solve_bicgs_diag) on the level's arrays;
+// phi is improved in place.  Work vectors: the CG's four and four more
+// (ref_bicgs_buffers, before any capture).
+void ref_bicgs_buffers(psb_ctx *C) {
+  RefHier &H = C->href;
+  for (size_t l = 0; l < H.L.size(); l++)
+    while (H.K[l].size() < 8) H.K[l].push_back(C->harena.alloc<double>(H.L[l].g.N));
+}
+
+void ref_bicgs(psb_ctx *C, int l, int maxit, double res_rat, double res_tol, double scale,
+               int stalecount) {
+  RefHier &H = C->href;
+  const Level<double> &lv = H.L[l];
+  const Grid &g = lv.g;
+  if (H.K[l].size() < 8) throw HipError{hipErrorInvalidValue};  // buffers not made
+  double *r = H.K[l][0], *rt = H.K[l][4], *s = H.K[l][5], *t = H.K[l][6];
+  double *p[2] = {H.K[l][1], H.K[l][2]}, *v[2] = {H.K[l][3], H.K[l][7]};
+  DevState *st = C->st64;
+  const DMat<double> A = lv.mat();
+  hipLaunchKernelGGL(k_stinit, dim3(1), dim3(64), 0, C->stream, st, maxit, res_rat, res_tol,
+                     scale, (double)g.N, stalecount);
+  for (int q = 0; q < 2; q++) {
+    zero(C, p[q], g.N);
+    zero(C, v[q], g.N);
+  }
+  start<double, true, 2>(C, g, A, lv.r, lv.e[0], r, rt, 0.0, st, &H.M[l]);
+  const int nb = nblocks<double>(g);
+  for (int it = 0; it < maxit; it++) {
+    const int par = it & 1;
+    with_v<double>(g, [&](auto vt) {
+      constexpr int V = decltype(vt)::value;
+      hipLaunchKernelGGL((k_bi1<double, V, true>), dim3(nb), dim3(kBlock), 0, C->stream, g, A, r,
+                         rt, p[par], v[par], p[par ^ 1], v[par ^ 1], st, C->work);
+      finalize<1>(C, FIN_BI1, nb, st);
+      hipLaunchKernelGGL((k_bi2<double, V, true>), dim3(nb), dim3(kBlock), 0, C->stream, g, A, r,
+                         v[par ^ 1], s, t, st, C->work);
+      finalize<3>(C, FIN_BI2, nb, st);
+      hipLaunchKernelGGL((k_bi3<double, V>), dim3(nb), dim3(kBlock), 0, C->stream, g, A.c,
+                         H.M[l], p[par ^ 1], s, t, rt, lv.e[0], r, st, C->work);
+      finalize<3>(C, FIN_BI3, nb, st);
+    });
+  }
+}
+
 // the level-solve parameters of one cycle: MaxIter and stalecount per stage
-// (pre, re-smoothing, post), AC::resrat / restol, the smoothers' scale 1/dt
+// (pre, re-smoothing, post), AC::resrat / restol, the smoothers' scale 1/dt,
+// the levels' Krylov solver (PSB_CG or PSB_BICGS)
 struct RefPar {
   int mv[3], ms[3];
   double rr, rt, scale;
+  int solver;
 };
+
+void ref_solve(psb_ctx *C, int l, int maxit, double rr, double rt, double scale, int stale,
+               const RefPar &q) {
+  if (q.solver == PSB_BICGS) ref_bicgs(C, l, maxit, rr, rt, scale, stale);
+  else ref_cg(C, l, maxit, rr, rt, scale, stale);
+}
 
 void ref_interp(psb_ctx *C, int lc) {  // level lc into lc - 1 (h.phi += H.phi)
   RefHier &H = C->href;
@@ -1757,17 +1813,17 @@ void ref_restrict(psb_ctx *C, int l) {  // residual of level l into l + 1, mean 
 }
 
 void ref_coarsest(psb_ctx *C, int l, const RefPar &q) {  // MaxIter 100, ResRat -1
-  ref_cg(C, l, 100, -1.0, q.rt, q.scale, -1);
+  ref_solve(C, l, 100, -1.0, q.rt, q.scale, -1, q);
   ref_interp(C, l);
 }
 
 // v_cycle (additive_recursive.cpp:4-34)
 void ref_v(psb_ctx *C, int l, const RefPar &q) {
   if (l != (int)C->href.L.size() - 1) {
-    ref_cg(C, l, q.mv[0], q.rr, q.rt, q.scale, q.ms[0]);
+    ref_solve(C, l, q.mv[0], q.rr, q.rt, q.scale, q.ms[0], q);
     ref_restrict(C, l);
     ref_v(C, l + 1, q);
-    ref_cg(C, l, q.mv[2], q.rr, q.rt, q.scale, q.ms[2]);
+    ref_solve(C, l, q.mv[2], q.rr, q.rt, q.scale, q.ms[2], q);
     if (l > 0) ref_interp(C, l);
   } else {
     ref_coarsest(C, l, q);
@@ -1780,14 +1836,14 @@ void ref_v(psb_ctx *C, int l, const RefPar &q) {
 // W-cycle) on the coarse phi the first one left
 void ref_fw(psb_ctx *C, int l, const RefPar &q, bool w) {
   if (l != (int)C->href.L.size() - 1) {
-    ref_cg(C, l, q.mv[0], q.rr, q.rt, q.scale, q.ms[0]);
+    ref_solve(C, l, q.mv[0], q.rr, q.rt, q.scale, q.ms[0], q);
     ref_restrict(C, l);
     ref_fw(C, l + 1, q, w);
-    ref_cg(C, l, q.mv[1], q.rr, q.rt, q.scale, q.ms[1]);
+    ref_solve(C, l, q.mv[1], q.rr, q.rt, q.scale, q.ms[1], q);
     ref_restrict(C, l);
     if (w) ref_fw(C, l + 1, q, true);
     else ref_v(C, l + 1, q);
-    ref_cg(C, l, q.mv[2], q.rr, q.rt, q.scale, q.ms[2]);
+    ref_solve(C, l, q.mv[2], q.rr, q.rt, q.scale, q.ms[2], q);
     if (l > 0) ref_interp(C, l);
   } else {
     ref_coarsest(C, l, q);
@@ -1814,7 +1870,7 @@ bool ref_flex(psb_ctx *C, const RefPar &q, int max_visits) {
   for (int v = 0; v < max_visits; v++) {
     if (l != nl - 1) {
       const int s = up ? 2 : 0;
-      ref_cg(C, l, q.mv[s], q.rr, q.rt, q.scale, q.ms[s]);
+      ref_solve(C, l, q.mv[s], q.rr, q.rt, q.scale, q.ms[s], q);
       read_state(C, C->st64);
       if (C->hst->converged) {
         if (l == 0) return true;
@@ -1884,9 +1940,11 @@ int ac_cycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_res
   if (P->loop < PSB_CYCLE_V || P->loop > PSB_CYCLE_FLEX) return PSB_EINVAL;
   if (P->loop == PSB_CYCLE_FLEX && P->max_visits < 1) return PSB_EINVAL;
   if (P->init < PSB_CYCLE_NONE || P->init > PSB_CYCLE_W) return PSB_EINVAL;
+  if (P->level_solver != PSB_CG && P->level_solver != PSB_BICGS) return PSB_EINVAL;
   for (int s = 0; s < 3; s++)
     if (P->max_stale[s] > kStaleMax) return PSB_EINVAL;
   build_ref(C);
+  if (P->level_solver == PSB_BICGS) ref_bicgs_buffers(C);
   RefHier &H = C->href;
   const Level<double> &L0 = H.L[0];
   const size_t N0 = L0.g.N;
@@ -1919,11 +1977,12 @@ int ac_cycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_res
     }
     const RefPar q{{P->max_iter[0], P->max_iter[1], P->max_iter[2]},
                    {P->max_stale[0], P->max_stale[1], P->max_stale[2]}, P->smooth_res_rat,
-                   P->smooth_res_tol, dti};
-    // the graphs hold the cycle's shape and MaxIter values; the tolerances
-    // and the scale are kernel arguments compared separately
+                   P->smooth_res_tol, dti, P->level_solver};
+    // the graphs hold the cycle's shape, MaxIter values and level solver; the
+    // tolerances and the scale are kernel arguments compared separately
     const long long key = (long long)q.mv[0] + ((long long)q.mv[1] << 16) +
-                          ((long long)q.mv[2] << 32) + ((long long)P->loop << 48);
+                          ((long long)q.mv[2] << 32) + ((long long)P->loop << 48) +
+                          ((long long)P->level_solver << 60);
     const long long fkey = key + ((long long)(P->init + 1) << 52);
     const bool same_ms = std::equal(q.ms, q.ms + 3, H.gms);
     if (H.gr.key != key || H.gr.fkey != dti || H.grr != q.rr || H.grt != q.rt || !same_ms) {
@@ -2537,6 +2596,7 @@ void psb_cycle_defaults(psb_cycle *c) {
   c->smooth_res_tol = kAtto;
   c->max_stale[0] = c->max_stale[1] = c->max_stale[2] = -1;  // additive.cpp:19
   c->max_visits = 1000;
+  c->level_solver = PSB_CG;
 }
 
 int psb_ac_vcycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_result *R) {

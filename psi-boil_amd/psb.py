@@ -85,7 +85,7 @@ class Cycle(C.Structure):
                 ("loop", C.c_int), ("init", C.c_int), ("stop_if_div", C.c_int),
                 ("priority_min_cyc", C.c_int), ("use_linf", C.c_int),
                 ("smooth_res_rat", C.c_double), ("smooth_res_tol", C.c_double),
-                ("max_stale", C.c_int * 3), ("max_visits", C.c_int)]
+                ("max_stale", C.c_int * 3), ("max_visits", C.c_int), ("level_solver", C.c_int)]
 
 
 def _cycle(**cyc):
@@ -98,6 +98,8 @@ def _cycle(**cyc):
             getattr(c, k)[:] = list(v)
         elif k in ("loop", "init") and (v is None or isinstance(v, str)):
             setattr(c, k, shapes[v])
+        elif k == "level_solver" and isinstance(v, str):
+            c.level_solver = {"cg": PSB_CG, "bicgs": PSB_BICGS}[v]
         else:
             setattr(c, k, v)
     return c

@@ -54,7 +54,7 @@ FLEX_CASES = ["bubbles16x12x8", "bubbles16"]  # <case>_flex.npz: AC::flexcycle r
 
 def load_cycles(name, kind="cycles"):
     """(phi of each run by tag, the runs' settings and logs) of <name>_<kind>.npz
-    (kind "cycles" or "flex")."""
+    (kind "cycles", "cycles_bicgs" or "flex")."""
     z = np.load(os.path.join(GOLDEN, name + "_" + kind + ".npz"), allow_pickle=False)
     meta = json.loads(str(z["meta"]))
     return {k: z[k] for k in z.files if k != "meta"}, meta["runs"]
@@ -67,4 +67,5 @@ def cycle_kwargs(run):
                 toler=-1.0, factor=run["factor"], max_cyc=run["max_cyc"],
                 min_cyc=run["min_cyc"], mv=tuple(run["mv"]), stop_if_div=run["stop_if_div"],
                 priority_min_cyc=run["priority_min_cyc"], use_linf=run["use_linf"],
-                resrat=run["resrat"], ms=tuple(run.get("ms", (-1, -1, -1))))
+                resrat=run["resrat"], ms=tuple(run.get("ms", (-1, -1, -1))),
+                solver="bicgs" if run["tag"].startswith("B") else "cg")

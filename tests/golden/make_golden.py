@@ -8,11 +8,13 @@ outputs, solver iterates after k iterations, AC level matrices, restriction /
 interpolation results and V-cycle iterates, plus the solver log lines (17
 significant digits) parsed into arrays.
 
-    python tests/golden/make_golden.py [cases] [cycles] [flex] [prec] [case names]   # needs oracle/_ref built
+    python tests/golden/make_golden.py [cases] [cycles] [cycles-bicgs] [flex] [prec] [case names]   # needs oracle/_ref built
 
 "cycles" writes <case>_cycles.npz: the reference's AC::cycle with the F, W
 and full-cycle shapes and its controls (ref_driver cycles), phi after each
-run and the run's residual log.  "flex" writes <case>_flex.npz: the same for
+run and the run's residual log.  "cycles-bicgs" writes <case>_cycles_bicgs.npz:
+six of those runs with BiCGS (Prec::di) as every level's solver, the solver
+the reference's AC hands its levels for a non-symmetric equation.  "flex" writes <case>_flex.npz: the same for
 AC::flexcycle on the level settings where its recursion ends.  "prec" writes <case>_prec.npz: CG with the
 reference's incomplete-Cholesky preconditioners IC0 and IC2 (ref_driver
 prec), x after 1 and 5 iterations and at ResRat 1e-8, and the final log
@@ -153,19 +155,22 @@ def parse_cycles(text):
     return runs
 
 
-def run_cycles(name):
+def run_cycles(name, mode="cycles"):
+    """mode "cycles": CG level solves; "cycles-bicgs": BiCGS (Prec::di) as
+    every level's solver, a subset of the runs (tags "B..."), written to
+    <case>_cycles_bicgs.npz."""
     nx, ny, nz, nsph, rmin, rmax, rin, rout, ss, sv, centre = CASES[name]
     with tempfile.TemporaryDirectory() as d:
-        args = [DRIVER, "cycles", d, str(nx), str(ny), str(nz), str(nsph), repr(rmin),
+        args = [DRIVER, mode, d, str(nx), str(ny), str(nz), str(nsph), repr(rmin),
                 repr(rmax), repr(rin), repr(rout), str(ss), str(sv)]
-        if centre is not None:
+        if centre is not None and mode == "cycles":
             args += [repr(c) for c in centre]
         out = subprocess.run(args, check=True, capture_output=True, text=True).stdout
         runs = parse_cycles(out)
         arrays = {r["tag"] + "_phi": np.fromfile(os.path.join(d, r["tag"] + "_phi.f64"))
                   for r in runs}
     arrays["meta"] = np.array(json.dumps({"name": name, "runs": runs}))
-    path = os.path.join(HERE, name + "_cycles.npz")
+    path = os.path.join(HERE, name + ("_cycles.npz" if mode == "cycles" else "_cycles_bicgs.npz"))
     np.savez_compressed(path, **arrays)
     return path
 
@@ -233,7 +238,7 @@ def run_prec(name):
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("oracle/_ref/ref_driver is not built (make -f oracle/ref.mk)")
-    what = sys.argv[1:] or ["cases", "cycles", "flex", "prec"]
+    what = sys.argv[1:] or ["cases", "cycles", "cycles-bicgs", "flex", "prec"]
     only = [w for w in what if w in CASES]  # restrict to the cases named
     if "cases" in what:
         for name, spec in CASES.items():
@@ -243,6 +248,10 @@ def main():
         for name in CYCLE_CASES:
             if not only or name in only:
                 print(run_cycles(name))
+    if "cycles-bicgs" in what:
+        for name in CYCLE_CASES:
+            if not only or name in only:
+                print(run_cycles(name, "cycles-bicgs"))
     if "flex" in what:
         for name in FLEX_RUNS:
             if not only or name in only:

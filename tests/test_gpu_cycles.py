@@ -2,9 +2,11 @@
 init / control fields) against the reference's own runs (cycles fixtures,
 written by its own build): F and W cycles, full-cycle starts, the linf
 control with its stale previous value, divergence and min-cycle controls,
-per-stage MaxIter and stale counts and the level solves' ResRat; and the
-flex cycle (flexcycle, its recursion bounded by max_visits) on the settings
-where the reference's own recursion ends.
+per-stage MaxIter and stale counts and the level solves' ResRat; six of
+those runs with BiCGS as every level's solver (the solver the reference's AC
+hands its levels for a non-symmetric equation); and the flex cycle
+(flexcycle, its recursion bounded by max_visits) on the settings where the
+reference's own recursion ends.
 
 The GPU sums dot products in a tree, the reference sequentially.  How much
 that matters is a property of each run, and it is measured: the oracle
@@ -34,8 +36,9 @@ _SHAPES = ["V", "F", "W", "flex"]
 def _runs():
     out = []
     for case in CYCLE_CASES:
-        _, runs = load_cycles(case)
-        out += [(case, "cycles", r["tag"]) for r in runs]
+        for kind in ("cycles", "cycles_bicgs"):  # CG / BiCGS level solves
+            _, runs = load_cycles(case, kind)
+            out += [(case, kind, r["tag"]) for r in runs]
     for case in FLEX_CASES:  # AC::flexcycle where the reference's recursion ends
         _, runs = load_cycles(case, "flex")
         out += [(case, "flex", r["tag"]) for r in runs]
@@ -48,7 +51,8 @@ def gpu_kwargs(run):
                 min_cycles=run["min_cyc"], max_iter=run["mv"],
                 stop_if_div=int(run["stop_if_div"]),
                 priority_min_cyc=int(run["priority_min_cyc"]), use_linf=int(run["use_linf"]),
-                smooth_res_rat=run["resrat"], max_stale=run.get("ms", (-1, -1, -1)))
+                smooth_res_rat=run["resrat"], max_stale=run.get("ms", (-1, -1, -1)),
+                level_solver="bicgs" if run["tag"].startswith("B") else "cg")
 
 
 _solvers = {}
@@ -69,6 +73,13 @@ def demeaned_rel(x, ref):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
+# BiCGS level solves: blocks down to 2 terms too, since the coarse levels
+# have fewer cells than the larger blocks and those would leave their sums
+# in the reference's order
+_VARIANTS = ((0, 0), (64, 0), (256, 0), (1024, 0), (0, 1), (0, 2))
+_VARIANTS_BICGS = _VARIANTS + ((2, 0), (4, 0), (8, 0), (16, 0), (2, 1), (4, 2))
+
+
 def oracle_variants(g, run, col):
     """The oracle's (history, phi, cycles) for the reference's order of sums
     and for blocked orders and perturbed b: the run's own spread."""
@@ -76,7 +87,7 @@ def oracle_variants(g, run, col):
     b = g["b"]
     out = []
     try:
-        for blk, seed in ((0, 0), (64, 0), (256, 0), (1024, 0), (0, 1), (0, 2)):
+        for blk, seed in (_VARIANTS_BICGS if run["tag"].startswith("B") else _VARIANTS):
             O.set_sum_block(blk)
             rhs = b if seed == 0 else \
                 b * (1.0 + 1e-15 * np.random.default_rng(seed).standard_normal(b.size))
@@ -122,6 +133,21 @@ def test_gpu_cycle_matches_reference(case, kind, tag):
             assert not x.any()
         else:
             assert demeaned_rel(x, ref) < max(1e-6, 1000.0 * xspread)
+    elif run["tag"].startswith("B") and spread.max() >= 1e-2:
+        # BiCGS level solves on the singular coarse levels amplify rounding
+        # differences 10^2-10^3 times per cycle (the reference's own runs
+        # under reordered sums part by percents within a few cycles): the
+        # GPU follows the reference within ten times that spread up to the
+        # first cycle where the spread reaches 1e-2, and beyond it only the
+        # stopping rules on its own history apply
+        k = int(np.argmax(spread >= 1e-2))
+        m = min(k + 1, len(h))
+        tol = 10.0 * np.maximum(spread[:m], 1e-6)
+        assert np.all(np.abs(h[:m] - ref_hist[:m]) <= tol * np.abs(ref_hist[:m]) + floor)
+        if r["converged"]:
+            assert h[-1] < run["factor"] * h[0]
+        elif r["status"] != psb.PSB_DIVERGED:
+            assert r["iterations"] == run["max_cyc"]
     else:
         # order-sensitive run: within ten times the reference algorithm's
         # own spread under reordered sums
@@ -181,3 +207,14 @@ def test_gpu_flex_cycle_bound():
     xo, info = O.AC(A, g["dV"], rule=0).cycle(x0, b, loop="flex", factor=1e-8, max_cyc=20,
                                               mv=(20, 20, 20), resrat=1e-3, max_visits=400)
     assert info["bound"] and info["ncyc"] == 1 and np.array_equal(xo, x0)
+
+
+def test_gpu_level_solver_is_checked():
+    """level_solver takes PSB_CG or PSB_BICGS only; the default is CG."""
+    g, s = solver("bubbles16x12x8")
+    b, n = g["b"], g["b"].size
+    with pytest.raises(psb.PsbError, match="EINVAL"):
+        s.ac_vcycle(np.zeros(n), b, res_rat=1e-8, level_solver=7)
+    x1, r1 = s.ac_vcycle(np.zeros(n), b, res_rat=1e-8, max_cycles=3)
+    x2, r2 = s.ac_vcycle(np.zeros(n), b, res_rat=1e-8, max_cycles=3, level_solver="cg")
+    assert np.array_equal(x1, x2) and r1["iterations"] == r2["iterations"]

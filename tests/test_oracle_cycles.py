@@ -3,7 +3,9 @@ cycles, full-cycle starts (fullcycle), the linf control with the reference's
 stale previous-linf value, the divergence check with and without
 priority_min_cyc, min_cycles, per-stage MaxIter and stale counts
 (stale_iter_array) and the level solves' ResRat -- phi after each run bit for bit, every logged residual (17
-significant digits) and the returned cycle count.  CPU only."""
+significant digits) and the returned cycle count; and six of those runs with
+BiCGS as every level's solver (the reference's AC with a non-symmetric
+equation's solver).  CPU only."""
 import numpy as np
 import pytest
 
@@ -14,8 +16,9 @@ from conftest import CYCLE_CASES, FLEX_CASES, cycle_kwargs, load_cycles, load_go
 def _runs():
     out = []
     for case in CYCLE_CASES:
-        _, runs = load_cycles(case)
-        out += [(case, r["tag"]) for r in runs]
+        for kind in ("cycles", "cycles_bicgs"):
+            _, runs = load_cycles(case, kind)
+            out += [(case, kind, r["tag"]) for r in runs]
     return out
 
 
@@ -24,13 +27,13 @@ def systems():
     res = {}
     for case in CYCLE_CASES:
         g = load_golden(case)
-        res[case] = (g, load_cycles(case))
+        res[case] = {kind: (g, load_cycles(case, kind)) for kind in ("cycles", "cycles_bicgs")}
     return res
 
 
-@pytest.mark.parametrize("case,tag", _runs())
-def test_cycle_matches_reference(systems, case, tag):
-    g, (phis, runs) = systems[case]
+@pytest.mark.parametrize("case,kind,tag", _runs())
+def test_cycle_matches_reference(systems, case, kind, tag):
+    g, (phis, runs) = systems[case][kind]
     run = [r for r in runs if r["tag"] == tag][0]
     A = O.Mat(*g.shape, g.mat())
     ac = O.AC(A, g["dV"], rule=0)
@@ -64,7 +67,7 @@ def test_fixtures_exercise_the_controls():
 def test_stale_counts_change_the_result(systems, tag):
     """The stale-count runs differ from the same settings with the counts off,
     so their fixtures pin where each count goes (pre-, re-, post-smoothing)."""
-    g, (phis, runs) = systems["bubbles16x12x8"]
+    g, (phis, runs) = systems["bubbles16x12x8"]["cycles"]
     run = [r for r in runs if r["tag"] == tag][0]
     kw = cycle_kwargs(run)
     A = O.Mat(*g.shape, g.mat())
@@ -113,3 +116,16 @@ def test_flex_cycle_bound_ends_a_run_the_reference_does_not_end():
                          mv=(20, 20, 20), resrat=0.1, max_visits=400)
     assert info["bound"] and not info["converged"]
     assert np.all(np.isfinite(phi))
+
+
+def test_bicgs_level_solves_change_the_result(systems):
+    """The BiCGS runs are not the CG runs under another name: the same
+    settings with CG level solves give another phi (so the fixtures pin the
+    level solver), and the oracle's BiCGS itself matches the reference's
+    single-level fixtures elsewhere (test_oracle)."""
+    g, (phis, runs) = systems["bubbles16x12x8"]["cycles_bicgs"]
+    A = O.Mat(*g.shape, g.mat())
+    for run in runs:
+        kw = dict(cycle_kwargs(run), solver="cg")
+        phi, _ = O.AC(A, g["dV"], rule=0).cycle(np.zeros(A.n), g["b"], **kw)
+        assert not np.array_equal(phi, phis[run["tag"] + "_phi"]), run["tag"]
