@@ -667,16 +667,38 @@ int ora_cg(const ora_mat *A, double *x, const double *b, const double *dV,
 /* ------------------------------------------------------------------------ */
 /* BiCGS, Src/Solver/Linear/Krylov/bicgs.cpp:9-229                           */
 /* ------------------------------------------------------------------------ */
-int ora_bicgs(const ora_mat *A, double *x, const double *b, const double *dV,
-              int mini, int maxit, double res_rat, double res_tol, double scale,
-              int stalecount, double *hist, ora_result *out) {
+/* z = M^-1 r of a BiCGS step: the diagonal (Diagonal::solve,
+ * preconditioner_solve.cpp, as r / c) when ac is NULL, else the multigrid
+ * cycle on level 0 (the V-cycle: the K-cycle's steps change the operator
+ * from one application to the next) */
+static int kcycle_mask = 0; /* levels whose cycle is the K-cycle (ora_set_kcycle) */
+
+typedef struct {
+  const double *Mc;
+  ora_ac *ac;
+  int nu1, nu2;
+  double omega, cfac;
+} bicgs_prec;
+
+static void bicgs_apply(const bicgs_prec *M, size_t N, const double *r, double *z) {
+  if (!M->ac) {
+    for (size_t P = 0; P < N; P++) z[P] = r[P] * M->Mc[P];
+    return;
+  }
+  const int keep = kcycle_mask;
+  kcycle_mask = 0;
+  ora_ac_precond(M->ac, 0, r, z, M->nu1, M->nu2, M->omega, M->cfac);
+  kcycle_mask = keep;
+}
+
+static int bicgs_run(const ora_mat *A, const bicgs_prec *M, double *x, const double *b,
+                     const double *dV, int mini, int maxit, double res_rat, double res_tol,
+                     double scale, int stalecount, double *hist, ora_result *out) {
   const size_t N = mat_n(A);
   double *p = dalloc(N), *r = dalloc(N), *s = dalloc(N), *t = dalloc(N),
-         *v = dalloc(N), *p_ = dalloc(N), *r_ = dalloc(N), *s_ = dalloc(N),
-         *Mc = dalloc(N);
+         *v = dalloc(N), *p_ = dalloc(N), *r_ = dalloc(N), *s_ = dalloc(N);
   double *win = stalecount > 0 ? dalloc((size_t)stalecount) : NULL;
   double alfa = 0.0, beta, omega = 1.0, rho, rho_old = 1.0, r_v, ss, ts, tt;
-  for (size_t P = 0; P < N; P++) Mc[P] = 1.0 / A->c[P];
   for (int q2 = 0; q2 < stalecount; q2++) win[q2] = UNREAL;
 
   ora_resid(A, b, x, r);
@@ -702,7 +724,7 @@ int ora_bicgs(const ora_mat *A, double *x, const double *b, const double *dV,
       double bo = beta * omega;
       for (size_t P = 0; P < N; P++) p[P] -= bo * v[P];
     }
-    for (size_t P = 0; P < N; P++) p_[P] = p[P] * Mc[P];
+    bicgs_apply(M, N, p, p_);
     ora_spmv(A, p_, v);
     r_v = ora_dot(N, r_, v);
     alfa = rho / r_v;
@@ -713,7 +735,7 @@ int ora_bicgs(const ora_mat *A, double *x, const double *b, const double *dV,
       R.reason = 5;
       break;
     }
-    for (size_t P = 0; P < N; P++) s_[P] = s[P] * Mc[P];
+    bicgs_apply(M, N, s, s_);
     ora_spmv(A, s_, t);
     ts = ora_dot(N, t, s);
     tt = ora_dot(N, t, t);
@@ -734,8 +756,21 @@ int ora_bicgs(const ora_mat *A, double *x, const double *b, const double *dV,
 done:
   if (out) *out = R;
   free(p); free(r); free(s); free(t); free(v); free(p_); free(r_); free(s_);
-  free(Mc); free(win);
+  free(win);
   return R.converged;
+}
+
+int ora_bicgs(const ora_mat *A, double *x, const double *b, const double *dV,
+              int mini, int maxit, double res_rat, double res_tol, double scale,
+              int stalecount, double *hist, ora_result *out) {
+  const size_t N = mat_n(A);
+  double *Mc = dalloc(N);
+  for (size_t P = 0; P < N; P++) Mc[P] = 1.0 / A->c[P];
+  const bicgs_prec M = {Mc, NULL, 0, 0, 0.0, 0.0};
+  const int ok = bicgs_run(A, &M, x, b, dV, mini, maxit, res_rat, res_tol, scale, stalecount,
+                           hist, out);
+  free(Mc);
+  return ok;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1449,7 +1484,6 @@ static void zline_sweep(const ora_mat *A, const double *r, double *e, double *t,
 /* levels whose coarse correction is two steps of flexible CG preconditioned
  * by the cycle below (the K-cycle of Notay and Vassilevski, Numer. Linear
  * Algebra Appl. 15 (2008) 473-487): bit l set for level l */
-static int kcycle_mask = 0;
 void ora_set_kcycle(int mask) { kcycle_mask = mask; }
 
 /* out ~ A_l^-1 r: c = B r, v = A c, alpha1 = c.r / c.v, rt = r - alpha1 v,
@@ -1646,6 +1680,19 @@ done:
   if (iters) *iters = it;
   free(r); free(z); free(p); free(q);
   return conv;
+}
+
+/* BiCGS (bicgs.cpp:9-229 steps and stopping rules) with the V-cycle of the
+ * hierarchy as its preconditioner in place of the diagonal: not a reference
+ * configuration (its BiCGS takes a diagonal or IC preconditioner) */
+int ora_acbicgs(ora_ac *ac, double *x, const double *b, const double *dV, int mini,
+                int maxit, double res_rat, double res_tol, double scale, int stalecount,
+                int nu1, int nu2, double omega, double cfac, double *hist,
+                ora_result *out) {
+  if (!ac->G) ora_ac_coarse_dense(ac);
+  const bicgs_prec M = {NULL, ac, nu1, nu2, omega, cfac};
+  return bicgs_run(&ac->A[0], &M, x, b, dV, mini, maxit, res_rat, res_tol, scale,
+                   stalecount, hist, out);
 }
 
 int ora_pcg_rel(const ora_mat *A, double *x, const double *b, double rtol,

@@ -227,6 +227,12 @@ int  ora_acpcg(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
 int  ora_acpcg_s(ora_ac *ac, double *x, const double *b, double rtol, int maxit,
                  int nu1, int nu2, double omega, double cfac, int smoother,
                  double *hist, int *iters);
+/* BiCGS with the reference's steps and stopping rules (bicgs.cpp:9-229, as
+ * ora_bicgs) and the V-cycle (nu1, nu2, omega, cfac) as its preconditioner */
+int  ora_acbicgs(ora_ac *ac, double *x, const double *b, const double *dV, int mini,
+                 int maxit, double res_rat, double res_tol, double scale, int stalecount,
+                 int nu1, int nu2, double omega, double cfac, double *hist,
+                 ora_result *out);
 /* Diagonal PCG with the same relative-2-norm stopping rule. */
 int  ora_pcg_rel(const ora_mat *A, double *x, const double *b, double rtol,
                  int maxit, double *hist, int *iters);

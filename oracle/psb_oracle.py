@@ -408,6 +408,18 @@ class AC:
             out["orth"] = ob[:max(0, it.value - 1)]
         return x, out
 
+    def acbicgs(self, x, b, dV, nu1=2, nu2=2, omega=0.8, cfac=2.0, **kw):
+        """BiCGS (bicgs.cpp:9-229 stopping rules, keywords as bicgs) with the
+        V-cycle as its preconditioner."""
+        if not self.ac.G:
+            self.coarse_dense()
+
+        def fn(A, xp, bp, dVp, mini, maxit, rr, rt, sc, stale, hp, R):
+            return lib().ora_acbicgs(C.byref(self.ac), xp, bp, dVp, mini, maxit, rr, rt, sc,
+                                     stale, nu1, nu2, C.c_double(omega), C.c_double(cfac),
+                                     hp, R)
+        return _krylov(fn, self._keep[0], x, b, dV, **kw)
+
 
 def pcg_rel(A, x, b, rtol=1e-8, maxit=1000):
     x = np.array(x, dtype=np.float64, copy=True)

@@ -776,7 +776,7 @@ void set_half(psb_ctx *C, bool on) {
 // ---------------------------------------------------------------------------
 template <typename T>
 T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega, T cfac,
-          int done = 0);
+          int done = 0, bool rho0 = true);
 
 // K-cycle levels chosen from the hierarchy (psb_stop.ac_kcycle = -1).
 // Halving every direction: every level but the finest and the coarsest with
@@ -917,10 +917,11 @@ T *kstep(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega,
 }
 
 // done: pre-sweeps already applied beyond the first (whose result is in
-// lv.e[done])
+// lv.e[done]); rho0: level 0's last post-sweep forms the CG's rho partials
+// (false: a plain sweep, for a Krylov method with other scalars)
 template <typename T>
 T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega, T cfac,
-          int done) {
+          int done, bool rho0) {
   Level<T> &lv = H.L[l];
   const int nl = (int)H.L.size();
   if (l == nl - 1) {
@@ -944,15 +945,16 @@ T *vcycle(psb_ctx *C, Hier<T> &H, int l, DevState *st, int nu1, int nu2, T omega
   const T cfac_l = T((kc ? 1.0 : (double)cfac) * (cL.scale / lv.scale));
   if (nu2 == 2) {  // both post-sweeps in one pass where the level suits it
     const int out = (cur + 2) % 3;
-    const bool done = l == 0 ? post2<T, true>(C, lv, lv.e[cur], lv.e[out], cL, zH, cfac_l, omega,
-                                              T(lv.scale), st, H.qf)
+    const bool done = l == 0 && rho0
+                          ? post2<T, true>(C, lv, lv.e[cur], lv.e[out], cL, zH, cfac_l, omega,
+                                           T(lv.scale), st, H.qf)
                              : post2<T, false>(C, lv, lv.e[cur], lv.e[out], cL, zH, cfac_l,
                                                omega, T(1), st);
     if (done) return lv.e[out];
   }
   for (int s = 0; s < nu2; s++) {
     const int nxt = (cur + 1) % 3;
-    const bool last = (s == nu2 - 1) && l == 0;
+    const bool last = (s == nu2 - 1) && l == 0 && rho0;
     const Level<T> *co = s == 0 ? &cL : nullptr;
     const T *eH = s == 0 ? zH : nullptr;
     if (last)
@@ -1386,7 +1388,8 @@ void solve_bicgs_diag(psb_ctx *C, double *x, const double *b, const psb_stop *p,
 // formed, e1 in level 0's e[0], st initialised).  Runs to st->done.
 // ---------------------------------------------------------------------------
 // the graph key of an AC-preconditioned solve with the context's current
-// settings (body: 0 FP64, 1 FP32 inner solve, 2 reliable updates)
+// settings (body: 0 FP64, 1 FP32 inner solve, 2 reliable updates, 3 FP64
+// BiCGS)
 GKey solve_key(const psb_ctx *C, const psb_stop *p, int body, const void *x, const void *b) {
   GKey k;
   k.body = body;
@@ -1396,7 +1399,7 @@ GKey solve_key(const psb_ctx *C, const psb_stop *p, int body, const void *x, con
   k.kmask = C->kmask;
   k.fuse = C->fuse;
   k.cg16 = C->cg_half;
-  k.half = body == 0 ? 0 : (C->h32.L.empty() ? 0 : C->h32.L[0].use16);
+  k.half = body == 0 || body == 3 ? 0 : (C->h32.L.empty() ? 0 : C->h32.L[0].use16);
   k.flex = C->flex;
   k.omega = p->ac_omega;
   k.cfac = p->ac_cfac;
@@ -1439,6 +1442,82 @@ void solve_ac64(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_r
   HIPCHK(hipMemcpyAsync(x, K.x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, C->stream));
   finish_result(R, *C->hst, 0);
   C->hist_count = std::min(C->hst->iter + 1, C->hist_cap);
+}
+
+// ---------------------------------------------------------------------------
+// BiCGS preconditioned by the multigrid V-cycle, FP64 (psb_stop.method =
+// PSB_BICGS with PSB_PREC_AC): the steps of bicgs.cpp:79-229 with the cycle
+// as M^-1, applied to p and to s (level 0's right-hand side pointed at each
+// in turn, its first sweep formed by the pass before).  The K-cycle's steps
+// change the preconditioner from one application to the next, which
+// BiCGStab does not allow for, so the levels run the V-cycle whatever
+// ac_kcycle says.
+// ---------------------------------------------------------------------------
+void solve_acbicgs64(psb_ctx *C, double *x, const double *b, const psb_stop *p,
+                     psb_result *R) {
+  KrylovVecs<double> &K = C->k64;
+  ensure_krylov(C, K, true);
+  if (!K.z) K.z = C->arena.alloc<double>(C->g.N);
+  build_hier64(C);
+  ensure_hist(C, p->max_iter);
+  Hier<double> &H = C->h64;
+  const Grid &g = C->g;
+  const DMat<double> A = mat64(C);
+  HIPCHK(hipMemcpyAsync(K.x, x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, C->stream));
+  init_state(C, C->st64, p, p->norm, p->res_rat, p->res_tol, p->max_iter);
+  for (int q = 0; q < 2; q++) {
+    zero(C, K.p[q], g.N);
+    zero(C, K.v[q], g.N);
+  }
+  start<double, true, 2>(C, g, A, b, K.x, K.r, K.rt, 0.0, C->st64);
+  HIPCHK(hipGetLastError());
+  read_state(C, C->st64);
+  const int chunk = chunk_of(p);
+  const int nb = nblocks<double>(g);
+  const int nu1 = p->ac_pre, nu2 = p->ac_post;
+  const double omega = p->ac_omega, cfac = p->ac_cfac;
+  DevState *st = C->st64;
+  H.qf = nullptr;
+  auto body = [&]() {
+    Level<double> &f = H.L[0];
+    double *r0 = f.r;
+    const double omj = omega / f.scale;
+    for (int it = 0; it < chunk; it++) {
+      const int par = it & 1;
+      with_v<double>(g, [&](auto vt) {
+        constexpr int V = decltype(vt)::value;
+        hipLaunchKernelGGL((k_abi_p<double, V>), dim3(nb), dim3(kBlock), 0, C->stream, g, f.c,
+                           K.r, K.p[par], K.v[par], K.p[par ^ 1], f.e[0], omj, st);
+      });
+      f.r = K.p[par ^ 1];
+      const double *z1 = vcycle<double>(C, H, 0, st, nu1, nu2, omega, cfac, 0, false);
+      with_v<double>(g, [&](auto vt) {
+        constexpr int V = decltype(vt)::value;
+        hipLaunchKernelGGL((k_abi_v<double, V, true>), dim3(nb), dim3(kBlock), 0, C->stream, g,
+                           A, z1, K.rt, K.v[par ^ 1], K.z, st, C->work);
+        finalize<1>(C, FIN_BI1, nb, st);
+        hipLaunchKernelGGL((k_abi_s<double, V>), dim3(nb), dim3(kBlock), 0, C->stream, g, f.c,
+                           K.r, K.v[par ^ 1], K.s, f.e[0], omj, st);
+      });
+      f.r = K.s;
+      const double *z2 = vcycle<double>(C, H, 0, st, nu1, nu2, omega, cfac, 0, false);
+      with_v<double>(g, [&](auto vt) {
+        constexpr int V = decltype(vt)::value;
+        hipLaunchKernelGGL((k_abi_t<double, V, true>), dim3(nb), dim3(kBlock), 0, C->stream, g,
+                           A, z2, K.s, K.t, st, C->work);
+        finalize<3>(C, FIN_BI2, nb, st);
+        hipLaunchKernelGGL((k_abi_x<double, V>), dim3(nb), dim3(kBlock), 0, C->stream, g,
+                           metric(C), K.z, z2, K.s, K.t, K.rt, K.x, K.r, st, C->work);
+        finalize<3>(C, FIN_BI3, nb, st);
+      });
+      f.r = r0;
+    }
+  };
+  Graph &gr = graph_for(C, solve_key(C, p, 3, nullptr, nullptr));
+  run_graph(C, gr, 0, st, p->max_iter, body);
+  HIPCHK(hipMemcpyAsync(x, K.x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, C->stream));
+  finish_result(R, *C->hst, 0);
+  C->hist_count = std::min(C->hst->reason == 7 ? 1 : C->hst->iter + 1, C->hist_cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -2068,6 +2147,20 @@ int ac_cycle(psb_ctx *C, double *x, const double *b, const psb_cycle *P, psb_res
 int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_result *R) {
   C->hist_on_host = false;
   if (p->precond == PSB_PREC_AC) {
+    if (p->method == PSB_BICGS) {  // the V-cycle as BiCGS's preconditioner, FP64
+      if (p->precision != PSB_FP64 || !C->sym || p->ac_pre < 1 || p->ac_post < 1 ||
+          p->ac_coarsen < 0 || p->ac_coarsen > 1)
+        return PSB_EINVAL;
+      ensure_rule(C, p->ac_coarsen);
+      C->cg_half = false;
+      C->fuse = false;
+      C->kmask = 0;
+      C->flex = false;
+      C->ac_omega = p->ac_omega;
+      solve_acbicgs64(C, x, b, p, R);
+      R->rel_l2 = true_rel(C, b, x);
+      return R->status;
+    }
     if (p->method != PSB_CG || !C->sym || p->ac_pre < 1 || p->ac_post < 1) return PSB_EINVAL;
     if (p->ac_coarsen < 0 || p->ac_coarsen > 1) return PSB_EINVAL;
     if (p->ac_store < 0 || p->ac_store > 2) return PSB_EINVAL;

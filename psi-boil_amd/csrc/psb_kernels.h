@@ -728,6 +728,137 @@ __global__ __launch_bounds__(kBlock) void k_bi3(Grid g, const T *c, Metric m, co
 }
 
 // ---------------------------------------------------------------------------
+// BiCGS preconditioned by the multigrid cycle (not in the reference, whose
+// BiCGS takes a diagonal or IC preconditioner): the steps of bicgs.cpp:79-229
+// with prec->solve replaced by the cycle, applied to p and to s.  The passes
+// around the two cycle applications; the scalar steps are FIN_BI1..3.
+// A1: p = r + beta (p_old - omega v_old) (p = r at i = 0), as k_bi1 forms
+//     it, and the level-0 first sweep e0 = omj p / c of the cycle on p
+// ---------------------------------------------------------------------------
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void k_abi_p(Grid g, const T *c, const T *r,
+                                                  const T *pold, const T *vold, T *pnew, T *e0,
+                                                  T omj, const DevState *st) {
+  if (st->done) return;
+  const bool first = st->iter == 0;
+  const T beta = first ? T(0) : (T)st->beta;
+  const T bo = first ? T(0) : (T)(st->beta * st->omega);
+  PSB_TILE_BEGIN(g, V)
+  const VecN<T, V> R = ldv<T, V>(r + P), Pp = ldv<T, V>(pold + P), Vv = ldv<T, V>(vold + P),
+                   Cc = ldv<T, V>(c + P);
+  VecN<T, V> Pn, E;
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    const T t = R.v[u] + beta * Pp.v[u];
+    Pn.v[u] = first ? R.v[u] : t - bo * Vv.v[u];
+    E.v[u] = omj * Pn.v[u] / Cc.v[u];
+  }
+  stv<T, V>(pnew + P, Pn);
+  stv<T, V>(e0 + P, E);
+  PSB_TILE_END
+}
+
+// A2: v = A z1 with z1 = M^-1 p (kept in zs for the x update), partial r~.v
+template <typename T, int V, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_abi_v(Grid g, DMat<T> A, const T *z1, const T *rt,
+                                                  T *v, T *zs, DevState *st, Work w) {
+  if (st->done) return;
+  const ArrX<T, V> X{z1};
+  double acc[1] = {0.0};
+  PSB_TILE_BEGIN(g, V)
+  const VecN<T, V> Rt = ldv<T, V>(rt + P);
+  const Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+  const VecN<T, V> Vv = apply_A<T, V>(G);
+#pragma unroll
+  for (int u = 0; u < V; u++) acc[0] += (double)(Rt.v[u] * Vv.v[u]);
+  stv<T, V>(v + P, Vv);
+  stv<T, V>(zs + P, G.x);
+  PSB_TILE_END
+  block_partial<1>(acc, w.partial);
+}
+
+// A3: s = r - alpha v, and the level-0 first sweep e0 = omj s / c of the
+// cycle on s
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void k_abi_s(Grid g, const T *c, const T *r, const T *v,
+                                                  T *s, T *e0, T omj, const DevState *st) {
+  if (st->done) return;
+  const T alpha = (T)st->alpha;
+  PSB_TILE_BEGIN(g, V)
+  const VecN<T, V> R = ldv<T, V>(r + P), Vv = ldv<T, V>(v + P), Cc = ldv<T, V>(c + P);
+  VecN<T, V> S, E;
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    S.v[u] = R.v[u] - alpha * Vv.v[u];
+    E.v[u] = omj * S.v[u] / Cc.v[u];
+  }
+  stv<T, V>(s + P, S);
+  stv<T, V>(e0 + P, E);
+  PSB_TILE_END
+}
+
+// A4: t = A z2 with z2 = M^-1 s; partials s.s, t.s, t.t (FIN_BI2's order)
+template <typename T, int V, bool SYM>
+__global__ __launch_bounds__(kBlock) void k_abi_t(Grid g, DMat<T> A, const T *z2, const T *s,
+                                                  T *t, DevState *st, Work w) {
+  if (st->done) return;
+  const ArrX<T, V> X{z2};
+  double acc[3] = {0.0, 0.0, 0.0};
+  PSB_TILE_BEGIN(g, V)
+  const VecN<T, V> S = ldv<T, V>(s + P);
+  const Gath<T, V> G = gather<T, V, SYM>(g, A, X, P, i, j, k);
+  const VecN<T, V> Tt = apply_A<T, V>(G);
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    acc[0] += (double)(S.v[u] * S.v[u]);
+    acc[1] += (double)(Tt.v[u] * S.v[u]);
+    acc[2] += (double)(Tt.v[u] * Tt.v[u]);
+  }
+  stv<T, V>(t + P, Tt);
+  PSB_TILE_END
+  block_partial<3>(acc, w.partial);
+}
+
+// A5: x += alpha z1; x += omega z2; r = s - omega t; partials of the norms
+// and rho_next = r~.r; with a small s only x += alpha z1 (as k_bi3)
+template <typename T, int V>
+__global__ __launch_bounds__(kBlock) void k_abi_x(Grid g, Metric m, const T *z1, const T *z2,
+                                                  const T *s, const T *t, const T *rt, T *x,
+                                                  T *r, DevState *st, Work w) {
+  if (st->done) return;
+  const bool small_s = sqrt(st->rv_ss) < st->res_tol;
+  const T alpha = (T)st->alpha, omega = (T)st->omega;
+  double acc[3] = {0.0, 0.0, 0.0};  // voldiv, r^2, rho_next
+  PSB_TILE_BEGIN(g, V)
+  VecN<T, V> X = ldv<T, V>(x + P);
+  const VecN<T, V> Z1 = ldv<T, V>(z1 + P);
+  if (small_s) {
+#pragma unroll
+    for (int u = 0; u < V; u++) X.v[u] = X.v[u] + alpha * Z1.v[u];
+    stv<T, V>(x + P, X);
+  } else {
+    const VecN<T, V> Z2 = ldv<T, V>(z2 + P), S = ldv<T, V>(s + P), Tt = ldv<T, V>(t + P),
+                     Rt = ldv<T, V>(rt + P);
+    VecN<T, V> R;
+#pragma unroll
+    for (int u = 0; u < V; u++) {
+      X.v[u] = X.v[u] + alpha * Z1.v[u];
+      X.v[u] = X.v[u] + omega * Z2.v[u];
+      R.v[u] = S.v[u] - omega * Tt.v[u];
+      const double rd = (double)R.v[u];
+      const double dv = m.dV(i, j, k + u);
+      acc[0] += rd * rd / dv / dv;
+      acc[1] += rd * rd;
+      acc[2] += (double)(Rt.v[u] * R.v[u]);
+    }
+    stv<T, V>(x + P, X);
+    stv<T, V>(r + P, R);
+  }
+  PSB_TILE_END
+  block_partial<3>(acc, w.partial);
+}
+
+// ---------------------------------------------------------------------------
 // pass 2 of the reductions: one block sums the partials of the kernel
 // before it (fixed order) and applies the scalar step
 // ---------------------------------------------------------------------------

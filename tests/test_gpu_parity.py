@@ -185,6 +185,46 @@ def test_acpcg_fp64_matches_oracle(shape):
     assert rel(x - x.mean(), xo - xo.mean()) < 1e-6
 
 
+@pytest.mark.parametrize("case", ["bubbles16", "gen32"])
+def test_ac_bicgs_fp64_matches_oracle(case):
+    """BiCGS with the V-cycle as its preconditioner (FP64, method PSB_BICGS
+    with PSB_PREC_AC) against the oracle's restatement: the first iterates
+    and residual histories to the order of the dot-product sums, the
+    converged solve within an iteration and to the stopping rule's
+    accuracy."""
+    if case == "bubbles16":
+        g = load_golden(case)
+        s = solver_for(g)
+        A = O.Mat(*g.shape, g.mat())
+        b, dV = g["b"], g["dV"]
+    else:
+        d, A = ac_case(32, nsph=5, seed=7)
+        s = psb.Solver(32, 32, 32)
+        s.set_matrix(d)
+        b, dV = d["rhs"], np.ones(d["rhs"].size)
+    n = b.size
+    ac = O.AC(A, dV, rule=1)
+    kw = dict(precond=psb.PSB_PREC_AC, method=psb.PSB_BICGS, res_tol=0.0)
+    okw = dict(nu1=2, nu2=2, omega=0.9, cfac=2.0, res_tol=0.0)
+    for k in (1, 3):
+        x, r = s.solve(np.zeros(n), b, max_iter=k, res_rat=1e-14, **kw)
+        xo, io = ac.acbicgs(np.zeros(n), b, dV, maxit=k, res_rat=1e-14, **okw)
+        assert r["iterations"] == io["iters"] == k + 1
+        assert rel(x, xo) < 1e-10, (k, rel(x, xo))
+        assert np.allclose(s.history(), io["hist"], rtol=1e-9, atol=0)
+    x, r = s.solve(np.zeros(n), b, max_iter=300, res_rat=1e-10, **kw)
+    xo, io = ac.acbicgs(np.zeros(n), b, dV, maxit=300, res_rat=1e-10, **okw)
+    assert r["converged"] and io["converged"] and r["status"] == psb.PSB_OK
+    assert abs(r["iterations"] - io["iters"]) <= 1, (r["iterations"], io["iters"])
+    assert r["iterations"] < 40
+    assert rel(x - x.mean(), xo - xo.mean()) < 1e-7
+    h = s.history()
+    assert len(h) == r["iterations"] + 1 and h[-1] < 1e-10 * h[0]
+    # only FP64, on a symmetric matrix
+    with pytest.raises(RuntimeError):
+        s.solve(np.zeros(n), b, precision=psb.PSB_MIXED, **kw)
+
+
 def test_mixed_precision_refinement_reaches_fp64_accuracy():
     d, A = ac_case(48, nsph=6, seed=3)
     b = d["rhs"]
