@@ -385,7 +385,9 @@ void psb_cycle_defaults(psb_cycle *c);
  * returns; PSB_DIVERGED when a cycle did not reduce the residual (x then
  * holds the previous cycle's phi, as the reference restores it).
  * psb_history gives the residual before the first cycle, after the
- * full-cycle start (when init is set) and after every cycle.              */
+ * full-cycle start (when init is set) and after every cycle.  A
+ * non-symmetric matrix (7 arrays) keeps 7 arrays on every level, as the
+ * reference's coarsen_system forms them (additive_coarsen_system.cpp).   */
 int psb_ac_vcycle(psb_ctx *ctx, double *x, const double *b, const psb_cycle *c,
                   psb_result *res);
 int psb_ac_vcycle_device(psb_ctx *ctx, double *d_x, const double *d_b,

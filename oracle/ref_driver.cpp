@@ -14,6 +14,7 @@
 //              given in units of h = 1 / max(nx, ny, nz))
 //   ref_driver cycles OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel [cx cy cz]
 //   ref_driver cycles-bicgs OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel
+//   ref_driver cycles-nonsym OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel
 //   ref_driver bck OUT IN.bck nx ny nz   (Scalar::load of IN, Scalar::save of a seeded field)
 //   ref_driver bench-cg nx nsph rmin rmax rho_in rho_out seed_sph seed_vel maxit
 //   ref_driver bench-ac nx nsph rmin rmax rho_in rho_out seed_sph seed_vel cycles
@@ -442,6 +443,34 @@ void dump_cycles(World &w, bool bicgs = false) {
   }
 }
 
+// a non-symmetric system in the reference's own storage, as the equations
+// that hold BiCGS make (EnthalpyFD's couplings differ on the two sides of a
+// face near an interface, and a time term adds to the diagonal): the east
+// and top couplings of each interior cell scaled by 1 + fe, 1 + ft (fixed
+// patterns of cell indices in [0, 0.25]), the diagonal raised by the added
+// coupling and by 5 % of itself, ci = 1 / c.  The matrix is written as
+// N_A_<c,w,e,s,n,b,t>.f64.
+void make_nonsymmetric(CasePressure &pr) {
+  Matrix &A = pr.A;
+  const Scalar &p = pr.phi;
+  for (int i = p.si(); i <= p.ei(); i++)
+    for (int j = p.sj(); j <= p.ej(); j++)
+      for (int k = p.sk(); k <= p.ek(); k++) {
+        const int ii = i - p.si(), jj = j - p.sj(), kk = k - p.sk();
+        const double fe = 0.025 * ((7 * ii + 3 * jj + 5 * kk) % 11);
+        const double ft = 0.25 / 6.0 * ((2 * ii + 5 * jj + 3 * kk) % 7);
+        const double de = fe * A.e[i][j][k], dt = ft * A.t[i][j][k];
+        A.e[i][j][k] += de;
+        A.t[i][j][k] += dt;
+        A.c[i][j][k] += de + dt;
+        A.c[i][j][k] += 0.05 * A.c[i][j][k];
+        A.ci[i][j][k] = 1.0 / A.c[i][j][k];
+      }
+  dump("N_A_c", interior(A.c)); dump("N_A_w", interior(A.w)); dump("N_A_e", interior(A.e));
+  dump("N_A_s", interior(A.s)); dump("N_A_n", interior(A.n)); dump("N_A_b", interior(A.b));
+  dump("N_A_t", interior(A.t));
+}
+
 // AC::flexcycle (additive.h:54-95, additive_recursive.cpp:161-200): from phi
 // = 0, the level solves' ResRat given (the recursion climbs back only when a
 // level's smoother meets it); tag, cycles and phi as dump_cycles writes them
@@ -539,6 +568,18 @@ int main(int argc, char **argv) {
     outdir = argv[2];
     Case cs = parse_case(argv + 6, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
     World w = build_world(cs, [](const Domain &d) -> Krylov * { return new BiCGS(d, Prec::di()); });
+    dump_cycles(w, true);
+    return 0;
+  }
+  if (mode == "cycles-nonsym") {
+    // ref_driver cycles-nonsym OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel:
+    // the cycles-bicgs runs on the case's matrix made non-symmetric
+    // (make_nonsymmetric), which AC coarsens at the start of every cycle
+    if (argc < 13) { std::fprintf(stderr, "cycles-nonsym: too few arguments\n"); return 2; }
+    outdir = argv[2];
+    Case cs = parse_case(argv + 6, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
+    World w = build_world(cs, [](const Domain &d) -> Krylov * { return new BiCGS(d, Prec::di()); });
+    make_nonsymmetric(*w.pr);
     dump_cycles(w, true);
     return 0;
   }

@@ -642,7 +642,8 @@ void build_hier64(psb_ctx *C) {
       lv.r = C->harena.alloc<double>(N);
       Level<double> &f = H.L[l - 1];
       hipLaunchKernelGGL(k_galerkin, dim3(grid_blocks(N)), dim3(kBlock), 0, stream, f.g,
-                         f.mat(), lv.g, lv.CI, lv.CJ, lv.CK, lv.c, lv.w, lv.s, lv.b);
+                         f.mat(), lv.g, lv.CI, lv.CJ, lv.CK, lv.c, lv.w, lv.s, lv.b, nullptr,
+                         nullptr, nullptr);
       if (f.aflag) {
         lv.aflag = C->harena.alloc<uint8_t>(N);
         hipLaunchKernelGGL(k_coarsen_flag, dim3(grid_blocks(N)), dim3(kBlock), 0, stream, f.g,
@@ -1727,7 +1728,6 @@ constexpr double kAtto = 1.0e-18;  // Src/Global/global_constants.cpp:19
 void build_ref(psb_ctx *C) {
   RefHier &H = C->href;
   if (H.built) return;
-  if (!C->sym) throw std::invalid_argument("the AC cycle needs a symmetric matrix");
   hipStream_t stream = C->stream;
   std::vector<int> d = level_dims(C->g.nx, C->g.ny, C->g.nz, 0, 2);
   const int nl = (int)d.size() / 3;
@@ -1747,6 +1747,7 @@ void build_ref(psb_ctx *C) {
     const size_t N = lv.g.N;
     if (l == 0) {
       lv.c = C->A64[0]; lv.w = C->A64[1]; lv.s = C->A64[2]; lv.b = C->A64[3];
+      if (!C->sym) { lv.ae = C->A64[4]; lv.an = C->A64[5]; lv.at = C->A64[6]; }
       lv.aflag = C->aflag0;
     } else {
       Level<double> &f = H.L[l - 1];
@@ -1755,8 +1756,13 @@ void build_ref(psb_ctx *C) {
       lv.CK = f.g.nz / lv.g.nz;
       lv.c = C->harena.alloc<double>(N); lv.w = C->harena.alloc<double>(N);
       lv.s = C->harena.alloc<double>(N); lv.b = C->harena.alloc<double>(N);
+      if (!C->sym) {  // a non-symmetric matrix keeps 7 arrays on every level
+        lv.ae = C->harena.alloc<double>(N); lv.an = C->harena.alloc<double>(N);
+        lv.at = C->harena.alloc<double>(N);
+      }
       hipLaunchKernelGGL(k_galerkin, dim3(grid_blocks(N)), dim3(kBlock), 0, stream, f.g,
-                         f.mat(), lv.g, lv.CI, lv.CJ, lv.CK, lv.c, lv.w, lv.s, lv.b);
+                         f.mat(), lv.g, lv.CI, lv.CJ, lv.CK, lv.c, lv.w, lv.s, lv.b, lv.ae,
+                         lv.an, lv.at);
       if (f.aflag) {
         lv.aflag = C->harena.alloc<uint8_t>(N);
         hipLaunchKernelGGL(k_coarsen_flag, dim3(grid_blocks(N)), dim3(kBlock), 0, stream, f.g,
@@ -1791,6 +1797,7 @@ void build_ref(psb_ctx *C) {
 
 // Linear::solve of one level by CG with the diagonal preconditioner
 // (call_smoother / call_solver): phi is improved in place
+template <bool SYM>
 void ref_cg(psb_ctx *C, int l, int maxit, double res_rat, double res_tol, double scale,
             int stalecount) {
   RefHier &H = C->href;
@@ -1802,10 +1809,10 @@ void ref_cg(psb_ctx *C, int l, int maxit, double res_rat, double res_tol, double
                      scale, (double)g.N, stalecount);
   zero(C, p0, g.N);
   zero(C, p1, g.N);
-  start<double, true, 0>(C, g, lv.mat(), lv.r, lv.e[0], r, nullptr, 0.0, st, &H.M[l]);
+  start<double, SYM, 0>(C, g, lv.mat(), lv.r, lv.e[0], r, nullptr, 0.0, st, &H.M[l]);
   for (int it = 0; it < maxit; it++) {
     double *pold = (it & 1) ? p1 : p0, *pnew = (it & 1) ? p0 : p1;
-    cg_pq<double, true, true>(C, g, lv.mat(), r, pold, pnew, q, nullptr, st);
+    cg_pq<double, SYM, true>(C, g, lv.mat(), r, pold, pnew, q, nullptr, st);
     cg_xr<double, true>(C, g, lv.c, pnew, q, lv.e[0], r, nullptr, 0.0, st, -1, &H.M[l]);
   }
 }
@@ -1823,6 +1830,7 @@ void ref_bicgs_buffers(psb_ctx *C) {
     while (H.K[l].size() < 8) H.K[l].push_back(C->harena.alloc<double>(H.L[l].g.N));
 }
 
+template <bool SYM>
 void ref_bicgs(psb_ctx *C, int l, int maxit, double res_rat, double res_tol, double scale,
                int stalecount) {
   RefHier &H = C->href;
@@ -1839,16 +1847,16 @@ void ref_bicgs(psb_ctx *C, int l, int maxit, double res_rat, double res_tol, dou
     zero(C, p[q], g.N);
     zero(C, v[q], g.N);
   }
-  start<double, true, 2>(C, g, A, lv.r, lv.e[0], r, rt, 0.0, st, &H.M[l]);
+  start<double, SYM, 2>(C, g, A, lv.r, lv.e[0], r, rt, 0.0, st, &H.M[l]);
   const int nb = nblocks<double>(g);
   for (int it = 0; it < maxit; it++) {
     const int par = it & 1;
     with_v<double>(g, [&](auto vt) {
       constexpr int V = decltype(vt)::value;
-      hipLaunchKernelGGL((k_bi1<double, V, true>), dim3(nb), dim3(kBlock), 0, C->stream, g, A, r,
+      hipLaunchKernelGGL((k_bi1<double, V, SYM>), dim3(nb), dim3(kBlock), 0, C->stream, g, A, r,
                          rt, p[par], v[par], p[par ^ 1], v[par ^ 1], st, C->work);
       finalize<1>(C, FIN_BI1, nb, st);
-      hipLaunchKernelGGL((k_bi2<double, V, true>), dim3(nb), dim3(kBlock), 0, C->stream, g, A, r,
+      hipLaunchKernelGGL((k_bi2<double, V, SYM>), dim3(nb), dim3(kBlock), 0, C->stream, g, A, r,
                          v[par ^ 1], s, t, st, C->work);
       finalize<3>(C, FIN_BI2, nb, st);
       hipLaunchKernelGGL((k_bi3<double, V>), dim3(nb), dim3(kBlock), 0, C->stream, g, A.c,
@@ -1869,8 +1877,14 @@ struct RefPar {
 
 void ref_solve(psb_ctx *C, int l, int maxit, double rr, double rt, double scale, int stale,
                const RefPar &q) {
-  if (q.solver == PSB_BICGS) ref_bicgs(C, l, maxit, rr, rt, scale, stale);
-  else ref_cg(C, l, maxit, rr, rt, scale, stale);
+  // levels of a non-symmetric matrix hold 7 arrays (build_ref)
+  if (q.solver == PSB_BICGS) {
+    if (C->sym) ref_bicgs<true>(C, l, maxit, rr, rt, scale, stale);
+    else ref_bicgs<false>(C, l, maxit, rr, rt, scale, stale);
+  } else {
+    if (C->sym) ref_cg<true>(C, l, maxit, rr, rt, scale, stale);
+    else ref_cg<false>(C, l, maxit, rr, rt, scale, stale);
+  }
 }
 
 void ref_interp(psb_ctx *C, int lc) {  // level lc into lc - 1 (h.phi += H.phi)
@@ -1989,12 +2003,17 @@ double ref_residual(psb_ctx *C, double *linf) {
   RefHier &H = C->href;
   const Level<double> &lv = H.L[0];
   const Grid &g = lv.g;
-  with_v<double>(g, [&](auto vt) {
-    constexpr int V = decltype(vt)::value;
-    hipLaunchKernelGGL((k_start<double, V, true, 0>), dim3(nblocks<double>(g)), dim3(kBlock), 0,
-                       C->stream, g, lv.mat(), H.M[0], lv.r, lv.e[0], H.K[0][0], nullptr, 0.0,
-                       C->work);
-  });
+  auto go = [&](auto symtag) {
+    constexpr bool S = decltype(symtag)::value;
+    with_v<double>(g, [&](auto vt) {
+      constexpr int V = decltype(vt)::value;
+      hipLaunchKernelGGL((k_start<double, V, S, 0>), dim3(nblocks<double>(g)), dim3(kBlock), 0,
+                         C->stream, g, lv.mat(), H.M[0], lv.r, lv.e[0], H.K[0][0], nullptr,
+                         0.0, C->work);
+    });
+  };
+  if (C->sym) go(std::true_type());
+  else go(std::false_type());
   finalize<3>(C, FIN_ACRES, nblocks<double>(g), C->st32, nullptr, C->relres);
   const int nb = grid_blocks(g.N);
   hipLaunchKernelGGL(k_absmax_voldiv, dim3(nb), dim3(kBlock), 0, C->stream, g, H.M[0],

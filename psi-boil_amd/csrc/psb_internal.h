@@ -134,7 +134,10 @@ struct Level {
   T *kc = nullptr, *kv = nullptr, *kz = nullptr;
   T *krt = nullptr;  // the second Krylov step's right-hand side (fused kc_r + sweep)
   double *kst = nullptr;
-  DMat<T> mat() const { return DMat<T>{c, w, s, b, nullptr, nullptr, nullptr}; }
+  // east, north and top couplings of a non-symmetric level (the reference's
+  // AC cycle on a 7-array matrix); NULL: symmetric storage (c, w, s, b)
+  T *ae = nullptr, *an = nullptr, *at = nullptr;
+  DMat<T> mat() const { return DMat<T>{c, w, s, b, ae, an, at}; }
   DMat<T> matflux() const { return DMat<T>{d, w, s, b, nullptr, nullptr, nullptr}; }
   DMat<_Float16> mat16() const {
     return DMat<_Float16>{c16, w16, s16, b16, nullptr, nullptr, nullptr};

@@ -1378,10 +1378,17 @@ static __global__ __launch_bounds__(kBlock) void k_ac_restrict(Grid g, DMat<doub
              k0 + c < g.nz;
     };
     auto PH = [&](int a, int b, int c) { return inside(a, b, c) ? phi[F(a, b, c)] : 0.0; };
-    // e, n, t of a cell (symmetric storage: the neighbour's w, s, b; 0 at walls)
-    auto e_ = [&](int a, int b, int c) { return i0 + a < g.nx - 1 ? A.w[F(a, b, c) + g.sI] : 0.0; };
-    auto n_ = [&](int a, int b, int c) { return j0 + b < g.ny - 1 ? A.s[F(a, b, c) + g.nz] : 0.0; };
-    auto t_ = [&](int a, int b, int c) { return k0 + c < g.nz - 1 ? A.b[F(a, b, c) + 1] : 0.0; };
+    // e, n, t of a cell: its own (7 arrays) or, in symmetric storage, the
+    // neighbour's w, s, b; 0 at walls
+    auto e_ = [&](int a, int b, int c) {
+      return i0 + a < g.nx - 1 ? (A.e ? A.e[F(a, b, c)] : A.w[F(a, b, c) + g.sI]) : 0.0;
+    };
+    auto n_ = [&](int a, int b, int c) {
+      return j0 + b < g.ny - 1 ? (A.n ? A.n[F(a, b, c)] : A.s[F(a, b, c) + g.nz]) : 0.0;
+    };
+    auto t_ = [&](int a, int b, int c) {
+      return k0 + c < g.nz - 1 ? (A.t ? A.t[F(a, b, c)] : A.b[F(a, b, c) + 1]) : 0.0;
+    };
     double s = 0.0;
     for (int a = 0; a < CI; a++)
       for (int b = 0; b < CJ; b++)
@@ -1718,21 +1725,23 @@ static __global__ void k_to_f32(const double *a, float *b, int n) {
 // n/s per (i,k), t/b per (i,j); external couplings summed over the faces).
 // The fine matrix is symmetric 4-array; so is the coarse one.
 static __global__ void k_galerkin(Grid g, DMat<double> A, Grid gH, int CI, int CJ, int CK,
-                           double *cH, double *wH, double *sH, double *bH) {
+                           double *cH, double *wH, double *sH, double *bH, double *eH,
+                           double *nH, double *tH) {
   for (int Q = blockIdx.x * blockDim.x + threadIdx.x; Q < gH.N; Q += gridDim.x * blockDim.x) {
     int I, J, K;
     gH.ijk(Q, I, J, K);
     auto F = [&](int a, int b, int c) {
       return ((I * CI + a) * g.ny + (J * CJ + b)) * g.nz + (K * CK + c);
     };
+    // e, n, t of a fine cell: its own (7 arrays) or the neighbour's w, s, b
     auto e_ = [&](int a, int b, int c) {
-      return I * CI + a < g.nx - 1 ? A.w[F(a, b, c) + g.sI] : 0.0;
+      return I * CI + a < g.nx - 1 ? (A.e ? A.e[F(a, b, c)] : A.w[F(a, b, c) + g.sI]) : 0.0;
     };
     auto n_ = [&](int a, int b, int c) {
-      return J * CJ + b < g.ny - 1 ? A.s[F(a, b, c) + g.nz] : 0.0;
+      return J * CJ + b < g.ny - 1 ? (A.n ? A.n[F(a, b, c)] : A.s[F(a, b, c) + g.nz]) : 0.0;
     };
     auto t_ = [&](int a, int b, int c) {
-      return K * CK + c < g.nz - 1 ? A.b[F(a, b, c) + 1] : 0.0;
+      return K * CK + c < g.nz - 1 ? (A.t ? A.t[F(a, b, c)] : A.b[F(a, b, c) + 1]) : 0.0;
     };
     double c = 0.0;
     for (int a = 0; a < CI; a++)
@@ -1743,29 +1752,43 @@ static __global__ void k_galerkin(Grid g, DMat<double> A, Grid gH, int CI, int C
         for (int a = 0; a < CI - 1; a++) c -= e_(a, b, cc);
         for (int a = 1; a < CI; a++) c -= A.w[F(a, b, cc)];
       }
-    double w = 0.0;
+    double w = 0.0, e = 0.0;
     for (int b = 0; b < CJ; b++)
-      for (int cc = 0; cc < CK; cc++) w += A.w[F(0, b, cc)];
+      for (int cc = 0; cc < CK; cc++) {
+        e += e_(CI - 1, b, cc);
+        w += A.w[F(0, b, cc)];
+      }
     for (int a = 0; a < CI; a++)
       for (int cc = 0; cc < CK; cc++) {
         for (int b = 0; b < CJ - 1; b++) c -= n_(a, b, cc);
         for (int b = 1; b < CJ; b++) c -= A.s[F(a, b, cc)];
       }
-    double s = 0.0;
+    double s = 0.0, n = 0.0;
     for (int a = 0; a < CI; a++)
-      for (int cc = 0; cc < CK; cc++) s += A.s[F(a, 0, cc)];
+      for (int cc = 0; cc < CK; cc++) {
+        n += n_(a, CJ - 1, cc);
+        s += A.s[F(a, 0, cc)];
+      }
     for (int a = 0; a < CI; a++)
       for (int b = 0; b < CJ; b++) {
         for (int cc = 0; cc < CK - 1; cc++) c -= t_(a, b, cc);
         for (int cc = 1; cc < CK; cc++) c -= A.b[F(a, b, cc)];
       }
-    double bb = 0.0;
+    double bb = 0.0, t = 0.0;
     for (int a = 0; a < CI; a++)
-      for (int b = 0; b < CJ; b++) bb += A.b[F(a, b, 0)];
+      for (int b = 0; b < CJ; b++) {
+        t += t_(a, b, CK - 1);
+        bb += A.b[F(a, b, 0)];
+      }
     cH[Q] = c;
     wH[Q] = w;
     sH[Q] = s;
     bH[Q] = bb;
+    if (eH) {  // non-symmetric levels keep their own e, n, t
+      eH[Q] = e;
+      nH[Q] = n;
+      tH[Q] = t;
+    }
   }
 }
 

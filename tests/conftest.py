@@ -54,10 +54,22 @@ FLEX_CASES = ["bubbles16x12x8", "bubbles16"]  # <case>_flex.npz: AC::flexcycle r
 
 def load_cycles(name, kind="cycles"):
     """(phi of each run by tag, the runs' settings and logs) of <name>_<kind>.npz
-    (kind "cycles", "cycles_bicgs" or "flex")."""
+    (kind "cycles", "cycles_bicgs", "cycles_nonsym" or "flex")."""
     z = np.load(os.path.join(GOLDEN, name + "_" + kind + ".npz"), allow_pickle=False)
     meta = json.loads(str(z["meta"]))
     return {k: z[k] for k in z.files if k != "meta"}, meta["runs"]
+
+
+CYCLE_KINDS = ("cycles", "cycles_bicgs", "cycles_nonsym")
+
+
+def cycle_matrix(g, kind, arrays):
+    """The seven coefficient arrays the runs of a cycles fixture used: the
+    case's own, or for "cycles_nonsym" the non-symmetric matrix the fixture
+    holds."""
+    if kind == "cycles_nonsym":
+        return {k: arrays["N_A_" + k] for k in "cwesnbt"}
+    return g.mat()
 
 
 def cycle_kwargs(run):

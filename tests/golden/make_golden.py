@@ -8,13 +8,16 @@ outputs, solver iterates after k iterations, AC level matrices, restriction /
 interpolation results and V-cycle iterates, plus the solver log lines (17
 significant digits) parsed into arrays.
 
-    python tests/golden/make_golden.py [cases] [cycles] [cycles-bicgs] [flex] [prec] [case names]   # needs oracle/_ref built
+    python tests/golden/make_golden.py [cases] [cycles] [cycles-bicgs] [cycles-nonsym] [flex] [prec] [case names]   # needs oracle/_ref built
 
 "cycles" writes <case>_cycles.npz: the reference's AC::cycle with the F, W
 and full-cycle shapes and its controls (ref_driver cycles), phi after each
 run and the run's residual log.  "cycles-bicgs" writes <case>_cycles_bicgs.npz:
 six of those runs with BiCGS (Prec::di) as every level's solver, the solver
-the reference's AC hands its levels for a non-symmetric equation.  "flex" writes <case>_flex.npz: the same for
+the reference's AC hands its levels for a non-symmetric equation.
+"cycles-nonsym" writes <case>_cycles_nonsym.npz: the same runs on the case's
+matrix made non-symmetric by the driver (east and top couplings scaled
+per cell, a time term on the diagonal), with that matrix (N_A_*).  "flex" writes <case>_flex.npz: the same for
 AC::flexcycle on the level settings where its recursion ends.  "prec" writes <case>_prec.npz: CG with the
 reference's incomplete-Cholesky preconditioners IC0 and IC2 (ref_driver
 prec), x after 1 and 5 iterations and at ResRat 1e-8, and the final log
@@ -158,7 +161,8 @@ def parse_cycles(text):
 def run_cycles(name, mode="cycles"):
     """mode "cycles": CG level solves; "cycles-bicgs": BiCGS (Prec::di) as
     every level's solver, a subset of the runs (tags "B..."), written to
-    <case>_cycles_bicgs.npz."""
+    <case>_cycles_bicgs.npz; "cycles-nonsym": those runs on a non-symmetric
+    matrix, written with it to <case>_cycles_nonsym.npz."""
     nx, ny, nz, nsph, rmin, rmax, rin, rout, ss, sv, centre = CASES[name]
     with tempfile.TemporaryDirectory() as d:
         args = [DRIVER, mode, d, str(nx), str(ny), str(nz), str(nsph), repr(rmin),
@@ -169,8 +173,11 @@ def run_cycles(name, mode="cycles"):
         runs = parse_cycles(out)
         arrays = {r["tag"] + "_phi": np.fromfile(os.path.join(d, r["tag"] + "_phi.f64"))
                   for r in runs}
+        if mode == "cycles-nonsym":
+            for k in "cwesnbt":
+                arrays["N_A_" + k] = np.fromfile(os.path.join(d, "N_A_" + k + ".f64"))
     arrays["meta"] = np.array(json.dumps({"name": name, "runs": runs}))
-    path = os.path.join(HERE, name + ("_cycles.npz" if mode == "cycles" else "_cycles_bicgs.npz"))
+    path = os.path.join(HERE, name + "_" + mode.replace("-", "_") + ".npz")
     np.savez_compressed(path, **arrays)
     return path
 
@@ -238,7 +245,7 @@ def run_prec(name):
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("oracle/_ref/ref_driver is not built (make -f oracle/ref.mk)")
-    what = sys.argv[1:] or ["cases", "cycles", "cycles-bicgs", "flex", "prec"]
+    what = sys.argv[1:] or ["cases", "cycles", "cycles-bicgs", "cycles-nonsym", "flex", "prec"]
     only = [w for w in what if w in CASES]  # restrict to the cases named
     if "cases" in what:
         for name, spec in CASES.items():
@@ -252,6 +259,10 @@ def main():
         for name in CYCLE_CASES:
             if not only or name in only:
                 print(run_cycles(name, "cycles-bicgs"))
+    if "cycles-nonsym" in what:
+        for name in CYCLE_CASES:
+            if not only or name in only:
+                print(run_cycles(name, "cycles-nonsym"))
     if "flex" in what:
         for name in FLEX_RUNS:
             if not only or name in only:

@@ -4,7 +4,8 @@ written by its own build): F and W cycles, full-cycle starts, the linf
 control with its stale previous value, divergence and min-cycle controls,
 per-stage MaxIter and stale counts and the level solves' ResRat; six of
 those runs with BiCGS as every level's solver (the solver the reference's AC
-hands its levels for a non-symmetric equation); and the flex cycle
+hands its levels for a non-symmetric equation), on the case's matrix and on
+a non-symmetric one (7-array levels); and the flex cycle
 (flexcycle, its recursion bounded by max_visits) on the settings where the
 reference's own recursion ends.
 
@@ -26,7 +27,8 @@ import pytest
 
 import psb
 import psb_oracle as O
-from conftest import CYCLE_CASES, FLEX_CASES, cycle_kwargs, load_cycles, load_golden
+from conftest import (CYCLE_CASES, CYCLE_KINDS, FLEX_CASES, cycle_kwargs, cycle_matrix,
+                      load_cycles, load_golden)
 
 pytestmark = pytest.mark.gpu
 
@@ -36,7 +38,7 @@ _SHAPES = ["V", "F", "W", "flex"]
 def _runs():
     out = []
     for case in CYCLE_CASES:
-        for kind in ("cycles", "cycles_bicgs"):  # CG / BiCGS level solves
+        for kind in CYCLE_KINDS:  # CG / BiCGS level solves, a non-symmetric matrix
             _, runs = load_cycles(case, kind)
             out += [(case, kind, r["tag"]) for r in runs]
     for case in FLEX_CASES:  # AC::flexcycle where the reference's recursion ends
@@ -58,14 +60,18 @@ def gpu_kwargs(run):
 _solvers = {}
 
 
-def solver(case):
-    if case not in _solvers:
+def solver(case, kind="cycles"):
+    """A context per case and matrix (the case's own or the non-symmetric
+    one of the cycles_nonsym fixture)."""
+    key = (case, kind == "cycles_nonsym")
+    if key not in _solvers:
         g = load_golden(case)
         s = psb.Solver(*g.shape)
-        s.set_matrix(g.mat())
+        M = cycle_matrix(g, kind, load_cycles(case, kind)[0]) if key[1] else g.mat()
+        s.set_matrix(M)
         s.set_geometry(g["dxc"], g["dyc"], g["dzc"])
-        _solvers[case] = (g, s)
-    return _solvers[case]
+        _solvers[key] = (g, s, M)
+    return _solvers[key]
 
 
 def demeaned_rel(x, ref):
@@ -80,10 +86,11 @@ _VARIANTS = ((0, 0), (64, 0), (256, 0), (1024, 0), (0, 1), (0, 2))
 _VARIANTS_BICGS = _VARIANTS + ((2, 0), (4, 0), (8, 0), (16, 0), (2, 1), (4, 2))
 
 
-def oracle_variants(g, run, col):
+def oracle_variants(g, run, col, M):
     """The oracle's (history, phi, cycles) for the reference's order of sums
-    and for blocked orders and perturbed b: the run's own spread."""
-    A = O.Mat(*g.shape, g.mat())
+    and for blocked orders and perturbed b: the run's own spread (M: the
+    matrix's arrays)."""
+    A = O.Mat(*g.shape, M)
     b = g["b"]
     out = []
     try:
@@ -101,7 +108,7 @@ def oracle_variants(g, run, col):
 
 @pytest.mark.parametrize("case,kind,tag", _runs())
 def test_gpu_cycle_matches_reference(case, kind, tag):
-    g, s = solver(case)
+    g, s, M = solver(case, kind)
     phis, runs = load_cycles(case, kind)
     run = [r for r in runs if r["tag"] == tag][0]
     col = 2 if run["use_linf"] else 1
@@ -110,7 +117,7 @@ def test_gpu_cycle_matches_reference(case, kind, tag):
     print(f"[cycles] {case} {tag}", flush=True)
     x, r = s.ac_vcycle(np.zeros(g["b"].size), g["b"], **gpu_kwargs(run))
     h = s.history()
-    var = oracle_variants(g, run, col)
+    var = oracle_variants(g, run, col, M)
     h0 = var[0][0]
     assert np.array_equal(h0, ref_hist)  # the oracle is the reference
     n = min(len(v[0]) for v in var)
@@ -163,7 +170,7 @@ def test_gpu_full_cycle_start_is_discarded_like_the_reference():
     """fullcycle(V): with the divergence check on, the reference compares the
     start's residual with itself and restores the initial phi; without the
     check the start is kept and the first cycle begins from it."""
-    g, s = solver("bubbles16x12x8")
+    g, s, _ = solver("bubbles16x12x8")
     b, n = g["b"], g["b"].size
     s.ac_vcycle(np.zeros(n), b, init="V", res_rat=1e-30, max_cycles=1)
     h1 = s.history()
@@ -177,7 +184,7 @@ def test_gpu_stale_counts_beyond_the_window_are_rejected():
     """Stale windows hold at most 64 residuals: a larger stale_iter_array
     entry, or a larger stalecount of a Krylov solve, is an invalid argument
     rather than a silently shortened window."""
-    g, s = solver("bubbles16x12x8")
+    g, s, _ = solver("bubbles16x12x8")
     b, n = g["b"], g["b"].size
     with pytest.raises(psb.PsbError, match="EINVAL"):
         s.ac_vcycle(np.zeros(n), b, res_rat=1e-8, max_stale=(3, 65, 3))
@@ -194,7 +201,7 @@ def test_gpu_flex_cycle_bound():
     max_visits level visits and ends the run as a divergence would:
     PSB_DIVERGED in the first cycle, phi the initial guess, the history only
     the initial residual; as the oracle's bound."""
-    g, s = solver("bubbles16")
+    g, s, _ = solver("bubbles16")
     b, n = g["b"], g["b"].size
     x0 = np.zeros(n)
     x, r = s.ac_vcycle(x0, b, loop="flex", res_rat=1e-8, max_cycles=20, smooth_res_rat=1e-3,
@@ -211,7 +218,7 @@ def test_gpu_flex_cycle_bound():
 
 def test_gpu_level_solver_is_checked():
     """level_solver takes PSB_CG or PSB_BICGS only; the default is CG."""
-    g, s = solver("bubbles16x12x8")
+    g, s, _ = solver("bubbles16x12x8")
     b, n = g["b"], g["b"].size
     with pytest.raises(psb.PsbError, match="EINVAL"):
         s.ac_vcycle(np.zeros(n), b, res_rat=1e-8, level_solver=7)

@@ -5,18 +5,20 @@ priority_min_cyc, min_cycles, per-stage MaxIter and stale counts
 (stale_iter_array) and the level solves' ResRat -- phi after each run bit for bit, every logged residual (17
 significant digits) and the returned cycle count; and six of those runs with
 BiCGS as every level's solver (the reference's AC with a non-symmetric
-equation's solver).  CPU only."""
+equation's solver), on the case's matrix and on a non-symmetric one.  CPU
+only."""
 import numpy as np
 import pytest
 
 import psb_oracle as O
-from conftest import CYCLE_CASES, FLEX_CASES, cycle_kwargs, load_cycles, load_golden
+from conftest import (CYCLE_CASES, CYCLE_KINDS, FLEX_CASES, cycle_kwargs, cycle_matrix,
+                      load_cycles, load_golden)
 
 
 def _runs():
     out = []
     for case in CYCLE_CASES:
-        for kind in ("cycles", "cycles_bicgs"):
+        for kind in CYCLE_KINDS:
             _, runs = load_cycles(case, kind)
             out += [(case, kind, r["tag"]) for r in runs]
     return out
@@ -27,7 +29,7 @@ def systems():
     res = {}
     for case in CYCLE_CASES:
         g = load_golden(case)
-        res[case] = {kind: (g, load_cycles(case, kind)) for kind in ("cycles", "cycles_bicgs")}
+        res[case] = {kind: (g, load_cycles(case, kind)) for kind in CYCLE_KINDS}
     return res
 
 
@@ -35,7 +37,7 @@ def systems():
 def test_cycle_matches_reference(systems, case, kind, tag):
     g, (phis, runs) = systems[case][kind]
     run = [r for r in runs if r["tag"] == tag][0]
-    A = O.Mat(*g.shape, g.mat())
+    A = O.Mat(*g.shape, cycle_matrix(g, kind, phis))
     ac = O.AC(A, g["dV"], rule=0)
     phi, info = ac.cycle(np.zeros(A.n), g["b"], **cycle_kwargs(run))
     assert np.array_equal(phi, phis[tag + "_phi"])
@@ -129,3 +131,21 @@ def test_bicgs_level_solves_change_the_result(systems):
         kw = dict(cycle_kwargs(run), solver="cg")
         phi, _ = O.AC(A, g["dV"], rule=0).cycle(np.zeros(A.n), g["b"], **kw)
         assert not np.array_equal(phi, phis[run["tag"] + "_phi"]), run["tag"]
+
+
+@pytest.mark.parametrize("case", CYCLE_CASES)
+def test_nonsymmetric_fixture_and_its_levels(case):
+    """The non-symmetric runs' matrix is non-symmetric on every level the
+    oracle coarsens (east coupling of a cell against the west coupling of
+    its neighbour), and its row sums exceed the couplings (the time term)."""
+    g = load_golden(case)
+    phis, _ = load_cycles(case, "cycles_nonsym")
+    M = cycle_matrix(g, "cycles_nonsym", phis)
+    A = O.Mat(*g.shape, M)
+    ac = O.AC(A, g["dV"], rule=0)
+    for l in range(ac.nlev - 1):
+        nx, ny, nz = ac.dims(l)
+        m = {k: np.asarray(v).reshape(nx, ny, nz) for k, v in ac.mat(l).items()}
+        assert np.abs(m["e"][:-1] - m["w"][1:]).max() > 1e-3 * np.abs(m["w"]).max(), l
+        off = sum(m[k] for k in "wesnbt")
+        assert np.all(m["c"] > off)
