@@ -50,9 +50,11 @@ enum psb_status {
 
 enum psb_method { PSB_CG = 0, PSB_BICGS = 1 };
 /* PSB_PREC_DIAG: Prec::di (preconditioner_form.cpp:4-13).  PSB_PREC_AC: the
- * additive-correction V-cycle, for CG (any precision) or for BiCGS (FP64,
- * symmetric matrix; the V-cycle on every level whatever ac_kcycle says, as
- * BiCGS needs the same operator at each application).  PSB_PREC_SGS: red-black symmetric
+ * additive-correction V-cycle, for CG (any precision, symmetric matrix) or
+ * for BiCGS (FP64; the V-cycle on every level whatever ac_kcycle says, as
+ * BiCGS needs the same operator at each application; a non-symmetric matrix
+ * keeps 7 arrays on every level, its coarsest level inverted by LU, and a
+ * singular one -- no time term -- is PSB_EINVAL).  PSB_PREC_SGS: red-black symmetric
  * Gauss-Seidel, M = (D - L) D^-1 (D - U) with the even cells ((i+j+k) % 2 = 0)
  * ordered first, the parallel replacement for Prec::ic0 / ic2 / ic3 (whose
  * triangular sweeps, preconditioner_solve.cpp:14-110, run cell after cell);

@@ -41,8 +41,8 @@ class GpuKrylov : public Krylov {
  public:
   /* method: PSB_CG (the reference's CG) or PSB_BICGS (BiCGS); precond:
    * PSB_PREC_DIAG (Prec::di()), PSB_PREC_SGS (in place of Prec::ic0/ic2/ic3)
-   * or PSB_PREC_AC (the library's multigrid preconditioner: for CG, and for
-   * BiCGS in FP64 on a symmetric matrix) */
+   * or PSB_PREC_AC (the library's multigrid preconditioner: for CG on a
+   * symmetric matrix, and for BiCGS in FP64 on any non-singular one) */
   explicit GpuKrylov(const Domain &d, int method = PSB_CG, int precond = PSB_PREC_DIAG)
       : Krylov(d) {
     psb_stop_defaults(&base_);

@@ -1340,9 +1340,97 @@ int ora_ac_vcycle(ora_ac *ac, double *phi0, const double *f0, double toler,
 /* ------------------------------------------------------------------------ */
 /* benchmark composition: V-cycle preconditioned CG (not in the reference)   */
 /* ------------------------------------------------------------------------ */
+/* e, n, t equal the neighbours' w, s, b (inside the box) */
+static int mat_symmetric(const ora_mat *A) {
+  const int nx = A->nx, ny = A->ny, nz = A->nz;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        size_t P = IDX(i, j, k, ny, nz);
+        if (i < nx - 1 && A->e[P] != A->w[IDX(i + 1, j, k, ny, nz)]) return 0;
+        if (j < ny - 1 && A->n[P] != A->s[IDX(i, j + 1, k, ny, nz)]) return 0;
+        if (k < nz - 1 && A->t[P] != A->b[IDX(i, j, k + 1, ny, nz)]) return 0;
+      }
+  return 1;
+}
+
+/* G = A^-1 on the active cells (0 elsewhere) of a non-symmetric coarsest
+ * level, by LU with partial pivoting (Golub and Van Loan, Matrix
+ * Computations, alg. 3.4.1) and a solve per unit column; such a level carries
+ * a time term: a pivot below 1e-14 of the largest |c| is reported (-1) */
+static int coarse_inverse(ora_ac *ac) {
+  const int l = ac->nlev - 1;
+  const ora_mat *A = &ac->A[l];
+  const int nx = A->nx, ny = A->ny, nz = A->nz;
+  const int m = nx * ny * nz;
+  const int *fl = ac->aflag[l];
+  int *pos = (int *)malloc(sizeof(int) * (size_t)m), *idx = (int *)malloc(sizeof(int) * (size_t)m);
+  int na = 0;
+  for (int P = 0; P < m; P++) {
+    pos[P] = fl[P] ? na : -1;
+    if (fl[P]) idx[na++] = P;
+  }
+  double *U = dalloc((size_t)na * na), *y = dalloc((size_t)na), cmax = 0.0;
+  int *perm = (int *)malloc(sizeof(int) * (size_t)(na ? na : 1));
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        const int P = (int)IDX(i, j, k, ny, nz);
+        if (!fl[P]) continue;
+        double *row = U + (size_t)pos[P] * na;
+        row[pos[P]] = A->c[P];
+        if (fabs(A->c[P]) > cmax) cmax = fabs(A->c[P]);
+        int Q;
+        if (i > 0 && fl[Q = (int)IDX(i - 1, j, k, ny, nz)]) row[pos[Q]] = -A->w[P];
+        if (i < nx - 1 && fl[Q = (int)IDX(i + 1, j, k, ny, nz)]) row[pos[Q]] = -A->e[P];
+        if (j > 0 && fl[Q = (int)IDX(i, j - 1, k, ny, nz)]) row[pos[Q]] = -A->s[P];
+        if (j < ny - 1 && fl[Q = (int)IDX(i, j + 1, k, ny, nz)]) row[pos[Q]] = -A->n[P];
+        if (k > 0 && fl[Q = (int)IDX(i, j, k - 1, ny, nz)]) row[pos[Q]] = -A->b[P];
+        if (k < nz - 1 && fl[Q = (int)IDX(i, j, k + 1, ny, nz)]) row[pos[Q]] = -A->t[P];
+      }
+  int ok = 1;
+  for (int q = 0; q < na; q++) perm[q] = q;
+  for (int k = 0; k < na && ok; k++) {
+    int p = k;
+    for (int i = k + 1; i < na; i++)
+      if (fabs(U[(size_t)i * na + k]) > fabs(U[(size_t)p * na + k])) p = i;
+    if (!(fabs(U[(size_t)p * na + k]) > 1e-14 * cmax)) { ok = 0; break; }
+    if (p != k) {
+      for (int j = 0; j < na; j++) {
+        double tmp = U[(size_t)p * na + j];
+        U[(size_t)p * na + j] = U[(size_t)k * na + j];
+        U[(size_t)k * na + j] = tmp;
+      }
+      int tp = perm[p]; perm[p] = perm[k]; perm[k] = tp;
+    }
+    for (int i = k + 1; i < na; i++) {
+      const double lik = U[(size_t)i * na + k] / U[(size_t)k * na + k];
+      U[(size_t)i * na + k] = lik;
+      for (int j = k + 1; j < na; j++) U[(size_t)i * na + j] -= lik * U[(size_t)k * na + j];
+    }
+  }
+  if (ok) {
+    free(ac->G);
+    ac->G = dalloc((size_t)m * m);
+    for (int col = 0; col < na; col++) {
+      for (int i = 0; i < na; i++) y[i] = perm[i] == col ? 1.0 : 0.0;
+      for (int i = 0; i < na; i++)
+        for (int k = 0; k < i; k++) y[i] -= U[(size_t)i * na + k] * y[k];
+      for (int i = na - 1; i >= 0; i--) {
+        for (int k = i + 1; k < na; k++) y[i] -= U[(size_t)i * na + k] * y[k];
+        y[i] /= U[(size_t)i * na + i];
+      }
+      for (int i = 0; i < na; i++) ac->G[(size_t)idx[i] * m + idx[col]] = y[i];
+    }
+  }
+  free(pos); free(idx); free(U); free(y); free(perm);
+  return ok ? m : -1;
+}
+
 int ora_ac_coarse_dense(ora_ac *ac) {
   const int l = ac->nlev - 1;
   const ora_mat *A = &ac->A[l];
+  if (!mat_symmetric(A)) return coarse_inverse(ac);
   const int nx = A->nx, ny = A->ny, nz = A->nz;
   const int m = nx * ny * nz;
   double *M = dalloc((size_t)m * m);
@@ -1689,7 +1777,7 @@ int ora_acbicgs(ora_ac *ac, double *x, const double *b, const double *dV, int mi
                 int maxit, double res_rat, double res_tol, double scale, int stalecount,
                 int nu1, int nu2, double omega, double cfac, double *hist,
                 ora_result *out) {
-  if (!ac->G) ora_ac_coarse_dense(ac);
+  if (!ac->G && ora_ac_coarse_dense(ac) < 0) return -1; /* singular coarsest level */
   const bicgs_prec M = {NULL, ac, nu1, nu2, omega, cfac};
   return bicgs_run(&ac->A[0], &M, x, b, dV, mini, maxit, res_rat, res_tol, scale,
                    stalecount, hist, out);

@@ -196,7 +196,10 @@ int  ora_ac_cycle(ora_ac *ac, double *phi0, const double *f0, const ora_cycle *P
 
 /* ---- the benchmark's composition (not in the reference) ----------------- */
 /* Dense operator of the coarsest level: G = Pi * (A_red)^-1 * Pi over active
- * cells, Pi the mean projector, A_red the matrix with its last pivot pinned. */
+ * cells, Pi the mean projector, A_red the matrix with its last pivot pinned;
+ * for a non-symmetric level (7 arrays) G = A^-1 on the active cells by LU
+ * with partial pivoting.  Returns the level's cell count, -1 for a singular
+ * non-symmetric level (G left unset). */
 int  ora_ac_coarse_dense(ora_ac *ac);
 /* One V-cycle as a preconditioner z = M^-1 r: nu1 damped-Jacobi sweeps from
  * zero, residual restriction (block sums), recursion, piecewise-constant

@@ -375,6 +375,8 @@ class AC:
 
     def coarse_dense(self):
         m = lib().ora_ac_coarse_dense(C.byref(self.ac))
+        if m < 0:
+            raise ValueError("singular coarsest level of a non-symmetric matrix")
         return np.ctypeslib.as_array(self.ac.G, shape=(m * m,)).reshape(m, m).copy()
 
     def precond(self, r, nu1=2, nu2=2, omega=0.8, cfac=1.0, smoother=0):

@@ -249,6 +249,20 @@ void jacobi_m(psb_ctx *C, const Level<T> &lv, DMat<M> A, const T *e, T *eout,
       pr.nYH = coarse->g.ny;
       pr.nZH = coarse->g.nz;
     }
+    if constexpr (std::is_same<T, double>::value && std::is_same<M, T>::value && !FLUX &&
+                  !RHO) {
+      if (lv.ae) {  // non-symmetric level (7 arrays): the span kernel
+        if (coarse)
+          hipLaunchKernelGGL((k_jacobi<T, V, false, true, false, M, false>),
+                             dim3(nblocks<T>(g)), dim3(kBlock), 0, C->stream, g, A, lv.r, e,
+                             eout, pr, omega, lv.aflag, st, C->work, zscale, qf);
+        else
+          hipLaunchKernelGGL((k_jacobi<T, V, false, false, false, M, false>),
+                             dim3(nblocks<T>(g)), dim3(kBlock), 0, C->stream, g, A, lv.r, e,
+                             eout, pr, omega, lv.aflag, st, C->work, zscale, qf);
+        return;
+      }
+    }
     MarchShape m;
     if (lds_use<T>(C, coarse ? kLdsProl : RHO ? kLdsLast : kLdsSweep) && V > 1 &&
         g.N >= kMarchMin && lds_ok<V>(g) && march_ok<V>(g, m)) {
@@ -368,6 +382,15 @@ void restrict_launch(psb_ctx *C, const Level<T> &f, DMat<M> A, const T *e, Level
   ProbeScope ps(C, PSB_K_RESTRICT, f.g,
                 4.0 * sizeof(M) + 2.0 * sizeof(T) + (f.aflag ? 1 : 0) +
                     (3.0 * sizeof(T) + (cL.aflag ? 1 : 0)) / (CI * CJ * CK));
+  if constexpr (std::is_same<T, double>::value && std::is_same<M, T>::value && !FLUX) {
+    if (f.ae) {  // non-symmetric level (7 arrays): the span kernel
+      const int nwork = cL.g.nx * cL.g.ny * (f.g.nz / (V * KV));
+      hipLaunchKernelGGL((k_restrict<T, V, KV, false, CI, CJ, CK, M, false>),
+                         dim3(grid_blocks(nwork)), dim3(kBlock), 0, C->stream, f.g, A, f.r, e,
+                         f.aflag, cL.g, cL.c, cL.aflag, cL.r, cL.e[0], omega, st);
+      return;
+    }
+  }
   if constexpr (CI == 2 && KV == 1 && V > 1) {
     if (kMarch && f.g.N >= kMarchMin && f.g.nx == 2 * cL.g.nx) {
       const MarchShape m = march_shape_n(cL.g.nx, (long long)cL.g.ny * (f.g.nz / V));
@@ -614,6 +637,80 @@ void ensure_krylov(psb_ctx *C, KrylovVecs<T> &K, bool bicgs) {
 template void ensure_krylov<float>(psb_ctx *, KrylovVecs<float> &, bool);
 template void ensure_krylov<double>(psb_ctx *, KrylovVecs<double> &, bool);
 
+// dense coarsest operator of a non-symmetric matrix (7 arrays): G = A^-1 on
+// the active cells (0 elsewhere) by LU with partial pivoting; such a level
+// carries a time term, and a singular one is an error
+std::vector<double> dense_inv(int nx, int ny, int nz, const std::vector<double> &c,
+                              const std::vector<double> &w, const std::vector<double> &s,
+                              const std::vector<double> &b, const std::vector<double> &e,
+                              const std::vector<double> &n, const std::vector<double> &t,
+                              const std::vector<uint8_t> &act) {
+  const int m = nx * ny * nz;
+  auto id = [&](int i, int j, int k) { return (i * ny + j) * nz + k; };
+  std::vector<int> idx, pos(m, -1);  // active cells, in order
+  for (int P = 0; P < m; P++)
+    if (act[P]) { pos[P] = (int)idx.size(); idx.push_back(P); }
+  const int na = (int)idx.size();
+  std::vector<double> U((size_t)na * na, 0.0);
+  double cmax = 0.0;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        const int P = id(i, j, k);
+        if (!act[P]) continue;
+        double *row = &U[(size_t)pos[P] * na];
+        row[pos[P]] = c[P];
+        cmax = std::max(cmax, std::fabs(c[P]));
+        auto cpl = [&](int Q, double a) {
+          if (act[Q]) row[pos[Q]] = -a;
+        };
+        if (i > 0) cpl(id(i - 1, j, k), w[P]);
+        if (i < nx - 1) cpl(id(i + 1, j, k), e[P]);
+        if (j > 0) cpl(id(i, j - 1, k), s[P]);
+        if (j < ny - 1) cpl(id(i, j + 1, k), n[P]);
+        if (k > 0) cpl(id(i, j, k - 1), b[P]);
+        if (k < nz - 1) cpl(id(i, j, k + 1), t[P]);
+      }
+  std::vector<int> perm(na);
+  for (int q = 0; q < na; q++) perm[q] = q;
+  for (int k = 0; k < na; k++) {
+    int p = k;
+    for (int i = k + 1; i < na; i++)
+      if (std::fabs(U[(size_t)i * na + k]) > std::fabs(U[(size_t)p * na + k])) p = i;
+    if (!(std::fabs(U[(size_t)p * na + k]) > 1e-14 * cmax))
+      throw std::invalid_argument("singular coarsest level of a non-symmetric matrix");
+    if (p != k) {
+      for (int j = 0; j < na; j++) std::swap(U[(size_t)p * na + j], U[(size_t)k * na + j]);
+      std::swap(perm[p], perm[k]);
+    }
+    const double piv = U[(size_t)k * na + k];
+    for (int i = k + 1; i < na; i++) {
+      double *Ui = &U[(size_t)i * na];
+      const double l = Ui[k] / piv;
+      Ui[k] = l;
+      if (l == 0.0) continue;
+      const double *Uk = &U[(size_t)k * na];
+      for (int j = k + 1; j < na; j++) Ui[j] -= l * Uk[j];
+    }
+  }
+  std::vector<double> G((size_t)m * m, 0.0), y(na);
+  for (int col = 0; col < na; col++) {
+    for (int i = 0; i < na; i++) y[i] = perm[i] == col ? 1.0 : 0.0;
+    for (int i = 0; i < na; i++) {
+      double v = y[i];
+      for (int k = 0; k < i; k++) v -= U[(size_t)i * na + k] * y[k];
+      y[i] = v;
+    }
+    for (int i = na - 1; i >= 0; i--) {
+      double v = y[i];
+      for (int k = i + 1; k < na; k++) v -= U[(size_t)i * na + k] * y[k];
+      y[i] = v / U[(size_t)i * na + i];
+    }
+    for (int i = 0; i < na; i++) G[(size_t)idx[i] * m + idx[col]] = y[i];
+  }
+  return G;
+}
+
 void build_hier64(psb_ctx *C) {
   Hier<double> &H = C->h64;
   if (H.built) return;
@@ -634,16 +731,21 @@ void build_hier64(psb_ctx *C) {
     const size_t N = lv.g.N;
     if (l == 0) {
       lv.c = C->A64[0]; lv.w = C->A64[1]; lv.s = C->A64[2]; lv.b = C->A64[3];
+      if (!C->sym) { lv.ae = C->A64[4]; lv.an = C->A64[5]; lv.at = C->A64[6]; }
       lv.r = C->k64.r;
       lv.aflag = C->aflag0;
     } else {
       lv.c = C->harena.alloc<double>(N); lv.w = C->harena.alloc<double>(N);
       lv.s = C->harena.alloc<double>(N); lv.b = C->harena.alloc<double>(N);
+      if (!C->sym) {  // a non-symmetric matrix keeps 7 arrays on every level
+        lv.ae = C->harena.alloc<double>(N); lv.an = C->harena.alloc<double>(N);
+        lv.at = C->harena.alloc<double>(N);
+      }
       lv.r = C->harena.alloc<double>(N);
       Level<double> &f = H.L[l - 1];
       hipLaunchKernelGGL(k_galerkin, dim3(grid_blocks(N)), dim3(kBlock), 0, stream, f.g,
-                         f.mat(), lv.g, lv.CI, lv.CJ, lv.CK, lv.c, lv.w, lv.s, lv.b, nullptr,
-                         nullptr, nullptr);
+                         f.mat(), lv.g, lv.CI, lv.CJ, lv.CK, lv.c, lv.w, lv.s, lv.b, lv.ae,
+                         lv.an, lv.at);
       if (f.aflag) {
         lv.aflag = C->harena.alloc<uint8_t>(N);
         hipLaunchKernelGGL(k_coarsen_flag, dim3(grid_blocks(N)), dim3(kBlock), 0, stream, f.g,
@@ -663,8 +765,16 @@ void build_hier64(psb_ctx *C) {
   HIPCHK(hipMemcpyAsync(s.data(), cl.s, m * 8, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipMemcpyAsync(b.data(), cl.b, m * 8, hipMemcpyDeviceToHost, stream));
   if (cl.aflag) HIPCHK(hipMemcpyAsync(act.data(), cl.aflag, m, hipMemcpyDeviceToHost, stream));
+  std::vector<double> e, n, t;
+  if (cl.ae) {
+    e.resize(m); n.resize(m); t.resize(m);
+    HIPCHK(hipMemcpyAsync(e.data(), cl.ae, m * 8, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(n.data(), cl.an, m * 8, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(t.data(), cl.at, m * 8, hipMemcpyDeviceToHost, stream));
+  }
   HIPCHK(hipStreamSynchronize(stream));
-  std::vector<double> G = dense_pinv(cl.g.nx, cl.g.ny, cl.g.nz, c, w, s, b, act);
+  std::vector<double> G = cl.ae ? dense_inv(cl.g.nx, cl.g.ny, cl.g.nz, c, w, s, b, e, n, t, act)
+                                : dense_pinv(cl.g.nx, cl.g.ny, cl.g.nz, c, w, s, b, act);
   H.m = m;
   H.G = C->harena.alloc<double>((size_t)m * m);
   HIPCHK(hipMemcpyAsync(H.G, G.data(), (size_t)m * m * 8, hipMemcpyHostToDevice, stream));
@@ -1449,11 +1559,14 @@ void solve_ac64(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_r
 // BiCGS preconditioned by the multigrid V-cycle, FP64 (psb_stop.method =
 // PSB_BICGS with PSB_PREC_AC): the steps of bicgs.cpp:79-229 with the cycle
 // as M^-1, applied to p and to s (level 0's right-hand side pointed at each
-// in turn, its first sweep formed by the pass before).  The K-cycle's steps
+// in turn, its first sweep formed by the pass before).  A non-symmetric
+// matrix keeps 7 arrays on every level (span sweeps and restriction, the
+// coarsest inverse by LU).  The K-cycle's steps
 // change the preconditioner from one application to the next, which
 // BiCGStab does not allow for, so the levels run the V-cycle whatever
 // ac_kcycle says.
 // ---------------------------------------------------------------------------
+template <bool SYM>
 void solve_acbicgs64(psb_ctx *C, double *x, const double *b, const psb_stop *p,
                      psb_result *R) {
   KrylovVecs<double> &K = C->k64;
@@ -1470,7 +1583,7 @@ void solve_acbicgs64(psb_ctx *C, double *x, const double *b, const psb_stop *p,
     zero(C, K.p[q], g.N);
     zero(C, K.v[q], g.N);
   }
-  start<double, true, 2>(C, g, A, b, K.x, K.r, K.rt, 0.0, C->st64);
+  start<double, SYM, 2>(C, g, A, b, K.x, K.r, K.rt, 0.0, C->st64);
   HIPCHK(hipGetLastError());
   read_state(C, C->st64);
   const int chunk = chunk_of(p);
@@ -1494,7 +1607,7 @@ void solve_acbicgs64(psb_ctx *C, double *x, const double *b, const psb_stop *p,
       const double *z1 = vcycle<double>(C, H, 0, st, nu1, nu2, omega, cfac, 0, false);
       with_v<double>(g, [&](auto vt) {
         constexpr int V = decltype(vt)::value;
-        hipLaunchKernelGGL((k_abi_v<double, V, true>), dim3(nb), dim3(kBlock), 0, C->stream, g,
+        hipLaunchKernelGGL((k_abi_v<double, V, SYM>), dim3(nb), dim3(kBlock), 0, C->stream, g,
                            A, z1, K.rt, K.v[par ^ 1], K.z, st, C->work);
         finalize<1>(C, FIN_BI1, nb, st);
         hipLaunchKernelGGL((k_abi_s<double, V>), dim3(nb), dim3(kBlock), 0, C->stream, g, f.c,
@@ -1504,7 +1617,7 @@ void solve_acbicgs64(psb_ctx *C, double *x, const double *b, const psb_stop *p,
       const double *z2 = vcycle<double>(C, H, 0, st, nu1, nu2, omega, cfac, 0, false);
       with_v<double>(g, [&](auto vt) {
         constexpr int V = decltype(vt)::value;
-        hipLaunchKernelGGL((k_abi_t<double, V, true>), dim3(nb), dim3(kBlock), 0, C->stream, g,
+        hipLaunchKernelGGL((k_abi_t<double, V, SYM>), dim3(nb), dim3(kBlock), 0, C->stream, g,
                            A, z2, K.s, K.t, st, C->work);
         finalize<3>(C, FIN_BI2, nb, st);
         hipLaunchKernelGGL((k_abi_x<double, V>), dim3(nb), dim3(kBlock), 0, C->stream, g,
@@ -2167,8 +2280,8 @@ int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_
   C->hist_on_host = false;
   if (p->precond == PSB_PREC_AC) {
     if (p->method == PSB_BICGS) {  // the V-cycle as BiCGS's preconditioner, FP64
-      if (p->precision != PSB_FP64 || !C->sym || p->ac_pre < 1 || p->ac_post < 1 ||
-          p->ac_coarsen < 0 || p->ac_coarsen > 1)
+      if (p->precision != PSB_FP64 || p->ac_pre < 1 || p->ac_post < 1 || p->ac_coarsen < 0 ||
+          p->ac_coarsen > 1)
         return PSB_EINVAL;
       ensure_rule(C, p->ac_coarsen);
       C->cg_half = false;
@@ -2176,7 +2289,8 @@ int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_
       C->kmask = 0;
       C->flex = false;
       C->ac_omega = p->ac_omega;
-      solve_acbicgs64(C, x, b, p, R);
+      if (C->sym) solve_acbicgs64<true>(C, x, b, p, R);
+      else solve_acbicgs64<false>(C, x, b, p, R);
       R->rel_l2 = true_rel(C, b, x);
       return R->status;
     }

@@ -11,7 +11,8 @@ import pytest
 
 import psb
 import psb_oracle as O
-from conftest import load_golden
+from conftest import cycle_matrix, load_cycles, load_golden
+from test_oracle_acbicgs import nonsym
 
 pytestmark = pytest.mark.gpu
 
@@ -185,23 +186,33 @@ def test_acpcg_fp64_matches_oracle(shape):
     assert rel(x - x.mean(), xo - xo.mean()) < 1e-6
 
 
-@pytest.mark.parametrize("case", ["bubbles16", "gen32"])
+@pytest.mark.parametrize("case", ["bubbles16", "gen32", "nonsym16", "nonsym32"])
 def test_ac_bicgs_fp64_matches_oracle(case):
     """BiCGS with the V-cycle as its preconditioner (FP64, method PSB_BICGS
     with PSB_PREC_AC) against the oracle's restatement: the first iterates
     and residual histories to the order of the dot-product sums, the
     converged solve within an iteration and to the stopping rule's
-    accuracy."""
-    if case == "bubbles16":
-        g = load_golden(case)
-        s = solver_for(g)
-        A = O.Mat(*g.shape, g.mat())
+    accuracy.  nonsym*: non-symmetric matrices (7-array levels, the
+    coarsest inverse by LU): the reference driver's matrix of its
+    non-symmetric cycle runs, and the same pattern on a 32^3 case."""
+    if case in ("bubbles16", "nonsym16"):
+        g = load_golden("bubbles16")
+        M = g.mat() if case == "bubbles16" else \
+            cycle_matrix(g, "cycles_nonsym", load_cycles("bubbles16", "cycles_nonsym")[0])
+        s = psb.Solver(*g.shape)
+        s.set_matrix(M)
+        s.set_geometry(g["dxc"], g["dyc"], g["dzc"])
+        A = O.Mat(*g.shape, M)
         b, dV = g["b"], g["dV"]
     else:
         d, A = ac_case(32, nsph=5, seed=7)
-        s = psb.Solver(32, 32, 32)
-        s.set_matrix(d)
         b, dV = d["rhs"], np.ones(d["rhs"].size)
+        M = d
+        if case == "nonsym32":
+            A = nonsym(A, (32, 32, 32))
+            M = A.a
+        s = psb.Solver(32, 32, 32)
+        s.set_matrix(M)
     n = b.size
     ac = O.AC(A, dV, rule=1)
     kw = dict(precond=psb.PSB_PREC_AC, method=psb.PSB_BICGS, res_tol=0.0)
@@ -217,12 +228,35 @@ def test_ac_bicgs_fp64_matches_oracle(case):
     assert r["converged"] and io["converged"] and r["status"] == psb.PSB_OK
     assert abs(r["iterations"] - io["iters"]) <= 1, (r["iterations"], io["iters"])
     assert r["iterations"] < 40
-    assert rel(x - x.mean(), xo - xo.mean()) < 1e-7
+    if case.startswith("nonsym"):  # non-singular: x itself
+        assert rel(x, xo) < 1e-7
+    else:
+        assert rel(x - x.mean(), xo - xo.mean()) < 1e-7
     h = s.history()
     assert len(h) == r["iterations"] + 1 and h[-1] < 1e-10 * h[0]
-    # only FP64, on a symmetric matrix
+    # FP64 only
     with pytest.raises(RuntimeError):
         s.solve(np.zeros(n), b, precision=psb.PSB_MIXED, **kw)
+
+
+def test_ac_bicgs_singular_nonsymmetric_matrix_is_rejected():
+    """A non-symmetric Neumann matrix without a time term is singular on its
+    coarsest level: PSB_EINVAL, as the oracle reports it."""
+    d, A = ac_case(32, nsph=5, seed=7)
+    M = {q: np.array(d[q], dtype=np.float64) for q in "cwesnbt"}
+    i = np.meshgrid(np.arange(32), np.arange(32), np.arange(32), indexing="ij")[0].ravel()
+    de = 0.2 * (i % 3) * M["e"]
+    M["e"] = M["e"] + de
+    M["c"] = M["c"] + de
+    s = psb.Solver(32, 32, 32)
+    s.set_matrix(M)
+    with pytest.raises(RuntimeError, match="EINVAL|invalid"):
+        s.solve(np.zeros(32 ** 3), d["rhs"], precond=psb.PSB_PREC_AC, method=psb.PSB_BICGS)
+    with pytest.raises(ValueError):
+        O.AC(O.Mat(32, 32, 32, M), np.ones(32 ** 3), rule=1).coarse_dense()
+    # the context stays usable: BiCGS-di on the same matrix
+    x, r = s.solve(np.zeros(32 ** 3), d["rhs"], method=psb.PSB_BICGS, max_iter=3)
+    assert r["iterations"] == 4
 
 
 def test_mixed_precision_refinement_reaches_fp64_accuracy():

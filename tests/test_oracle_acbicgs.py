@@ -130,3 +130,65 @@ def test_acbicgs_small_s_exit_and_zero_rhs():
                          **KW)
     assert info["reason"] == 5 and info["iters"] == 1 and not info["converged"]
     assert np.linalg.norm(x - alpha * z) <= 1e-12 * np.linalg.norm(x)
+
+
+def nonsym(A, shape):
+    """The pattern the reference driver uses for its non-symmetric cycle
+    runs: east and top couplings scaled per cell, the diagonal raised by the
+    added coupling and by 5 % of itself."""
+    nx, ny, nz = shape
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    fe = (0.025 * ((7 * i + 3 * j + 5 * k) % 11)).ravel()
+    ft = (0.25 / 6.0 * ((2 * i + 5 * j + 3 * k) % 7)).ravel()
+    M = {q: np.array(A.a[q], dtype=np.float64) for q in "cwesnbt"}
+    de, dt = fe * M["e"], ft * M["t"]
+    M["e"] = M["e"] + de
+    M["t"] = M["t"] + dt
+    M["c"] = M["c"] + de + dt
+    M["c"] = M["c"] + 0.05 * M["c"]
+    return O.Mat(nx, ny, nz, M)
+
+
+def test_nonsymmetric_coarsest_inverse_and_right_preconditioning():
+    """A non-symmetric matrix: the coarsest operator is the inverse of that
+    level's matrix (G A = I), and the preconditioned BiCGS is again plain
+    BiCGStab on the dense A M^-1."""
+    A0, dV, b = case()
+    A = nonsym(A0, (16, 8, 8))
+    n = b.size
+    ac = O.AC(A, dV, rule=1)
+    G = ac.coarse_dense()
+    nx, ny, nz = ac.dims(ac.nlev - 1)
+    Ac = dense_A(O.Mat(nx, ny, nz, ac.mat(ac.nlev - 1)), nx * ny * nz)
+    assert np.abs(Ac - Ac.T).max() > 1e-3 * np.abs(Ac).max()
+    assert np.abs(G @ Ac - np.eye(nx * ny * nz)).max() < 1e-12
+    Minv = dense_minv(ac, n)
+    Ad = dense_A(A, n)
+    K = 6
+    h_ref, xs = bicgstab_right(Ad, Minv, b, np.zeros(n), K)
+    x, info = ac.acbicgs(np.zeros(n), b, dV, maxit=K, res_rat=0.0, res_tol=0.0, **KW)
+    assert np.allclose(info["hist"] * n, h_ref, rtol=1e-8, atol=0)
+    assert np.linalg.norm(x - xs[-1]) <= 1e-8 * np.linalg.norm(xs[-1])
+    # converged: the dense solution of the non-singular system
+    x, info = ac.acbicgs(np.zeros(n), b, dV, maxit=100, res_rat=1e-11, res_tol=0.0, **KW)
+    assert info["converged"] and info["iters"] < 30
+    xd = np.linalg.solve(Ad, b)
+    assert np.linalg.norm(x - xd) < 1e-9 * np.linalg.norm(xd)
+
+
+def test_singular_nonsymmetric_coarsest_level_is_reported():
+    """Without the time term a non-symmetric Neumann matrix is singular on
+    every level: the coarsest inverse reports it instead of dividing by a
+    vanishing pivot."""
+    A0, dV, b = case()
+    M = {q: np.array(A0.a[q], dtype=np.float64) for q in "cwesnbt"}
+    nx, ny, nz = 16, 8, 8
+    i = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")[0].ravel()
+    de = 0.2 * (i % 3) * M["e"]
+    M["e"] = M["e"] + de
+    M["c"] = M["c"] + de  # row sums stay zero: the constant is a right null vector
+    ac = O.AC(O.Mat(nx, ny, nz, M), dV, rule=1)
+    with pytest.raises(ValueError):
+        ac.coarse_dense()
+    with pytest.raises(ValueError):
+        ac.acbicgs(np.zeros(b.size), b, dV, maxit=5, **KW)
