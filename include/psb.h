@@ -223,6 +223,21 @@ int psb_assemble_pressure_device(psb_ctx *ctx, const double *d_rho,
 int psb_update_rhs_device(psb_ctx *ctx, const double *d_u, const double *d_v,
                           const double *d_w, int zero_mean, double *d_b);
 
+/* Pressure::ghost (Src/Equation/Centered/Pressure/pressure_ghost.cpp:10-190),
+ * the ghost-fluid method's surface-tension source, after
+ * psb_assemble_pressure*: d_fext (compact, the caller's Pressure::fext that
+ * update_rhs adds to b, pressure_update_rhs.cpp:135) += iphase dS sigma
+ * kappa_f / (rho_f d) for every face of a cell that the colour function
+ * crosses at 0.5 (iphase -1 on the cell's side below it; rho_f the mean of
+ * the two cells' densities, kappa_f their curvatures' harmonic mean when of
+ * one sign, else the arithmetic mean; d the centre distance, dys also on the
+ * north face as the reference has it).  d_c, d_kappa, d_rho: colour,
+ * curvature and density with one ghost layer (the layout of
+ * psb_assemble_pressure's rho); solid cells are left unchanged.
+ * Bit-identical with the reference.                                         */
+int psb_pressure_ghost_device(psb_ctx *ctx, const double *d_c, const double *d_kappa,
+                              const double *d_rho, double sigma, double *d_fext);
+
 /* Momentum::project (Src/Equation/Staggered/Momentum/momentum_project.cpp:
  * 10-76), the step after the pressure solve, on device arrays after
  * psb_assemble_pressure*: the interior faces of u, v, w (layout as in

@@ -347,6 +347,66 @@ void ora_project(int nx, int ny, int nz, const double *dxw, const double *dys,
 #undef RHO
 }
 
+/* kappa_avei / avej / avek (pressure_ghost.cpp:192-213): the harmonic mean
+ * of two curvatures of one sign, else their arithmetic mean */
+static double kappa_face(double a, double b) {
+  double k = 0.5 * (a + b);
+  if (a * b > 0.0) k = 2.0 * a * b / (a + b);
+  return k;
+}
+
+/* Pressure::ghost (pressure_ghost.cpp:10-190): for each face of a cell that
+ * the colour function crosses at c = 0.5 (one side below, the other above),
+ * fext += iphase dS sigma kappa_face / (rho_face d), iphase = -1 when the
+ * cell's c is below 0.5.  Faces in the order w, e, s, n, b, t; rho_face =
+ * 0.5 (rho_nb + rho_P) as PropertyMix::value(Comp, ...) forms it
+ * (property_value.cpp:59-83); d = dxw, dxe, dys, dys (the north face uses
+ * dys, :74), dzb, dzt.  c, kappa, rho with their ghost layer (as
+ * ora_discretize's rho); solid cells (off) are skipped (:96).               */
+void ora_ghost(int nx, int ny, int nz, const double *dxc, const double *dxw,
+               const double *dxe, const double *dyc, const double *dys,
+               const double *dzc, const double *dzb, const double *dzt,
+               const double *c, const double *kappa, const double *rho, double sigma,
+               const uint8_t *off, double *fext) {
+  const int gy = ny + 2, gz = nz + 2;
+#define G3(a, i, j, k) a[((size_t)((i) + 1) * gy + ((j) + 1)) * gz + ((k) + 1)]
+  const double cint = 0.5;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        const size_t P = IDX(i, j, k, ny, nz);
+        if (off && off[P]) continue;
+        const double cc = G3(c, i, j, k);
+        const double iphase = cc < cint ? -1.0 : 1.0;
+        double f = fext[P];
+        /* one face: neighbour (a, b, cn) offsets, face area, distance; the
+         * face's density and curvature pair are (nb, P) for the low side and
+         * (P, nb) for the high side, as the reference indexes them */
+#define FACE(di, dj, dk, area, dd)                                                      \
+  do {                                                                                  \
+    const double cn = G3(c, i + (di), j + (dj), k + (dk));                              \
+    if ((cc - cint) * (cn - cint) < 0.0) {                                              \
+      const int lo = (di) + (dj) + (dk) < 0;                                            \
+      const double r0 = lo ? G3(rho, i + (di), j + (dj), k + (dk)) : G3(rho, i, j, k);  \
+      const double r1 = lo ? G3(rho, i, j, k) : G3(rho, i + (di), j + (dj), k + (dk));  \
+      const double k0 = lo ? G3(kappa, i + (di), j + (dj), k + (dk)) : G3(kappa, i, j, k); \
+      const double k1 = lo ? G3(kappa, i, j, k) : G3(kappa, i + (di), j + (dj), k + (dk)); \
+      const double rf = 0.5 * (r1 + r0);                                                \
+      f += iphase * (area) * sigma * kappa_face(k0, k1) / (rf * (dd));                  \
+    }                                                                                   \
+  } while (0)
+        FACE(-1, 0, 0, dyc[j] * dzc[k], dxw[i]);
+        FACE(1, 0, 0, dyc[j] * dzc[k], dxe[i]);
+        FACE(0, -1, 0, dxc[i] * dzc[k], dys[j]);
+        FACE(0, 1, 0, dxc[i] * dzc[k], dys[j]);
+        FACE(0, 0, -1, dxc[i] * dyc[j], dzb[k]);
+        FACE(0, 0, 1, dxc[i] * dyc[j], dzt[k]);
+#undef FACE
+        fext[P] = f;
+      }
+#undef G3
+}
+
 /* ------------------------------------------------------------------------ */
 /* vector kernels                                                            */
 /* ------------------------------------------------------------------------ */

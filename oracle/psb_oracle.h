@@ -88,6 +88,16 @@ void ora_rhs(int nx, int ny, int nz, const double *dxc, const double *dyc,
 void ora_face_velocity(uint64_t seed, int nx, int ny, int nz,
                        double *u, double *v, double *w);
 
+/* Pressure::ghost (pressure_ghost.cpp:10-190): fext += the ghost-fluid
+ * surface-tension source of every face the colour function c crosses at 0.5;
+ * c, kappa, rho with one ghost layer (as ora_discretize's rho), fext
+ * compact; solid cells (off) skipped.                                       */
+void ora_ghost(int nx, int ny, int nz, const double *dxc, const double *dxw,
+               const double *dxe, const double *dyc, const double *dys,
+               const double *dzc, const double *dzb, const double *dzt,
+               const double *c, const double *kappa, const double *rho, double sigma,
+               const uint8_t *off, double *fext);
+
 /* Momentum::project (momentum_project.cpp:10-76) on the interior faces of
  * u, v, w (layout as ora_rhs) with the pressure p and the ghosted density
  * of ora_discretize; off != NULL selects the body form (faces between two

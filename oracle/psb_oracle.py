@@ -183,6 +183,22 @@ def project(mx, my, mz, rho, p, u, v, w, dt=1.0, off=None, mask=7):
     return u, v, w
 
 
+def ghost(mx, my, mz, c, kappa, rho, sigma, fext=None, off=None):
+    """Pressure::ghost: fext (a copy; zeros if None) plus the ghost-fluid
+    surface-tension source; mx = (dxc, dxw, dxe) etc., c, kappa, rho with
+    their ghost layer."""
+    nx, ny, nz = len(mx[0]), len(my[0]), len(mz[0])
+    f = np.zeros(nx * ny * nz) if fext is None else np.array(fext, dtype=np.float64)
+    offp = None
+    if off is not None:
+        off = np.ascontiguousarray(off, dtype=np.uint8).ravel()
+        offp = _p(off)
+    a = [np.ascontiguousarray(v, dtype=np.float64)
+         for v in (mx[0], mx[1], mx[2], my[0], my[1], mz[0], mz[1], mz[2], c, kappa, rho)]
+    lib().ora_ghost(nx, ny, nz, *[_p(v) for v in a], C.c_double(sigma), offp, _p(f))
+    return f
+
+
 def set_kcycle(mask):
     """Levels (bit l) whose coarse correction is two flexible-CG steps
     preconditioned by the cycle below (K-cycle); 0: the V-cycle."""

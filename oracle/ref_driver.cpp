@@ -15,6 +15,7 @@
 //   ref_driver cycles OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel [cx cy cz]
 //   ref_driver cycles-bicgs OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel
 //   ref_driver cycles-nonsym OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel
+//   ref_driver ghost[-s] OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel sigma seed [stretch]
 //   ref_driver bck OUT IN.bck nx ny nz   (Scalar::load of IN, Scalar::save of a seeded field)
 //   ref_driver bench-cg nx nsph rmin rmax rho_in rho_out seed_sph seed_vel maxit
 //   ref_driver bench-ac nx nsph rmin rmax rho_in rho_out seed_sph seed_vel cycles
@@ -471,6 +472,39 @@ void make_nonsymmetric(CasePressure &pr) {
   dump("N_A_t", interior(A.t));
 }
 
+// a field with its one-cell ghost layer, [i][j][k] with k fastest
+std::vector<double> ghosted(const Scalar &s) {
+  std::vector<double> v;
+  for (int i = s.si() - 1; i <= s.ei() + 1; i++)
+    for (int j = s.sj() - 1; j <= s.ej() + 1; j++)
+      for (int k = s.sk() - 1; k <= s.ek() + 1; k++) v.push_back(s[i][j][k]);
+  return v;
+}
+
+// Pressure::ghost (pressure_ghost.cpp:10-190): the ghost-fluid method's
+// surface-tension source added to fext, for the case's colour function, the
+// mixture's surface tension sigma and a seeded curvature field (15 % zeros,
+// the rest in [-20, 30), so both of kappa_ave*'s averages occur), Neumann
+// ghosts.  Writes c_g, kappa_g (ghosted) and fext (interior).
+void dump_ghost(World &w, double sigma, uint64_t seed) {
+  CasePressure &pr = *w.pr;
+  w.mixed->sigma(sigma);
+  Scalar kappa(*w.d);
+  neumann(kappa);
+  uint64_t st = seed;
+  for_vijk(kappa, i, j, k) {
+    const double u = ora_u01(&st);
+    kappa[i][j][k] = u < 0.15 ? 0.0 : (u - 0.4) * 50.0;
+  }
+  kappa.bnd_update();
+  kappa.exchange();
+  pr.fext = 0.0;
+  pr.ghost(*w.c, kappa);
+  dump("c_g", ghosted(*w.c));
+  dump("kappa_g", ghosted(kappa));
+  dump("fext", interior(pr.fext));
+}
+
 // AC::flexcycle (additive.h:54-95, additive_recursive.cpp:161-200): from phi
 // = 0, the level solves' ResRat given (the recursion climbs back only when a
 // level's smoother meets it); tag, cycles and phi as dump_cycles writes them
@@ -569,6 +603,25 @@ int main(int argc, char **argv) {
     Case cs = parse_case(argv + 6, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
     World w = build_world(cs, [](const Domain &d) -> Krylov * { return new BiCGS(d, Prec::di()); });
     dump_cycles(w, true);
+    return 0;
+  }
+  if (mode == "ghost" || mode == "ghost-s") {
+    // ref_driver ghost OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel sigma seed
+    //            [d1x dnx d1y dny d1z dnz (ghost-s)]: the case (as "case" writes it) and
+    //            Pressure::ghost's fext
+    if (argc < 15) { std::fprintf(stderr, "ghost: too few arguments\n"); return 2; }
+    outdir = argv[2];
+    Case cs = parse_case(argv + 6, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
+    if (mode == "ghost-s") {
+      if (argc < 21) { std::fprintf(stderr, "ghost-s: too few arguments\n"); return 2; }
+      for (int q = 0; q < 3; q++) {
+        cs.d1[q] = std::atof(argv[15 + 2 * q]);
+        cs.dn[q] = std::atof(argv[16 + 2 * q]);
+      }
+    }
+    World w = build_world(cs);
+    dump_case(w, cs);
+    dump_ghost(w, std::atof(argv[13]), std::strtoull(argv[14], nullptr, 10));
     return 0;
   }
   if (mode == "cycles-nonsym") {

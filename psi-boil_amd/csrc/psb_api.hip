@@ -2690,6 +2690,20 @@ int psb_project_device(psb_ctx *C, const double *d_rho, const double *d_p, doubl
   });
 }
 
+int psb_pressure_ghost_device(psb_ctx *C, const double *d_c, const double *d_kappa,
+                              const double *d_rho, double sigma, double *d_fext) {
+  if (!C || !d_c || !d_kappa || !d_rho || !d_fext) return PSB_EINVAL;
+  if (!C->have_matrix || !C->metrics.dxc) return PSB_ENOTREADY;
+  return guarded(C, [&]() {
+    const Grid &g = C->g;
+    hipLaunchKernelGGL(k_ghost, dim3(grid_blocks(g.N)), dim3(kBlock), 0, C->stream, g,
+                       C->metrics, d_c, d_kappa, d_rho, sigma, C->off_dev, d_fext);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return (int)PSB_OK;
+  });
+}
+
 int psb_save_bck_device(psb_ctx *C, const char *path, const double *d_x) {
   if (!C || !path || !d_x) return PSB_EINVAL;
   return guarded(C, [&]() {

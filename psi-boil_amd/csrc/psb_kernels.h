@@ -1619,6 +1619,54 @@ static __global__ __launch_bounds__(kBlock) void k_project(Grid g, DevMetrics m,
   }
 }
 
+// Pressure::ghost (pressure_ghost.cpp:10-190), one cell per thread: each of
+// the six faces (w, e, s, n, b, t, in that order) that the colour function
+// crosses at 0.5 adds iphase dS sigma kappa_f / (rho_f d) to fext, iphase =
+// -1 below 0.5; rho_f = (rho_hi + rho_lo) / 2 of the face's upper and lower
+// cell (property_value.cpp:59-83), kappa_f their harmonic mean when both
+// have one sign, else the arithmetic one (:192-213); d = dxw, dxe, dys, dys
+// (the reference's north face uses dys, :74), dzb, dzt.  c, kappa, rho with
+// their ghost layer; solid cells are skipped.  The reference's operations in
+// its order (bit-identical with -ffp-contract=off).
+static __global__ __launch_bounds__(kBlock) void k_ghost(Grid g, DevMetrics m, const double *c,
+                                                         const double *kap, const double *rho,
+                                                         double sigma, const uint8_t *off,
+                                                         double *fext) {
+  const int gy = g.ny + 2, gz = g.nz + 2;
+  for (int P = blockIdx.x * blockDim.x + threadIdx.x; P < g.N; P += gridDim.x * blockDim.x) {
+    if (off && off[P]) continue;
+    int i, j, k;
+    g.ijk(P, i, j, k);
+    auto at = [&](const double *a, int di, int dj, int dk) {
+      return a[((size_t)(i + di + 1) * gy + (j + dj + 1)) * gz + (k + dk + 1)];
+    };
+    const double cint = 0.5, cc = at(c, 0, 0, 0);
+    const double iphase = cc < cint ? -1.0 : 1.0;
+    double f = fext[P];
+    auto face = [&](int di, int dj, int dk, double area, double d) {
+      const double cn = at(c, di, dj, dk);
+      if (!((cc - cint) * (cn - cint) < 0.0)) return;
+      const bool lo = di + dj + dk < 0;  // the neighbour is the face's lower cell
+      const double rn = at(rho, di, dj, dk), rp = at(rho, 0, 0, 0);
+      const double kn = at(kap, di, dj, dk), kp = at(kap, 0, 0, 0);
+      const double r0 = lo ? rn : rp, r1 = lo ? rp : rn;
+      const double k0 = lo ? kn : kp, k1 = lo ? kp : kn;
+      const double rf = 0.5 * (r1 + r0);
+      double kf = 0.5 * (k0 + k1);
+      if (k0 * k1 > 0.0) kf = 2.0 * k0 * k1 / (k0 + k1);
+      f += iphase * area * sigma * kf / (rf * d);
+    };
+    const double ax = m.dyc[j] * m.dzc[k], ay = m.dxc[i] * m.dzc[k], az = m.dxc[i] * m.dyc[j];
+    face(-1, 0, 0, ax, m.dxw[i]);
+    face(1, 0, 0, ax, m.dxe[i]);
+    face(0, -1, 0, ay, m.dys[j]);
+    face(0, 1, 0, ay, m.dys[j]);
+    face(0, 0, -1, az, m.dzb[k]);
+    face(0, 0, 1, az, m.dzt[k]);
+    fext[P] = f;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // set-up kernels
 // ---------------------------------------------------------------------------

@@ -123,7 +123,7 @@ EXPORTS = ["psb_stop_defaults", "psb_status_string", "psb_create", "psb_destroy"
            "psb_ac_vcycle_device", "psb_assemble_pressure", "psb_assemble_pressure_device",
            "psb_update_rhs_device", "psb_project_device", "psb_bck_name", "psb_save_bck",
            "psb_load_bck", "psb_save_bck_device", "psb_load_bck_device",
-           "psb_profile_solve_device", "psb_solve_device_static"]
+           "psb_profile_solve_device", "psb_solve_device_static", "psb_pressure_ghost_device"]
 
 _lib = None
 
@@ -292,6 +292,14 @@ class Solver:
                                                C.c_double(dt), int(mask), C.c_void_p(d_u),
                                                C.c_void_p(d_v), C.c_void_p(d_w)),
                       "psb_project_device")
+
+    def pressure_ghost_device(self, d_c, d_kappa, d_rho, sigma, d_fext):
+        """Pressure::ghost: d_fext += the ghost-fluid surface-tension source
+        (d_c, d_kappa, d_rho ghosted as the assembly's density)."""
+        return _check(lib().psb_pressure_ghost_device(self._c, C.c_void_p(d_c),
+                                                      C.c_void_p(d_kappa), C.c_void_p(d_rho),
+                                                      C.c_double(sigma), C.c_void_p(d_fext)),
+                      "psb_pressure_ghost_device")
 
     def save_bck_device(self, path, d_x):
         return _check(lib().psb_save_bck_device(self._c, path.encode(), C.c_void_p(d_x)),

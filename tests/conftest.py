@@ -60,6 +60,22 @@ def load_cycles(name, kind="cycles"):
     return {k: z[k] for k in z.files if k != "meta"}, meta["runs"]
 
 
+GHOST_CASES = ["bubbles16x12x8", "stretch12x10x8"]  # <case>_ghost.npz: Pressure::ghost runs
+
+
+def load_ghost(name):
+    """Colour, curvature (ghosted), sigma and the reference's fext of one
+    Pressure::ghost run."""
+    z = np.load(os.path.join(GOLDEN, name + "_ghost.npz"), allow_pickle=False)
+    return {k: z[k] for k in z.files}
+
+
+def case_metrics(g):
+    """(dxc, dxw, dxe), (dyc, dys, dyn), (dzc, dzb, dzt) of a golden case."""
+    return (g["dxc"], g["dxw"], g["dxe"]), (g["dyc"], g["dys"], g["dyn"]), \
+        (g["dzc"], g["dzb"], g["dzt"])
+
+
 CYCLE_KINDS = ("cycles", "cycles_bicgs", "cycles_nonsym")
 
 

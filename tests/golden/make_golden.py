@@ -8,13 +8,15 @@ outputs, solver iterates after k iterations, AC level matrices, restriction /
 interpolation results and V-cycle iterates, plus the solver log lines (17
 significant digits) parsed into arrays.
 
-    python tests/golden/make_golden.py [cases] [cycles] [cycles-bicgs] [cycles-nonsym] [flex] [prec] [case names]   # needs oracle/_ref built
+    python tests/golden/make_golden.py [cases] [cycles] [cycles-bicgs] [cycles-nonsym] [flex] [prec] [ghost] [case names]   # needs oracle/_ref built
 
 "cycles" writes <case>_cycles.npz: the reference's AC::cycle with the F, W
 and full-cycle shapes and its controls (ref_driver cycles), phi after each
 run and the run's residual log.  "cycles-bicgs" writes <case>_cycles_bicgs.npz:
 six of those runs with BiCGS (Prec::di) as every level's solver, the solver
 the reference's AC hands its levels for a non-symmetric equation.
+"ghost" writes <case>_ghost.npz: the reference's Pressure::ghost source
+(fext) for the case's colour function and a seeded curvature field.
 "cycles-nonsym" writes <case>_cycles_nonsym.npz: the same runs on the case's
 matrix made non-symmetric by the driver (east and top couplings scaled
 per cell, a time term on the diagonal), with that matrix (N_A_*).  "flex" writes <case>_flex.npz: the same for
@@ -182,6 +184,34 @@ def run_cycles(name, mode="cycles"):
     return path
 
 
+# Pressure::ghost runs (ref_driver ghost): surface tension and the seed of
+# the curvature field
+GHOST_RUNS = {"bubbles16x12x8": (0.072, 77), "stretch12x10x8": (0.5, 78)}
+
+
+def run_ghost(name):
+    """<case>_ghost.npz: the colour function and a seeded curvature field
+    with their ghost layers, sigma, and the fext that the reference's
+    Pressure::ghost adds (ref_driver ghost; the case's metrics and density
+    are those of <case>.npz)."""
+    nx, ny, nz, nsph, rmin, rmax, rin, rout, ss, sv, centre = CASES[name]
+    sigma, seed = GHOST_RUNS[name]
+    stretch = STRETCH.get(name)
+    with tempfile.TemporaryDirectory() as d:
+        args = [DRIVER, "ghost-s" if stretch else "ghost", d, str(nx), str(ny), str(nz),
+                str(nsph), repr(rmin), repr(rmax), repr(rin), repr(rout), str(ss), str(sv),
+                repr(sigma), str(seed)]
+        if stretch is not None:
+            args += [repr(v) for v in stretch]
+        subprocess.run(args, check=True, capture_output=True, text=True)
+        arrays = {k: np.fromfile(os.path.join(d, k + ".f64"))
+                  for k in ("c_g", "kappa_g", "fext", "rho", "dxc")}
+    arrays["sigma"] = np.array(sigma)
+    path = os.path.join(HERE, name + "_ghost.npz")
+    np.savez_compressed(path, **arrays)
+    return path
+
+
 # AC::flexcycle runs (ref_driver flex): level-solve ResRat and MaxIter for
 # which the reference's recursion ends (it does not on many settings: once a
 # level's smoother cannot meet its ResRat it alternates between two levels)
@@ -245,7 +275,8 @@ def run_prec(name):
 def main():
     if not os.path.exists(DRIVER):
         sys.exit("oracle/_ref/ref_driver is not built (make -f oracle/ref.mk)")
-    what = sys.argv[1:] or ["cases", "cycles", "cycles-bicgs", "cycles-nonsym", "flex", "prec"]
+    what = sys.argv[1:] or ["cases", "cycles", "cycles-bicgs", "cycles-nonsym", "flex", "prec",
+                            "ghost"]
     only = [w for w in what if w in CASES]  # restrict to the cases named
     if "cases" in what:
         for name, spec in CASES.items():
@@ -263,6 +294,10 @@ def main():
         for name in CYCLE_CASES:
             if not only or name in only:
                 print(run_cycles(name, "cycles-nonsym"))
+    if "ghost" in what:
+        for name in GHOST_RUNS:
+            if not only or name in only:
+                print(run_ghost(name))
     if "flex" in what:
         for name in FLEX_RUNS:
             if not only or name in only:

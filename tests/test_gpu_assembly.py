@@ -2,7 +2,8 @@
 ABI) against the oracle's restatement of the reference, bit for bit: the
 assembled matrix is checked through the bit-exact SpMV (A x on the device
 equals the oracle's A x for random x) and by solving with it; the right-hand
-side directly."""
+side directly; Pressure::ghost's surface-tension source against the
+reference's own fext."""
 import math
 
 import numpy as np
@@ -10,6 +11,7 @@ import pytest
 
 import psb
 import psb_oracle as O
+from conftest import GHOST_CASES, case_metrics, load_ghost, load_golden
 
 pytestmark = pytest.mark.gpu
 
@@ -144,3 +146,33 @@ def test_checkpoint_warm_start_on_the_device(tmp_path):
     assert np.array_equal(dy.download(), dx.download())
     r2 = s.solve_device(dy.addr, db.addr, **stop)
     assert r2["converged"] and r2["iterations"] <= 1
+
+
+@pytest.mark.parametrize("case", GHOST_CASES)
+def test_ghost_source_is_the_references(case):
+    """Pressure::ghost on the device (psb_pressure_ghost_device) against the
+    reference's own fext bit for bit (uniform and stretched grid), its
+    accumulation into a given fext against the oracle, and solid cells
+    skipped."""
+    g, z = load_golden(case), load_ghost(case)
+    mx, my, mz = case_metrics(g)
+    s = psb.Solver(*g.shape)
+    s.assemble_pressure(g["rho"], mx, my, mz)
+    dc, dk, dr = (psb.DeviceArray.from_host(a) for a in (z["c_g"], z["kappa_g"], g["rho"]))
+    n = int(np.prod(g.shape))
+    df = psb.DeviceArray.from_host(np.zeros(n))
+    sigma = float(z["sigma"])
+    s.pressure_ghost_device(dc.addr, dk.addr, dr.addr, sigma, df.addr)
+    f1 = df.download()
+    assert np.array_equal(f1, z["fext"])
+    s.pressure_ghost_device(dc.addr, dk.addr, dr.addr, sigma, df.addr)
+    assert np.array_equal(df.download(),
+                          O.ghost(mx, my, mz, z["c_g"], z["kappa_g"], g["rho"], sigma, fext=f1))
+    off = np.zeros(n, dtype=np.uint8)
+    off[np.flatnonzero(f1)[::3]] = 1
+    s2 = psb.Solver(*g.shape)
+    s2.assemble_pressure(g["rho"], mx, my, mz, off)
+    df2 = psb.DeviceArray.from_host(np.zeros(n))
+    s2.pressure_ghost_device(dc.addr, dk.addr, dr.addr, sigma, df2.addr)
+    assert np.array_equal(df2.download(),
+                          O.ghost(mx, my, mz, z["c_g"], z["kappa_g"], g["rho"], sigma, off=off))
