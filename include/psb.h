@@ -207,6 +207,23 @@ typedef struct {   /* per direction, as Domain::dxc/dxw/dxe (domain.h:82-90) */
   const double *dzc, *dzb, *dzt;
 } psb_metrics;
 
+/* The pressure's boundary condition on each side of the box (imin, imax,
+ * jmin, jmax, kmin, kmax) for psb_assemble_pressure*, as
+ * Centered::create_system_bnd applies it (centered_system_bnd.cpp:5-61,
+ * 119-160): PSB_BND_NEUMANN for the reference's wall, symmetry, outlet and
+ * neumann types (the side's coupling moves out of c; value * face area joins
+ * the boundary source), PSB_BND_DIRICHLET for dirichlet and inlet (c keeps
+ * the coupling, value * coupling joins the source; the matrix is then
+ * non-singular).  The source is added to b by psb_update_rhs_device, as
+ * update_rhs adds fbnd (pressure_update_rhs.cpp:135).  Set before the
+ * assembly; the default is PSB_BND_NEUMANN with value 0 on every side.     */
+enum psb_bnd_type { PSB_BND_NEUMANN = 0, PSB_BND_DIRICHLET = 1 };
+typedef struct {
+  int type[6];      /* psb_bnd_type per side                                */
+  double value[6];  /* the condition's value (the reference's BndCnd value) */
+} psb_bnd;
+int psb_set_boundary(psb_ctx *ctx, const psb_bnd *bnd);
+
 /* rho: density with one ghost layer, (nx+2)(ny+2)(nz+2) values, [i][j][k]
  * with k fastest (the reference's colour field with Neumann ghosts); off:
  * solid cells (compact interior order, 1 = solid) or NULL.  Replaces
@@ -218,8 +235,9 @@ int psb_assemble_pressure_device(psb_ctx *ctx, const double *d_rho,
 
 /* After psb_assemble_pressure*: d_b = (u_w - u_e) dSx + (v_s - v_n) dSy +
  * (w_b - w_t) dSz from the device face velocities u ((nx+1) ny nz), v
- * (nx (ny+1) nz), w (nx ny (nz+1)), 0 on solid cells; zero_mean: minus the
- * mean over the fluid cells (the benchmark's compatible right-hand side).    */
+ * (nx (ny+1) nz), w (nx ny (nz+1)), 0 on solid cells, plus the boundary
+ * source of psb_set_boundary's conditions; zero_mean: minus the mean over
+ * the fluid cells (the benchmark's compatible right-hand side).             */
 int psb_update_rhs_device(psb_ctx *ctx, const double *d_u, const double *d_v,
                           const double *d_w, int zero_mean, double *d_b);
 

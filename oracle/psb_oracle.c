@@ -152,6 +152,16 @@ void ora_discretize(int nx, int ny, int nz, const double *dxc, const double *dxw
                     const double *dyn, const double *dzc, const double *dzb,
                     const double *dzt, const double *rho, const uint8_t *off,
                     ora_mat *A, double *ci, double *dV) {
+  ora_discretize_bnd(nx, ny, nz, dxc, dxw, dxe, dyc, dys, dyn, dzc, dzb, dzt, rho, off, NULL,
+                     NULL, A, ci, dV, NULL);
+}
+
+void ora_discretize_bnd(int nx, int ny, int nz, const double *dxc, const double *dxw,
+                        const double *dxe, const double *dyc, const double *dys,
+                        const double *dyn, const double *dzc, const double *dzb,
+                        const double *dzt, const double *rho, const uint8_t *off,
+                        const int *btype, const double *bval, ora_mat *A, double *ci,
+                        double *dV, double *fbnd) {
   const int gy = ny + 2, gz = nz + 2;
 #define RHO(i, j, k) rho[((size_t)((i) + 1) * gy + ((j) + 1)) * gz + ((k) + 1)]
   const size_t N = (size_t)nx * ny * nz;
@@ -196,37 +206,40 @@ void ora_discretize(int nx, int ny, int nz, const double *dxc, const double *dxw
       A->c[P] = off[P] ? csum[P]
                        : A->w[P] + A->e[P] + A->s[P] + A->n[P] + A->b[P] + A->t[P];
   }
-  /* Neumann walls in the order imin, imax, jmin, jmax, kmin, kmax
-   * (centered_system_bnd.cpp:27-61) */
-  for (int j = 0; j < ny; j++)
-    for (int k = 0; k < nz; k++) {
-      size_t P = IDX(0, j, k, ny, nz);
-      A->c[P] -= A->w[P]; A->w[P] = 0.0;
-    }
-  for (int j = 0; j < ny; j++)
-    for (int k = 0; k < nz; k++) {
-      size_t P = IDX(nx - 1, j, k, ny, nz);
-      A->c[P] -= A->e[P]; A->e[P] = 0.0;
-    }
-  for (int i = 0; i < nx; i++)
-    for (int k = 0; k < nz; k++) {
-      size_t P = IDX(i, 0, k, ny, nz);
-      A->c[P] -= A->s[P]; A->s[P] = 0.0;
-    }
-  for (int i = 0; i < nx; i++)
-    for (int k = 0; k < nz; k++) {
-      size_t P = IDX(i, ny - 1, k, ny, nz);
-      A->c[P] -= A->n[P]; A->n[P] = 0.0;
-    }
-  for (int i = 0; i < nx; i++)
-    for (int j = 0; j < ny; j++) {
-      size_t P = IDX(i, j, 0, ny, nz);
-      A->c[P] -= A->b[P]; A->b[P] = 0.0;
-    }
-  for (int i = 0; i < nx; i++)
-    for (int j = 0; j < ny; j++) {
-      size_t P = IDX(i, j, nz - 1, ny, nz);
-      A->c[P] -= A->t[P]; A->t[P] = 0.0;
+  /* boundary sides (centered_system_bnd.cpp:5-61, 119-160): first the
+   * Neumann group (wall, symmetry, outlet, neumann; btype 0) in the order
+   * imin, imax, jmin, jmax, kmin, kmax -- the side's coupling moves out of c,
+   * value * dS joins fbnd -- then the Dirichlet group (dirichlet, inlet;
+   * btype 1), whose coupling stays in c and joins fbnd times the value the
+   * ghost holds.  btype NULL: walls everywhere. */
+  if (fbnd)
+    for (size_t P = 0; P < N; P++) fbnd[P] = 0.0;
+  for (int grp = 0; grp < 2; grp++)
+    for (int side = 0; side < 6; side++) {
+      const int t = btype ? btype[side] : 0;
+      const double v = bval ? bval[side] : 0.0;
+      if (t != grp) continue;
+      const int ax = side / 2, hi = side & 1;
+      const int n1 = ax == 0 ? ny : nx, n2 = ax == 2 ? ny : nz;
+      for (int p = 0; p < n1; p++)
+        for (int q = 0; q < n2; q++) {
+          int i = 0, j = 0, k = 0;
+          if (ax == 0) { i = hi ? nx - 1 : 0; j = p; k = q; }
+          if (ax == 1) { i = p; j = hi ? ny - 1 : 0; k = q; }
+          if (ax == 2) { i = p; j = q; k = hi ? nz - 1 : 0; }
+          const size_t P = IDX(i, j, k, ny, nz);
+          double *cp = ax == 0 ? (hi ? A->e : A->w) : ax == 1 ? (hi ? A->n : A->s)
+                                                              : (hi ? A->t : A->b);
+          const double dS = ax == 0 ? dyc[j] * dzc[k] : ax == 1 ? dxc[i] * dzc[k]
+                                                                : dxc[i] * dyc[j];
+          if (grp == 0) {
+            if (fbnd) fbnd[P] += v * dS;
+            A->c[P] -= cp[P];
+          } else if (fbnd) {
+            fbnd[P] += v * cp[P];
+          }
+          cp[P] = 0.0;
+        }
     }
   /* ci = 1/c before the off-cell scaling (:127), then c *= pico (:132-148) */
   for (size_t P = 0; P < N; P++) {
@@ -269,6 +282,12 @@ void ora_face_velocity(uint64_t seed, int nx, int ny, int nz, double *u,
 void ora_rhs(int nx, int ny, int nz, const double *dxc, const double *dyc,
              const double *dzc, const double *u, const double *v, const double *w,
              const uint8_t *off, int zero_mean, double *b) {
+  ora_rhs_bnd(nx, ny, nz, dxc, dyc, dzc, u, v, w, off, NULL, zero_mean, b);
+}
+
+void ora_rhs_bnd(int nx, int ny, int nz, const double *dxc, const double *dyc,
+                 const double *dzc, const double *u, const double *v, const double *w,
+                 const uint8_t *off, const double *fbnd, int zero_mean, double *b) {
   for (int i = 0; i < nx; i++)
     for (int j = 0; j < ny; j++)
       for (int k = 0; k < nz; k++) {
@@ -284,6 +303,7 @@ void ora_rhs(int nx, int ny, int nz, const double *dxc, const double *dyc,
         f += a_z * w[((size_t)i * ny + j) * (nz + 1) + k];
         f -= a_z * w[((size_t)i * ny + j) * (nz + 1) + k + 1];
         if (off && off[P]) f = 0.0;
+        if (fbnd) f += 0.0 + fbnd[P]; /* fnew += fext + fbnd (:135), fext 0 */
         b[P] = f;
       }
   if (zero_mean) {
@@ -1554,9 +1574,28 @@ int ora_ac_coarse_dense(ora_ac *ac) {
     }
     for (int i = 0; i < m; i++) B[(size_t)i * m + col] = y[i];
   }
-  /* G = Pi B Pi over active cells */
-  int na = 0;
+  /* G = Pi B Pi over active cells; B itself for a non-singular operator:
+   * an active row of the finest matrix that does not sum to zero (Dirichlet
+   * sides).  The finest level decides: on the coarse levels the Galerkin
+   * sums' cancellation and the scaled solid cells leave row sums of a
+   * singular operator well above rounding. */
+  int na = 0, singular = 1;
   for (int P = 0; P < m; P++) na += fl[P] ? 1 : 0;
+  {
+    const ora_mat *F = &ac->A[0];
+    const size_t NF = mat_n(F);
+    for (size_t P = 0; P < NF && singular; P++) {
+      if (!ac->aflag[0][P]) continue;
+      const double off = F->w[P] + F->e[P] + F->s[P] + F->n[P] + F->b[P] + F->t[P];
+      if (fabs(F->c[P] - off) > 1e-10 * fabs(F->c[P])) singular = 0;
+    }
+  }
+  if (!singular) {
+    free(ac->G);
+    ac->G = B;
+    free(y); free(M); free(pin);
+    return m;
+  }
   double *rm = dalloc((size_t)m), *cm = dalloc((size_t)m);
   double tm = 0.0;
   for (int i = 0; i < m; i++) {

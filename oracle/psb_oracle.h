@@ -73,6 +73,17 @@ void ora_discretize(int nx, int ny, int nz,
                     const double *dzc, const double *dzb, const double *dzt,
                     const double *rho, const uint8_t *off,
                     ora_mat *A, double *ci, double *dV);
+/* ora_discretize with the pressure's boundary conditions per side (imin,
+ * imax, jmin, jmax, kmin, kmax; centered_system_bnd.cpp): btype 0 the
+ * Neumann group (wall, symmetry, outlet, neumann), 1 the Dirichlet group
+ * (dirichlet, inlet); bval the side's value; fbnd (compact) receives the
+ * boundary source.  btype NULL: walls everywhere (ora_discretize).         */
+void ora_discretize_bnd(int nx, int ny, int nz, const double *dxc, const double *dxw,
+                        const double *dxe, const double *dyc, const double *dys,
+                        const double *dyn, const double *dzc, const double *dzb,
+                        const double *dzt, const double *rho, const uint8_t *off,
+                        const int *btype, const double *bval, ora_mat *A, double *ci,
+                        double *dV, double *fbnd);
 
 /* Pressure::update_rhs with dti = 1: b = (u_w-u_e)dSx + (v_s-v_n)dSy +
  * (w_b-w_t)dSz (pressure_update_rhs.cpp:27-52), b = 0 on off cells, then the
@@ -82,6 +93,11 @@ void ora_discretize(int nx, int ny, int nz,
 void ora_rhs(int nx, int ny, int nz, const double *dxc, const double *dyc,
              const double *dzc, const double *u, const double *v,
              const double *w, const uint8_t *off, int zero_mean, double *b);
+/* ora_rhs plus fbnd (NULL: none) before the optional mean removal, as
+ * update_rhs adds fext + fbnd (pressure_update_rhs.cpp:135)               */
+void ora_rhs_bnd(int nx, int ny, int nz, const double *dxc, const double *dyc,
+                 const double *dzc, const double *u, const double *v, const double *w,
+                 const uint8_t *off, const double *fbnd, int zero_mean, double *b);
 
 /* Face velocities uniform(-1,1), wall-normal boundary faces 0.
  * Draw order: all u faces, then v, then w, each in storage order.           */
@@ -206,7 +222,9 @@ int  ora_ac_cycle(ora_ac *ac, double *phi0, const double *f0, const ora_cycle *P
 
 /* ---- the benchmark's composition (not in the reference) ----------------- */
 /* Dense operator of the coarsest level: G = Pi * (A_red)^-1 * Pi over active
- * cells, Pi the mean projector, A_red the matrix with its last pivot pinned;
+ * cells, Pi the mean projector, A_red the matrix with its last pivot pinned
+ * (A^-1 itself when an active row of the finest matrix does not sum to
+ * zero: a non-singular operator, Dirichlet sides);
  * for a non-symmetric level (7 arrays) G = A^-1 on the active cells by LU
  * with partial pivoting.  Returns the level's cell count, -1 for a singular
  * non-symmetric level (G left unset). */

@@ -131,8 +131,10 @@ def density(xc, yc, zc, sph, rho_in, rho_out):
     return rho
 
 
-def discretize(mx, my, mz, rho, off=None):
-    """mx = (dxc, dxw, dxe) etc.  Returns (Mat, ci, dV)."""
+def discretize(mx, my, mz, rho, off=None, bnd=None):
+    """mx = (dxc, dxw, dxe) etc.  Returns (Mat, ci, dV); with bnd = (types,
+    values) per side (imin .. kmax; 0 Neumann group, 1 Dirichlet group) also
+    the boundary source fbnd."""
     nx, ny, nz = len(mx[0]), len(my[0]), len(mz[0])
     A = Mat(nx, ny, nz)
     ci = np.zeros(A.n)
@@ -142,9 +144,16 @@ def discretize(mx, my, mz, rho, off=None):
         off = np.ascontiguousarray(off, dtype=np.uint8).ravel()
         offp = _p(off)
     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (*mx, *my, *mz)]
-    lib().ora_discretize(nx, ny, nz, *[_p(a) for a in arrs], _p(rho), offp,
-                         C.byref(A.s), _p(ci), _p(dV))
-    return A, ci, dV
+    if bnd is None:
+        lib().ora_discretize(nx, ny, nz, *[_p(a) for a in arrs], _p(rho), offp,
+                             C.byref(A.s), _p(ci), _p(dV))
+        return A, ci, dV
+    bt = (C.c_int * 6)(*[int(t) for t in bnd[0]])
+    bv = (C.c_double * 6)(*[float(v) for v in bnd[1]])
+    fbnd = np.zeros(A.n)
+    lib().ora_discretize_bnd(nx, ny, nz, *[_p(a) for a in arrs], _p(rho), offp, bt, bv,
+                             C.byref(A.s), _p(ci), _p(dV), _p(fbnd))
+    return A, ci, dV, fbnd
 
 
 def face_velocity(seed, nx, ny, nz):
@@ -155,15 +164,16 @@ def face_velocity(seed, nx, ny, nz):
     return u, v, w
 
 
-def rhs(dxc, dyc, dzc, u, v, w, off=None, zero_mean=True):
+def rhs(dxc, dyc, dzc, u, v, w, off=None, zero_mean=True, fbnd=None):
     nx, ny, nz = len(dxc), len(dyc), len(dzc)
     b = np.zeros(nx * ny * nz)
     offp = None
     if off is not None:
         off = np.ascontiguousarray(off, dtype=np.uint8).ravel()
         offp = _p(off)
-    lib().ora_rhs(nx, ny, nz, _p(dxc), _p(dyc), _p(dzc), _p(u), _p(v), _p(w), offp,
-                  int(zero_mean), _p(b))
+    fb = None if fbnd is None else _p(np.ascontiguousarray(fbnd, dtype=np.float64))
+    lib().ora_rhs_bnd(nx, ny, nz, _p(dxc), _p(dyc), _p(dzc), _p(u), _p(v), _p(w), offp, fb,
+                      int(zero_mean), _p(b))
     return b
 
 

@@ -15,6 +15,7 @@
 //   ref_driver cycles OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel [cx cy cz]
 //   ref_driver cycles-bicgs OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel
 //   ref_driver cycles-nonsym OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel
+//   ref_driver case-bnd OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel t0 v0 .. t5 v5
 //   ref_driver ghost[-s] OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel sigma seed [stretch]
 //   ref_driver bck OUT IN.bck nx ny nz   (Scalar::load of IN, Scalar::save of a seeded field)
 //   ref_driver bench-cg nx nsph rmin rmax rho_in rho_out seed_sph seed_vel maxit
@@ -144,6 +145,11 @@ struct Case {
   // first and last cell size per direction in units of h (1, 1: uniform);
   // Grid1D distributes the nodes in between (grid1d_distribute_nodes_inside.cpp)
   double d1[3] = {1.0, 1.0, 1.0}, dn[3] = {1.0, 1.0, 1.0};
+  // pressure boundary condition per side (imin, imax, jmin, jmax, kmin, kmax):
+  // 0 neumann, 1 dirichlet, with its value; pbnd false: neumann 0 everywhere
+  bool pbnd = false;
+  int pbt[6] = {0, 0, 0, 0, 0, 0};
+  double pbv[6] = {0, 0, 0, 0, 0, 0};
 };
 
 struct World {
@@ -185,8 +191,16 @@ World build_world(const Case &cs, Krylov *(*make_solver)(const Domain &) = nullp
     w.uvw->bc(m).add(BndCnd(Dir::kmin(), BndType::wall()));
     w.uvw->bc(m).add(BndCnd(Dir::kmax(), BndType::wall()));
   }
-  Scalar *sc[2] = {w.p, w.c};
-  for (Scalar *s : sc) neumann(*s);
+  neumann(*w.c);
+  if (!cs.pbnd) {
+    neumann(*w.p);
+  } else {
+    const Dir dirs[6] = {Dir::imin(), Dir::imax(), Dir::jmin(), Dir::jmax(), Dir::kmin(),
+                         Dir::kmax()};
+    for (int q = 0; q < 6; q++)
+      w.p->bc().add(BndCnd(dirs[q], cs.pbt[q] ? BndType::dirichlet() : BndType::neumann(),
+                           cs.pbv[q]));
+  }
   // colour function: 1 inside any sphere (cell-centre test), 0 outside
   if (cs.centred) {
     w.sph = {cs.cx, cs.cy, cs.cz, cs.rmin};
@@ -622,6 +636,24 @@ int main(int argc, char **argv) {
     World w = build_world(cs);
     dump_case(w, cs);
     dump_ghost(w, std::atof(argv[13]), std::strtoull(argv[14], nullptr, 10));
+    return 0;
+  }
+  if (mode == "case-bnd") {
+    // ref_driver case-bnd OUT nx ny nz nsph rmin rmax rho_in rho_out seed_sph seed_vel
+    //                     t0 v0 .. t5 v5: the case with the pressure's boundary
+    //                     condition per side (0 neumann, 1 dirichlet; value), written
+    //                     as "case" writes it, plus fbnd (Centered::create_system_bnd)
+    if (argc < 25) { std::fprintf(stderr, "case-bnd: too few arguments\n"); return 2; }
+    outdir = argv[2];
+    Case cs = parse_case(argv + 6, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]));
+    cs.pbnd = true;
+    for (int q = 0; q < 6; q++) {
+      cs.pbt[q] = std::atoi(argv[13 + 2 * q]);
+      cs.pbv[q] = std::atof(argv[14 + 2 * q]);
+    }
+    World w = build_world(cs);
+    dump_case(w, cs);
+    dump("fbnd", interior(w.pr->fbnd));
     return 0;
   }
   if (mode == "cycles-nonsym") {

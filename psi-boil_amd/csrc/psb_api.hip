@@ -456,9 +456,12 @@ void restrict_(psb_ctx *C, const Level<T> &f, const T *e, Level<T> &cL, T omega,
 // dense coarsest operator: G = Pi (A_pinned)^-1 Pi over active cells.  A
 // Cholesky factorisation that fixes to zero every unknown whose pivot
 // vanishes (the constant null direction of each connected Neumann block),
-// then the inverse of the factor pair, then projection onto zero mean.
+// then the inverse of the factor pair, then projection onto zero mean (the
+// projection also removes what a last pivot that rounding left just above
+// the pinning threshold put along the constant); project false (a
+// non-singular operator: Dirichlet sides) keeps the inverse.
 // ---------------------------------------------------------------------------
-std::vector<double> dense_pinv(int nx, int ny, int nz, const std::vector<double> &c,
+std::vector<double> dense_pinv(bool project, int nx, int ny, int nz, const std::vector<double> &c,
                                const std::vector<double> &w, const std::vector<double> &s,
                                const std::vector<double> &b, const std::vector<uint8_t> &act) {
   const int m = nx * ny * nz;
@@ -546,6 +549,7 @@ std::vector<double> dense_pinv(int nx, int ny, int nz, const std::vector<double>
   for (int t = 0; t < nt; t++)
     pool.emplace_back(columns, (int)((long long)m * t / nt), (int)((long long)m * (t + 1) / nt));
   for (auto &th : pool) th.join();
+  if (!project) return B;  // a non-singular operator: the inverse itself
   int na = 0;
   for (int P = 0; P < m; P++) na += act[P] ? 1 : 0;
   std::vector<double> rm(m, 0.0), cm(m, 0.0);
@@ -774,7 +778,8 @@ void build_hier64(psb_ctx *C) {
   }
   HIPCHK(hipStreamSynchronize(stream));
   std::vector<double> G = cl.ae ? dense_inv(cl.g.nx, cl.g.ny, cl.g.nz, c, w, s, b, e, n, t, act)
-                                : dense_pinv(cl.g.nx, cl.g.ny, cl.g.nz, c, w, s, b, act);
+                                : dense_pinv(C->singular, cl.g.nx, cl.g.ny, cl.g.nz, c, w, s, b,
+                                             act);
   H.m = m;
   H.G = C->harena.alloc<double>((size_t)m * m);
   HIPCHK(hipMemcpyAsync(H.G, G.data(), (size_t)m * m * 8, hipMemcpyHostToDevice, stream));
@@ -1212,6 +1217,7 @@ void init_state(psb_ctx *C, DevState *st, const psb_stop *p, int norm, double re
   h.norm = norm;
   h.hist = record ? C->hist : nullptr;
   h.hist_cap = record ? C->hist_cap : 0;
+  h.keepmean = C->singular ? 0 : 1;
   HIPCHK(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, C->stream));
   HIPCHK(hipStreamSynchronize(C->stream));
 }
@@ -2519,8 +2525,10 @@ int psb_set_matrix(psb_ctx *C, const psb_matrix7 *A) {
     for (int q = 0; q < 7; q++) gather_host(L, src[q], a[q]);
     const std::vector<double> &w = a[1], &s = a[2], &b = a[3], &e = a[4], &n = a[5], &t = a[6];
     auto id = [&](int i, int j, int k) { return ((size_t)i * ny + j) * nz + k; };
-    // wall couplings must already be folded into the system (zero)
-    bool sym = true;
+    // wall couplings must already be folded into the system (zero); the
+    // operator is singular when every active row sums to zero (Neumann
+    // walls, no time term)
+    bool sym = true, singular = true;
     for (int i = 0; i < nx; i++)
       for (int j = 0; j < ny; j++)
         for (int k = 0; k < nz; k++) {
@@ -2532,6 +2540,9 @@ int psb_set_matrix(psb_ctx *C, const psb_matrix7 *A) {
           if (i < nx - 1 && e[P] != w[id(i + 1, j, k)]) sym = false;
           if (j < ny - 1 && n[P] != s[id(i, j + 1, k)]) sym = false;
           if (k < nz - 1 && t[P] != b[id(i, j, k + 1)]) sym = false;
+          const double cP = a[0][P], off = w[P] + e[P] + s[P] + n[P] + b[P] + t[P];
+          const bool active = C->off.empty() || !C->off[P];
+          if (active && std::fabs(cP - off) > 1e-10 * std::fabs(cP)) singular = false;
         }
     const int N = C->g.N;
     const int nstore = sym ? 4 : 7;
@@ -2565,6 +2576,7 @@ int psb_set_matrix(psb_ctx *C, const psb_matrix7 *A) {
     }
     HIPCHK(hipStreamSynchronize(C->stream));
     C->sym = sym;
+    C->singular = singular;
     C->have_matrix = true;
     C->version++;
     return PSB_OK;
@@ -2581,6 +2593,7 @@ void reset_matrix(psb_ctx *C) {
   C->arena.release();
   C->harena.release();
   for (int q = 0; q < 7; q++) C->A64[q] = nullptr;
+  C->fbnd = nullptr;
   C->A32[0] = C->A32[1] = C->A32[2] = C->A32[3] = nullptr;
   C->have32 = false;
   C->h64 = Hier<double>();
@@ -2629,13 +2642,19 @@ int assemble_pressure(psb_ctx *C, const double *rho, bool rho_dev, const psb_met
     C->aflag0 = nullptr;
   }
   for (int q = 0; q < 4; q++) C->A64[q] = C->arena.alloc<double>(g.N);
+  // the boundary source exists when a side is Dirichlet or has a value
+  bool src = false;
+  for (int q = 0; q < 6; q++) src = src || C->bnd.type[q] != PSB_BND_NEUMANN || C->bnd.value[q] != 0.0;
+  C->fbnd = src ? C->arena.alloc<double>(g.N) : nullptr;
   hipLaunchKernelGGL(k_discretize, dim3(grid_blocks(g.N)), dim3(kBlock), 0, C->stream, g, M,
-                     drho, doff, C->A64[0], C->A64[1], C->A64[2], C->A64[3]);
+                     drho, doff, C->bnd, C->A64[0], C->A64[1], C->A64[2], C->A64[3], C->fbnd);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(C->stream));  // host sources above are the caller's or local
   C->metrics = M;
   C->off_dev = doff;
   C->sym = true;
+  C->singular = true;  // unless a side is Dirichlet
+  for (int q = 0; q < 6; q++) C->singular = C->singular && C->bnd.type[q] == PSB_BND_NEUMANN;
   C->have_matrix = true;
   C->version++;
   return PSB_OK;
@@ -2663,7 +2682,7 @@ int psb_update_rhs_device(psb_ctx *C, const double *d_u, const double *d_v, cons
     const Grid &g = C->g;
     const int nb = grid_blocks(g.N);
     hipLaunchKernelGGL(k_update_rhs, dim3(nb), dim3(kBlock), 0, C->stream, g, C->metrics, d_u,
-                       d_v, d_w, C->off_dev, d_b, C->work);
+                       d_v, d_w, C->off_dev, C->fbnd, d_b, C->work);
     if (zero_mean) {
       finalize<2>(C, FIN_ACMEAN, nb, C->st32);
       hipLaunchKernelGGL(k_rhs_center, dim3(nb), dim3(kBlock), 0, C->stream, g, d_b,
@@ -2688,6 +2707,16 @@ int psb_project_device(psb_ctx *C, const double *d_rho, const double *d_p, doubl
     HIPCHK(hipStreamSynchronize(C->stream));
     return (int)PSB_OK;
   });
+}
+
+int psb_set_boundary(psb_ctx *C, const psb_bnd *bnd) {
+  if (!C || !bnd) return PSB_EINVAL;
+  for (int q = 0; q < 6; q++)
+    if ((bnd->type[q] != PSB_BND_NEUMANN && bnd->type[q] != PSB_BND_DIRICHLET) ||
+        !std::isfinite(bnd->value[q]))
+      return PSB_EINVAL;
+  C->bnd = *bnd;  // used by the next assembly
+  return PSB_OK;
 }
 
 int psb_pressure_ghost_device(psb_ctx *C, const double *d_c, const double *d_kappa,

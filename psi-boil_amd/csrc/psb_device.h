@@ -97,7 +97,11 @@ struct DevState {
   double res_rat, res_tol, scale, ntot;        // stopping parameters
   int mini, maxit, stalecount, norm;           // norm: 0 reference, 1 L2
   int iter, done, converged, reason;           // progress (reason as oracle)
-  int report_iters, pad0;
+  int report_iters;
+  int keepmean;  // 1: a non-singular operator (Dirichlet sides, a time term): the
+                 // preconditioned residual and the inner right-hand side keep
+                 // their mean; 0: the singular Neumann operator, whose constant
+                 // null direction they lose
   double *hist;                                // residual per iteration
   int hist_cap, pad1;
   double stale[kStaleMax];

@@ -280,6 +280,14 @@ struct psb_ctx {
   // the K-cycle's residual update fused with the next cycle's second sweep
   // (PSB_NO_KFUSE set: separate launches, for comparisons)
   bool kfuse = true;
+  // the pressure's boundary conditions for the device assembly (all Neumann,
+  // value 0: walls) and the boundary source it formed (NULL: none)
+  psb_bnd bnd{};
+  double *fbnd = nullptr;
+  // the matrix is the singular Neumann operator (every active row sums to
+  // zero): the AC-preconditioned CG keeps its directions free of the
+  // constant null direction only then
+  bool singular = true;
 };
 
 namespace psb {

@@ -12,6 +12,7 @@
 // graph of several iterations stops doing work once the solver has finished.
 #pragma once
 
+#include "../../include/psb.h"
 #include "psb_device.h"
 
 namespace psb {
@@ -1064,7 +1065,7 @@ __device__ inline void fin_apply(int op, const double *tot, DevState *st, DevSta
       break;
     case FIN_RHO:
     case FIN_RHO_FLEX: {
-      const double m = tot[3] > 0.0 ? tot[1] / tot[3] : 0.0;
+      const double m = !st->keepmean && tot[3] > 0.0 ? tot[1] / tot[3] : 0.0;
       const double rho = tot[0] - m * tot[2];
       st->zmean = m;
       if (st->iter == 0) {
@@ -1084,7 +1085,7 @@ __device__ inline void fin_apply(int op, const double *tot, DevState *st, DevSta
     case FIN_IR_RESID:  // st: FP64 outer state, st2: FP32 inner state
       st->rsq = tot[0];
       st->res = st->norm == 0 ? ref_norm(st, tot[1]) : sqrt(tot[0]);
-      st2->mean[0] = tot[3] > 0.0 ? tot[2] / tot[3] : 0.0;
+      st2->mean[0] = !st2->keepmean && tot[3] > 0.0 ? tot[2] / tot[3] : 0.0;
       break;
     case FIN_CENTER: {
       krylov_init(st, sqrt(tot[0]), tot[0], 0.0);
@@ -1479,12 +1480,14 @@ struct DevMetrics {  // per direction: cell size, distance to the lower / upper 
 // cell per thread: the six couplings dS / (d (rho_P + rho_nb)/2) from the
 // density with its ghost layer, c their sum, couplings to solid cells
 // zeroed on both sides (c of a fluid cell re-summed, of a solid cell kept),
-// the wall couplings moved out of c (imin, imax, jmin, jmax, kmin, kmax),
+// the boundary sides' couplings treated by their condition (bd: walls moved
+// out of c, Dirichlet sides kept in c, their sources into F),
 // c of solid cells scaled by 1e-12.  The same operations in the same order
 // as the reference, so the matrix is bit-identical; stored symmetric
 // (c, w, s, b: e, n, t are the neighbours' w, s, b).
 static __global__ void k_discretize(Grid g, DevMetrics m, const double *rho, const uint8_t *off,
-                                    double *C, double *W, double *S, double *B) {
+                                    psb_bnd bd, double *C, double *W, double *S, double *B,
+                                    double *F) {
   const int gy = g.ny + 2, gz = g.nz + 2;
   for (int P = blockIdx.x * blockDim.x + threadIdx.x; P < g.N; P += gridDim.x * blockDim.x) {
     int i, j, k;
@@ -1516,28 +1519,47 @@ static __global__ void k_discretize(Grid g, DevMetrics m, const double *rho, con
         c = w + e + s + n + b + t;
       }
     }
-    if (i == 0) { c -= w; w = 0.0; }
-    if (i == g.nx - 1) { c -= e; }
-    if (j == 0) { c -= s; s = 0.0; }
-    if (j == g.ny - 1) { c -= n; }
-    if (k == 0) { c -= b; b = 0.0; }
-    if (k == g.nz - 1) { c -= t; }
+    // boundary sides (centered_system_bnd.cpp:5-61, 119-160): the Neumann
+    // group (wall, symmetry, outlet, neumann) in the order imin .. kmax moves
+    // the side's coupling out of c and adds value * dS to fbnd; then the
+    // Dirichlet group (dirichlet, inlet) keeps the coupling in c and adds
+    // value * coupling
+    const bool at[6] = {i == 0, i == g.nx - 1, j == 0, j == g.ny - 1, k == 0, k == g.nz - 1};
+    double cp[6] = {w, e, s, n, b, t};
+    const double dS[6] = {a_x, a_x, a_y, a_y, a_z, a_z};
+    double fb = 0.0;
+    for (int grp = 0; grp < 2; grp++)
+      for (int q = 0; q < 6; q++) {
+        if (!at[q] || bd.type[q] != grp) continue;
+        if (grp == 0) {
+          fb += bd.value[q] * dS[q];
+          c -= cp[q];
+        } else {
+          fb += bd.value[q] * cp[q];
+        }
+        cp[q] = 0.0;
+      }
+    w = cp[0];
+    s = cp[2];
+    b = cp[4];
     if (off && off[P]) c *= 1.0e-12;  // Src/Global/global_constants.cpp:17 (pico)
     C[P] = c;
     W[P] = w;
     S[P] = s;
     B[P] = b;
+    if (F) F[P] = fb;
   }
 }
 
 // pressure_update_rhs.cpp:27-52 with dt = 1: b = (u_w - u_e) dSx + (v_s -
 // v_n) dSy + (w_b - w_t) dSz from the staggered face velocities (u of
 // (nx+1) x ny x nz faces, v of nx x (ny+1) x nz, w of nx x ny x (nz+1)); 0 on
-// solid cells.  Partials: sum b and fluid cells (for the optional zero mean).
+// solid cells; plus the boundary source fbnd when the assembly formed one.  Partials: sum b and fluid cells (for the optional zero mean).
 static __global__ __launch_bounds__(kBlock) void k_update_rhs(Grid g, DevMetrics m,
                                                               const double *u, const double *v,
                                                               const double *wv,
-                                                              const uint8_t *off, double *bout,
+                                                              const uint8_t *off,
+                                                              const double *fbnd, double *bout,
                                                               Work w) {
   double acc[2] = {0.0, 0.0};
   for (int P = blockIdx.x * blockDim.x + threadIdx.x; P < g.N; P += gridDim.x * blockDim.x) {
@@ -1556,6 +1578,7 @@ static __global__ __launch_bounds__(kBlock) void k_update_rhs(Grid g, DevMetrics
     f -= a_z * wv[((size_t)i * ny + j) * (nz + 1) + k + 1];
     const bool solid = off && off[P];
     if (solid) f = 0.0;
+    if (fbnd) f += 0.0 + fbnd[P];  // fnew += fext + fbnd (:135), fext 0 here
     bout[P] = f;
     if (!solid) {
       acc[0] += f;

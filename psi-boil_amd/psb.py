@@ -114,6 +114,10 @@ class Case(C.Structure):
                 ("lz", C.c_double), ("cylinder", C.c_int), ("cyl_d", C.c_double)]
 
 
+class Bnd(C.Structure):
+    _fields_ = [("type", C.c_int * 6), ("value", C.c_double * 6)]
+
+
 EXPORTS = ["psb_stop_defaults", "psb_status_string", "psb_create", "psb_destroy",
            "psb_set_geometry", "psb_set_matrix", "psb_set_active", "psb_solve",
            "psb_solve_device", "psb_apply", "psb_history", "psb_generate",
@@ -123,7 +127,8 @@ EXPORTS = ["psb_stop_defaults", "psb_status_string", "psb_create", "psb_destroy"
            "psb_ac_vcycle_device", "psb_assemble_pressure", "psb_assemble_pressure_device",
            "psb_update_rhs_device", "psb_project_device", "psb_bck_name", "psb_save_bck",
            "psb_load_bck", "psb_save_bck_device", "psb_load_bck_device",
-           "psb_profile_solve_device", "psb_solve_device_static", "psb_pressure_ghost_device"]
+           "psb_profile_solve_device", "psb_solve_device_static", "psb_pressure_ghost_device",
+           "psb_set_boundary"]
 
 _lib = None
 
@@ -292,6 +297,13 @@ class Solver:
                                                C.c_double(dt), int(mask), C.c_void_p(d_u),
                                                C.c_void_p(d_v), C.c_void_p(d_w)),
                       "psb_project_device")
+
+    def set_boundary(self, types, values):
+        """The pressure's boundary condition per side (imin .. kmax): types 0
+        Neumann group, 1 Dirichlet group; used by the next assembly."""
+        b = Bnd((C.c_int * 6)(*[int(t) for t in types]),
+                (C.c_double * 6)(*[float(v) for v in values]))
+        return _check(lib().psb_set_boundary(self._c, C.byref(b)), "psb_set_boundary")
 
     def pressure_ghost_device(self, d_c, d_kappa, d_rho, sigma, d_fext):
         """Pressure::ghost: d_fext += the ghost-fluid surface-tension source

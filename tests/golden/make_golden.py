@@ -8,13 +8,15 @@ outputs, solver iterates after k iterations, AC level matrices, restriction /
 interpolation results and V-cycle iterates, plus the solver log lines (17
 significant digits) parsed into arrays.
 
-    python tests/golden/make_golden.py [cases] [cycles] [cycles-bicgs] [cycles-nonsym] [flex] [prec] [ghost] [case names]   # needs oracle/_ref built
+    python tests/golden/make_golden.py [cases] [cycles] [cycles-bicgs] [cycles-nonsym] [flex] [prec] [ghost] [bnd] [case names]   # needs oracle/_ref built
 
 "cycles" writes <case>_cycles.npz: the reference's AC::cycle with the F, W
 and full-cycle shapes and its controls (ref_driver cycles), phi after each
 run and the run's residual log.  "cycles-bicgs" writes <case>_cycles_bicgs.npz:
 six of those runs with BiCGS (Prec::di) as every level's solver, the solver
 the reference's AC hands its levels for a non-symmetric equation.
+"bnd" writes <case>_bnd.npz: the matrix, boundary source fbnd and right-hand
+side with Dirichlet and non-zero Neumann pressure boundaries.
 "ghost" writes <case>_ghost.npz: the reference's Pressure::ghost source
 (fext) for the case's colour function and a seeded curvature field.
 "cycles-nonsym" writes <case>_cycles_nonsym.npz: the same runs on the case's
@@ -184,6 +186,32 @@ def run_cycles(name, mode="cycles"):
     return path
 
 
+# pressure boundary conditions per side (imin, imax, jmin, jmax, kmin, kmax):
+# (type 0 neumann / 1 dirichlet, value) -- ref_driver case-bnd
+BND_RUNS = {"bubbles16x12x8": ((1, 0.3), (0, 0.0), (0, 0.0), (0, 0.05), (0, 0.0), (1, -0.2))}
+
+
+def run_bnd(name):
+    """<case>_bnd.npz: the case's matrix, fbnd and right-hand side with the
+    pressure boundary conditions of BND_RUNS (Centered::create_system_bnd;
+    the metrics, density and face velocities are those of <case>.npz)."""
+    nx, ny, nz, nsph, rmin, rmax, rin, rout, ss, sv, centre = CASES[name]
+    sides = BND_RUNS[name]
+    with tempfile.TemporaryDirectory() as d:
+        args = [DRIVER, "case-bnd", d, str(nx), str(ny), str(nz), str(nsph), repr(rmin),
+                repr(rmax), repr(rin), repr(rout), str(ss), str(sv)]
+        for t, v in sides:
+            args += [str(t), repr(v)]
+        subprocess.run(args, check=True, capture_output=True, text=True)
+        arrays = {k: np.fromfile(os.path.join(d, k + ".f64"))
+                  for k in ("A_c", "A_w", "A_e", "A_s", "A_n", "A_b", "A_t", "b", "fbnd", "rho")}
+    arrays["types"] = np.array([t for t, _ in sides], dtype=np.int32)
+    arrays["values"] = np.array([v for _, v in sides])
+    path = os.path.join(HERE, name + "_bnd.npz")
+    np.savez_compressed(path, **arrays)
+    return path
+
+
 # Pressure::ghost runs (ref_driver ghost): surface tension and the seed of
 # the curvature field
 GHOST_RUNS = {"bubbles16x12x8": (0.072, 77), "stretch12x10x8": (0.5, 78)}
@@ -276,7 +304,7 @@ def main():
     if not os.path.exists(DRIVER):
         sys.exit("oracle/_ref/ref_driver is not built (make -f oracle/ref.mk)")
     what = sys.argv[1:] or ["cases", "cycles", "cycles-bicgs", "cycles-nonsym", "flex", "prec",
-                            "ghost"]
+                            "ghost", "bnd"]
     only = [w for w in what if w in CASES]  # restrict to the cases named
     if "cases" in what:
         for name, spec in CASES.items():
@@ -294,6 +322,10 @@ def main():
         for name in CYCLE_CASES:
             if not only or name in only:
                 print(run_cycles(name, "cycles-nonsym"))
+    if "bnd" in what:
+        for name in BND_RUNS:
+            if not only or name in only:
+                print(run_bnd(name))
     if "ghost" in what:
         for name in GHOST_RUNS:
             if not only or name in only:

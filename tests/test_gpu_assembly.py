@@ -11,7 +11,7 @@ import pytest
 
 import psb
 import psb_oracle as O
-from conftest import GHOST_CASES, case_metrics, load_ghost, load_golden
+from conftest import GHOST_CASES, GOLDEN, case_metrics, load_ghost, load_golden
 
 pytestmark = pytest.mark.gpu
 
@@ -176,3 +176,62 @@ def test_ghost_source_is_the_references(case):
     s2.pressure_ghost_device(dc.addr, dk.addr, dr.addr, sigma, df2.addr)
     assert np.array_equal(df2.download(),
                           O.ghost(mx, my, mz, z["c_g"], z["kappa_g"], g["rho"], sigma, off=off))
+
+
+def test_boundary_conditions_are_the_references():
+    """psb_set_boundary + psb_assemble_pressure with Dirichlet pressure sides
+    and a Neumann side with a value: the SpMV of the assembled matrix and the
+    right-hand side with the boundary source bit for bit against the
+    reference's own build (bnd fixture), and the multigrid-preconditioned CG
+    on the now non-singular system (its coarsest level the plain inverse)
+    reaches the direct solution."""
+    g = load_golden("bubbles16x12x8")
+    z = np.load(f"{GOLDEN}/bubbles16x12x8_bnd.npz", allow_pickle=False)
+    nx, ny, nz = g.shape
+    n = nx * ny * nz
+    mx, my, mz = case_metrics(g)
+    s = psb.Solver(nx, ny, nz)
+    s.set_boundary(z["types"], z["values"])
+    s.assemble_pressure(g["rho"], mx, my, mz)
+    A = O.Mat(nx, ny, nz, {k: z["A_" + k] for k in "cwesnbt"})
+    rng = np.random.default_rng(3)
+    for _ in range(2):
+        x = rng.standard_normal(n)
+        assert np.array_equal(s.apply(x), O.spmv(A, x))
+    u, v, w = O.face_velocity(1005, nx, ny, nz)
+    du, dv, dw = (psb.DeviceArray.from_host(a) for a in (u, v, w))
+    db = psb.DeviceArray(n)
+    s.update_rhs_device(du.addr, dv.addr, dw.addr, db.addr, zero_mean=False)
+    fb = O.discretize(mx, my, mz, g["rho"], bnd=(z["types"], z["values"]))[3]
+    assert np.array_equal(db.download(), O.rhs(mx[0], my[0], mz[0], u, v, w, zero_mean=False,
+                                               fbnd=fb))
+    s.update_rhs_device(du.addr, dv.addr, dw.addr, db.addr, zero_mean=True)
+    b = db.download()
+    assert np.max(np.abs(b - z["b"])) <= 1e-14 * np.max(np.abs(z["b"]))
+    x, r = s.solve(np.zeros(n), b, precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL,
+                   res_rat=1e-11, res_tol=0.0, max_iter=200)
+    assert r["converged"]
+    import scipy.sparse.linalg as spla
+    xd = spla.spsolve(dense_csr(A, (nx, ny, nz)), b)
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+    # walls again: no boundary source
+    s.set_boundary([0] * 6, [0.0] * 6)
+    s.assemble_pressure(g["rho"], mx, my, mz)
+    s.update_rhs_device(du.addr, dv.addr, dw.addr, db.addr, zero_mean=False)
+    assert np.array_equal(db.download(), O.rhs(mx[0], my[0], mz[0], u, v, w, zero_mean=False))
+
+
+def dense_csr(A, dims):
+    import scipy.sparse as sp
+    nx, ny, nz = dims
+    N = nx * ny * nz
+    idx = np.arange(N).reshape(dims)
+    rows, cols, vals = [np.arange(N)], [np.arange(N)], [A.a["c"]]
+    for key, sh in (("w", (-1, 0, 0)), ("e", (1, 0, 0)), ("s", (0, -1, 0)), ("n", (0, 1, 0)),
+                    ("b", (0, 0, -1)), ("t", (0, 0, 1))):
+        a = A.a[key].reshape(dims)
+        dst = np.roll(idx, -np.array(sh), axis=(0, 1, 2))
+        m = a != 0
+        rows.append(idx[m]); cols.append(dst[m]); vals.append(-a[m])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(N, N))
