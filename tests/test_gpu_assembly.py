@@ -214,6 +214,15 @@ def test_boundary_conditions_are_the_references():
     import scipy.sparse.linalg as spla
     xd = spla.spsolve(dense_csr(A, (nx, ny, nz)), b)
     assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+    # the mixed-precision forms and the preconditioned BiCGS reach it too
+    for kw in (dict(precision=psb.PSB_MIXED, inner_adapt=2, inner_rtol=1e-2),
+               dict(precision=psb.PSB_MIXED, inner_adapt=0, inner_rtol=1e-4),
+               dict(precision=psb.PSB_MIXED, inner_adapt=2, inner_rtol=1e-2, ac_kcycle=-1),
+               dict(method=psb.PSB_BICGS)):
+        xm, rm = s.solve(np.zeros(n), b, precond=psb.PSB_PREC_AC, norm=psb.PSB_NORM_L2REL,
+                         res_rat=1e-10, res_tol=0.0, max_iter=300, **kw)
+        assert rm["converged"], kw
+        assert np.linalg.norm(xm - xd) <= 1e-7 * np.linalg.norm(xd), kw
     # walls again: no boundary source
     s.set_boundary([0] * 6, [0.0] * 6)
     s.assemble_pressure(g["rho"], mx, my, mz)
