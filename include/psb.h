@@ -51,8 +51,9 @@ enum psb_status {
 enum psb_method { PSB_CG = 0, PSB_BICGS = 1 };
 /* PSB_PREC_DIAG: Prec::di (preconditioner_form.cpp:4-13).  PSB_PREC_AC: the
  * additive-correction V-cycle, for CG (any precision, symmetric matrix) or
- * for BiCGS (FP64; the V-cycle on every level whatever ac_kcycle says, as
- * BiCGS needs the same operator at each application; a non-symmetric matrix
+ * for BiCGS (FP64, or PSB_MIXED for an FP32 cycle on a symmetric matrix with
+ * the BiCGS itself in FP64; the V-cycle on every level whatever ac_kcycle
+ * says, as BiCGS needs the same operator at each application; a non-symmetric matrix
  * keeps 7 arrays on every level, its coarsest level inverted by LU, and a
  * singular one -- no time term -- is PSB_EINVAL).  PSB_PREC_SGS: red-black symmetric
  * Gauss-Seidel, M = (D - L) D^-1 (D - U) with the even cells ((i+j+k) % 2 = 0)

@@ -1506,7 +1506,7 @@ void solve_bicgs_diag(psb_ctx *C, double *x, const double *b, const psb_stop *p,
 // ---------------------------------------------------------------------------
 // the graph key of an AC-preconditioned solve with the context's current
 // settings (body: 0 FP64, 1 FP32 inner solve, 2 reliable updates, 3 FP64
-// BiCGS)
+// BiCGS, 4 BiCGS with the FP32 cycle)
 GKey solve_key(const psb_ctx *C, const psb_stop *p, int body, const void *x, const void *b) {
   GKey k;
   k.body = body;
@@ -1516,7 +1516,7 @@ GKey solve_key(const psb_ctx *C, const psb_stop *p, int body, const void *x, con
   k.kmask = C->kmask;
   k.fuse = C->fuse;
   k.cg16 = C->cg_half;
-  k.half = body == 0 || body == 3 ? 0 : (C->h32.L.empty() ? 0 : C->h32.L[0].use16);
+  k.half = body == 0 || body >= 3 ? 0 : (C->h32.L.empty() ? 0 : C->h32.L[0].use16);
   k.flex = C->flex;
   k.omega = p->ac_omega;
   k.cfac = p->ac_cfac;
@@ -1634,6 +1634,90 @@ void solve_acbicgs64(psb_ctx *C, double *x, const double *b, const psb_stop *p,
     }
   };
   Graph &gr = graph_for(C, solve_key(C, p, 3, nullptr, nullptr));
+  run_graph(C, gr, 0, st, p->max_iter, body);
+  HIPCHK(hipMemcpyAsync(x, K.x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, C->stream));
+  finish_result(R, *C->hst, 0);
+  C->hist_count = std::min(C->hst->reason == 7 ? 1 : C->hst->iter + 1, C->hist_cap);
+}
+
+// ---------------------------------------------------------------------------
+// the same BiCGS with the V-cycle on the FP32 levels (psb_stop.precision =
+// PSB_MIXED, symmetric matrices): p and s go to FP32 with the level-0 first
+// sweep, the cycle's output comes back to FP64; the BiCGS vectors, products
+// and scalars stay FP64.  The FP32 levels' smoother reads FP32 matrices
+// (no half-precision copies: the cycle's output is then unscaled).
+// ---------------------------------------------------------------------------
+void solve_acbicgs_mixed(psb_ctx *C, double *x, const double *b, const psb_stop *p,
+                         psb_result *R) {
+  KrylovVecs<double> &K = C->k64;
+  ensure_krylov(C, K, true);
+  if (!K.z) K.z = C->arena.alloc<double>(C->g.N);
+  build_hier32(C);
+  set_half(C, false);
+  ensure_krylov(C, C->k32, true);
+  ensure_hist(C, p->max_iter);
+  Hier<float> &H = C->h32;
+  const Grid &g = C->g;
+  const DMat<double> A = mat64(C);
+  HIPCHK(hipMemcpyAsync(K.x, x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, C->stream));
+  init_state(C, C->st64, p, p->norm, p->res_rat, p->res_tol, p->max_iter);
+  for (int q = 0; q < 2; q++) {
+    zero(C, K.p[q], g.N);
+    zero(C, K.v[q], g.N);
+  }
+  start<double, true, 2>(C, g, A, b, K.x, K.r, K.rt, 0.0, C->st64);
+  HIPCHK(hipGetLastError());
+  read_state(C, C->st64);
+  const int chunk = chunk_of(p);
+  const int nb = nblocks<double>(g);
+  const int nu1 = p->ac_pre, nu2 = p->ac_post;
+  const float omega = (float)p->ac_omega, cfac = (float)p->ac_cfac;
+  DevState *st = C->st64;
+  H.qf = nullptr;
+  double *z2 = K.q;  // the second cycle's output in FP64
+  auto body = [&]() {
+    Level<float> &f = H.L[0];
+    float *r0 = f.r;
+    for (int it = 0; it < chunk; it++) {
+      const int par = it & 1;
+      with_v<double>(g, [&](auto vt) {
+        constexpr int V = decltype(vt)::value;
+        hipLaunchKernelGGL((k_abi_p<double, V>), dim3(nb), dim3(kBlock), 0, C->stream, g, A.c,
+                           K.r, K.p[par], K.v[par], K.p[par ^ 1], nullptr, 0.0, st);
+        hipLaunchKernelGGL((k_abi_to32<V>), dim3(nb), dim3(kBlock), 0, C->stream, g,
+                           K.p[par ^ 1], f.c, C->k32.p[0], f.e[0], omega, st);
+      });
+      f.r = C->k32.p[0];
+      const float *z1 = vcycle<float>(C, H, 0, st, nu1, nu2, omega, cfac, 0, false);
+      with_v<double>(g, [&](auto vt) {
+        constexpr int V = decltype(vt)::value;
+        hipLaunchKernelGGL((k_abi_to64<V>), dim3(nb), dim3(kBlock), 0, C->stream, g, z1, K.z,
+                           st);
+        hipLaunchKernelGGL((k_abi_v<double, V, true>), dim3(nb), dim3(kBlock), 0, C->stream, g,
+                           A, K.z, K.rt, K.v[par ^ 1], nullptr, st, C->work);
+        finalize<1>(C, FIN_BI1, nb, st);
+        hipLaunchKernelGGL((k_abi_s<double, V>), dim3(nb), dim3(kBlock), 0, C->stream, g, A.c,
+                           K.r, K.v[par ^ 1], K.s, nullptr, 0.0, st);
+        hipLaunchKernelGGL((k_abi_to32<V>), dim3(nb), dim3(kBlock), 0, C->stream, g, K.s, f.c,
+                           C->k32.s, f.e[0], omega, st);
+      });
+      f.r = C->k32.s;
+      const float *z2f = vcycle<float>(C, H, 0, st, nu1, nu2, omega, cfac, 0, false);
+      with_v<double>(g, [&](auto vt) {
+        constexpr int V = decltype(vt)::value;
+        hipLaunchKernelGGL((k_abi_to64<V>), dim3(nb), dim3(kBlock), 0, C->stream, g, z2f, z2,
+                           st);
+        hipLaunchKernelGGL((k_abi_t<double, V, true>), dim3(nb), dim3(kBlock), 0, C->stream, g,
+                           A, z2, K.s, K.t, st, C->work);
+        finalize<3>(C, FIN_BI2, nb, st);
+        hipLaunchKernelGGL((k_abi_x<double, V>), dim3(nb), dim3(kBlock), 0, C->stream, g,
+                           metric(C), K.z, z2, K.s, K.t, K.rt, K.x, K.r, st, C->work);
+        finalize<3>(C, FIN_BI3, nb, st);
+      });
+      f.r = r0;
+    }
+  };
+  Graph &gr = graph_for(C, solve_key(C, p, 4, nullptr, nullptr));
   run_graph(C, gr, 0, st, p->max_iter, body);
   HIPCHK(hipMemcpyAsync(x, K.x, (size_t)g.N * 8, hipMemcpyDeviceToDevice, C->stream));
   finish_result(R, *C->hst, 0);
@@ -2286,8 +2370,8 @@ int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_
   C->hist_on_host = false;
   if (p->precond == PSB_PREC_AC) {
     if (p->method == PSB_BICGS) {  // the V-cycle as BiCGS's preconditioner, FP64
-      if (p->precision != PSB_FP64 || p->ac_pre < 1 || p->ac_post < 1 || p->ac_coarsen < 0 ||
-          p->ac_coarsen > 1)
+      if ((p->precision != PSB_FP64 && !(p->precision == PSB_MIXED && C->sym)) ||
+          p->ac_pre < 1 || p->ac_post < 1 || p->ac_coarsen < 0 || p->ac_coarsen > 1)
         return PSB_EINVAL;
       ensure_rule(C, p->ac_coarsen);
       C->cg_half = false;
@@ -2295,7 +2379,8 @@ int solve_device(psb_ctx *C, double *x, const double *b, const psb_stop *p, psb_
       C->kmask = 0;
       C->flex = false;
       C->ac_omega = p->ac_omega;
-      if (C->sym) solve_acbicgs64<true>(C, x, b, p, R);
+      if (p->precision == PSB_MIXED) solve_acbicgs_mixed(C, x, b, p, R);
+      else if (C->sym) solve_acbicgs64<true>(C, x, b, p, R);
       else solve_acbicgs64<false>(C, x, b, p, R);
       R->rel_l2 = true_rel(C, b, x);
       return R->status;

@@ -755,7 +755,7 @@ __global__ __launch_bounds__(kBlock) void k_abi_p(Grid g, const T *c, const T *r
     E.v[u] = omj * Pn.v[u] / Cc.v[u];
   }
   stv<T, V>(pnew + P, Pn);
-  stv<T, V>(e0 + P, E);
+  if (e0) stv<T, V>(e0 + P, E);
   PSB_TILE_END
 }
 
@@ -773,7 +773,7 @@ __global__ __launch_bounds__(kBlock) void k_abi_v(Grid g, DMat<T> A, const T *z1
 #pragma unroll
   for (int u = 0; u < V; u++) acc[0] += (double)(Rt.v[u] * Vv.v[u]);
   stv<T, V>(v + P, Vv);
-  stv<T, V>(zs + P, G.x);
+  if (zs) stv<T, V>(zs + P, G.x);
   PSB_TILE_END
   block_partial<1>(acc, w.partial);
 }
@@ -794,7 +794,7 @@ __global__ __launch_bounds__(kBlock) void k_abi_s(Grid g, const T *c, const T *r
     E.v[u] = omj * S.v[u] / Cc.v[u];
   }
   stv<T, V>(s + P, S);
-  stv<T, V>(e0 + P, E);
+  if (e0) stv<T, V>(e0 + P, E);
   PSB_TILE_END
 }
 
@@ -818,6 +818,40 @@ __global__ __launch_bounds__(kBlock) void k_abi_t(Grid g, DMat<T> A, const T *z2
   stv<T, V>(t + P, Tt);
   PSB_TILE_END
   block_partial<3>(acc, w.partial);
+}
+
+// mixed form (FP32 V-cycle inside the FP64 BiCGS): x32 = (float)x and the
+// FP32 level-0 first sweep e32 = omj x32 / c32; back: z64 = (double)z32
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_abi_to32(Grid g, const double *x, const float *c32,
+                                                     float *x32, float *e32, float omj,
+                                                     const DevState *st) {
+  if (st->done) return;
+  PSB_TILE_BEGIN(g, V)
+  const VecN<double, V> X = ldv<double, V>(x + P);
+  const VecN<float, V> Cc = ldv<float, V>(c32 + P);
+  VecN<float, V> Y, E;
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    Y.v[u] = (float)X.v[u];
+    E.v[u] = omj * Y.v[u] / Cc.v[u];
+  }
+  stv<float, V>(x32 + P, Y);
+  stv<float, V>(e32 + P, E);
+  PSB_TILE_END
+}
+
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_abi_to64(Grid g, const float *z32, double *z,
+                                                     const DevState *st) {
+  if (st->done) return;
+  PSB_TILE_BEGIN(g, V)
+  const VecN<float, V> Z = ldv<float, V>(z32 + P);
+  VecN<double, V> Y;
+#pragma unroll
+  for (int u = 0; u < V; u++) Y.v[u] = (double)Z.v[u];
+  stv<double, V>(z + P, Y);
+  PSB_TILE_END
 }
 
 // A5: x += alpha z1; x += omega z2; r = s - omega t; partials of the norms
